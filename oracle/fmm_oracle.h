@@ -1,0 +1,126 @@
+/*
+ * fmm_oracle.h -- CPU oracle for the particle FMM path of fmmlib3d.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is a plain-C restatement of the
+ * reference algorithm (Laplace lfmm3dparttarg / l3dpartdirect and Helmholtz
+ * hfmm3dparttarg / h3dpartdirect, src/lfmm3dpart.f, src/hfmm3dpart.f) used
+ * by tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg as the
+ * checker.  The product (fmmlib3d_amd) never links or calls it.
+ *
+ * Conventions (identical to the reference's "_trunc" routines, which the
+ * reference FMM uses):
+ *   Yhat_n^m(theta,phi) = (-1)^m sqrt((n-|m|)!/(n+|m|)!) P_n^|m|(cos theta)
+ *                         e^{i m phi},  P without Condon-Shortley phase,
+ *   regular   R_n^m(x) = |x|^n Yhat_n^m,   irregular S_n^m(x) = Yhat_n^m/|x|^{n+1},
+ *   multipole M_n^m = rscale^n  sum_j q_j conj(R_n^m(y_j - c)),
+ *   local     L_n^m = rscale^-n sum_j q_j conj(S_n^m(y_j - c)),
+ * evaluated as sum M_n^m S_n^m / rscale^n and sum L_n^m R_n^m rscale^n.
+ * Expansions are stored packed: coefficient (n,m) at index n*n+n+m.
+ * The reference stores them as mpole(0:p,-p:p); tests convert.
+ */
+#ifndef FMM_ORACLE_H
+#define FMM_ORACLE_H
+#include <complex.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef double _Complex zdouble;
+
+/* ---- expansion orders (src/l3dterms.f, src/h3dterms.f) ---- */
+int  orc_l3dterms(double eps, int *ier);
+int  orc_l3dterms_eval(int itype, double eps, int *ier);
+void orc_l3dterms_list2(double eps, int *itable /* 7*7*7, [i+3][j+3][k+3] */, int *ier);
+int  orc_h3dterms(double size, zdouble zk, double eps, int *ier);
+void orc_h3dterms_list2(double size, zdouble zk, double eps, int *itable, int *ier);
+
+/* ---- solid harmonics (packed, n <= p) ---- */
+void orc_regular(int p, const double *x, zdouble *R);
+void orc_irregular(int p, const double *x, zdouble *S);
+
+/* ---- Laplace operators (accumulate into the output expansion) ---- */
+void orc_l3dformmp(double rscale, const double *src, const zdouble *q, int ns,
+                   const double *center, int p, zdouble *mp);
+void orc_l3dformmp_dp(double rscale, const double *src, const zdouble *ds,
+                      const double *dv, int ns, const double *center, int p, zdouble *mp);
+void orc_l3dformta(double rscale, const double *src, const zdouble *q, int ns,
+                   const double *center, int p, zdouble *loc);
+void orc_l3dformta_dp(double rscale, const double *src, const zdouble *ds,
+                      const double *dv, int ns, const double *center, int p, zdouble *loc);
+void orc_l3dmpeval(double rscale, const double *center, const zdouble *mp, int p,
+                   const double *trg, int nt, int ifpot, zdouble *pot, int iffld, zdouble *fld);
+void orc_l3dtaeval(double rscale, const double *center, const zdouble *loc, int p,
+                   const double *trg, int nt, int ifpot, zdouble *pot, int iffld, zdouble *fld);
+void orc_l3dmpmp(double sc1, const double *c1, const zdouble *mp1, int p1,
+                 double sc2, const double *c2, zdouble *mp2, int p2);
+void orc_l3dmploc(double sc1, const double *c1, const zdouble *mp, int p1, int ptrunc,
+                  double sc2, const double *c2, zdouble *loc, int p2);
+void orc_l3dlocloc(double sc1, const double *c1, const zdouble *loc1, int p1,
+                   double sc2, const double *c2, zdouble *loc2, int p2);
+
+/* rotation about y by theta of a packed expansion (coefficient convention above) */
+void orc_rotate_y(int p, double theta, const zdouble *in, zdouble *out);
+
+/* ---- direct sums (src/lfmm3dpart.f:1391, src/hfmm3dpart.f:1268) ---- */
+void orc_l3dpartdirect(int ns, const double *src, int ifcharge, const zdouble *q,
+                       int ifdipole, const zdouble *ds, const double *dv,
+                       int ifpot, zdouble *pot, int iffld, zdouble *fld,
+                       int nt, const double *trg, int ifpottarg, zdouble *pottarg,
+                       int iffldtarg, zdouble *fldtarg);
+/* direct sum evaluated at a subset of targets/sources given by index lists */
+void orc_l3ddirect_sample(int ns, const double *src, int ifcharge, const zdouble *q,
+                          int ifdipole, const zdouble *ds, const double *dv,
+                          int nt, const double *trg, const int *self_index,
+                          zdouble *pot, zdouble *fld);
+void orc_h3dpartdirect(zdouble zk, int ns, const double *src, int ifcharge, const zdouble *q,
+                       int ifdipole, const zdouble *ds, const double *dv,
+                       int ifpot, zdouble *pot, int iffld, zdouble *fld,
+                       int nt, const double *trg, int ifpottarg, zdouble *pottarg,
+                       int iffldtarg, zdouble *fldtarg);
+void orc_h3ddirect_sample(zdouble zk, int ns, const double *src, int ifcharge, const zdouble *q,
+                          int ifdipole, const zdouble *ds, const double *dv,
+                          int nt, const double *trg, const int *self_index,
+                          zdouble *pot, zdouble *fld);
+
+/* ---- adaptive octree (src/d3tstrcr.f) ---- */
+typedef struct {
+  int level, ijk[3], parent, child[8], nchild;
+  int isrc, nsrc, itrg, ntrg;      /* 0-based start into the sorted order */
+  double center[3];
+} orc_box;
+
+typedef struct {
+  int nboxes, nlev;
+  double center[3], size;
+  orc_box *boxes;
+  int *isrc, *itrg;                /* sorted -> original index (0-based) */
+  int *ladd;                       /* first box of each level, nlev+2 entries */
+  int *lst_off[5], *lst[5];        /* CSR lists 1..5 (index 0 = list 1) */
+} orc_tree;
+
+int  orc_tree_build(const double *src, int ns, const double *trg, int nt, int nbox,
+                    orc_tree *t);
+void orc_tree_free(orc_tree *t);
+/* flattened accessors for tests: boxes as 20 ints like d3tgetb (1-based ids) */
+int  orc_tree_summary(const double *src, int ns, const double *trg, int nt, int nbox,
+                      int *nboxes, int *nlev, int *boxes20, int maxboxes,
+                      int *isrc, int *itrg, int *list_counts /* 5*maxboxes */,
+                      int *lists /* concatenated, sorted ascending per box */, int maxlist);
+
+/* ---- FMM drivers (reference calling sequences, 0 = ok) ---- */
+int orc_lfmm3dparttarg(int iprec, int ns, const double *src,
+                       int ifcharge, const zdouble *q, int ifdipole, const zdouble *ds,
+                       const double *dv, int ifpot, zdouble *pot, int iffld, zdouble *fld,
+                       int nt, const double *trg, int ifpottarg, zdouble *pottarg,
+                       int iffldtarg, zdouble *fldtarg);
+int orc_hfmm3dparttarg(int iprec, zdouble zk, int ns, const double *src,
+                       int ifcharge, const zdouble *q, int ifdipole, const zdouble *ds,
+                       const double *dv, int ifpot, zdouble *pot, int iffld, zdouble *fld,
+                       int nt, const double *trg, int ifpottarg, zdouble *pottarg,
+                       int iffldtarg, zdouble *fldtarg);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,265 @@
+"""ctypes bindings for the CPU oracle (oracle/liboracle.so) and, where it has
+been built, the reference Fortran library (oracle/_ref/libfmmlib3d_ref.so).
+
+Test infrastructure: used by tests/, __graft_entry__.smoke() and bench.py's
+cpu_baseline leg only.  Arrays follow the reference's Fortran layouts:
+source(3,n) is a C-contiguous (n,3) float64 array, charges complex128,
+fld(3,n) a C-contiguous (n,3) complex128 array.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+ORACLE_SO = os.path.join(ROOT, "oracle", "liboracle.so")
+REF_SO = os.path.join(ROOT, "oracle", "_ref", "libfmmlib3d_ref.so")
+
+_oracle = None
+_ref = None
+
+
+def _vp(a):
+    return None if a is None else a.ctypes.data_as(ctypes.c_void_p)
+
+
+def oracle():
+    global _oracle
+    if _oracle is None:
+        if not os.path.exists(ORACLE_SO):
+            raise RuntimeError("oracle not built: run `make -C oracle`")
+        _oracle = ctypes.CDLL(ORACLE_SO)
+        _oracle.orc_l3dterms.restype = ctypes.c_int
+        _oracle.orc_lfmm3dparttarg.restype = ctypes.c_int
+    return _oracle
+
+
+def ref_available():
+    return os.path.exists(REF_SO)
+
+
+def ref():
+    global _ref
+    if _ref is None:
+        if not ref_available():
+            raise RuntimeError("reference library not built: run `make -C oracle ref`")
+        _ref = ctypes.CDLL(REF_SO)
+    return _ref
+
+
+def _i(v):
+    return ctypes.byref(ctypes.c_int(int(v)))
+
+
+def _d(v):
+    return ctypes.byref(ctypes.c_double(float(v)))
+
+
+def _z(v):
+    a = np.array([complex(v)], dtype=np.complex128)
+    return a, _vp(a)
+
+
+# --------------------------------------------------------------------------
+# packed (n*n+n+m) <-> reference mpole(0:p,-p:p) column-major
+# --------------------------------------------------------------------------
+def packed_to_ref(M, p):
+    A = np.zeros((2 * p + 1, p + 1), dtype=np.complex128)  # [m+p, n] -> Fortran (n,m)
+    for n in range(p + 1):
+        for m in range(-n, n + 1):
+            A[m + p, n] = M[n * n + n + m]
+    return np.ascontiguousarray(A).reshape(-1)
+
+
+def ref_to_packed(A, p):
+    A = A.reshape(2 * p + 1, p + 1)
+    M = np.zeros((p + 1) ** 2, dtype=np.complex128)
+    for n in range(p + 1):
+        for m in range(-n, n + 1):
+            M[n * n + n + m] = A[m + p, n]
+    return M
+
+
+# --------------------------------------------------------------------------
+# oracle wrappers
+# --------------------------------------------------------------------------
+def _c(a, dt):
+    return None if a is None else np.ascontiguousarray(a, dtype=dt)
+
+
+def orc_lfmm3dparttarg(iprec, source, charge=None, dipstr=None, dipvec=None, target=None,
+                       ifpot=1, iffld=1, ifpottarg=1, iffldtarg=1):
+    o = oracle()
+    src = _c(source, np.float64)
+    ns = src.shape[0]
+    trg = _c(target, np.float64) if target is not None else np.zeros((1, 3))
+    nt = 0 if target is None else trg.shape[0]
+    q = _c(charge, np.complex128)
+    ds = _c(dipstr, np.complex128)
+    dv = _c(dipvec, np.float64)
+    pot = np.zeros(ns, np.complex128)
+    fld = np.zeros((ns, 3), np.complex128)
+    pt = np.zeros(max(nt, 1), np.complex128)
+    ft = np.zeros((max(nt, 1), 3), np.complex128)
+    ier = o.orc_lfmm3dparttarg(int(iprec), ns, _vp(src), int(q is not None), _vp(q),
+                               int(ds is not None), _vp(ds), _vp(dv), ifpot, _vp(pot), iffld,
+                               _vp(fld), nt, _vp(trg), ifpottarg if nt else 0, _vp(pt),
+                               iffldtarg if nt else 0, _vp(ft))
+    return ier, pot, fld, pt[:nt], ft[:nt]
+
+
+def orc_l3dpartdirect(source, charge=None, dipstr=None, dipvec=None, target=None):
+    o = oracle()
+    src = _c(source, np.float64)
+    ns = src.shape[0]
+    trg = _c(target, np.float64) if target is not None else np.zeros((1, 3))
+    nt = 0 if target is None else trg.shape[0]
+    q = _c(charge, np.complex128)
+    ds = _c(dipstr, np.complex128)
+    dv = _c(dipvec, np.float64)
+    pot = np.zeros(ns, np.complex128)
+    fld = np.zeros((ns, 3), np.complex128)
+    pt = np.zeros(max(nt, 1), np.complex128)
+    ft = np.zeros((max(nt, 1), 3), np.complex128)
+    o.orc_l3dpartdirect(ns, _vp(src), int(q is not None), _vp(q), int(ds is not None), _vp(ds),
+                        _vp(dv), 1, _vp(pot), 1, _vp(fld), nt, _vp(trg), 1 if nt else 0, _vp(pt),
+                        1 if nt else 0, _vp(ft))
+    return pot, fld, pt[:nt], ft[:nt]
+
+
+def orc_direct_sample(source, points, self_index, charge=None, dipstr=None, dipvec=None, zk=None):
+    """FP64 direct sum at `points`; self_index[j] >= 0 skips that source."""
+    o = oracle()
+    src = _c(source, np.float64)
+    pts = _c(points, np.float64)
+    si = _c(self_index, np.int32)
+    q = _c(charge, np.complex128)
+    ds = _c(dipstr, np.complex128)
+    dv = _c(dipvec, np.float64)
+    pot = np.zeros(pts.shape[0], np.complex128)
+    fld = np.zeros((pts.shape[0], 3), np.complex128)
+    if zk is None:
+        o.orc_l3ddirect_sample(src.shape[0], _vp(src), int(q is not None), _vp(q),
+                               int(ds is not None), _vp(ds), _vp(dv), pts.shape[0], _vp(pts),
+                               _vp(si), _vp(pot), _vp(fld))
+    else:
+        zk_c = ctypes.c_double * 2
+        o.orc_h3ddirect_sample.argtypes = [zk_c] + [ctypes.c_void_p] * 0
+        raise NotImplementedError
+    return pot, fld
+
+
+# --------------------------------------------------------------------------
+# reference wrappers (only when oracle/_ref is built)
+# --------------------------------------------------------------------------
+def ref_lfmm3dparttarg(iprec, source, charge=None, dipstr=None, dipvec=None, target=None,
+                       ifpot=1, iffld=1, ifpottarg=1, iffldtarg=1):
+    r = ref()
+    src = _c(source, np.float64)
+    ns = src.shape[0]
+    trg = _c(target, np.float64) if target is not None else np.zeros((1, 3))
+    nt = 0 if target is None else trg.shape[0]
+    q = _c(charge, np.complex128) if charge is not None else np.zeros(1, np.complex128)
+    ds = _c(dipstr, np.complex128) if dipstr is not None else np.zeros(1, np.complex128)
+    dv = _c(dipvec, np.float64) if dipvec is not None else np.zeros(3)
+    pot = np.zeros(ns, np.complex128)
+    fld = np.zeros((ns, 3), np.complex128)
+    pt = np.zeros(max(nt, 1), np.complex128)
+    ft = np.zeros((max(nt, 1), 3), np.complex128)
+    ier = ctypes.c_int(0)
+    r.lfmm3dparttarg_(ctypes.byref(ier), _i(iprec), _i(ns), _vp(src), _i(charge is not None),
+                      _vp(q), _i(dipstr is not None), _vp(ds), _vp(dv), _i(ifpot), _vp(pot),
+                      _i(iffld), _vp(fld), _i(nt), _vp(trg), _i(ifpottarg if nt else 0), _vp(pt),
+                      _i(iffldtarg if nt else 0), _vp(ft))
+    return ier.value, pot, fld, pt[:nt], ft[:nt]
+
+
+def ref_l3dpartdirect(source, charge=None, dipstr=None, dipvec=None, target=None):
+    r = ref()
+    src = _c(source, np.float64)
+    ns = src.shape[0]
+    trg = _c(target, np.float64) if target is not None else np.zeros((1, 3))
+    nt = 0 if target is None else trg.shape[0]
+    q = _c(charge, np.complex128) if charge is not None else np.zeros(1, np.complex128)
+    ds = _c(dipstr, np.complex128) if dipstr is not None else np.zeros(1, np.complex128)
+    dv = _c(dipvec, np.float64) if dipvec is not None else np.zeros(3)
+    pot = np.zeros(ns, np.complex128)
+    fld = np.zeros((ns, 3), np.complex128)
+    pt = np.zeros(max(nt, 1), np.complex128)
+    ft = np.zeros((max(nt, 1), 3), np.complex128)
+    r.l3dpartdirect_(_i(ns), _vp(src), _i(charge is not None), _vp(q), _i(dipstr is not None),
+                     _vp(ds), _vp(dv), _i(1), _vp(pot), _i(1), _vp(fld), _i(nt), _vp(trg),
+                     _i(1 if nt else 0), _vp(pt), _i(1 if nt else 0), _vp(ft))
+    return pot, fld, pt[:nt], ft[:nt]
+
+
+def ref_tree(source, target, nbox):
+    """Run the reference d3tstrcr and return (boxes20, isrc, itrg, lists) with
+    lists[k][b] a sorted tuple of 1-based box ids (k = 0..4 for lists 1..5)."""
+    r = ref()
+    src = _c(source, np.float64)
+    trg = _c(target, np.float64) if target is not None else np.zeros((1, 3))
+    ns = src.shape[0]
+    nt = 0 if target is None else trg.shape[0]
+    lw = 200 * (ns + nt) + 100000
+    w = np.zeros(lw, np.int32)
+    iz = np.zeros(ns, np.int32)
+    izt = np.zeros(max(nt, 1), np.int32)
+    laddr = np.zeros(400, np.int32)
+    center = np.zeros(3)
+    ier = ctypes.c_int(0)
+    nboxes = ctypes.c_int(0)
+    nlev = ctypes.c_int(0)
+    size = ctypes.c_double(0)
+    lused = ctypes.c_int(0)
+    r.d3tstrcr_(ctypes.byref(ier), _vp(src), _i(ns), _i(nbox), ctypes.byref(nboxes), _vp(iz),
+                _vp(laddr), ctypes.byref(nlev), _vp(center), ctypes.byref(size), _vp(trg), _i(nt),
+                _vp(izt), _vp(w), _i(lw), ctypes.byref(lused))
+    if ier.value:
+        raise RuntimeError("d3tstrcr ier=%d" % ier.value)
+    nb = nboxes.value
+    boxes = np.zeros((nb, 20), np.int32)
+    box = np.zeros(20, np.int32)
+    c0 = np.zeros(3)
+    corners = np.zeros(24)
+    lst = np.zeros(20000, np.int32)
+    nl = ctypes.c_int(0)
+    lists = [[() for _ in range(nb)] for _ in range(5)]
+    for b in range(1, nb + 1):
+        r.d3tgetb_(ctypes.byref(ier), _i(b), _vp(box), _vp(c0), _vp(corners), _vp(w))
+        boxes[b - 1] = box
+        for k in range(1, 6):
+            nl.value = 0
+            r.d3tgetl_(ctypes.byref(ier), _i(b), _i(k), _vp(lst), ctypes.byref(nl), _vp(w))
+            if ier.value == 0 and nl.value > 0:
+                lists[k - 1][b - 1] = tuple(sorted(lst[: nl.value].tolist()))
+    return boxes, iz.copy(), izt[:nt].copy(), lists, nlev.value
+
+
+def orc_tree(source, target, nbox, maxboxes=200000, maxlist=20000000):
+    o = oracle()
+    src = _c(source, np.float64)
+    trg = _c(target, np.float64) if target is not None else np.zeros((1, 3))
+    ns = src.shape[0]
+    nt = 0 if target is None else trg.shape[0]
+    nboxes = ctypes.c_int(0)
+    nlev = ctypes.c_int(0)
+    boxes = np.zeros((maxboxes, 20), np.int32)
+    isrc = np.zeros(ns, np.int32)
+    itrg = np.zeros(max(nt, 1), np.int32)
+    counts = np.zeros(5 * maxboxes, np.int32)
+    lists = np.zeros(maxlist, np.int32)
+    ier = o.orc_tree_summary(_vp(src), ns, _vp(trg), nt, int(nbox), ctypes.byref(nboxes),
+                             ctypes.byref(nlev), _vp(boxes), maxboxes, _vp(isrc), _vp(itrg),
+                             _vp(counts), _vp(lists), maxlist)
+    if ier:
+        raise RuntimeError("orc_tree_summary ier=%d" % ier)
+    nb = nboxes.value
+    out = [[() for _ in range(nb)] for _ in range(5)]
+    o_ = 0
+    for k in range(5):
+        for b in range(nb):
+            c = counts[k * maxboxes + b]
+            out[k][b] = tuple(lists[o_: o_ + c].tolist())
+            o_ += c
+    return boxes[:nb], isrc, itrg[:nt], out, nlev.value
