@@ -1,0 +1,156 @@
+"""fmmlib3d_amd: MI355X (gfx950) particle FMM with fmmlib3d's calling sequences.
+
+Python mirror of the reference routines (src/lfmm3dpart.f, src/hfmm3dpart.f)
+over the C ABI of libfmm3d_mi355x.so (include/fmm3d_mi355x.h).  Arguments and
+flags mean what they mean in the reference; like the reference's f2py
+intents (lfmm3dpart.f:205-226) the outputs are returned:
+
+    ier, pot, fld, pottarg, fldtarg = lfmm3dparttarg(iprec, source, charge=q,
+                                                     target=t)
+
+Arrays: source/target (n,3) float64, charge/dipstr (n,) complex128,
+dipvec (n,3) float64; pot (n,) complex128, fld (n,3) complex128.
+There is no CPU fallback: when the HIP library is missing or no GPU is
+visible the calls raise.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "lib", "libfmm3d_mi355x.so")
+
+PREC_FP64 = 0
+PREC_MIXED = 1
+
+# every symbol include/fmm3d_mi355x.h declares
+EXPORTED = (
+    "fmm3d_lfmm3dparttarg", "fmm3d_lfmm3dpartself", "fmm3d_l3dpartdirect",
+    "fmm3d_lfmm3dparttarg_ex", "fmm3d_lfmm3dparttarg_dev", "fmm3d_last_timings",
+    "fmm3d_version",
+)
+
+_lib = None
+
+
+class FMMError(RuntimeError):
+    def __init__(self, ier, msg=""):
+        super().__init__("ier=%d %s" % (ier, msg))
+        self.ier = ier
+
+
+def load():
+    """Load the native library (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError("libfmm3d_mi355x.so not built: run `python -m fmmlib3d_amd.build`")
+        lib = ctypes.CDLL(LIB_PATH)
+        vp = ctypes.c_void_p
+        i = ctypes.c_int
+        lib.fmm3d_version.restype = ctypes.c_char_p
+        targ = [i, i, vp, i, vp, i, vp, vp, i, vp, i, vp, i, vp, i, vp, i, vp]
+        lib.fmm3d_lfmm3dparttarg.argtypes = targ
+        lib.fmm3d_lfmm3dparttarg.restype = i
+        lib.fmm3d_lfmm3dparttarg_ex.argtypes = [i] + targ
+        lib.fmm3d_lfmm3dparttarg_ex.restype = i
+        lib.fmm3d_lfmm3dparttarg_dev.argtypes = [i] + targ + [vp]
+        lib.fmm3d_lfmm3dparttarg_dev.restype = i
+        lib.fmm3d_lfmm3dpartself.argtypes = [i, i, vp, i, vp, i, vp, vp, i, vp, i, vp]
+        lib.fmm3d_lfmm3dpartself.restype = i
+        lib.fmm3d_l3dpartdirect.argtypes = [i, vp, i, vp, i, vp, vp, i, vp, i, vp, i, vp, i, vp, i, vp]
+        lib.fmm3d_l3dpartdirect.restype = None
+        lib.fmm3d_last_timings.argtypes = [vp, i]
+        lib.fmm3d_last_timings.restype = None
+        _lib = lib
+    return _lib
+
+
+def version():
+    return load().fmm3d_version().decode()
+
+
+def _ptr(a):
+    return None if a is None else a.ctypes.data_as(ctypes.c_void_p)
+
+
+def _in(a, dt, shape_last=None):
+    if a is None:
+        return None
+    a = np.ascontiguousarray(a, dtype=dt)
+    return a
+
+
+def _prep(source, charge, dipstr, dipvec, target):
+    src = _in(source, np.float64)
+    if src.ndim != 2 or src.shape[1] != 3:
+        raise ValueError("source must be (n,3)")
+    ns = src.shape[0]
+    q = _in(charge, np.complex128)
+    ds = _in(dipstr, np.complex128)
+    dv = _in(dipvec, np.float64)
+    if q is not None and q.shape != (ns,):
+        raise ValueError("charge must be (nsource,)")
+    if (ds is None) != (dv is None):
+        raise ValueError("dipstr and dipvec go together")
+    if ds is not None and (ds.shape != (ns,) or dv.shape != (ns, 3)):
+        raise ValueError("dipstr (n,) and dipvec (n,3)")
+    trg = _in(target, np.float64)
+    if trg is not None and (trg.ndim != 2 or trg.shape[1] != 3):
+        raise ValueError("target must be (n,3)")
+    return src, q, ds, dv, trg
+
+
+def lfmm3dparttarg(iprec, source, charge=None, dipstr=None, dipvec=None, target=None,
+                   ifpot=1, iffld=1, ifpottarg=1, iffldtarg=1, precision=PREC_FP64):
+    """Laplace FMM (lfmm3dparttarg, src/lfmm3dpart.f:138).  Returns
+    (ier, pot, fld, pottarg, fldtarg); outputs whose flag is 0 are None."""
+    lib = load()
+    src, q, ds, dv, trg = _prep(source, charge, dipstr, dipvec, target)
+    ns = src.shape[0]
+    nt = 0 if trg is None else trg.shape[0]
+    pot = np.zeros(ns, np.complex128) if ifpot else None
+    fld = np.zeros((ns, 3), np.complex128) if iffld else None
+    pt = np.zeros(nt, np.complex128) if (ifpottarg and nt) else None
+    ft = np.zeros((nt, 3), np.complex128) if (iffldtarg and nt) else None
+    ier = lib.fmm3d_lfmm3dparttarg_ex(
+        int(precision), int(iprec), ns, _ptr(src), int(q is not None), _ptr(q),
+        int(ds is not None), _ptr(ds), _ptr(dv), int(bool(ifpot)), _ptr(pot), int(bool(iffld)),
+        _ptr(fld), nt, _ptr(trg), int(pt is not None), _ptr(pt), int(ft is not None), _ptr(ft))
+    return ier, pot, fld, pt, ft
+
+
+def lfmm3dpartself(iprec, source, charge=None, dipstr=None, dipvec=None, ifpot=1, iffld=1,
+                   precision=PREC_FP64):
+    """Laplace FMM at the sources only (lfmm3dpartself, src/lfmm3dpart.f:87)."""
+    ier, pot, fld, _, _ = lfmm3dparttarg(iprec, source, charge, dipstr, dipvec, None, ifpot, iffld,
+                                         0, 0, precision)
+    return ier, pot, fld
+
+
+def l3dpartdirect(source, charge=None, dipstr=None, dipvec=None, target=None,
+                  ifpot=1, iffld=1, ifpottarg=1, iffldtarg=1):
+    """O(N^2) direct sum on the GPU (l3dpartdirect, src/lfmm3dpart.f:1391)."""
+    lib = load()
+    src, q, ds, dv, trg = _prep(source, charge, dipstr, dipvec, target)
+    ns = src.shape[0]
+    nt = 0 if trg is None else trg.shape[0]
+    pot = np.zeros(ns, np.complex128) if ifpot else None
+    fld = np.zeros((ns, 3), np.complex128) if iffld else None
+    pt = np.zeros(nt, np.complex128) if (ifpottarg and nt) else None
+    ft = np.zeros((nt, 3), np.complex128) if (iffldtarg and nt) else None
+    lib.fmm3d_l3dpartdirect(ns, _ptr(src), int(q is not None), _ptr(q), int(ds is not None),
+                            _ptr(ds), _ptr(dv), int(bool(ifpot)), _ptr(pot), int(bool(iffld)),
+                            _ptr(fld), nt, _ptr(trg), int(pt is not None), _ptr(pt),
+                            int(ft is not None), _ptr(ft))
+    return pot, fld, pt, ft
+
+
+def last_timings():
+    """Per-stage milliseconds of the last call on this thread (see
+    fmm3d_last_timings in include/fmm3d_mi355x.h)."""
+    out = np.zeros(11)
+    load().fmm3d_last_timings(_ptr(out), 11)
+    keys = ("tree_plan", "upload", "p2m", "p2l", "m2m", "m2l", "l2l", "l2p", "m2p", "p2p", "total")
+    return dict(zip(keys, out.tolist()))

@@ -1,0 +1,187 @@
+// C ABI of libfmm3d_mi355x.so (declared in include/fmm3d_mi355x.h).
+// Argument checking and ier mapping follow lfmm3dparttarg
+// (src/lfmm3dpart.f:138-522); host arrays are uploaded once, the solve runs
+// on the device and the results come back in the caller's order.
+#include <hip/hip_runtime.h>
+
+#include <cstring>
+#include <vector>
+
+#include "../../include/fmm3d_mi355x.h"
+#include "fmm3d_internal.hpp"
+
+using namespace fmm3d;
+
+namespace {
+
+template <typename F>
+int guarded(F&& f) {
+  try {
+    return f();
+  } catch (const AllocError& e) {
+    return e.ier;
+  } catch (const HipError&) {
+    (void)hipGetLastError();
+    return 64;
+  } catch (const std::bad_alloc&) {
+    return 8;
+  }
+}
+
+double* upload(Context& ctx, const char* name, const double* h, size_t n) {
+  if (!h || n == 0) return nullptr;
+  double* d = (double*)ctx.arena.get(name, n * sizeof(double));
+  FMM_HIP(hipMemcpyAsync(d, h, n * sizeof(double), hipMemcpyHostToDevice, ctx.stream));
+  return d;
+}
+
+void download(Context& ctx, double* h, const double* d, size_t n) {
+  if (!h || !d || n == 0) return;
+  FMM_HIP(hipMemcpyAsync(h, d, n * sizeof(double), hipMemcpyDeviceToHost, ctx.stream));
+}
+
+int lap_host(int prec_mode, int iprec, int ns, const double* source, int ifcharge,
+             const double* charge, int ifdipole, const double* dipstr, const double* dipvec,
+             int ifpot, double* pot, int iffld, double* fld, int nt, const double* target,
+             int ifpottarg, double* pottarg, int iffldtarg, double* fldtarg) {
+  if (nt < 0) return 128;
+  if (ns < 1) return 4;  // lfmm3dpart.f:309-330 after d3tstrcr's ier=128
+  if (iprec < -2 || iprec > 6) return 32;
+  if (prec_mode != FMM3D_PREC_FP64 && prec_mode != FMM3D_PREC_MIXED) return 128;
+  Context& ctx = default_context();
+  std::lock_guard<std::mutex> lock(ctx.mu);
+  return guarded([&] {
+    LapCall c;
+    c.iprec = iprec; c.ns = ns; c.nt = nt;
+    c.ifcharge = ifcharge == 1; c.ifdipole = ifdipole == 1;
+    c.ifpot = ifpot == 1; c.iffld = iffld == 1;
+    c.ifpottarg = nt > 0 && ifpottarg == 1; c.iffldtarg = nt > 0 && iffldtarg == 1;
+    c.src_host = source;
+    c.trg_host = nt > 0 ? target : nullptr;
+    c.src_dev = upload(ctx, "in_src", source, 3 * (size_t)ns);
+    c.q_dev = c.ifcharge ? upload(ctx, "in_q", charge, 2 * (size_t)ns) : nullptr;
+    c.ds_dev = c.ifdipole ? upload(ctx, "in_ds", dipstr, 2 * (size_t)ns) : nullptr;
+    c.dv_dev = c.ifdipole ? upload(ctx, "in_dv", dipvec, 3 * (size_t)ns) : nullptr;
+    c.trg_dev = nt > 0 ? upload(ctx, "in_trg", target, 3 * (size_t)nt) : nullptr;
+    if (c.ifpot) c.pot_dev = (double*)ctx.arena.get("out_pot", 2 * sizeof(double) * ns);
+    if (c.iffld) c.fld_dev = (double*)ctx.arena.get("out_fld", 6 * sizeof(double) * ns);
+    if (c.ifpottarg) c.pottarg_dev = (double*)ctx.arena.get("out_pt", 2 * sizeof(double) * nt);
+    if (c.iffldtarg) c.fldtarg_dev = (double*)ctx.arena.get("out_ft", 6 * sizeof(double) * nt);
+    int ier = prec_mode == FMM3D_PREC_MIXED ? lap_solve<float>(ctx, c) : lap_solve<double>(ctx, c);
+    if (ier) return ier;
+    if (c.ifpot) download(ctx, pot, c.pot_dev, 2 * (size_t)ns);
+    if (c.iffld) download(ctx, fld, c.fld_dev, 6 * (size_t)ns);
+    if (c.ifpottarg) download(ctx, pottarg, c.pottarg_dev, 2 * (size_t)nt);
+    if (c.iffldtarg) download(ctx, fldtarg, c.fldtarg_dev, 6 * (size_t)nt);
+    FMM_HIP(hipStreamSynchronize(ctx.stream));
+    return 0;
+  });
+}
+
+}  // namespace
+
+extern "C" {
+
+int fmm3d_lfmm3dparttarg(int iprec, int nsource, const double* source, int ifcharge,
+                         const double* charge, int ifdipole, const double* dipstr,
+                         const double* dipvec, int ifpot, double* pot, int iffld, double* fld,
+                         int ntarget, const double* target, int ifpottarg, double* pottarg,
+                         int iffldtarg, double* fldtarg) {
+  return lap_host(FMM3D_PREC_FP64, iprec, nsource, source, ifcharge, charge, ifdipole, dipstr,
+                  dipvec, ifpot, pot, iffld, fld, ntarget, target, ifpottarg, pottarg, iffldtarg,
+                  fldtarg);
+}
+
+int fmm3d_lfmm3dparttarg_ex(int prec_mode, int iprec, int nsource, const double* source,
+                            int ifcharge, const double* charge, int ifdipole, const double* dipstr,
+                            const double* dipvec, int ifpot, double* pot, int iffld, double* fld,
+                            int ntarget, const double* target, int ifpottarg, double* pottarg,
+                            int iffldtarg, double* fldtarg) {
+  return lap_host(prec_mode, iprec, nsource, source, ifcharge, charge, ifdipole, dipstr, dipvec,
+                  ifpot, pot, iffld, fld, ntarget, target, ifpottarg, pottarg, iffldtarg, fldtarg);
+}
+
+int fmm3d_lfmm3dpartself(int iprec, int nsource, const double* source, int ifcharge,
+                         const double* charge, int ifdipole, const double* dipstr,
+                         const double* dipvec, int ifpot, double* pot, int iffld, double* fld) {
+  return lap_host(FMM3D_PREC_FP64, iprec, nsource, source, ifcharge, charge, ifdipole, dipstr,
+                  dipvec, ifpot, pot, iffld, fld, 0, nullptr, 0, nullptr, 0, nullptr);
+}
+
+int fmm3d_lfmm3dparttarg_dev(int prec_mode, int iprec, int ns, const double* source_dev,
+                             int ifcharge, const double* charge_dev, int ifdipole,
+                             const double* dipstr_dev, const double* dipvec_dev, int ifpot,
+                             double* pot_dev, int iffld, double* fld_dev, int nt,
+                             const double* target_dev, int ifpottarg, double* pottarg_dev,
+                             int iffldtarg, double* fldtarg_dev, void* stream) {
+  if (nt < 0) return 128;
+  if (ns < 1) return 4;
+  if (iprec < -2 || iprec > 6) return 32;
+  if (prec_mode != FMM3D_PREC_FP64 && prec_mode != FMM3D_PREC_MIXED) return 128;
+  Context& ctx = default_context();
+  std::lock_guard<std::mutex> lock(ctx.mu);
+  hipStream_t saved = ctx.stream;
+  ctx.stream = (hipStream_t)stream;
+  int ier = guarded([&] {
+    // the octree is built on the host: bring the coordinates back
+    std::vector<double> hs(3 * (size_t)ns), ht(3 * (size_t)std::max(nt, 0));
+    FMM_HIP(hipMemcpyAsync(hs.data(), source_dev, hs.size() * sizeof(double), hipMemcpyDeviceToHost, ctx.stream));
+    if (nt > 0)
+      FMM_HIP(hipMemcpyAsync(ht.data(), target_dev, ht.size() * sizeof(double), hipMemcpyDeviceToHost, ctx.stream));
+    FMM_HIP(hipStreamSynchronize(ctx.stream));
+    LapCall c;
+    c.iprec = iprec; c.ns = ns; c.nt = nt;
+    c.ifcharge = ifcharge == 1; c.ifdipole = ifdipole == 1;
+    c.ifpot = ifpot == 1; c.iffld = iffld == 1;
+    c.ifpottarg = nt > 0 && ifpottarg == 1; c.iffldtarg = nt > 0 && iffldtarg == 1;
+    c.src_host = hs.data(); c.trg_host = nt > 0 ? ht.data() : nullptr;
+    c.src_dev = source_dev; c.q_dev = charge_dev; c.ds_dev = dipstr_dev; c.dv_dev = dipvec_dev;
+    c.trg_dev = target_dev;
+    c.pot_dev = pot_dev; c.fld_dev = fld_dev; c.pottarg_dev = pottarg_dev; c.fldtarg_dev = fldtarg_dev;
+    return prec_mode == FMM3D_PREC_MIXED ? lap_solve<float>(ctx, c) : lap_solve<double>(ctx, c);
+  });
+  ctx.stream = saved;
+  return ier;
+}
+
+void fmm3d_l3dpartdirect(int nsource, const double* source, int ifcharge, const double* charge,
+                         int ifdipole, const double* dipstr, const double* dipvec, int ifpot,
+                         double* pot, int iffld, double* fld, int ntarget, const double* target,
+                         int ifpottarg, double* pottarg, int iffldtarg, double* fldtarg) {
+  if (nsource < 0 || ntarget < 0) return;
+  Context& ctx = default_context();
+  std::lock_guard<std::mutex> lock(ctx.mu);
+  (void)guarded([&] {
+    LapCall c;
+    c.ns = nsource; c.nt = ntarget;
+    c.ifcharge = ifcharge == 1; c.ifdipole = ifdipole == 1;
+    c.ifpot = ifpot == 1; c.iffld = iffld == 1;
+    c.ifpottarg = ifpottarg == 1; c.iffldtarg = iffldtarg == 1;
+    c.src_dev = upload(ctx, "in_src", source, 3 * (size_t)nsource);
+    c.q_dev = c.ifcharge ? upload(ctx, "in_q", charge, 2 * (size_t)nsource) : nullptr;
+    c.ds_dev = c.ifdipole ? upload(ctx, "in_ds", dipstr, 2 * (size_t)nsource) : nullptr;
+    c.dv_dev = c.ifdipole ? upload(ctx, "in_dv", dipvec, 3 * (size_t)nsource) : nullptr;
+    c.trg_dev = ntarget > 0 ? upload(ctx, "in_trg", target, 3 * (size_t)ntarget) : nullptr;
+    if (c.ifpot) c.pot_dev = (double*)ctx.arena.get("out_pot", 2 * sizeof(double) * std::max(nsource, 1));
+    if (c.iffld) c.fld_dev = (double*)ctx.arena.get("out_fld", 6 * sizeof(double) * std::max(nsource, 1));
+    if (c.ifpottarg) c.pottarg_dev = (double*)ctx.arena.get("out_pt", 2 * sizeof(double) * std::max(ntarget, 1));
+    if (c.iffldtarg) c.fldtarg_dev = (double*)ctx.arena.get("out_ft", 6 * sizeof(double) * std::max(ntarget, 1));
+    int ier = lap_direct(ctx, c);
+    if (ier) return ier;
+    if (c.ifpot) download(ctx, pot, c.pot_dev, 2 * (size_t)nsource);
+    if (c.iffld) download(ctx, fld, c.fld_dev, 6 * (size_t)nsource);
+    if (c.ifpottarg) download(ctx, pottarg, c.pottarg_dev, 2 * (size_t)ntarget);
+    if (c.iffldtarg) download(ctx, fldtarg, c.fldtarg_dev, 6 * (size_t)ntarget);
+    FMM_HIP(hipStreamSynchronize(ctx.stream));
+    return 0;
+  });
+}
+
+void fmm3d_last_timings(double* ms, int n) {
+  const auto& t = last_timings();
+  for (int i = 0; i < n; i++) ms[i] = i < (int)t.size() ? t[i] : 0.0;
+}
+
+const char* fmm3d_version(void) { return "fmm3d-mi355x 0.1 (gfx950)"; }
+
+}  // extern "C"

@@ -1,0 +1,23 @@
+// Internal interfaces between the C ABI layer and the FMM drivers.
+#pragma once
+#include "runtime.hpp"
+
+namespace fmm3d {
+
+// One Laplace evaluation.  *_host pointers are host copies of the
+// coordinates (the octree is built on the host); *_dev pointers are device
+// buffers in the reference layouts (complex as interleaved doubles).
+struct LapCall {
+  int iprec = 1, ns = 0, nt = 0;
+  int ifcharge = 0, ifdipole = 0, ifpot = 0, iffld = 0, ifpottarg = 0, iffldtarg = 0;
+  const double *src_host = nullptr, *trg_host = nullptr;
+  const double *src_dev = nullptr, *q_dev = nullptr, *ds_dev = nullptr, *dv_dev = nullptr,
+               *trg_dev = nullptr;
+  double *pot_dev = nullptr, *fld_dev = nullptr, *pottarg_dev = nullptr, *fldtarg_dev = nullptr;
+};
+
+template <typename T>
+int lap_solve(Context& ctx, const LapCall& c);
+int lap_direct(Context& ctx, const LapCall& c);
+
+}  // namespace fmm3d

@@ -1,0 +1,437 @@
+// Laplace particle FMM driver: host octree -> flat plan -> one upload ->
+// the eight FMM steps of lfmm3dparttargmain (src/lfmm3dpart.f:528-1169) as
+// HIP kernels on one stream -> scatter back to the caller's order.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "fmm3d_internal.hpp"
+#include "lap_kernels.hip"
+#include "octree.hpp"
+#include "runtime.hpp"
+#include "tables.hpp"
+
+namespace fmm3d {
+
+namespace {
+
+constexpr int kP2PBlock = 256;
+
+// iprec -> tolerance and leaf size (src/lfmm3dpart.f:277-300)
+bool iprec_params(int iprec, int ns, int nt, double& eps, int& nbox) {
+  static const double e[] = {.5e0, .5e-1, .5e-2, .5e-3, .5e-6, .5e-9, .5e-12, .5e-15, 0.0};
+  static const int nb[] = {40, 50, 80, 160, 400, 800, 1200, 1400, 0};
+  if (iprec < -2 || iprec > 6) return false;
+  eps = e[iprec + 2];
+  nbox = (iprec == 6) ? ns + nt : nb[iprec + 2];
+  return true;
+}
+
+struct Flat {
+  std::vector<int> I;
+  std::vector<double> D;
+  size_t i(const std::vector<int>& v) {
+    size_t o = I.size();
+    I.insert(I.end(), v.begin(), v.end());
+    while (I.size() % 16) I.push_back(0);
+    return o;
+  }
+  size_t d(const std::vector<double>& v) {
+    size_t o = D.size();
+    D.insert(D.end(), v.begin(), v.end());
+    while (D.size() % 8) D.push_back(0.0);
+    return o;
+  }
+};
+
+struct PlanOffsets {
+  size_t level, isrc, nsrc, itrg, ntrg, nchild, child, ijk, l1o, l1i, l2o, l2i, l3o, l3i, l4o, l4i;
+  size_t p2m, l2p, m2p, p2l, m2l, m2m, l2l, p2p0, p2p1, sperm, tperm;
+  size_t center, inv_side;
+  int n_p2m, n_l2p, n_m2p, n_p2l, n_m2l, n_p2p0, n_p2p1;
+  std::vector<std::pair<int, int>> m2m_groups, l2l_groups;  // (first, count) in launch order
+};
+
+struct Flags {
+  int ifcharge, ifdipole, ifpot, iffld, ifpottarg, iffldtarg;
+};
+
+void build_plan(const Octree& t, const Flags& f, Flat& F, PlanOffsets& O) {
+  const int nb = t.nboxes;
+  const bool want_src = f.ifpot || f.iffld, want_trg = f.ifpottarg || f.iffldtarg;
+  O.level = F.i(t.level); O.isrc = F.i(t.isrc); O.nsrc = F.i(t.nsrc);
+  O.itrg = F.i(t.itrg); O.ntrg = F.i(t.ntrg); O.nchild = F.i(t.nchild);
+  O.child = F.i(t.child); O.ijk = F.i(t.ijk);
+  O.l1o = F.i(t.list[0].off); O.l1i = F.i(t.list[0].idx);
+  // list 2 pruned as lfmm3drouts.f:385-387: sourceless sources, and
+  // targetless targets when no source outputs are wanted
+  {
+    std::vector<int> off(nb + 1, 0), idx;
+    for (int b = 0; b < nb; b++) {
+      off[b] = (int)idx.size();
+      bool skip_b = t.level[b] < 2 || (!want_src && t.ntrg[b] == 0);
+      if (!skip_b)
+        for (int k = t.list[1].off[b]; k < t.list[1].off[b + 1]; k++) {
+          int j = t.list[1].idx[k];
+          if (t.nsrc[j] > 0) idx.push_back(j);
+        }
+    }
+    off[nb] = (int)idx.size();
+    O.l2o = F.i(off); O.l2i = F.i(idx);
+    std::vector<int> m2l;
+    for (int b = 0; b < nb; b++)
+      if (off[b + 1] > off[b]) m2l.push_back(b);
+    O.m2l = F.i(m2l); O.n_m2l = (int)m2l.size();
+  }
+  O.l3o = F.i(t.list[2].off); O.l3i = F.i(t.list[2].idx);
+  O.l4o = F.i(t.list[3].off); O.l4i = F.i(t.list[3].idx);
+  std::vector<int> p2m, l2p, m2p, p2l, p2p0, p2p1;
+  for (int b = 0; b < nb; b++) {
+    const bool leaf = t.nchild[b] == 0;
+    const bool has_pts = (want_src && t.nsrc[b] > 0) || (want_trg && t.ntrg[b] > 0);
+    if (leaf && t.nsrc[b] > 0) p2m.push_back(b);
+    if (leaf && t.level[b] >= 2 && has_pts) l2p.push_back(b);
+    if (t.list[2].count(b) > 0 && has_pts) m2p.push_back(b);
+    if (t.list[3].count(b) > 0) {
+      bool any = false;
+      for (int k = t.list[3].off[b]; k < t.list[3].off[b + 1]; k++) any |= t.nsrc[t.list[3].idx[k]] > 0;
+      if (any) p2l.push_back(b);
+    }
+    if (leaf && want_src)
+      for (int s = 0; s < t.nsrc[b]; s += kP2PBlock) { p2p0.push_back(b); p2p0.push_back(0); p2p0.push_back(s); }
+    if (leaf && want_trg)
+      for (int s = 0; s < t.ntrg[b]; s += kP2PBlock) { p2p1.push_back(b); p2p1.push_back(1); p2p1.push_back(s); }
+  }
+  O.p2m = F.i(p2m); O.n_p2m = (int)p2m.size();
+  O.l2p = F.i(l2p); O.n_l2p = (int)l2p.size();
+  O.m2p = F.i(m2p); O.n_m2p = (int)m2p.size();
+  O.p2l = F.i(p2l); O.n_p2l = (int)p2l.size();
+  O.p2p0 = F.i(p2p0); O.n_p2p0 = (int)p2p0.size() / 3;
+  O.p2p1 = F.i(p2p1); O.n_p2p1 = (int)p2p1.size() / 3;
+  // M2M parents, levels nlev-1 .. 2 (lfmm3drouts.f:195-258); L2L parents,
+  // levels 2 .. nlev-1 (lfmm3drouts.f:498-558)
+  std::vector<int> m2m, l2l;
+  O.m2m_groups.clear();
+  O.l2l_groups.clear();
+  for (int l = t.nlev - 1; l >= 2; l--) {
+    int first = (int)m2m.size();
+    for (int b = t.ladd[l]; b < t.ladd[l + 1]; b++)
+      if (t.nchild[b] > 0 && t.nsrc[b] > 0) m2m.push_back(b);
+    if ((int)m2m.size() > first) O.m2m_groups.push_back({first, (int)m2m.size() - first});
+  }
+  for (int l = 2; l <= t.nlev - 1; l++) {
+    int first = (int)l2l.size();
+    for (int b = t.ladd[l]; b < t.ladd[l + 1]; b++)
+      if (t.nchild[b] > 0) l2l.push_back(b);
+    if ((int)l2l.size() > first) O.l2l_groups.push_back({first, (int)l2l.size() - first});
+  }
+  O.m2m = F.i(m2m);
+  O.l2l = F.i(l2l);
+  O.sperm = F.i(t.src_perm);
+  O.tperm = F.i(t.trg_perm);
+  O.center = F.d(t.bcenter);
+  std::vector<double> inv(t.nlev + 1);
+  for (int l = 0; l <= t.nlev; l++) inv[l] = 1.0 / t.side(l);
+  O.inv_side = F.d(inv);
+}
+
+DevPlan make_devplan(const Octree& t, int p, const PlanOffsets& O, const int* I, const double* D) {
+  DevPlan P;
+  P.nboxes = t.nboxes; P.nlev = t.nlev; P.p = p; P.K = (p + 1) * (p + 1);
+  P.ns = (int)t.src_perm.size(); P.nt = (int)t.trg_perm.size();
+  P.level = I + O.level; P.isrc = I + O.isrc; P.nsrc = I + O.nsrc; P.itrg = I + O.itrg;
+  P.ntrg = I + O.ntrg; P.nchild = I + O.nchild; P.child = I + O.child; P.ijk = I + O.ijk;
+  P.center = D + O.center; P.inv_side = D + O.inv_side;
+  P.l1_off = I + O.l1o; P.l1_idx = I + O.l1i; P.l2_off = I + O.l2o; P.l2_idx = I + O.l2i;
+  P.l3_off = I + O.l3o; P.l3_idx = I + O.l3i; P.l4_off = I + O.l4o; P.l4_idx = I + O.l4i;
+  P.p2m_boxes = I + O.p2m; P.n_p2m = O.n_p2m;
+  P.l2p_boxes = I + O.l2p; P.n_l2p = O.n_l2p;
+  P.m2p_boxes = I + O.m2p; P.n_m2p = O.n_m2p;
+  P.p2l_boxes = I + O.p2l; P.n_p2l = O.n_p2l;
+  P.m2l_boxes = I + O.m2l; P.n_m2l = O.n_m2l;
+  P.m2m_boxes = I + O.m2m; P.m2m_lev_off = nullptr;
+  P.l2l_boxes = I + O.l2l; P.l2l_lev_off = nullptr;
+  P.p2p_items = I + O.p2p0; P.n_p2p = O.n_p2p0;
+  P.src_perm = I + O.sperm; P.trg_perm = I + O.tperm;
+  return P;
+}
+
+// Upload (once per order p) the translation tables.
+DevTables ensure_tables(Context& ctx, int p, double eps) {
+  if (ctx.tab_p != p) {
+    HostTables H;
+    build_tables(p, eps, H);
+    // ints: off_rot | off_dist | off_ptrunc | oct_rot ; doubles: fac | off_phase |
+    // oct_phase | rot | z_m2l | z_m2m | z_l2l ; longs: rot_off
+    ctx.tab_host_i.clear();
+    ctx.tab_host_d.clear();
+    auto& Ii = ctx.tab_host_i;
+    auto& Dd = ctx.tab_host_d;
+    Ii.insert(Ii.end(), H.off_rot.begin(), H.off_rot.end());
+    Ii.insert(Ii.end(), H.off_dist.begin(), H.off_dist.end());
+    Ii.insert(Ii.end(), H.off_ptrunc.begin(), H.off_ptrunc.end());
+    Ii.insert(Ii.end(), H.oct_rot.begin(), H.oct_rot.end());
+    size_t o_fac = 0;
+    Dd.insert(Dd.end(), H.fac.begin(), H.fac.end());
+    size_t o_ph = Dd.size();
+    Dd.insert(Dd.end(), H.off_phase.begin(), H.off_phase.end());
+    size_t o_oph = Dd.size();
+    Dd.insert(Dd.end(), H.oct_phase.begin(), H.oct_phase.end());
+    size_t o_rot = Dd.size();
+    Dd.insert(Dd.end(), H.rot.begin(), H.rot.end());
+    size_t o_zl = Dd.size();
+    Dd.insert(Dd.end(), H.z_m2l.begin(), H.z_m2l.end());
+    size_t o_zm = Dd.size();
+    Dd.insert(Dd.end(), H.z_m2m.begin(), H.z_m2m.end());
+    size_t o_zll = Dd.size();
+    Dd.insert(Dd.end(), H.z_l2l.begin(), H.z_l2l.end());
+    ctx.tab_host_l = H.rot_off;
+    ctx.tab_off[0] = o_fac; ctx.tab_off[1] = o_ph; ctx.tab_off[2] = o_oph; ctx.tab_off[3] = o_rot;
+    ctx.tab_off[4] = o_zl; ctx.tab_off[5] = o_zm; ctx.tab_off[6] = o_zll;
+    void* di = ctx.arena.get("tab_i", Ii.size() * sizeof(int));
+    void* dd = ctx.arena.get("tab_d", Dd.size() * sizeof(double));
+    void* dl = ctx.arena.get("tab_l", ctx.tab_host_l.size() * sizeof(long));
+    FMM_HIP(hipMemcpyAsync(di, Ii.data(), Ii.size() * sizeof(int), hipMemcpyHostToDevice, ctx.stream));
+    FMM_HIP(hipMemcpyAsync(dd, Dd.data(), Dd.size() * sizeof(double), hipMemcpyHostToDevice, ctx.stream));
+    FMM_HIP(hipMemcpyAsync(dl, ctx.tab_host_l.data(), ctx.tab_host_l.size() * sizeof(long),
+                           hipMemcpyHostToDevice, ctx.stream));
+    FMM_HIP(hipStreamSynchronize(ctx.stream));
+    ctx.tab_p = p;
+  }
+  const int* di = (const int*)ctx.arena.get("tab_i", ctx.tab_host_i.size() * sizeof(int));
+  const double* dd = (const double*)ctx.arena.get("tab_d", ctx.tab_host_d.size() * sizeof(double));
+  const long* dl = (const long*)ctx.arena.get("tab_l", ctx.tab_host_l.size() * sizeof(long));
+  DevTables T;
+  T.p = p;
+  T.off_rot = di; T.off_dist = di + 343; T.off_ptrunc = di + 686; T.oct_rot = di + 1029;
+  T.fac = dd + ctx.tab_off[0]; T.off_phase = dd + ctx.tab_off[1]; T.oct_phase = dd + ctx.tab_off[2];
+  T.rot = dd + ctx.tab_off[3]; T.z_m2l = dd + ctx.tab_off[4]; T.z_m2m = dd + ctx.tab_off[5];
+  T.z_l2l = dd + ctx.tab_off[6];
+  T.rot_off = dl;
+  return T;
+}
+
+size_t max_lds_bytes(Context& ctx) {
+  static size_t v = 0;
+  if (!v) {
+    hipDeviceProp_t prop;
+    FMM_HIP(hipGetDeviceProperties(&prop, ctx.device));
+    v = prop.sharedMemPerBlock;
+  }
+  return v;
+}
+
+inline unsigned blocks(size_t n, unsigned b) { return (unsigned)((n + b - 1) / b); }
+
+template <typename T>
+void launch_p2p(const DevPlan& P, const int* items, int n, int kind, const T* rec, const T* tpos,
+                int ifcharge, int ifdipole, bool fld, T* pot, T* fo, hipStream_t s) {
+  if (n == 0) return;
+  DevPlan Q = P;
+  Q.p2p_items = items;
+  Q.n_p2p = n;
+  dim3 g(n), b(kP2PBlock);
+#define P2P_CASE(C, Dp, F)                                                                     \
+  if (ifcharge == C && ifdipole == Dp && fld == F) {                                          \
+    hipLaunchKernelGGL((k_p2p<T, C, Dp, F>), g, b, 0, s, Q, rec, tpos, kind, pot, fo);         \
+    return;                                                                                    \
+  }
+  P2P_CASE(1, 0, false) P2P_CASE(1, 0, true) P2P_CASE(0, 1, false) P2P_CASE(0, 1, true)
+  P2P_CASE(1, 1, false) P2P_CASE(1, 1, true)
+#undef P2P_CASE
+}
+
+}  // namespace
+
+template <typename T>
+int lap_solve(Context& ctx, const LapCall& c) {
+  auto t0 = std::chrono::steady_clock::now();
+  double eps;
+  int nbox;
+  if (!iprec_params(c.iprec, c.ns, c.nt, eps, nbox)) return 32;
+  const int ns = c.ns, nt = c.nt;
+  const Flags f{c.ifcharge ? 1 : 0, c.ifdipole ? 1 : 0, c.ifpot ? 1 : 0, c.iffld ? 1 : 0,
+                nt > 0 && c.ifpottarg ? 1 : 0, nt > 0 && c.iffldtarg ? 1 : 0};
+  Octree tree;
+  if (build_octree(c.src_host, ns, c.trg_host, nt, nbox, tree) != 0) return 4;
+  const int p = laplace_nterms(eps);
+  const int K = (p + 1) * (p + 1);
+  if (sizeof(T) * 4 * (size_t)K > max_lds_bytes(ctx)) return 16;
+  Flat F;
+  PlanOffsets O;
+  build_plan(tree, f, F, O);
+  auto t1 = std::chrono::steady_clock::now();
+  hipStream_t s = ctx.stream;
+  Timer tm;
+  tm.start(s, 11);
+  // ---- device buffers
+  int* dI = (int*)ctx.arena.get("plan_i", F.I.size() * sizeof(int));
+  double* dD = (double*)ctx.arena.get("plan_d", F.D.size() * sizeof(double));
+  FMM_HIP(hipMemcpyAsync(dI, F.I.data(), F.I.size() * sizeof(int), hipMemcpyHostToDevice, s));
+  FMM_HIP(hipMemcpyAsync(dD, F.D.data(), F.D.size() * sizeof(double), hipMemcpyHostToDevice, s));
+  DevTables Tb = ensure_tables(ctx, p, eps);
+  DevPlan P = make_devplan(tree, p, O, dI, dD);
+  const size_t nb = tree.nboxes;
+  T* rec = (T*)ctx.arena.get("rec", sizeof(T) * 12 * (size_t)ns);
+  T* tpos = (T*)ctx.arena.get("tpos", sizeof(T) * 3 * (size_t)std::max(nt, 1));
+  T* mp = (T*)ctx.arena.get("mp", sizeof(T) * 2 * K * nb, 16);
+  T* lo = (T*)ctx.arena.get("lo", sizeof(T) * 2 * K * nb, 16);
+  T* pot_s = (T*)ctx.arena.get("pot_s", sizeof(T) * 2 * (size_t)ns);
+  T* fld_s = (T*)ctx.arena.get("fld_s", sizeof(T) * 6 * (size_t)ns);
+  T* pot_t = (T*)ctx.arena.get("pot_t", sizeof(T) * 2 * (size_t)std::max(nt, 1));
+  T* fld_t = (T*)ctx.arena.get("fld_t", sizeof(T) * 6 * (size_t)std::max(nt, 1));
+  const int* sperm = dI + O.sperm;
+  const int* tperm = dI + O.tperm;
+  hipLaunchKernelGGL((k_gather_sources<T>), dim3(blocks(ns, 256)), dim3(256), 0, s, ns, sperm,
+                     c.src_dev, c.q_dev, c.ds_dev, c.dv_dev, f.ifcharge, f.ifdipole, rec);
+  if (nt > 0)
+    hipLaunchKernelGGL((k_gather_targets<T>), dim3(blocks(nt, 256)), dim3(256), 0, s, nt, tperm,
+                       c.trg_dev, tpos);
+  auto zero = [&](T* a, size_t n) {
+    hipLaunchKernelGGL((k_fill_zero<T>), dim3(std::min<size_t>(blocks(n, 256), 4096)), dim3(256),
+                       0, s, n, a);
+  };
+  zero(mp, 2 * (size_t)K * nb);
+  zero(lo, 2 * (size_t)K * nb);
+  zero(pot_s, 2 * (size_t)ns);
+  zero(fld_s, 6 * (size_t)ns);
+  if (nt > 0) {
+    zero(pot_t, 2 * (size_t)nt);
+    zero(fld_t, 6 * (size_t)nt);
+  }
+  tm.mark(1);
+  const double* fac = Tb.fac;
+  // ---- STEP 1: P2M
+  {
+    const int H = (p + 1) * (p + 2) / 2;
+    int ts = (int)((65536 / sizeof(T) - 2 * K) / (2 * H + 11));
+    ts = std::max(1, std::min(32, ts));
+    size_t lds = sizeof(T) * (2 * K + (size_t)ts * (2 * H + 11));
+    if (P.n_p2m)
+      hipLaunchKernelGGL((k_form<T, false>), dim3(P.n_p2m), dim3(256), lds, s, P, fac, rec,
+                         f.ifcharge, f.ifdipole, ts, mp);
+  }
+  tm.mark(2);
+  // ---- STEP 2: P2L (list 4)
+  {
+    const int H = (p + 2) * (p + 3) / 2;
+    int ts = (int)((65536 / sizeof(T) - 2 * K) / (2 * H + 11));
+    ts = std::max(1, std::min(32, ts));
+    size_t lds = sizeof(T) * (2 * K + (size_t)ts * (2 * H + 11));
+    if (P.n_p2l)
+      hipLaunchKernelGGL((k_form<T, true>), dim3(P.n_p2l), dim3(256), lds, s, P, fac, rec,
+                         f.ifcharge, f.ifdipole, ts, lo);
+  }
+  tm.mark(3);
+  // ---- STEP 3: M2M, bottom-up
+  const size_t lds_ps = sizeof(T) * 4 * K;
+  for (auto& g : O.m2m_groups)
+    hipLaunchKernelGGL((k_m2m<T>), dim3(g.second), dim3(256), lds_ps, s, P, Tb, g.first, mp);
+  tm.mark(4);
+  // ---- STEP 4: M2L
+  if (P.n_m2l) hipLaunchKernelGGL((k_m2l<T>), dim3(P.n_m2l), dim3(128), lds_ps, s, P, Tb, mp, lo);
+  tm.mark(5);
+  // ---- STEP 5: L2L, top-down
+  for (auto& g : O.l2l_groups)
+    hipLaunchKernelGGL((k_l2l<T>), dim3(g.second), dim3(256), lds_ps, s, P, Tb, g.first, lo);
+  tm.mark(6);
+  // ---- STEP 7: L2P
+  const size_t lds_e = sizeof(T) * 2 * K;
+  if (P.n_l2p)
+    hipLaunchKernelGGL((k_eval<T, false>), dim3(P.n_l2p), dim3(256), lds_e, s, P, fac, lo, rec, tpos,
+                       f.ifpot, f.iffld, f.ifpottarg, f.iffldtarg, pot_s, fld_s, pot_t, fld_t);
+  tm.mark(7);
+  // ---- STEP 6: M2P (list 3)
+  if (P.n_m2p)
+    hipLaunchKernelGGL((k_eval<T, true>), dim3(P.n_m2p), dim3(256), lds_e, s, P, fac, mp, rec, tpos,
+                       f.ifpot, f.iffld, f.ifpottarg, f.iffldtarg, pot_s, fld_s, pot_t, fld_t);
+  tm.mark(8);
+  // ---- STEP 8: P2P (self + list 1)
+  if (f.ifcharge || f.ifdipole) {
+    launch_p2p<T>(P, dI + O.p2p0, O.n_p2p0, 0, rec, tpos, f.ifcharge, f.ifdipole, f.iffld != 0,
+                  pot_s, fld_s, s);
+    launch_p2p<T>(P, dI + O.p2p1, O.n_p2p1, 1, rec, tpos, f.ifcharge, f.ifdipole, f.iffldtarg != 0,
+                  pot_t, fld_t, s);
+  }
+  tm.mark(9);
+  // ---- scatter back to the caller's order (l3dpsort / l3dfsort)
+  if (f.ifpot) hipLaunchKernelGGL((k_scatter<T>), dim3(blocks(ns, 256)), dim3(256), 0, s, ns, 1, sperm, pot_s, c.pot_dev);
+  if (f.iffld) hipLaunchKernelGGL((k_scatter<T>), dim3(blocks(ns, 256)), dim3(256), 0, s, ns, 3, sperm, fld_s, c.fld_dev);
+  if (f.ifpottarg) hipLaunchKernelGGL((k_scatter<T>), dim3(blocks(nt, 256)), dim3(256), 0, s, nt, 1, tperm, pot_t, c.pottarg_dev);
+  if (f.iffldtarg) hipLaunchKernelGGL((k_scatter<T>), dim3(blocks(nt, 256)), dim3(256), 0, s, nt, 3, tperm, fld_t, c.fldtarg_dev);
+  tm.mark(10);
+  FMM_HIP(hipGetLastError());
+  FMM_HIP(hipStreamSynchronize(s));
+  auto& lt = last_timings();
+  lt.assign(11, 0.0);
+  lt[0] = std::chrono::duration<double, std::milli>(t1 - t0).count();
+  auto el = tm.elapsed();
+  for (int i = 0; i < 9 && i < (int)el.size(); i++) lt[1 + i] = el[i];
+  double tot = 0;
+  for (auto v : el) tot += v;
+  lt[10] = tot + lt[0];
+  return 0;
+}
+
+template int lap_solve<double>(Context&, const LapCall&);
+template int lap_solve<float>(Context&, const LapCall&);
+
+// Direct sum on the device (l3dpartdirect): identity order, one pass.
+int lap_direct(Context& ctx, const LapCall& c) {
+  const int ns = c.ns, nt = c.nt;
+  hipStream_t s = ctx.stream;
+  double* rec = (double*)ctx.arena.get("rec", sizeof(double) * 12 * (size_t)std::max(ns, 1));
+  std::vector<int> id(std::max(ns, nt) + 1);
+  for (size_t i = 0; i < id.size(); i++) id[i] = (int)i;
+  int* dperm = (int*)ctx.arena.get("plan_i", id.size() * sizeof(int));
+  FMM_HIP(hipMemcpyAsync(dperm, id.data(), id.size() * sizeof(int), hipMemcpyHostToDevice, s));
+  double* tpos = (double*)ctx.arena.get("tpos", sizeof(double) * 3 * (size_t)std::max(nt, 1));
+  double* pot = (double*)ctx.arena.get("pot_s", sizeof(double) * 2 * (size_t)std::max(ns, nt));
+  double* fld = (double*)ctx.arena.get("fld_s", sizeof(double) * 6 * (size_t)std::max(ns, nt));
+  if (ns > 0)
+    hipLaunchKernelGGL((k_gather_sources<double>), dim3(blocks(ns, 256)), dim3(256), 0, s, ns, dperm,
+                       c.src_dev, c.q_dev, c.ds_dev, c.dv_dev, c.ifcharge ? 1 : 0, c.ifdipole ? 1 : 0, rec);
+  auto run = [&](int npts, const double* pts, int are_src) {
+    if (npts <= 0) return;
+#define DCASE(C, D)                                                                                 \
+  if ((c.ifcharge ? 1 : 0) == C && (c.ifdipole ? 1 : 0) == D)                                       \
+    hipLaunchKernelGGL((k_direct<double, C, D>), dim3(blocks(npts, 256)), dim3(256), 0, s, ns, rec, \
+                       npts, pts, are_src, pot, fld);
+    DCASE(1, 0) DCASE(0, 1) DCASE(1, 1)
+#undef DCASE
+  };
+  auto copy_out = [&](int npts, int wpot, double* po, int wfld, double* fo) {
+    // direct results are already in caller order (identity permutation)
+    if (wpot) FMM_HIP(hipMemcpyAsync(po, pot, sizeof(double) * 2 * npts, hipMemcpyDeviceToDevice, s));
+    if (wfld) FMM_HIP(hipMemcpyAsync(fo, fld, sizeof(double) * 6 * npts, hipMemcpyDeviceToDevice, s));
+  };
+  if (c.ifpot || c.iffld) {
+    if (ns > 0 && !c.ifcharge && !c.ifdipole) {
+      hipLaunchKernelGGL((k_fill_zero<double>), dim3(blocks(2 * (size_t)ns, 256)), dim3(256), 0, s, 2 * (size_t)ns, pot);
+      hipLaunchKernelGGL((k_fill_zero<double>), dim3(blocks(6 * (size_t)ns, 256)), dim3(256), 0, s, 6 * (size_t)ns, fld);
+    } else {
+      run(ns, nullptr, 1);
+    }
+    copy_out(ns, c.ifpot, c.pot_dev, c.iffld, c.fld_dev);
+  }
+  if (nt > 0 && (c.ifpottarg || c.iffldtarg)) {
+    hipLaunchKernelGGL((k_gather_targets<double>), dim3(blocks(nt, 256)), dim3(256), 0, s, nt, dperm,
+                       c.trg_dev, tpos);
+    if (!c.ifcharge && !c.ifdipole) {
+      hipLaunchKernelGGL((k_fill_zero<double>), dim3(blocks(2 * (size_t)nt, 256)), dim3(256), 0, s, 2 * (size_t)nt, pot);
+      hipLaunchKernelGGL((k_fill_zero<double>), dim3(blocks(6 * (size_t)nt, 256)), dim3(256), 0, s, 6 * (size_t)nt, fld);
+    } else {
+      run(nt, tpos, 0);
+    }
+    copy_out(nt, c.ifpottarg, c.pottarg_dev, c.iffldtarg, c.fldtarg_dev);
+  }
+  FMM_HIP(hipGetLastError());
+  FMM_HIP(hipStreamSynchronize(s));
+  return 0;
+}
+
+}  // namespace fmm3d

@@ -1,0 +1,697 @@
+// Laplace particle-FMM kernels for gfx950 (MI355X).
+//
+// Every kernel is templated on the arithmetic type T (double for the FP64
+// path, float for the mixed path).  Expansions are packed complex arrays,
+// coefficient (n,m) at eidx(n,m), and carry rscale = 1/side of their box, so
+// translation tables are level independent.  Conventions match the
+// reference's "_trunc" routines (src/laprouts3d.f:12-66 with the (-1)^m
+// phase of ylgndrufw): R_n^m = r^n Yhat_n^m, S_n^m = Yhat_n^m / r^{n+1},
+// multipole M_n^m = sum q conj(R_n^m(u)), local L_n^m = sum q conj(S_n^m(u))/h.
+#include <hip/hip_runtime.h>
+
+#include "lap_kernels.hpp"
+
+namespace fmm3d {
+
+// ---------------------------------------------------------------------------
+// small complex helpers
+// ---------------------------------------------------------------------------
+template <typename T>
+struct C2 {
+  T re, im;
+};
+template <typename T>
+__device__ __forceinline__ C2<T> cmul(C2<T> a, C2<T> b) {
+  return {a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re};
+}
+template <typename T>
+__device__ __forceinline__ C2<T> cconj(C2<T> a) { return {a.re, -a.im}; }
+template <typename T>
+__device__ __forceinline__ void cfma(C2<T>& acc, C2<T> a, C2<T> b) {
+  acc.re += a.re * b.re - a.im * b.im;
+  acc.im += a.re * b.im + a.im * b.re;
+}
+template <typename T>
+__device__ __forceinline__ C2<T> cld(const T* p, int k) { return {p[2 * k], p[2 * k + 1]}; }
+template <typename T>
+__device__ __forceinline__ void cst(T* p, int k, C2<T> v) { p[2 * k] = v.re; p[2 * k + 1] = v.im; }
+template <typename T>
+__device__ __forceinline__ void cadd(T* p, int k, C2<T> v) { p[2 * k] += v.re; p[2 * k + 1] += v.im; }
+
+// Factor table layout (built on the host for an order p, doubles):
+//   [F_MM + m]      sqrt((2m-1)/(2m))          m=1..p+1
+//   [F_M1 + m]      sqrt(2m+1)                 m=0..p+1
+//   [F_RA + h(n,m)] sqrt((n+m-1)(n-m-1))       n>=m+2 (recurrence)
+//   [F_RB + h(n,m)] 1/sqrt((n-m)(n+m))
+//   [F_AR + e(n,m)], [F_BR..], [F_CR..]  derivative factors of R (n<=p+1)
+//   [F_AS + e(n,m)], [F_BS..], [F_CS..]  derivative factors of S (n<=p+1)
+struct FacLayout {
+  int MM, M1, RA, RB, AR, BR, CR, AS, BS, CS, total;
+  __host__ __device__ explicit FacLayout(int p) {
+    int P = p + 2;                       // tables cover degrees up to p+1
+    int H = P * (P + 1) / 2, E = P * P;
+    MM = 0; M1 = MM + P + 1; RA = M1 + P + 1; RB = RA + H; AR = RB + H;
+    BR = AR + E; CR = BR + E; AS = CR + E; BS = AS + E; CS = BS + E; total = CS + E;
+  }
+};
+
+// Regular solid harmonics R_n^m(u), 0<=m<=n<=pmax, for one point into a half
+// table (stride `st` between entries).  Column recurrence in n per m.
+template <typename T>
+__device__ __forceinline__ void regular_column(const double* fac, const FacLayout& L, int pmax,
+                                               int m, T ux, T uy, T uz, T r2, T* out, int st) {
+  // R_m^m = prod_k (-(ux+i uy)) sqrt((2k-1)/(2k))
+  C2<T> rmm = {T(1), T(0)};
+  for (int k = 1; k <= m; k++) {
+    T f = (T)fac[L.MM + k];
+    C2<T> w = {-ux * f, -uy * f};
+    rmm = cmul(w, rmm);
+  }
+  cst(out, hidx(m, m) * st / 1, rmm);
+  if (m + 1 > pmax) return;
+  C2<T> r1 = {(T)fac[L.M1 + m] * uz * rmm.re, (T)fac[L.M1 + m] * uz * rmm.im};
+  cst(out, hidx(m + 1, m) * st, r1);
+  C2<T> r0 = rmm;
+  for (int n = m + 2; n <= pmax; n++) {
+    T a = (T)(2 * n - 1) * uz, b = (T)fac[L.RA + hidx(n, m)] * r2, c = (T)fac[L.RB + hidx(n, m)];
+    C2<T> rn = {(a * r1.re - b * r0.re) * c, (a * r1.im - b * r0.im) * c};
+    cst(out, hidx(n, m) * st, rn);
+    r0 = r1;
+    r1 = rn;
+  }
+}
+
+// Irregular solid harmonics S_n^m(u) for 0<=m<=n<=pmax.
+template <typename T>
+__device__ __forceinline__ void irregular_column(const double* fac, const FacLayout& L, int pmax,
+                                                 int m, T ux, T uy, T uz, T ir2, T ir, T* out,
+                                                 int st) {
+  C2<T> smm = {ir, T(0)};
+  for (int k = 1; k <= m; k++) {
+    T f = (T)fac[L.MM + k] * ir2;
+    C2<T> w = {-ux * f, -uy * f};
+    smm = cmul(w, smm);
+  }
+  cst(out, hidx(m, m) * st, smm);
+  if (m + 1 > pmax) return;
+  T g = (T)fac[L.M1 + m] * uz * ir2;
+  C2<T> s1 = {g * smm.re, g * smm.im};
+  cst(out, hidx(m + 1, m) * st, s1);
+  C2<T> s0 = smm;
+  for (int n = m + 2; n <= pmax; n++) {
+    T a = (T)(2 * n - 1) * uz, b = (T)fac[L.RA + hidx(n, m)], c = (T)fac[L.RB + hidx(n, m)] * ir2;
+    C2<T> sn = {(a * s1.re - b * s0.re) * c, (a * s1.im - b * s0.im) * c};
+    cst(out, hidx(n, m) * st, sn);
+    s0 = s1;
+    s1 = sn;
+  }
+}
+
+// value of a harmonic with signed order from a half table
+template <typename T>
+__device__ __forceinline__ C2<T> half_get(const T* tab, int n, int m, int st) {
+  if (m >= 0) return cld(tab, hidx(n, m) * st);
+  return cconj(cld(tab, hidx(n, -m) * st));
+}
+
+// ---------------------------------------------------------------------------
+// gather / scatter / fill
+// ---------------------------------------------------------------------------
+
+// Sorted source records, 12 values per source:
+//   x y z | q.re q.im | vx.re vx.im vy.re vy.im vz.re vz.im | 0
+// where v = dipstr * dipvec (complex dipole moment vector).
+template <typename T>
+__global__ void k_gather_sources(int ns, const int* __restrict__ perm, const double* __restrict__ src,
+                                 const double* __restrict__ q, const double* __restrict__ ds,
+                                 const double* __restrict__ dv, int ifcharge, int ifdipole,
+                                 T* __restrict__ rec) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= ns) return;
+  int o = perm[i];
+  T* r = rec + 12 * (size_t)i;
+  r[0] = (T)src[3 * o]; r[1] = (T)src[3 * o + 1]; r[2] = (T)src[3 * o + 2];
+  r[3] = ifcharge ? (T)q[2 * o] : T(0);
+  r[4] = ifcharge ? (T)q[2 * o + 1] : T(0);
+  if (ifdipole) {
+    double dr = ds[2 * o], di = ds[2 * o + 1];
+    for (int k = 0; k < 3; k++) {
+      double v = dv[3 * o + k];
+      r[5 + 2 * k] = (T)(dr * v);
+      r[6 + 2 * k] = (T)(di * v);
+    }
+  } else {
+    for (int k = 5; k < 11; k++) r[k] = T(0);
+  }
+  r[11] = T(0);
+}
+
+template <typename T>
+__global__ void k_gather_targets(int nt, const int* __restrict__ perm, const double* __restrict__ trg,
+                                 T* __restrict__ tp) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nt) return;
+  int o = perm[i];
+  tp[3 * (size_t)i] = (T)trg[3 * o];
+  tp[3 * (size_t)i + 1] = (T)trg[3 * o + 1];
+  tp[3 * (size_t)i + 2] = (T)trg[3 * o + 2];
+}
+
+// out[perm[i]] = sorted[i]; `w` complex values per point (1 = pot, 3 = fld)
+template <typename T>
+__global__ void k_scatter(int n, int w, const int* __restrict__ perm, const T* __restrict__ sorted,
+                          double* __restrict__ out) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  int o = perm[i];
+  for (int k = 0; k < 2 * w; k++) out[(size_t)2 * w * o + k] = (double)sorted[(size_t)2 * w * i + k];
+}
+
+template <typename T>
+__global__ void k_fill_zero(size_t n, T* __restrict__ a) {
+  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  size_t stride = (size_t)gridDim.x * blockDim.x;
+  for (; i < n; i += stride) a[i] = T(0);
+}
+
+// ---------------------------------------------------------------------------
+// P2M (src/laprouts3d.f:3210, :3835) and P2L (src/laprouts3d.f:3550, :4207)
+// One workgroup per box.  Sources are processed in tiles of `ts`: phase A
+// fills an LDS table of harmonics (one thread per (source, m) column),
+// phase B contracts it against the strengths (one thread per coefficient).
+// IRREG selects P2L (irregular harmonics at the target box centre, sources
+// taken from the boxes of list 4) instead of P2M.
+// ---------------------------------------------------------------------------
+template <typename T, bool IRREG>
+__global__ void k_form(DevPlan P, const double* __restrict__ fac, const T* __restrict__ rec,
+                       int ifcharge, int ifdipole, int ts, T* __restrict__ expn) {
+  extern __shared__ unsigned char smem_raw[];
+  T* smem = reinterpret_cast<T*>(smem_raw);
+  const int p = P.p, K = P.K;
+  const int pt = IRREG ? p + 1 : p;  // table degree (dipoles need one more for S)
+  const int H = (pt + 1) * (pt + 2) / 2;
+  FacLayout L(p);
+  T* acc = smem;                    // 2*K
+  T* tab = acc + 2 * K;             // ts * H complex
+  T* str = tab + 2 * (size_t)ts * H; // ts * 8 : q, dz, du, dd (complex)
+  T* pos = str + 8 * ts;            // ts * 3 scaled offsets
+
+  const int b = IRREG ? P.p2l_boxes[blockIdx.x] : P.p2m_boxes[blockIdx.x];
+  const T ih = (T)P.inv_side[P.level[b]];
+  const double cx = P.center[3 * b], cy = P.center[3 * b + 1], cz = P.center[3 * b + 2];
+  for (int k = threadIdx.x; k < 2 * K; k += blockDim.x) acc[k] = T(0);
+
+  int jbeg, jend;
+  if (IRREG) { jbeg = P.l4_off[b]; jend = P.l4_off[b + 1]; }
+  else { jbeg = 0; jend = 1; }
+  for (int jj = jbeg; jj < jend; jj++) {
+    const int j = IRREG ? P.l4_idx[jj] : b;
+    const int s0 = P.isrc[j], sn = P.nsrc[j];
+    for (int t0 = 0; t0 < sn; t0 += ts) {
+      const int tn = min(ts, sn - t0);
+      __syncthreads();
+      for (int s = threadIdx.x; s < tn; s += blockDim.x) {
+        const T* r = rec + 12 * (size_t)(s0 + t0 + s);
+        pos[3 * s] = (T)(((double)r[0] - cx) * (double)ih);
+        pos[3 * s + 1] = (T)(((double)r[1] - cy) * (double)ih);
+        pos[3 * s + 2] = (T)(((double)r[2] - cz) * (double)ih);
+        // charge, and dipole components dz = v_z, du = (v_x - i v_y)/2, dd = (v_x + i v_y)/2
+        str[8 * s] = r[3]; str[8 * s + 1] = r[4];
+        str[8 * s + 2] = r[9]; str[8 * s + 3] = r[10];
+        str[8 * s + 4] = T(0.5) * (r[5] + r[8]); str[8 * s + 5] = T(0.5) * (r[6] - r[7]);
+        str[8 * s + 6] = T(0.5) * (r[5] - r[8]); str[8 * s + 7] = T(0.5) * (r[6] + r[7]);
+      }
+      __syncthreads();
+      // phase A: harmonic columns
+      for (int task = threadIdx.x; task < tn * (pt + 1); task += blockDim.x) {
+        int s = task / (pt + 1), m = task - s * (pt + 1);
+        T ux = pos[3 * s], uy = pos[3 * s + 1], uz = pos[3 * s + 2];
+        T r2 = ux * ux + uy * uy + uz * uz;
+        if (IRREG) {
+          T ir2 = T(1) / r2, ir = sqrt(ir2);
+          irregular_column<T>(fac, L, pt, m, ux, uy, uz, ir2, ir, tab + 2 * (size_t)s * H, 1);
+        } else {
+          regular_column<T>(fac, L, pt, m, ux, uy, uz, r2, tab + 2 * (size_t)s * H, 1);
+        }
+      }
+      __syncthreads();
+      // phase B: contraction, one thread per coefficient (n,m)
+      for (int k = threadIdx.x; k < K; k += blockDim.x) {
+        int n = (int)sqrtf((float)k);
+        while (n * n > k) n--;
+        while ((n + 1) * (n + 1) <= k) n++;
+        int m = k - n * n - n;
+        C2<T> a = {T(0), T(0)};
+        // derivative factors of the harmonic with order -m (grad of conj(H_n^m) = grad H_n^{-m})
+        int km = -m;
+        T fa = 0, fb = 0, fcf = 0;
+        if (IRREG) {
+          fa = (T)fac[L.AS + eidx(n, km)]; fb = (T)fac[L.BS + eidx(n, km)]; fcf = (T)fac[L.CS + eidx(n, km)];
+        } else if (n >= 1) {
+          fa = (T)fac[L.AR + eidx(n, km)]; fb = (T)fac[L.BR + eidx(n, km)]; fcf = (T)fac[L.CR + eidx(n, km)];
+        }
+        for (int s = 0; s < tn; s++) {
+          const T* tb = tab + 2 * (size_t)s * H;
+          if (ifcharge) {
+            C2<T> hv = half_get(tb, n, km, 1);  // conj(H_n^m) = H_n^{-m}
+            cfma(a, C2<T>{str[8 * s], str[8 * s + 1]}, hv);
+          }
+          if (ifdipole) {
+            C2<T> g = {T(0), T(0)};
+            int nn = IRREG ? n + 1 : n - 1;
+            if (IRREG || n >= 1) {
+              if (abs(km) <= nn) { C2<T> h = half_get(tb, nn, km, 1); cfma(g, C2<T>{fa * str[8 * s + 2], fa * str[8 * s + 3]}, h); }
+              if (abs(km + 1) <= nn) { C2<T> h = half_get(tb, nn, km + 1, 1); cfma(g, C2<T>{fb * str[8 * s + 4], fb * str[8 * s + 5]}, h); }
+              if (abs(km - 1) <= nn) { C2<T> h = half_get(tb, nn, km - 1, 1); cfma(g, C2<T>{fcf * str[8 * s + 6], fcf * str[8 * s + 7]}, h); }
+            }
+            // d/dy of H(u) with u = (y-c)/h gives a factor 1/h
+            a.re += g.re * ih;
+            a.im += g.im * ih;
+          }
+        }
+        acc[2 * k] += a.re;
+        acc[2 * k + 1] += a.im;
+      }
+    }
+  }
+  __syncthreads();
+  // P2L: L = sum q conj(S(u)) / h ; P2M: M = sum q conj(R(u))
+  const T sc = IRREG ? ih : T(1);
+  T* out = expn + 2 * (size_t)b * K;
+  for (int k = threadIdx.x; k < 2 * K; k += blockDim.x) out[k] += acc[k] * sc;
+}
+
+// ---------------------------------------------------------------------------
+// L2P (src/laprouts3d.f:2772) and M2P (src/laprouts3d.f:2432)
+// One workgroup per leaf box; one thread per evaluation point.  All threads
+// walk the coefficients in the same order (LDS broadcast reads) and build
+// their own harmonics column by column.
+// ---------------------------------------------------------------------------
+template <typename T, bool IRREG>
+__device__ __forceinline__ void eval_expansion(const double* fac, const FacLayout& L, int p,
+                                               const T* E, T ux, T uy, T uz, C2<T>& pot,
+                                               C2<T>& gz, C2<T>& gu, C2<T>& gd, bool grad) {
+  T r2 = ux * ux + uy * uy + uz * uz;
+  T ir2 = IRREG ? T(1) / r2 : T(0);
+  T ir = IRREG ? sqrt(ir2) : T(0);
+  const int pmax = IRREG ? p + 1 : p;  // S needs one more degree for the gradient
+  // Walk columns m >= 0; in each column the harmonic H_n^m for n = m..pmax.
+  C2<T> hmm = {IRREG ? ir : T(1), T(0)};
+  for (int m = 0; m <= pmax; m++) {
+    if (m > 0) {
+      T f = (T)fac[L.MM + m] * (IRREG ? ir2 : T(1));
+      C2<T> w = {-ux * f, -uy * f};
+      hmm = cmul(w, hmm);
+    }
+    C2<T> h0 = hmm, h1 = {T(0), T(0)};
+    for (int n = m; n <= pmax; n++) {
+      C2<T> h;
+      if (n == m) h = hmm;
+      else if (n == m + 1) {
+        T g = (T)fac[L.M1 + m] * uz * (IRREG ? ir2 : T(1));
+        h = {g * hmm.re, g * hmm.im};
+        h1 = h;
+      } else {
+        T a = (T)(2 * n - 1) * uz;
+        T bq = (T)fac[L.RA + hidx(n, m)] * (IRREG ? T(1) : r2);
+        T c = (T)fac[L.RB + hidx(n, m)] * (IRREG ? ir2 : T(1));
+        h = {(a * h1.re - bq * h0.re) * c, (a * h1.im - bq * h0.im) * c};
+        h0 = h1;
+        h1 = h;
+      }
+      if (n == m + 1) h0 = hmm;
+      C2<T> hc = cconj(h);
+      // potential
+      if (n <= p) {
+        cfma(pot, cld(E, eidx(n, m)), h);
+        if (m > 0) cfma(pot, cld(E, eidx(n, -m)), hc);
+      }
+      if (!grad) continue;
+      if (!IRREG) {
+        // R_n^m appears in the derivatives of R_{n+1}^{m}, R_{n+1}^{m-1}, R_{n+1}^{m+1}
+        if (n + 1 <= p) {
+          int n1 = n + 1;
+          cfma(gz, C2<T>{(T)fac[L.AR + eidx(n1, m)] * E[2 * eidx(n1, m)], (T)fac[L.AR + eidx(n1, m)] * E[2 * eidx(n1, m) + 1]}, h);
+          cfma(gu, C2<T>{(T)fac[L.BR + eidx(n1, m - 1)] * E[2 * eidx(n1, m - 1)], (T)fac[L.BR + eidx(n1, m - 1)] * E[2 * eidx(n1, m - 1) + 1]}, h);
+          cfma(gd, C2<T>{(T)fac[L.CR + eidx(n1, m + 1)] * E[2 * eidx(n1, m + 1)], (T)fac[L.CR + eidx(n1, m + 1)] * E[2 * eidx(n1, m + 1) + 1]}, h);
+          if (m > 0) {
+            cfma(gz, C2<T>{(T)fac[L.AR + eidx(n1, -m)] * E[2 * eidx(n1, -m)], (T)fac[L.AR + eidx(n1, -m)] * E[2 * eidx(n1, -m) + 1]}, hc);
+            cfma(gu, C2<T>{(T)fac[L.BR + eidx(n1, -m - 1)] * E[2 * eidx(n1, -m - 1)], (T)fac[L.BR + eidx(n1, -m - 1)] * E[2 * eidx(n1, -m - 1) + 1]}, hc);
+            cfma(gd, C2<T>{(T)fac[L.CR + eidx(n1, 1 - m)] * E[2 * eidx(n1, 1 - m)], (T)fac[L.CR + eidx(n1, 1 - m)] * E[2 * eidx(n1, 1 - m) + 1]}, hc);
+          }
+        }
+      } else {
+        // S_n^m appears in the derivatives of S_{n-1}^{m}, S_{n-1}^{m+1}, S_{n-1}^{m-1}
+        if (n >= 1) {
+          int n1 = n - 1;
+          if (m <= n1) {
+            cfma(gz, C2<T>{(T)fac[L.AS + eidx(n1, m)] * E[2 * eidx(n1, m)], (T)fac[L.AS + eidx(n1, m)] * E[2 * eidx(n1, m) + 1]}, h);
+          }
+          if (m - 1 <= n1 && m - 1 >= -n1) {
+            cfma(gu, C2<T>{(T)fac[L.BS + eidx(n1, m - 1)] * E[2 * eidx(n1, m - 1)], (T)fac[L.BS + eidx(n1, m - 1)] * E[2 * eidx(n1, m - 1) + 1]}, h);
+          }
+          if (m + 1 <= n1) {
+            cfma(gd, C2<T>{(T)fac[L.CS + eidx(n1, m + 1)] * E[2 * eidx(n1, m + 1)], (T)fac[L.CS + eidx(n1, m + 1)] * E[2 * eidx(n1, m + 1) + 1]}, h);
+          }
+          if (m > 0) {
+            if (m <= n1) {
+              cfma(gz, C2<T>{(T)fac[L.AS + eidx(n1, -m)] * E[2 * eidx(n1, -m)], (T)fac[L.AS + eidx(n1, -m)] * E[2 * eidx(n1, -m) + 1]}, hc);
+            }
+            if (-m - 1 >= -n1) {
+              cfma(gu, C2<T>{(T)fac[L.BS + eidx(n1, -m - 1)] * E[2 * eidx(n1, -m - 1)], (T)fac[L.BS + eidx(n1, -m - 1)] * E[2 * eidx(n1, -m - 1) + 1]}, hc);
+            }
+            if (-m + 1 <= n1 && -m + 1 >= -n1) {
+              cfma(gd, C2<T>{(T)fac[L.CS + eidx(n1, -m + 1)] * E[2 * eidx(n1, -m + 1)], (T)fac[L.CS + eidx(n1, -m + 1)] * E[2 * eidx(n1, -m + 1) + 1]}, hc);
+            }
+          }
+        }
+      }
+    }
+  }
+}
+
+// kind 0: sources of the box as evaluation points, kind 1: targets
+template <typename T, bool IRREG>
+__global__ void k_eval(DevPlan P, const double* __restrict__ fac, const T* __restrict__ expn,
+                       const T* __restrict__ rec, const T* __restrict__ tpos, int wpot0, int wfld0,
+                       int wpot1, int wfld1, T* __restrict__ pot0, T* __restrict__ fld0,
+                       T* __restrict__ pot1, T* __restrict__ fld1) {
+  extern __shared__ unsigned char smem_raw[];
+  T* E = reinterpret_cast<T*>(smem_raw);
+  const int p = P.p, K = P.K;
+  FacLayout L(p);
+  const int b = IRREG ? P.m2p_boxes[blockIdx.x] : P.l2p_boxes[blockIdx.x];
+  int jbeg = 0, jend = 1;
+  if (IRREG) { jbeg = P.l3_off[b]; jend = P.l3_off[b + 1]; }
+  for (int jj = jbeg; jj < jend; jj++) {
+    const int j = IRREG ? P.l3_idx[jj] : b;
+    const T ih = (T)P.inv_side[P.level[j]];
+    const double cx = P.center[3 * j], cy = P.center[3 * j + 1], cz = P.center[3 * j + 2];
+    __syncthreads();
+    for (int k = threadIdx.x; k < 2 * K; k += blockDim.x) E[k] = expn[2 * (size_t)j * K + k];
+    __syncthreads();
+    for (int kind = 0; kind < 2; kind++) {
+      const int wp = kind ? wpot1 : wpot0, wf = kind ? wfld1 : wfld0;
+      if (!wp && !wf) continue;
+      const int n0 = kind ? P.itrg[b] : P.isrc[b];
+      const int nn = kind ? P.ntrg[b] : P.nsrc[b];
+      for (int i0 = 0; i0 < nn; i0 += blockDim.x) {
+        const int i = i0 + threadIdx.x;
+        const bool active = i < nn;
+        double x = 0, y = 0, z = 0;
+        if (active) {
+          const T* q = kind ? tpos + 3 * (size_t)(n0 + i) : rec + 12 * (size_t)(n0 + i);
+          x = q[0]; y = q[1]; z = q[2];
+        }
+        T ux = active ? (T)((x - cx) * (double)ih) : T(1);
+        T uy = active ? (T)((y - cy) * (double)ih) : T(1);
+        T uz = active ? (T)((z - cz) * (double)ih) : T(1);
+        C2<T> pv = {0, 0}, gz = {0, 0}, gu = {0, 0}, gd = {0, 0};
+        eval_expansion<T, IRREG>(fac, L, p, E, ux, uy, uz, pv, gz, gu, gd, wf != 0);
+        if (!active) continue;
+        // M2P: pot = sum M S(u)/h ; L2P: pot = sum L R(u); gradient w.r.t. x adds 1/h
+        T sp = IRREG ? ih : T(1), sg = IRREG ? ih * ih : ih;
+        T* po = kind ? pot1 : pot0;
+        T* fo = kind ? fld1 : fld0;
+        size_t id = (size_t)(n0 + i);
+        if (wp) { po[2 * id] += pv.re * sp; po[2 * id + 1] += pv.im * sp; }
+        if (wf) {
+          // fld = -grad: d/dx = (U+D)/2, d/dy = (U-D)/(2i), d/dz
+          C2<T> dx = {T(0.5) * (gu.re + gd.re), T(0.5) * (gu.im + gd.im)};
+          C2<T> dy = {T(0.5) * (gu.im - gd.im), -T(0.5) * (gu.re - gd.re)};
+          fo[6 * id + 0] -= dx.re * sg; fo[6 * id + 1] -= dx.im * sg;
+          fo[6 * id + 2] -= dy.re * sg; fo[6 * id + 3] -= dy.im * sg;
+          fo[6 * id + 4] -= gz.re * sg; fo[6 * id + 5] -= gz.im * sg;
+        }
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Point-and-shoot translation of one expansion inside a workgroup:
+//   A = in * e^{i m phi}           (n <= pin)
+//   B = rotate(A)                   (rotation class `rc`)
+//   A = zshift(B)  with table Z    (n <= pin -> l <= pout)
+//   B = rotate back(A)
+//   out += B * e^{-i m phi} * scale
+// src/l3dtrans.f:223 (M2M), :868 (M2L), :1676 (L2L) do the same sequence.
+// ZMODE 0: Z[m][out][in] ; the same dense (p+1)^3 layout for every shift.
+// ---------------------------------------------------------------------------
+template <typename T>
+__device__ void rotate_block(const DevTables& Tb, int rc, int pmax, const T* in, T* out, bool back) {
+  const int p = Tb.p;
+  const int Kt = (pmax + 1) * (pmax + 1);
+  for (int k = threadIdx.x; k < Kt; k += blockDim.x) {
+    int n = (int)sqrtf((float)k);
+    while (n * n > k) n--;
+    while ((n + 1) * (n + 1) <= k) n++;
+    int mp = k - n * n - n;
+    const double* D = Tb.rot + Tb.rot_off[rc * (p + 1) + n];
+    const int dim = 2 * n + 1;
+    T are = 0, aim = 0;
+    for (int m = -n; m <= n; m++) {
+      T d = (T)D[(m + n) * dim + (mp + n)];
+      T xr = in[2 * eidx(n, m)], xi = in[2 * eidx(n, m) + 1];
+      if (back && (m & 1)) { xr = -xr; xi = -xi; }
+      are += d * xr;
+      aim += d * xi;
+    }
+    if (back && (mp & 1)) { are = -are; aim = -aim; }
+    out[2 * k] = are;
+    out[2 * k + 1] = aim;
+  }
+}
+
+template <typename T>
+__device__ void zshift_block(const double* Z, int p, int pin, int pout, const T* in, T* out) {
+  const int Kt = (pout + 1) * (pout + 1);
+  const int P1 = p + 1;
+  for (int k = threadIdx.x; k < Kt; k += blockDim.x) {
+    int l = (int)sqrtf((float)k);
+    while (l * l > k) l--;
+    while ((l + 1) * (l + 1) <= k) l++;
+    int m = k - l * l - l, am = m < 0 ? -m : m;
+    const double* z = Z + ((size_t)am * P1 + l) * P1;
+    T are = 0, aim = 0;
+    for (int n = am; n <= pin; n++) {
+      T c = (T)z[n];
+      are += c * in[2 * eidx(n, m)];
+      aim += c * in[2 * eidx(n, m) + 1];
+    }
+    out[2 * k] = are;
+    out[2 * k + 1] = aim;
+  }
+}
+
+template <typename T>
+__device__ void point_and_shoot(const DevTables& Tb, int rc, const double* phase, const double* Z,
+                                int pin, int pout, const T* in, T* out, T scale, T* A, T* B) {
+  const int pm = pin > pout ? pin : pout;
+  const int Ki = (pin + 1) * (pin + 1);
+  for (int k = threadIdx.x; k < (pm + 1) * (pm + 1); k += blockDim.x) {
+    if (k < Ki) {
+      int n = (int)sqrtf((float)k);
+      while (n * n > k) n--;
+      while ((n + 1) * (n + 1) <= k) n++;
+      int m = k - n * n - n, am = m < 0 ? -m : m;
+      T er = (T)phase[2 * am], ei = (T)phase[2 * am + 1];
+      if (m < 0) ei = -ei;
+      T xr = in[2 * k], xi = in[2 * k + 1];
+      A[2 * k] = xr * er - xi * ei;
+      A[2 * k + 1] = xr * ei + xi * er;
+    } else {
+      A[2 * k] = 0;
+      A[2 * k + 1] = 0;
+    }
+  }
+  __syncthreads();
+  rotate_block<T>(Tb, rc, pin, A, B, false);
+  __syncthreads();
+  zshift_block<T>(Z, Tb.p, pin, pout, B, A);
+  __syncthreads();
+  rotate_block<T>(Tb, rc, pout, A, B, true);
+  __syncthreads();
+  const int Ko = (pout + 1) * (pout + 1);
+  for (int k = threadIdx.x; k < Ko; k += blockDim.x) {
+    int n = (int)sqrtf((float)k);
+    while (n * n > k) n--;
+    while ((n + 1) * (n + 1) <= k) n++;
+    int m = k - n * n - n, am = m < 0 ? -m : m;
+    T er = (T)phase[2 * am], ei = -(T)phase[2 * am + 1];  // e^{-i m phi}
+    if (m < 0) ei = -ei;
+    T xr = B[2 * k], xi = B[2 * k + 1];
+    out[2 * k] += (xr * er - xi * ei) * scale;
+    out[2 * k + 1] += (xr * ei + xi * er) * scale;
+  }
+  __syncthreads();
+}
+
+// M2M for the parents of one level (src/lfmm3drouts.f:195-258)
+template <typename T>
+__global__ void k_m2m(DevPlan P, DevTables Tb, int first, T* __restrict__ mp) {
+  extern __shared__ unsigned char smem_raw[];
+  T* A = reinterpret_cast<T*>(smem_raw);
+  T* B = A + 2 * P.K;
+  const int b = P.m2m_boxes[first + blockIdx.x];
+  const int p = P.p;
+  for (int c = 0; c < P.nchild[b]; c++) {
+    const int ch = P.child[8 * b + c];
+    if (P.nsrc[ch] == 0) continue;
+    // octant of the child relative to the parent
+    int o = (P.ijk[3 * ch] & 1 ? 0 : 1) | (P.ijk[3 * ch + 1] & 1 ? 0 : 2) | (P.ijk[3 * ch + 2] & 1 ? 0 : 4);
+    const double* ph = Tb.oct_phase + (size_t)(o * 2 + 0) * 2 * (p + 1);
+    point_and_shoot<T>(Tb, Tb.oct_rot[o * 2 + 0], ph, Tb.z_m2m, p, p, mp + 2 * (size_t)ch * P.K,
+                       mp + 2 * (size_t)b * P.K, T(1), A, B);
+  }
+}
+
+// L2L for the parents of one level (src/lfmm3drouts.f:498-558)
+template <typename T>
+__global__ void k_l2l(DevPlan P, DevTables Tb, int first, T* __restrict__ lo) {
+  extern __shared__ unsigned char smem_raw[];
+  T* A = reinterpret_cast<T*>(smem_raw);
+  T* B = A + 2 * P.K;
+  const int b = P.l2l_boxes[first + blockIdx.x];
+  const int p = P.p;
+  for (int c = 0; c < P.nchild[b]; c++) {
+    const int ch = P.child[8 * b + c];
+    int o = (P.ijk[3 * ch] & 1 ? 0 : 1) | (P.ijk[3 * ch + 1] & 1 ? 0 : 2) | (P.ijk[3 * ch + 2] & 1 ? 0 : 4);
+    const double* ph = Tb.oct_phase + (size_t)(o * 2 + 1) * 2 * (p + 1);
+    point_and_shoot<T>(Tb, Tb.oct_rot[o * 2 + 1], ph, Tb.z_l2l, p, p, lo + 2 * (size_t)b * P.K,
+                       lo + 2 * (size_t)ch * P.K, T(1), A, B);
+  }
+}
+
+// M2L over the (pruned) list 2 of each target box (src/lfmm3drouts.f:347-483)
+template <typename T>
+__global__ void k_m2l(DevPlan P, DevTables Tb, const T* __restrict__ mp, T* __restrict__ lo) {
+  extern __shared__ unsigned char smem_raw[];
+  T* A = reinterpret_cast<T*>(smem_raw);
+  T* B = A + 2 * P.K;
+  const int b = P.m2l_boxes[blockIdx.x];
+  const int p = P.p;
+  const T ih = (T)P.inv_side[P.level[b]];
+  const size_t zsz = (size_t)(p + 1) * (p + 1) * (p + 1);
+  for (int jj = P.l2_off[b]; jj < P.l2_off[b + 1]; jj++) {
+    const int j = P.l2_idx[jj];
+    int di = P.ijk[3 * j] - P.ijk[3 * b] + 3, dj = P.ijk[3 * j + 1] - P.ijk[3 * b + 1] + 3,
+        dk = P.ijk[3 * j + 2] - P.ijk[3 * b + 2] + 3;
+    int code = (di * 7 + dj) * 7 + dk;
+    int pt = Tb.off_ptrunc[code];
+    point_and_shoot<T>(Tb, Tb.off_rot[code], Tb.off_phase + (size_t)code * 2 * (p + 1),
+                       Tb.z_m2l + (size_t)Tb.off_dist[code] * zsz, pt, pt,
+                       mp + 2 * (size_t)j * P.K, lo + 2 * (size_t)b * P.K, ih, A, B);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// P2P near field (src/lfmm3dpart.f:1076-1141, lfmm3dpart_direct_self/_targ).
+// One workgroup per (leaf, chunk of evaluation points); one point per thread.
+// The source loop is wave-uniform, so source records stream through the
+// scalar cache and every lane reuses them from SGPRs.
+// ---------------------------------------------------------------------------
+template <typename T, bool CHG, bool DIP, bool FLD>
+__global__ __launch_bounds__(256) void k_p2p(DevPlan P, const T* __restrict__ rec,
+                                             const T* __restrict__ tpos, int kind,
+                                             T* __restrict__ pot, T* __restrict__ fld) {
+  const int w = blockIdx.x;
+  const int b = P.p2p_items[3 * w], start = P.p2p_items[3 * w + 2];
+  const int n0 = kind ? P.itrg[b] : P.isrc[b];
+  const int nn = kind ? P.ntrg[b] : P.nsrc[b];
+  const int i = start + threadIdx.x;
+  const bool active = i < nn;
+  const int me = n0 + (active ? i : 0);
+  T x, y, z;
+  if (kind) { x = tpos[3 * (size_t)me]; y = tpos[3 * (size_t)me + 1]; z = tpos[3 * (size_t)me + 2]; }
+  else { x = rec[12 * (size_t)me]; y = rec[12 * (size_t)me + 1]; z = rec[12 * (size_t)me + 2]; }
+  T pr = 0, pi = 0, fxr = 0, fxi = 0, fyr = 0, fyi = 0, fzr = 0, fzi = 0;
+  const int jb = P.l1_off[b], je = P.l1_off[b + 1];
+  for (int jj = jb - 1; jj < je; jj++) {
+    const int j = jj < jb ? b : P.l1_idx[jj];
+    const int s0 = P.isrc[j], s1 = s0 + P.nsrc[j];
+    const bool self = (kind == 0) && (j == b);
+    for (int s = s0; s < s1; s++) {
+      const T* r = rec + 12 * (size_t)s;
+      T dx = x - r[0], dy = y - r[1], dz = z - r[2];
+      T r2 = dx * dx + dy * dy + dz * dz;
+      T ri = rsqrt(r2);
+      if (self && s == me) ri = 0;
+      T ri2 = ri * ri, ri3 = ri * ri2;
+      if (CHG) {
+        pr += r[3] * ri;
+        pi += r[4] * ri;
+        if (FLD) {
+          T cr = r[3] * ri3, ci = r[4] * ri3;
+          fxr += cr * dx; fxi += ci * dx;
+          fyr += cr * dy; fyi += ci * dy;
+          fzr += cr * dz; fzi += ci * dz;
+        }
+      }
+      if (DIP) {
+        T dotr = r[5] * dx + r[7] * dy + r[9] * dz;
+        T doti = r[6] * dx + r[8] * dy + r[10] * dz;
+        pr += dotr * ri3;
+        pi += doti * ri3;
+        if (FLD) {
+          T t = T(3) * ri3 * ri2;
+          T tr = t * dotr, ti = t * doti;
+          fxr += tr * dx - ri3 * r[5]; fxi += ti * dx - ri3 * r[6];
+          fyr += tr * dy - ri3 * r[7]; fyi += ti * dy - ri3 * r[8];
+          fzr += tr * dz - ri3 * r[9]; fzi += ti * dz - ri3 * r[10];
+        }
+      }
+    }
+  }
+  if (!active) return;
+  pot[2 * (size_t)me] += pr;
+  pot[2 * (size_t)me + 1] += pi;
+  if (FLD) {
+    fld[6 * (size_t)me + 0] += fxr; fld[6 * (size_t)me + 1] += fxi;
+    fld[6 * (size_t)me + 2] += fyr; fld[6 * (size_t)me + 3] += fyi;
+    fld[6 * (size_t)me + 4] += fzr; fld[6 * (size_t)me + 5] += fzi;
+  }
+}
+
+// Plain direct sum over all sources (l3dpartdirect, src/lfmm3dpart.f:1391):
+// the same pair kernel with one "box" holding everything.
+template <typename T, bool CHG, bool DIP>
+__global__ __launch_bounds__(256) void k_direct(int ns, const T* __restrict__ rec, int npts,
+                                                const T* __restrict__ pts, int pts_are_sources,
+                                                T* __restrict__ pot, T* __restrict__ fld) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const bool active = i < npts;
+  const int me = active ? i : 0;
+  T x, y, z;
+  if (pts_are_sources) { x = rec[12 * (size_t)me]; y = rec[12 * (size_t)me + 1]; z = rec[12 * (size_t)me + 2]; }
+  else { x = pts[3 * (size_t)me]; y = pts[3 * (size_t)me + 1]; z = pts[3 * (size_t)me + 2]; }
+  T pr = 0, pi = 0, fxr = 0, fxi = 0, fyr = 0, fyi = 0, fzr = 0, fzi = 0;
+  for (int s = 0; s < ns; s++) {
+    const T* r = rec + 12 * (size_t)s;
+    T dx = x - r[0], dy = y - r[1], dz = z - r[2];
+    T r2 = dx * dx + dy * dy + dz * dz;
+    T ri = rsqrt(r2);
+    if (pts_are_sources && s == me) ri = 0;
+    T ri2 = ri * ri, ri3 = ri * ri2;
+    if (CHG) {
+      pr += r[3] * ri; pi += r[4] * ri;
+      T cr = r[3] * ri3, ci = r[4] * ri3;
+      fxr += cr * dx; fxi += ci * dx; fyr += cr * dy; fyi += ci * dy; fzr += cr * dz; fzi += ci * dz;
+    }
+    if (DIP) {
+      T dotr = r[5] * dx + r[7] * dy + r[9] * dz, doti = r[6] * dx + r[8] * dy + r[10] * dz;
+      pr += dotr * ri3; pi += doti * ri3;
+      T t = T(3) * ri3 * ri2, tr = t * dotr, ti = t * doti;
+      fxr += tr * dx - ri3 * r[5]; fxi += ti * dx - ri3 * r[6];
+      fyr += tr * dy - ri3 * r[7]; fyi += ti * dy - ri3 * r[8];
+      fzr += tr * dz - ri3 * r[9]; fzi += ti * dz - ri3 * r[10];
+    }
+  }
+  if (!active) return;
+  pot[2 * (size_t)i] = pr; pot[2 * (size_t)i + 1] = pi;
+  fld[6 * (size_t)i + 0] = fxr; fld[6 * (size_t)i + 1] = fxi;
+  fld[6 * (size_t)i + 2] = fyr; fld[6 * (size_t)i + 3] = fyi;
+  fld[6 * (size_t)i + 4] = fzr; fld[6 * (size_t)i + 5] = fzi;
+}
+
+}  // namespace fmm3d

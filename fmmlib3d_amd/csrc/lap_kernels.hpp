@@ -1,0 +1,55 @@
+// Device-side data layout shared by the Laplace kernels and the host plan.
+#pragma once
+#include <hip/hip_runtime.h>
+
+namespace fmm3d {
+
+// Index of expansion coefficient (n,m), -n<=m<=n, in a packed expansion.
+__host__ __device__ inline int eidx(int n, int m) { return n * n + n + m; }
+// Index of (n,m), 0<=m<=n, in a half table.
+__host__ __device__ inline int hidx(int n, int m) { return n * (n + 1) / 2 + m; }
+
+// Tree + schedule as flat device arrays (all ints live in one buffer, all
+// doubles in another; the pointers below point into them).
+struct DevPlan {
+  int nboxes, nlev, p, K;  // K = (p+1)^2
+  int ns, nt;
+  // boxes
+  const int *level, *isrc, *nsrc, *itrg, *ntrg, *nchild, *child, *ijk;
+  const double *center;   // 3 per box
+  const double *inv_side; // 1/side per level
+  // lists (CSR over boxes)
+  const int *l1_off, *l1_idx, *l2_off, *l2_idx, *l3_off, *l3_idx, *l4_off, *l4_idx;
+  // work lists
+  const int *p2m_boxes;  int n_p2m;
+  const int *l2p_boxes;  int n_l2p;
+  const int *m2p_boxes;  int n_m2p;   // leaves with list 3
+  const int *p2l_boxes;  int n_p2l;   // boxes with list 4
+  const int *m2l_boxes;  int n_m2l;   // targets with (pruned) list 2
+  const int *m2m_boxes;  const int *m2m_lev_off;   // parents grouped by level
+  const int *l2l_boxes;  const int *l2l_lev_off;
+  const int *p2p_items;  int n_p2p;   // (leaf, kind, start) triples
+  // permutations
+  const int *src_perm, *trg_perm;
+};
+
+// Translation tables for one expansion order p (level independent because
+// expansions carry rscale = 1/side).
+struct DevTables {
+  int p;
+  // offset code (i+3)*49+(j+3)*7+(k+3) for source-minus-target in [-3,3]^3
+  const int *off_rot;     // rotation class per offset code
+  const int *off_dist;    // distance class per offset code
+  const int *off_ptrunc;  // list-2 truncation order per offset code
+  const double *off_phase;  // e^{i m phi}, m=0..p, per offset code (re,im)
+  const double *rot;        // rotation blocks per class
+  const long *rot_off;      // start of class c, degree n block: rot_off[c*(p+1)+n]
+  const double *z_m2l;      // per distance class: [m][l][n] dense (p+1)^3
+  const double *z_m2m;      // [m][n][l] dense (p+1)^3 : parent n from child l
+  const double *z_l2l;      // [m][l][n] dense (p+1)^3 : child l from parent n
+  const double *oct_phase;  // 8 octants x 2 (m2m, l2l) x (p+1) complex
+  const int *oct_rot;       // 8 octants x 2 -> rotation class (m2m, l2l)
+  const double *fac;        // derivative factors, see lap_kernels.hip
+};
+
+}  // namespace fmm3d

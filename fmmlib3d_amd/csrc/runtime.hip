@@ -1,0 +1,85 @@
+// Device runtime: arena, timers, default context.
+#include "runtime.hpp"
+
+namespace fmm3d {
+
+Arena::~Arena() { release_all(); }
+
+void* Arena::get(const std::string& name, size_t bytes, int ier_on_fail) {
+  if (bytes == 0) bytes = 256;
+  bytes = (bytes + 255) & ~(size_t)255;
+  Buf& b = bufs_[name];
+  if (b.n >= bytes) return b.p;
+  if (b.p) {
+    (void)hipFree(b.p);
+    b.p = nullptr;
+    b.n = 0;
+  }
+  // grow with headroom so that slightly larger calls reuse the buffer
+  size_t want = bytes + bytes / 8;
+  if (hipMalloc(&b.p, want) != hipSuccess) {
+    (void)hipGetLastError();
+    b.p = nullptr;
+    if (hipMalloc(&b.p, bytes) != hipSuccess) {
+      (void)hipGetLastError();
+      b.p = nullptr;
+      throw AllocError("device allocation failed for " + name, ier_on_fail);
+    }
+    want = bytes;
+  }
+  b.n = want;
+  return b.p;
+}
+
+void Arena::release_all() {
+  for (auto& kv : bufs_)
+    if (kv.second.p) (void)hipFree(kv.second.p);
+  bufs_.clear();
+}
+
+void Timer::start(hipStream_t stream, int n) {
+  s = stream;
+  if ((int)ev.size() < n) {
+    for (int i = (int)ev.size(); i < n; i++) {
+      hipEvent_t e;
+      FMM_HIP(hipEventCreate(&e));
+      ev.push_back(e);
+    }
+  }
+  FMM_HIP(hipEventRecord(ev[0], s));
+}
+
+void Timer::mark(int i) { FMM_HIP(hipEventRecord(ev[i], s)); }
+
+std::vector<double> Timer::elapsed() {
+  std::vector<double> out;
+  for (size_t i = 1; i < ev.size(); i++) {
+    float ms = 0;
+    if (hipEventElapsedTime(&ms, ev[i - 1], ev[i]) != hipSuccess) {
+      (void)hipGetLastError();
+      ms = 0;
+    }
+    out.push_back(ms);
+  }
+  return out;
+}
+
+Timer::~Timer() {
+  for (auto e : ev) (void)hipEventDestroy(e);
+}
+
+Context& default_context() {
+  static Context* ctx = [] {
+    auto* c = new Context();
+    (void)hipGetDevice(&c->device);
+    return c;
+  }();
+  return *ctx;
+}
+
+std::vector<double>& last_timings() {
+  static thread_local std::vector<double> t(11, 0.0);
+  return t;
+}
+
+}  // namespace fmm3d

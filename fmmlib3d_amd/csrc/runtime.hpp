@@ -1,0 +1,68 @@
+// Device runtime shared by the FMM drivers: a per-device context with its
+// stream, a grow-only buffer arena, event timers and HIP error mapping.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <map>
+#include <mutex>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+namespace fmm3d {
+
+struct HipError : std::runtime_error {
+  explicit HipError(const std::string& s) : std::runtime_error(s) {}
+};
+struct AllocError : std::runtime_error {
+  int ier;
+  AllocError(const std::string& s, int code) : std::runtime_error(s), ier(code) {}
+};
+
+#define FMM_HIP(call)                                                              \
+  do {                                                                             \
+    hipError_t e_ = (call);                                                        \
+    if (e_ != hipSuccess)                                                          \
+      throw ::fmm3d::HipError(std::string(#call) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+// Grow-only named device buffers; released when the context dies.
+class Arena {
+ public:
+  ~Arena();
+  void* get(const std::string& name, size_t bytes, int ier_on_fail = 8);
+  void release_all();
+
+ private:
+  struct Buf { void* p = nullptr; size_t n = 0; };
+  std::map<std::string, Buf> bufs_;
+};
+
+struct Timer {
+  std::vector<hipEvent_t> ev;
+  hipStream_t s = nullptr;
+  void start(hipStream_t stream, int n);
+  void mark(int i);
+  std::vector<double> elapsed();  // ms between consecutive marks
+  ~Timer();
+};
+
+struct Context {
+  int device = 0;
+  hipStream_t stream = nullptr;  // null = legacy default stream
+  Arena arena;
+  std::mutex mu;
+  // cached host-built translation tables (by order p)
+  int tab_p = -1;
+  std::vector<int> tab_host_i;
+  std::vector<double> tab_host_d;
+  std::vector<long> tab_host_l;
+  size_t tab_off[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+};
+
+Context& default_context();
+
+// last per-stage timings of this thread
+std::vector<double>& last_timings();
+
+}  // namespace fmm3d

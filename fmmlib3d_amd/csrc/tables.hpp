@@ -1,0 +1,25 @@
+// Host-built translation tables (see tables.cpp) and expansion orders.
+#pragma once
+#include <vector>
+
+namespace fmm3d {
+
+struct HostTables {
+  int p = -1, nrot = 0, ndist = 0;
+  std::vector<double> fac;                         // FacLayout(p)
+  std::vector<int> off_rot, off_dist, off_ptrunc;  // 343 offset codes
+  std::vector<double> off_phase;                   // 343 x (p+1) complex
+  std::vector<int> oct_rot;                        // 8 octants x {M2M, L2L}
+  std::vector<double> oct_phase;                   // 16 x (p+1) complex
+  std::vector<double> rot;                         // rotation blocks
+  std::vector<long> rot_off;                       // nrot x (p+1)
+  std::vector<double> z_m2l, z_m2m, z_l2l;         // dense [m][out][in]
+};
+
+// l3dterms (src/l3dterms.f:34): expansion order for tolerance eps
+int laplace_nterms(double eps);
+// l3dterms_list2 (src/l3dterms.f:136): 7^3 table indexed by offset code
+std::vector<int> laplace_itable(double eps);
+void build_tables(int p, double eps, HostTables& H);
+
+}  // namespace fmm3d

@@ -1,0 +1,90 @@
+/*
+ * fmm3d_mi355x.h -- C ABI of libfmm3d_mi355x.so, the MI355X (gfx950) particle
+ * FMM that replaces fmmlib3d's lfmm3dpart* / hfmm3dpart* / l3dpartdirect /
+ * h3dpartdirect.
+ *
+ * Calling conventions mirror the reference routines one for one:
+ *  - arrays use the reference's Fortran layouts: source(3,n) and target(3,n)
+ *    are n consecutive (x,y,z) doubles, charge/dipstr/pot are complex*16
+ *    (interleaved re,im doubles), dipvec(3,n) doubles, fld(3,n) complex*16
+ *    (x,y,z components of a point consecutive);
+ *  - flags are 0/1 exactly as in the reference; outputs are written only when
+ *    their flag is 1 and are fully overwritten (not accumulated);
+ *  - arrays whose flag is 0 may be NULL;
+ *  - the return value is the reference's `ier`: 0 ok, 4 tree failure (also
+ *    returned for nsource < 1, as the reference does through d3tstrcr's
+ *    ier=128 and ten retries), 8 bulk workspace, 16 expansion workspace.
+ *    Extensions: 32 invalid iprec (the reference leaves it undefined),
+ *    64 HIP runtime error, 128 invalid argument (negative sizes, zk == 0).
+ * All pointers are host pointers unless the name ends in _dev.
+ * The functions are thread safe; calls on one device are serialised.
+ */
+#ifndef FMM3D_MI355X_H
+#define FMM3D_MI355X_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Precision modes for the *_ex entry points. */
+#define FMM3D_PREC_FP64 0  /* every stage in FP64 (the default)                 */
+#define FMM3D_PREC_MIXED 1 /* FP32 near field + FP32 translations where safe    */
+
+/* Replaces lfmm3dparttarg, src/lfmm3dpart.f:138-141. */
+int fmm3d_lfmm3dparttarg(int iprec, int nsource, const double *source,
+                         int ifcharge, const double *charge,
+                         int ifdipole, const double *dipstr, const double *dipvec,
+                         int ifpot, double *pot, int iffld, double *fld,
+                         int ntarget, const double *target,
+                         int ifpottarg, double *pottarg, int iffldtarg, double *fldtarg);
+
+/* Replaces lfmm3dpartself, src/lfmm3dpart.f:87-89 (and lfmm3dpart, :36-38). */
+int fmm3d_lfmm3dpartself(int iprec, int nsource, const double *source,
+                         int ifcharge, const double *charge,
+                         int ifdipole, const double *dipstr, const double *dipvec,
+                         int ifpot, double *pot, int iffld, double *fld);
+
+/* Replaces l3dpartdirect, src/lfmm3dpart.f:1391-1394 (O(N^2) direct sum). */
+void fmm3d_l3dpartdirect(int nsource, const double *source,
+                         int ifcharge, const double *charge,
+                         int ifdipole, const double *dipstr, const double *dipvec,
+                         int ifpot, double *pot, int iffld, double *fld,
+                         int ntarget, const double *target,
+                         int ifpottarg, double *pottarg, int iffldtarg, double *fldtarg);
+
+/* Same as fmm3d_lfmm3dparttarg with an explicit precision mode (FMM3D_PREC_*).
+ * Extension with no reference counterpart: the reference has FP64 only. */
+int fmm3d_lfmm3dparttarg_ex(int prec_mode, int iprec, int nsource, const double *source,
+                            int ifcharge, const double *charge,
+                            int ifdipole, const double *dipstr, const double *dipvec,
+                            int ifpot, double *pot, int iffld, double *fld,
+                            int ntarget, const double *target,
+                            int ifpottarg, double *pottarg, int iffldtarg, double *fldtarg);
+
+/* Device-resident variant of lfmm3dparttarg: every array is a device pointer
+ * in the same layout; `stream` is a hipStream_t (NULL = default stream).
+ * The call returns when the results are complete on the device.  The octree
+ * is still built on the host (the coordinates are copied back for it).
+ * Extension with no reference counterpart, for callers whose data already
+ * lives in HBM (benchmarks, the serving loop). */
+int fmm3d_lfmm3dparttarg_dev(int prec_mode, int iprec, int nsource, const double *source_dev,
+                             int ifcharge, const double *charge_dev,
+                             int ifdipole, const double *dipstr_dev, const double *dipvec_dev,
+                             int ifpot, double *pot_dev, int iffld, double *fld_dev,
+                             int ntarget, const double *target_dev,
+                             int ifpottarg, double *pottarg_dev, int iffldtarg, double *fldtarg_dev,
+                             void *stream);
+
+/* Per-stage timings (milliseconds, device time from HIP events) of the last
+ * FMM call on this thread: [0] tree+plan (host), [1] upload, [2] P2M,
+ * [3] P2L, [4] M2M, [5] M2L, [6] L2L, [7] L2P, [8] M2P, [9] P2P, [10] total.
+ * Mirrors the reference's timeinfo(1..8) (src/lfmm3dpart.f:1155). */
+void fmm3d_last_timings(double *ms, int n);
+
+/* Library version string. */
+const char *fmm3d_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* FMM3D_MI355X_H */
