@@ -21,14 +21,16 @@ import numpy as np
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "lib", "libfmm3d_mi355x.so")
 
-PREC_FP64 = 0
-PREC_MIXED = 1
+PREC_FP64 = 0     # every stage in FP64
+PREC_MIXED = 1    # FP32 near-field pair arithmetic, FP64 far field
+PREC_FAR32 = 2    # FP32 expansions/translations, FP64 near field
+PREC_FP32 = 3     # FP32 near and far field
 
 # every symbol include/fmm3d_mi355x.h declares
 EXPORTED = (
     "fmm3d_lfmm3dparttarg", "fmm3d_lfmm3dpartself", "fmm3d_l3dpartdirect",
     "fmm3d_lfmm3dparttarg_ex", "fmm3d_lfmm3dparttarg_dev", "fmm3d_last_timings",
-    "fmm3d_version",
+    "fmm3d_version", "fmm3d_d3tstrcr",
 )
 
 _lib = None
@@ -61,6 +63,8 @@ def load():
         lib.fmm3d_lfmm3dpartself.restype = i
         lib.fmm3d_l3dpartdirect.argtypes = [i, vp, i, vp, i, vp, vp, i, vp, i, vp, i, vp, i, vp, i, vp]
         lib.fmm3d_l3dpartdirect.restype = None
+        lib.fmm3d_d3tstrcr.argtypes = [i, vp, i, vp, i, vp, vp, vp, vp, vp, i, vp, vp, vp, vp, i, vp]
+        lib.fmm3d_d3tstrcr.restype = i
         lib.fmm3d_last_timings.argtypes = [vp, i]
         lib.fmm3d_last_timings.restype = None
         _lib = lib
@@ -145,6 +149,45 @@ def l3dpartdirect(source, charge=None, dipstr=None, dipvec=None, target=None,
                             _ptr(fld), nt, _ptr(trg), int(pt is not None), _ptr(pt),
                             int(ft is not None), _ptr(ft))
     return pot, fld, pt, ft
+
+
+def d3tstrcr(source, target, nbox):
+    """The host octree (d3tstrcr, src/d3tstrcr.f:24): returns
+    (ier, boxes20, isource, itarget, lists, nlev, center, size) with lists[k][b]
+    a tuple of 1-based box ids (k = 0..3 for lists 1..4)."""
+    lib = load()
+    src = np.ascontiguousarray(source, np.float64)
+    trg = None if target is None else np.ascontiguousarray(target, np.float64)
+    ns = src.shape[0]
+    nt = 0 if trg is None else trg.shape[0]
+    nb = ctypes.c_int(0)
+    nlev = ctypes.c_int(0)
+    nlist = ctypes.c_int(0)
+    center = np.zeros(3)
+    size = ctypes.c_double(0)
+    ier = lib.fmm3d_d3tstrcr(ns, _ptr(src), nt, _ptr(trg), int(nbox), ctypes.byref(nb),
+                             ctypes.byref(nlev), _ptr(center), ctypes.byref(size),
+                             ctypes.byref(nlist), 0, None, None, None, None, 0, None)
+    if ier:
+        return ier, None, None, None, None, 0, center, 0.0
+    M = nb.value
+    boxes = np.zeros((M, 20), np.int32)
+    isrc = np.zeros(ns, np.int32)
+    itrg = np.zeros(max(nt, 1), np.int32)
+    counts = np.zeros(4 * M, np.int32)
+    lists = np.zeros(max(nlist.value, 1), np.int32)
+    ier = lib.fmm3d_d3tstrcr(ns, _ptr(src), nt, _ptr(trg), int(nbox), ctypes.byref(nb),
+                             ctypes.byref(nlev), _ptr(center), ctypes.byref(size),
+                             ctypes.byref(nlist), M, _ptr(boxes), _ptr(isrc), _ptr(itrg),
+                             _ptr(counts), lists.size, _ptr(lists))
+    out = [[() for _ in range(M)] for _ in range(4)]
+    o = 0
+    for k in range(4):
+        for b in range(M):
+            c = counts[k * M + b]
+            out[k][b] = tuple(lists[o:o + c].tolist())
+            o += c
+    return ier, boxes, isrc, itrg[:nt], out, nlev.value, center, size.value
 
 
 def last_timings():

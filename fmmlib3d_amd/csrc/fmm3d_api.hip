@@ -9,6 +9,7 @@
 
 #include "../../include/fmm3d_mi355x.h"
 #include "fmm3d_internal.hpp"
+#include "octree.hpp"
 
 using namespace fmm3d;
 
@@ -47,7 +48,7 @@ int lap_host(int prec_mode, int iprec, int ns, const double* source, int ifcharg
   if (nt < 0) return 128;
   if (ns < 1) return 4;  // lfmm3dpart.f:309-330 after d3tstrcr's ier=128
   if (iprec < -2 || iprec > 6) return 32;
-  if (prec_mode != FMM3D_PREC_FP64 && prec_mode != FMM3D_PREC_MIXED) return 128;
+  if (prec_mode < 0 || prec_mode > 3) return 128;
   Context& ctx = default_context();
   std::lock_guard<std::mutex> lock(ctx.mu);
   return guarded([&] {
@@ -67,7 +68,7 @@ int lap_host(int prec_mode, int iprec, int ns, const double* source, int ifcharg
     if (c.iffld) c.fld_dev = (double*)ctx.arena.get("out_fld", 6 * sizeof(double) * ns);
     if (c.ifpottarg) c.pottarg_dev = (double*)ctx.arena.get("out_pt", 2 * sizeof(double) * nt);
     if (c.iffldtarg) c.fldtarg_dev = (double*)ctx.arena.get("out_ft", 6 * sizeof(double) * nt);
-    int ier = prec_mode == FMM3D_PREC_MIXED ? lap_solve<float>(ctx, c) : lap_solve<double>(ctx, c);
+    int ier = lap_solve_mode(ctx, c, prec_mode);
     if (ier) return ier;
     if (c.ifpot) download(ctx, pot, c.pot_dev, 2 * (size_t)ns);
     if (c.iffld) download(ctx, fld, c.fld_dev, 6 * (size_t)ns);
@@ -117,7 +118,7 @@ int fmm3d_lfmm3dparttarg_dev(int prec_mode, int iprec, int ns, const double* sou
   if (nt < 0) return 128;
   if (ns < 1) return 4;
   if (iprec < -2 || iprec > 6) return 32;
-  if (prec_mode != FMM3D_PREC_FP64 && prec_mode != FMM3D_PREC_MIXED) return 128;
+  if (prec_mode < 0 || prec_mode > 3) return 128;
   Context& ctx = default_context();
   std::lock_guard<std::mutex> lock(ctx.mu);
   hipStream_t saved = ctx.stream;
@@ -138,7 +139,7 @@ int fmm3d_lfmm3dparttarg_dev(int prec_mode, int iprec, int ns, const double* sou
     c.src_dev = source_dev; c.q_dev = charge_dev; c.ds_dev = dipstr_dev; c.dv_dev = dipvec_dev;
     c.trg_dev = target_dev;
     c.pot_dev = pot_dev; c.fld_dev = fld_dev; c.pottarg_dev = pottarg_dev; c.fldtarg_dev = fldtarg_dev;
-    return prec_mode == FMM3D_PREC_MIXED ? lap_solve<float>(ctx, c) : lap_solve<double>(ctx, c);
+    return lap_solve_mode(ctx, c, prec_mode);
   });
   ctx.stream = saved;
   return ier;
@@ -185,3 +186,40 @@ void fmm3d_last_timings(double* ms, int n) {
 const char* fmm3d_version(void) { return "fmm3d-mi355x 0.1 (gfx950)"; }
 
 }  // extern "C"
+
+extern "C" int fmm3d_d3tstrcr(int nsource, const double* source, int ntarget, const double* target,
+                              int nbox, int* nboxes, int* nlev, double* center, double* size,
+                              int* nlist, int maxboxes, int* boxes20, int* isource, int* itarget,
+                              int* list_counts, int maxlist, int* lists) {
+  if (ntarget < 0) return 128;
+  Octree t;
+  int ier = build_octree(source, nsource, target, ntarget, nbox, t);
+  if (ier) return ier;
+  *nboxes = t.nboxes;
+  *nlev = t.nlev;
+  for (int d = 0; d < 3; d++) center[d] = t.center[d];
+  *size = t.size;
+  long tot = 0;
+  for (int k = 0; k < 4; k++) tot += (long)t.list[k].idx.size();
+  *nlist = (int)tot;
+  if (maxboxes == 0) return 0;
+  if (maxboxes < t.nboxes || maxlist < tot) return 32;
+  for (int b = 0; b < t.nboxes; b++) {
+    int* o = boxes20 + 20 * (size_t)b;
+    std::memset(o, 0, sizeof(int) * 20);
+    o[0] = t.level[b];
+    o[1] = t.ijk[3 * b]; o[2] = t.ijk[3 * b + 1]; o[3] = t.ijk[3 * b + 2];
+    o[4] = t.parent[b] + 1;
+    for (int c = 0; c < t.nchild[b]; c++) o[5 + c] = t.child[8 * b + c] + 1;
+    o[13] = t.isrc[b] + 1; o[14] = t.nsrc[b]; o[15] = t.itrg[b] + 1; o[16] = t.ntrg[b];
+  }
+  for (int i = 0; i < nsource; i++) isource[i] = t.src_perm[i] + 1;
+  for (int i = 0; i < ntarget; i++) itarget[i] = t.trg_perm[i] + 1;
+  long o = 0;
+  for (int k = 0; k < 4; k++)
+    for (int b = 0; b < t.nboxes; b++) {
+      list_counts[k * (size_t)maxboxes + b] = t.list[k].count(b);
+      for (int q = t.list[k].off[b]; q < t.list[k].off[b + 1]; q++) lists[o++] = t.list[k].idx[q] + 1;
+    }
+  return 0;
+}

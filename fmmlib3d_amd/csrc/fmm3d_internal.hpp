@@ -16,8 +16,12 @@ struct LapCall {
   double *pot_dev = nullptr, *fld_dev = nullptr, *pottarg_dev = nullptr, *fldtarg_dev = nullptr;
 };
 
-template <typename T>
+// T: far-field arithmetic and expansion storage; TN: near-field arithmetic
+template <typename T, typename TN>
 int lap_solve(Context& ctx, const LapCall& c);
+
+// precision mode bitmask (FMM3D_PREC_* in include/fmm3d_mi355x.h)
+int lap_solve_mode(Context& ctx, const LapCall& c, int mode);
 int lap_direct(Context& ctx, const LapCall& c);
 
 }  // namespace fmm3d

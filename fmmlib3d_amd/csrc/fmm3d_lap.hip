@@ -7,6 +7,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstring>
+#include <type_traits>
 #include <vector>
 
 #include "fmm3d_internal.hpp"
@@ -227,18 +228,19 @@ size_t max_lds_bytes(Context& ctx) {
 
 inline unsigned blocks(size_t n, unsigned b) { return (unsigned)((n + b - 1) / b); }
 
-template <typename T>
-void launch_p2p(const DevPlan& P, const int* items, int n, int kind, const T* rec, const T* tpos,
-                int ifcharge, int ifdipole, bool fld, T* pot, T* fo, hipStream_t s) {
+template <typename TN>
+void launch_p2p(const DevPlan& P, const int* items, int n, int kind, const double* spos,
+                const TN* str, const double* tpos, int ifcharge, int ifdipole, bool fld, double* pot,
+                double* fo, hipStream_t s) {
   if (n == 0) return;
   DevPlan Q = P;
   Q.p2p_items = items;
   Q.n_p2p = n;
   dim3 g(n), b(kP2PBlock);
-#define P2P_CASE(C, Dp, F)                                                                     \
-  if (ifcharge == C && ifdipole == Dp && fld == F) {                                          \
-    hipLaunchKernelGGL((k_p2p<T, C, Dp, F>), g, b, 0, s, Q, rec, tpos, kind, pot, fo);         \
-    return;                                                                                    \
+#define P2P_CASE(C, Dp, F)                                                                        \
+  if (ifcharge == C && ifdipole == Dp && fld == F) {                                             \
+    hipLaunchKernelGGL((k_p2p<TN, C, Dp, F>), g, b, 0, s, Q, spos, str, tpos, kind, pot, fo);    \
+    return;                                                                                       \
   }
   P2P_CASE(1, 0, false) P2P_CASE(1, 0, true) P2P_CASE(0, 1, false) P2P_CASE(0, 1, true)
   P2P_CASE(1, 1, false) P2P_CASE(1, 1, true)
@@ -247,7 +249,7 @@ void launch_p2p(const DevPlan& P, const int* items, int n, int kind, const T* re
 
 }  // namespace
 
-template <typename T>
+template <typename T, typename TN>
 int lap_solve(Context& ctx, const LapCall& c) {
   auto t0 = std::chrono::steady_clock::now();
   double eps;
@@ -276,24 +278,27 @@ int lap_solve(Context& ctx, const LapCall& c) {
   DevTables Tb = ensure_tables(ctx, p, eps);
   DevPlan P = make_devplan(tree, p, O, dI, dD);
   const size_t nb = tree.nboxes;
-  T* rec = (T*)ctx.arena.get("rec", sizeof(T) * 12 * (size_t)ns);
-  T* tpos = (T*)ctx.arena.get("tpos", sizeof(T) * 3 * (size_t)std::max(nt, 1));
+  constexpr bool near32 = sizeof(TN) < sizeof(double);
+  double* spos = (double*)ctx.arena.get("spos", sizeof(double) * 4 * (size_t)ns);
+  double* str64 = (double*)ctx.arena.get("str64", sizeof(double) * 8 * (size_t)ns);
+  float* str32 = near32 ? (float*)ctx.arena.get("str32", sizeof(float) * 8 * (size_t)ns) : nullptr;
+  double* tpos = (double*)ctx.arena.get("tpos", sizeof(double) * 4 * (size_t)std::max(nt, 1));
   T* mp = (T*)ctx.arena.get("mp", sizeof(T) * 2 * K * nb, 16);
   T* lo = (T*)ctx.arena.get("lo", sizeof(T) * 2 * K * nb, 16);
-  T* pot_s = (T*)ctx.arena.get("pot_s", sizeof(T) * 2 * (size_t)ns);
-  T* fld_s = (T*)ctx.arena.get("fld_s", sizeof(T) * 6 * (size_t)ns);
-  T* pot_t = (T*)ctx.arena.get("pot_t", sizeof(T) * 2 * (size_t)std::max(nt, 1));
-  T* fld_t = (T*)ctx.arena.get("fld_t", sizeof(T) * 6 * (size_t)std::max(nt, 1));
+  double* pot_s = (double*)ctx.arena.get("pot_s", sizeof(double) * 2 * (size_t)ns);
+  double* fld_s = (double*)ctx.arena.get("fld_s", sizeof(double) * 6 * (size_t)ns);
+  double* pot_t = (double*)ctx.arena.get("pot_t", sizeof(double) * 2 * (size_t)std::max(nt, 1));
+  double* fld_t = (double*)ctx.arena.get("fld_t", sizeof(double) * 6 * (size_t)std::max(nt, 1));
+  const TN* strn = near32 ? (const TN*)(const void*)str32 : (const TN*)(const void*)str64;
   const int* sperm = dI + O.sperm;
   const int* tperm = dI + O.tperm;
-  hipLaunchKernelGGL((k_gather_sources<T>), dim3(blocks(ns, 256)), dim3(256), 0, s, ns, sperm,
-                     c.src_dev, c.q_dev, c.ds_dev, c.dv_dev, f.ifcharge, f.ifdipole, rec);
+  hipLaunchKernelGGL(k_gather_sources, dim3(blocks(ns, 256)), dim3(256), 0, s, ns, sperm, c.src_dev,
+                     c.q_dev, c.ds_dev, c.dv_dev, f.ifcharge, f.ifdipole, spos, str64, str32);
   if (nt > 0)
-    hipLaunchKernelGGL((k_gather_targets<T>), dim3(blocks(nt, 256)), dim3(256), 0, s, nt, tperm,
-                       c.trg_dev, tpos);
-  auto zero = [&](T* a, size_t n) {
-    hipLaunchKernelGGL((k_fill_zero<T>), dim3(std::min<size_t>(blocks(n, 256), 4096)), dim3(256),
-                       0, s, n, a);
+    hipLaunchKernelGGL(k_gather_targets, dim3(blocks(nt, 256)), dim3(256), 0, s, nt, tperm, c.trg_dev, tpos);
+  auto zero = [&](auto* a, size_t n) {
+    using E = std::remove_pointer_t<decltype(a)>;
+    hipLaunchKernelGGL((k_fill_zero<E>), dim3(std::min<size_t>(blocks(n, 256), 4096)), dim3(256), 0, s, n, a);
   };
   zero(mp, 2 * (size_t)K * nb);
   zero(lo, 2 * (size_t)K * nb);
@@ -312,7 +317,7 @@ int lap_solve(Context& ctx, const LapCall& c) {
     ts = std::max(1, std::min(32, ts));
     size_t lds = sizeof(T) * (2 * K + (size_t)ts * (2 * H + 11));
     if (P.n_p2m)
-      hipLaunchKernelGGL((k_form<T, false>), dim3(P.n_p2m), dim3(256), lds, s, P, fac, rec,
+      hipLaunchKernelGGL((k_form<T, false>), dim3(P.n_p2m), dim3(256), lds, s, P, fac, spos, str64,
                          f.ifcharge, f.ifdipole, ts, mp);
   }
   tm.mark(2);
@@ -323,7 +328,7 @@ int lap_solve(Context& ctx, const LapCall& c) {
     ts = std::max(1, std::min(32, ts));
     size_t lds = sizeof(T) * (2 * K + (size_t)ts * (2 * H + 11));
     if (P.n_p2l)
-      hipLaunchKernelGGL((k_form<T, true>), dim3(P.n_p2l), dim3(256), lds, s, P, fac, rec,
+      hipLaunchKernelGGL((k_form<T, true>), dim3(P.n_p2l), dim3(256), lds, s, P, fac, spos, str64,
                          f.ifcharge, f.ifdipole, ts, lo);
   }
   tm.mark(3);
@@ -342,27 +347,27 @@ int lap_solve(Context& ctx, const LapCall& c) {
   // ---- STEP 7: L2P
   const size_t lds_e = sizeof(T) * 2 * K;
   if (P.n_l2p)
-    hipLaunchKernelGGL((k_eval<T, false>), dim3(P.n_l2p), dim3(256), lds_e, s, P, fac, lo, rec, tpos,
+    hipLaunchKernelGGL((k_eval<T, false>), dim3(P.n_l2p), dim3(256), lds_e, s, P, fac, lo, spos, tpos,
                        f.ifpot, f.iffld, f.ifpottarg, f.iffldtarg, pot_s, fld_s, pot_t, fld_t);
   tm.mark(7);
   // ---- STEP 6: M2P (list 3)
   if (P.n_m2p)
-    hipLaunchKernelGGL((k_eval<T, true>), dim3(P.n_m2p), dim3(256), lds_e, s, P, fac, mp, rec, tpos,
+    hipLaunchKernelGGL((k_eval<T, true>), dim3(P.n_m2p), dim3(256), lds_e, s, P, fac, mp, spos, tpos,
                        f.ifpot, f.iffld, f.ifpottarg, f.iffldtarg, pot_s, fld_s, pot_t, fld_t);
   tm.mark(8);
   // ---- STEP 8: P2P (self + list 1)
   if (f.ifcharge || f.ifdipole) {
-    launch_p2p<T>(P, dI + O.p2p0, O.n_p2p0, 0, rec, tpos, f.ifcharge, f.ifdipole, f.iffld != 0,
-                  pot_s, fld_s, s);
-    launch_p2p<T>(P, dI + O.p2p1, O.n_p2p1, 1, rec, tpos, f.ifcharge, f.ifdipole, f.iffldtarg != 0,
-                  pot_t, fld_t, s);
+    launch_p2p<TN>(P, dI + O.p2p0, O.n_p2p0, 0, spos, strn, tpos, f.ifcharge, f.ifdipole,
+                   f.iffld != 0, pot_s, fld_s, s);
+    launch_p2p<TN>(P, dI + O.p2p1, O.n_p2p1, 1, spos, strn, tpos, f.ifcharge, f.ifdipole,
+                   f.iffldtarg != 0, pot_t, fld_t, s);
   }
   tm.mark(9);
   // ---- scatter back to the caller's order (l3dpsort / l3dfsort)
-  if (f.ifpot) hipLaunchKernelGGL((k_scatter<T>), dim3(blocks(ns, 256)), dim3(256), 0, s, ns, 1, sperm, pot_s, c.pot_dev);
-  if (f.iffld) hipLaunchKernelGGL((k_scatter<T>), dim3(blocks(ns, 256)), dim3(256), 0, s, ns, 3, sperm, fld_s, c.fld_dev);
-  if (f.ifpottarg) hipLaunchKernelGGL((k_scatter<T>), dim3(blocks(nt, 256)), dim3(256), 0, s, nt, 1, tperm, pot_t, c.pottarg_dev);
-  if (f.iffldtarg) hipLaunchKernelGGL((k_scatter<T>), dim3(blocks(nt, 256)), dim3(256), 0, s, nt, 3, tperm, fld_t, c.fldtarg_dev);
+  if (f.ifpot) hipLaunchKernelGGL(k_scatter, dim3(blocks(ns, 256)), dim3(256), 0, s, ns, 1, sperm, pot_s, c.pot_dev);
+  if (f.iffld) hipLaunchKernelGGL(k_scatter, dim3(blocks(ns, 256)), dim3(256), 0, s, ns, 3, sperm, fld_s, c.fld_dev);
+  if (f.ifpottarg) hipLaunchKernelGGL(k_scatter, dim3(blocks(nt, 256)), dim3(256), 0, s, nt, 1, tperm, pot_t, c.pottarg_dev);
+  if (f.iffldtarg) hipLaunchKernelGGL(k_scatter, dim3(blocks(nt, 256)), dim3(256), 0, s, nt, 3, tperm, fld_t, c.fldtarg_dev);
   tm.mark(10);
   FMM_HIP(hipGetLastError());
   FMM_HIP(hipStreamSynchronize(s));
@@ -377,56 +382,55 @@ int lap_solve(Context& ctx, const LapCall& c) {
   return 0;
 }
 
-template int lap_solve<double>(Context&, const LapCall&);
-template int lap_solve<float>(Context&, const LapCall&);
+template int lap_solve<double, double>(Context&, const LapCall&);
+template int lap_solve<double, float>(Context&, const LapCall&);
+template int lap_solve<float, double>(Context&, const LapCall&);
+template int lap_solve<float, float>(Context&, const LapCall&);
 
 // Direct sum on the device (l3dpartdirect): identity order, one pass.
 int lap_direct(Context& ctx, const LapCall& c) {
   const int ns = c.ns, nt = c.nt;
   hipStream_t s = ctx.stream;
-  double* rec = (double*)ctx.arena.get("rec", sizeof(double) * 12 * (size_t)std::max(ns, 1));
-  std::vector<int> id(std::max(ns, nt) + 1);
+  const size_t nmax = (size_t)std::max(std::max(ns, nt), 1);
+  std::vector<int> id(nmax);
   for (size_t i = 0; i < id.size(); i++) id[i] = (int)i;
   int* dperm = (int*)ctx.arena.get("plan_i", id.size() * sizeof(int));
   FMM_HIP(hipMemcpyAsync(dperm, id.data(), id.size() * sizeof(int), hipMemcpyHostToDevice, s));
-  double* tpos = (double*)ctx.arena.get("tpos", sizeof(double) * 3 * (size_t)std::max(nt, 1));
-  double* pot = (double*)ctx.arena.get("pot_s", sizeof(double) * 2 * (size_t)std::max(ns, nt));
-  double* fld = (double*)ctx.arena.get("fld_s", sizeof(double) * 6 * (size_t)std::max(ns, nt));
+  double* spos = (double*)ctx.arena.get("spos", sizeof(double) * 4 * (size_t)std::max(ns, 1));
+  double* str64 = (double*)ctx.arena.get("str64", sizeof(double) * 8 * (size_t)std::max(ns, 1));
+  double* tpos = (double*)ctx.arena.get("tpos", sizeof(double) * 4 * (size_t)std::max(nt, 1));
+  double* pot = (double*)ctx.arena.get("pot_s", sizeof(double) * 2 * nmax);
+  double* fld = (double*)ctx.arena.get("fld_s", sizeof(double) * 6 * nmax);
   if (ns > 0)
-    hipLaunchKernelGGL((k_gather_sources<double>), dim3(blocks(ns, 256)), dim3(256), 0, s, ns, dperm,
-                       c.src_dev, c.q_dev, c.ds_dev, c.dv_dev, c.ifcharge ? 1 : 0, c.ifdipole ? 1 : 0, rec);
+    hipLaunchKernelGGL(k_gather_sources, dim3(blocks(ns, 256)), dim3(256), 0, s, ns, dperm, c.src_dev,
+                       c.q_dev, c.ds_dev, c.dv_dev, c.ifcharge ? 1 : 0, c.ifdipole ? 1 : 0, spos, str64,
+                       (float*)nullptr);
   auto run = [&](int npts, const double* pts, int are_src) {
     if (npts <= 0) return;
-#define DCASE(C, D)                                                                                 \
-  if ((c.ifcharge ? 1 : 0) == C && (c.ifdipole ? 1 : 0) == D)                                       \
-    hipLaunchKernelGGL((k_direct<double, C, D>), dim3(blocks(npts, 256)), dim3(256), 0, s, ns, rec, \
-                       npts, pts, are_src, pot, fld);
+    if (ns == 0 || (!c.ifcharge && !c.ifdipole)) {
+      hipLaunchKernelGGL((k_fill_zero<double>), dim3(blocks(2 * (size_t)npts, 256)), dim3(256), 0, s, 2 * (size_t)npts, pot);
+      hipLaunchKernelGGL((k_fill_zero<double>), dim3(blocks(6 * (size_t)npts, 256)), dim3(256), 0, s, 6 * (size_t)npts, fld);
+      return;
+    }
+#define DCASE(C, D)                                                                                   \
+  if ((c.ifcharge ? 1 : 0) == C && (c.ifdipole ? 1 : 0) == D)                                         \
+    hipLaunchKernelGGL((k_direct<C, D>), dim3(blocks(npts, 256)), dim3(256), 0, s, ns, spos, str64, npts, \
+                       pts, are_src, pot, fld);
     DCASE(1, 0) DCASE(0, 1) DCASE(1, 1)
 #undef DCASE
   };
   auto copy_out = [&](int npts, int wpot, double* po, int wfld, double* fo) {
-    // direct results are already in caller order (identity permutation)
+    // the direct sum runs in the caller's order (identity permutation)
     if (wpot) FMM_HIP(hipMemcpyAsync(po, pot, sizeof(double) * 2 * npts, hipMemcpyDeviceToDevice, s));
     if (wfld) FMM_HIP(hipMemcpyAsync(fo, fld, sizeof(double) * 6 * npts, hipMemcpyDeviceToDevice, s));
   };
-  if (c.ifpot || c.iffld) {
-    if (ns > 0 && !c.ifcharge && !c.ifdipole) {
-      hipLaunchKernelGGL((k_fill_zero<double>), dim3(blocks(2 * (size_t)ns, 256)), dim3(256), 0, s, 2 * (size_t)ns, pot);
-      hipLaunchKernelGGL((k_fill_zero<double>), dim3(blocks(6 * (size_t)ns, 256)), dim3(256), 0, s, 6 * (size_t)ns, fld);
-    } else {
-      run(ns, nullptr, 1);
-    }
+  if (ns > 0 && (c.ifpot || c.iffld)) {
+    run(ns, nullptr, 1);
     copy_out(ns, c.ifpot, c.pot_dev, c.iffld, c.fld_dev);
   }
   if (nt > 0 && (c.ifpottarg || c.iffldtarg)) {
-    hipLaunchKernelGGL((k_gather_targets<double>), dim3(blocks(nt, 256)), dim3(256), 0, s, nt, dperm,
-                       c.trg_dev, tpos);
-    if (!c.ifcharge && !c.ifdipole) {
-      hipLaunchKernelGGL((k_fill_zero<double>), dim3(blocks(2 * (size_t)nt, 256)), dim3(256), 0, s, 2 * (size_t)nt, pot);
-      hipLaunchKernelGGL((k_fill_zero<double>), dim3(blocks(6 * (size_t)nt, 256)), dim3(256), 0, s, 6 * (size_t)nt, fld);
-    } else {
-      run(nt, tpos, 0);
-    }
+    hipLaunchKernelGGL(k_gather_targets, dim3(blocks(nt, 256)), dim3(256), 0, s, nt, dperm, c.trg_dev, tpos);
+    run(nt, tpos, 0);
     copy_out(nt, c.ifpottarg, c.pottarg_dev, c.iffldtarg, c.fldtarg_dev);
   }
   FMM_HIP(hipGetLastError());
@@ -434,4 +438,16 @@ int lap_direct(Context& ctx, const LapCall& c) {
   return 0;
 }
 
+}  // namespace fmm3d
+
+namespace fmm3d {
+int lap_solve_mode(Context& ctx, const LapCall& c, int mode) {
+  switch (mode) {
+    case 0: return lap_solve<double, double>(ctx, c);
+    case 1: return lap_solve<double, float>(ctx, c);
+    case 2: return lap_solve<float, double>(ctx, c);
+    case 3: return lap_solve<float, float>(ctx, c);
+  }
+  return 128;
+}
 }  // namespace fmm3d

@@ -118,53 +118,51 @@ __device__ __forceinline__ C2<T> half_get(const T* tab, int n, int m, int st) {
 // gather / scatter / fill
 // ---------------------------------------------------------------------------
 
-// Sorted source records, 12 values per source:
-//   x y z | q.re q.im | vx.re vx.im vy.re vy.im vz.re vz.im | 0
-// where v = dipstr * dipvec (complex dipole moment vector).
-template <typename T>
+// Sorted sources: positions in FP64 (x, y, z, 0) for every precision, and
+// strengths as 8 values (q.re q.im vx.re vx.im vy.re vy.im vz.re vz.im)
+// where v = dipstr * dipvec is the complex dipole moment vector; an FP32 copy
+// of the strengths is written when the near field runs in FP32.
 __global__ void k_gather_sources(int ns, const int* __restrict__ perm, const double* __restrict__ src,
                                  const double* __restrict__ q, const double* __restrict__ ds,
                                  const double* __restrict__ dv, int ifcharge, int ifdipole,
-                                 T* __restrict__ rec) {
+                                 double* __restrict__ spos, double* __restrict__ str64,
+                                 float* __restrict__ str32) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= ns) return;
   int o = perm[i];
-  T* r = rec + 12 * (size_t)i;
-  r[0] = (T)src[3 * o]; r[1] = (T)src[3 * o + 1]; r[2] = (T)src[3 * o + 2];
-  r[3] = ifcharge ? (T)q[2 * o] : T(0);
-  r[4] = ifcharge ? (T)q[2 * o + 1] : T(0);
-  if (ifdipole) {
-    double dr = ds[2 * o], di = ds[2 * o + 1];
-    for (int k = 0; k < 3; k++) {
-      double v = dv[3 * o + k];
-      r[5 + 2 * k] = (T)(dr * v);
-      r[6 + 2 * k] = (T)(di * v);
-    }
-  } else {
-    for (int k = 5; k < 11; k++) r[k] = T(0);
+  double* x = spos + 4 * (size_t)i;
+  x[0] = src[3 * o]; x[1] = src[3 * o + 1]; x[2] = src[3 * o + 2]; x[3] = 0.0;
+  double v[8];
+  v[0] = ifcharge ? q[2 * o] : 0.0;
+  v[1] = ifcharge ? q[2 * o + 1] : 0.0;
+  for (int k = 0; k < 3; k++) {
+    double d = ifdipole ? dv[3 * o + k] : 0.0;
+    v[2 + 2 * k] = ifdipole ? ds[2 * o] * d : 0.0;
+    v[3 + 2 * k] = ifdipole ? ds[2 * o + 1] * d : 0.0;
   }
-  r[11] = T(0);
+  for (int k = 0; k < 8; k++) str64[8 * (size_t)i + k] = v[k];
+  if (str32)
+    for (int k = 0; k < 8; k++) str32[8 * (size_t)i + k] = (float)v[k];
 }
 
-template <typename T>
 __global__ void k_gather_targets(int nt, const int* __restrict__ perm, const double* __restrict__ trg,
-                                 T* __restrict__ tp) {
+                                 double* __restrict__ tpos) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= nt) return;
   int o = perm[i];
-  tp[3 * (size_t)i] = (T)trg[3 * o];
-  tp[3 * (size_t)i + 1] = (T)trg[3 * o + 1];
-  tp[3 * (size_t)i + 2] = (T)trg[3 * o + 2];
+  tpos[4 * (size_t)i] = trg[3 * o];
+  tpos[4 * (size_t)i + 1] = trg[3 * o + 1];
+  tpos[4 * (size_t)i + 2] = trg[3 * o + 2];
+  tpos[4 * (size_t)i + 3] = 0.0;
 }
 
 // out[perm[i]] = sorted[i]; `w` complex values per point (1 = pot, 3 = fld)
-template <typename T>
-__global__ void k_scatter(int n, int w, const int* __restrict__ perm, const T* __restrict__ sorted,
+__global__ void k_scatter(int n, int w, const int* __restrict__ perm, const double* __restrict__ sorted,
                           double* __restrict__ out) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   int o = perm[i];
-  for (int k = 0; k < 2 * w; k++) out[(size_t)2 * w * o + k] = (double)sorted[(size_t)2 * w * i + k];
+  for (int k = 0; k < 2 * w; k++) out[(size_t)2 * w * o + k] = sorted[(size_t)2 * w * i + k];
 }
 
 template <typename T>
@@ -183,8 +181,9 @@ __global__ void k_fill_zero(size_t n, T* __restrict__ a) {
 // taken from the boxes of list 4) instead of P2M.
 // ---------------------------------------------------------------------------
 template <typename T, bool IRREG>
-__global__ void k_form(DevPlan P, const double* __restrict__ fac, const T* __restrict__ rec,
-                       int ifcharge, int ifdipole, int ts, T* __restrict__ expn) {
+__global__ void k_form(DevPlan P, const double* __restrict__ fac, const double* __restrict__ spos,
+                       const double* __restrict__ str, int ifcharge, int ifdipole, int ts,
+                       T* __restrict__ expn) {
   extern __shared__ unsigned char smem_raw[];
   T* smem = reinterpret_cast<T*>(smem_raw);
   const int p = P.p, K = P.K;
@@ -193,11 +192,12 @@ __global__ void k_form(DevPlan P, const double* __restrict__ fac, const T* __res
   FacLayout L(p);
   T* acc = smem;                    // 2*K
   T* tab = acc + 2 * K;             // ts * H complex
-  T* str = tab + 2 * (size_t)ts * H; // ts * 8 : q, dz, du, dd (complex)
-  T* pos = str + 8 * ts;            // ts * 3 scaled offsets
+  T* sst = tab + 2 * (size_t)ts * H; // ts * 8 : q, dz, du, dd (complex)
+  T* pos = sst + 8 * ts;            // ts * 3 scaled offsets
 
   const int b = IRREG ? P.p2l_boxes[blockIdx.x] : P.p2m_boxes[blockIdx.x];
-  const T ih = (T)P.inv_side[P.level[b]];
+  const double ihd = P.inv_side[P.level[b]];
+  const T ih = (T)ihd;
   const double cx = P.center[3 * b], cy = P.center[3 * b + 1], cz = P.center[3 * b + 2];
   for (int k = threadIdx.x; k < 2 * K; k += blockDim.x) acc[k] = T(0);
 
@@ -211,15 +211,16 @@ __global__ void k_form(DevPlan P, const double* __restrict__ fac, const T* __res
       const int tn = min(ts, sn - t0);
       __syncthreads();
       for (int s = threadIdx.x; s < tn; s += blockDim.x) {
-        const T* r = rec + 12 * (size_t)(s0 + t0 + s);
-        pos[3 * s] = (T)(((double)r[0] - cx) * (double)ih);
-        pos[3 * s + 1] = (T)(((double)r[1] - cy) * (double)ih);
-        pos[3 * s + 2] = (T)(((double)r[2] - cz) * (double)ih);
+        const double* x = spos + 4 * (size_t)(s0 + t0 + s);
+        const double* r = str + 8 * (size_t)(s0 + t0 + s);
+        pos[3 * s] = (T)((x[0] - cx) * ihd);
+        pos[3 * s + 1] = (T)((x[1] - cy) * ihd);
+        pos[3 * s + 2] = (T)((x[2] - cz) * ihd);
         // charge, and dipole components dz = v_z, du = (v_x - i v_y)/2, dd = (v_x + i v_y)/2
-        str[8 * s] = r[3]; str[8 * s + 1] = r[4];
-        str[8 * s + 2] = r[9]; str[8 * s + 3] = r[10];
-        str[8 * s + 4] = T(0.5) * (r[5] + r[8]); str[8 * s + 5] = T(0.5) * (r[6] - r[7]);
-        str[8 * s + 6] = T(0.5) * (r[5] - r[8]); str[8 * s + 7] = T(0.5) * (r[6] + r[7]);
+        sst[8 * s] = (T)r[0]; sst[8 * s + 1] = (T)r[1];
+        sst[8 * s + 2] = (T)r[6]; sst[8 * s + 3] = (T)r[7];
+        sst[8 * s + 4] = (T)(0.5 * (r[2] + r[5])); sst[8 * s + 5] = (T)(0.5 * (r[3] - r[4]));
+        sst[8 * s + 6] = (T)(0.5 * (r[2] - r[5])); sst[8 * s + 7] = (T)(0.5 * (r[3] + r[4]));
       }
       __syncthreads();
       // phase A: harmonic columns
@@ -254,15 +255,15 @@ __global__ void k_form(DevPlan P, const double* __restrict__ fac, const T* __res
           const T* tb = tab + 2 * (size_t)s * H;
           if (ifcharge) {
             C2<T> hv = half_get(tb, n, km, 1);  // conj(H_n^m) = H_n^{-m}
-            cfma(a, C2<T>{str[8 * s], str[8 * s + 1]}, hv);
+            cfma(a, C2<T>{sst[8 * s], sst[8 * s + 1]}, hv);
           }
           if (ifdipole) {
             C2<T> g = {T(0), T(0)};
             int nn = IRREG ? n + 1 : n - 1;
             if (IRREG || n >= 1) {
-              if (abs(km) <= nn) { C2<T> h = half_get(tb, nn, km, 1); cfma(g, C2<T>{fa * str[8 * s + 2], fa * str[8 * s + 3]}, h); }
-              if (abs(km + 1) <= nn) { C2<T> h = half_get(tb, nn, km + 1, 1); cfma(g, C2<T>{fb * str[8 * s + 4], fb * str[8 * s + 5]}, h); }
-              if (abs(km - 1) <= nn) { C2<T> h = half_get(tb, nn, km - 1, 1); cfma(g, C2<T>{fcf * str[8 * s + 6], fcf * str[8 * s + 7]}, h); }
+              if (abs(km) <= nn) { C2<T> h = half_get(tb, nn, km, 1); cfma(g, C2<T>{fa * sst[8 * s + 2], fa * sst[8 * s + 3]}, h); }
+              if (abs(km + 1) <= nn) { C2<T> h = half_get(tb, nn, km + 1, 1); cfma(g, C2<T>{fb * sst[8 * s + 4], fb * sst[8 * s + 5]}, h); }
+              if (abs(km - 1) <= nn) { C2<T> h = half_get(tb, nn, km - 1, 1); cfma(g, C2<T>{fcf * sst[8 * s + 6], fcf * sst[8 * s + 7]}, h); }
             }
             // d/dy of H(u) with u = (y-c)/h gives a factor 1/h
             a.re += g.re * ih;
@@ -373,9 +374,10 @@ __device__ __forceinline__ void eval_expansion(const double* fac, const FacLayou
 // kind 0: sources of the box as evaluation points, kind 1: targets
 template <typename T, bool IRREG>
 __global__ void k_eval(DevPlan P, const double* __restrict__ fac, const T* __restrict__ expn,
-                       const T* __restrict__ rec, const T* __restrict__ tpos, int wpot0, int wfld0,
-                       int wpot1, int wfld1, T* __restrict__ pot0, T* __restrict__ fld0,
-                       T* __restrict__ pot1, T* __restrict__ fld1) {
+                       const double* __restrict__ spos, const double* __restrict__ tpos, int wpot0,
+                       int wfld0, int wpot1, int wfld1, double* __restrict__ pot0,
+                       double* __restrict__ fld0, double* __restrict__ pot1,
+                       double* __restrict__ fld1) {
   extern __shared__ unsigned char smem_raw[];
   T* E = reinterpret_cast<T*>(smem_raw);
   const int p = P.p, K = P.K;
@@ -400,7 +402,7 @@ __global__ void k_eval(DevPlan P, const double* __restrict__ fac, const T* __res
         const bool active = i < nn;
         double x = 0, y = 0, z = 0;
         if (active) {
-          const T* q = kind ? tpos + 3 * (size_t)(n0 + i) : rec + 12 * (size_t)(n0 + i);
+          const double* q = (kind ? tpos : spos) + 4 * (size_t)(n0 + i);
           x = q[0]; y = q[1]; z = q[2];
         }
         T ux = active ? (T)((x - cx) * (double)ih) : T(1);
@@ -411,8 +413,8 @@ __global__ void k_eval(DevPlan P, const double* __restrict__ fac, const T* __res
         if (!active) continue;
         // M2P: pot = sum M S(u)/h ; L2P: pot = sum L R(u); gradient w.r.t. x adds 1/h
         T sp = IRREG ? ih : T(1), sg = IRREG ? ih * ih : ih;
-        T* po = kind ? pot1 : pot0;
-        T* fo = kind ? fld1 : fld0;
+        double* po = kind ? pot1 : pot0;
+        double* fo = kind ? fld1 : fld0;
         size_t id = (size_t)(n0 + i);
         if (wp) { po[2 * id] += pv.re * sp; po[2 * id + 1] += pv.im * sp; }
         if (wf) {
@@ -588,13 +590,72 @@ __global__ void k_m2l(DevPlan P, DevTables Tb, const T* __restrict__ mp, T* __re
 // ---------------------------------------------------------------------------
 // P2P near field (src/lfmm3dpart.f:1076-1141, lfmm3dpart_direct_self/_targ).
 // One workgroup per (leaf, chunk of evaluation points); one point per thread.
-// The source loop is wave-uniform, so source records stream through the
-// scalar cache and every lane reuses them from SGPRs.
+// The source loop is wave-uniform, so positions and strengths stream through
+// the scalar cache and every lane reuses them from SGPRs.  Offsets and r^2
+// are formed in FP64 for every precision; with TN = float the rest of the
+// pair is FP32, summed in FP32 over runs of at most kFold sources and folded
+// into FP64 accumulators.
 // ---------------------------------------------------------------------------
-template <typename T, bool CHG, bool DIP, bool FLD>
-__global__ __launch_bounds__(256) void k_p2p(DevPlan P, const T* __restrict__ rec,
-                                             const T* __restrict__ tpos, int kind,
-                                             T* __restrict__ pot, T* __restrict__ fld) {
+constexpr int kFold = 32;
+
+template <typename TN>
+__device__ __forceinline__ TN rsqrt_t(TN x);
+template <>
+__device__ __forceinline__ double rsqrt_t<double>(double x) { return rsqrt(x); }
+template <>
+__device__ __forceinline__ float rsqrt_t<float>(float x) { return rsqrtf(x); }
+
+template <typename TN, bool CHG, bool DIP, bool FLD>
+struct PairAcc {
+  TN pr = 0, pi = 0, fxr = 0, fxi = 0, fyr = 0, fyi = 0, fzr = 0, fzi = 0;
+  __device__ __forceinline__ void add(double xt, double yt, double zt, const double* __restrict__ sp,
+                                      const TN* __restrict__ st, bool mask) {
+    const double dxd = xt - sp[0], dyd = yt - sp[1], dzd = zt - sp[2];
+    const double r2d = fma(dxd, dxd, fma(dyd, dyd, dzd * dzd));
+    const TN dx = (TN)dxd, dy = (TN)dyd, dz = (TN)dzd;
+    TN ri = rsqrt_t<TN>((TN)r2d);
+    if (mask) ri = TN(0);
+    const TN ri2 = ri * ri, ri3 = ri * ri2;
+    TN cr = 0, ci = 0;
+    if (CHG) {
+      pr = fma(st[0], ri, pr);
+      pi = fma(st[1], ri, pi);
+      if (FLD) { cr = st[0] * ri3; ci = st[1] * ri3; }
+    }
+    if (DIP) {
+      const TN dotr = fma(st[2], dx, fma(st[4], dy, st[6] * dz));
+      const TN doti = fma(st[3], dx, fma(st[5], dy, st[7] * dz));
+      pr = fma(dotr, ri3, pr);
+      pi = fma(doti, ri3, pi);
+      if (FLD) {
+        const TN t = TN(3) * ri2 * ri3;
+        cr = fma(t, dotr, cr);
+        ci = fma(t, doti, ci);
+        fxr = fma(-ri3, st[2], fxr); fxi = fma(-ri3, st[3], fxi);
+        fyr = fma(-ri3, st[4], fyr); fyi = fma(-ri3, st[5], fyi);
+        fzr = fma(-ri3, st[6], fzr); fzi = fma(-ri3, st[7], fzi);
+      }
+    }
+    if (FLD) {
+      fxr = fma(cr, dx, fxr); fxi = fma(ci, dx, fxi);
+      fyr = fma(cr, dy, fyr); fyi = fma(ci, dy, fyi);
+      fzr = fma(cr, dz, fzr); fzi = fma(ci, dz, fzi);
+    }
+  }
+  __device__ __forceinline__ void fold_into(double* a) {
+    a[0] += (double)pr; a[1] += (double)pi;
+    if (FLD) {
+      a[2] += (double)fxr; a[3] += (double)fxi; a[4] += (double)fyr;
+      a[5] += (double)fyi; a[6] += (double)fzr; a[7] += (double)fzi;
+    }
+    pr = pi = fxr = fxi = fyr = fyi = fzr = fzi = TN(0);
+  }
+};
+
+template <typename TN, bool CHG, bool DIP, bool FLD>
+__global__ __launch_bounds__(256) void k_p2p(DevPlan P, const double* __restrict__ spos,
+                                             const TN* __restrict__ str, const double* __restrict__ tpos,
+                                             int kind, double* __restrict__ pot, double* __restrict__ fld) {
   const int w = blockIdx.x;
   const int b = P.p2p_items[3 * w], start = P.p2p_items[3 * w + 2];
   const int n0 = kind ? P.itrg[b] : P.isrc[b];
@@ -602,96 +663,58 @@ __global__ __launch_bounds__(256) void k_p2p(DevPlan P, const T* __restrict__ re
   const int i = start + threadIdx.x;
   const bool active = i < nn;
   const int me = n0 + (active ? i : 0);
-  T x, y, z;
-  if (kind) { x = tpos[3 * (size_t)me]; y = tpos[3 * (size_t)me + 1]; z = tpos[3 * (size_t)me + 2]; }
-  else { x = rec[12 * (size_t)me]; y = rec[12 * (size_t)me + 1]; z = rec[12 * (size_t)me + 2]; }
-  T pr = 0, pi = 0, fxr = 0, fxi = 0, fyr = 0, fyi = 0, fzr = 0, fzi = 0;
-  const int jb = P.l1_off[b], je = P.l1_off[b + 1];
-  for (int jj = jb - 1; jj < je; jj++) {
-    const int j = jj < jb ? b : P.l1_idx[jj];
-    const int s0 = P.isrc[j], s1 = s0 + P.nsrc[j];
-    const bool self = (kind == 0) && (j == b);
-    for (int s = s0; s < s1; s++) {
-      const T* r = rec + 12 * (size_t)s;
-      T dx = x - r[0], dy = y - r[1], dz = z - r[2];
-      T r2 = dx * dx + dy * dy + dz * dz;
-      T ri = rsqrt(r2);
-      if (self && s == me) ri = 0;
-      T ri2 = ri * ri, ri3 = ri * ri2;
-      if (CHG) {
-        pr += r[3] * ri;
-        pi += r[4] * ri;
-        if (FLD) {
-          T cr = r[3] * ri3, ci = r[4] * ri3;
-          fxr += cr * dx; fxi += ci * dx;
-          fyr += cr * dy; fyi += ci * dy;
-          fzr += cr * dz; fzi += ci * dz;
-        }
-      }
-      if (DIP) {
-        T dotr = r[5] * dx + r[7] * dy + r[9] * dz;
-        T doti = r[6] * dx + r[8] * dy + r[10] * dz;
-        pr += dotr * ri3;
-        pi += doti * ri3;
-        if (FLD) {
-          T t = T(3) * ri3 * ri2;
-          T tr = t * dotr, ti = t * doti;
-          fxr += tr * dx - ri3 * r[5]; fxi += ti * dx - ri3 * r[6];
-          fyr += tr * dy - ri3 * r[7]; fyi += ti * dy - ri3 * r[8];
-          fzr += tr * dz - ri3 * r[9]; fzi += ti * dz - ri3 * r[10];
-        }
-      }
+  const double* mp = (kind ? tpos : spos) + 4 * (size_t)me;
+  const double x = mp[0], y = mp[1], z = mp[2];
+  double tot[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  PairAcc<TN, CHG, DIP, FLD> acc;
+  constexpr bool fold = sizeof(TN) < sizeof(double);
+  // own box first: a source never acts on itself (index test, lfmm3dpart.f:1193)
+  {
+    const int s0 = P.isrc[b], s1 = s0 + P.nsrc[b];
+    const int self = kind ? -1 : me;
+    for (int c = s0; c < s1; c += (fold ? kFold : (1 << 30))) {
+      const int ce = fold ? min(c + kFold, s1) : s1;
+      for (int s = c; s < ce; s++) acc.add(x, y, z, spos + 4 * (size_t)s, str + 8 * (size_t)s, s == self);
+      if (fold) acc.fold_into(tot);
     }
   }
-  if (!active) return;
-  pot[2 * (size_t)me] += pr;
-  pot[2 * (size_t)me + 1] += pi;
-  if (FLD) {
-    fld[6 * (size_t)me + 0] += fxr; fld[6 * (size_t)me + 1] += fxi;
-    fld[6 * (size_t)me + 2] += fyr; fld[6 * (size_t)me + 3] += fyi;
-    fld[6 * (size_t)me + 4] += fzr; fld[6 * (size_t)me + 5] += fzi;
+  for (int jj = P.l1_off[b]; jj < P.l1_off[b + 1]; jj++) {
+    const int j = P.l1_idx[jj];
+    const int s0 = P.isrc[j], s1 = s0 + P.nsrc[j];
+    for (int c = s0; c < s1; c += (fold ? kFold : (1 << 30))) {
+      const int ce = fold ? min(c + kFold, s1) : s1;
+      for (int s = c; s < ce; s++) acc.add(x, y, z, spos + 4 * (size_t)s, str + 8 * (size_t)s, false);
+      if (fold) acc.fold_into(tot);
+    }
   }
+  if (!fold) acc.fold_into(tot);
+  if (!active) return;
+  pot[2 * (size_t)me] += tot[0];
+  pot[2 * (size_t)me + 1] += tot[1];
+  if (FLD)
+    for (int k = 0; k < 6; k++) fld[6 * (size_t)me + k] += tot[2 + k];
 }
 
 // Plain direct sum over all sources (l3dpartdirect, src/lfmm3dpart.f:1391):
-// the same pair kernel with one "box" holding everything.
-template <typename T, bool CHG, bool DIP>
-__global__ __launch_bounds__(256) void k_direct(int ns, const T* __restrict__ rec, int npts,
-                                                const T* __restrict__ pts, int pts_are_sources,
-                                                T* __restrict__ pot, T* __restrict__ fld) {
+// the same pair arithmetic in FP64 with every source acting on every point.
+template <bool CHG, bool DIP>
+__global__ __launch_bounds__(256) void k_direct(int ns, const double* __restrict__ spos,
+                                                const double* __restrict__ str, int npts,
+                                                const double* __restrict__ pts, int pts_are_sources,
+                                                double* __restrict__ pot, double* __restrict__ fld) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   const bool active = i < npts;
   const int me = active ? i : 0;
-  T x, y, z;
-  if (pts_are_sources) { x = rec[12 * (size_t)me]; y = rec[12 * (size_t)me + 1]; z = rec[12 * (size_t)me + 2]; }
-  else { x = pts[3 * (size_t)me]; y = pts[3 * (size_t)me + 1]; z = pts[3 * (size_t)me + 2]; }
-  T pr = 0, pi = 0, fxr = 0, fxi = 0, fyr = 0, fyi = 0, fzr = 0, fzi = 0;
-  for (int s = 0; s < ns; s++) {
-    const T* r = rec + 12 * (size_t)s;
-    T dx = x - r[0], dy = y - r[1], dz = z - r[2];
-    T r2 = dx * dx + dy * dy + dz * dz;
-    T ri = rsqrt(r2);
-    if (pts_are_sources && s == me) ri = 0;
-    T ri2 = ri * ri, ri3 = ri * ri2;
-    if (CHG) {
-      pr += r[3] * ri; pi += r[4] * ri;
-      T cr = r[3] * ri3, ci = r[4] * ri3;
-      fxr += cr * dx; fxi += ci * dx; fyr += cr * dy; fyi += ci * dy; fzr += cr * dz; fzi += ci * dz;
-    }
-    if (DIP) {
-      T dotr = r[5] * dx + r[7] * dy + r[9] * dz, doti = r[6] * dx + r[8] * dy + r[10] * dz;
-      pr += dotr * ri3; pi += doti * ri3;
-      T t = T(3) * ri3 * ri2, tr = t * dotr, ti = t * doti;
-      fxr += tr * dx - ri3 * r[5]; fxi += ti * dx - ri3 * r[6];
-      fyr += tr * dy - ri3 * r[7]; fyi += ti * dy - ri3 * r[8];
-      fzr += tr * dz - ri3 * r[9]; fzi += ti * dz - ri3 * r[10];
-    }
-  }
+  const double* mp = (pts_are_sources ? spos : pts) + 4 * (size_t)me;
+  const double x = mp[0], y = mp[1], z = mp[2];
+  PairAcc<double, CHG, DIP, true> acc;
+  const int self = pts_are_sources ? me : -1;
+  for (int s = 0; s < ns; s++) acc.add(x, y, z, spos + 4 * (size_t)s, str + 8 * (size_t)s, s == self);
   if (!active) return;
-  pot[2 * (size_t)i] = pr; pot[2 * (size_t)i + 1] = pi;
-  fld[6 * (size_t)i + 0] = fxr; fld[6 * (size_t)i + 1] = fxi;
-  fld[6 * (size_t)i + 2] = fyr; fld[6 * (size_t)i + 3] = fyi;
-  fld[6 * (size_t)i + 4] = fzr; fld[6 * (size_t)i + 5] = fzi;
+  double t[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  acc.fold_into(t);
+  pot[2 * (size_t)i] = t[0]; pot[2 * (size_t)i + 1] = t[1];
+  for (int k = 0; k < 6; k++) fld[6 * (size_t)i + k] = t[2 + k];
 }
 
 }  // namespace fmm3d

@@ -26,9 +26,12 @@
 extern "C" {
 #endif
 
-/* Precision modes for the *_ex entry points. */
-#define FMM3D_PREC_FP64 0  /* every stage in FP64 (the default)                 */
-#define FMM3D_PREC_MIXED 1 /* FP32 near field + FP32 translations where safe    */
+/* Precision modes for the *_ex / *_dev entry points (a bit mask).  Point
+ * positions, offsets and output accumulation stay FP64 in every mode. */
+#define FMM3D_PREC_FP64 0      /* every stage in FP64 (the default)                 */
+#define FMM3D_PREC_MIXED 1     /* FP32 near-field pair arithmetic, FP64 far field   */
+#define FMM3D_PREC_FAR32 2     /* FP32 expansions and translations, FP64 near field */
+#define FMM3D_PREC_FP32 3      /* FP32 near and far field                          */
 
 /* Replaces lfmm3dparttarg, src/lfmm3dpart.f:138-141. */
 int fmm3d_lfmm3dparttarg(int iprec, int nsource, const double *source,
@@ -74,6 +77,21 @@ int fmm3d_lfmm3dparttarg_dev(int prec_mode, int iprec, int nsource, const double
                              int ntarget, const double *target_dev,
                              int ifpottarg, double *pottarg_dev, int iffldtarg, double *fldtarg_dev,
                              void *stream);
+
+/* Replaces d3tstrcr with its accessors d3tgetb/d3tgetl
+ * (src/d3tstrcr.f:24-259, :580-689): builds the adaptive octree the FMM
+ * uses (host code) and returns it flattened.  boxes20 is the d3tgetb
+ * layout (level, i, j, k, parent, 8 children, source start, source count,
+ * target start, target count, 3 unused; ids and starts 1-based),
+ * isource/itarget the 1-based permutations, list_counts[k*maxboxes+b] the
+ * length of list k+1 (k = 0..3) of box b and `lists` their entries (1-based,
+ * ascending per box) concatenated list by list, box by box.  Call with
+ * maxboxes = 0 to obtain nboxes and the total list length (*nlist) first.
+ * Returns 0, 128 (nsource < 1) or 32 (arrays too small). */
+int fmm3d_d3tstrcr(int nsource, const double *source, int ntarget, const double *target,
+                   int nbox, int *nboxes, int *nlev, double *center, double *size, int *nlist,
+                   int maxboxes, int *boxes20, int *isource, int *itarget,
+                   int *list_counts, int maxlist, int *lists);
 
 /* Per-stage timings (milliseconds, device time from HIP events) of the last
  * FMM call on this thread: [0] tree+plan (host), [1] upload, [2] P2M,

@@ -1,0 +1,60 @@
+"""Seeded input generators shared by the tests and bench.py (the workloads of
+the benchmark configurations and smaller parity cases)."""
+import numpy as np
+
+
+def points(rng, n, dist):
+    if n == 0:
+        return None
+    if dist == "cube":
+        return rng.random((n, 3))
+    if dist == "sphere":
+        x = rng.normal(size=(n, 3))
+        return x / np.linalg.norm(x, axis=1)[:, None]
+    if dist == "shell":
+        x = rng.normal(size=(n, 3))
+        x /= np.linalg.norm(x, axis=1)[:, None]
+        return x * rng.uniform(0.995, 1.005, n)[:, None]
+    if dist == "clusters":
+        c = rng.random((8, 3)) * 0.8 + 0.1
+        return np.clip(c[rng.integers(0, 8, n)] + 0.02 * rng.normal(size=(n, 3)), 0, 1)
+    if dist == "plummer":
+        out = np.empty((0, 3))
+        while out.shape[0] < n:
+            m = 2 * (n - out.shape[0]) + 16
+            u = rng.uniform(1e-6, 1, m)
+            r = 0.1 / np.sqrt(u ** (-2.0 / 3.0) - 1.0)
+            d = rng.normal(size=(m, 3))
+            d /= np.linalg.norm(d, axis=1)[:, None]
+            x = 0.5 + r[:, None] * d
+            x = x[np.all((x >= 0) & (x <= 1), axis=1)]
+            out = np.concatenate([out, x])
+        return out[:n]
+    if dist == "plane":
+        x = rng.random((n, 3))
+        x[:, 2] = 0.25
+        return x
+    raise ValueError(dist)
+
+
+def strengths(rng, n, ifcharge=True, ifdipole=False, unit_dipvec=True):
+    q = (rng.uniform(-1, 1, n) + 1j * rng.uniform(-1, 1, n)) if ifcharge else None
+    ds = dv = None
+    if ifdipole:
+        ds = rng.uniform(-1, 1, n) + 1j * rng.uniform(-1, 1, n)
+        if unit_dipvec:
+            z = rng.uniform(-1, 1, n)
+            ph = rng.uniform(0, 2 * np.pi, n)
+            s = np.sqrt(1 - z * z)
+            dv = np.stack([s * np.cos(ph), s * np.sin(ph), z], 1)
+        else:
+            dv = rng.normal(size=(n, 3))
+    return q, ds, dv
+
+
+def rel(a, b):
+    return float(np.linalg.norm(np.asarray(a) - np.asarray(b)) / np.linalg.norm(np.asarray(b)))
+
+
+# tolerance implied by iprec (src/lfmm3dpart.f:277-285)
+IPREC_EPS = {-2: .5e0, -1: .5e-1, 0: .5e-2, 1: .5e-3, 2: .5e-6, 3: .5e-9, 4: .5e-12, 5: .5e-15}

@@ -1,0 +1,233 @@
+"""GPU parity of the Laplace particle FMM (lfmm3dparttarg and friends through
+the C ABI) against the CPU oracle, the reference's golden fixtures and the
+FP64 direct sum.  Bars: FP64 path within 1e-12 relative L2 of the oracle
+(same tree, orders and truncation; only roundoff differs); mixed path
+within the iprec tolerance of the FP64 direct sum."""
+import glob
+import os
+
+import numpy as np
+import pytest
+
+import fmmlib3d_amd as F
+import oracle_bindings as ob
+from fmm_inputs import IPREC_EPS, points, rel, strengths
+
+pytestmark = pytest.mark.gpu
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+FMM_FIXTURES = sorted(glob.glob(os.path.join(GOLD, "lap_targ_*.npz")) + glob.glob(os.path.join(GOLD, "lap_self_*.npz")))
+FP64_TOL = 1e-12
+
+
+def _check(a, b, tol):
+    if b is None:
+        assert a is None
+        return
+    nb = np.linalg.norm(b)
+    if nb == 0:
+        assert np.linalg.norm(a) == 0
+    else:
+        assert rel(a, b) < tol, rel(a, b)
+
+
+@pytest.mark.parametrize("path", FMM_FIXTURES, ids=[os.path.basename(p)[:-4] for p in FMM_FIXTURES])
+def test_gpu_matches_reference_fixture(path):
+    f = np.load(path, allow_pickle=False)
+    g = lambda k: f[k] if k in f.files else None  # noqa: E731
+    ier, pot, fld, pt, ft = F.lfmm3dparttarg(int(f["iprec"]), f["source"], g("charge"), g("dipstr"),
+                                             g("dipvec"), g("target"))
+    assert ier == 0
+    _check(pot, f["pot"], FP64_TOL)
+    _check(fld, f["fld"], FP64_TOL)
+    if "target" in f.files:
+        _check(pt, f["pottarg"], FP64_TOL)
+        _check(ft, f["fldtarg"], FP64_TOL)
+
+
+def test_gpu_direct_matches_reference_fixture():
+    f = np.load(os.path.join(GOLD, "lap_direct.npz"), allow_pickle=False)
+    pot, fld, pt, ft = F.l3dpartdirect(f["source"], f["charge"], f["dipstr"], f["dipvec"], f["target"])
+    for a, k in ((pot, "pot"), (fld, "fld"), (pt, "pottarg"), (ft, "fldtarg")):
+        _check(a, f[k], 1e-13)
+
+
+CASES = [
+    # ns, nt, iprec, dist, ifcharge, ifdipole
+    (5000, 3000, 2, "cube", 1, 1),
+    (20000, 0, 1, "cube", 1, 0),
+    (6000, 0, 2, "sphere", 1, 1),
+    (8000, 4000, 1, "clusters", 1, 1),
+    (3000, 2000, 3, "clusters", 0, 1),
+    (4000, 4000, 0, "plummer", 1, 1),
+    (3000, 1000, -1, "shell", 1, 0),
+    (2000, 500, -2, "plane", 1, 1),
+    (1, 3, 1, "cube", 1, 1),          # a single source
+    (2, 0, 2, "cube", 1, 1),
+    (150, 40, 1, "cube", 1, 1),       # N <= nbox: root leaf, no far field
+    (300, 100, 6, "cube", 1, 1),      # iprec=6: nbox = N, the FMM is a direct sum
+]
+
+
+@pytest.mark.parametrize("ns,nt,iprec,dist,ifc,ifd", CASES)
+def test_gpu_vs_oracle(rng, ns, nt, iprec, dist, ifc, ifd):
+    src = points(rng, ns, dist)
+    trg = points(rng, nt, dist)
+    q, ds, dv = strengths(rng, ns, ifc, ifd, unit_dipvec=False)
+    g = F.lfmm3dparttarg(iprec, src, q, ds, dv, trg)
+    o = ob.orc_lfmm3dparttarg(iprec, src, q, ds, dv, trg)
+    assert g[0] == o[0] == 0
+    for k in range(1, 5 if nt else 3):
+        _check(g[k], o[k], FP64_TOL)
+
+
+@pytest.mark.parametrize("flags", [(1, 0, 0, 0), (0, 1, 0, 0), (0, 0, 1, 0), (0, 0, 0, 1), (1, 1, 0, 1), (0, 0, 1, 1)])
+def test_gpu_output_flags(rng, flags):
+    """Outputs are written only when their flag is 1 (src/lfmm3dpart.f:511-519)."""
+    src = points(rng, 4000, "cube")
+    trg = points(rng, 1500, "cube")
+    q, ds, dv = strengths(rng, 4000, True, True)
+    ifpot, iffld, ifpt, ifft = flags
+    g = F.lfmm3dparttarg(1, src, q, ds, dv, trg, ifpot, iffld, ifpt, ifft)
+    o = ob.orc_lfmm3dparttarg(1, src, q, ds, dv, trg, ifpot, iffld, ifpt, ifft)
+    assert g[0] == 0
+    for k, flag in zip(range(1, 5), flags):
+        if flag:
+            _check(g[k], o[k], FP64_TOL)
+        else:
+            assert g[k] is None
+
+
+def test_gpu_scaled_and_shifted_coordinates(rng):
+    """The tree and the expansions only see relative geometry."""
+    src = points(rng, 5000, "clusters")
+    trg = points(rng, 2000, "clusters")
+    q, ds, dv = strengths(rng, 5000, True, True)
+    base = F.lfmm3dparttarg(2, src, q, ds, dv, trg)
+    s, shift = 1e-2, 3.0
+    moved = F.lfmm3dparttarg(2, src * s + shift, q, ds, dv, trg * s + shift)
+    o = ob.orc_lfmm3dparttarg(2, src * s + shift, q, ds, dv, trg * s + shift)
+    for k in range(1, 5):
+        _check(moved[k], o[k], 1e-11)
+    # potential scales as 1/s, field as 1/s^2 (the shifted coordinates carry a
+    # rounding of 3 * 2^-53 / 1e-2 relative to the box size)
+    qonly = F.lfmm3dparttarg(2, src, q, None, None, trg)
+    qmoved = F.lfmm3dparttarg(2, src * s + shift, q, None, None, trg * s + shift)
+    _check(qmoved[3] * s, qonly[3], 1e-9)
+    _check(qmoved[4] * s * s, qonly[4], 1e-9)
+    assert base[0] == 0
+
+
+def test_gpu_deterministic(rng):
+    src = points(rng, 30000, "cube")
+    trg = points(rng, 10000, "cube")
+    q, ds, dv = strengths(rng, 30000, True, True)
+    a = F.lfmm3dparttarg(2, src, q, ds, dv, trg)
+    b = F.lfmm3dparttarg(2, src, q, ds, dv, trg)
+    for k in range(1, 5):
+        assert np.array_equal(a[k], b[k])
+
+
+def test_gpu_selfcall_matches_targ(rng):
+    src = points(rng, 5000, "sphere")
+    q, ds, dv = strengths(rng, 5000, True, True)
+    ier, pot, fld = F.lfmm3dpartself(2, src, q, ds, dv)
+    o = ob.orc_lfmm3dparttarg(2, src, q, ds, dv, None)
+    assert ier == 0
+    _check(pot, o[1], FP64_TOL)
+    _check(fld, o[2], FP64_TOL)
+
+
+def test_gpu_direct_vs_oracle(rng):
+    src = points(rng, 3000, "cube")
+    trg = points(rng, 1000, "cube")
+    q, ds, dv = strengths(rng, 3000, True, True)
+    g = F.l3dpartdirect(src, q, ds, dv, trg)
+    o = ob.orc_l3dpartdirect(src, q, ds, dv, trg)
+    for a, b in zip(g, o):
+        _check(a, b, 1e-13)
+
+
+@pytest.mark.parametrize("mode", [F.PREC_MIXED, F.PREC_FAR32, F.PREC_FP32])
+def test_gpu_reduced_precision_within_tolerance(rng, mode):
+    """FP32 modes against the FP64 direct sum, at the tolerance implied by
+    iprec for the potential and one digit less for the field, never tighter
+    than 1e-6 (the headline bar)."""
+    for iprec, dist in ((1, "cube"), (2, "cube"), (2, "clusters")):
+        src = points(rng, 20000, dist)
+        trg = points(rng, 5000, dist)
+        q, ds, dv = strengths(rng, 20000, True, True)
+        g = F.lfmm3dparttarg(iprec, src, q, ds, dv, trg, precision=mode)
+        assert g[0] == 0
+        idx = rng.choice(5000, 400, replace=False)
+        pd, fd = ob.orc_direct_sample(src, trg[idx], -np.ones(400, np.int32), q, ds, dv)
+        tol = max(IPREC_EPS[iprec], 1e-6)
+        assert rel(g[3][idx], pd) < tol
+        assert rel(g[4][idx], fd) < 10 * tol
+
+
+def _hip():
+    """The HIP runtime the library itself uses (no torch in this process)."""
+    import ctypes
+    rt = ctypes.CDLL(os.path.join(os.environ.get("ROCM_PATH", "/opt/rocm"), "lib", "libamdhip64.so"))
+    rt.hipMalloc.argtypes = [ctypes.POINTER(ctypes.c_void_p), ctypes.c_size_t]
+    rt.hipFree.argtypes = [ctypes.c_void_p]
+    rt.hipMemcpy.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int]
+    rt.hipDeviceSynchronize.argtypes = []
+    return rt
+
+
+def test_gpu_device_api_matches_host_api(rng):
+    import ctypes
+    rt = _hip()
+    ns, nt = 20000, 8000
+    src = points(rng, ns, "cube")
+    trg = points(rng, nt, "cube")
+    q, ds, dv = strengths(rng, ns, True, True)
+    host = F.lfmm3dparttarg(2, src, q, ds, dv, trg)
+    bufs = []
+
+    def dev(a):
+        p = ctypes.c_void_p()
+        assert rt.hipMalloc(ctypes.byref(p), a.nbytes) == 0
+        assert rt.hipMemcpy(p, a.ctypes.data_as(ctypes.c_void_p), a.nbytes, 1) == 0  # H2D
+        bufs.append(p)
+        return p
+
+    try:
+        ins = [dev(np.ascontiguousarray(a)) for a in (src, q, ds, dv, trg)]
+        outs_h = [np.zeros(ns, np.complex128), np.zeros((ns, 3), np.complex128),
+                  np.zeros(nt, np.complex128), np.zeros((nt, 3), np.complex128)]
+        outs = [dev(a) for a in outs_h]
+        ier = F.load().fmm3d_lfmm3dparttarg_dev(0, 2, ns, ins[0], 1, ins[1], 1, ins[2], ins[3], 1, outs[0], 1,
+                                                outs[1], nt, ins[4], 1, outs[2], 1, outs[3], None)
+        assert ier == 0
+        assert rt.hipDeviceSynchronize() == 0
+        for h, d, b in zip(outs_h, outs, host[1:]):
+            assert rt.hipMemcpy(h.ctypes.data_as(ctypes.c_void_p), d, h.nbytes, 2) == 0  # D2H
+            assert np.array_equal(h, b)
+    finally:
+        for p in bufs:
+            rt.hipFree(p)
+
+
+@pytest.mark.slow
+def test_gpu_headline_config_accuracy_and_linearity():
+    """1M sources + 1M targets in the unit cube, charges + unit dipoles,
+    iprec=2: FP64 error vs the FP64 direct sum on sampled targets, and the
+    result is linear in the strengths."""
+    rng = np.random.default_rng(2)
+    n = 1_000_000
+    src = points(rng, n, "cube")
+    trg = points(rng, n, "cube")
+    q, ds, dv = strengths(rng, n, True, True)
+    ier, pot, fld, pt, ft = F.lfmm3dparttarg(2, src, q, ds, dv, trg, ifpot=0, iffld=0)
+    assert ier == 0
+    idx = rng.choice(n, 300, replace=False)
+    pd, fd = ob.orc_direct_sample(src, trg[idx], -np.ones(300, np.int32), q, ds, dv)
+    assert rel(pt[idx], pd) < IPREC_EPS[2]
+    assert rel(ft[idx], fd) < 10 * IPREC_EPS[2]
+    q2 = (rng.uniform(-1, 1, n) + 1j * rng.uniform(-1, 1, n))
+    a = F.lfmm3dparttarg(2, src, q2, None, None, trg, ifpot=0, iffld=0)
+    b = F.lfmm3dparttarg(2, src, q + q2, ds, dv, trg, ifpot=0, iffld=0)
+    assert rel(pt + a[3], b[3]) < 1e-12
+    assert rel(ft + a[4], b[4]) < 1e-12

@@ -1,0 +1,85 @@
+"""The C-ABI library: it loads, exports what include/*.h declares, rejects
+bad arguments with the reference's ier values before touching the GPU, and
+its host octree is the reference's tree.  No device compute here."""
+import glob
+import os
+import re
+
+import numpy as np
+import pytest
+
+import fmmlib3d_amd as F
+import oracle_bindings as ob
+from fmm_inputs import points
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared():
+    names = set()
+    for h in glob.glob(os.path.join(ROOT, "include", "*.h")):
+        txt = re.sub(r"/\*.*?\*/", "", open(h).read(), flags=re.S)
+        names |= set(re.findall(r"\b(fmm3d_\w+)\s*\(", txt))
+    return names
+
+
+def test_library_exports_every_declared_symbol():
+    lib = F.load()
+    names = declared()
+    assert names, "no declarations found"
+    for n in sorted(names):
+        assert hasattr(lib, n), n
+    assert set(F.EXPORTED) == names
+
+
+def test_version_string():
+    assert "gfx950" in F.version()
+
+
+def test_argument_errors_before_device_use():
+    src = np.zeros((4, 3))
+    q = np.ones(4, np.complex128)
+    assert F.lfmm3dparttarg(1, np.zeros((0, 3)), np.zeros(0, np.complex128))[0] == 4
+    assert F.lfmm3dparttarg(7, src, q)[0] == 32
+    assert F.lfmm3dparttarg(-3, src, q)[0] == 32
+    lib = F.load()
+    # negative ntarget
+    assert lib.fmm3d_lfmm3dparttarg(1, 4, F._ptr(src), 1, F._ptr(q), 0, None, None, 1, None, 0, None,
+                                    -1, None, 0, None, 0, None) == 128
+    # unknown precision mode
+    assert F.lfmm3dparttarg(1, src, q, precision=9)[0] == 128
+
+
+@pytest.mark.parametrize("ns,nt,nbox,dist", [(3000, 2000, 40, "cube"), (5000, 0, 50, "sphere"),
+                                             (4000, 3000, 80, "clusters"), (3000, 100, 30, "plummer"),
+                                             (900, 300, 40, "plane"), (50, 10, 160, "cube")])
+def test_host_octree_equals_oracle(rng, ns, nt, nbox, dist):
+    src = points(rng, ns, dist)
+    trg = points(rng, nt, dist)
+    ier, b1, i1, t1, l1, n1, c1, s1 = F.d3tstrcr(src, trg, nbox)
+    b2, i2, t2, l2, n2 = ob.orc_tree(src, trg, nbox)
+    assert ier == 0 and n1 == n2
+    assert np.array_equal(b1[:, :17], b2[:, :17])
+    assert np.array_equal(i1, i2) and np.array_equal(t1, t2)
+    for k in range(4):
+        assert l1[k] == l2[k], "list %d" % (k + 1)
+
+
+def test_host_octree_equals_golden_tree():
+    f = np.load(os.path.join(ROOT, "tests", "golden", "tree_clusters.npz"), allow_pickle=False)
+    ier, boxes, isrc, itrg, lists, nlev, _, _ = F.d3tstrcr(f["source"], f["target"], int(f["nbox"]))
+    assert ier == 0 and nlev == int(f["nlev"])
+    assert np.array_equal(boxes[:, :17], f["boxes"][:, :17])
+    assert np.array_equal(isrc, f["isrc"]) and np.array_equal(itrg, f["itrg"])
+    counts, flat = f["list_counts"], f["lists"]
+    o = 0
+    for k in range(5):
+        for b in range(boxes.shape[0]):
+            c = counts[k, b]
+            if k < 4:
+                assert lists[k][b] == tuple(flat[o:o + c].tolist())
+            o += c
+
+
+def test_host_octree_rejects_empty():
+    assert F.d3tstrcr(np.zeros((0, 3)), None, 10)[0] == 128
