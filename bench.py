@@ -8,7 +8,10 @@ HBM before the timed region; a step is one call of the device-resident entry
 point (host octree on copied-back coordinates, upload of the plan, the eight
 FMM stages, scatter back), so `value` = ntarget / seconds per step.
 
-Both precision modes run; `value` is the fastest mode whose relative L2
+Three precision modes run (fp64; mixed_near: FP32 pair arithmetic in the
+near field; mixed: FP32 near field and FP32 expansions/translations, with FP64
+positions, offsets and accumulation in every mode); `value` is the fastest
+mode whose relative L2
 error against an FP64 direct sum on 1000 sampled targets is <= 1e-6 for the
 potential and the field.  The P2P roofline uses the exact pair count of the
 tree; the CPU baseline times the reference library (oracle/_ref) on a
@@ -132,9 +135,21 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-baseline-child", type=int, default=0, help=argparse.SUPPRESS)
+    ap.add_argument("--calibrate-copy-mb", type=int, default=0,
+                    help="only run a plain device-to-device copy of this many MiB (HBM counter calibration)")
     args = ap.parse_args()
     if args.cpu_baseline_child:
         cpu_baseline_child(args.cpu_baseline_child)
+        return
+    if args.calibrate_copy_mb:
+        import torch
+        n = args.calibrate_copy_mb * (1 << 20) // 8
+        a = torch.ones(n, dtype=torch.float64, device="cuda:0")
+        b = torch.empty_like(a)
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_read": n * 8, "copy_bytes_written": n * 8, "launches": 3}))
         return
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
@@ -159,7 +174,7 @@ def main():
     pairs = p2p_pairs(src, trg)
 
     modes = {}
-    for name, prec in (("fp64", F.PREC_FP64), ("mixed", F.PREC_MIXED)):
+    for name, prec in (("fp64", F.PREC_FP64), ("mixed_near", F.PREC_MIXED), ("mixed", F.PREC_FP32)):
         total, lat, stages = run_mode(F, torch, prec, dev_in, dev_out, args.steps, args.warmup)
         pt = dev_out[2].cpu().numpy()
         ft = dev_out[3].cpu().numpy()
@@ -176,7 +191,7 @@ def main():
               % (name, modes[name]["targets_per_s"], modes[name]["ms_per_step"], e_pot, e_fld,
                  {k: round(v, 2) for k, v in modes[name]["stages_ms"].items()}), file=sys.stderr, flush=True)
 
-    ok = [m for m in ("mixed", "fp64") if modes[m]["err_pottarg"] <= 1e-6 and modes[m]["err_fldtarg"] <= 1e-6]
+    ok = [m for m in modes if modes[m]["err_pottarg"] <= 1e-6 and modes[m]["err_fldtarg"] <= 1e-6]
     best = max(ok, key=lambda m: modes[m]["targets_per_s"]) if ok else "fp64"
     M = modes[best]
     flops = pairs * FLOPS_PER_PAIR

@@ -60,20 +60,27 @@ def build_hip(verbose=False):
     return LIB
 
 
+FDRIVER = os.path.join(LIBDIR, "lfmm3dpart_gpu_driver")
+
+
 def build_fortran(verbose=False):
-    """ISO_C_BINDING module exporting the reference's F77 entry names."""
+    """ISO_C_BINDING library exporting the reference's F77 entry names, and a
+    Fortran test driver linked against it."""
     if not os.path.exists(FC):
         return None
     src = os.path.join(HERE, "fortran", "fmm3d_mi355x.f90")
-    if not os.path.exists(src):
-        return None
     os.makedirs(OBJDIR, exist_ok=True)
     tmp = FLIB + ".tmp"
-    _run([FC, "-O2", "-fPIC", "-shared", "-J", OBJDIR, src, "-o", tmp,
+    _run([FC, "-O2", "-fPIC", "-shared", "-module-dir", OBJDIR, src, "-o", tmp,
           "-L", LIBDIR, "-lfmm3d_mi355x", "-Wl,-rpath,$ORIGIN"])
     os.replace(tmp, FLIB)
+    drv = os.path.join(ROOT, "tests", "fortran", "lfmm3dpart_gpu_driver.f90")
+    tmp = FDRIVER + ".tmp"
+    _run([FC, "-O2", "-module-dir", OBJDIR, drv, "-o", tmp, "-L", LIBDIR, "-lfmm3d_mi355x_f",
+          "-lfmm3d_mi355x", "-Wl,-rpath,$ORIGIN"])
+    os.replace(tmp, FDRIVER)
     if verbose:
-        print("built", FLIB)
+        print("built", FLIB, "and", FDRIVER)
     return FLIB
 
 

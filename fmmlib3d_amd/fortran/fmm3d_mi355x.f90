@@ -1,0 +1,77 @@
+! fmm3d_mi355x.f90 -- ISO_C_BINDING drop-in for fmmlib3d's particle FMM.
+!
+! Exports the reference's F77 entry points with their exact calling
+! sequences (src/lfmm3dpart.f:36-141, :1391-1394): lfmm3dpart,
+! lfmm3dpartself, lfmm3dparttarg and l3dpartdirect.  Each one forwards its
+! arguments to the C ABI of libfmm3d_mi355x.so (include/fmm3d_mi355x.h), so a
+! Fortran program links against this library instead of the reference
+! objects and runs the FMM on the GPU.  Arrays keep the reference layouts and
+! are passed through without copies; ier keeps the reference's meaning.
+module fmm3d_mi355x_c
+  use iso_c_binding
+  implicit none
+  interface
+    integer(c_int) function c_lfmm3dparttarg(iprec, nsource, source, ifcharge, charge, &
+        ifdipole, dipstr, dipvec, ifpot, pot, iffld, fld, ntarget, target, ifpottarg, &
+        pottarg, iffldtarg, fldtarg) bind(C, name="fmm3d_lfmm3dparttarg")
+      import :: c_int, c_double, c_double_complex
+      integer(c_int), value :: iprec, nsource, ifcharge, ifdipole, ifpot, iffld, ntarget
+      integer(c_int), value :: ifpottarg, iffldtarg
+      real(c_double) :: source(*), dipvec(*), target(*)
+      complex(c_double_complex) :: charge(*), dipstr(*), pot(*), fld(*), pottarg(*), fldtarg(*)
+    end function c_lfmm3dparttarg
+
+    subroutine c_l3dpartdirect(nsource, source, ifcharge, charge, ifdipole, dipstr, dipvec, &
+        ifpot, pot, iffld, fld, ntarget, target, ifpottarg, pottarg, iffldtarg, fldtarg) &
+        bind(C, name="fmm3d_l3dpartdirect")
+      import :: c_int, c_double, c_double_complex
+      integer(c_int), value :: nsource, ifcharge, ifdipole, ifpot, iffld, ntarget
+      integer(c_int), value :: ifpottarg, iffldtarg
+      real(c_double) :: source(*), dipvec(*), target(*)
+      complex(c_double_complex) :: charge(*), dipstr(*), pot(*), fld(*), pottarg(*), fldtarg(*)
+    end subroutine c_l3dpartdirect
+  end interface
+end module fmm3d_mi355x_c
+
+subroutine lfmm3dparttarg(ier, iprec, nsource, source, ifcharge, charge, ifdipole, dipstr, &
+    dipvec, ifpot, pot, iffld, fld, ntarget, target, ifpottarg, pottarg, iffldtarg, fldtarg)
+  use fmm3d_mi355x_c
+  implicit none
+  integer :: ier, iprec, nsource, ifcharge, ifdipole, ifpot, iffld, ntarget, ifpottarg, iffldtarg
+  real(8) :: source(*), dipvec(*), target(*)
+  complex(8) :: charge(*), dipstr(*), pot(*), fld(*), pottarg(*), fldtarg(*)
+  ier = c_lfmm3dparttarg(iprec, nsource, source, ifcharge, charge, ifdipole, dipstr, dipvec, &
+      ifpot, pot, iffld, fld, ntarget, target, ifpottarg, pottarg, iffldtarg, fldtarg)
+end subroutine lfmm3dparttarg
+
+subroutine lfmm3dpartself(ier, iprec, nsource, source, ifcharge, charge, ifdipole, dipstr, &
+    dipvec, ifpot, pot, iffld, fld)
+  use fmm3d_mi355x_c
+  implicit none
+  integer :: ier, iprec, nsource, ifcharge, ifdipole, ifpot, iffld
+  real(8) :: source(*), dipvec(*), dummy(3)
+  complex(8) :: charge(*), dipstr(*), pot(*), fld(*), cdummy(3)
+  ier = c_lfmm3dparttarg(iprec, nsource, source, ifcharge, charge, ifdipole, dipstr, dipvec, &
+      ifpot, pot, iffld, fld, 0, dummy, 0, cdummy, 0, cdummy)
+end subroutine lfmm3dpartself
+
+subroutine lfmm3dpart(ier, iprec, nsource, source, ifcharge, charge, ifdipole, dipstr, &
+    dipvec, ifpot, pot, iffld, fld)
+  implicit none
+  integer :: ier, iprec, nsource, ifcharge, ifdipole, ifpot, iffld
+  real(8) :: source(*), dipvec(*)
+  complex(8) :: charge(*), dipstr(*), pot(*), fld(*)
+  call lfmm3dpartself(ier, iprec, nsource, source, ifcharge, charge, ifdipole, dipstr, &
+      dipvec, ifpot, pot, iffld, fld)
+end subroutine lfmm3dpart
+
+subroutine l3dpartdirect(nsource, source, ifcharge, charge, ifdipole, dipstr, dipvec, &
+    ifpot, pot, iffld, fld, ntarget, target, ifpottarg, pottarg, iffldtarg, fldtarg)
+  use fmm3d_mi355x_c
+  implicit none
+  integer :: nsource, ifcharge, ifdipole, ifpot, iffld, ntarget, ifpottarg, iffldtarg
+  real(8) :: source(*), dipvec(*), target(*)
+  complex(8) :: charge(*), dipstr(*), pot(*), fld(*), pottarg(*), fldtarg(*)
+  call c_l3dpartdirect(nsource, source, ifcharge, charge, ifdipole, dipstr, dipvec, &
+      ifpot, pot, iffld, fld, ntarget, target, ifpottarg, pottarg, iffldtarg, fldtarg)
+end subroutine l3dpartdirect
