@@ -59,6 +59,30 @@ void orc_l3dmploc(double sc1, const double *c1, const zdouble *mp, int p1, int p
 void orc_l3dlocloc(double sc1, const double *c1, const zdouble *loc1, int p1,
                    double sc2, const double *c2, zdouble *loc2, int p2);
 
+/* ---- Helmholtz special functions and operators (fmm_oracle_helm.c; the
+ * coefficient convention is stated there).  Expansions packed as above,
+ * operators accumulate into their output. ---- */
+void orc_h3dall(int n, zdouble z, double sc, zdouble *hs);      /* h_i(z) sc^i, i<=n */
+int  orc_jfuns3d(int n, zdouble z, double sc, zdouble *fj);     /* j_i(z)/sc^i, i<=n */
+void orc_h3dformmp(zdouble zk, double sc, const double *src, const zdouble *q, int ns,
+                   const double *center, int p, zdouble *mp);
+void orc_h3dformmp_dp(zdouble zk, double sc, const double *src, const zdouble *ds, const double *dv,
+                      int ns, const double *center, int p, zdouble *mp);
+void orc_h3dformta(zdouble zk, double sc, const double *src, const zdouble *q, int ns,
+                   const double *center, int p, zdouble *loc);
+void orc_h3dformta_dp(zdouble zk, double sc, const double *src, const zdouble *ds, const double *dv,
+                      int ns, const double *center, int p, zdouble *loc);
+void orc_h3dmpeval(zdouble zk, double sc, const double *center, const zdouble *mp, int p,
+                   const double *trg, int nt, int ifpot, zdouble *pot, int iffld, zdouble *fld);
+void orc_h3dtaeval(zdouble zk, double sc, const double *center, const zdouble *loc, int p,
+                   const double *trg, int nt, int ifpot, zdouble *pot, int iffld, zdouble *fld);
+void orc_h3dmpmp(zdouble zk, double sc1, const double *c1, const zdouble *mp1, int p1,
+                 double sc2, const double *c2, zdouble *mp2, int p2);
+void orc_h3dmploc(zdouble zk, double sc1, const double *c1, const zdouble *mp, int p1, int ptrunc,
+                  double sc2, const double *c2, zdouble *loc, int p2);
+void orc_h3dlocloc(zdouble zk, double sc1, const double *c1, const zdouble *loc1, int p1,
+                   double sc2, const double *c2, zdouble *loc2, int p2);
+
 /* rotation about y by theta of a packed expansion (coefficient convention above) */
 void orc_rotate_y(int p, double theta, const zdouble *in, zdouble *out);
 

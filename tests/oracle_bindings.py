@@ -143,10 +143,69 @@ def orc_direct_sample(source, points, self_index, charge=None, dipstr=None, dipv
                                int(ds is not None), _vp(ds), _vp(dv), pts.shape[0], _vp(pts),
                                _vp(si), _vp(pot), _vp(fld))
     else:
-        zk_c = ctypes.c_double * 2
-        o.orc_h3ddirect_sample.argtypes = [zk_c] + [ctypes.c_void_p] * 0
-        raise NotImplementedError
+        o.orc_h3ddirect_sample.argtypes = [ZC] + [ctypes.c_int, ctypes.c_void_p, ctypes.c_int,
+                                                 ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p,
+                                                 ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p,
+                                                 ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]
+        o.orc_h3ddirect_sample(zc(zk), src.shape[0], _vp(src), int(q is not None), _vp(q),
+                               int(ds is not None), _vp(ds), _vp(dv), pts.shape[0], _vp(pts),
+                               _vp(si), _vp(pot), _vp(fld))
     return pot, fld
+
+
+class ZC(ctypes.Structure):
+    """C `double _Complex` passed by value (two doubles, SysV classes SSE,SSE)."""
+    _fields_ = [("re", ctypes.c_double), ("im", ctypes.c_double)]
+
+
+def zc(v):
+    v = complex(v)
+    return ZC(v.real, v.imag)
+
+
+def orc_hfmm3dparttarg(iprec, zk, source, charge=None, dipstr=None, dipvec=None, target=None,
+                       ifpot=1, iffld=1, ifpottarg=1, iffldtarg=1):
+    o = oracle()
+    src = _c(source, np.float64)
+    ns = src.shape[0]
+    trg = _c(target, np.float64) if target is not None else np.zeros((1, 3))
+    nt = 0 if target is None else trg.shape[0]
+    q = _c(charge, np.complex128)
+    ds = _c(dipstr, np.complex128)
+    dv = _c(dipvec, np.float64)
+    pot = np.zeros(ns, np.complex128)
+    fld = np.zeros((ns, 3), np.complex128)
+    pt = np.zeros(max(nt, 1), np.complex128)
+    ft = np.zeros((max(nt, 1), 3), np.complex128)
+    V, I_ = ctypes.c_void_p, ctypes.c_int
+    o.orc_hfmm3dparttarg.restype = I_
+    o.orc_hfmm3dparttarg.argtypes = [I_, ZC, I_, V, I_, V, I_, V, V, I_, V, I_, V, I_, V, I_, V, I_, V]
+    ier = o.orc_hfmm3dparttarg(int(iprec), zc(zk), ns, _vp(src), int(q is not None), _vp(q),
+                               int(ds is not None), _vp(ds), _vp(dv), ifpot, _vp(pot), iffld,
+                               _vp(fld), nt, _vp(trg), ifpottarg if nt else 0, _vp(pt),
+                               iffldtarg if nt else 0, _vp(ft))
+    return ier, pot, fld, pt[:nt], ft[:nt]
+
+
+def orc_h3dpartdirect(zk, source, charge=None, dipstr=None, dipvec=None, target=None):
+    o = oracle()
+    src = _c(source, np.float64)
+    ns = src.shape[0]
+    trg = _c(target, np.float64) if target is not None else np.zeros((1, 3))
+    nt = 0 if target is None else trg.shape[0]
+    q = _c(charge, np.complex128)
+    ds = _c(dipstr, np.complex128)
+    dv = _c(dipvec, np.float64)
+    pot = np.zeros(ns, np.complex128)
+    fld = np.zeros((ns, 3), np.complex128)
+    pt = np.zeros(max(nt, 1), np.complex128)
+    ft = np.zeros((max(nt, 1), 3), np.complex128)
+    V, I_ = ctypes.c_void_p, ctypes.c_int
+    o.orc_h3dpartdirect.argtypes = [ZC, I_, V, I_, V, I_, V, V, I_, V, I_, V, I_, V, I_, V, I_, V]
+    o.orc_h3dpartdirect(zc(zk), ns, _vp(src), int(q is not None), _vp(q), int(ds is not None), _vp(ds),
+                        _vp(dv), 1, _vp(pot), 1, _vp(fld), nt, _vp(trg), 1 if nt else 0, _vp(pt),
+                        1 if nt else 0, _vp(ft))
+    return pot, fld, pt[:nt], ft[:nt]
 
 
 # --------------------------------------------------------------------------
@@ -188,6 +247,49 @@ def ref_l3dpartdirect(source, charge=None, dipstr=None, dipvec=None, target=None
     pt = np.zeros(max(nt, 1), np.complex128)
     ft = np.zeros((max(nt, 1), 3), np.complex128)
     r.l3dpartdirect_(_i(ns), _vp(src), _i(charge is not None), _vp(q), _i(dipstr is not None),
+                     _vp(ds), _vp(dv), _i(1), _vp(pot), _i(1), _vp(fld), _i(nt), _vp(trg),
+                     _i(1 if nt else 0), _vp(pt), _i(1 if nt else 0), _vp(ft))
+    return pot, fld, pt[:nt], ft[:nt]
+
+
+def ref_hfmm3dparttarg(iprec, zk, source, charge=None, dipstr=None, dipvec=None, target=None,
+                       ifpot=1, iffld=1, ifpottarg=1, iffldtarg=1):
+    r = ref()
+    src = _c(source, np.float64)
+    ns = src.shape[0]
+    trg = _c(target, np.float64) if target is not None else np.zeros((1, 3))
+    nt = 0 if target is None else trg.shape[0]
+    q = _c(charge, np.complex128) if charge is not None else np.zeros(1, np.complex128)
+    ds = _c(dipstr, np.complex128) if dipstr is not None else np.zeros(1, np.complex128)
+    dv = _c(dipvec, np.float64) if dipvec is not None else np.zeros(3)
+    pot = np.zeros(ns, np.complex128)
+    fld = np.zeros((ns, 3), np.complex128)
+    pt = np.zeros(max(nt, 1), np.complex128)
+    ft = np.zeros((max(nt, 1), 3), np.complex128)
+    ier = ctypes.c_int(0)
+    zka, zkp = _z(zk)
+    r.hfmm3dparttarg_(ctypes.byref(ier), _i(iprec), zkp, _i(ns), _vp(src), _i(charge is not None),
+                      _vp(q), _i(dipstr is not None), _vp(ds), _vp(dv), _i(ifpot), _vp(pot),
+                      _i(iffld), _vp(fld), _i(nt), _vp(trg), _i(ifpottarg if nt else 0), _vp(pt),
+                      _i(iffldtarg if nt else 0), _vp(ft))
+    return ier.value, pot, fld, pt[:nt], ft[:nt]
+
+
+def ref_h3dpartdirect(zk, source, charge=None, dipstr=None, dipvec=None, target=None):
+    r = ref()
+    src = _c(source, np.float64)
+    ns = src.shape[0]
+    trg = _c(target, np.float64) if target is not None else np.zeros((1, 3))
+    nt = 0 if target is None else trg.shape[0]
+    q = _c(charge, np.complex128) if charge is not None else np.zeros(1, np.complex128)
+    ds = _c(dipstr, np.complex128) if dipstr is not None else np.zeros(1, np.complex128)
+    dv = _c(dipvec, np.float64) if dipvec is not None else np.zeros(3)
+    pot = np.zeros(ns, np.complex128)
+    fld = np.zeros((ns, 3), np.complex128)
+    pt = np.zeros(max(nt, 1), np.complex128)
+    ft = np.zeros((max(nt, 1), 3), np.complex128)
+    zka, zkp = _z(zk)
+    r.h3dpartdirect_(zkp, _i(ns), _vp(src), _i(charge is not None), _vp(q), _i(dipstr is not None),
                      _vp(ds), _vp(dv), _i(1), _vp(pot), _i(1), _vp(fld), _i(nt), _vp(trg),
                      _i(1 if nt else 0), _vp(pt), _i(1 if nt else 0), _vp(ft))
     return pot, fld, pt[:nt], ft[:nt]

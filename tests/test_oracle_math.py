@@ -51,7 +51,8 @@ def test_addition_theorem(rng):
     o = ob.oracle()
     p = 40
     x = np.array([0.9, -1.1, 0.7])
-    y = 0.3 * rng.normal(size=3)
+    d = rng.normal(size=3)
+    y = 0.4 * d / np.linalg.norm(d)      # |y|/|x| = 0.25: the series has converged at p = 40
     R = np.zeros((p + 1) ** 2, np.complex128)
     S = np.zeros((p + 1) ** 2, np.complex128)
     o.orc_regular(p, _vp(y), _vp(R))

@@ -55,3 +55,26 @@ def test_nsource_zero_gives_ier4():
     assert ier.value == 4
     assert ob.oracle().orc_lfmm3dparttarg(1, 0, _vp(src), 1, _vp(z), 0, None, None, 1, _vp(z), 1, _vp(z),
                                           0, None, 0, None, 0, None) == 4
+
+
+HCASES = [(3000, 1000, 1, "cube", 8.0, 1, 1), (4000, 0, 2, "sphere", 20.0, 1, 0),
+          (3000, 800, 2, "clusters", 2.0 + 0.5j, 1, 1), (2500, 500, 0, "plummer", 25.0, 0, 1),
+          (2000, 0, 3, "cube", 1.5, 1, 1), (1, 4, 1, "cube", 3.0, 1, 1)]
+
+
+@pytest.mark.parametrize("ns,nt,iprec,dist,zk,ifc,ifd", HCASES)
+def test_helm_fmm_vs_reference(rng, ns, nt, iprec, dist, zk, ifc, ifd):
+    """The Helmholtz oracle and the reference agree to a tenth of the iprec
+    tolerance (the translations differ by the reference's quadrature only)."""
+    from fmm_inputs import IPREC_EPS
+    src = points(rng, ns, dist)
+    trg = points(rng, nt, dist)
+    q, ds, dv = strengths(rng, ns, ifc, ifd, unit_dipvec=False)
+    r = ob.ref_hfmm3dparttarg(iprec, zk, src, q, ds, dv, trg)
+    o = ob.orc_hfmm3dparttarg(iprec, zk, src, q, ds, dv, trg)
+    assert r[0] == o[0] == 0
+    for k in range(1, 5 if nt else 3):
+        if np.linalg.norm(r[k]) == 0:
+            assert np.linalg.norm(o[k]) == 0
+        else:
+            assert rel(o[k], r[k]) < 0.1 * IPREC_EPS[iprec]
