@@ -31,6 +31,8 @@ EXPORTED = (
     "fmm3d_lfmm3dparttarg", "fmm3d_lfmm3dpartself", "fmm3d_l3dpartdirect",
     "fmm3d_lfmm3dparttarg_ex", "fmm3d_lfmm3dparttarg_dev", "fmm3d_last_timings",
     "fmm3d_version", "fmm3d_d3tstrcr",
+    "fmm3d_hfmm3dparttarg", "fmm3d_hfmm3dpartself", "fmm3d_h3dpartdirect",
+    "fmm3d_hfmm3dparttarg_ex", "fmm3d_hfmm3dparttarg_dev",
 )
 
 _lib = None
@@ -65,6 +67,17 @@ def load():
         lib.fmm3d_l3dpartdirect.restype = None
         lib.fmm3d_d3tstrcr.argtypes = [i, vp, i, vp, i, vp, vp, vp, vp, vp, i, vp, vp, vp, vp, i, vp]
         lib.fmm3d_d3tstrcr.restype = i
+        htarg = [i, vp] + targ[1:]
+        lib.fmm3d_hfmm3dparttarg.argtypes = htarg
+        lib.fmm3d_hfmm3dparttarg.restype = i
+        lib.fmm3d_hfmm3dparttarg_ex.argtypes = [i] + htarg
+        lib.fmm3d_hfmm3dparttarg_ex.restype = i
+        lib.fmm3d_hfmm3dparttarg_dev.argtypes = [i] + htarg + [vp]
+        lib.fmm3d_hfmm3dparttarg_dev.restype = i
+        lib.fmm3d_hfmm3dpartself.argtypes = [i, vp, i, vp, i, vp, i, vp, vp, i, vp, i, vp]
+        lib.fmm3d_hfmm3dpartself.restype = i
+        lib.fmm3d_h3dpartdirect.argtypes = [vp, i, vp, i, vp, i, vp, vp, i, vp, i, vp, i, vp, i, vp, i, vp]
+        lib.fmm3d_h3dpartdirect.restype = None
         lib.fmm3d_last_timings.argtypes = [vp, i]
         lib.fmm3d_last_timings.restype = None
         _lib = lib
@@ -145,6 +158,57 @@ def l3dpartdirect(source, charge=None, dipstr=None, dipvec=None, target=None,
     pt = np.zeros(nt, np.complex128) if (ifpottarg and nt) else None
     ft = np.zeros((nt, 3), np.complex128) if (iffldtarg and nt) else None
     lib.fmm3d_l3dpartdirect(ns, _ptr(src), int(q is not None), _ptr(q), int(ds is not None),
+                            _ptr(ds), _ptr(dv), int(bool(ifpot)), _ptr(pot), int(bool(iffld)),
+                            _ptr(fld), nt, _ptr(trg), int(pt is not None), _ptr(pt),
+                            int(ft is not None), _ptr(ft))
+    return pot, fld, pt, ft
+
+
+def _zk(zk):
+    return np.array([complex(zk).real, complex(zk).imag], dtype=np.float64)
+
+
+def hfmm3dparttarg(iprec, zk, source, charge=None, dipstr=None, dipvec=None, target=None,
+                   ifpot=1, iffld=1, ifpottarg=1, iffldtarg=1, precision=PREC_FP64):
+    """Helmholtz FMM, kernel e^{ik r}/r (hfmm3dparttarg, src/hfmm3dpart.f:141).
+    Returns (ier, pot, fld, pottarg, fldtarg); outputs whose flag is 0 are None."""
+    lib = load()
+    src, q, ds, dv, trg = _prep(source, charge, dipstr, dipvec, target)
+    ns = src.shape[0]
+    nt = 0 if trg is None else trg.shape[0]
+    pot = np.zeros(ns, np.complex128) if ifpot else None
+    fld = np.zeros((ns, 3), np.complex128) if iffld else None
+    pt = np.zeros(nt, np.complex128) if (ifpottarg and nt) else None
+    ft = np.zeros((nt, 3), np.complex128) if (iffldtarg and nt) else None
+    z = _zk(zk)
+    ier = lib.fmm3d_hfmm3dparttarg_ex(
+        int(precision), int(iprec), _ptr(z), ns, _ptr(src), int(q is not None), _ptr(q),
+        int(ds is not None), _ptr(ds), _ptr(dv), int(bool(ifpot)), _ptr(pot), int(bool(iffld)),
+        _ptr(fld), nt, _ptr(trg), int(pt is not None), _ptr(pt), int(ft is not None), _ptr(ft))
+    return ier, pot, fld, pt, ft
+
+
+def hfmm3dpartself(iprec, zk, source, charge=None, dipstr=None, dipvec=None, ifpot=1, iffld=1,
+                   precision=PREC_FP64):
+    """Helmholtz FMM at the sources only (hfmm3dpartself, src/hfmm3dpart.f:90)."""
+    ier, pot, fld, _, _ = hfmm3dparttarg(iprec, zk, source, charge, dipstr, dipvec, None, ifpot,
+                                         iffld, 0, 0, precision)
+    return ier, pot, fld
+
+
+def h3dpartdirect(zk, source, charge=None, dipstr=None, dipvec=None, target=None,
+                  ifpot=1, iffld=1, ifpottarg=1, iffldtarg=1):
+    """O(N^2) Helmholtz direct sum on the GPU (h3dpartdirect, src/hfmm3dpart.f:1268)."""
+    lib = load()
+    src, q, ds, dv, trg = _prep(source, charge, dipstr, dipvec, target)
+    ns = src.shape[0]
+    nt = 0 if trg is None else trg.shape[0]
+    pot = np.zeros(ns, np.complex128) if ifpot else None
+    fld = np.zeros((ns, 3), np.complex128) if iffld else None
+    pt = np.zeros(nt, np.complex128) if (ifpottarg and nt) else None
+    ft = np.zeros((nt, 3), np.complex128) if (iffldtarg and nt) else None
+    z = _zk(zk)
+    lib.fmm3d_h3dpartdirect(_ptr(z), ns, _ptr(src), int(q is not None), _ptr(q), int(ds is not None),
                             _ptr(ds), _ptr(dv), int(bool(ifpot)), _ptr(pot), int(bool(iffld)),
                             _ptr(fld), nt, _ptr(trg), int(pt is not None), _ptr(pt),
                             int(ft is not None), _ptr(ft))

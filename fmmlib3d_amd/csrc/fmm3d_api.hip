@@ -41,19 +41,35 @@ void download(Context& ctx, double* h, const double* d, size_t n) {
   FMM_HIP(hipMemcpyAsync(h, d, n * sizeof(double), hipMemcpyDeviceToHost, ctx.stream));
 }
 
-int lap_host(int prec_mode, int iprec, int ns, const double* source, int ifcharge,
+// kernel: nullptr = Laplace, else the Helmholtz wavenumber (re, im)
+int run_solve(Context& ctx, const double* zk, const LapCall& c, int prec_mode) {
+  return zk ? helm_solve_mode(ctx, c, prec_mode) : lap_solve_mode(ctx, c, prec_mode);
+}
+
+int check_args(const double* zk, int prec_mode, int iprec, int ns, int nt) {
+  if (nt < 0) return 128;
+  if (ns < 1) return 4;  // lfmm3dpart.f:309-330 / hfmm3dpart.f:307-330 after d3tstrcr's ier=128
+  if (iprec < -2 || iprec > 6) return 32;
+  if (zk) {
+    if (zk[0] == 0.0 && zk[1] == 0.0) return 128;  // zk = 0: the Helmholtz scaling degenerates
+    if (prec_mode != 0) return 128;
+  } else if (prec_mode < 0 || prec_mode > 3) {
+    return 128;
+  }
+  return 0;
+}
+
+int fmm_host(const double* zk, int prec_mode, int iprec, int ns, const double* source, int ifcharge,
              const double* charge, int ifdipole, const double* dipstr, const double* dipvec,
              int ifpot, double* pot, int iffld, double* fld, int nt, const double* target,
              int ifpottarg, double* pottarg, int iffldtarg, double* fldtarg) {
-  if (nt < 0) return 128;
-  if (ns < 1) return 4;  // lfmm3dpart.f:309-330 after d3tstrcr's ier=128
-  if (iprec < -2 || iprec > 6) return 32;
-  if (prec_mode < 0 || prec_mode > 3) return 128;
+  if (int e = check_args(zk, prec_mode, iprec, ns, nt)) return e;
   Context& ctx = default_context();
   std::lock_guard<std::mutex> lock(ctx.mu);
   return guarded([&] {
     LapCall c;
     c.iprec = iprec; c.ns = ns; c.nt = nt;
+    if (zk) { c.zk_re = zk[0]; c.zk_im = zk[1]; }
     c.ifcharge = ifcharge == 1; c.ifdipole = ifdipole == 1;
     c.ifpot = ifpot == 1; c.iffld = iffld == 1;
     c.ifpottarg = nt > 0 && ifpottarg == 1; c.iffldtarg = nt > 0 && iffldtarg == 1;
@@ -68,7 +84,7 @@ int lap_host(int prec_mode, int iprec, int ns, const double* source, int ifcharg
     if (c.iffld) c.fld_dev = (double*)ctx.arena.get("out_fld", 6 * sizeof(double) * ns);
     if (c.ifpottarg) c.pottarg_dev = (double*)ctx.arena.get("out_pt", 2 * sizeof(double) * nt);
     if (c.iffldtarg) c.fldtarg_dev = (double*)ctx.arena.get("out_ft", 6 * sizeof(double) * nt);
-    int ier = lap_solve_mode(ctx, c, prec_mode);
+    int ier = run_solve(ctx, zk, c, prec_mode);
     if (ier) return ier;
     if (c.ifpot) download(ctx, pot, c.pot_dev, 2 * (size_t)ns);
     if (c.iffld) download(ctx, fld, c.fld_dev, 6 * (size_t)ns);
@@ -88,7 +104,7 @@ int fmm3d_lfmm3dparttarg(int iprec, int nsource, const double* source, int ifcha
                          const double* dipvec, int ifpot, double* pot, int iffld, double* fld,
                          int ntarget, const double* target, int ifpottarg, double* pottarg,
                          int iffldtarg, double* fldtarg) {
-  return lap_host(FMM3D_PREC_FP64, iprec, nsource, source, ifcharge, charge, ifdipole, dipstr,
+  return fmm_host(nullptr, FMM3D_PREC_FP64, iprec, nsource, source, ifcharge, charge, ifdipole, dipstr,
                   dipvec, ifpot, pot, iffld, fld, ntarget, target, ifpottarg, pottarg, iffldtarg,
                   fldtarg);
 }
@@ -98,27 +114,27 @@ int fmm3d_lfmm3dparttarg_ex(int prec_mode, int iprec, int nsource, const double*
                             const double* dipvec, int ifpot, double* pot, int iffld, double* fld,
                             int ntarget, const double* target, int ifpottarg, double* pottarg,
                             int iffldtarg, double* fldtarg) {
-  return lap_host(prec_mode, iprec, nsource, source, ifcharge, charge, ifdipole, dipstr, dipvec,
+  return fmm_host(nullptr, prec_mode, iprec, nsource, source, ifcharge, charge, ifdipole, dipstr, dipvec,
                   ifpot, pot, iffld, fld, ntarget, target, ifpottarg, pottarg, iffldtarg, fldtarg);
 }
 
 int fmm3d_lfmm3dpartself(int iprec, int nsource, const double* source, int ifcharge,
                          const double* charge, int ifdipole, const double* dipstr,
                          const double* dipvec, int ifpot, double* pot, int iffld, double* fld) {
-  return lap_host(FMM3D_PREC_FP64, iprec, nsource, source, ifcharge, charge, ifdipole, dipstr,
+  return fmm_host(nullptr, FMM3D_PREC_FP64, iprec, nsource, source, ifcharge, charge, ifdipole, dipstr,
                   dipvec, ifpot, pot, iffld, fld, 0, nullptr, 0, nullptr, 0, nullptr);
 }
 
-int fmm3d_lfmm3dparttarg_dev(int prec_mode, int iprec, int ns, const double* source_dev,
-                             int ifcharge, const double* charge_dev, int ifdipole,
-                             const double* dipstr_dev, const double* dipvec_dev, int ifpot,
-                             double* pot_dev, int iffld, double* fld_dev, int nt,
-                             const double* target_dev, int ifpottarg, double* pottarg_dev,
-                             int iffldtarg, double* fldtarg_dev, void* stream) {
-  if (nt < 0) return 128;
-  if (ns < 1) return 4;
-  if (iprec < -2 || iprec > 6) return 32;
-  if (prec_mode < 0 || prec_mode > 3) return 128;
+}  // extern "C"
+
+namespace {
+
+int fmm_dev(const double* zk, int prec_mode, int iprec, int ns, const double* source_dev,
+            int ifcharge, const double* charge_dev, int ifdipole, const double* dipstr_dev,
+            const double* dipvec_dev, int ifpot, double* pot_dev, int iffld, double* fld_dev, int nt,
+            const double* target_dev, int ifpottarg, double* pottarg_dev, int iffldtarg,
+            double* fldtarg_dev, void* stream) {
+  if (int e = check_args(zk, prec_mode, iprec, ns, nt)) return e;
   Context& ctx = default_context();
   std::lock_guard<std::mutex> lock(ctx.mu);
   hipStream_t saved = ctx.stream;
@@ -139,22 +155,83 @@ int fmm3d_lfmm3dparttarg_dev(int prec_mode, int iprec, int ns, const double* sou
     c.src_dev = source_dev; c.q_dev = charge_dev; c.ds_dev = dipstr_dev; c.dv_dev = dipvec_dev;
     c.trg_dev = target_dev;
     c.pot_dev = pot_dev; c.fld_dev = fld_dev; c.pottarg_dev = pottarg_dev; c.fldtarg_dev = fldtarg_dev;
-    return lap_solve_mode(ctx, c, prec_mode);
+    if (zk) { c.zk_re = zk[0]; c.zk_im = zk[1]; }
+    return run_solve(ctx, zk, c, prec_mode);
   });
   ctx.stream = saved;
   return ier;
 }
 
-void fmm3d_l3dpartdirect(int nsource, const double* source, int ifcharge, const double* charge,
-                         int ifdipole, const double* dipstr, const double* dipvec, int ifpot,
-                         double* pot, int iffld, double* fld, int ntarget, const double* target,
-                         int ifpottarg, double* pottarg, int iffldtarg, double* fldtarg) {
+}  // namespace
+
+extern "C" {
+
+int fmm3d_lfmm3dparttarg_dev(int prec_mode, int iprec, int ns, const double* source_dev,
+                             int ifcharge, const double* charge_dev, int ifdipole,
+                             const double* dipstr_dev, const double* dipvec_dev, int ifpot,
+                             double* pot_dev, int iffld, double* fld_dev, int nt,
+                             const double* target_dev, int ifpottarg, double* pottarg_dev,
+                             int iffldtarg, double* fldtarg_dev, void* stream) {
+  return fmm_dev(nullptr, prec_mode, iprec, ns, source_dev, ifcharge, charge_dev, ifdipole, dipstr_dev,
+                 dipvec_dev, ifpot, pot_dev, iffld, fld_dev, nt, target_dev, ifpottarg, pottarg_dev,
+                 iffldtarg, fldtarg_dev, stream);
+}
+
+int fmm3d_hfmm3dparttarg_dev(int prec_mode, int iprec, const double* zk, int ns, const double* source_dev,
+                             int ifcharge, const double* charge_dev, int ifdipole,
+                             const double* dipstr_dev, const double* dipvec_dev, int ifpot,
+                             double* pot_dev, int iffld, double* fld_dev, int nt,
+                             const double* target_dev, int ifpottarg, double* pottarg_dev,
+                             int iffldtarg, double* fldtarg_dev, void* stream) {
+  if (!zk) return 128;
+  return fmm_dev(zk, prec_mode, iprec, ns, source_dev, ifcharge, charge_dev, ifdipole, dipstr_dev,
+                 dipvec_dev, ifpot, pot_dev, iffld, fld_dev, nt, target_dev, ifpottarg, pottarg_dev,
+                 iffldtarg, fldtarg_dev, stream);
+}
+
+int fmm3d_hfmm3dparttarg(int iprec, const double* zk, int nsource, const double* source, int ifcharge,
+                         const double* charge, int ifdipole, const double* dipstr,
+                         const double* dipvec, int ifpot, double* pot, int iffld, double* fld,
+                         int ntarget, const double* target, int ifpottarg, double* pottarg,
+                         int iffldtarg, double* fldtarg) {
+  if (!zk) return 128;
+  return fmm_host(zk, FMM3D_PREC_FP64, iprec, nsource, source, ifcharge, charge, ifdipole, dipstr, dipvec,
+                  ifpot, pot, iffld, fld, ntarget, target, ifpottarg, pottarg, iffldtarg, fldtarg);
+}
+
+int fmm3d_hfmm3dparttarg_ex(int prec_mode, int iprec, const double* zk, int nsource, const double* source,
+                            int ifcharge, const double* charge, int ifdipole, const double* dipstr,
+                            const double* dipvec, int ifpot, double* pot, int iffld, double* fld,
+                            int ntarget, const double* target, int ifpottarg, double* pottarg,
+                            int iffldtarg, double* fldtarg) {
+  if (!zk) return 128;
+  return fmm_host(zk, prec_mode, iprec, nsource, source, ifcharge, charge, ifdipole, dipstr, dipvec,
+                  ifpot, pot, iffld, fld, ntarget, target, ifpottarg, pottarg, iffldtarg, fldtarg);
+}
+
+int fmm3d_hfmm3dpartself(int iprec, const double* zk, int nsource, const double* source, int ifcharge,
+                         const double* charge, int ifdipole, const double* dipstr,
+                         const double* dipvec, int ifpot, double* pot, int iffld, double* fld) {
+  if (!zk) return 128;
+  return fmm_host(zk, FMM3D_PREC_FP64, iprec, nsource, source, ifcharge, charge, ifdipole, dipstr, dipvec,
+                  ifpot, pot, iffld, fld, 0, nullptr, 0, nullptr, 0, nullptr);
+}
+
+}  // extern "C"
+
+namespace {
+
+void direct_host(const double* zk, int nsource, const double* source, int ifcharge, const double* charge,
+                 int ifdipole, const double* dipstr, const double* dipvec, int ifpot, double* pot,
+                 int iffld, double* fld, int ntarget, const double* target, int ifpottarg,
+                 double* pottarg, int iffldtarg, double* fldtarg) {
   if (nsource < 0 || ntarget < 0) return;
   Context& ctx = default_context();
   std::lock_guard<std::mutex> lock(ctx.mu);
   (void)guarded([&] {
     LapCall c;
     c.ns = nsource; c.nt = ntarget;
+    if (zk) { c.zk_re = zk[0]; c.zk_im = zk[1]; }
     c.ifcharge = ifcharge == 1; c.ifdipole = ifdipole == 1;
     c.ifpot = ifpot == 1; c.iffld = iffld == 1;
     c.ifpottarg = ifpottarg == 1; c.iffldtarg = iffldtarg == 1;
@@ -167,7 +244,7 @@ void fmm3d_l3dpartdirect(int nsource, const double* source, int ifcharge, const 
     if (c.iffld) c.fld_dev = (double*)ctx.arena.get("out_fld", 6 * sizeof(double) * std::max(nsource, 1));
     if (c.ifpottarg) c.pottarg_dev = (double*)ctx.arena.get("out_pt", 2 * sizeof(double) * std::max(ntarget, 1));
     if (c.iffldtarg) c.fldtarg_dev = (double*)ctx.arena.get("out_ft", 6 * sizeof(double) * std::max(ntarget, 1));
-    int ier = lap_direct(ctx, c);
+    int ier = zk ? helm_direct(ctx, c) : lap_direct(ctx, c);
     if (ier) return ier;
     if (c.ifpot) download(ctx, pot, c.pot_dev, 2 * (size_t)nsource);
     if (c.iffld) download(ctx, fld, c.fld_dev, 6 * (size_t)nsource);
@@ -176,6 +253,27 @@ void fmm3d_l3dpartdirect(int nsource, const double* source, int ifcharge, const 
     FMM_HIP(hipStreamSynchronize(ctx.stream));
     return 0;
   });
+}
+
+}  // namespace
+
+extern "C" {
+
+void fmm3d_l3dpartdirect(int nsource, const double* source, int ifcharge, const double* charge,
+                         int ifdipole, const double* dipstr, const double* dipvec, int ifpot,
+                         double* pot, int iffld, double* fld, int ntarget, const double* target,
+                         int ifpottarg, double* pottarg, int iffldtarg, double* fldtarg) {
+  direct_host(nullptr, nsource, source, ifcharge, charge, ifdipole, dipstr, dipvec, ifpot, pot, iffld,
+              fld, ntarget, target, ifpottarg, pottarg, iffldtarg, fldtarg);
+}
+
+void fmm3d_h3dpartdirect(const double* zk, int nsource, const double* source, int ifcharge,
+                         const double* charge, int ifdipole, const double* dipstr, const double* dipvec,
+                         int ifpot, double* pot, int iffld, double* fld, int ntarget, const double* target,
+                         int ifpottarg, double* pottarg, int iffldtarg, double* fldtarg) {
+  if (!zk) return;
+  direct_host(zk, nsource, source, ifcharge, charge, ifdipole, dipstr, dipvec, ifpot, pot, iffld,
+              fld, ntarget, target, ifpottarg, pottarg, iffldtarg, fldtarg);
 }
 
 void fmm3d_last_timings(double* ms, int n) {

@@ -1,0 +1,311 @@
+// Helmholtz particle FMM driver: host octree (the same d3tstrcr tree as the
+// Laplace path) -> per-level orders and translation tables -> one upload ->
+// the eight steps of hfmm3dparttargmain (src/hfmm3dpart.f:545-1086) and
+// hfmm3d_list2 (src/hfmm3drouts.f:153-564) as HIP kernels on one stream ->
+// scatter back to the caller's order.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <vector>
+
+#include "fmm3d_internal.hpp"
+#include "helm_kernels.hip"
+#include "helm_tables.hpp"
+#include "octree.hpp"
+#include "plan.hpp"
+#include "runtime.hpp"
+
+namespace fmm3d {
+
+namespace {
+
+// Host tables and their device copies, cached for repeated calls with the
+// same wavenumber, root box, depth and tolerance.
+struct HelmCache {
+  HelmHostTables H;
+  std::vector<int> I;
+  std::vector<double> D;
+  std::vector<long> Lg;
+  size_t o_offrot = 0, o_offdist = 0, o_octrot = 0, o_plev = 0, o_ptrunc = 0;
+  size_t o_fac = 0, o_offph = 0, o_octph = 0, o_rot = 0, o_sclev = 0, o_z = 0;
+  size_t o_rotoff = 0, o_coax = 0;
+  int m2m_base = 0, l2l_base = 0;
+};
+
+HelmCache* helm_tables(Context& ctx, cplx zk, double size, int nlev, double eps, int& ier) {
+  ier = 0;
+  auto* c = static_cast<HelmCache*>(ctx.helm_tables.get());
+  const bool hit = c && c->H.zk == zk && c->H.size == size && c->H.nlev == nlev && c->H.eps == eps;
+  if (!hit) {
+    auto nc = std::make_shared<HelmCache>();
+    if (!build_helm_tables(zk, size, nlev, eps, nc->H)) {
+      ier = 16;
+      return nullptr;
+    }
+    const HelmHostTables& H = nc->H;
+    auto& I = nc->I;
+    auto& D = nc->D;
+    auto& Lg = nc->Lg;
+    auto put_i = [&](const std::vector<int>& v) { size_t o = I.size(); I.insert(I.end(), v.begin(), v.end()); while (I.size() % 16) I.push_back(0); return o; };
+    auto put_d = [&](const std::vector<double>& v) { size_t o = D.size(); D.insert(D.end(), v.begin(), v.end()); while (D.size() % 8) D.push_back(0.0); return o; };
+    nc->o_offrot = put_i(H.geo.off_rot);
+    nc->o_offdist = put_i(H.geo.off_dist);
+    nc->o_octrot = put_i(H.geo.oct_rot);
+    nc->o_plev = put_i(H.plev);
+    nc->o_ptrunc = put_i(H.ptrunc);
+    nc->o_fac = put_d(H.geo.fac);
+    nc->o_offph = put_d(H.geo.off_phase);
+    nc->o_octph = put_d(H.geo.oct_phase);
+    nc->o_rot = put_d(H.geo.rot);
+    nc->o_sclev = put_d(H.sclev);
+    nc->o_z = put_d(H.z);
+    nc->o_rotoff = 0;
+    Lg = H.geo.rot_off;
+    nc->o_coax = Lg.size();
+    const int nd = H.geo.ndist;
+    auto put_desc = [&](const CoaxDesc& d) { Lg.push_back(d.base); Lg.push_back(d.pin); Lg.push_back(d.pout); };
+    for (const auto& d : H.m2l) put_desc(d);
+    nc->m2m_base = (nlev + 1) * nd;
+    for (const auto& d : H.m2m) put_desc(d);
+    nc->l2l_base = nc->m2m_base + nlev + 1;
+    for (const auto& d : H.l2l) put_desc(d);
+    void* di = ctx.arena.get("htab_i", I.size() * sizeof(int));
+    void* dd = ctx.arena.get("htab_d", D.size() * sizeof(double));
+    void* dl = ctx.arena.get("htab_l", Lg.size() * sizeof(long));
+    FMM_HIP(hipMemcpyAsync(di, I.data(), I.size() * sizeof(int), hipMemcpyHostToDevice, ctx.stream));
+    FMM_HIP(hipMemcpyAsync(dd, D.data(), D.size() * sizeof(double), hipMemcpyHostToDevice, ctx.stream));
+    FMM_HIP(hipMemcpyAsync(dl, Lg.data(), Lg.size() * sizeof(long), hipMemcpyHostToDevice, ctx.stream));
+    FMM_HIP(hipStreamSynchronize(ctx.stream));
+    ctx.helm_tables = nc;
+    c = nc.get();
+  }
+  return c;
+}
+
+template <bool CHG, bool DIP, bool FLD>
+void hp2p(const DevPlan& P, const int* items, int n, int kind, double kre, double kim, const double* spos,
+          const double* str, const double* tpos, double* pot, double* fo, hipStream_t s) {
+  DevPlan Q = P;
+  Q.p2p_items = items;
+  Q.n_p2p = n;
+  hipLaunchKernelGGL((k_hp2p<CHG, DIP, FLD>), dim3(n), dim3(kP2PBlock), 0, s, Q, kre, kim, spos, str, tpos, kind, pot, fo);
+}
+
+void launch_hp2p(const DevPlan& P, const int* items, int n, int kind, double kre, double kim,
+                 const double* spos, const double* str, const double* tpos, int ifcharge, int ifdipole,
+                 bool fld, double* pot, double* fo, hipStream_t s) {
+  if (n == 0) return;
+#define HCASE(C, Dp, F) \
+  if (ifcharge == C && ifdipole == Dp && fld == F) return hp2p<C, Dp, F>(P, items, n, kind, kre, kim, spos, str, tpos, pot, fo, s);
+  HCASE(1, 0, false) HCASE(1, 0, true) HCASE(0, 1, false) HCASE(0, 1, true) HCASE(1, 1, false) HCASE(1, 1, true)
+#undef HCASE
+}
+
+}  // namespace
+
+int helm_solve(Context& ctx, const LapCall& c) {
+  auto t0 = std::chrono::steady_clock::now();
+  double eps;
+  int nbox;
+  if (!iprec_params(c.iprec, c.ns, c.nt, eps, nbox)) return 32;
+  const int ns = c.ns, nt = c.nt;
+  const cplx zk(c.zk_re, c.zk_im);
+  const Flags f{c.ifcharge ? 1 : 0, c.ifdipole ? 1 : 0, c.ifpot ? 1 : 0, c.iffld ? 1 : 0,
+                nt > 0 && c.ifpottarg ? 1 : 0, nt > 0 && c.iffldtarg ? 1 : 0};
+  thread_local Octree tree;  // storage reused across calls
+  if (build_octree(c.src_host, ns, c.trg_host, nt, nbox, tree) != 0) return 4;
+  int ier = 0;
+  HelmCache* HC = helm_tables(ctx, zk, tree.size, tree.nlev, eps, ier);
+  if (!HC) return ier;
+  const HelmHostTables& H = HC->H;
+  const int pmax = H.pmax, K = (pmax + 1) * (pmax + 1);
+  // LDS budgets of the kernels at this order
+  const size_t ldsmax = max_lds_bytes(ctx);
+  const int Hh = (pmax + 2) * (pmax + 3) / 2, Fn = pmax + 2;
+  const size_t per_src = 2 * (size_t)Hh + 2 * (size_t)Fn + 12;
+  if (sizeof(double) * (4 * (size_t)K + per_src) > ldsmax) return 16;
+  int ts = (int)((ldsmax / sizeof(double) - 4 * (size_t)K) / per_src);
+  ts = std::max(1, std::min(32, ts));
+  const size_t lds_form = sizeof(double) * (4 * (size_t)K + (size_t)ts * per_src);
+  const size_t lds_eval = sizeof(double) * (2 * (size_t)K + 2 * 64 * (size_t)Fn);
+  const size_t lds_ps = sizeof(double) * 4 * (size_t)K;
+  const bool m2l_lds = sizeof(double) * 6 * (size_t)K <= ldsmax;
+  if (lds_eval > ldsmax || lds_ps > ldsmax) return 16;
+  Flat F;
+  PlanOffsets O;
+  build_plan(tree, f, F, O, 2);
+  auto t1 = std::chrono::steady_clock::now();
+  hipStream_t s = ctx.stream;
+  Timer tm;
+  tm.start(s, 11);
+  int* dI = (int*)ctx.arena.get("plan_i", F.I.size() * sizeof(int));
+  double* dD = (double*)ctx.arena.get("plan_d", F.D.size() * sizeof(double));
+  FMM_HIP(hipMemcpyAsync(dI, F.I.data(), F.I.size() * sizeof(int), hipMemcpyHostToDevice, s));
+  FMM_HIP(hipMemcpyAsync(dD, F.D.data(), F.D.size() * sizeof(double), hipMemcpyHostToDevice, s));
+  DevPlan P = make_devplan(tree, pmax, O, dI, dD);
+  const int* ti = (const int*)ctx.arena.get("htab_i", HC->I.size() * sizeof(int));
+  const double* td = (const double*)ctx.arena.get("htab_d", HC->D.size() * sizeof(double));
+  const long* tl = (const long*)ctx.arena.get("htab_l", HC->Lg.size() * sizeof(long));
+  DevTables Tb{};
+  Tb.p = pmax;
+  Tb.off_rot = ti + HC->o_offrot; Tb.off_dist = ti + HC->o_offdist; Tb.oct_rot = ti + HC->o_octrot;
+  Tb.fac = td + HC->o_fac; Tb.off_phase = td + HC->o_offph; Tb.oct_phase = td + HC->o_octph;
+  Tb.rot = td + HC->o_rot; Tb.rot_off = tl + HC->o_rotoff;
+  HelmDev Hd;
+  Hd.kre = zk.real(); Hd.kim = zk.imag();
+  Hd.nd = H.geo.ndist;
+  Hd.plev = ti + HC->o_plev; Hd.sclev = td + HC->o_sclev; Hd.ptrunc = ti + HC->o_ptrunc;
+  Hd.coax = tl + HC->o_coax; Hd.z = td + HC->o_z;
+  const size_t nb = tree.nboxes;
+  double* spos = (double*)ctx.arena.get("spos", sizeof(double) * 4 * (size_t)ns);
+  double* str64 = (double*)ctx.arena.get("str64", sizeof(double) * 8 * (size_t)ns);
+  double* tpos = (double*)ctx.arena.get("tpos", sizeof(double) * 4 * (size_t)std::max(nt, 1));
+  double* mp = (double*)ctx.arena.get("mp", sizeof(double) * 2 * K * nb, 16);
+  double* lo = (double*)ctx.arena.get("lo", sizeof(double) * 2 * K * nb, 16);
+  double* pot_s = (double*)ctx.arena.get("pot_s", sizeof(double) * 2 * (size_t)ns);
+  double* fld_s = (double*)ctx.arena.get("fld_s", sizeof(double) * 6 * (size_t)ns);
+  double* pot_t = (double*)ctx.arena.get("pot_t", sizeof(double) * 2 * (size_t)std::max(nt, 1));
+  double* fld_t = (double*)ctx.arena.get("fld_t", sizeof(double) * 6 * (size_t)std::max(nt, 1));
+  const int* sperm = dI + O.sperm;
+  const int* tperm = dI + O.tperm;
+  hipLaunchKernelGGL(k_gather_sources, dim3(blocks(ns, 256)), dim3(256), 0, s, ns, sperm, c.src_dev,
+                     c.q_dev, c.ds_dev, c.dv_dev, f.ifcharge, f.ifdipole, spos, str64, (float*)nullptr);
+  if (nt > 0)
+    hipLaunchKernelGGL(k_gather_targets, dim3(blocks(nt, 256)), dim3(256), 0, s, nt, tperm, c.trg_dev, tpos);
+  auto zero = [&](double* a, size_t n) {
+    hipLaunchKernelGGL((k_fill_zero<double>), dim3(std::min<size_t>(blocks(n, 256), 4096)), dim3(256), 0, s, n, a);
+  };
+  zero(mp, 2 * (size_t)K * nb);
+  zero(lo, 2 * (size_t)K * nb);
+  zero(pot_s, 2 * (size_t)ns);
+  zero(fld_s, 6 * (size_t)ns);
+  if (nt > 0) {
+    zero(pot_t, 2 * (size_t)nt);
+    zero(fld_t, 6 * (size_t)nt);
+  }
+  tm.mark(1);
+  const double* fac = Tb.fac;
+  // STEP 1: P2M (hfmm3dpart.f:681-757)
+  if (P.n_p2m)
+    hipLaunchKernelGGL((k_hform<false>), dim3(P.n_p2m), dim3(256), lds_form, s, P, Hd, fac, spos, str64,
+                       f.ifcharge, f.ifdipole, ts, mp);
+  tm.mark(2);
+  // STEP 2: P2L (hfmm3dpart.f:765-830)
+  if (P.n_p2l)
+    hipLaunchKernelGGL((k_hform<true>), dim3(P.n_p2l), dim3(256), lds_form, s, P, Hd, fac, spos, str64,
+                       f.ifcharge, f.ifdipole, ts, lo);
+  tm.mark(3);
+  // STEP 3: M2M, bottom-up
+  for (size_t g = 0; g < O.m2m_groups.size(); g++)
+    hipLaunchKernelGGL(k_hm2m, dim3(O.m2m_groups[g].second), dim3(256), lds_ps, s, P, Tb, Hd,
+                       O.m2m_groups[g].first, HC->m2m_base + O.m2m_group_level[g], mp);
+  tm.mark(4);
+  // STEP 4: M2L
+  if (P.n_m2l) {
+    if (m2l_lds)
+      hipLaunchKernelGGL((k_hm2l<true>), dim3(P.n_m2l), dim3(256), sizeof(double) * 6 * (size_t)K, s, P, Tb, Hd, mp, lo);
+    else
+      hipLaunchKernelGGL((k_hm2l<false>), dim3(P.n_m2l), dim3(256), lds_ps, s, P, Tb, Hd, mp, lo);
+  }
+  tm.mark(5);
+  // STEP 5: L2L, top-down
+  for (size_t g = 0; g < O.l2l_groups.size(); g++)
+    hipLaunchKernelGGL(k_hl2l, dim3(O.l2l_groups[g].second), dim3(256), lds_ps, s, P, Tb, Hd,
+                       O.l2l_groups[g].first, HC->l2l_base + O.l2l_group_level[g], lo);
+  tm.mark(6);
+  // STEP 7: L2P
+  if (P.n_l2p)
+    hipLaunchKernelGGL((k_heval<false>), dim3(P.n_l2p), dim3(64), lds_eval, s, P, Hd, fac, lo, spos, tpos,
+                       f.ifpot, f.iffld, f.ifpottarg, f.iffldtarg, pot_s, fld_s, pot_t, fld_t);
+  tm.mark(7);
+  // STEP 6: M2P (list 3)
+  if (P.n_m2p)
+    hipLaunchKernelGGL((k_heval<true>), dim3(P.n_m2p), dim3(64), lds_eval, s, P, Hd, fac, mp, spos, tpos,
+                       f.ifpot, f.iffld, f.ifpottarg, f.iffldtarg, pot_s, fld_s, pot_t, fld_t);
+  tm.mark(8);
+  // STEP 8: P2P
+  if (f.ifcharge || f.ifdipole) {
+    launch_hp2p(P, dI + O.p2p0, O.n_p2p0, 0, Hd.kre, Hd.kim, spos, str64, tpos, f.ifcharge, f.ifdipole,
+                f.iffld != 0, pot_s, fld_s, s);
+    launch_hp2p(P, dI + O.p2p1, O.n_p2p1, 1, Hd.kre, Hd.kim, spos, str64, tpos, f.ifcharge, f.ifdipole,
+                f.iffldtarg != 0, pot_t, fld_t, s);
+  }
+  tm.mark(9);
+  if (f.ifpot) hipLaunchKernelGGL(k_scatter, dim3(blocks(ns, 256)), dim3(256), 0, s, ns, 1, sperm, pot_s, c.pot_dev);
+  if (f.iffld) hipLaunchKernelGGL(k_scatter, dim3(blocks(ns, 256)), dim3(256), 0, s, ns, 3, sperm, fld_s, c.fld_dev);
+  if (f.ifpottarg) hipLaunchKernelGGL(k_scatter, dim3(blocks(nt, 256)), dim3(256), 0, s, nt, 1, tperm, pot_t, c.pottarg_dev);
+  if (f.iffldtarg) hipLaunchKernelGGL(k_scatter, dim3(blocks(nt, 256)), dim3(256), 0, s, nt, 3, tperm, fld_t, c.fldtarg_dev);
+  tm.mark(10);
+  FMM_HIP(hipGetLastError());
+  FMM_HIP(hipStreamSynchronize(s));
+  auto& lt = last_timings();
+  lt.assign(11, 0.0);
+  lt[0] = std::chrono::duration<double, std::milli>(t1 - t0).count();
+  auto el = tm.elapsed();
+  for (int i = 0; i < 9 && i < (int)el.size(); i++) lt[1 + i] = el[i];
+  double tot = 0;
+  for (auto v : el) tot += v;
+  lt[10] = tot + lt[0];
+  return 0;
+}
+
+int helm_solve_mode(Context& ctx, const LapCall& c, int mode) {
+  if (mode != 0) return 128;
+  return helm_solve(ctx, c);
+}
+
+// h3dpartdirect on the device: identity order, one pass.
+int helm_direct(Context& ctx, const LapCall& c) {
+  const int ns = c.ns, nt = c.nt;
+  hipStream_t s = ctx.stream;
+  const size_t nmax = (size_t)std::max(std::max(ns, nt), 1);
+  std::vector<int> id(nmax);
+  for (size_t i = 0; i < id.size(); i++) id[i] = (int)i;
+  int* dperm = (int*)ctx.arena.get("plan_i", id.size() * sizeof(int));
+  FMM_HIP(hipMemcpyAsync(dperm, id.data(), id.size() * sizeof(int), hipMemcpyHostToDevice, s));
+  double* spos = (double*)ctx.arena.get("spos", sizeof(double) * 4 * (size_t)std::max(ns, 1));
+  double* str64 = (double*)ctx.arena.get("str64", sizeof(double) * 8 * (size_t)std::max(ns, 1));
+  double* tpos = (double*)ctx.arena.get("tpos", sizeof(double) * 4 * (size_t)std::max(nt, 1));
+  double* pot = (double*)ctx.arena.get("pot_s", sizeof(double) * 2 * nmax);
+  double* fld = (double*)ctx.arena.get("fld_s", sizeof(double) * 6 * nmax);
+  if (ns > 0)
+    hipLaunchKernelGGL(k_gather_sources, dim3(blocks(ns, 256)), dim3(256), 0, s, ns, dperm, c.src_dev,
+                       c.q_dev, c.ds_dev, c.dv_dev, c.ifcharge ? 1 : 0, c.ifdipole ? 1 : 0, spos, str64,
+                       (float*)nullptr);
+  auto run = [&](int npts, const double* pts, int are_src) {
+    if (npts <= 0) return;
+    if (ns == 0 || (!c.ifcharge && !c.ifdipole)) {
+      hipLaunchKernelGGL((k_fill_zero<double>), dim3(blocks(2 * (size_t)npts, 256)), dim3(256), 0, s, 2 * (size_t)npts, pot);
+      hipLaunchKernelGGL((k_fill_zero<double>), dim3(blocks(6 * (size_t)npts, 256)), dim3(256), 0, s, 6 * (size_t)npts, fld);
+      return;
+    }
+#define HDCASE(C, D)                                                                                       \
+  if ((c.ifcharge ? 1 : 0) == C && (c.ifdipole ? 1 : 0) == D)                                              \
+    hipLaunchKernelGGL((k_hdirect<C, D>), dim3(blocks(npts, 256)), dim3(256), 0, s, c.zk_re, c.zk_im, ns, \
+                       spos, str64, npts, pts, are_src, pot, fld);
+    HDCASE(1, 0) HDCASE(0, 1) HDCASE(1, 1)
+#undef HDCASE
+  };
+  auto copy_out = [&](int npts, int wpot, double* po, int wfld, double* fo) {
+    if (wpot) FMM_HIP(hipMemcpyAsync(po, pot, sizeof(double) * 2 * npts, hipMemcpyDeviceToDevice, s));
+    if (wfld) FMM_HIP(hipMemcpyAsync(fo, fld, sizeof(double) * 6 * npts, hipMemcpyDeviceToDevice, s));
+  };
+  if (ns > 0 && (c.ifpot || c.iffld)) {
+    run(ns, nullptr, 1);
+    copy_out(ns, c.ifpot, c.pot_dev, c.iffld, c.fld_dev);
+  }
+  if (nt > 0 && (c.ifpottarg || c.iffldtarg)) {
+    hipLaunchKernelGGL(k_gather_targets, dim3(blocks(nt, 256)), dim3(256), 0, s, nt, dperm, c.trg_dev, tpos);
+    run(nt, tpos, 0);
+    copy_out(nt, c.ifpottarg, c.pottarg_dev, c.iffldtarg, c.fldtarg_dev);
+  }
+  FMM_HIP(hipGetLastError());
+  FMM_HIP(hipStreamSynchronize(s));
+  return 0;
+}
+
+}  // namespace fmm3d

@@ -14,6 +14,7 @@ struct LapCall {
   const double *src_dev = nullptr, *q_dev = nullptr, *ds_dev = nullptr, *dv_dev = nullptr,
                *trg_dev = nullptr;
   double *pot_dev = nullptr, *fld_dev = nullptr, *pottarg_dev = nullptr, *fldtarg_dev = nullptr;
+  double zk_re = 0.0, zk_im = 0.0;  // Helmholtz wavenumber (unused for Laplace)
 };
 
 // T: far-field arithmetic and expansion storage; TN: near-field arithmetic
@@ -23,5 +24,9 @@ int lap_solve(Context& ctx, const LapCall& c);
 // precision mode bitmask (FMM3D_PREC_* in include/fmm3d_mi355x.h)
 int lap_solve_mode(Context& ctx, const LapCall& c, int mode);
 int lap_direct(Context& ctx, const LapCall& c);
+
+// Helmholtz (hfmm3dparttarg / h3dpartdirect); mode 0 = FP64, 1 = FP32 near field
+int helm_solve_mode(Context& ctx, const LapCall& c, int mode);
+int helm_direct(Context& ctx, const LapCall& c);
 
 }  // namespace fmm3d

@@ -13,153 +13,13 @@
 #include "fmm3d_internal.hpp"
 #include "lap_kernels.hip"
 #include "octree.hpp"
+#include "plan.hpp"
 #include "runtime.hpp"
 #include "tables.hpp"
 
 namespace fmm3d {
 
 namespace {
-
-constexpr int kP2PBlock = 256;
-
-// iprec -> tolerance and leaf size (src/lfmm3dpart.f:277-300)
-bool iprec_params(int iprec, int ns, int nt, double& eps, int& nbox) {
-  static const double e[] = {.5e0, .5e-1, .5e-2, .5e-3, .5e-6, .5e-9, .5e-12, .5e-15, 0.0};
-  static const int nb[] = {40, 50, 80, 160, 400, 800, 1200, 1400, 0};
-  if (iprec < -2 || iprec > 6) return false;
-  eps = e[iprec + 2];
-  nbox = (iprec == 6) ? ns + nt : nb[iprec + 2];
-  return true;
-}
-
-struct Flat {
-  std::vector<int> I;
-  std::vector<double> D;
-  size_t i(const std::vector<int>& v) {
-    size_t o = I.size();
-    I.insert(I.end(), v.begin(), v.end());
-    while (I.size() % 16) I.push_back(0);
-    return o;
-  }
-  size_t d(const std::vector<double>& v) {
-    size_t o = D.size();
-    D.insert(D.end(), v.begin(), v.end());
-    while (D.size() % 8) D.push_back(0.0);
-    return o;
-  }
-};
-
-struct PlanOffsets {
-  size_t level, isrc, nsrc, itrg, ntrg, nchild, child, ijk, l1o, l1i, l2o, l2i, l3o, l3i, l4o, l4i;
-  size_t p2m, l2p, m2p, p2l, m2l, m2m, l2l, p2p0, p2p1, sperm, tperm;
-  size_t center, inv_side;
-  int n_p2m, n_l2p, n_m2p, n_p2l, n_m2l, n_p2p0, n_p2p1;
-  std::vector<std::pair<int, int>> m2m_groups, l2l_groups;  // (first, count) in launch order
-};
-
-struct Flags {
-  int ifcharge, ifdipole, ifpot, iffld, ifpottarg, iffldtarg;
-};
-
-void build_plan(const Octree& t, const Flags& f, Flat& F, PlanOffsets& O) {
-  const int nb = t.nboxes;
-  const bool want_src = f.ifpot || f.iffld, want_trg = f.ifpottarg || f.iffldtarg;
-  O.level = F.i(t.level); O.isrc = F.i(t.isrc); O.nsrc = F.i(t.nsrc);
-  O.itrg = F.i(t.itrg); O.ntrg = F.i(t.ntrg); O.nchild = F.i(t.nchild);
-  O.child = F.i(t.child); O.ijk = F.i(t.ijk);
-  O.l1o = F.i(t.list[0].off); O.l1i = F.i(t.list[0].idx);
-  // list 2 pruned as lfmm3drouts.f:385-387: sourceless sources, and
-  // targetless targets when no source outputs are wanted
-  {
-    std::vector<int> off(nb + 1, 0), idx;
-    for (int b = 0; b < nb; b++) {
-      off[b] = (int)idx.size();
-      bool skip_b = t.level[b] < 2 || (!want_src && t.ntrg[b] == 0);
-      if (!skip_b)
-        for (int k = t.list[1].off[b]; k < t.list[1].off[b + 1]; k++) {
-          int j = t.list[1].idx[k];
-          if (t.nsrc[j] > 0) idx.push_back(j);
-        }
-    }
-    off[nb] = (int)idx.size();
-    O.l2o = F.i(off); O.l2i = F.i(idx);
-    std::vector<int> m2l;
-    for (int b = 0; b < nb; b++)
-      if (off[b + 1] > off[b]) m2l.push_back(b);
-    O.m2l = F.i(m2l); O.n_m2l = (int)m2l.size();
-  }
-  O.l3o = F.i(t.list[2].off); O.l3i = F.i(t.list[2].idx);
-  O.l4o = F.i(t.list[3].off); O.l4i = F.i(t.list[3].idx);
-  std::vector<int> p2m, l2p, m2p, p2l, p2p0, p2p1;
-  for (int b = 0; b < nb; b++) {
-    const bool leaf = t.nchild[b] == 0;
-    const bool has_pts = (want_src && t.nsrc[b] > 0) || (want_trg && t.ntrg[b] > 0);
-    if (leaf && t.nsrc[b] > 0) p2m.push_back(b);
-    if (leaf && t.level[b] >= 2 && has_pts) l2p.push_back(b);
-    if (t.list[2].count(b) > 0 && has_pts) m2p.push_back(b);
-    if (t.list[3].count(b) > 0) {
-      bool any = false;
-      for (int k = t.list[3].off[b]; k < t.list[3].off[b + 1]; k++) any |= t.nsrc[t.list[3].idx[k]] > 0;
-      if (any) p2l.push_back(b);
-    }
-    if (leaf && want_src)
-      for (int s = 0; s < t.nsrc[b]; s += kP2PBlock) { p2p0.push_back(b); p2p0.push_back(0); p2p0.push_back(s); }
-    if (leaf && want_trg)
-      for (int s = 0; s < t.ntrg[b]; s += kP2PBlock) { p2p1.push_back(b); p2p1.push_back(1); p2p1.push_back(s); }
-  }
-  O.p2m = F.i(p2m); O.n_p2m = (int)p2m.size();
-  O.l2p = F.i(l2p); O.n_l2p = (int)l2p.size();
-  O.m2p = F.i(m2p); O.n_m2p = (int)m2p.size();
-  O.p2l = F.i(p2l); O.n_p2l = (int)p2l.size();
-  O.p2p0 = F.i(p2p0); O.n_p2p0 = (int)p2p0.size() / 3;
-  O.p2p1 = F.i(p2p1); O.n_p2p1 = (int)p2p1.size() / 3;
-  // M2M parents, levels nlev-1 .. 2 (lfmm3drouts.f:195-258); L2L parents,
-  // levels 2 .. nlev-1 (lfmm3drouts.f:498-558)
-  std::vector<int> m2m, l2l;
-  O.m2m_groups.clear();
-  O.l2l_groups.clear();
-  for (int l = t.nlev - 1; l >= 2; l--) {
-    int first = (int)m2m.size();
-    for (int b = t.ladd[l]; b < t.ladd[l + 1]; b++)
-      if (t.nchild[b] > 0 && t.nsrc[b] > 0) m2m.push_back(b);
-    if ((int)m2m.size() > first) O.m2m_groups.push_back({first, (int)m2m.size() - first});
-  }
-  for (int l = 2; l <= t.nlev - 1; l++) {
-    int first = (int)l2l.size();
-    for (int b = t.ladd[l]; b < t.ladd[l + 1]; b++)
-      if (t.nchild[b] > 0) l2l.push_back(b);
-    if ((int)l2l.size() > first) O.l2l_groups.push_back({first, (int)l2l.size() - first});
-  }
-  O.m2m = F.i(m2m);
-  O.l2l = F.i(l2l);
-  O.sperm = F.i(t.src_perm);
-  O.tperm = F.i(t.trg_perm);
-  O.center = F.d(t.bcenter);
-  std::vector<double> inv(t.nlev + 1);
-  for (int l = 0; l <= t.nlev; l++) inv[l] = 1.0 / t.side(l);
-  O.inv_side = F.d(inv);
-}
-
-DevPlan make_devplan(const Octree& t, int p, const PlanOffsets& O, const int* I, const double* D) {
-  DevPlan P;
-  P.nboxes = t.nboxes; P.nlev = t.nlev; P.p = p; P.K = (p + 1) * (p + 1);
-  P.ns = (int)t.src_perm.size(); P.nt = (int)t.trg_perm.size();
-  P.level = I + O.level; P.isrc = I + O.isrc; P.nsrc = I + O.nsrc; P.itrg = I + O.itrg;
-  P.ntrg = I + O.ntrg; P.nchild = I + O.nchild; P.child = I + O.child; P.ijk = I + O.ijk;
-  P.center = D + O.center; P.inv_side = D + O.inv_side;
-  P.l1_off = I + O.l1o; P.l1_idx = I + O.l1i; P.l2_off = I + O.l2o; P.l2_idx = I + O.l2i;
-  P.l3_off = I + O.l3o; P.l3_idx = I + O.l3i; P.l4_off = I + O.l4o; P.l4_idx = I + O.l4i;
-  P.p2m_boxes = I + O.p2m; P.n_p2m = O.n_p2m;
-  P.l2p_boxes = I + O.l2p; P.n_l2p = O.n_l2p;
-  P.m2p_boxes = I + O.m2p; P.n_m2p = O.n_m2p;
-  P.p2l_boxes = I + O.p2l; P.n_p2l = O.n_p2l;
-  P.m2l_boxes = I + O.m2l; P.n_m2l = O.n_m2l;
-  P.m2m_boxes = I + O.m2m; P.m2m_lev_off = nullptr;
-  P.l2l_boxes = I + O.l2l; P.l2l_lev_off = nullptr;
-  P.p2p_items = I + O.p2p0; P.n_p2p = O.n_p2p0;
-  P.src_perm = I + O.sperm; P.trg_perm = I + O.tperm;
-  return P;
-}
 
 // Upload (once per order p) the translation tables.
 DevTables ensure_tables(Context& ctx, int p, double eps) {
@@ -216,18 +76,6 @@ DevTables ensure_tables(Context& ctx, int p, double eps) {
   return T;
 }
 
-size_t max_lds_bytes(Context& ctx) {
-  static size_t v = 0;
-  if (!v) {
-    hipDeviceProp_t prop;
-    FMM_HIP(hipGetDeviceProperties(&prop, ctx.device));
-    v = prop.sharedMemPerBlock;
-  }
-  return v;
-}
-
-inline unsigned blocks(size_t n, unsigned b) { return (unsigned)((n + b - 1) / b); }
-
 template <typename TN>
 void launch_p2p(const DevPlan& P, const int* items, int n, int kind, const double* spos,
                 const TN* str, const double* tpos, int ifcharge, int ifdipole, bool fld, double* pot,
@@ -258,14 +106,14 @@ int lap_solve(Context& ctx, const LapCall& c) {
   const int ns = c.ns, nt = c.nt;
   const Flags f{c.ifcharge ? 1 : 0, c.ifdipole ? 1 : 0, c.ifpot ? 1 : 0, c.iffld ? 1 : 0,
                 nt > 0 && c.ifpottarg ? 1 : 0, nt > 0 && c.iffldtarg ? 1 : 0};
-  Octree tree;
+  thread_local Octree tree;  // storage reused across calls
   if (build_octree(c.src_host, ns, c.trg_host, nt, nbox, tree) != 0) return 4;
   const int p = laplace_nterms(eps);
   const int K = (p + 1) * (p + 1);
   if (sizeof(T) * 4 * (size_t)K > max_lds_bytes(ctx)) return 16;
   Flat F;
   PlanOffsets O;
-  build_plan(tree, f, F, O);
+  build_plan(tree, f, F, O, 0);
   auto t1 = std::chrono::steady_clock::now();
   hipStream_t s = ctx.stream;
   Timer tm;

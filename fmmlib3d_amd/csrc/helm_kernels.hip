@@ -1,0 +1,643 @@
+// Helmholtz particle-FMM kernels for gfx950 (MI355X), FP64.
+//
+// Expansions use the reference's convention (src/helmrouts3d.f:3119 and
+// :3438): M_n^m = ik sum q j_n(kr)/sc^n Y_n^{-m}(u), L_n^m = ik sum q h_n(kr)
+// sc^n Y_n^{-m}(u), Y_n^m = sqrt(2n+1) Yhat_n^m, sc = the level's rscale.
+// Each box stores (pmax+1)^2 coefficients and uses the first (p_l+1)^2,
+// p_l = the order of its level.  Gradients use the exact identities
+//   dz f_n Yhat_n^m = k/(2n+1) [AR(n,m) f_{n-1} Yhat_{n-1}^m + AS(n,m) f_{n+1} Yhat_{n+1}^m]
+//   U  f_n Yhat_n^m = k/(2n+1) [BR(n,m) f_{n-1} Yhat_{n-1}^{m+1} + BS(n,m) f_{n+1} Yhat_{n+1}^{m+1}]
+//   D  f_n Yhat_n^m = k/(2n+1) [CR(n,m) f_{n-1} Yhat_{n-1}^{m-1} + CS(n,m) f_{n+1} Yhat_{n+1}^{m-1}]
+// (f = j or h, U = d/dx + i d/dy, D = d/dx - i d/dy, AR..CS the factor table
+// of the Laplace kernels), so the field of an expansion is itself a short
+// sum over the same basis.
+#include <hip/hip_runtime.h>
+
+#include "kern_common.hpp"
+
+namespace fmm3d {
+
+using Cd = C2<double>;
+
+// Helmholtz-specific device data (all pointers into the uploaded tables).
+struct HelmDev {
+  double kre, kim;
+  int nd;                  // distance classes of the list-2 offsets
+  const int* plev;         // order per level
+  const double* sclev;     // rscale per level
+  const int* ptrunc;       // (nlev+1) x 343 list-2 orders
+  const long* coax;        // per table: base, pin, pout
+  const double* z;         // coaxial blocks, complex
+};
+
+__device__ __forceinline__ Cd cadd2(Cd a, Cd b) { return {a.re + b.re, a.im + b.im}; }
+__device__ __forceinline__ Cd csub2(Cd a, Cd b) { return {a.re - b.re, a.im - b.im}; }
+__device__ __forceinline__ Cd cscale(Cd a, double s) { return {a.re * s, a.im * s}; }
+__device__ __forceinline__ double cnorm(Cd a) { return a.re * a.re + a.im * a.im; }
+__device__ __forceinline__ Cd cdiv(Cd a, Cd b) {
+  const double d = 1.0 / cnorm(b);
+  return {(a.re * b.re + a.im * b.im) * d, (a.im * b.re - a.re * b.im) * d};
+}
+
+// h_i(z) sc^i, i = 0..n, upward recurrence (src/helmrouts3d.f:690 h3dall)
+__device__ void dev_hankel(int n, Cd z, double sc, Cd* out, int st) {
+  if (cnorm(z) < 1.0e-30) {
+    for (int i = 0; i <= n; i++) out[(size_t)i * st] = {0.0, 0.0};
+    return;
+  }
+  double s, c;
+  sincos(z.re, &s, &c);
+  const double e = exp(-z.im);
+  const Cd h0 = cdiv(Cd{e * c, e * s}, Cd{-z.im, z.re});   // e^{iz}/(iz)
+  out[0] = h0;
+  if (n < 1) return;
+  const Cd zi = cdiv(Cd{1.0, 0.0}, z);
+  Cd h1 = cmul(h0, Cd{zi.re, zi.im - 1.0});
+  h1 = cscale(h1, sc);
+  out[st] = h1;
+  const Cd zinv = cscale(zi, sc);
+  const double sc2 = sc * sc;
+  Cd hm = h0, hc = h1;
+  for (int i = 1; i < n; i++) {
+    const Cd zt = cscale(zinv, 2.0 * i + 1.0);
+    const Cd hn = csub2(cmul(zt, hc), cscale(hm, sc2));
+    out[(size_t)(i + 1) * st] = hn;
+    hm = hc;
+    hc = hn;
+  }
+}
+
+// j_i(z)/sc^i, i = 0..n (1 <= n <= 127), Miller's backward recurrence as
+// src/cdjseval3d.f:15 jfuns3d: start where the forward recurrence passes
+// 1e20, rescale by 1e-40 whenever the backward sweep would overflow,
+// normalise with j_0 or j_1.
+__device__ void dev_bessel(int n, Cd z, double sc, Cd* out, int st) {
+  if (fabs(z.re) < 1e-200 && fabs(z.im) < 1e-200) {
+    out[0] = {1.0, 0.0};
+    for (int i = 1; i <= n; i++) out[(size_t)i * st] = {0.0, 0.0};
+    return;
+  }
+  const Cd zinv = cdiv(Cd{1.0, 0.0}, z);
+  Cd fm = {0.0, 0.0}, fc = {1.0, 0.0};
+  int ntop = 0;
+  for (int i = n; i <= n + 1000; i++) {
+    const Cd t = csub2(cmul(cscale(zinv, 2.0 * i + 1.0), fc), fm);
+    fm = fc;
+    fc = t;
+    if (cnorm(t) > 1.0e40) { ntop = i + 1; break; }
+  }
+  if (ntop == 0) {  // |z| beyond ~1000: outside any FMM box of this code
+    for (int i = 0; i <= n; i++) out[(size_t)i * st] = {0.0, 0.0};
+    return;
+  }
+  unsigned long long ev0 = 0, ev1 = 0;   // rescale events at index e (e <= 127)
+  Cd fa = {0.0, 0.0}, fb = {1.0, 0.0};    // f[i+1], f[i] with i = ntop-1
+  if (ntop - 1 <= n) out[(size_t)(ntop - 1) * st] = fb;
+  for (int i = ntop - 1; i >= 1; i--) {
+    Cd t = csub2(cmul(cscale(zinv, 2.0 * i + 1.0), fb), fa);
+    if (cnorm(t) > 1.0e40) {
+      fb = cscale(fb, 1.0e-40);
+      t = cscale(t, 1.0e-40);
+      if (i <= n) out[(size_t)i * st] = fb;
+      if (i < 64) ev0 |= 1ull << i;
+      else if (i < 128) ev1 |= 1ull << (i - 64);
+    }
+    if (i - 1 <= n) out[(size_t)(i - 1) * st] = t;
+    fa = fb;
+    fb = t;
+  }
+  const double scinv = 1.0 / sc;
+  double sctot = 1.0;
+  for (int i = 1; i <= n; i++) {
+    sctot *= scinv;
+    const int e = i - 1;
+    if ((e < 64 && ((ev0 >> e) & 1)) || (e >= 64 && ((ev1 >> (e - 64)) & 1))) sctot *= 1.0e-40;
+    out[(size_t)i * st] = cscale(out[(size_t)i * st], sctot);
+  }
+  // j0 = sin z / z, j1 = j0 / z - cos z / z
+  double sx, cx;
+  sincos(z.re, &sx, &cx);
+  const double ch = cosh(z.im), sh = sinh(z.im);
+  const Cd sz = {sx * ch, cx * sh}, cz = {cx * ch, -sx * sh};
+  const Cd j0 = cmul(sz, zinv);
+  const Cd j1 = csub2(cmul(j0, zinv), cmul(cz, zinv));
+  Cd zs;
+  if (cnorm(j1) > cnorm(j0)) zs = cdiv(j1, cscale(out[st], sc));
+  else zs = cdiv(j0, out[0]);
+  for (int i = 0; i <= n; i++) out[(size_t)i * st] = cmul(out[(size_t)i * st], zs);
+}
+
+// Yhat_n^{mu} from a half table of Yhat_n^m (m >= 0): conj for mu < 0
+__device__ __forceinline__ Cd yget(const double* Y, int n, int mu) {
+  if (mu >= 0) return cld(Y, hidx(n, mu));
+  return cconj(cld(Y, hidx(n, -mu)));
+}
+// f_n Yhat_n^mu, zero outside 0 <= |mu| <= n <= nmax
+__device__ __forceinline__ Cd fyget(const double* Y, const double* f, int nmax, int n, int mu) {
+  if (n < 0 || n > nmax || mu > n || mu < -n) return {0.0, 0.0};
+  return cmul(cld(f, n), yget(Y, n, mu));
+}
+
+// ---------------------------------------------------------------------------
+// P2M (src/helmrouts3d.f:2972, :3580) and P2L (:3299, :3978).  One workgroup
+// per box; sources in tiles of `ts`.  Phase A: per source, the unit-vector
+// harmonics Yhat (one thread per column m) and the radial series f_n(k r)
+// (one thread per source); phase B: one thread per coefficient.
+// ---------------------------------------------------------------------------
+template <bool IRREG>
+__global__ void k_hform(DevPlan P, HelmDev Hd, const double* __restrict__ fac,
+                        const double* __restrict__ spos, const double* __restrict__ str, int ifcharge,
+                        int ifdipole, int ts, double* __restrict__ expn) {
+  extern __shared__ unsigned char smem_raw[];
+  double* smem = reinterpret_cast<double*>(smem_raw);
+  const int b = IRREG ? P.p2l_boxes[blockIdx.x] : P.p2m_boxes[blockIdx.x];
+  const int lev = P.level[b];
+  if (lev < 2) return;
+  const int pmaxg = P.p, Kg = P.K;
+  const int p = Hd.plev[lev], K = (p + 1) * (p + 1);
+  const int pt = ifdipole ? p + 1 : p;            // degrees the tile needs
+  const int Hh = (pmaxg + 2) * (pmaxg + 3) / 2;   // half-table stride (fixed by the launch)
+  const int Fn = pmaxg + 2;
+  FacLayout L(pmaxg);
+  double* acc = smem;                       // 2*Kg: charge part, then dipole bracket
+  double* accd = acc + 2 * (size_t)Kg;      // 2*Kg
+  double* tab = accd + 2 * (size_t)Kg;      // ts * Hh complex
+  double* fser = tab + 2 * (size_t)ts * Hh; // ts * Fn complex
+  double* sst = fser + 2 * (size_t)ts * Fn; // ts * 8: q, wz, wU, wD
+  double* geo = sst + 8 * (size_t)ts;       // ts * 4: unit vector, r
+  const double sc = Hd.sclev[lev];
+  const double lo = IRREG ? sc : 1.0 / sc, hi = IRREG ? 1.0 / sc : sc;
+  const double cx = P.center[3 * b], cy = P.center[3 * b + 1], cz = P.center[3 * b + 2];
+  for (int k = threadIdx.x; k < 4 * Kg; k += blockDim.x) acc[k] = 0.0;
+
+  int jbeg = 0, jend = 1;
+  if (IRREG) { jbeg = P.l4_off[b]; jend = P.l4_off[b + 1]; }
+  for (int jj = jbeg; jj < jend; jj++) {
+    const int j = IRREG ? P.l4_idx[jj] : b;
+    const int s0 = P.isrc[j], sn = P.nsrc[j];
+    for (int t0 = 0; t0 < sn; t0 += ts) {
+      const int tn = min(ts, sn - t0);
+      __syncthreads();
+      for (int s = threadIdx.x; s < tn; s += blockDim.x) {
+        const double* x = spos + 4 * (size_t)(s0 + t0 + s);
+        const double* r = str + 8 * (size_t)(s0 + t0 + s);
+        const double ux = x[0] - cx, uy = x[1] - cy, uz = x[2] - cz;
+        const double rr = sqrt(ux * ux + uy * uy + uz * uz);
+        const double ir = rr > 0.0 ? 1.0 / rr : 0.0;
+        geo[4 * s] = rr > 0.0 ? ux * ir : 0.0;
+        geo[4 * s + 1] = rr > 0.0 ? uy * ir : 0.0;
+        geo[4 * s + 2] = rr > 0.0 ? uz * ir : 1.0;
+        geo[4 * s + 3] = rr;
+        sst[8 * s] = r[0]; sst[8 * s + 1] = r[1];
+        sst[8 * s + 2] = r[6]; sst[8 * s + 3] = r[7];
+        sst[8 * s + 4] = 0.5 * (r[2] + r[5]); sst[8 * s + 5] = 0.5 * (r[3] - r[4]);
+        sst[8 * s + 6] = 0.5 * (r[2] - r[5]); sst[8 * s + 7] = 0.5 * (r[3] + r[4]);
+      }
+      __syncthreads();
+      // phase A
+      for (int task = threadIdx.x; task < tn * (pt + 2); task += blockDim.x) {
+        const int s = task / (pt + 2), m = task - s * (pt + 2);
+        if (m <= pt) {
+          regular_column<double>(fac, L, pt, m, geo[4 * s], geo[4 * s + 1], geo[4 * s + 2], 1.0,
+                                 tab + 2 * (size_t)s * Hh, 1);
+        } else {
+          const double rr = geo[4 * s + 3];
+          const Cd zz = {Hd.kre * rr, Hd.kim * rr};
+          Cd* f = reinterpret_cast<Cd*>(fser + 2 * (size_t)s * Fn);
+          if (IRREG) {
+            if (cnorm(zz) < 1.0e-30) {      // h3dformta_trunc0 skips such a source
+              for (int i = 0; i <= pt; i++) f[i] = {0.0, 0.0};
+            } else {
+              dev_hankel(pt, zz, sc, f, 1);
+            }
+          } else {
+            dev_bessel(max(pt, 1), zz, sc, f, 1);
+          }
+        }
+      }
+      __syncthreads();
+      // phase B
+      for (int k = threadIdx.x; k < K; k += blockDim.x) {
+        int n = (int)sqrtf((float)k);
+        while (n * n > k) n--;
+        while ((n + 1) * (n + 1) <= k) n++;
+        const int m = k - n * n - n, mu = -m;
+        Cd a = {0.0, 0.0}, g = {0.0, 0.0};
+        const double far_ = (double)fac[L.AR + eidx(n, mu)], fas = (double)fac[L.AS + eidx(n, mu)];
+        const double fbr = n >= 1 ? (double)fac[L.BR + eidx(n, mu)] : 0.0, fbs = (double)fac[L.BS + eidx(n, mu)];
+        const double fcr = n >= 1 ? (double)fac[L.CR + eidx(n, mu)] : 0.0, fcs = (double)fac[L.CS + eidx(n, mu)];
+        for (int s = 0; s < tn; s++) {
+          const double* Y = tab + 2 * (size_t)s * Hh;
+          const double* f = fser + 2 * (size_t)s * Fn;
+          if (ifcharge) cfma(a, Cd{sst[8 * s], sst[8 * s + 1]}, fyget(Y, f, pt, n, mu));
+          if (ifdipole) {
+            Cd tz = cadd2(cscale(fyget(Y, f, pt, n - 1, mu), far_ * lo), cscale(fyget(Y, f, pt, n + 1, mu), fas * hi));
+            Cd tu = cadd2(cscale(fyget(Y, f, pt, n - 1, mu + 1), fbr * lo), cscale(fyget(Y, f, pt, n + 1, mu + 1), fbs * hi));
+            Cd td = cadd2(cscale(fyget(Y, f, pt, n - 1, mu - 1), fcr * lo), cscale(fyget(Y, f, pt, n + 1, mu - 1), fcs * hi));
+            cfma(g, Cd{sst[8 * s + 2], sst[8 * s + 3]}, tz);
+            cfma(g, Cd{sst[8 * s + 4], sst[8 * s + 5]}, tu);
+            cfma(g, Cd{sst[8 * s + 6], sst[8 * s + 7]}, td);
+          }
+        }
+        acc[2 * k] += a.re; acc[2 * k + 1] += a.im;
+        accd[2 * k] += g.re; accd[2 * k + 1] += g.im;
+      }
+    }
+  }
+  __syncthreads();
+  // out += ik sqrt(2n+1) (charge part + k/(2n+1) dipole bracket)
+  double* out = expn + 2 * (size_t)b * Kg;
+  const Cd ik = {-Hd.kim, Hd.kre}, kk = {Hd.kre, Hd.kim};
+  for (int k = threadIdx.x; k < K; k += blockDim.x) {
+    int n = (int)sqrtf((float)k);
+    while (n * n > k) n--;
+    while ((n + 1) * (n + 1) <= k) n++;
+    Cd v = {acc[2 * k], acc[2 * k + 1]};
+    Cd d = cmul(kk, Cd{accd[2 * k], accd[2 * k + 1]});
+    v = cadd2(v, cscale(d, 1.0 / (2.0 * n + 1.0)));
+    v = cscale(cmul(ik, v), (double)fac[L.M1 + n]);
+    out[2 * k] += v.re;
+    out[2 * k + 1] += v.im;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// M2P (src/helmrouts3d.f:2211) and L2P (:2550).  One workgroup of NT threads
+// per box, one point per thread; the radial series of every thread live in
+// LDS (stride NT), the expansion premultiplied by sqrt(2n+1) (potential) and
+// by sqrt(2n+1) k/(2n+1) (gradient) is broadcast from LDS.
+// ---------------------------------------------------------------------------
+template <bool IRREG>
+__global__ __launch_bounds__(64) void k_heval(DevPlan P, HelmDev Hd, const double* __restrict__ fac,
+                                              const double* __restrict__ expn,
+                                              const double* __restrict__ spos,
+                                              const double* __restrict__ tpos, int wpot0, int wfld0,
+                                              int wpot1, int wfld1, double* __restrict__ pot0,
+                                              double* __restrict__ fld0, double* __restrict__ pot1,
+                                              double* __restrict__ fld1) {
+  constexpr int NT = 64;
+  extern __shared__ unsigned char smem_raw[];
+  double* Ew = reinterpret_cast<double*>(smem_raw);   // 2*Kg
+  double* fb = Ew + 2 * (size_t)P.K;                  // NT * (pmax+2) complex
+  const int pmaxg = P.p;
+  FacLayout L(pmaxg);
+  const int b = IRREG ? P.m2p_boxes[blockIdx.x] : P.l2p_boxes[blockIdx.x];
+  const bool grad = (wfld0 != 0) || (wfld1 != 0);
+  int jbeg = 0, jend = 1;
+  if (IRREG) { jbeg = P.l3_off[b]; jend = P.l3_off[b + 1]; }
+  for (int jj = jbeg; jj < jend; jj++) {
+    const int j = IRREG ? P.l3_idx[jj] : b;
+    const int lev = P.level[j];
+    if (lev < 2) continue;
+    if (IRREG && P.nsrc[j] == 0) continue;
+    const int p = Hd.plev[lev], K = (p + 1) * (p + 1);
+    const double sc = Hd.sclev[lev];
+    const double lo = IRREG ? sc : 1.0 / sc, hi = IRREG ? 1.0 / sc : sc;
+    const double cx = P.center[3 * j], cy = P.center[3 * j + 1], cz = P.center[3 * j + 2];
+    __syncthreads();
+    const Cd kk = {Hd.kre, Hd.kim};
+    for (int k = threadIdx.x; k < K; k += NT) {
+      int n = (int)sqrtf((float)k);
+      while (n * n > k) n--;
+      while ((n + 1) * (n + 1) <= k) n++;
+      Cd e = cscale(Cd{expn[2 * ((size_t)j * P.K + k)], expn[2 * ((size_t)j * P.K + k) + 1]}, (double)fac[L.M1 + n]);
+      Ew[2 * k] = e.re; Ew[2 * k + 1] = e.im;
+    }
+    __syncthreads();
+    const int pd = grad ? p + 1 : p;
+    for (int kind = 0; kind < 2; kind++) {
+      const int wp = kind ? wpot1 : wpot0, wf = kind ? wfld1 : wfld0;
+      if (!wp && !wf) continue;
+      const int n0 = kind ? P.itrg[b] : P.isrc[b];
+      const int nn = kind ? P.ntrg[b] : P.nsrc[b];
+      for (int i0 = 0; i0 < nn; i0 += NT) {
+        const int i = i0 + threadIdx.x;
+        if (i >= nn) continue;
+        const double* q = (kind ? tpos : spos) + 4 * (size_t)(n0 + i);
+        const double ux0 = q[0] - cx, uy0 = q[1] - cy, uz0 = q[2] - cz;
+        const double rr = sqrt(ux0 * ux0 + uy0 * uy0 + uz0 * uz0);
+        const Cd zz = {Hd.kre * rr, Hd.kim * rr};
+        if (IRREG && cnorm(zz) < 1.0e-30) continue;   // h3dmpeval_trunc0 returns early
+        const double ir = rr > 0.0 ? 1.0 / rr : 0.0;
+        const double ux = rr > 0.0 ? ux0 * ir : 0.0, uy = rr > 0.0 ? uy0 * ir : 0.0, uz = rr > 0.0 ? uz0 * ir : 1.0;
+        double* f = fb + 2 * (size_t)threadIdx.x;   // f_n at f[2*NT*n]
+        if (IRREG) dev_hankel(pd, zz, sc, reinterpret_cast<Cd*>(f), NT);
+        else dev_bessel(max(pd, 1), zz, sc, reinterpret_cast<Cd*>(f), NT);
+        Cd pot = {0.0, 0.0}, gz = {0.0, 0.0}, gu = {0.0, 0.0}, gd = {0.0, 0.0};
+        auto E = [&](const double* A, int n, int m) -> Cd { return cld(A, eidx(n, m)); };
+        // gradient coefficient of degree n: sqrt(2n+1) E_n^m k/(2n+1)
+        auto Ekv = [&](int n, int m) -> Cd { return cscale(cmul(kk, cld(Ew, eidx(n, m))), 1.0 / (2.0 * n + 1.0)); };
+        Cd ymm = {1.0, 0.0};
+        for (int m = 0; m <= pd; m++) {
+          if (m > 0) {
+            const double fm = fac[L.MM + m];
+            ymm = cmul(Cd{-ux * fm, -uy * fm}, ymm);
+          }
+          Cd y0 = ymm, y1 = {0.0, 0.0};
+          for (int n = m; n <= pd; n++) {
+            Cd y;
+            if (n == m) y = ymm;
+            else if (n == m + 1) { y = cscale(ymm, fac[L.M1 + m] * uz); y1 = y; }
+            else {
+              const double a = (2.0 * n - 1.0) * uz, bq = fac[L.RA + hidx(n, m)], c = fac[L.RB + hidx(n, m)];
+              y = {(a * y1.re - bq * y0.re) * c, (a * y1.im - bq * y0.im) * c};
+              y0 = y1;
+              y1 = y;
+            }
+            if (n == m + 1) y0 = ymm;
+            const Cd fn = {f[2 * (size_t)NT * n], f[2 * (size_t)NT * n + 1]};
+            const Cd B = cmul(fn, y), Bm = cmul(fn, cconj(y));
+            if (n <= p) {
+              cfma(pot, E(Ew, n, m), B);
+              if (m > 0) cfma(pot, E(Ew, n, -m), Bm);
+            }
+            if (!grad) continue;
+            for (int sgn = 0; sgn < (m > 0 ? 2 : 1); sgn++) {
+              const int mm = sgn ? -m : m;
+              const Cd Bv = sgn ? Bm : B;
+              // contributions from coefficient degree n+1 (lo terms) and n-1 (hi terms)
+              Cd cz_ = {0.0, 0.0}, cu_ = {0.0, 0.0}, cd_ = {0.0, 0.0};
+              if (n + 1 <= p) {
+                const int n1 = n + 1;
+                cz_ = cadd2(cz_, cscale(Ekv(n1, mm), fac[L.AR + eidx(n1, mm)] * lo));
+                if (mm - 1 >= -n1) cu_ = cadd2(cu_, cscale(Ekv(n1, mm - 1), fac[L.BR + eidx(n1, mm - 1)] * lo));
+                if (mm + 1 <= n1) cd_ = cadd2(cd_, cscale(Ekv(n1, mm + 1), fac[L.CR + eidx(n1, mm + 1)] * lo));
+              }
+              if (n >= 1 && n - 1 <= p) {
+                const int n1 = n - 1;
+                if (mm <= n1 && mm >= -n1) cz_ = cadd2(cz_, cscale(Ekv(n1, mm), fac[L.AS + eidx(n1, mm)] * hi));
+                if (mm - 1 <= n1 && mm - 1 >= -n1) cu_ = cadd2(cu_, cscale(Ekv(n1, mm - 1), fac[L.BS + eidx(n1, mm - 1)] * hi));
+                if (mm + 1 <= n1 && mm + 1 >= -n1) cd_ = cadd2(cd_, cscale(Ekv(n1, mm + 1), fac[L.CS + eidx(n1, mm + 1)] * hi));
+              }
+              cfma(gz, cz_, Bv);
+              cfma(gu, cu_, Bv);
+              cfma(gd, cd_, Bv);
+            }
+          }
+        }
+        double* po = kind ? pot1 : pot0;
+        double* fo = kind ? fld1 : fld0;
+        const size_t id = (size_t)(n0 + i);
+        if (wp) { po[2 * id] += pot.re; po[2 * id + 1] += pot.im; }
+        if (wf) {
+          const Cd dx = {0.5 * (gu.re + gd.re), 0.5 * (gu.im + gd.im)};
+          const Cd dy = {0.5 * (gu.im - gd.im), -0.5 * (gu.re - gd.re)};
+          fo[6 * id + 0] -= dx.re; fo[6 * id + 1] -= dx.im;
+          fo[6 * id + 2] -= dy.re; fo[6 * id + 3] -= dy.im;
+          fo[6 * id + 4] -= gz.re; fo[6 * id + 5] -= gz.im;
+        }
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Translations: phase, rotate, coaxial shift (complex packed block), rotate
+// back, phase back (src/h3dtrans.f:211, :1229, :1998 apply the same
+// operators by projection).  `acc` accumulates in LDS or global memory.
+// ---------------------------------------------------------------------------
+__device__ void hzshift_block(const HelmDev& Hd, int t, int pin, int pout, const double* in, double* out) {
+  const long base = Hd.coax[3 * t];
+  const int tin = (int)Hd.coax[3 * t + 1], tout = (int)Hd.coax[3 * t + 2];
+  const double* Z = Hd.z + 2 * base;
+  const int Kt = (pout + 1) * (pout + 1);
+  for (int k = threadIdx.x; k < Kt; k += blockDim.x) {
+    int l = (int)sqrtf((float)k);
+    while (l * l > k) l--;
+    while ((l + 1) * (l + 1) <= k) l++;
+    const int m = k - l * l - l, am = m < 0 ? -m : m;
+    Cd a = {0.0, 0.0};
+    if (am <= pin) {
+      const long A = tout + 1, Bc = tin + 1, M = am;
+      const long mb = M * A * Bc - (A + Bc) * M * (M - 1) / 2 + (M - 1) * M * (2 * M - 1) / 6;
+      const double* zr = Z + 2 * (mb + (long)(l - am) * (tin + 1 - am));
+      for (int n = am; n <= pin; n++) cfma(a, Cd{zr[2 * (n - am)], zr[2 * (n - am) + 1]}, cld(in, eidx(n, m)));
+    }
+    out[2 * k] = a.re;
+    out[2 * k + 1] = a.im;
+  }
+}
+
+__device__ void hpoint_and_shoot(const DevTables& Tb, const HelmDev& Hd, int rc, const double* phase, int t,
+                                 int pin, int pout, const double* in, double* out, double* A, double* B) {
+  const int pm = pin > pout ? pin : pout;
+  const int Ki = (pin + 1) * (pin + 1);
+  for (int k = threadIdx.x; k < (pm + 1) * (pm + 1); k += blockDim.x) {
+    if (k < Ki) {
+      int n = (int)sqrtf((float)k);
+      while (n * n > k) n--;
+      while ((n + 1) * (n + 1) <= k) n++;
+      const int m = k - n * n - n, am = m < 0 ? -m : m;
+      double er = phase[2 * am], ei = phase[2 * am + 1];
+      if (m < 0) ei = -ei;
+      const double xr = in[2 * k], xi = in[2 * k + 1];
+      A[2 * k] = xr * er - xi * ei;
+      A[2 * k + 1] = xr * ei + xi * er;
+    } else {
+      A[2 * k] = 0.0;
+      A[2 * k + 1] = 0.0;
+    }
+  }
+  __syncthreads();
+  rotate_block<double>(Tb, rc, pin, A, B, false);
+  __syncthreads();
+  hzshift_block(Hd, t, pin, pout, B, A);
+  __syncthreads();
+  rotate_block<double>(Tb, rc, pout, A, B, true);
+  __syncthreads();
+  const int Ko = (pout + 1) * (pout + 1);
+  for (int k = threadIdx.x; k < Ko; k += blockDim.x) {
+    int n = (int)sqrtf((float)k);
+    while (n * n > k) n--;
+    while ((n + 1) * (n + 1) <= k) n++;
+    const int m = k - n * n - n, am = m < 0 ? -m : m;
+    double er = phase[2 * am], ei = -phase[2 * am + 1];
+    if (m < 0) ei = -ei;
+    const double xr = B[2 * k], xi = B[2 * k + 1];
+    out[2 * k] += xr * er - xi * ei;
+    out[2 * k + 1] += xr * ei + xi * er;
+  }
+  __syncthreads();
+}
+
+// M2M into the parents of one level (src/hfmm3drouts.f:193-266)
+__global__ void k_hm2m(DevPlan P, DevTables Tb, HelmDev Hd, int first, int tab, double* __restrict__ mp) {
+  extern __shared__ unsigned char smem_raw[];
+  double* A = reinterpret_cast<double*>(smem_raw);
+  double* B = A + 2 * P.K;
+  const int b = P.m2m_boxes[first + blockIdx.x];
+  const int lev = P.level[b];
+  const int pp = Hd.plev[lev], pc = Hd.plev[lev + 1];
+  for (int c = 0; c < P.nchild[b]; c++) {
+    const int ch = P.child[8 * b + c];
+    if (P.nsrc[ch] == 0) continue;
+    const int o = (P.ijk[3 * ch] & 1 ? 0 : 1) | (P.ijk[3 * ch + 1] & 1 ? 0 : 2) | (P.ijk[3 * ch + 2] & 1 ? 0 : 4);
+    const double* ph = Tb.oct_phase + (size_t)(o * 2 + 0) * 2 * (Tb.p + 1);
+    hpoint_and_shoot(Tb, Hd, Tb.oct_rot[o * 2 + 0], ph, tab, pc, pp, mp + 2 * (size_t)ch * P.K,
+                     mp + 2 * (size_t)b * P.K, A, B);
+  }
+}
+
+// L2L from the parents of one level (src/hfmm3drouts.f:483-556)
+__global__ void k_hl2l(DevPlan P, DevTables Tb, HelmDev Hd, int first, int tab, double* __restrict__ lo) {
+  extern __shared__ unsigned char smem_raw[];
+  double* A = reinterpret_cast<double*>(smem_raw);
+  double* B = A + 2 * P.K;
+  const int b = P.l2l_boxes[first + blockIdx.x];
+  const int lev = P.level[b];
+  const int pp = Hd.plev[lev], pc = Hd.plev[lev + 1];
+  for (int c = 0; c < P.nchild[b]; c++) {
+    const int ch = P.child[8 * b + c];
+    const int o = (P.ijk[3 * ch] & 1 ? 0 : 1) | (P.ijk[3 * ch + 1] & 1 ? 0 : 2) | (P.ijk[3 * ch + 2] & 1 ? 0 : 4);
+    const double* ph = Tb.oct_phase + (size_t)(o * 2 + 1) * 2 * (Tb.p + 1);
+    hpoint_and_shoot(Tb, Hd, Tb.oct_rot[o * 2 + 1], ph, tab, pp, pc, lo + 2 * (size_t)b * P.K,
+                     lo + 2 * (size_t)ch * P.K, A, B);
+  }
+}
+
+// M2L over the pruned list 2 of each target box (src/hfmm3drouts.f:330-477);
+// the target's local accumulates in LDS and is added to memory once.
+template <bool LDSACC>
+__global__ void k_hm2l(DevPlan P, DevTables Tb, HelmDev Hd, const double* __restrict__ mp,
+                       double* __restrict__ lo) {
+  extern __shared__ unsigned char smem_raw[];
+  double* A = reinterpret_cast<double*>(smem_raw);
+  double* B = A + 2 * P.K;
+  const int b = P.m2l_boxes[blockIdx.x];
+  const int lev = P.level[b];
+  const int p = Hd.plev[lev], K = (p + 1) * (p + 1);
+  double* acc = LDSACC ? B + 2 * P.K : lo + 2 * (size_t)b * P.K;
+  if (LDSACC)
+    for (int k = threadIdx.x; k < 2 * K; k += blockDim.x) acc[k] = 0.0;
+  __syncthreads();
+  for (int jj = P.l2_off[b]; jj < P.l2_off[b + 1]; jj++) {
+    const int j = P.l2_idx[jj];
+    const int di = P.ijk[3 * j] - P.ijk[3 * b] + 3, dj = P.ijk[3 * j + 1] - P.ijk[3 * b + 1] + 3,
+              dk = P.ijk[3 * j + 2] - P.ijk[3 * b + 2] + 3;
+    const int code = (di * 7 + dj) * 7 + dk;
+    const int pt = Hd.ptrunc[lev * 343 + code];
+    const int tab = lev * Hd.nd + Tb.off_dist[code];
+    hpoint_and_shoot(Tb, Hd, Tb.off_rot[code], Tb.off_phase + (size_t)code * 2 * (Tb.p + 1), tab, pt, pt,
+                     mp + 2 * (size_t)j * P.K, acc, A, B);
+  }
+  if (LDSACC) {
+    double* out = lo + 2 * (size_t)b * P.K;
+    for (int k = threadIdx.x; k < 2 * K; k += blockDim.x) out[k] += acc[k];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// P2P near field (src/hfmm3dpart.f:1092-1262 hfmm3dpart_direct_self/_targ with
+// hpotfld3d / hpotfld3d_dp, src/helmrouts3d.f:1335, :1844):
+//   charge  pot = q e^{ikr}/r,  fld = q (1-ikr) e^{ikr} x / r^3
+//   dipole  pot = (v.x) (1-ikr) e^{ikr}/r^3,
+//           fld = -(v (1-ikr) e^{ikr}/r^3 + (v.x) x e^{ikr} (k^2 r^2 + 3ikr - 3)/r^5)
+// with v = dipstr dipvec.  Same work decomposition as the Laplace kernel.
+// ---------------------------------------------------------------------------
+template <bool CHG, bool DIP, bool FLD>
+struct HPairAcc {
+  double a[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  __device__ __forceinline__ void add(double kre, double kim, double xt, double yt, double zt,
+                                      const double* __restrict__ sp, const double* __restrict__ st,
+                                      bool mask) {
+    const double dx = xt - sp[0], dy = yt - sp[1], dz = zt - sp[2];
+    const double r2 = fma(dx, dx, fma(dy, dy, dz * dz));
+    double ir = rsqrt(r2);
+    ir = ir * (1.5 - 0.5 * r2 * ir * ir);   // one Newton step: rsqrt to full FP64 accuracy
+    if (mask) ir = 0.0;                      // a source never acts on itself
+    const double r = r2 * ir;
+    double s, c;
+    sincos(kre * r, &s, &c);
+    const double damp = kim == 0.0 ? 1.0 : exp(-kim * r);
+    const double gr = c * damp * ir, gi = s * damp * ir;   // G = e^{ikr}/r
+    const double ir2 = ir * ir;
+    // om = 1 - ikr = (1 + kim r) - i kre r
+    const double omr = 1.0 + kim * r, omi = -kre * r;
+    // g = om G / r^2
+    const double g_r = (omr * gr - omi * gi) * ir2, g_i = (omr * gi + omi * gr) * ir2;
+    if (CHG) {
+      a[0] += st[0] * gr - st[1] * gi;
+      a[1] += st[0] * gi + st[1] * gr;
+      if (FLD) {
+        const double cr = st[0] * g_r - st[1] * g_i, ci = st[0] * g_i + st[1] * g_r;
+        a[2] += cr * dx; a[3] += ci * dx;
+        a[4] += cr * dy; a[5] += ci * dy;
+        a[6] += cr * dz; a[7] += ci * dz;
+      }
+    }
+    if (DIP) {
+      const double dotr = st[2] * dx + st[4] * dy + st[6] * dz;
+      const double doti = st[3] * dx + st[5] * dy + st[7] * dz;
+      a[0] += dotr * g_r - doti * g_i;
+      a[1] += dotr * g_i + doti * g_r;
+      if (FLD) {
+        // h = G (k^2 r^2 + 3 i k r - 3) / r^4 ; k^2 = kre^2 - kim^2 + 2i kre kim, ikr = -kim r + i kre r
+        const double k2r = (kre * kre - kim * kim) * r2, k2i = 2.0 * kre * kim * r2;
+        const double wr = k2r - 3.0 * kim * r - 3.0, wi = k2i + 3.0 * kre * r;
+        const double ir4 = ir2 * ir2;
+        const double hr = (gr * wr - gi * wi) * ir4, hi = (gr * wi + gi * wr) * ir4;
+        const double ur = dotr * hr - doti * hi, ui = dotr * hi + doti * hr;   // (v.x) h
+        // fld -= v g + (v.x) h x
+        a[2] -= (st[2] * g_r - st[3] * g_i) + ur * dx;
+        a[3] -= (st[2] * g_i + st[3] * g_r) + ui * dx;
+        a[4] -= (st[4] * g_r - st[5] * g_i) + ur * dy;
+        a[5] -= (st[4] * g_i + st[5] * g_r) + ui * dy;
+        a[6] -= (st[6] * g_r - st[7] * g_i) + ur * dz;
+        a[7] -= (st[6] * g_i + st[7] * g_r) + ui * dz;
+      }
+    }
+  }
+};
+
+template <bool CHG, bool DIP, bool FLD>
+__global__ __launch_bounds__(256) void k_hp2p(DevPlan P, double kre, double kim, const double* __restrict__ spos,
+                                              const double* __restrict__ str, const double* __restrict__ tpos,
+                                              int kind, double* __restrict__ pot, double* __restrict__ fld) {
+  const int w = blockIdx.x;
+  const int b = P.p2p_items[3 * w], start = P.p2p_items[3 * w + 2];
+  const int n0 = kind ? P.itrg[b] : P.isrc[b];
+  const int nn = kind ? P.ntrg[b] : P.nsrc[b];
+  const int i = start + threadIdx.x;
+  const bool active = i < nn;
+  const int me = n0 + (active ? i : 0);
+  const double* mp = (kind ? tpos : spos) + 4 * (size_t)me;
+  const double x = mp[0], y = mp[1], z = mp[2];
+  HPairAcc<CHG, DIP, FLD> acc;
+  {
+    const int s0 = P.isrc[b], s1 = s0 + P.nsrc[b];
+    const int self = kind ? -1 : me;
+    for (int s = s0; s < s1; s++) acc.add(kre, kim, x, y, z, spos + 4 * (size_t)s, str + 8 * (size_t)s, s == self);
+  }
+  for (int jj = P.l1_off[b]; jj < P.l1_off[b + 1]; jj++) {
+    const int j = P.l1_idx[jj];
+    const int s0 = P.isrc[j], s1 = s0 + P.nsrc[j];
+    for (int s = s0; s < s1; s++) acc.add(kre, kim, x, y, z, spos + 4 * (size_t)s, str + 8 * (size_t)s, false);
+  }
+  if (!active) return;
+  pot[2 * (size_t)me] += acc.a[0];
+  pot[2 * (size_t)me + 1] += acc.a[1];
+  if (FLD)
+    for (int k = 0; k < 6; k++) fld[6 * (size_t)me + k] += acc.a[2 + k];
+}
+
+// h3dpartdirect (src/hfmm3dpart.f:1268): every source on every point
+template <bool CHG, bool DIP>
+__global__ __launch_bounds__(256) void k_hdirect(double kre, double kim, int ns, const double* __restrict__ spos,
+                                                 const double* __restrict__ str, int npts,
+                                                 const double* __restrict__ pts, int pts_are_sources,
+                                                 double* __restrict__ pot, double* __restrict__ fld) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const bool active = i < npts;
+  const int me = active ? i : 0;
+  const double* mp = (pts_are_sources ? spos : pts) + 4 * (size_t)me;
+  const double x = mp[0], y = mp[1], z = mp[2];
+  HPairAcc<CHG, DIP, true> acc;
+  const int self = pts_are_sources ? me : -1;
+  for (int s = 0; s < ns; s++) acc.add(kre, kim, x, y, z, spos + 4 * (size_t)s, str + 8 * (size_t)s, s == self);
+  if (!active) return;
+  pot[2 * (size_t)i] = acc.a[0];
+  pot[2 * (size_t)i + 1] = acc.a[1];
+  for (int k = 0; k < 6; k++) fld[6 * (size_t)i + k] = acc.a[2 + k];
+}
+
+}  // namespace fmm3d

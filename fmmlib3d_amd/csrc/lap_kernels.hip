@@ -9,168 +9,9 @@
 // multipole M_n^m = sum q conj(R_n^m(u)), local L_n^m = sum q conj(S_n^m(u))/h.
 #include <hip/hip_runtime.h>
 
-#include "lap_kernels.hpp"
+#include "kern_common.hpp"
 
 namespace fmm3d {
-
-// ---------------------------------------------------------------------------
-// small complex helpers
-// ---------------------------------------------------------------------------
-template <typename T>
-struct C2 {
-  T re, im;
-};
-template <typename T>
-__device__ __forceinline__ C2<T> cmul(C2<T> a, C2<T> b) {
-  return {a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re};
-}
-template <typename T>
-__device__ __forceinline__ C2<T> cconj(C2<T> a) { return {a.re, -a.im}; }
-template <typename T>
-__device__ __forceinline__ void cfma(C2<T>& acc, C2<T> a, C2<T> b) {
-  acc.re += a.re * b.re - a.im * b.im;
-  acc.im += a.re * b.im + a.im * b.re;
-}
-template <typename T>
-__device__ __forceinline__ C2<T> cld(const T* p, int k) { return {p[2 * k], p[2 * k + 1]}; }
-template <typename T>
-__device__ __forceinline__ void cst(T* p, int k, C2<T> v) { p[2 * k] = v.re; p[2 * k + 1] = v.im; }
-template <typename T>
-__device__ __forceinline__ void cadd(T* p, int k, C2<T> v) { p[2 * k] += v.re; p[2 * k + 1] += v.im; }
-
-// Factor table layout (built on the host for an order p, doubles):
-//   [F_MM + m]      sqrt((2m-1)/(2m))          m=1..p+1
-//   [F_M1 + m]      sqrt(2m+1)                 m=0..p+1
-//   [F_RA + h(n,m)] sqrt((n+m-1)(n-m-1))       n>=m+2 (recurrence)
-//   [F_RB + h(n,m)] 1/sqrt((n-m)(n+m))
-//   [F_AR + e(n,m)], [F_BR..], [F_CR..]  derivative factors of R (n<=p+1)
-//   [F_AS + e(n,m)], [F_BS..], [F_CS..]  derivative factors of S (n<=p+1)
-struct FacLayout {
-  int MM, M1, RA, RB, AR, BR, CR, AS, BS, CS, total;
-  __host__ __device__ explicit FacLayout(int p) {
-    int P = p + 2;                       // tables cover degrees up to p+1
-    int H = P * (P + 1) / 2, E = P * P;
-    MM = 0; M1 = MM + P + 1; RA = M1 + P + 1; RB = RA + H; AR = RB + H;
-    BR = AR + E; CR = BR + E; AS = CR + E; BS = AS + E; CS = BS + E; total = CS + E;
-  }
-};
-
-// Regular solid harmonics R_n^m(u), 0<=m<=n<=pmax, for one point into a half
-// table (stride `st` between entries).  Column recurrence in n per m.
-template <typename T>
-__device__ __forceinline__ void regular_column(const double* fac, const FacLayout& L, int pmax,
-                                               int m, T ux, T uy, T uz, T r2, T* out, int st) {
-  // R_m^m = prod_k (-(ux+i uy)) sqrt((2k-1)/(2k))
-  C2<T> rmm = {T(1), T(0)};
-  for (int k = 1; k <= m; k++) {
-    T f = (T)fac[L.MM + k];
-    C2<T> w = {-ux * f, -uy * f};
-    rmm = cmul(w, rmm);
-  }
-  cst(out, hidx(m, m) * st / 1, rmm);
-  if (m + 1 > pmax) return;
-  C2<T> r1 = {(T)fac[L.M1 + m] * uz * rmm.re, (T)fac[L.M1 + m] * uz * rmm.im};
-  cst(out, hidx(m + 1, m) * st, r1);
-  C2<T> r0 = rmm;
-  for (int n = m + 2; n <= pmax; n++) {
-    T a = (T)(2 * n - 1) * uz, b = (T)fac[L.RA + hidx(n, m)] * r2, c = (T)fac[L.RB + hidx(n, m)];
-    C2<T> rn = {(a * r1.re - b * r0.re) * c, (a * r1.im - b * r0.im) * c};
-    cst(out, hidx(n, m) * st, rn);
-    r0 = r1;
-    r1 = rn;
-  }
-}
-
-// Irregular solid harmonics S_n^m(u) for 0<=m<=n<=pmax.
-template <typename T>
-__device__ __forceinline__ void irregular_column(const double* fac, const FacLayout& L, int pmax,
-                                                 int m, T ux, T uy, T uz, T ir2, T ir, T* out,
-                                                 int st) {
-  C2<T> smm = {ir, T(0)};
-  for (int k = 1; k <= m; k++) {
-    T f = (T)fac[L.MM + k] * ir2;
-    C2<T> w = {-ux * f, -uy * f};
-    smm = cmul(w, smm);
-  }
-  cst(out, hidx(m, m) * st, smm);
-  if (m + 1 > pmax) return;
-  T g = (T)fac[L.M1 + m] * uz * ir2;
-  C2<T> s1 = {g * smm.re, g * smm.im};
-  cst(out, hidx(m + 1, m) * st, s1);
-  C2<T> s0 = smm;
-  for (int n = m + 2; n <= pmax; n++) {
-    T a = (T)(2 * n - 1) * uz, b = (T)fac[L.RA + hidx(n, m)], c = (T)fac[L.RB + hidx(n, m)] * ir2;
-    C2<T> sn = {(a * s1.re - b * s0.re) * c, (a * s1.im - b * s0.im) * c};
-    cst(out, hidx(n, m) * st, sn);
-    s0 = s1;
-    s1 = sn;
-  }
-}
-
-// value of a harmonic with signed order from a half table
-template <typename T>
-__device__ __forceinline__ C2<T> half_get(const T* tab, int n, int m, int st) {
-  if (m >= 0) return cld(tab, hidx(n, m) * st);
-  return cconj(cld(tab, hidx(n, -m) * st));
-}
-
-// ---------------------------------------------------------------------------
-// gather / scatter / fill
-// ---------------------------------------------------------------------------
-
-// Sorted sources: positions in FP64 (x, y, z, 0) for every precision, and
-// strengths as 8 values (q.re q.im vx.re vx.im vy.re vy.im vz.re vz.im)
-// where v = dipstr * dipvec is the complex dipole moment vector; an FP32 copy
-// of the strengths is written when the near field runs in FP32.
-__global__ void k_gather_sources(int ns, const int* __restrict__ perm, const double* __restrict__ src,
-                                 const double* __restrict__ q, const double* __restrict__ ds,
-                                 const double* __restrict__ dv, int ifcharge, int ifdipole,
-                                 double* __restrict__ spos, double* __restrict__ str64,
-                                 float* __restrict__ str32) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= ns) return;
-  int o = perm[i];
-  double* x = spos + 4 * (size_t)i;
-  x[0] = src[3 * o]; x[1] = src[3 * o + 1]; x[2] = src[3 * o + 2]; x[3] = 0.0;
-  double v[8];
-  v[0] = ifcharge ? q[2 * o] : 0.0;
-  v[1] = ifcharge ? q[2 * o + 1] : 0.0;
-  for (int k = 0; k < 3; k++) {
-    double d = ifdipole ? dv[3 * o + k] : 0.0;
-    v[2 + 2 * k] = ifdipole ? ds[2 * o] * d : 0.0;
-    v[3 + 2 * k] = ifdipole ? ds[2 * o + 1] * d : 0.0;
-  }
-  for (int k = 0; k < 8; k++) str64[8 * (size_t)i + k] = v[k];
-  if (str32)
-    for (int k = 0; k < 8; k++) str32[8 * (size_t)i + k] = (float)v[k];
-}
-
-__global__ void k_gather_targets(int nt, const int* __restrict__ perm, const double* __restrict__ trg,
-                                 double* __restrict__ tpos) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= nt) return;
-  int o = perm[i];
-  tpos[4 * (size_t)i] = trg[3 * o];
-  tpos[4 * (size_t)i + 1] = trg[3 * o + 1];
-  tpos[4 * (size_t)i + 2] = trg[3 * o + 2];
-  tpos[4 * (size_t)i + 3] = 0.0;
-}
-
-// out[perm[i]] = sorted[i]; `w` complex values per point (1 = pot, 3 = fld)
-__global__ void k_scatter(int n, int w, const int* __restrict__ perm, const double* __restrict__ sorted,
-                          double* __restrict__ out) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  int o = perm[i];
-  for (int k = 0; k < 2 * w; k++) out[(size_t)2 * w * o + k] = sorted[(size_t)2 * w * i + k];
-}
-
-template <typename T>
-__global__ void k_fill_zero(size_t n, T* __restrict__ a) {
-  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  size_t stride = (size_t)gridDim.x * blockDim.x;
-  for (; i < n; i += stride) a[i] = T(0);
-}
 
 // ---------------------------------------------------------------------------
 // P2M (src/laprouts3d.f:3210, :3835) and P2L (src/laprouts3d.f:3550, :4207)
@@ -440,31 +281,6 @@ __global__ void k_eval(DevPlan P, const double* __restrict__ fac, const T* __res
 // src/l3dtrans.f:223 (M2M), :868 (M2L), :1676 (L2L) do the same sequence.
 // ZMODE 0: Z[m][out][in] ; the same dense (p+1)^3 layout for every shift.
 // ---------------------------------------------------------------------------
-template <typename T>
-__device__ void rotate_block(const DevTables& Tb, int rc, int pmax, const T* in, T* out, bool back) {
-  const int p = Tb.p;
-  const int Kt = (pmax + 1) * (pmax + 1);
-  for (int k = threadIdx.x; k < Kt; k += blockDim.x) {
-    int n = (int)sqrtf((float)k);
-    while (n * n > k) n--;
-    while ((n + 1) * (n + 1) <= k) n++;
-    int mp = k - n * n - n;
-    const double* D = Tb.rot + Tb.rot_off[rc * (p + 1) + n];
-    const int dim = 2 * n + 1;
-    T are = 0, aim = 0;
-    for (int m = -n; m <= n; m++) {
-      T d = (T)D[(m + n) * dim + (mp + n)];
-      T xr = in[2 * eidx(n, m)], xi = in[2 * eidx(n, m) + 1];
-      if (back && (m & 1)) { xr = -xr; xi = -xi; }
-      are += d * xr;
-      aim += d * xi;
-    }
-    if (back && (mp & 1)) { are = -are; aim = -aim; }
-    out[2 * k] = are;
-    out[2 * k + 1] = aim;
-  }
-}
-
 template <typename T>
 __device__ void zshift_block(const double* Z, int p, int pin, int pout, const T* in, T* out) {
   const int Kt = (pout + 1) * (pout + 1);

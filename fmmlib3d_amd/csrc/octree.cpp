@@ -1,11 +1,10 @@
 // Host octree builder; see octree.hpp.  The tree is identical, box for box, to
 // the one d3tstrcr builds (src/d3tstrcr.f), but the construction is ours:
-// every level is split in one parallel pass over the boxes that exceed nbox
-// (each box does a stable 8-way counting partition of its own point range,
-// large boxes split their range across threads), child ids are assigned by a
-// prefix over the level, and the interaction lists are derived from integer
-// box coordinates and a per-level hash of (i,j,k) instead of d3tstrcr's
-// floating-point corner tests and linked-list store.
+// up to four levels are resolved per pass over the points (octant digit
+// codes, a histogram that decides every split, one stable counting sort),
+// child ids are assigned breadth first, and the interaction lists are
+// derived from integer box coordinates and a per-level hash of (i,j,k)
+// instead of d3tstrcr's floating-point corner tests and linked-list store.
 #include "octree.hpp"
 
 #include <omp.h>
@@ -19,23 +18,20 @@ namespace fmm3d {
 
 double Octree::side(int l) const { return size / std::ldexp(1.0, l); }
 
-namespace {
-
-struct PointSet {
-  std::vector<double> x, y, z;
-  std::vector<int> perm;
-  std::vector<double> tx, ty, tz;
-  std::vector<int> tperm;
-  std::vector<uint8_t> oct;
-  void init(const double* p, int n) {
-    x.resize(n); y.resize(n); z.resize(n); perm.resize(n);
-    tx.resize(n); ty.resize(n); tz.resize(n); tperm.resize(n); oct.resize(n);
-#pragma omp parallel for schedule(static)
-    for (int i = 0; i < n; i++) {
-      x[i] = p[3 * i]; y[i] = p[3 * i + 1]; z[i] = p[3 * i + 2]; perm[i] = i;
-    }
+void Octree::reset() {
+  nboxes = nlev = 0;
+  center[0] = center[1] = center[2] = 0.0;
+  size = 0.0;
+  for (auto* v : {&level, &parent, &nchild, &child, &ijk, &isrc, &nsrc, &itrg, &ntrg, &ladd, &src_perm, &trg_perm})
+    v->clear();
+  bcenter.clear();
+  for (auto& c : list) {
+    c.off.clear();
+    c.idx.clear();
   }
-};
+}
+
+namespace {
 
 // Centre of box (i,j,k) at `level`: the same arithmetic as d3tcentf
 // (d3tstrcr.f:1675-1700), kept free of FMA contraction so that ties at the
@@ -48,70 +44,53 @@ inline double box_center_1d(double c0, double size, int level, int i) {
   return x0 + (i - 1) * side + side2;
 }
 
-inline uint8_t octant(double x, double y, double z, const double* c) {
-  return (uint8_t)((x > c[0]) | ((y > c[1]) << 1) | ((z > c[2]) << 2));
-}
+// One point in the partition workspace: position, original index, and the
+// octant digits below the box being split.
+struct Pt {
+  double x, y, z;
+  int idx;
+  uint32_t code;
+};
 
-// Stable 8-way partition of [a, a+n) by octant; counts returned in cnt.
-void partition_serial(PointSet& P, int a, int n, const double* c, int* cnt) {
-  for (int k = 0; k < 8; k++) cnt[k] = 0;
-  for (int i = a; i < a + n; i++) {
-    uint8_t o = octant(P.x[i], P.y[i], P.z[i], c);
-    P.oct[i] = o;
-    cnt[o]++;
-  }
-  int off[8], s = a;
-  for (int k = 0; k < 8; k++) { off[k] = s; s += cnt[k]; }
-  for (int i = a; i < a + n; i++) {
-    int d = off[P.oct[i]]++;
-    P.tx[d] = P.x[i]; P.ty[d] = P.y[i]; P.tz[d] = P.z[i]; P.tperm[d] = P.perm[i];
-  }
-  std::memcpy(&P.x[a], &P.tx[a], sizeof(double) * n);
-  std::memcpy(&P.y[a], &P.ty[a], sizeof(double) * n);
-  std::memcpy(&P.z[a], &P.tz[a], sizeof(double) * n);
-  std::memcpy(&P.perm[a], &P.tperm[a], sizeof(int) * n);
-}
+// Persistent per-thread workspace (no page faults after the first call).
+struct Work {
+  std::vector<Pt> a, b;
+};
+thread_local Work g_src, g_trg;
 
-// Same result as partition_serial, with the range split across threads:
-// per-chunk counts, an (octant, chunk)-major prefix, then a stable scatter.
-void partition_parallel(PointSet& P, int a, int n, const double* c, int* cnt) {
-  int T = omp_get_max_threads();
-  std::vector<int> counts(8 * T, 0);
+// Stable counting sort of a[lo, lo+n) into b[lo, lo+n) by key(p) < nbins,
+// split over `T` threads (chunk histograms, bin-major prefix, stable scatter).
+template <typename Key>
+void stable_bin_sort(const Pt* a, Pt* b, int lo, int n, int nbins, int T, const Key& key) {
+  if (n <= 0) return;
+  if (T <= 1 || n < (1 << 15)) {
+    std::vector<int> off(nbins + 1, 0);
+    for (int i = lo; i < lo + n; i++) off[key(a[i]) + 1]++;
+    off[0] = lo;
+    for (int k = 0; k < nbins; k++) off[k + 1] += off[k];
+    for (int i = lo; i < lo + n; i++) b[off[key(a[i])]++] = a[i];
+    return;
+  }
+  std::vector<int> h((size_t)T * nbins, 0);
 #pragma omp parallel num_threads(T)
   {
-    int t = omp_get_thread_num();
-    int lo = a + (int)((long)n * t / T), hi = a + (int)((long)n * (t + 1) / T);
-    int* ct = &counts[8 * t];
-    for (int i = lo; i < hi; i++) {
-      uint8_t o = octant(P.x[i], P.y[i], P.z[i], c);
-      P.oct[i] = o;
-      ct[o]++;
-    }
+    const int t = omp_get_thread_num();
+    const int c0 = lo + (int)((long)n * t / T), c1 = lo + (int)((long)n * (t + 1) / T);
+    int* ht = &h[(size_t)t * nbins];
+    for (int i = c0; i < c1; i++) ht[key(a[i])]++;
 #pragma omp barrier
 #pragma omp single
     {
-      int s = a;
-      for (int k = 0; k < 8; k++)
+      int s = lo;
+      for (int k = 0; k < nbins; k++)
         for (int tt = 0; tt < T; tt++) {
-          int v = counts[8 * tt + k];
-          counts[8 * tt + k] = s;
+          int v = h[(size_t)tt * nbins + k];
+          h[(size_t)tt * nbins + k] = s;
           s += v;
         }
     }
-    int off[8];
-    for (int k = 0; k < 8; k++) off[k] = ct[k];
-    for (int i = lo; i < hi; i++) {
-      int d = off[P.oct[i]]++;
-      P.tx[d] = P.x[i]; P.ty[d] = P.y[i]; P.tz[d] = P.z[i]; P.tperm[d] = P.perm[i];
-    }
-#pragma omp barrier
-#pragma omp for schedule(static)
-    for (int i = a; i < a + n; i++) {
-      P.x[i] = P.tx[i]; P.y[i] = P.ty[i]; P.z[i] = P.tz[i]; P.perm[i] = P.tperm[i];
-    }
+    for (int i = c0; i < c1; i++) b[ht[key(a[i])]++] = a[i];
   }
-  for (int k = 0; k < 8; k++) cnt[k] = 0;
-  for (int i = a; i < a + n; i++) cnt[P.oct[i]]++;
 }
 
 inline uint64_t ijk_key(int i, int j, int k) {
@@ -145,7 +124,7 @@ void flatten(std::vector<std::vector<int>>& per, Csr& c) {
 }  // namespace
 
 int build_octree(const double* src, int ns, const double* trg, int nt, int nbox, Octree& t) {
-  t = Octree();
+  t.reset();
   if (ns < 1) return 128;
   if (nt < 0) nt = 0;
   // bounding cube of sources and targets (d3tstrcr.f:1357-1395)
@@ -168,9 +147,19 @@ int build_octree(const double* src, int ns, const double* trg, int nt, int nbox,
   for (int d = 0; d < 3; d++) t.center[d] = (lo[d] + hi[d]) / 2;
   t.size = size;
 
-  PointSet S, T;
-  S.init(src, ns);
-  T.init(trg, nt);
+  // workspace: positions with their original index
+  Work& WS = g_src;
+  Work& WT = g_trg;
+  if ((int)WS.a.size() < ns) { WS.a.resize(ns); WS.b.resize(ns); }
+  if ((int)WT.a.size() < nt) { WT.a.resize(nt); WT.b.resize(nt); }
+  Pt* SA = WS.a.data();
+  Pt* SB = WS.b.data();
+  Pt* TA = WT.a.data();
+  Pt* TB = WT.b.data();
+#pragma omp parallel for schedule(static)
+  for (int i = 0; i < ns; i++) SA[i] = Pt{src[3 * i], src[3 * i + 1], src[3 * i + 2], i, 0u};
+#pragma omp parallel for schedule(static)
+  for (int i = 0; i < nt; i++) TA[i] = Pt{trg[3 * i], trg[3 * i + 1], trg[3 * i + 2], i, 0u};
 
   auto push_box = [&](int level, int i, int j, int k, int par, int is, int n_s, int it, int n_t) {
     t.level.push_back(level); t.parent.push_back(par); t.nchild.push_back(0);
@@ -183,57 +172,192 @@ int build_octree(const double* src, int ns, const double* trg, int nt, int nbox,
   };
   push_box(0, 1, 1, 1, -1, 0, ns, 0, nt);
   t.ladd = {0, 1};
-  int nthreads = omp_get_max_threads();
-  for (int level = 0; level < 100; level++) {
-    int first = t.ladd[level], last = t.ladd[level + 1];
-    std::vector<int> split;
-    for (int b = first; b < last; b++)
-      if (t.nsrc[b] > nbox || t.ntrg[b] > nbox) split.push_back(b);
-    if (split.empty()) break;
-    int ns_split = (int)split.size();
-    std::vector<int> cs(8 * ns_split), ct(8 * ns_split);
-    auto do_box = [&](int q, bool par) {
-      int b = split[q];
-      const double* c = &t.bcenter[3 * b];
-      if (par) {
-        partition_parallel(S, t.isrc[b], t.nsrc[b], c, &cs[8 * q]);
-        partition_parallel(T, t.itrg[b], t.ntrg[b], c, &ct[8 * q]);
-      } else {
-        partition_serial(S, t.isrc[b], t.nsrc[b], c, &cs[8 * q]);
-        partition_serial(T, t.itrg[b], t.ntrg[b], c, &ct[8 * q]);
+  const int nthreads = omp_get_max_threads();
+  constexpr int kMaxLevel = 100;
+  std::vector<int> frontier;
+  if (ns > nbox || nt > nbox) frontier.push_back(0);
+  int L = 0;  // level of the frontier boxes
+  // Each pass resolves up to D levels below the frontier at once: every point
+  // gets the D octant digits of its path (the same centre arithmetic and
+  // strict ">" test as one split per level), a histogram of the digit codes
+  // gives every sub-box's counts and hence which sub-boxes split, and one
+  // stable counting sort by the code truncated at the point's leaf puts the
+  // points in the order D successive stable 8-way partitions would.
+  while (!frontier.empty() && L < kMaxLevel) {
+    const int nf = (int)frontier.size();
+    int D = 4;
+    while (D > 1 && (long)nf << (3 * D) > (1L << 22)) D--;
+    D = std::min(D, kMaxLevel - L);
+    const int NB = 1 << (3 * D);
+    // centre tables per frontier box: ctab[f][dim][depth d][sub-index]
+    const int CT = (1 << D) * 2;  // sum_{d<D} 2^d < 2^D, padded
+    std::vector<double> ctab((size_t)nf * 3 * D * CT);
+    for (int f = 0; f < nf; f++) {
+      const int b = frontier[f];
+      for (int dim = 0; dim < 3; dim++)
+        for (int d = 0; d < D; d++)
+          for (int ii = 0; ii < (1 << d); ii++)
+            ctab[(((size_t)f * 3 + dim) * D + d) * CT + ii] =
+                box_center_1d(t.center[dim], size, L + d, (t.ijk[3 * b + dim] - 1) * (1 << d) + 1 + ii);
+    }
+    auto code_of = [&](const Pt& p, int f) -> uint32_t {
+      const double* cx = &ctab[(((size_t)f * 3 + 0) * D) * CT];
+      const double* cy = &ctab[(((size_t)f * 3 + 1) * D) * CT];
+      const double* cz = &ctab[(((size_t)f * 3 + 2) * D) * CT];
+      uint32_t code = 0;
+      int ii = 0, jj = 0, kk = 0;
+      for (int d = 0; d < D; d++) {
+        const int o = (p.x > cx[d * CT + ii]) | ((p.y > cy[d * CT + jj]) << 1) | ((p.z > cz[d * CT + kk]) << 2);
+        code = (code << 3) | (uint32_t)o;
+        ii = 2 * ii + (o & 1);
+        jj = 2 * jj + ((o >> 1) & 1);
+        kk = 2 * kk + ((o >> 2) & 1);
       }
+      return code;
     };
-    if (ns_split < nthreads) {
-      for (int q = 0; q < ns_split; q++) {
-        bool big = (t.nsrc[split[q]] + t.ntrg[split[q]]) > (1 << 16);
-        do_box(q, big);
+    // digit codes and per-box histograms (exclusive prefix sums)
+    std::vector<int> hs((size_t)nf * (NB + 1), 0), ht((size_t)nf * (NB + 1), 0);
+    auto codes_and_hist = [&](Pt* P, int lo, int n, int f, int* h, bool par) {
+      if (par) {
+        std::vector<int> part((size_t)nthreads * NB, 0);
+#pragma omp parallel num_threads(nthreads)
+        {
+          const int th = omp_get_thread_num();
+          const int c0 = lo + (int)((long)n * th / nthreads), c1 = lo + (int)((long)n * (th + 1) / nthreads);
+          int* pt = &part[(size_t)th * NB];
+          for (int i = c0; i < c1; i++) {
+            P[i].code = code_of(P[i], f);
+            pt[P[i].code]++;
+          }
+        }
+        for (int th = 0; th < nthreads; th++)
+          for (int k = 0; k < NB; k++) h[k + 1] += part[(size_t)th * NB + k];
+      } else {
+        for (int i = lo; i < lo + n; i++) {
+          P[i].code = code_of(P[i], f);
+          h[P[i].code + 1]++;
+        }
+      }
+      for (int k = 0; k < NB; k++) h[k + 1] += h[k];
+    };
+    const bool few = nf < 2 * nthreads;
+    if (few) {
+      for (int f = 0; f < nf; f++) {
+        const int b = frontier[f];
+        codes_and_hist(SA, t.isrc[b], t.nsrc[b], f, &hs[(size_t)f * (NB + 1)], t.nsrc[b] > (1 << 15));
+        codes_and_hist(TA, t.itrg[b], t.ntrg[b], f, &ht[(size_t)f * (NB + 1)], t.ntrg[b] > (1 << 15));
       }
     } else {
 #pragma omp parallel for schedule(dynamic, 1)
-      for (int q = 0; q < ns_split; q++) do_box(q, false);
-    }
-    // children in (box, octant) order; empty ones dropped (d3tstrcr.f:1529)
-    for (int q = 0; q < ns_split; q++) {
-      int b = split[q];
-      int os = t.isrc[b], ot = t.itrg[b];
-      int bi = t.ijk[3 * b], bj = t.ijk[3 * b + 1], bk = t.ijk[3 * b + 2];
-      for (int o = 0; o < 8; o++) {
-        int a = cs[8 * q + o], e = ct[8 * q + o];
-        if (a == 0 && e == 0) continue;
-        int id = (int)t.level.size();
-        push_box(level + 1, 2 * (bi - 1) + 1 + (o & 1), 2 * (bj - 1) + 1 + ((o >> 1) & 1),
-                 2 * (bk - 1) + 1 + ((o >> 2) & 1), b, os, a, ot, e);
-        t.child[8 * b + t.nchild[b]++] = id;
-        os += a;
-        ot += e;
+      for (int f = 0; f < nf; f++) {
+        const int b = frontier[f];
+        codes_and_hist(SA, t.isrc[b], t.nsrc[b], f, &hs[(size_t)f * (NB + 1)], false);
+        codes_and_hist(TA, t.itrg[b], t.ntrg[b], f, &ht[(size_t)f * (NB + 1)], false);
       }
     }
-    t.ladd.push_back((int)t.level.size());
-    t.nlev = level + 1;
+    // sub-boxes, level by level across all frontier boxes (breadth-first ids);
+    // node (d, prefix) of frontier box f: id and whether it splits further
+    const int NS = NB * 2;  // sum_{d<=D} 8^d < 2*8^D, stored at [pos(d) + prefix]
+    auto pos = [&](int d) { int o = 0; for (int e = 1; e < d; e++) o += 1 << (3 * e); return o; };
+    std::vector<int> node((size_t)nf * NS, -1);
+    std::vector<char> splits((size_t)nf * NS, 0);
+    std::vector<int> next;
+    for (int d = 1; d <= D; d++) {
+      const int sh = 3 * (D - d), pd = pos(d), pp = pos(d - 1);
+      for (int f = 0; f < nf; f++) {
+        const int fb = frontier[f];
+        const int* hsf = &hs[(size_t)f * (NB + 1)];
+        const int* htf = &ht[(size_t)f * (NB + 1)];
+        const int nparents = d == 1 ? 1 : 1 << (3 * (d - 1));
+        for (int P0 = 0; P0 < nparents; P0++) {
+          int par;
+          if (d == 1) par = fb;
+          else {
+            if (!splits[(size_t)f * NS + pp + P0]) continue;
+            par = node[(size_t)f * NS + pp + P0];
+          }
+          const int bi = t.ijk[3 * par], bj = t.ijk[3 * par + 1], bk = t.ijk[3 * par + 2];
+          for (int o = 0; o < 8; o++) {
+            const int P = P0 * 8 + o;
+            const int s0 = hsf[P << sh], s1 = hsf[(P + 1) << sh];
+            const int u0 = htf[P << sh], u1 = htf[(P + 1) << sh];
+            const int a = s1 - s0, e = u1 - u0;
+            if (a == 0 && e == 0) continue;
+            const int id = (int)t.level.size();
+            push_box(L + d, 2 * (bi - 1) + 1 + (o & 1), 2 * (bj - 1) + 1 + ((o >> 1) & 1),
+                     2 * (bk - 1) + 1 + ((o >> 2) & 1), par, t.isrc[fb] + s0, a, t.itrg[fb] + u0, e);
+            t.child[8 * par + t.nchild[par]++] = id;
+            node[(size_t)f * NS + pd + P] = id;
+            if (a > nbox || e > nbox) {
+              if (d < D) splits[(size_t)f * NS + pd + P] = 1;
+              else next.push_back(id);
+            }
+          }
+        }
+      }
+      t.ladd.push_back((int)t.level.size());
+      t.nlev = L + d;
+      bool more = false;
+      for (int f = 0; f < nf && !more; f++)
+        for (int P = 0; P < (1 << (3 * d)) && !more; P++) more = splits[(size_t)f * NS + pd + P] != 0;
+      if (!more && d < D) {
+        // nothing below this depth: the codes are truncated here for every point
+        D = d;
+        break;
+      }
+    }
+    // stable sort of each frontier box's points by the code truncated at
+    // the depth where the point's box stops splitting
+    const int Dk = D;
+    int posd[8];
+    for (int d = 0; d <= Dk; d++) posd[d] = pos(d);
+    auto eff = [&](uint32_t code, int f, int Dfull) -> uint32_t {
+      // Dfull: digits stored in the code; Dk: depth resolved in this pass
+      const uint32_t c = code >> (3 * (Dfull - Dk));
+      const char* sp = &splits[(size_t)f * NS];
+      for (int d = 1; d < Dk; d++) {
+        const uint32_t pre = c >> (3 * (Dk - d));
+        if (!sp[posd[d] + pre]) return pre << (3 * (Dk - d));
+      }
+      return c;
+    };
+    const int Dfull = [&] { int x = 4; while (x > 1 && (long)nf << (3 * x) > (1L << 22)) x--; return std::min(x, kMaxLevel - L); }();
+    const int NBk = 1 << (3 * Dk);
+    auto sort_box = [&](Pt* A, Pt* Bf, int lo, int n, int f, int T) {
+      stable_bin_sort(A, Bf, lo, n, NBk, T, [&](const Pt& p) { return (int)eff(p.code, f, Dfull); });
+      std::memcpy(A + lo, Bf + lo, sizeof(Pt) * (size_t)n);
+    };
+    if (nf == 1 && frontier[0] == 0) {
+      // the root: every point moves, so sort into the second buffer and swap
+      stable_bin_sort(SA, SB, 0, ns, NBk, nthreads, [&](const Pt& p) { return (int)eff(p.code, 0, Dfull); });
+      stable_bin_sort(TA, TB, 0, nt, NBk, nthreads, [&](const Pt& p) { return (int)eff(p.code, 0, Dfull); });
+      std::swap(SA, SB);
+      std::swap(TA, TB);
+    } else if (few) {
+      for (int f = 0; f < nf; f++) {
+        const int b = frontier[f];
+        sort_box(SA, SB, t.isrc[b], t.nsrc[b], f, nthreads);
+        sort_box(TA, TB, t.itrg[b], t.ntrg[b], f, nthreads);
+      }
+    } else {
+#pragma omp parallel for schedule(dynamic, 1)
+      for (int f = 0; f < nf; f++) {
+        const int b = frontier[f];
+        sort_box(SA, SB, t.isrc[b], t.nsrc[b], f, 1);
+        sort_box(TA, TB, t.itrg[b], t.ntrg[b], f, 1);
+      }
+    }
+    frontier.swap(next);
+    next.clear();
+    L += D;
   }
   t.nboxes = (int)t.level.size();
-  t.src_perm.swap(S.perm);
-  t.trg_perm.swap(T.perm);
+  t.src_perm.resize(ns);
+  t.trg_perm.resize(nt);
+#pragma omp parallel for schedule(static)
+  for (int i = 0; i < ns; i++) t.src_perm[i] = SA[i].idx;
+#pragma omp parallel for schedule(static)
+  for (int i = 0; i < nt; i++) t.trg_perm[i] = TA[i].idx;
   const int nb = t.nboxes;
 
   // colleagues through a per-level hash of (i,j,k)

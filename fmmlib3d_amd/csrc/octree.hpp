@@ -31,6 +31,8 @@ struct Octree {
   Csr list[4];
 
   bool is_leaf(int b) const { return nchild[b] == 0; }
+  // empties every array but keeps its storage (repeated builds reuse it)
+  void reset();
   double side(int l) const;
 };
 

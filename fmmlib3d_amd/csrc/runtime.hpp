@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <map>
+#include <memory>
 #include <mutex>
 #include <stdexcept>
 #include <string>
@@ -58,6 +59,8 @@ struct Context {
   std::vector<double> tab_host_d;
   std::vector<long> tab_host_l;
   size_t tab_off[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  // cached Helmholtz tables (owned by the Helmholtz driver)
+  std::shared_ptr<void> helm_tables;
 };
 
 Context& default_context();

@@ -102,6 +102,50 @@ struct SqrtBinom {
 };
 
 void build_tables(int p, double eps, HostTables& H) {
+  build_geometry_tables(p, H);
+  const int P1 = p + 1;
+  const std::vector<int> itab = laplace_itable(eps);
+  std::vector<double> dists;
+  for (int code = 0; code < 343; code++) H.off_ptrunc[code] = 0;
+  for (int i = -3; i <= 3; i++)
+    for (int j = -3; j <= 3; j++)
+      for (int k = -3; k <= 3; k++) {
+        int code = ((i + 3) * 7 + (j + 3)) * 7 + (k + 3);
+        if (std::max(std::abs(i), std::max(std::abs(j), std::abs(k))) < 2) continue;
+        H.off_ptrunc[code] = std::min(p, itab[code]);
+      }
+  for (int dc = 0; dc < H.ndist; dc++) dists.push_back(H.dist_value[dc]);
+  // z-shift tables, dense [m][out][in]
+  const SqrtBinom sqrt_binom(2 * p + 2);
+  const size_t Z = (size_t)P1 * P1 * P1;
+  H.z_m2l.assign(Z * H.ndist, 0.0);
+  for (int dc = 0; dc < H.ndist; dc++) {
+    double rho = dists[dc];
+    double* z = &H.z_m2l[Z * dc];
+    for (int m = 0; m <= p; m++)
+      for (int l = m; l <= p; l++)
+        for (int n = m; n <= p; n++) {
+          double c = sqrt_binom(n + l, n - m) * sqrt_binom(n + l, n + m) * std::pow(rho, -(n + l + 1));
+          if ((n + m) & 1) c = -c;
+          z[((size_t)m * P1 + l) * P1 + n] = c;
+        }
+  }
+  H.z_m2m.assign(Z, 0.0);
+  H.z_l2l.assign(Z, 0.0);
+  const double r = std::sqrt(3.0) / 4.0;
+  for (int m = 0; m <= p; m++)
+    for (int n = m; n <= p; n++)
+      for (int l = m; l <= n; l++) {
+        double c = sqrt_binom(n + m, l + m) * sqrt_binom(n - m, l - m) * std::ldexp(1.0, -l);
+        // parent degree n from child degree l (M2M), child degree l from parent degree n (L2L)
+        H.z_m2m[((size_t)m * P1 + n) * P1 + l] = c * std::pow(-r, n - l);
+        H.z_l2l[((size_t)m * P1 + l) * P1 + n] = c * std::pow(r, n - l);
+      }
+}
+
+// Factor table, rotation classes, rotation blocks and phases for order p:
+// the kernel-independent half of the translation tables.
+void build_geometry_tables(int p, HostTables& H) {
   H = HostTables();
   H.p = p;
   const int P1 = p + 1;
@@ -143,7 +187,6 @@ void build_tables(int p, double eps, HostTables& H) {
     thetas.push_back(th);
     return c;
   };
-  const std::vector<int> itab = laplace_itable(eps);
   std::map<long long, int> dist_class;
   std::vector<double> dists;
   H.off_rot.assign(343, 0);
@@ -168,7 +211,6 @@ void build_tables(int p, double eps, HostTables& H) {
         } else {
           H.off_dist[code] = it->second;
         }
-        H.off_ptrunc[code] = std::min(p, itab[code]);
         for (int m = 0; m <= p; m++) {
           H.off_phase[(size_t)code * 2 * P1 + 2 * m] = std::cos(m * ph);
           H.off_phase[(size_t)code * 2 * P1 + 2 * m + 1] = std::sin(m * ph);
@@ -214,33 +256,8 @@ void build_tables(int p, double eps, HostTables& H) {
     }
   }
   H.nrot = nrot;
-  // z-shift tables, dense [m][out][in]
-  const SqrtBinom sqrt_binom(2 * p + 2);
-  const size_t Z = (size_t)P1 * P1 * P1;
   H.ndist = (int)dists.size();
-  H.z_m2l.assign(Z * H.ndist, 0.0);
-  for (int dc = 0; dc < H.ndist; dc++) {
-    double rho = dists[dc];
-    double* z = &H.z_m2l[Z * dc];
-    for (int m = 0; m <= p; m++)
-      for (int l = m; l <= p; l++)
-        for (int n = m; n <= p; n++) {
-          double c = sqrt_binom(n + l, n - m) * sqrt_binom(n + l, n + m) * std::pow(rho, -(n + l + 1));
-          if ((n + m) & 1) c = -c;
-          z[((size_t)m * P1 + l) * P1 + n] = c;
-        }
-  }
-  H.z_m2m.assign(Z, 0.0);
-  H.z_l2l.assign(Z, 0.0);
-  const double r = std::sqrt(3.0) / 4.0;
-  for (int m = 0; m <= p; m++)
-    for (int n = m; n <= p; n++)
-      for (int l = m; l <= n; l++) {
-        double c = sqrt_binom(n + m, l + m) * sqrt_binom(n - m, l - m) * std::ldexp(1.0, -l);
-        // parent degree n from child degree l (M2M), child degree l from parent degree n (L2L)
-        H.z_m2m[((size_t)m * P1 + n) * P1 + l] = c * std::pow(-r, n - l);
-        H.z_l2l[((size_t)m * P1 + l) * P1 + n] = c * std::pow(r, n - l);
-      }
+  H.dist_value = dists;
 }
 
 }  // namespace fmm3d

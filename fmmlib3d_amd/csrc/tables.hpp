@@ -14,6 +14,7 @@ struct HostTables {
   std::vector<double> rot;                         // rotation blocks
   std::vector<long> rot_off;                       // nrot x (p+1)
   std::vector<double> z_m2l, z_m2m, z_l2l;         // dense [m][out][in]
+  std::vector<double> dist_value;                  // |offset| of each distance class
 };
 
 // l3dterms (src/l3dterms.f:34): expansion order for tolerance eps
@@ -21,5 +22,8 @@ int laplace_nterms(double eps);
 // l3dterms_list2 (src/l3dterms.f:136): 7^3 table indexed by offset code
 std::vector<int> laplace_itable(double eps);
 void build_tables(int p, double eps, HostTables& H);
+// the kernel-independent part (factors, rotation classes and blocks, phases,
+// distance classes) that build_tables starts from
+void build_geometry_tables(int p, HostTables& H);
 
 }  // namespace fmm3d

@@ -1,8 +1,9 @@
 ! fmm3d_mi355x.f90 -- ISO_C_BINDING drop-in for fmmlib3d's particle FMM.
 !
 ! Exports the reference's F77 entry points with their exact calling
-! sequences (src/lfmm3dpart.f:36-141, :1391-1394): lfmm3dpart,
-! lfmm3dpartself, lfmm3dparttarg and l3dpartdirect.  Each one forwards its
+! sequences (src/lfmm3dpart.f:36-141, :1391-1394, src/hfmm3dpart.f:39-144,
+! :1268-1271): lfmm3dpart, lfmm3dpartself, lfmm3dparttarg, l3dpartdirect,
+! hfmm3dpart, hfmm3dpartself, hfmm3dparttarg and h3dpartdirect.  Each one forwards its
 ! arguments to the C ABI of libfmm3d_mi355x.so (include/fmm3d_mi355x.h), so a
 ! Fortran program links against this library instead of the reference
 ! objects and runs the FMM on the GPU.  Arrays keep the reference layouts and
@@ -30,6 +31,28 @@ module fmm3d_mi355x_c
       real(c_double) :: source(*), dipvec(*), target(*)
       complex(c_double_complex) :: charge(*), dipstr(*), pot(*), fld(*), pottarg(*), fldtarg(*)
     end subroutine c_l3dpartdirect
+
+    integer(c_int) function c_hfmm3dparttarg(iprec, zk, nsource, source, ifcharge, charge, &
+        ifdipole, dipstr, dipvec, ifpot, pot, iffld, fld, ntarget, target, ifpottarg, &
+        pottarg, iffldtarg, fldtarg) bind(C, name="fmm3d_hfmm3dparttarg")
+      import :: c_int, c_double, c_double_complex
+      integer(c_int), value :: iprec, nsource, ifcharge, ifdipole, ifpot, iffld, ntarget
+      integer(c_int), value :: ifpottarg, iffldtarg
+      complex(c_double_complex) :: zk
+      real(c_double) :: source(*), dipvec(*), target(*)
+      complex(c_double_complex) :: charge(*), dipstr(*), pot(*), fld(*), pottarg(*), fldtarg(*)
+    end function c_hfmm3dparttarg
+
+    subroutine c_h3dpartdirect(zk, nsource, source, ifcharge, charge, ifdipole, dipstr, dipvec, &
+        ifpot, pot, iffld, fld, ntarget, target, ifpottarg, pottarg, iffldtarg, fldtarg) &
+        bind(C, name="fmm3d_h3dpartdirect")
+      import :: c_int, c_double, c_double_complex
+      integer(c_int), value :: nsource, ifcharge, ifdipole, ifpot, iffld, ntarget
+      integer(c_int), value :: ifpottarg, iffldtarg
+      complex(c_double_complex) :: zk
+      real(c_double) :: source(*), dipvec(*), target(*)
+      complex(c_double_complex) :: charge(*), dipstr(*), pot(*), fld(*), pottarg(*), fldtarg(*)
+    end subroutine c_h3dpartdirect
   end interface
 end module fmm3d_mi355x_c
 
@@ -75,3 +98,50 @@ subroutine l3dpartdirect(nsource, source, ifcharge, charge, ifdipole, dipstr, di
   call c_l3dpartdirect(nsource, source, ifcharge, charge, ifdipole, dipstr, dipvec, &
       ifpot, pot, iffld, fld, ntarget, target, ifpottarg, pottarg, iffldtarg, fldtarg)
 end subroutine l3dpartdirect
+
+subroutine hfmm3dparttarg(ier, iprec, zk, nsource, source, ifcharge, charge, ifdipole, dipstr, &
+    dipvec, ifpot, pot, iffld, fld, ntarget, target, ifpottarg, pottarg, iffldtarg, fldtarg)
+  use fmm3d_mi355x_c
+  implicit none
+  integer :: ier, iprec, nsource, ifcharge, ifdipole, ifpot, iffld, ntarget, ifpottarg, iffldtarg
+  complex(8) :: zk
+  real(8) :: source(*), dipvec(*), target(*)
+  complex(8) :: charge(*), dipstr(*), pot(*), fld(*), pottarg(*), fldtarg(*)
+  ier = c_hfmm3dparttarg(iprec, zk, nsource, source, ifcharge, charge, ifdipole, dipstr, dipvec, &
+      ifpot, pot, iffld, fld, ntarget, target, ifpottarg, pottarg, iffldtarg, fldtarg)
+end subroutine hfmm3dparttarg
+
+subroutine hfmm3dpartself(ier, iprec, zk, nsource, source, ifcharge, charge, ifdipole, dipstr, &
+    dipvec, ifpot, pot, iffld, fld)
+  use fmm3d_mi355x_c
+  implicit none
+  integer :: ier, iprec, nsource, ifcharge, ifdipole, ifpot, iffld
+  complex(8) :: zk
+  real(8) :: source(*), dipvec(*), dummy(3)
+  complex(8) :: charge(*), dipstr(*), pot(*), fld(*), cdummy(3)
+  ier = c_hfmm3dparttarg(iprec, zk, nsource, source, ifcharge, charge, ifdipole, dipstr, dipvec, &
+      ifpot, pot, iffld, fld, 0, dummy, 0, cdummy, 0, cdummy)
+end subroutine hfmm3dpartself
+
+subroutine hfmm3dpart(ier, iprec, zk, nsource, source, ifcharge, charge, ifdipole, dipstr, &
+    dipvec, ifpot, pot, iffld, fld)
+  implicit none
+  integer :: ier, iprec, nsource, ifcharge, ifdipole, ifpot, iffld
+  complex(8) :: zk
+  real(8) :: source(*), dipvec(*)
+  complex(8) :: charge(*), dipstr(*), pot(*), fld(*)
+  call hfmm3dpartself(ier, iprec, zk, nsource, source, ifcharge, charge, ifdipole, dipstr, &
+      dipvec, ifpot, pot, iffld, fld)
+end subroutine hfmm3dpart
+
+subroutine h3dpartdirect(zk, nsource, source, ifcharge, charge, ifdipole, dipstr, dipvec, &
+    ifpot, pot, iffld, fld, ntarget, target, ifpottarg, pottarg, iffldtarg, fldtarg)
+  use fmm3d_mi355x_c
+  implicit none
+  integer :: nsource, ifcharge, ifdipole, ifpot, iffld, ntarget, ifpottarg, iffldtarg
+  complex(8) :: zk
+  real(8) :: source(*), dipvec(*), target(*)
+  complex(8) :: charge(*), dipstr(*), pot(*), fld(*), pottarg(*), fldtarg(*)
+  call c_h3dpartdirect(zk, nsource, source, ifcharge, charge, ifdipole, dipstr, dipvec, &
+      ifpot, pot, iffld, fld, ntarget, target, ifpottarg, pottarg, iffldtarg, fldtarg)
+end subroutine h3dpartdirect

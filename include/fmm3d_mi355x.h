@@ -78,6 +78,49 @@ int fmm3d_lfmm3dparttarg_dev(int prec_mode, int iprec, int nsource, const double
                              int ifpottarg, double *pottarg_dev, int iffldtarg, double *fldtarg_dev,
                              void *stream);
 
+/* Replaces hfmm3dparttarg, src/hfmm3dpart.f:141-144: Helmholtz kernel
+ * e^{ik r}/r, zk = complex*16 wavenumber passed by reference as (re, im)
+ * (zk = 0 returns 128).  Other arguments as fmm3d_lfmm3dparttarg. */
+int fmm3d_hfmm3dparttarg(int iprec, const double *zk, int nsource, const double *source,
+                         int ifcharge, const double *charge,
+                         int ifdipole, const double *dipstr, const double *dipvec,
+                         int ifpot, double *pot, int iffld, double *fld,
+                         int ntarget, const double *target,
+                         int ifpottarg, double *pottarg, int iffldtarg, double *fldtarg);
+
+/* Replaces hfmm3dpartself, src/hfmm3dpart.f:90-92 (and hfmm3dpart, :39-41). */
+int fmm3d_hfmm3dpartself(int iprec, const double *zk, int nsource, const double *source,
+                         int ifcharge, const double *charge,
+                         int ifdipole, const double *dipstr, const double *dipvec,
+                         int ifpot, double *pot, int iffld, double *fld);
+
+/* Replaces h3dpartdirect, src/hfmm3dpart.f:1268-1271 (O(N^2) direct sum). */
+void fmm3d_h3dpartdirect(const double *zk, int nsource, const double *source,
+                         int ifcharge, const double *charge,
+                         int ifdipole, const double *dipstr, const double *dipvec,
+                         int ifpot, double *pot, int iffld, double *fld,
+                         int ntarget, const double *target,
+                         int ifpottarg, double *pottarg, int iffldtarg, double *fldtarg);
+
+/* fmm3d_hfmm3dparttarg with a precision mode (FMM3D_PREC_FP64 only so far;
+ * other modes return 128).  Extension with no reference counterpart. */
+int fmm3d_hfmm3dparttarg_ex(int prec_mode, int iprec, const double *zk, int nsource,
+                            const double *source, int ifcharge, const double *charge,
+                            int ifdipole, const double *dipstr, const double *dipvec,
+                            int ifpot, double *pot, int iffld, double *fld,
+                            int ntarget, const double *target,
+                            int ifpottarg, double *pottarg, int iffldtarg, double *fldtarg);
+
+/* Device-resident variant of hfmm3dparttarg (device pointers, hipStream_t
+ * `stream`; zk is a host pointer).  Extension with no reference counterpart. */
+int fmm3d_hfmm3dparttarg_dev(int prec_mode, int iprec, const double *zk, int nsource,
+                             const double *source_dev, int ifcharge, const double *charge_dev,
+                             int ifdipole, const double *dipstr_dev, const double *dipvec_dev,
+                             int ifpot, double *pot_dev, int iffld, double *fld_dev,
+                             int ntarget, const double *target_dev,
+                             int ifpottarg, double *pottarg_dev, int iffldtarg, double *fldtarg_dev,
+                             void *stream);
+
 /* Replaces d3tstrcr with its accessors d3tgetb/d3tgetl
  * (src/d3tstrcr.f:24-259, :580-689): builds the adaptive octree the FMM
  * uses (host code) and returns it flattened.  boxes20 is the d3tgetb

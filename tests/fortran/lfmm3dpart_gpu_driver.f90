@@ -3,7 +3,9 @@
 ! reference's sample driver), charges and dipoles, iprec = 1.  It calls
 ! lfmm3dparttarg and l3dpartdirect exactly as a reference user would (the
 ! names resolve to libfmm3d_mi355x_f.so) and prints the relative L2 errors
-! of the FMM against the direct sum at the first 100 sources and targets.
+! of the FMM against the direct sum at the first 100 sources and targets;
+! then the same with hfmm3dparttarg / h3dpartdirect at zk = 6 + 0.1i
+! (targets only).
 program lfmm3dpart_gpu_driver
   implicit none
   integer, parameter :: n = 16000, nt = 16000, m = 100
@@ -13,6 +15,7 @@ program lfmm3dpart_gpu_driver
   real(8) :: rdum(3)
   integer :: ier, i, seed(64), ns
   real(8) :: e1, e2, e3, e4
+  complex(8) :: zk
   pi = 4 * atan(1.0d0)
   call random_seed(size=ns)
   seed = 20120418
@@ -53,6 +56,18 @@ program lfmm3dpart_gpu_driver
   e3 = sqrt(sum(abs(pottarg(1:m) - pt2)**2) / sum(abs(pt2)**2))
   e4 = sqrt(sum(abs(fldtarg(:, 1:m) - ft2)**2) / sum(abs(ft2)**2))
   write (*, '(a,4es12.4)') 'relerr pot fld pottarg fldtarg ', e1, e2, e3, e4
+  zk = cmplx(6.0d0, 0.1d0, kind=8)
+  call hfmm3dparttarg(ier, 1, zk, n, source, 1, charge, 1, dipstr, dipvec, 1, pot, 1, fld, &
+      nt, target, 1, pottarg, 1, fldtarg)
+  if (ier /= 0) then
+    print *, 'hfmm ier=', ier
+    stop 1
+  end if
+  call h3dpartdirect(zk, n, source, 1, charge, 1, dipstr, dipvec, 0, cdum, 0, cdum, &
+      m, target, 1, pt2, 1, ft2)
+  e3 = sqrt(sum(abs(pottarg(1:m) - pt2)**2) / sum(abs(pt2)**2))
+  e4 = sqrt(sum(abs(fldtarg(:, 1:m) - ft2)**2) / sum(abs(ft2)**2))
+  write (*, '(a,2es12.4)') 'hrelerr pottarg fldtarg ', e3, e4
 contains
   subroutine direct_at_sources(n, src, q, ds, dv, m, p, f)
     integer :: n, m, i, j

@@ -20,3 +20,6 @@ def test_fortran_driver_runs_on_gpu():
     # iprec = 1: tolerance 0.5e-3 for the potential, a digit less for the field
     assert errs[0] < 5e-4 and errs[2] < 5e-4
     assert errs[1] < 5e-3 and errs[3] < 5e-3
+    line = [l for l in out.stdout.splitlines() if l.startswith("hrelerr")][-1]
+    errs = [float(x) for x in line.split()[-2:]]
+    assert errs[0] < 5e-4 and errs[1] < 5e-3

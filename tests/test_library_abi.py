@@ -48,11 +48,20 @@ def test_argument_errors_before_device_use():
                                     -1, None, 0, None, 0, None) == 128
     # unknown precision mode
     assert F.lfmm3dparttarg(1, src, q, precision=9)[0] == 128
+    # Helmholtz: the same checks, plus zk = 0 and modes other than FP64
+    assert F.hfmm3dparttarg(1, 2.0, np.zeros((0, 3)), np.zeros(0, np.complex128))[0] == 4
+    assert F.hfmm3dparttarg(7, 2.0, src, q)[0] == 32
+    assert F.hfmm3dparttarg(1, 0.0, src, q)[0] == 128
+    assert F.hfmm3dparttarg(1, 2.0, src, q, precision=F.PREC_FP32)[0] == 128
+    assert lib.fmm3d_hfmm3dparttarg(1, None, 4, F._ptr(src), 1, F._ptr(q), 0, None, None, 1, None, 0,
+                                    None, 0, None, 0, None, 0, None) == 128
 
 
 @pytest.mark.parametrize("ns,nt,nbox,dist", [(3000, 2000, 40, "cube"), (5000, 0, 50, "sphere"),
                                              (4000, 3000, 80, "clusters"), (3000, 100, 30, "plummer"),
-                                             (900, 300, 40, "plane"), (50, 10, 160, "cube")])
+                                             (900, 300, 40, "plane"), (50, 10, 160, "cube"),
+                                             (20000, 5000, 10, "plummer"), (30000, 0, 20, "clusters"),
+                                             (40000, 40000, 400, "cube")])
 def test_host_octree_equals_oracle(rng, ns, nt, nbox, dist):
     src = points(rng, ns, dist)
     trg = points(rng, nt, dist)
