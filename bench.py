@@ -17,6 +17,11 @@ potential and the field.  The P2P roofline uses the exact pair count of the
 tree; the CPU baseline times the reference library (oracle/_ref) on a
 125,000 + 125,000 sample with the same leaf fill (244 points per leaf).
 
+Two more lines ride along in the same JSON object: "helmholtz" (config 3,
+hfmm3dpartself on 500,000 points on the unit sphere, zk = 20, iprec = 2) and
+"serving" (a stream of mixed-size Laplace requests through the request server,
+p50/p99 latency and throughput).
+
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
 """
 import argparse
@@ -99,6 +104,103 @@ def cpu_baseline(n=125_000):
                       % (n, n, res["seconds"])}
 
 
+def helmholtz_leg(F, torch, ob, steps, warmup):
+    """Config 3: hfmm3dpartself on 500,000 points on the unit sphere (normalised
+    Gaussian vectors), zk = 20, iprec = 2, charges with N(0,1) parts, FP64;
+    inputs resident in HBM; error against the FP64 direct sum at 200 sources."""
+    import numpy as np
+    from fmm_inputs import points, rel
+    n = 500_000
+    rng = np.random.default_rng(3)
+    src = points(rng, n, "sphere")
+    q = rng.normal(size=n) + 1j * rng.normal(size=n)
+    dev = torch.device("cuda:0")
+    ts = torch.from_numpy(src).to(dev)
+    tq = torch.from_numpy(q).to(dev)
+    pot = torch.zeros(n, dtype=torch.complex128, device=dev)
+    fld = torch.zeros(n, 3, dtype=torch.complex128, device=dev)
+    zk = np.array([20.0, 0.0])
+    p = lambda x: ctypes.c_void_p(x.data_ptr())  # noqa: E731
+    stream = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+    lib = F.load()
+
+    def step():
+        ier = lib.fmm3d_hfmm3dparttarg_dev(0, 2, zk.ctypes.data_as(ctypes.c_void_p), n, p(ts), 1, p(tq), 0,
+                                           None, None, 1, p(pot), 1, p(fld), 0, None, 0, None, 0, None, stream)
+        if ier != 0:
+            raise RuntimeError("hfmm ier=%d" % ier)
+    for _ in range(warmup):
+        step()
+    torch.cuda.synchronize()
+    lat, stages = [], []
+    for _ in range(steps):
+        a = time.perf_counter()
+        step()
+        torch.cuda.synchronize()
+        lat.append((time.perf_counter() - a) * 1e3)
+        stages.append(F.last_timings())
+    idx = rng.choice(n, 200, replace=False)
+    pd, fd = ob.orc_direct_sample(src, src[idx], idx.astype(np.int32), q, zk=20.0)
+    po, fo = pot.cpu().numpy(), fld.cpu().numpy()
+    ms = float(np.mean(lat))
+    return {"workload": "hfmm3dpartself: 500,000 points on the unit sphere, zk=20, iprec=2, charges N(0,1), FP64",
+            "ms_per_step": ms, "points_per_s": n / (ms * 1e-3), "p50_ms": float(np.percentile(lat, 50)),
+            "err_pot": rel(po[idx], pd), "err_fld": rel(fo[idx], fd), "sampled": 200,
+            "stages_ms": {k: float(np.mean([s_[k] for s_ in stages])) for k in stages[0]}}
+
+
+def serving_leg(F, nreq, nworkers, seed=4):
+    """Config 4 shape: lfmm3dpartself requests with N log-uniform in
+    [1e4, 2e6], points from the unit cube, a sphere shell (r in [0.995, 1.005])
+    or 8 Gaussian clusters (sigma 0.02), iprec uniform in {1,2,3}, charges in
+    [-1,1].  Phase 1 submits all at once (saturated throughput); phase 2
+    submits the same requests at Poisson arrival times at 70% of that rate and
+    reports latency from submission to completion."""
+    import numpy as np
+    from fmm_inputs import points
+    rng = np.random.default_rng(seed)
+    reqs = []
+    for _ in range(nreq):
+        n = int(10 ** rng.uniform(4, np.log10(2e6)))
+        dist = ["cube", "shell", "clusters"][rng.integers(0, 3)]
+        x = points(rng, n, dist)
+        q = rng.uniform(-1, 1, n) + 1j * rng.uniform(-1, 1, n)
+        reqs.append((int(rng.integers(1, 4)), x, q))
+    npts = sum(r[1].shape[0] for r in reqs)
+    with F.Server(nworkers=nworkers, mem_budget=96 << 30) as srv:
+        # warm every worker (tables, arenas)
+        for i in [srv.submit(0, it, x, q) for (it, x, q) in reqs[:nworkers]]:
+            srv.wait(i)
+        t0 = time.perf_counter()
+        ids = [srv.submit(0, it, x, q) for (it, x, q) in reqs]
+        for i in ids:
+            assert srv.wait(i)[0] == 0
+        t_sat = time.perf_counter() - t0
+        rate = 0.7 * nreq / t_sat
+        gaps = rng.exponential(1.0 / rate, nreq)
+        t0 = time.perf_counter()
+        ids, tnext = [], 0.0
+        for (it, x, q), g in zip(reqs, gaps):
+            tnext += g
+            d = t0 + tnext - time.perf_counter()
+            if d > 0:
+                time.sleep(d)
+            ids.append(srv.submit(0, it, x, q))
+        lat = []
+        for i in ids:
+            r = srv.wait(i)
+            assert r[0] == 0
+            lat.append(r[5])
+        t_stream = time.perf_counter() - t0
+    return {"workload": "%d lfmm3dpartself requests, N log-uniform in [1e4, 2e6], cube/shell/clusters, "
+                        "iprec in {1,2,3}; Poisson arrivals at 70%% of the saturated rate" % nreq,
+            "workers": nworkers, "points_total": npts,
+            "saturated_req_per_s": nreq / t_sat, "saturated_points_per_s": npts / t_sat,
+            "arrival_rate_hz": rate, "stream_s": t_stream,
+            "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
+            "mean_ms": float(np.mean(lat))}
+
+
 def run_mode(F, torch, prec, dev_in, dev_out, steps, warmup):
     ts, tq, tds, tdv, tt = dev_in
     pot, fld, pt, ft = dev_out
@@ -134,6 +236,10 @@ def main():
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-helm", action="store_true", help="skip the Helmholtz (config 3) line")
+    ap.add_argument("--no-serve", action="store_true", help="skip the serving-stream line")
+    ap.add_argument("--serve-requests", type=int, default=48)
+    ap.add_argument("--serve-workers", type=int, default=4)
     ap.add_argument("--cpu-baseline-child", type=int, default=0, help=argparse.SUPPRESS)
     ap.add_argument("--calibrate-copy-mb", type=int, default=0,
                     help="only run a plain device-to-device copy of this many MiB (HBM counter calibration)")
@@ -197,6 +303,16 @@ def main():
     flops = pairs * FLOPS_PER_PAIR
     achieved = flops / (M["p2p_ms"] * 1e-3) / 1e12
     peak = FP64_PEAK_TFLOPS if best == "fp64" else FP32_PEAK_TFLOPS
+    helm = None if args.no_helm else helmholtz_leg(F, torch, ob, steps=max(2, min(args.steps, 3)), warmup=1)
+    if helm:
+        print("# helmholtz: %.1f ms/step, err pot %.2e fld %.2e, stages %s"
+              % (helm["ms_per_step"], helm["err_pot"], helm["err_fld"],
+                 {k: round(v, 2) for k, v in helm["stages_ms"].items()}), file=sys.stderr, flush=True)
+    serve = None if args.no_serve else serving_leg(F, args.serve_requests, args.serve_workers)
+    if serve:
+        print("# serving: p50 %.1f ms p99 %.1f ms, saturated %.2f req/s (%.3e points/s)"
+              % (serve["p50_ms"], serve["p99_ms"], serve["saturated_req_per_s"], serve["saturated_points_per_s"]),
+              file=sys.stderr, flush=True)
     cpu = {"value": None, "unit": "targets/s", "cores": 0, "kind": "reference", "sample": "skipped"} \
         if args.no_cpu else cpu_baseline()
     line = {
@@ -217,6 +333,8 @@ def main():
                              "algorithmic flops = %d pairs x %d" % (pairs, FLOPS_PER_PAIR)},
         "cpu_baseline": cpu,
         "modes": {k: {kk: vv for kk, vv in v.items() if kk != "latencies_ms"} for k, v in modes.items()},
+        "helmholtz": helm,
+        "serving": serve,
     }
     print(json.dumps(line))
 

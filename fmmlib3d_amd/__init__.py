@@ -33,6 +33,8 @@ EXPORTED = (
     "fmm3d_version", "fmm3d_d3tstrcr",
     "fmm3d_hfmm3dparttarg", "fmm3d_hfmm3dpartself", "fmm3d_h3dpartdirect",
     "fmm3d_hfmm3dparttarg_ex", "fmm3d_hfmm3dparttarg_dev",
+    "fmm3d_server_create", "fmm3d_server_submit", "fmm3d_server_wait", "fmm3d_server_stats",
+    "fmm3d_server_destroy",
 )
 
 _lib = None
@@ -78,6 +80,17 @@ def load():
         lib.fmm3d_hfmm3dpartself.restype = i
         lib.fmm3d_h3dpartdirect.argtypes = [vp, i, vp, i, vp, i, vp, vp, i, vp, i, vp, i, vp, i, vp, i, vp]
         lib.fmm3d_h3dpartdirect.restype = None
+        ll = ctypes.c_longlong
+        lib.fmm3d_server_create.argtypes = [i, ll]
+        lib.fmm3d_server_create.restype = vp
+        lib.fmm3d_server_submit.argtypes = [vp, i, i, i, vp, i, vp, i, vp, i, vp, vp, i, i, i, vp, i, i]
+        lib.fmm3d_server_submit.restype = ll
+        lib.fmm3d_server_wait.argtypes = [vp, ll, i, vp, vp, vp, vp, vp, vp]
+        lib.fmm3d_server_wait.restype = i
+        lib.fmm3d_server_stats.argtypes = [vp, vp, vp, vp]
+        lib.fmm3d_server_stats.restype = None
+        lib.fmm3d_server_destroy.argtypes = [vp]
+        lib.fmm3d_server_destroy.restype = None
         lib.fmm3d_last_timings.argtypes = [vp, i]
         lib.fmm3d_last_timings.restype = None
         _lib = lib
@@ -261,3 +274,74 @@ def last_timings():
     load().fmm3d_last_timings(_ptr(out), 11)
     keys = ("tree_plan", "upload", "p2m", "p2l", "m2m", "m2l", "l2l", "l2p", "m2p", "p2p", "total")
     return dict(zip(keys, out.tolist()))
+
+
+class Server:
+    """Request server over the C ABI (fmm3d_server_* in include/fmm3d_mi355x.h):
+    several evaluations run at once on the GPU, each worker with its own
+    stream and resident device memory, admitted against a memory budget.
+
+        with Server(nworkers=4, mem_budget=64 << 30) as srv:
+            rid = srv.submit(0, 2, source, charge=q)          # Laplace, iprec 2
+            ier, pot, fld, pottarg, fldtarg, latency_ms = srv.wait(rid)
+    """
+
+    def __init__(self, nworkers=4, mem_budget=64 << 30):
+        self._lib = load()
+        self._h = self._lib.fmm3d_server_create(int(nworkers), int(mem_budget))
+        if not self._h:
+            raise RuntimeError("fmm3d_server_create failed (no GPU, or bad arguments)")
+        self._meta = {}
+
+    def submit(self, kernel, iprec, source, charge=None, dipstr=None, dipvec=None, target=None,
+               zk=None, ifpot=1, iffld=1, ifpottarg=1, iffldtarg=1, precision=PREC_FP64):
+        """kernel 0 = Laplace, 1 = Helmholtz (zk required).  Returns a request id."""
+        src, q, ds, dv, trg = _prep(source, charge, dipstr, dipvec, target)
+        ns = src.shape[0]
+        nt = 0 if trg is None else trg.shape[0]
+        z = _zk(zk) if zk is not None else None
+        rid = self._lib.fmm3d_server_submit(
+            self._h, int(kernel), int(precision), int(iprec), _ptr(z), ns, _ptr(src), int(q is not None),
+            _ptr(q), int(ds is not None), _ptr(ds), _ptr(dv), int(bool(ifpot)), int(bool(iffld)), nt,
+            _ptr(trg), int(bool(ifpottarg) and nt > 0), int(bool(iffldtarg) and nt > 0))
+        if rid < 0:
+            raise FMMError(-rid, "submission rejected")
+        self._meta[rid] = (ns, nt, bool(ifpot), bool(iffld), bool(ifpottarg) and nt > 0, bool(iffldtarg) and nt > 0)
+        return rid
+
+    def wait(self, rid, block=True):
+        """(ier, pot, fld, pottarg, fldtarg, latency_ms), or None if not done
+        and block is False."""
+        ns, nt, wp, wf, wpt, wft = self._meta[rid]
+        pot = np.zeros(ns, np.complex128) if wp else None
+        fld = np.zeros((ns, 3), np.complex128) if wf else None
+        pt = np.zeros(nt, np.complex128) if wpt else None
+        ft = np.zeros((nt, 3), np.complex128) if wft else None
+        ier = ctypes.c_int(0)
+        lat = ctypes.c_double(0)
+        r = self._lib.fmm3d_server_wait(self._h, rid, int(bool(block)), _ptr(pot), _ptr(fld), _ptr(pt), _ptr(ft),
+                                        ctypes.byref(ier), ctypes.byref(lat))
+        if r == 0:
+            return None
+        if r < 0:
+            raise KeyError(rid)
+        del self._meta[rid]
+        return ier.value, pot, fld, pt, ft, lat.value
+
+    def stats(self):
+        q = ctypes.c_longlong(0)
+        r = ctypes.c_int(0)
+        b = ctypes.c_longlong(0)
+        self._lib.fmm3d_server_stats(self._h, ctypes.byref(q), ctypes.byref(r), ctypes.byref(b))
+        return {"queued": q.value, "running": r.value, "bytes_in_use": b.value}
+
+    def close(self):
+        if self._h:
+            self._lib.fmm3d_server_destroy(self._h)
+            self._h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()

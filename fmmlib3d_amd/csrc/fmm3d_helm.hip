@@ -218,14 +218,27 @@ int helm_solve(Context& ctx, const LapCall& c) {
                        O.l2l_groups[g].first, HC->l2l_base + O.l2l_group_level[g], lo);
   tm.mark(6);
   // STEP 7: L2P
-  if (P.n_l2p)
-    hipLaunchKernelGGL((k_heval<false>), dim3(P.n_l2p), dim3(64), lds_eval, s, P, Hd, fac, lo, spos, tpos,
-                       f.ifpot, f.iffld, f.ifpottarg, f.iffldtarg, pot_s, fld_s, pot_t, fld_t);
+  const size_t Kx = (size_t)(pmax + 2) * (pmax + 2);
+  const size_t lds_eval2 = sizeof(double) * (8 * Kx + 2 * 64 * (size_t)Fn);
+  const bool eval2 = lds_eval2 <= ldsmax;
+  if (P.n_l2p) {
+    if (eval2)
+      hipLaunchKernelGGL((k_heval2<false>), dim3(P.n_l2p), dim3(64), lds_eval2, s, P, Hd, fac, lo, spos, tpos,
+                         f.ifpot, f.iffld, f.ifpottarg, f.iffldtarg, pot_s, fld_s, pot_t, fld_t);
+    else
+      hipLaunchKernelGGL((k_heval<false>), dim3(P.n_l2p), dim3(64), lds_eval, s, P, Hd, fac, lo, spos, tpos,
+                         f.ifpot, f.iffld, f.ifpottarg, f.iffldtarg, pot_s, fld_s, pot_t, fld_t);
+  }
   tm.mark(7);
   // STEP 6: M2P (list 3)
-  if (P.n_m2p)
-    hipLaunchKernelGGL((k_heval<true>), dim3(P.n_m2p), dim3(64), lds_eval, s, P, Hd, fac, mp, spos, tpos,
-                       f.ifpot, f.iffld, f.ifpottarg, f.iffldtarg, pot_s, fld_s, pot_t, fld_t);
+  if (P.n_m2p) {
+    if (eval2)
+      hipLaunchKernelGGL((k_heval2<true>), dim3(P.n_m2p), dim3(64), lds_eval2, s, P, Hd, fac, mp, spos, tpos,
+                         f.ifpot, f.iffld, f.ifpottarg, f.iffldtarg, pot_s, fld_s, pot_t, fld_t);
+    else
+      hipLaunchKernelGGL((k_heval<true>), dim3(P.n_m2p), dim3(64), lds_eval, s, P, Hd, fac, mp, spos, tpos,
+                         f.ifpot, f.iffld, f.ifpottarg, f.iffldtarg, pot_s, fld_s, pot_t, fld_t);
+  }
   tm.mark(8);
   // STEP 8: P2P
   if (f.ifcharge || f.ifdipole) {

@@ -186,7 +186,12 @@ int lap_solve(Context& ctx, const LapCall& c) {
     hipLaunchKernelGGL((k_m2m<T>), dim3(g.second), dim3(256), lds_ps, s, P, Tb, g.first, mp);
   tm.mark(4);
   // ---- STEP 4: M2L
-  if (P.n_m2l) hipLaunchKernelGGL((k_m2l<T>), dim3(P.n_m2l), dim3(128), lds_ps, s, P, Tb, mp, lo);
+  if (P.n_m2l) {
+    if (sizeof(T) * 6 * (size_t)K <= max_lds_bytes(ctx))
+      hipLaunchKernelGGL((k_m2l<T, true>), dim3(P.n_m2l), dim3(256), sizeof(T) * 6 * K, s, P, Tb, mp, lo);
+    else
+      hipLaunchKernelGGL((k_m2l<T, false>), dim3(P.n_m2l), dim3(256), lds_ps, s, P, Tb, mp, lo);
+  }
   tm.mark(5);
   // ---- STEP 5: L2L, top-down
   for (auto& g : O.l2l_groups)

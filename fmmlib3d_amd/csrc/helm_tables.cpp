@@ -229,6 +229,11 @@ void coax(const Gaunt& G, int mode, double sigma, cplx zk, double d, double sc_i
   else sph_bessel_scaled(S, zk * d, base, f.data());
   const int mtop = std::min(pin, pout);
   const long total = coax_block(mtop + 1, pin, pout);
+  // powers used below: base^e for 0 <= e <= S, ratio^e for 0 <= e <= max(pin, pout)
+  std::vector<double> pb(S + 1, 1.0), pr(std::max(pin, pout) + 1, 1.0);
+  const double ratio = (mode == 0) ? sc_in / sc_out : (mode == 2 ? sc_out / sc_in : 1.0);
+  for (int e = 1; e <= S; e++) pb[e] = pb[e - 1] * base;
+  for (int e = 1; e < (int)pr.size(); e++) pr[e] = pr[e - 1] * ratio;
   const size_t o0 = z.size();
   z.resize(o0 + 2 * (size_t)total, 0.0);
   for (int m = 0; m <= mtop; m++) {
@@ -243,9 +248,9 @@ void coax(const Gaunt& G, int mode, double sigma, cplx zk, double d, double sc_i
           if (sigma < 0 && (s & 1)) sg = -sg;
           double c = 4.0 * M_PI * sg * std::sqrt((2.0 * s + 1.0) / (4.0 * M_PI)) * G.g[go + k];
           double pw;
-          if (mode == 1) pw = std::pow(base, n + l - s);
-          else if (mode == 0) pw = std::pow(base, s - l + n) * std::pow(sc_in / sc_out, n);
-          else pw = std::pow(base, s - n + l) * std::pow(sc_out / sc_in, l);
+          if (mode == 1) pw = pb[n + l - s];
+          else if (mode == 0) pw = pb[s - l + n] * pr[n];
+          else pw = pb[s - n + l] * pr[l];
           acc += c * pw * f[s];
         }
         const size_t e = o0 + 2 * (size_t)(mb + (long)(l - m) * cols + (n - m));
@@ -295,10 +300,19 @@ bool build_helm_tables(cplx zk, double size, int nlev, double eps, HelmHostTable
       int dc = H.geo.off_dist[code];
       need[dc] = std::max(need[dc], pt);
     }
+    std::vector<std::vector<double>> part(nd);
+    std::vector<CoaxDesc> desc(nd);
+#pragma omp parallel for schedule(dynamic, 1)
     for (int dc = 0; dc < nd; dc++)
       if (need[dc] > 0)
         coax(*G, 1, -1.0, zk, bsize[l] * H.geo.dist_value[dc], H.sclev[l], H.sclev[l], need[dc], need[dc],
-             H.z, H.m2l[(size_t)l * nd + dc]);
+             part[dc], desc[dc]);
+    for (int dc = 0; dc < nd; dc++) {
+      if (need[dc] == 0) continue;
+      desc[dc].base = (long)(H.z.size() / 2);
+      H.z.insert(H.z.end(), part[dc].begin(), part[dc].end());
+      H.m2l[(size_t)l * nd + dc] = desc[dc];
+    }
   }
   for (int l = 2; l <= nlev - 1; l++) {
     const double d = bsize[l + 1] * std::sqrt(3.0) / 2.0;

@@ -381,16 +381,24 @@ __global__ void k_l2l(DevPlan P, DevTables Tb, int first, T* __restrict__ lo) {
   }
 }
 
-// M2L over the (pruned) list 2 of each target box (src/lfmm3drouts.f:347-483)
-template <typename T>
+// M2L over the (pruned) list 2 of each target box (src/lfmm3drouts.f:347-483).
+// The target's local accumulates in LDS (the third 2K block) and is added to
+// memory once, so HBM sees one read per source multipole and one
+// read-modify-write per target.  Without LDS room (LDSACC = false, high
+// orders) the local is accumulated in memory directly.
+template <typename T, bool LDSACC>
 __global__ void k_m2l(DevPlan P, DevTables Tb, const T* __restrict__ mp, T* __restrict__ lo) {
   extern __shared__ unsigned char smem_raw[];
   T* A = reinterpret_cast<T*>(smem_raw);
   T* B = A + 2 * P.K;
   const int b = P.m2l_boxes[blockIdx.x];
+  T* acc = LDSACC ? B + 2 * P.K : lo + 2 * (size_t)b * P.K;
   const int p = P.p;
   const T ih = (T)P.inv_side[P.level[b]];
   const size_t zsz = (size_t)(p + 1) * (p + 1) * (p + 1);
+  if (LDSACC)
+    for (int k = threadIdx.x; k < 2 * P.K; k += blockDim.x) acc[k] = T(0);
+  __syncthreads();
   for (int jj = P.l2_off[b]; jj < P.l2_off[b + 1]; jj++) {
     const int j = P.l2_idx[jj];
     int di = P.ijk[3 * j] - P.ijk[3 * b] + 3, dj = P.ijk[3 * j + 1] - P.ijk[3 * b + 1] + 3,
@@ -399,7 +407,11 @@ __global__ void k_m2l(DevPlan P, DevTables Tb, const T* __restrict__ mp, T* __re
     int pt = Tb.off_ptrunc[code];
     point_and_shoot<T>(Tb, Tb.off_rot[code], Tb.off_phase + (size_t)code * 2 * (p + 1),
                        Tb.z_m2l + (size_t)Tb.off_dist[code] * zsz, pt, pt,
-                       mp + 2 * (size_t)j * P.K, lo + 2 * (size_t)b * P.K, ih, A, B);
+                       mp + 2 * (size_t)j * P.K, acc, ih, A, B);
+  }
+  if (LDSACC) {
+    T* out = lo + 2 * (size_t)b * P.K;
+    for (int k = threadIdx.x; k < 2 * P.K; k += blockDim.x) out[k] += acc[k];
   }
 }
 

@@ -136,6 +136,31 @@ int fmm3d_d3tstrcr(int nsource, const double *source, int ntarget, const double 
                    int maxboxes, int *boxes20, int *isource, int *itarget,
                    int *list_counts, int maxlist, int *lists);
 
+/* ---- Request server (extension with no reference counterpart; not part of
+ * the drop-in contract).  A persistent loop that runs several evaluations at
+ * once on the GPU: `nworkers` slots, each with its own HIP stream and
+ * device arena (trees, expansions and tables stay resident between
+ * requests); a queued request starts when a slot is free and its
+ * device-memory estimate fits what is left of `mem_budget_bytes`.  Inputs are
+ * copied at submission.  All functions are thread safe. */
+void *fmm3d_server_create(int nworkers, long long mem_budget_bytes);
+/* kernel 0 = Laplace (zk ignored), 1 = Helmholtz (zk = (re, im)); other
+ * arguments as fmm3d_lfmm3dparttarg_ex.  Returns a request id >= 0, or
+ * -ier for an invalid argument. */
+long long fmm3d_server_submit(void *server, int kernel, int prec_mode, int iprec, const double *zk,
+                              int nsource, const double *source, int ifcharge, const double *charge,
+                              int ifdipole, const double *dipstr, const double *dipvec,
+                              int ifpot, int iffld, int ntarget, const double *target,
+                              int ifpottarg, int iffldtarg);
+/* 1 when the request has finished (outputs copied, *ier and the latency from
+ * submission to completion set; the id is released), 0 when it is still
+ * pending (block = 0), -1 for an unknown id. */
+int fmm3d_server_wait(void *server, long long id, int block, double *pot, double *fld,
+                      double *pottarg, double *fldtarg, int *ier, double *latency_ms);
+void fmm3d_server_stats(void *server, long long *queued, int *running, long long *bytes_in_use);
+/* Finishes the queued requests, then releases the workers and their memory. */
+void fmm3d_server_destroy(void *server);
+
 /* Per-stage timings (milliseconds, device time from HIP events) of the last
  * FMM call on this thread: [0] tree+plan (host), [1] upload, [2] P2M,
  * [3] P2L, [4] M2M, [5] M2L, [6] L2L, [7] L2P, [8] M2P, [9] P2P, [10] total.
