@@ -52,7 +52,7 @@ int check_args(const double* zk, int prec_mode, int iprec, int ns, int nt) {
   if (iprec < -2 || iprec > 6) return 32;
   if (zk) {
     if (zk[0] == 0.0 && zk[1] == 0.0) return 128;  // zk = 0: the Helmholtz scaling degenerates
-    if (prec_mode != 0) return 128;
+    if (prec_mode != 0 && prec_mode != 1) return 128;
   } else if (prec_mode < 0 || prec_mode > 3) {
     return 128;
   }

@@ -86,27 +86,30 @@ HelmCache* helm_tables(Context& ctx, cplx zk, double size, int nlev, double eps,
   return c;
 }
 
-template <bool CHG, bool DIP, bool FLD>
+template <typename TN, bool CHG, bool DIP, bool FLD>
 void hp2p(const DevPlan& P, const int* items, int n, int kind, double kre, double kim, const double* spos,
-          const double* str, const double* tpos, double* pot, double* fo, hipStream_t s) {
+          const TN* str, const double* tpos, double* pot, double* fo, hipStream_t s) {
   DevPlan Q = P;
   Q.p2p_items = items;
   Q.n_p2p = n;
-  hipLaunchKernelGGL((k_hp2p<CHG, DIP, FLD>), dim3(n), dim3(kP2PBlock), 0, s, Q, kre, kim, spos, str, tpos, kind, pot, fo);
+  hipLaunchKernelGGL((k_hp2p<TN, CHG, DIP, FLD>), dim3(n), dim3(kP2PBlock), 0, s, Q, kre, kim, spos, str, tpos, kind, pot, fo);
 }
 
+template <typename TN>
 void launch_hp2p(const DevPlan& P, const int* items, int n, int kind, double kre, double kim,
-                 const double* spos, const double* str, const double* tpos, int ifcharge, int ifdipole,
+                 const double* spos, const TN* str, const double* tpos, int ifcharge, int ifdipole,
                  bool fld, double* pot, double* fo, hipStream_t s) {
   if (n == 0) return;
 #define HCASE(C, Dp, F) \
-  if (ifcharge == C && ifdipole == Dp && fld == F) return hp2p<C, Dp, F>(P, items, n, kind, kre, kim, spos, str, tpos, pot, fo, s);
+  if (ifcharge == C && ifdipole == Dp && fld == F) return hp2p<TN, C, Dp, F>(P, items, n, kind, kre, kim, spos, str, tpos, pot, fo, s);
   HCASE(1, 0, false) HCASE(1, 0, true) HCASE(0, 1, false) HCASE(0, 1, true) HCASE(1, 1, false) HCASE(1, 1, true)
 #undef HCASE
 }
 
 }  // namespace
 
+// TN: near-field pair arithmetic (double, or float with FP64 offsets and accumulation)
+template <typename TN>
 int helm_solve(Context& ctx, const LapCall& c) {
   auto t0 = std::chrono::steady_clock::now();
   double eps;
@@ -163,6 +166,9 @@ int helm_solve(Context& ctx, const LapCall& c) {
   const size_t nb = tree.nboxes;
   double* spos = (double*)ctx.arena.get("spos", sizeof(double) * 4 * (size_t)ns);
   double* str64 = (double*)ctx.arena.get("str64", sizeof(double) * 8 * (size_t)ns);
+  constexpr bool near32 = sizeof(TN) < sizeof(double);
+  float* str32 = near32 ? (float*)ctx.arena.get("str32", sizeof(float) * 8 * (size_t)ns) : nullptr;
+  const TN* strn = near32 ? (const TN*)(const void*)str32 : (const TN*)(const void*)str64;
   double* tpos = (double*)ctx.arena.get("tpos", sizeof(double) * 4 * (size_t)std::max(nt, 1));
   double* mp = (double*)ctx.arena.get("mp", sizeof(double) * 2 * K * nb, 16);
   double* lo = (double*)ctx.arena.get("lo", sizeof(double) * 2 * K * nb, 16);
@@ -173,7 +179,7 @@ int helm_solve(Context& ctx, const LapCall& c) {
   const int* sperm = dI + O.sperm;
   const int* tperm = dI + O.tperm;
   hipLaunchKernelGGL(k_gather_sources, dim3(blocks(ns, 256)), dim3(256), 0, s, ns, sperm, c.src_dev,
-                     c.q_dev, c.ds_dev, c.dv_dev, f.ifcharge, f.ifdipole, spos, str64, (float*)nullptr);
+                     c.q_dev, c.ds_dev, c.dv_dev, f.ifcharge, f.ifdipole, spos, str64, str32);
   if (nt > 0)
     hipLaunchKernelGGL(k_gather_targets, dim3(blocks(nt, 256)), dim3(256), 0, s, nt, tperm, c.trg_dev, tpos);
   auto zero = [&](double* a, size_t n) {
@@ -242,10 +248,10 @@ int helm_solve(Context& ctx, const LapCall& c) {
   tm.mark(8);
   // STEP 8: P2P
   if (f.ifcharge || f.ifdipole) {
-    launch_hp2p(P, dI + O.p2p0, O.n_p2p0, 0, Hd.kre, Hd.kim, spos, str64, tpos, f.ifcharge, f.ifdipole,
-                f.iffld != 0, pot_s, fld_s, s);
-    launch_hp2p(P, dI + O.p2p1, O.n_p2p1, 1, Hd.kre, Hd.kim, spos, str64, tpos, f.ifcharge, f.ifdipole,
-                f.iffldtarg != 0, pot_t, fld_t, s);
+    launch_hp2p<TN>(P, dI + O.p2p0, O.n_p2p0, 0, Hd.kre, Hd.kim, spos, strn, tpos, f.ifcharge, f.ifdipole,
+                    f.iffld != 0, pot_s, fld_s, s);
+    launch_hp2p<TN>(P, dI + O.p2p1, O.n_p2p1, 1, Hd.kre, Hd.kim, spos, strn, tpos, f.ifcharge, f.ifdipole,
+                    f.iffldtarg != 0, pot_t, fld_t, s);
   }
   tm.mark(9);
   if (f.ifpot) hipLaunchKernelGGL(k_scatter, dim3(blocks(ns, 256)), dim3(256), 0, s, ns, 1, sperm, pot_s, c.pot_dev);
@@ -267,8 +273,11 @@ int helm_solve(Context& ctx, const LapCall& c) {
 }
 
 int helm_solve_mode(Context& ctx, const LapCall& c, int mode) {
-  if (mode != 0) return 128;
-  return helm_solve(ctx, c);
+  switch (mode) {
+    case 0: return helm_solve<double>(ctx, c);
+    case 1: return helm_solve<float>(ctx, c);
+  }
+  return 128;
 }
 
 // h3dpartdirect on the device: identity order, one pass.

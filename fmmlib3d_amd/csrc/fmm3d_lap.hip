@@ -6,6 +6,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <type_traits>
 #include <vector>
@@ -50,7 +51,15 @@ DevTables ensure_tables(Context& ctx, int p, double eps) {
     Dd.insert(Dd.end(), H.z_m2m.begin(), H.z_m2m.end());
     size_t o_zll = Dd.size();
     Dd.insert(Dd.end(), H.z_l2l.begin(), H.z_l2l.end());
+    size_t o_del = Dd.size();
+    Dd.insert(Dd.end(), H.delta.begin(), H.delta.end());
+    size_t o_re = Dd.size();
+    Dd.insert(Dd.end(), H.rot_e.begin(), H.rot_e.end());
     ctx.tab_host_l = H.rot_off;
+    ctx.tab_off_l = ctx.tab_host_l.size();
+    ctx.tab_host_l.insert(ctx.tab_host_l.end(), H.delta_off.begin(), H.delta_off.end());
+    ctx.tab_off[7] = o_del;
+    ctx.tab_off_re = o_re;
     ctx.tab_off[0] = o_fac; ctx.tab_off[1] = o_ph; ctx.tab_off[2] = o_oph; ctx.tab_off[3] = o_rot;
     ctx.tab_off[4] = o_zl; ctx.tab_off[5] = o_zm; ctx.tab_off[6] = o_zll;
     void* di = ctx.arena.get("tab_i", Ii.size() * sizeof(int));
@@ -73,6 +82,9 @@ DevTables ensure_tables(Context& ctx, int p, double eps) {
   T.rot = dd + ctx.tab_off[3]; T.z_m2l = dd + ctx.tab_off[4]; T.z_m2m = dd + ctx.tab_off[5];
   T.z_l2l = dd + ctx.tab_off[6];
   T.rot_off = dl;
+  T.delta = dd + ctx.tab_off[7];
+  T.delta_off = dl + ctx.tab_off_l;
+  T.rot_e = dd + ctx.tab_off_re;
   return T;
 }
 
@@ -187,10 +199,22 @@ int lap_solve(Context& ctx, const LapCall& c) {
   tm.mark(4);
   // ---- STEP 4: M2L
   if (P.n_m2l) {
-    if (sizeof(T) * 6 * (size_t)K <= max_lds_bytes(ctx))
-      hipLaunchKernelGGL((k_m2l<T, true>), dim3(P.n_m2l), dim3(256), sizeof(T) * 6 * K, s, P, Tb, mp, lo);
-    else
-      hipLaunchKernelGGL((k_m2l<T, false>), dim3(P.n_m2l), dim3(256), lds_ps, s, P, Tb, mp, lo);
+    // per-class rotation blocks by default (measured faster on MI355X at
+    // iprec 2: the factored form doubles the flops and M2L is not bound by
+    // the table reads); FMM3D_M2L_ROTATION=factored selects the other form
+    static const bool tables_rot = [] {
+      const char* v = std::getenv("FMM3D_M2L_ROTATION");
+      return !(v && std::strcmp(v, "factored") == 0);
+    }();
+    const bool lds = sizeof(T) * 6 * (size_t)K <= max_lds_bytes(ctx);
+    const size_t sh = lds ? sizeof(T) * 6 * K : lds_ps;
+    if (tables_rot) {
+      if (lds) hipLaunchKernelGGL((k_m2l<T, true>), dim3(P.n_m2l), dim3(256), sh, s, P, Tb, mp, lo);
+      else hipLaunchKernelGGL((k_m2l<T, false>), dim3(P.n_m2l), dim3(256), sh, s, P, Tb, mp, lo);
+    } else {
+      if (lds) hipLaunchKernelGGL((k_m2l_fact<T, true>), dim3(P.n_m2l), dim3(256), sh, s, P, Tb, mp, lo);
+      else hipLaunchKernelGGL((k_m2l_fact<T, false>), dim3(P.n_m2l), dim3(256), sh, s, P, Tb, mp, lo);
+    }
   }
   tm.mark(5);
   // ---- STEP 5: L2L, top-down

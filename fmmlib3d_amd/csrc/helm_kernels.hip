@@ -677,50 +677,64 @@ __global__ void k_hm2l(DevPlan P, DevTables Tb, HelmDev Hd, const double* __rest
 //           fld = -(v (1-ikr) e^{ikr}/r^3 + (v.x) x e^{ikr} (k^2 r^2 + 3ikr - 3)/r^5)
 // with v = dipstr dipvec.  Same work decomposition as the Laplace kernel.
 // ---------------------------------------------------------------------------
-template <bool CHG, bool DIP, bool FLD>
+template <typename TN>
+__device__ __forceinline__ void sincos_t(TN x, TN* s, TN* c);
+template <>
+__device__ __forceinline__ void sincos_t<double>(double x, double* s, double* c) { sincos(x, s, c); }
+template <>
+__device__ __forceinline__ void sincos_t<float>(float x, float* s, float* c) { sincosf(x, s, c); }
+template <typename TN>
+__device__ __forceinline__ TN exp_t(TN x);
+template <>
+__device__ __forceinline__ double exp_t<double>(double x) { return exp(x); }
+template <>
+__device__ __forceinline__ float exp_t<float>(float x) { return expf(x); }
+
+// Offsets and r^2 in FP64 always; with TN = float the rest of the pair is
+// FP32 (sums folded into FP64 by the caller every kFold sources).
+template <typename TN, bool CHG, bool DIP, bool FLD>
 struct HPairAcc {
-  double a[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-  __device__ __forceinline__ void add(double kre, double kim, double xt, double yt, double zt,
-                                      const double* __restrict__ sp, const double* __restrict__ st,
+  TN a[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  __device__ __forceinline__ void add(double kre_d, double kim_d, double xt, double yt, double zt,
+                                      const double* __restrict__ sp, const TN* __restrict__ st,
                                       bool mask) {
-    const double dx = xt - sp[0], dy = yt - sp[1], dz = zt - sp[2];
-    const double r2 = fma(dx, dx, fma(dy, dy, dz * dz));
-    double ir = rsqrt(r2);
-    ir = ir * (1.5 - 0.5 * r2 * ir * ir);   // one Newton step: rsqrt to full FP64 accuracy
-    if (mask) ir = 0.0;                      // a source never acts on itself
-    const double r = r2 * ir;
-    double s, c;
-    sincos(kre * r, &s, &c);
-    const double damp = kim == 0.0 ? 1.0 : exp(-kim * r);
-    const double gr = c * damp * ir, gi = s * damp * ir;   // G = e^{ikr}/r
-    const double ir2 = ir * ir;
-    // om = 1 - ikr = (1 + kim r) - i kre r
-    const double omr = 1.0 + kim * r, omi = -kre * r;
-    // g = om G / r^2
-    const double g_r = (omr * gr - omi * gi) * ir2, g_i = (omr * gi + omi * gr) * ir2;
+    const double dxd = xt - sp[0], dyd = yt - sp[1], dzd = zt - sp[2];
+    const double r2d = fma(dxd, dxd, fma(dyd, dyd, dzd * dzd));
+    const TN dx = (TN)dxd, dy = (TN)dyd, dz = (TN)dzd, r2 = (TN)r2d;
+    const TN kre = (TN)kre_d, kim = (TN)kim_d;
+    TN ir = rsqrt_t<TN>(r2);
+    ir = ir * (TN(1.5) - TN(0.5) * r2 * ir * ir);   // one Newton step
+    if (mask) ir = TN(0);                            // a source never acts on itself
+    const TN r = r2 * ir;
+    TN s, c;
+    sincos_t<TN>(kre * r, &s, &c);
+    const TN damp = kim == TN(0) ? TN(1) : exp_t<TN>(-kim * r);
+    const TN gr = c * damp * ir, gi = s * damp * ir;   // G = e^{ikr}/r
+    const TN ir2 = ir * ir;
+    const TN omr = TN(1) + kim * r, omi = -kre * r;     // 1 - ikr
+    const TN g_r = (omr * gr - omi * gi) * ir2, g_i = (omr * gi + omi * gr) * ir2;
     if (CHG) {
       a[0] += st[0] * gr - st[1] * gi;
       a[1] += st[0] * gi + st[1] * gr;
       if (FLD) {
-        const double cr = st[0] * g_r - st[1] * g_i, ci = st[0] * g_i + st[1] * g_r;
+        const TN cr = st[0] * g_r - st[1] * g_i, ci = st[0] * g_i + st[1] * g_r;
         a[2] += cr * dx; a[3] += ci * dx;
         a[4] += cr * dy; a[5] += ci * dy;
         a[6] += cr * dz; a[7] += ci * dz;
       }
     }
     if (DIP) {
-      const double dotr = st[2] * dx + st[4] * dy + st[6] * dz;
-      const double doti = st[3] * dx + st[5] * dy + st[7] * dz;
+      const TN dotr = st[2] * dx + st[4] * dy + st[6] * dz;
+      const TN doti = st[3] * dx + st[5] * dy + st[7] * dz;
       a[0] += dotr * g_r - doti * g_i;
       a[1] += dotr * g_i + doti * g_r;
       if (FLD) {
-        // h = G (k^2 r^2 + 3 i k r - 3) / r^4 ; k^2 = kre^2 - kim^2 + 2i kre kim, ikr = -kim r + i kre r
-        const double k2r = (kre * kre - kim * kim) * r2, k2i = 2.0 * kre * kim * r2;
-        const double wr = k2r - 3.0 * kim * r - 3.0, wi = k2i + 3.0 * kre * r;
-        const double ir4 = ir2 * ir2;
-        const double hr = (gr * wr - gi * wi) * ir4, hi = (gr * wi + gi * wr) * ir4;
-        const double ur = dotr * hr - doti * hi, ui = dotr * hi + doti * hr;   // (v.x) h
-        // fld -= v g + (v.x) h x
+        // h = G (k^2 r^2 + 3 i k r - 3) / r^4
+        const TN k2r = (kre * kre - kim * kim) * r2, k2i = TN(2) * kre * kim * r2;
+        const TN wr = k2r - TN(3) * kim * r - TN(3), wi = k2i + TN(3) * kre * r;
+        const TN ir4 = ir2 * ir2;
+        const TN hr = (gr * wr - gi * wi) * ir4, hi = (gr * wi + gi * wr) * ir4;
+        const TN ur = dotr * hr - doti * hi, ui = dotr * hi + doti * hr;
         a[2] -= (st[2] * g_r - st[3] * g_i) + ur * dx;
         a[3] -= (st[2] * g_i + st[3] * g_r) + ui * dx;
         a[4] -= (st[4] * g_r - st[5] * g_i) + ur * dy;
@@ -730,11 +744,17 @@ struct HPairAcc {
       }
     }
   }
+  __device__ __forceinline__ void fold_into(double* t) {
+    for (int k = 0; k < 8; k++) {
+      t[k] += (double)a[k];
+      a[k] = TN(0);
+    }
+  }
 };
 
-template <bool CHG, bool DIP, bool FLD>
+template <typename TN, bool CHG, bool DIP, bool FLD>
 __global__ __launch_bounds__(256) void k_hp2p(DevPlan P, double kre, double kim, const double* __restrict__ spos,
-                                              const double* __restrict__ str, const double* __restrict__ tpos,
+                                              const TN* __restrict__ str, const double* __restrict__ tpos,
                                               int kind, double* __restrict__ pot, double* __restrict__ fld) {
   const int w = blockIdx.x;
   const int b = P.p2p_items[3 * w], start = P.p2p_items[3 * w + 2];
@@ -745,22 +765,33 @@ __global__ __launch_bounds__(256) void k_hp2p(DevPlan P, double kre, double kim,
   const int me = n0 + (active ? i : 0);
   const double* mp = (kind ? tpos : spos) + 4 * (size_t)me;
   const double x = mp[0], y = mp[1], z = mp[2];
-  HPairAcc<CHG, DIP, FLD> acc;
+  HPairAcc<TN, CHG, DIP, FLD> acc;
+  double tot[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  constexpr bool fold = sizeof(TN) < sizeof(double);
   {
     const int s0 = P.isrc[b], s1 = s0 + P.nsrc[b];
     const int self = kind ? -1 : me;
-    for (int s = s0; s < s1; s++) acc.add(kre, kim, x, y, z, spos + 4 * (size_t)s, str + 8 * (size_t)s, s == self);
+    for (int c = s0; c < s1; c += (fold ? kFold : (1 << 30))) {
+      const int ce = fold ? min(c + kFold, s1) : s1;
+      for (int s = c; s < ce; s++) acc.add(kre, kim, x, y, z, spos + 4 * (size_t)s, str + 8 * (size_t)s, s == self);
+      if (fold) acc.fold_into(tot);
+    }
   }
   for (int jj = P.l1_off[b]; jj < P.l1_off[b + 1]; jj++) {
     const int j = P.l1_idx[jj];
     const int s0 = P.isrc[j], s1 = s0 + P.nsrc[j];
-    for (int s = s0; s < s1; s++) acc.add(kre, kim, x, y, z, spos + 4 * (size_t)s, str + 8 * (size_t)s, false);
+    for (int c = s0; c < s1; c += (fold ? kFold : (1 << 30))) {
+      const int ce = fold ? min(c + kFold, s1) : s1;
+      for (int s = c; s < ce; s++) acc.add(kre, kim, x, y, z, spos + 4 * (size_t)s, str + 8 * (size_t)s, false);
+      if (fold) acc.fold_into(tot);
+    }
   }
+  if (!fold) acc.fold_into(tot);
   if (!active) return;
-  pot[2 * (size_t)me] += acc.a[0];
-  pot[2 * (size_t)me + 1] += acc.a[1];
+  pot[2 * (size_t)me] += tot[0];
+  pot[2 * (size_t)me + 1] += tot[1];
   if (FLD)
-    for (int k = 0; k < 6; k++) fld[6 * (size_t)me + k] += acc.a[2 + k];
+    for (int k = 0; k < 6; k++) fld[6 * (size_t)me + k] += tot[2 + k];
 }
 
 // h3dpartdirect (src/hfmm3dpart.f:1268): every source on every point
@@ -774,7 +805,7 @@ __global__ __launch_bounds__(256) void k_hdirect(double kre, double kim, int ns,
   const int me = active ? i : 0;
   const double* mp = (pts_are_sources ? spos : pts) + 4 * (size_t)me;
   const double x = mp[0], y = mp[1], z = mp[2];
-  HPairAcc<CHG, DIP, true> acc;
+  HPairAcc<double, CHG, DIP, true> acc;
   const int self = pts_are_sources ? me : -1;
   for (int s = 0; s < ns; s++) acc.add(kre, kim, x, y, z, spos + 4 * (size_t)s, str + 8 * (size_t)s, s == self);
   if (!active) return;

@@ -195,4 +195,15 @@ __device__ void rotate_block(const DevTables& Tb, int rc, int pmax, const T* in,
   }
 }
 
+// Near-field precision helpers: the pair arithmetic after r^2 runs in TN
+// (double or float); FP32 partial sums are folded into FP64 every kFold sources.
+constexpr int kFold = 32;
+
+template <typename TN>
+__device__ __forceinline__ TN rsqrt_t(TN x);
+template <>
+__device__ __forceinline__ double rsqrt_t<double>(double x) { return rsqrt(x); }
+template <>
+__device__ __forceinline__ float rsqrt_t<float>(float x) { return rsqrtf(x); }
+
 }  // namespace fmm3d

@@ -416,6 +416,178 @@ __global__ void k_m2l(DevPlan P, DevTables Tb, const T* __restrict__ mp, T* __re
 }
 
 // ---------------------------------------------------------------------------
+// M2L with factored rotations.  A rotation by theta about y is
+//   d^n(theta) = P Delta^n E(theta) Delta^nT P*,  Delta^n = d^n(pi/2),
+//   P = diag(i^m), E = diag(e^{i m theta}),
+// so every pair uses the same Delta blocks (they stay in cache) and only a
+// diagonal phase depends on the pair; each rotation costs two products with
+// Delta instead of one with a per-class matrix streamed from memory.
+// Both products read Delta column-wise through d_ab = (-1)^{a-b} d_ba.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ int mod4(int m) { return ((m % 4) + 4) % 4; }
+
+// multiply (re, im) by i^r, r = 0..3
+template <typename T>
+__device__ __forceinline__ void mul_ipow(T& re, T& im, int r) {
+  T a = re, b = im;
+  if (r == 1) { re = -b; im = a; }
+  else if (r == 2) { re = -a; im = -b; }
+  else if (r == 3) { re = b; im = -a; }
+}
+
+// out_k = sum_m Delta^n_{mk} in_m for every (n, k) with n <= pmax
+// (transpose product), then out_k *= e^{i s k theta} (s = +1 or -1).
+template <typename T>
+__device__ void delta_t_times(const DevTables& Tb, int pmax, const double* e, int s, const T* in, T* out) {
+  const int Kt = (pmax + 1) * (pmax + 1);
+  for (int idx = threadIdx.x; idx < Kt; idx += blockDim.x) {
+    int n = (int)sqrtf((float)idx);
+    while (n * n > idx) n--;
+    while ((n + 1) * (n + 1) <= idx) n++;
+    const int k = idx - n * n - n, dim = 2 * n + 1;
+    const double* D = Tb.delta + Tb.delta_off[n];
+    T ar = 0, ai = 0;
+    for (int m = -n; m <= n; m++) {
+      const T d = (T)D[(m + n) * dim + (k + n)];
+      ar += d * in[2 * eidx(n, m)];
+      ai += d * in[2 * eidx(n, m) + 1];
+    }
+    const int ak = k < 0 ? -k : k;
+    const T er = (T)e[2 * ak];
+    T ei = (T)e[2 * ak + 1];
+    if ((k < 0) != (s < 0)) ei = -ei;
+    out[2 * idx] = ar * er - ai * ei;
+    out[2 * idx + 1] = ar * ei + ai * er;
+  }
+}
+
+// out_a = sum_b Delta^n_{ab} in_b = sum_b (-1)^{a-b} Delta^n_{ba} in_b
+template <typename T>
+__device__ void delta_times(const DevTables& Tb, int pmax, const T* in, T* out) {
+  const int Kt = (pmax + 1) * (pmax + 1);
+  for (int idx = threadIdx.x; idx < Kt; idx += blockDim.x) {
+    int n = (int)sqrtf((float)idx);
+    while (n * n > idx) n--;
+    while ((n + 1) * (n + 1) <= idx) n++;
+    const int a = idx - n * n - n, dim = 2 * n + 1;
+    const double* D = Tb.delta + Tb.delta_off[n];
+    T ar = 0, ai = 0;
+    for (int b = -n; b <= n; b++) {
+      T d = (T)D[(b + n) * dim + (a + n)];
+      if ((a - b) & 1) d = -d;
+      ar += d * in[2 * eidx(n, b)];
+      ai += d * in[2 * eidx(n, b) + 1];
+    }
+    out[2 * idx] = ar;
+    out[2 * idx + 1] = ai;
+  }
+}
+
+// x_m *= (-i)^m e_m over degrees <= pmax (input factor of a factored rotation)
+template <typename T>
+__device__ void pre_rot(int pmax, T* x) {
+  const int Kt = (pmax + 1) * (pmax + 1);
+  for (int idx = threadIdx.x; idx < Kt; idx += blockDim.x) {
+    int n = (int)sqrtf((float)idx);
+    while (n * n > idx) n--;
+    while ((n + 1) * (n + 1) <= idx) n++;
+    const int m = idx - n * n - n;
+    T re = x[2 * idx], im = x[2 * idx + 1];
+    mul_ipow(re, im, mod4(-m));
+    if (m > 0 && (m & 1)) { re = -re; im = -im; }
+    x[2 * idx] = re;
+    x[2 * idx + 1] = im;
+  }
+}
+
+template <typename T, bool LDSACC>
+__global__ void k_m2l_fact(DevPlan P, DevTables Tb, const T* __restrict__ mp, T* __restrict__ lo) {
+  extern __shared__ unsigned char smem_raw[];
+  T* A = reinterpret_cast<T*>(smem_raw);
+  T* B = A + 2 * P.K;
+  const int b = P.m2l_boxes[blockIdx.x];
+  T* acc = LDSACC ? B + 2 * P.K : lo + 2 * (size_t)b * P.K;
+  const int p = P.p;
+  const T ih = (T)P.inv_side[P.level[b]];
+  const size_t zsz = (size_t)(p + 1) * (p + 1) * (p + 1);
+  if (LDSACC)
+    for (int k = threadIdx.x; k < 2 * P.K; k += blockDim.x) acc[k] = T(0);
+  __syncthreads();
+  for (int jj = P.l2_off[b]; jj < P.l2_off[b + 1]; jj++) {
+    const int j = P.l2_idx[jj];
+    const int di = P.ijk[3 * j] - P.ijk[3 * b] + 3, dj = P.ijk[3 * j + 1] - P.ijk[3 * b + 1] + 3,
+              dk = P.ijk[3 * j + 2] - P.ijk[3 * b + 2] + 3;
+    const int code = (di * 7 + dj) * 7 + dk;
+    const int pt = Tb.off_ptrunc[code];
+    const int Ki = (pt + 1) * (pt + 1);
+    const double* phase = Tb.off_phase + (size_t)code * 2 * (p + 1);
+    const double* e = Tb.rot_e + (size_t)Tb.off_rot[code] * 2 * (p + 1);
+    const double* Z = Tb.z_m2l + (size_t)Tb.off_dist[code] * zsz;
+    const T* in = mp + 2 * (size_t)j * P.K;
+    // A = in e^{i m phi} (-i)^m e_m
+    for (int k = threadIdx.x; k < Ki; k += blockDim.x) {
+      int n = (int)sqrtf((float)k);
+      while (n * n > k) n--;
+      while ((n + 1) * (n + 1) <= k) n++;
+      const int m = k - n * n - n, am = m < 0 ? -m : m;
+      const T er = (T)phase[2 * am];
+      const T ei = m < 0 ? -(T)phase[2 * am + 1] : (T)phase[2 * am + 1];
+      const T xr = in[2 * k], xi = in[2 * k + 1];
+      T re = xr * er - xi * ei, im = xr * ei + xi * er;
+      mul_ipow(re, im, mod4(-m));
+      if (m > 0 && (m & 1)) { re = -re; im = -im; }
+      A[2 * k] = re;
+      A[2 * k + 1] = im;
+    }
+    __syncthreads();
+    delta_t_times<T>(Tb, pt, e, +1, A, B);   // B_k = e^{ik theta} sum_m Delta_mk A_m
+    __syncthreads();
+    delta_times<T>(Tb, pt, B, A);            // A_a = sum_k Delta_ak B_k
+    __syncthreads();
+    // the rotated vector is e_a i^a A_a; fold those factors into the shift
+    for (int k = threadIdx.x; k < Ki; k += blockDim.x) {
+      int n = (int)sqrtf((float)k);
+      while (n * n > k) n--;
+      while ((n + 1) * (n + 1) <= k) n++;
+      const int m = k - n * n - n;
+      T re = A[2 * k], im = A[2 * k + 1];
+      mul_ipow(re, im, mod4(m));
+      if (m > 0 && (m & 1)) { re = -re; im = -im; }
+      A[2 * k] = re;
+      A[2 * k + 1] = im;
+    }
+    __syncthreads();
+    zshift_block<T>(Z, p, pt, pt, A, B);
+    __syncthreads();
+    pre_rot<T>(pt, B);
+    __syncthreads();
+    delta_t_times<T>(Tb, pt, e, -1, B, A);   // back rotation: conj(E)
+    __syncthreads();
+    delta_times<T>(Tb, pt, A, B);
+    __syncthreads();
+    // acc += e_a i^a B_a e^{-i a phi} / h
+    for (int k = threadIdx.x; k < Ki; k += blockDim.x) {
+      int n = (int)sqrtf((float)k);
+      while (n * n > k) n--;
+      while ((n + 1) * (n + 1) <= k) n++;
+      const int m = k - n * n - n, am = m < 0 ? -m : m;
+      T re = B[2 * k], im = B[2 * k + 1];
+      mul_ipow(re, im, mod4(m));
+      if (m > 0 && (m & 1)) { re = -re; im = -im; }
+      const T er = (T)phase[2 * am];
+      const T ei = m < 0 ? (T)phase[2 * am + 1] : -(T)phase[2 * am + 1];
+      acc[2 * k] += (re * er - im * ei) * ih;
+      acc[2 * k + 1] += (re * ei + im * er) * ih;
+    }
+    __syncthreads();
+  }
+  if (LDSACC) {
+    T* out = lo + 2 * (size_t)b * P.K;
+    for (int k = threadIdx.x; k < 2 * P.K; k += blockDim.x) out[k] += acc[k];
+  }
+}
+
+// ---------------------------------------------------------------------------
 // P2P near field (src/lfmm3dpart.f:1076-1141, lfmm3dpart_direct_self/_targ).
 // One workgroup per (leaf, chunk of evaluation points); one point per thread.
 // The source loop is wave-uniform, so positions and strengths stream through
@@ -424,14 +596,6 @@ __global__ void k_m2l(DevPlan P, DevTables Tb, const T* __restrict__ mp, T* __re
 // pair is FP32, summed in FP32 over runs of at most kFold sources and folded
 // into FP64 accumulators.
 // ---------------------------------------------------------------------------
-constexpr int kFold = 32;
-
-template <typename TN>
-__device__ __forceinline__ TN rsqrt_t(TN x);
-template <>
-__device__ __forceinline__ double rsqrt_t<double>(double x) { return rsqrt(x); }
-template <>
-__device__ __forceinline__ float rsqrt_t<float>(float x) { return rsqrtf(x); }
 
 template <typename TN, bool CHG, bool DIP, bool FLD>
 struct PairAcc {

@@ -50,6 +50,9 @@ struct DevTables {
   const double *oct_phase;  // 8 octants x 2 (m2m, l2l) x (p+1) complex
   const int *oct_rot;       // 8 octants x 2 -> rotation class (m2m, l2l)
   const double *fac;        // derivative factors, see lap_kernels.hip
+  const double *delta;      // d^n(pi/2) per degree, row-major (2n+1)^2
+  const long *delta_off;    // start of degree n
+  const double *rot_e;      // per rotation class: e^{i k theta}, k = 0..p
 };
 
 }  // namespace fmm3d

@@ -59,6 +59,7 @@ struct Context {
   std::vector<double> tab_host_d;
   std::vector<long> tab_host_l;
   size_t tab_off[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  size_t tab_off_l = 0, tab_off_re = 0;
   // cached Helmholtz tables (owned by the Helmholtz driver)
   std::shared_ptr<void> helm_tables;
 };

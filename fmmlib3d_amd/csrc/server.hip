@@ -209,7 +209,7 @@ long long fmm3d_server_submit(void* server, int kernel, int prec_mode, int iprec
   auto* S = static_cast<Server*>(server);
   if (!S || nsource < 1 || ntarget < 0 || (kernel != 0 && kernel != 1)) return -128;
   if (iprec < -2 || iprec > 6) return -32;
-  if (kernel == 1 && (!zk || (zk[0] == 0.0 && zk[1] == 0.0) || prec_mode != 0)) return -128;
+  if (kernel == 1 && (!zk || (zk[0] == 0.0 && zk[1] == 0.0) || (prec_mode != 0 && prec_mode != 1))) return -128;
   if (kernel == 0 && (prec_mode < 0 || prec_mode > 3)) return -128;
   auto r = std::make_shared<fmm3d::Request>();
   r->kernel = kernel; r->prec = prec_mode; r->iprec = iprec;

@@ -256,6 +256,28 @@ void build_geometry_tables(int p, HostTables& H) {
     }
   }
   H.nrot = nrot;
+  // factored form: one Delta = d(pi/2) for every class plus a phase per class
+  {
+    auto dd = wigner_d(p, M_PI / 2.0);
+    H.delta_off.assign(P1, 0);
+    long tot = 0;
+    for (int n = 0; n <= p; n++) {
+      H.delta_off[n] = tot;
+      tot += (long)(2 * n + 1) * (2 * n + 1);
+    }
+    H.delta.assign(tot, 0.0);
+    for (int n = 0; n <= p; n++) {
+      const int dim = 2 * n + 1;
+      for (int a = 0; a < dim; a++)
+        for (int b = 0; b < dim; b++) H.delta[H.delta_off[n] + (long)a * dim + b] = dd[n][(size_t)a * dim + b];
+    }
+    H.rot_e.assign((size_t)nrot * 2 * P1, 0.0);
+    for (int c = 0; c < nrot; c++)
+      for (int k = 0; k <= p; k++) {
+        H.rot_e[((size_t)c * P1 + k) * 2] = std::cos(k * thetas[c]);
+        H.rot_e[((size_t)c * P1 + k) * 2 + 1] = std::sin(k * thetas[c]);
+      }
+  }
   H.ndist = (int)dists.size();
   H.dist_value = dists;
 }

@@ -15,6 +15,11 @@ struct HostTables {
   std::vector<long> rot_off;                       // nrot x (p+1)
   std::vector<double> z_m2l, z_m2m, z_l2l;         // dense [m][out][in]
   std::vector<double> dist_value;                  // |offset| of each distance class
+  // Factored rotations: d^n(theta) = P Delta^n E(theta) Delta^nT P*, with
+  // Delta^n = d^n(pi/2), P = diag(i^m), E = diag(e^{i m theta}).
+  std::vector<double> delta;                       // per degree (2n+1)^2, row a col b = d^n_ab(pi/2)
+  std::vector<long> delta_off;                     // p+1
+  std::vector<double> rot_e;                       // nrot x (p+1) complex: e^{i k theta_class}
 };
 
 // l3dterms (src/l3dterms.f:34): expansion order for tolerance eps

@@ -102,8 +102,9 @@ void fmm3d_h3dpartdirect(const double *zk, int nsource, const double *source,
                          int ntarget, const double *target,
                          int ifpottarg, double *pottarg, int iffldtarg, double *fldtarg);
 
-/* fmm3d_hfmm3dparttarg with a precision mode (FMM3D_PREC_FP64 only so far;
- * other modes return 128).  Extension with no reference counterpart. */
+/* fmm3d_hfmm3dparttarg with a precision mode: FMM3D_PREC_FP64, or
+ * FMM3D_PREC_MIXED (FP32 near-field pair arithmetic, FP64 translations);
+ * other modes return 128.  Extension with no reference counterpart. */
 int fmm3d_hfmm3dparttarg_ex(int prec_mode, int iprec, const double *zk, int nsource,
                             const double *source, int ifcharge, const double *charge,
                             int ifdipole, const double *dipstr, const double *dipvec,

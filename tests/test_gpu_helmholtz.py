@@ -151,3 +151,20 @@ def test_gpu_helm_sphere_500k_accuracy():
     a = F.hfmm3dpartself(2, 20.0, src, q2)
     b = F.hfmm3dpartself(2, 20.0, src, q + q2)
     assert rel(pot + a[1], b[1]) < 1e-12
+
+
+def test_gpu_helm_fp32_near_field_within_tolerance(rng):
+    """FMM3D_PREC_MIXED: FP32 pair arithmetic in the near field, FP64
+    translations; against the FP64 direct sum at the iprec tolerance (never
+    tighter than 1e-6) for the potential and a digit less for the field."""
+    for iprec, dist, zk in ((1, "cube", 6.0), (2, "sphere", 20.0), (2, "clusters", 2.0 + 0.5j)):
+        src = points(rng, 20000, dist)
+        trg = points(rng, 4000, dist)
+        q, ds, dv = strengths(rng, 20000, True, True)
+        g = F.hfmm3dparttarg(iprec, zk, src, q, ds, dv, trg, precision=F.PREC_MIXED)
+        assert g[0] == 0
+        idx = rng.choice(4000, 300, replace=False)
+        pd, fd = ob.orc_direct_sample(src, trg[idx], -np.ones(300, np.int32), q, ds, dv, zk=zk)
+        tol = max(IPREC_EPS[iprec], 1e-6)
+        assert rel(g[3][idx], pd) < tol
+        assert rel(g[4][idx], fd) < 10 * tol

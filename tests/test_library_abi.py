@@ -53,6 +53,7 @@ def test_argument_errors_before_device_use():
     assert F.hfmm3dparttarg(7, 2.0, src, q)[0] == 32
     assert F.hfmm3dparttarg(1, 0.0, src, q)[0] == 128
     assert F.hfmm3dparttarg(1, 2.0, src, q, precision=F.PREC_FP32)[0] == 128
+    assert F.hfmm3dparttarg(1, 2.0, src, q, precision=F.PREC_FAR32)[0] == 128
     assert lib.fmm3d_hfmm3dparttarg(1, None, 4, F._ptr(src), 1, F._ptr(q), 0, None, None, 1, None, 0,
                                     None, 0, None, 0, None, 0, None) == 128
 
