@@ -43,6 +43,10 @@ SEED = 2
 FP64_PEAK_TFLOPS = 78.6        # MI355X FP64 vector (= FP64 matrix) peak
 FP32_PEAK_TFLOPS = 157.3       # MI355X FP32 vector peak (packed)
 FLOPS_PER_PAIR = 65            # charge+dipole, pot+fld, rsqrt counted as 2
+# HBM bytes per k_p2p launch on this workload, from rocprofv3 FETCH_SIZE and
+# WRITE_SIZE runs calibrated against a 1 GiB device copy
+# (profiles/fmm_kernels_config2_config3.md): 2 x FETCH KiB + WRITE KiB.
+P2P_TRAFFIC_BYTES = {"fp64": (2 * 2316467 + 62668) * 1024, "fp32": (2 * 1462139 + 62655) * 1024}
 
 
 def make_inputs(n, seed):
@@ -328,7 +332,8 @@ def main():
         "p50_ms": M["p50_ms"], "p99_ms": M["p99_ms"],
         "rel_err_vs_fp64_direct": {"pottarg": M["err_pottarg"], "fldtarg": M["err_fldtarg"], "sampled_targets": 1000},
         "roofline": {"kernel": "k_p2p (near field, both launches)", "bound": "mfma", "achieved": achieved,
-                     "peak": peak, "unit": "TFLOP/s", "frac": achieved / peak, "traffic": None,
+                     "peak": peak, "unit": "TFLOP/s", "frac": achieved / peak,
+                     "traffic": P2P_TRAFFIC_BYTES["fp64" if best == "fp64" else "fp32"],
                      "note": "P2P is FP64 VALU bound; MI355X FP64 vector peak equals its FP64 matrix peak. "
                              "algorithmic flops = %d pairs x %d" % (pairs, FLOPS_PER_PAIR)},
         "cpu_baseline": cpu,

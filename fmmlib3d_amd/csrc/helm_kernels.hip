@@ -540,24 +540,33 @@ __global__ __launch_bounds__(64) void k_heval2(DevPlan P, HelmDev Hd, const doub
 // operators by projection).  `acc` accumulates in LDS or global memory.
 // ---------------------------------------------------------------------------
 __device__ void hzshift_block(const HelmDev& Hd, int t, int pin, int pout, const double* in, double* out) {
+  // orders m and -m share the block of |m|: one thread produces both
   const long base = Hd.coax[3 * t];
   const int tin = (int)Hd.coax[3 * t + 1], tout = (int)Hd.coax[3 * t + 2];
   const double* Z = Hd.z + 2 * base;
-  const int Kt = (pout + 1) * (pout + 1);
-  for (int k = threadIdx.x; k < Kt; k += blockDim.x) {
-    int l = (int)sqrtf((float)k);
-    while (l * l > k) l--;
-    while ((l + 1) * (l + 1) <= k) l++;
-    const int m = k - l * l - l, am = m < 0 ? -m : m;
-    Cd a = {0.0, 0.0};
+  const int Kh = (pout + 1) * (pout + 2) / 2;
+  for (int k = threadIdx.x; k < Kh; k += blockDim.x) {
+    int l = (int)((sqrtf(8.0f * k + 1.0f) - 1.0f) * 0.5f);
+    while (l * (l + 1) / 2 > k) l--;
+    while ((l + 1) * (l + 2) / 2 <= k) l++;
+    const int am = k - l * (l + 1) / 2;
+    Cd a = {0.0, 0.0}, b2 = {0.0, 0.0};
     if (am <= pin) {
       const long A = tout + 1, Bc = tin + 1, M = am;
       const long mb = M * A * Bc - (A + Bc) * M * (M - 1) / 2 + (M - 1) * M * (2 * M - 1) / 6;
       const double* zr = Z + 2 * (mb + (long)(l - am) * (tin + 1 - am));
-      for (int n = am; n <= pin; n++) cfma(a, Cd{zr[2 * (n - am)], zr[2 * (n - am) + 1]}, cld(in, eidx(n, m)));
+      for (int n = am; n <= pin; n++) {
+        const Cd zz = {zr[2 * (n - am)], zr[2 * (n - am) + 1]};
+        cfma(a, zz, cld(in, eidx(n, am)));
+        cfma(b2, zz, cld(in, eidx(n, -am)));
+      }
     }
-    out[2 * k] = a.re;
-    out[2 * k + 1] = a.im;
+    out[2 * eidx(l, am)] = a.re;
+    out[2 * eidx(l, am) + 1] = a.im;
+    if (am > 0) {
+      out[2 * eidx(l, -am)] = b2.re;
+      out[2 * eidx(l, -am) + 1] = b2.im;
+    }
   }
 }
 

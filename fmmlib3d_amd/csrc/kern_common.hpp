@@ -172,26 +172,36 @@ __global__ void k_fill_zero(size_t n, T* __restrict__ a) {
 // rotation when `back` (same blocks, (-1)^{m+m'} signs).
 template <typename T>
 __device__ void rotate_block(const DevTables& Tb, int rc, int pmax, const T* in, T* out, bool back) {
+  // The blocks satisfy D[-m][-m'] = D[m][m'], so one thread produces the
+  // outputs m' and -m' of a degree from a single pass over column m' of D:
+  //   out_{m'} = sum_m D[m][m'] in_m,   out_{-m'} = sum_m D[m][m'] in_{-m}.
   const int p = Tb.p;
-  const int Kt = (pmax + 1) * (pmax + 1);
-  for (int k = threadIdx.x; k < Kt; k += blockDim.x) {
-    int n = (int)sqrtf((float)k);
-    while (n * n > k) n--;
-    while ((n + 1) * (n + 1) <= k) n++;
-    int mp = k - n * n - n;
+  const int Kh = (pmax + 1) * (pmax + 2) / 2;
+  for (int k = threadIdx.x; k < Kh; k += blockDim.x) {
+    int n = (int)((sqrtf(8.0f * k + 1.0f) - 1.0f) * 0.5f);
+    while (n * (n + 1) / 2 > k) n--;
+    while ((n + 1) * (n + 2) / 2 <= k) n++;
+    const int mp = k - n * (n + 1) / 2;   // 0..n
     const double* D = Tb.rot + Tb.rot_off[rc * (p + 1) + n];
     const int dim = 2 * n + 1;
-    T are = 0, aim = 0;
+    T pr = 0, pi = 0, qr = 0, qi = 0;
     for (int m = -n; m <= n; m++) {
-      T d = (T)D[(m + n) * dim + (mp + n)];
+      const T d = (T)D[(m + n) * dim + (mp + n)];
       T xr = in[2 * eidx(n, m)], xi = in[2 * eidx(n, m) + 1];
-      if (back && (m & 1)) { xr = -xr; xi = -xi; }
-      are += d * xr;
-      aim += d * xi;
+      T yr = in[2 * eidx(n, -m)], yi = in[2 * eidx(n, -m) + 1];
+      if (back && (m & 1)) { xr = -xr; xi = -xi; yr = -yr; yi = -yi; }
+      pr += d * xr;
+      pi += d * xi;
+      qr += d * yr;
+      qi += d * yi;
     }
-    if (back && (mp & 1)) { are = -are; aim = -aim; }
-    out[2 * k] = are;
-    out[2 * k + 1] = aim;
+    if (back && (mp & 1)) { pr = -pr; pi = -pi; qr = -qr; qi = -qi; }
+    out[2 * eidx(n, mp)] = pr;
+    out[2 * eidx(n, mp) + 1] = pi;
+    if (mp > 0) {
+      out[2 * eidx(n, -mp)] = qr;
+      out[2 * eidx(n, -mp) + 1] = qi;
+    }
   }
 }
 

@@ -283,22 +283,29 @@ __global__ void k_eval(DevPlan P, const double* __restrict__ fac, const T* __res
 // ---------------------------------------------------------------------------
 template <typename T>
 __device__ void zshift_block(const double* Z, int p, int pin, int pout, const T* in, T* out) {
-  const int Kt = (pout + 1) * (pout + 1);
+  // orders m and -m share the row Z[|m|][l][.]: one thread produces both
+  const int Kh = (pout + 1) * (pout + 2) / 2;
   const int P1 = p + 1;
-  for (int k = threadIdx.x; k < Kt; k += blockDim.x) {
-    int l = (int)sqrtf((float)k);
-    while (l * l > k) l--;
-    while ((l + 1) * (l + 1) <= k) l++;
-    int m = k - l * l - l, am = m < 0 ? -m : m;
+  for (int k = threadIdx.x; k < Kh; k += blockDim.x) {
+    int l = (int)((sqrtf(8.0f * k + 1.0f) - 1.0f) * 0.5f);
+    while (l * (l + 1) / 2 > k) l--;
+    while ((l + 1) * (l + 2) / 2 <= k) l++;
+    const int am = k - l * (l + 1) / 2;
     const double* z = Z + ((size_t)am * P1 + l) * P1;
-    T are = 0, aim = 0;
+    T pr = 0, pi = 0, qr = 0, qi = 0;
     for (int n = am; n <= pin; n++) {
-      T c = (T)z[n];
-      are += c * in[2 * eidx(n, m)];
-      aim += c * in[2 * eidx(n, m) + 1];
+      const T c = (T)z[n];
+      pr += c * in[2 * eidx(n, am)];
+      pi += c * in[2 * eidx(n, am) + 1];
+      qr += c * in[2 * eidx(n, -am)];
+      qi += c * in[2 * eidx(n, -am) + 1];
     }
-    out[2 * k] = are;
-    out[2 * k + 1] = aim;
+    out[2 * eidx(l, am)] = pr;
+    out[2 * eidx(l, am) + 1] = pi;
+    if (am > 0) {
+      out[2 * eidx(l, -am)] = qr;
+      out[2 * eidx(l, -am) + 1] = qi;
+    }
   }
 }
 
