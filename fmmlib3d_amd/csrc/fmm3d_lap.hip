@@ -37,6 +37,11 @@ DevTables ensure_tables(Context& ctx, int p, double eps) {
     Ii.insert(Ii.end(), H.off_dist.begin(), H.off_dist.end());
     Ii.insert(Ii.end(), H.off_ptrunc.begin(), H.off_ptrunc.end());
     Ii.insert(Ii.end(), H.oct_rot.begin(), H.oct_rot.end());
+    const std::vector<int>* gi[4] = {&H.grp_ritems, &H.grp_rcount, &H.grp_zitems, &H.grp_zoff};
+    for (int q = 0; q < 4; q++) {
+      ctx.tab_off_i[q] = Ii.size();
+      Ii.insert(Ii.end(), gi[q]->begin(), gi[q]->end());
+    }
     size_t o_fac = 0;
     Dd.insert(Dd.end(), H.fac.begin(), H.fac.end());
     size_t o_ph = Dd.size();
@@ -55,9 +60,21 @@ DevTables ensure_tables(Context& ctx, int p, double eps) {
     Dd.insert(Dd.end(), H.delta.begin(), H.delta.end());
     size_t o_re = Dd.size();
     Dd.insert(Dd.end(), H.rot_e.begin(), H.rot_e.end());
+    // 32-byte aligned starts for the padded tables of the grouped M2L
+    while (Dd.size() % 4) Dd.push_back(0.0);
+    ctx.tab_off_g[0] = Dd.size();
+    Dd.insert(Dd.end(), H.rot_half.begin(), H.rot_half.end());
+    while (Dd.size() % 4) Dd.push_back(0.0);
+    ctx.tab_off_g[1] = Dd.size();
+    Dd.insert(Dd.end(), H.z_m2l_t.begin(), H.z_m2l_t.end());
     ctx.tab_host_l = H.rot_off;
     ctx.tab_off_l = ctx.tab_host_l.size();
     ctx.tab_host_l.insert(ctx.tab_host_l.end(), H.delta_off.begin(), H.delta_off.end());
+    ctx.tab_off_g[2] = ctx.tab_host_l.size();
+    ctx.tab_host_l.insert(ctx.tab_host_l.end(), H.rot_half_off.begin(), H.rot_half_off.end());
+    ctx.tab_off_g[3] = ctx.tab_host_l.size();
+    ctx.tab_host_l.insert(ctx.tab_host_l.end(), H.zt_off.begin(), H.zt_off.end());
+    ctx.tab_zt_size = H.zt_size;
     ctx.tab_off[7] = o_del;
     ctx.tab_off_re = o_re;
     ctx.tab_off[0] = o_fac; ctx.tab_off[1] = o_ph; ctx.tab_off[2] = o_oph; ctx.tab_off[3] = o_rot;
@@ -85,6 +102,11 @@ DevTables ensure_tables(Context& ctx, int p, double eps) {
   T.delta = dd + ctx.tab_off[7];
   T.delta_off = dl + ctx.tab_off_l;
   T.rot_e = dd + ctx.tab_off_re;
+  T.grp_ritems = di + ctx.tab_off_i[0]; T.grp_rcount = di + ctx.tab_off_i[1];
+  T.grp_zitems = di + ctx.tab_off_i[2]; T.grp_zoff = di + ctx.tab_off_i[3];
+  T.rot_half = dd + ctx.tab_off_g[0]; T.z_m2l_t = dd + ctx.tab_off_g[1];
+  T.rot_half_off = dl + ctx.tab_off_g[2]; T.zt_off = dl + ctx.tab_off_g[3];
+  T.zt_size = ctx.tab_zt_size;
   return T;
 }
 
@@ -199,16 +221,31 @@ int lap_solve(Context& ctx, const LapCall& c) {
   tm.mark(4);
   // ---- STEP 4: M2L
   if (P.n_m2l) {
-    // per-class rotation blocks by default (measured faster on MI355X at
-    // iprec 2: the factored form doubles the flops and M2L is not bound by
-    // the table reads); FMM3D_M2L_ROTATION=factored selects the other form
-    static const bool tables_rot = [] {
+    // Default: pairs grouped by translation class (k_m2l_grp).
+    // FMM3D_M2L_ROTATION=pairs selects one pair at a time with per-class
+    // rotation blocks, =factored the factored rotations.
+    static const int m2l_form = [] {
       const char* v = std::getenv("FMM3D_M2L_ROTATION");
-      return !(v && std::strcmp(v, "factored") == 0);
+      if (v && std::strcmp(v, "factored") == 0) return 2;
+      if (v && std::strcmp(v, "pairs") == 0) return 1;
+      return 0;
     }();
     const bool lds = sizeof(T) * 6 * (size_t)K <= max_lds_bytes(ctx);
     const size_t sh = lds ? sizeof(T) * 6 * K : lds_ps;
-    if (tables_rot) {
+    // group size: work items of one stage must fit kM2LItems per thread, the
+    // group's expansions must fit LDS
+    int nI = 0;
+    for (int n = 0; n <= p; n++) nI += n / 4 + 1;
+    int gmax = 0;
+    for (int g = kM2LGroup; g >= 1 && !gmax; g--) {
+      const int gs = g > 2 ? 4 : g;
+      const size_t need = 6 * kM2LChunk * sizeof(int) + sizeof(T) * (size_t)(1 + g) * 2 * K;
+      if (nI * gs <= kM2LItems * 256 && need <= max_lds_bytes(ctx)) gmax = g;
+    }
+    if (m2l_form == 0 && gmax > 0) {
+      const size_t need = 6 * kM2LChunk * sizeof(int) + sizeof(T) * (size_t)(1 + gmax) * 2 * K;
+      hipLaunchKernelGGL((k_m2l_grp<T>), dim3(P.n_m2l), dim3(256), need, s, P, Tb, gmax, mp, lo);
+    } else if (m2l_form != 2) {
       if (lds) hipLaunchKernelGGL((k_m2l<T, true>), dim3(P.n_m2l), dim3(256), sh, s, P, Tb, mp, lo);
       else hipLaunchKernelGGL((k_m2l<T, false>), dim3(P.n_m2l), dim3(256), sh, s, P, Tb, mp, lo);
     } else {

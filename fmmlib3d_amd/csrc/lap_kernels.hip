@@ -423,6 +423,328 @@ __global__ void k_m2l(DevPlan P, DevTables Tb, const T* __restrict__ mp, T* __re
 }
 
 // ---------------------------------------------------------------------------
+// M2L grouped by translation class.  List-2 sources of one target whose
+// offsets share (dx^2 + dy^2, dz) share the rotation block, the z-shift table
+// and the truncation order; only the azimuthal phase differs.  Up to
+// kM2LGroup of them are translated together: the gs lanes of one work item
+// hold the group's members and read the same table entry, so every table
+// entry is fetched once per group, and each lane produces four consecutive
+// orders m' (rotation) or degrees l (z-shift) of one member, so one LDS read
+// of an input coefficient feeds four FMAs per output part.  Stages run in
+// place (results held in registers across a barrier); the back rotation
+// applies each member's phase, sums the members across lanes in a fixed order
+// and adds the sum to the target's local in LDS.
+// ---------------------------------------------------------------------------
+constexpr int kM2LGroup = 4;   // members per group
+constexpr int kM2LItems = 2;   // items per thread and stage (256 threads)
+constexpr int kM2LChunk = 192; // list-2 entries ordered at a time (list 2 has <= 189)
+__device__ __forceinline__ int grp_w(int n) { return 4 * (n / 4 + 1); }  // padded row width
+
+// out_{m'} = sum_m D[m][m'] in_m and out_{-m'} = sum_m D[m][m'] in_{-m} for
+// m' = mp0..mp0+3 (columns beyond n are zero in the padded table); BACK
+// multiplies by (-1)^{m+m'} (the transpose).  Dh: half block of degree n,
+// rows of width W; four rows are fetched before they are used.
+template <bool BACK, typename T>
+__device__ __forceinline__ void grp_acc4(T (&r)[16], const double4& d, int m, T xr, T xi, T yr, T yi) {
+  if (BACK && (m & 1)) { xr = -xr; xi = -xi; yr = -yr; yi = -yi; }
+  const T dv[4] = {(T)d.x, (T)d.y, (T)d.z, (T)d.w};
+#pragma unroll
+  for (int c = 0; c < 4; c++) {
+    r[4 * c] += dv[c] * xr;
+    r[4 * c + 1] += dv[c] * xi;
+    r[4 * c + 2] += dv[c] * yr;
+    r[4 * c + 3] += dv[c] * yi;
+  }
+}
+
+// one batch of four rows: row r0+u multiplies input degree n, order
+// m = r0+u-n (rows past `last` are clamped and contribute 0)
+template <typename T, bool BACK>
+__device__ __forceinline__ void grp_rot_batch(T (&r)[16], const double4 (&d)[4], int r0, int n, int last,
+                                              const T* in) {
+#pragma unroll
+  for (int u = 0; u < 4; u++) {
+    const bool ok = r0 + u <= last;
+    const int m = min(r0 + u, last) - n;
+    T xr = in[2 * eidx(n, m)], xi = in[2 * eidx(n, m) + 1];
+    T yr = in[2 * eidx(n, -m)], yi = in[2 * eidx(n, -m) + 1];
+    if (!ok) { xr = 0; xi = 0; yr = 0; yi = 0; }
+    grp_acc4<BACK>(r, d[u], m, xr, xi, yr, yi);
+  }
+}
+
+template <typename T, bool BACK>
+__device__ __forceinline__ void grp_rot_item(const double* __restrict__ Dh, int W, int n, int mp0, const T* in,
+                                             T (&r)[16]) {
+  // rows m = -n..n in batches of four, two register buffers: the next batch
+  // is in flight while the current one is used
+  const double* col = Dh + mp0;
+  const int last = 2 * n;
+  double4 d0[4], d1[4];
+#pragma unroll
+  for (int u = 0; u < 4; u++) d0[u] = *reinterpret_cast<const double4*>(col + (size_t)min(u, last) * W);
+  for (int r0 = 0;;) {
+#pragma unroll
+    for (int u = 0; u < 4; u++) d1[u] = *reinterpret_cast<const double4*>(col + (size_t)min(r0 + 4 + u, last) * W);
+    __builtin_amdgcn_sched_barrier(0);
+    grp_rot_batch<T, BACK>(r, d0, r0, n, last, in);
+    r0 += 4;
+    if (r0 > last) break;
+#pragma unroll
+    for (int u = 0; u < 4; u++) d0[u] = *reinterpret_cast<const double4*>(col + (size_t)min(r0 + 4 + u, last) * W);
+    __builtin_amdgcn_sched_barrier(0);
+    grp_rot_batch<T, BACK>(r, d1, r0, n, last, in);
+    r0 += 4;
+    if (r0 > last) break;
+  }
+  if (BACK) {
+#pragma unroll
+    for (int c = 0; c < 4; c++)
+      if ((mp0 + c) & 1) { r[4 * c] = -r[4 * c]; r[4 * c + 1] = -r[4 * c + 1]; r[4 * c + 2] = -r[4 * c + 2]; r[4 * c + 3] = -r[4 * c + 3]; }
+  }
+}
+
+template <typename T>
+__device__ __forceinline__ void grp_rot_store(int n, int mp0, const T (&r)[16], T* out) {
+#pragma unroll
+  for (int c = 0; c < 4; c++) {
+    const int mp = mp0 + c;
+    if (mp > n) break;
+    out[2 * eidx(n, mp)] = r[4 * c];
+    out[2 * eidx(n, mp) + 1] = r[4 * c + 1];
+    if (mp > 0) {
+      out[2 * eidx(n, -mp)] = r[4 * c + 2];
+      out[2 * eidx(n, -mp) + 1] = r[4 * c + 3];
+    }
+  }
+}
+
+// out_l^{+-am} = sum_{n=am}^{pt} Z[am][l][n] in_n^{+-am} for l = l0..l0+3;
+// Zm: rows n of order am in the padded transposed table (width W, column
+// l - am), so one 32-byte load fetches the four degrees of a work item.
+template <typename T>
+__device__ __forceinline__ void grp_z_batch(T (&r)[16], const double4 (&z)[4], int n0, int am, int pt, const T* in) {
+#pragma unroll
+  for (int u = 0; u < 4; u++) {
+    const bool ok = n0 + u <= pt;
+    const int n = min(n0 + u, pt);
+    T xr = in[2 * eidx(n, am)], xi = in[2 * eidx(n, am) + 1];
+    T yr = in[2 * eidx(n, -am)], yi = in[2 * eidx(n, -am) + 1];
+    if (!ok) { xr = 0; xi = 0; yr = 0; yi = 0; }
+    grp_acc4<false>(r, z[u], 0, xr, xi, yr, yi);
+  }
+}
+
+template <typename T>
+__device__ __forceinline__ void grp_z_item(const double* __restrict__ Zm, int W, int am, int l0, int pt, const T* in,
+                                           T (&r)[16]) {
+  // rows n = am..pt in batches of four, fetched one batch ahead
+  const double* col = Zm + (l0 - am);
+  double4 z0[4], z1[4];
+#pragma unroll
+  for (int u = 0; u < 4; u++) z0[u] = *reinterpret_cast<const double4*>(col + (size_t)min(am + u, pt) * W);
+  for (int n0 = am;;) {
+#pragma unroll
+    for (int u = 0; u < 4; u++) z1[u] = *reinterpret_cast<const double4*>(col + (size_t)min(n0 + 4 + u, pt) * W);
+    __builtin_amdgcn_sched_barrier(0);
+    grp_z_batch<T>(r, z0, n0, am, pt, in);
+    n0 += 4;
+    if (n0 > pt) break;
+#pragma unroll
+    for (int u = 0; u < 4; u++) z0[u] = *reinterpret_cast<const double4*>(col + (size_t)min(n0 + 4 + u, pt) * W);
+    __builtin_amdgcn_sched_barrier(0);
+    grp_z_batch<T>(r, z1, n0, am, pt, in);
+    n0 += 4;
+    if (n0 > pt) break;
+  }
+}
+
+template <typename T>
+__device__ __forceinline__ void grp_z_store(int am, int l0, int pt, const T (&r)[16], T* out) {
+#pragma unroll
+  for (int c = 0; c < 4; c++) {
+    const int l = l0 + c;
+    if (l > pt) break;
+    out[2 * eidx(l, am)] = r[4 * c];
+    out[2 * eidx(l, am) + 1] = r[4 * c + 1];
+    if (am > 0) {
+      out[2 * eidx(l, -am)] = r[4 * c + 2];
+      out[2 * eidx(l, -am) + 1] = r[4 * c + 3];
+    }
+  }
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_m2l_grp(DevPlan P, DevTables Tb, int gmax, const T* __restrict__ mp,
+                                                  T* __restrict__ lo) {
+  extern __shared__ unsigned char smem_raw[];
+  int* s_key = reinterpret_cast<int*>(smem_raw);
+  int* s_box = s_key + kM2LChunk;
+  int* s_code = s_box + kM2LChunk;
+  int* t_key = s_code + kM2LChunk;
+  int* t_box = t_key + kM2LChunk;
+  int* t_code = t_box + kM2LChunk;
+  T* acc = reinterpret_cast<T*>(t_code + kM2LChunk);
+  T* A = acc + 2 * P.K;
+  const int tid = threadIdx.x, bs = blockDim.x;
+  const int b = P.m2l_boxes[blockIdx.x];
+  const int p = P.p, P1 = p + 1, K2 = 2 * P.K;
+  const T ih = (T)P.inv_side[P.level[b]];
+  for (int k = tid; k < K2; k += bs) acc[k] = T(0);
+  const int off0 = P.l2_off[b], n2 = P.l2_off[b + 1] - off0;
+  const int bi = P.ijk[3 * b], bj = P.ijk[3 * b + 1], bk = P.ijk[3 * b + 2];
+  for (int c0 = 0; c0 < n2; c0 += kM2LChunk) {
+    const int cn = min(kM2LChunk, n2 - c0);
+    __syncthreads();
+    if (tid < cn) {
+      const int j = P.l2_idx[off0 + c0 + tid];
+      const int di = P.ijk[3 * j] - bi, dj = P.ijk[3 * j + 1] - bj, dk = P.ijk[3 * j + 2] - bk;
+      s_box[tid] = j;
+      s_code[tid] = ((di + 3) * 7 + (dj + 3)) * 7 + (dk + 3);
+      s_key[tid] = (di * di + dj * dj) * 7 + (dk + 3);
+    }
+    __syncthreads();
+    if (tid < cn) {  // stable rank by key
+      const int key = s_key[tid];
+      int rank = 0;
+      for (int t = 0; t < cn; t++) {
+        const int kt = s_key[t];
+        rank += (kt < key) || (kt == key && t < tid);
+      }
+      t_key[rank] = key;
+      t_box[rank] = s_box[tid];
+      t_code[rank] = s_code[tid];
+    }
+    __syncthreads();
+    for (int s0 = 0; s0 < cn;) {
+      const int key0 = t_key[s0];
+      int g = 1;
+      while (g < gmax && s0 + g < cn && t_key[s0 + g] == key0) g++;
+      const int gs = g > 2 ? 4 : g;  // lanes per item
+      const int lg = gs == 4 ? 2 : gs - 1;
+      const int code0 = t_code[s0];
+      const int pt = Tb.off_ptrunc[code0];
+      const int rc = Tb.off_rot[code0];
+      const double* Z = Tb.z_m2l_t + (size_t)Tb.off_dist[code0] * Tb.zt_size;
+      const int Kp = (pt + 1) * (pt + 1);
+      // (a) members times e^{i m phi}
+      for (int idx = tid; idx < g * Kp; idx += bs) {
+        const int jj = idx / Kp, k = idx - jj * Kp;
+        int n = (int)sqrtf((float)k);
+        while (n * n > k) n--;
+        while ((n + 1) * (n + 1) <= k) n++;
+        const int m = k - n * n - n, am = m < 0 ? -m : m;
+        const double* ph = Tb.off_phase + (size_t)t_code[s0 + jj] * 2 * P1;
+        const T er = (T)ph[2 * am], ei = m < 0 ? -(T)ph[2 * am + 1] : (T)ph[2 * am + 1];
+        const T* src = mp + (size_t)t_box[s0 + jj] * K2;
+        const T xr = src[2 * k], xi = src[2 * k + 1];
+        A[jj * K2 + 2 * k] = xr * er - xi * ei;
+        A[jj * K2 + 2 * k + 1] = xr * ei + xi * er;
+      }
+      __syncthreads();
+      const int nit = Tb.grp_rcount[pt] << lg;
+      T r[kM2LItems][16];
+      // (b) rotate to the z axis
+#pragma unroll
+      for (int q = 0; q < kM2LItems; q++) {
+#pragma unroll
+        for (int e = 0; e < 16; e++) r[q][e] = T(0);
+        const int i = tid + q * bs, jj = i & (gs - 1);
+        if (i < nit && jj < g) {
+          const int it = Tb.grp_ritems[i >> lg], n = it >> 8;
+          grp_rot_item<T, false>(Tb.rot_half + Tb.rot_half_off[rc * P1 + n], grp_w(n), n, 4 * (it & 255),
+                                 A + jj * K2, r[q]);
+        }
+      }
+      __syncthreads();
+#pragma unroll
+      for (int q = 0; q < kM2LItems; q++) {
+        const int i = tid + q * bs, jj = i & (gs - 1);
+        if (i < nit && jj < g) {
+          const int it = Tb.grp_ritems[i >> lg];
+          grp_rot_store<T>(it >> 8, 4 * (it & 255), r[q], A + jj * K2);
+        }
+      }
+      __syncthreads();
+      // (c) shift along z
+      const int zo = Tb.grp_zoff[pt], nzt = (Tb.grp_zoff[pt + 1] - zo) << lg;
+#pragma unroll
+      for (int q = 0; q < kM2LItems; q++) {
+#pragma unroll
+        for (int e = 0; e < 16; e++) r[q][e] = T(0);
+        const int i = tid + q * bs, jj = i & (gs - 1);
+        if (i < nzt && jj < g) {
+          const int it = Tb.grp_zitems[zo + (i >> lg)], am = it >> 8;
+          grp_z_item<T>(Z + Tb.zt_off[am], grp_w(p - am), am, am + 4 * (it & 255), pt, A + jj * K2, r[q]);
+        }
+      }
+      __syncthreads();
+#pragma unroll
+      for (int q = 0; q < kM2LItems; q++) {
+        const int i = tid + q * bs, jj = i & (gs - 1);
+        if (i < nzt && jj < g) {
+          const int it = Tb.grp_zitems[zo + (i >> lg)], am = it >> 8;
+          grp_z_store<T>(am, am + 4 * (it & 255), pt, r[q], A + jj * K2);
+        }
+      }
+      __syncthreads();
+      // (d) rotate back, apply e^{-i m phi} and the scale, sum the members
+#pragma unroll
+      for (int q = 0; q < kM2LItems; q++) {
+#pragma unroll
+        for (int e = 0; e < 16; e++) r[q][e] = T(0);
+        const int i = tid + q * bs, jj = i & (gs - 1);
+        int n = 0, mp0 = 0;
+        if (i < nit) {
+          const int it = Tb.grp_ritems[i >> lg];
+          n = it >> 8;
+          mp0 = 4 * (it & 255);
+          if (jj < g) {
+            grp_rot_item<T, true>(Tb.rot_half + Tb.rot_half_off[rc * P1 + n], grp_w(n), n, mp0, A + jj * K2, r[q]);
+            const double* ph = Tb.off_phase + (size_t)t_code[s0 + jj] * 2 * P1;
+#pragma unroll
+            for (int c = 0; c < 4; c++) {
+              const int mq = mp0 + c <= n ? mp0 + c : n;
+              const T er = (T)ph[2 * mq] * ih, ei = (T)ph[2 * mq + 1] * ih;
+              const T pr = r[q][4 * c], pi = r[q][4 * c + 1], qr = r[q][4 * c + 2], qi = r[q][4 * c + 3];
+              r[q][4 * c] = pr * er + pi * ei;       // (pr + i pi) e^{-i m' phi}
+              r[q][4 * c + 1] = pi * er - pr * ei;
+              r[q][4 * c + 2] = qr * er - qi * ei;   // (qr + i qi) e^{+i m' phi}
+              r[q][4 * c + 3] = qi * er + qr * ei;
+            }
+          }
+        }
+#pragma unroll
+        for (int e = 0; e < 16; e++) {
+          T v = r[q][e];
+          if (gs > 1) v += __shfl_xor(v, 1);
+          if (gs > 2) v += __shfl_xor(v, 2);
+          r[q][e] = v;
+        }
+        if (i < nit && jj == 0) {
+#pragma unroll
+          for (int c = 0; c < 4; c++) {
+            const int mq = mp0 + c;
+            if (mq > n) break;
+            acc[2 * eidx(n, mq)] += r[q][4 * c];
+            acc[2 * eidx(n, mq) + 1] += r[q][4 * c + 1];
+            if (mq > 0) {
+              acc[2 * eidx(n, -mq)] += r[q][4 * c + 2];
+              acc[2 * eidx(n, -mq) + 1] += r[q][4 * c + 3];
+            }
+          }
+        }
+      }
+      __syncthreads();
+      s0 += g;
+    }
+  }
+  __syncthreads();
+  T* out = lo + (size_t)b * K2;
+  for (int k = tid; k < K2; k += bs) out[k] += acc[k];
+}
+
+// ---------------------------------------------------------------------------
 // M2L with factored rotations.  A rotation by theta about y is
 //   d^n(theta) = P Delta^n E(theta) Delta^nT P*,  Delta^n = d^n(pi/2),
 //   P = diag(i^m), E = diag(e^{i m theta}),

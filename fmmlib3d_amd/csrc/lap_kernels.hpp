@@ -53,6 +53,15 @@ struct DevTables {
   const double *delta;      // d^n(pi/2) per degree, row-major (2n+1)^2
   const long *delta_off;    // start of degree n
   const double *rot_e;      // per rotation class: e^{i k theta}, k = 0..p
+  // grouped M2L work items: rotation (n << 8 | q) by degree, count with n <= pt;
+  // z-shift (|m| << 8 | q) per truncation order pt, start of pt's items
+  const int *grp_ritems, *grp_rcount, *grp_zitems, *grp_zoff;
+  // padded tables of the grouped M2L: half rotation blocks [m][m' >= 0] per
+  // (class, degree) at rot_half_off[c*(p+1)+n]; z-shift [m][n][l-m] per
+  // distance class (zt_size doubles each), order m at zt_off[m]
+  const double *rot_half, *z_m2l_t;
+  const long *rot_half_off, *zt_off;
+  long zt_size;
 };
 
 }  // namespace fmm3d

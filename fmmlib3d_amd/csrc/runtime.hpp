@@ -60,6 +60,9 @@ struct Context {
   std::vector<long> tab_host_l;
   size_t tab_off[8] = {0, 0, 0, 0, 0, 0, 0, 0};
   size_t tab_off_l = 0, tab_off_re = 0;
+  size_t tab_off_i[4] = {0, 0, 0, 0};  // grouped M2L items: ritems, rcount, zitems, zoff
+  size_t tab_off_g[4] = {0, 0, 0, 0};  // grouped M2L tables: rot_half, z_m2l_t (doubles); their offsets (longs)
+  long tab_zt_size = 0;
   // cached Helmholtz tables (owned by the Helmholtz driver)
   std::shared_ptr<void> helm_tables;
 };

@@ -130,6 +130,20 @@ void build_tables(int p, double eps, HostTables& H) {
           z[((size_t)m * P1 + l) * P1 + n] = c;
         }
   }
+  // the same coefficients for the grouped M2L, [m][in n][out l - m], rows
+  // padded to a multiple of 4 (one 32-byte load per four output degrees)
+  H.zt_off.assign(P1 + 1, 0);
+  for (int m = 0; m <= p; m++) H.zt_off[m + 1] = H.zt_off[m] + (long)P1 * grp_width(p - m);
+  H.zt_size = H.zt_off[P1];
+  H.z_m2l_t.assign((size_t)H.zt_size * H.ndist, 0.0);
+  for (int dc = 0; dc < H.ndist; dc++)
+    for (int m = 0; m <= p; m++) {
+      const int W = grp_width(p - m);
+      for (int n = m; n <= p; n++)
+        for (int l = m; l <= p; l++)
+          H.z_m2l_t[(size_t)H.zt_size * dc + H.zt_off[m] + (size_t)n * W + (l - m)] =
+              H.z_m2l[Z * dc + ((size_t)m * P1 + l) * P1 + n];
+    }
   H.z_m2m.assign(Z, 0.0);
   H.z_l2l.assign(Z, 0.0);
   const double r = std::sqrt(3.0) / 4.0;
@@ -280,6 +294,42 @@ void build_geometry_tables(int p, HostTables& H) {
   }
   H.ndist = (int)dists.size();
   H.dist_value = dists;
+  // work items of the grouped M2L: a rotation item is (degree n, columns
+  // 4q..4q+3), ordered by n so that the items of degrees <= pt are a prefix;
+  // a z-shift item is (order |m|, degrees l = |m|+4q..+3), listed per pt
+  H.grp_ritems.clear();
+  H.grp_rcount.assign(P1, 0);
+  for (int n = 0; n <= p; n++) {
+    for (int q = 0; 4 * q <= n; q++) H.grp_ritems.push_back((n << 8) | q);
+    H.grp_rcount[n] = (int)H.grp_ritems.size();
+  }
+  H.grp_zitems.clear();
+  H.grp_zoff.assign(P1 + 1, 0);
+  for (int pt = 0; pt <= p; pt++) {
+    H.grp_zoff[pt] = (int)H.grp_zitems.size();
+    for (int am = 0; am <= pt; am++)
+      for (int q = 0; am + 4 * q <= pt; q++) H.grp_zitems.push_back((am << 8) | q);
+  }
+  H.grp_zoff[P1] = (int)H.grp_zitems.size();
+  // half rotation blocks for the grouped M2L: columns m' >= 0 only (the
+  // others follow from D[-m][-m'] = D[m][m']), rows padded to a multiple of 4
+  // so that one 32-byte load fetches the four columns of a work item
+  H.rot_half_off.assign((size_t)nrot * P1, 0);
+  long th = 0;
+  for (int c = 0; c < nrot; c++)
+    for (int n = 0; n <= p; n++) {
+      H.rot_half_off[(size_t)c * P1 + n] = th;
+      th += (long)(2 * n + 1) * grp_width(n);
+    }
+  H.rot_half.assign(th, 0.0);
+  for (int c = 0; c < nrot; c++)
+    for (int n = 0; n <= p; n++) {
+      const int dim = 2 * n + 1, W = grp_width(n);
+      const double* D = &H.rot[H.rot_off[(size_t)c * P1 + n]];
+      double* Dh = &H.rot_half[H.rot_half_off[(size_t)c * P1 + n]];
+      for (int m = -n; m <= n; m++)
+        for (int mp = 0; mp <= n; mp++) Dh[(size_t)(m + n) * W + mp] = D[(m + n) * dim + (mp + n)];
+    }
 }
 
 }  // namespace fmm3d

@@ -4,6 +4,9 @@
 
 namespace fmm3d {
 
+// row width of the padded tables of the grouped M2L: n+1 rounded up to 4
+inline int grp_width(int n) { return 4 * (n / 4 + 1); }
+
 struct HostTables {
   int p = -1, nrot = 0, ndist = 0;
   std::vector<double> fac;                         // FacLayout(p)
@@ -20,6 +23,14 @@ struct HostTables {
   std::vector<double> delta;                       // per degree (2n+1)^2, row a col b = d^n_ab(pi/2)
   std::vector<long> delta_off;                     // p+1
   std::vector<double> rot_e;                       // nrot x (p+1) complex: e^{i k theta_class}
+  // grouped M2L work items (see build_geometry_tables)
+  std::vector<int> grp_ritems, grp_rcount;         // (n << 8 | q); count of items with n <= pt
+  std::vector<int> grp_zitems, grp_zoff;           // (|m| << 8 | q) per pt; start of pt's items
+  std::vector<double> rot_half;                    // per class and degree: [m][m' >= 0], width grp_width(n)
+  std::vector<long> rot_half_off;                  // nrot x (p+1)
+  std::vector<double> z_m2l_t;                     // per distance class: [m][n][l - m], width grp_width(p - m)
+  std::vector<long> zt_off;                        // start of order m within a class, p+2 entries
+  long zt_size = 0;                                // doubles per distance class
 };
 
 // l3dterms (src/l3dterms.f:34): expansion order for tolerance eps
