@@ -192,27 +192,35 @@ int lap_solve(Context& ctx, const LapCall& c) {
   }
   tm.mark(1);
   const double* fac = Tb.fac;
-  // ---- STEP 1: P2M
-  {
-    const int H = (p + 1) * (p + 2) / 2;
-    int ts = (int)((65536 / sizeof(T) - 2 * K) / (2 * H + 11));
-    ts = std::max(1, std::min(32, ts));
-    size_t lds = sizeof(T) * (2 * K + (size_t)ts * (2 * H + 11));
-    if (P.n_p2m)
-      hipLaunchKernelGGL((k_form<T, false>), dim3(P.n_p2m), dim3(256), lds, s, P, fac, spos, str64,
-                         f.ifcharge, f.ifdipole, ts, mp);
-  }
+  // ---- STEP 1: P2M, STEP 2: P2L (list 4)
+  // k_form2 (per-thread sums) when its half table fits two entries per
+  // thread, else k_form (per-coefficient contraction)
+  auto form = [&](auto irreg_tag, int nbox_form, T* dst) {
+    constexpr bool IR = decltype(irreg_tag)::value;
+    if (nbox_form == 0 || (!f.ifcharge && !f.ifdipole)) return;
+    const int pt = IR ? p + 1 : p;
+    const int H = (pt + 1) * (pt + 2) / 2;
+    if (H <= kFormItems * 256) {
+      const size_t P2 = (size_t)(pt + 1) * (pt + 1);
+      int ts = (int)(65536 / (2 * sizeof(T) * (size_t)H));
+      ts = std::max(1, std::min(32, ts));
+      const size_t lds = 2 * sizeof(T) * std::max((size_t)ts * H, 4 * P2);
+#define FORM2(C, D)                                                                                    \
+  if (f.ifcharge == C && f.ifdipole == D)                                                              \
+    hipLaunchKernelGGL((k_form2<T, IR, C, D>), dim3(nbox_form), dim3(256), lds, s, P, fac, spos, str64, ts, dst);
+      FORM2(1, 0) FORM2(0, 1) FORM2(1, 1)
+#undef FORM2
+    } else {
+      int ts = (int)((65536 / sizeof(T) - 2 * K) / (2 * H + 11));
+      ts = std::max(1, std::min(32, ts));
+      const size_t lds = sizeof(T) * (2 * K + (size_t)ts * (2 * H + 11));
+      hipLaunchKernelGGL((k_form<T, IR>), dim3(nbox_form), dim3(256), lds, s, P, fac, spos, str64,
+                         f.ifcharge, f.ifdipole, ts, dst);
+    }
+  };
+  form(std::false_type{}, P.n_p2m, mp);
   tm.mark(2);
-  // ---- STEP 2: P2L (list 4)
-  {
-    const int H = (p + 2) * (p + 3) / 2;
-    int ts = (int)((65536 / sizeof(T) - 2 * K) / (2 * H + 11));
-    ts = std::max(1, std::min(32, ts));
-    size_t lds = sizeof(T) * (2 * K + (size_t)ts * (2 * H + 11));
-    if (P.n_p2l)
-      hipLaunchKernelGGL((k_form<T, true>), dim3(P.n_p2l), dim3(256), lds, s, P, fac, spos, str64,
-                         f.ifcharge, f.ifdipole, ts, lo);
-  }
+  form(std::true_type{}, P.n_p2l, lo);
   tm.mark(3);
   // ---- STEP 3: M2M, bottom-up
   const size_t lds_ps = sizeof(T) * 4 * K;
@@ -258,11 +266,18 @@ int lap_solve(Context& ctx, const LapCall& c) {
   for (auto& g : O.l2l_groups)
     hipLaunchKernelGGL((k_l2l<T>), dim3(g.second), dim3(256), lds_ps, s, P, Tb, g.first, lo);
   tm.mark(6);
-  // ---- STEP 7: L2P
+  // ---- STEP 7: L2P (packed local + gradient expansions, wave-uniform reads)
   const size_t lds_e = sizeof(T) * 2 * K;
-  if (P.n_l2p)
-    hipLaunchKernelGGL((k_eval<T, false>), dim3(P.n_l2p), dim3(256), lds_e, s, P, fac, lo, spos, tpos,
-                       f.ifpot, f.iffld, f.ifpottarg, f.iffldtarg, pot_s, fld_s, pot_t, fld_t);
+  if (P.n_l2p) {
+    const size_t H = (size_t)(p + 1) * (p + 2) / 2;
+    T* pack = (T*)ctx.arena.get("l2p_pack", sizeof(T) * 16 * H * (size_t)P.n_l2p);
+    const size_t tot = H * (size_t)P.n_l2p;
+    hipLaunchKernelGGL((k_l2p_pack<T>), dim3((unsigned)std::min<size_t>(blocks(tot, 256), 65536)), dim3(256), 0, s,
+                       P, fac, lo, pack);
+    if (P.n_l2pi)
+      hipLaunchKernelGGL((k_l2p2<T>), dim3(P.n_l2pi), dim3(256), 0, s, P, fac, pack, spos, tpos, f.ifpot, f.iffld,
+                         f.ifpottarg, f.iffldtarg, pot_s, fld_s, pot_t, fld_t);
+  }
   tm.mark(7);
   // ---- STEP 6: M2P (list 3)
   if (P.n_m2p)

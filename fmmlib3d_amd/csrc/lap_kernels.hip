@@ -124,6 +124,154 @@ __global__ void k_form(DevPlan P, const double* __restrict__ fac, const double* 
 }
 
 // ---------------------------------------------------------------------------
+// P2M / P2L with per-thread sums.  Each thread owns up to two half-table
+// entries (n', a >= 0) and accumulates, in registers over every source of the
+// box, S^w(n', +-a) = sum_s w_s H_{n'}^{+-a}(u_s) for the weights
+// w = q, v_x, v_y, v_z (H^{-a} = conj(H^a)); the weights are wave-uniform
+// loads.  The harmonic table of a tile of `ts` sources is built in LDS, one
+// thread per (source, column pair a, pt - a).  At the end the sums go to LDS
+// and one thread per coefficient combines them: charge S^q(n, -m), dipole
+// through the derivative factors and degree n-1 (P2M) or n+1 (P2L), with
+// (v_x -+ i v_y)/2 formed from S^{v_x}, S^{v_y} by linearity.
+// ---------------------------------------------------------------------------
+constexpr int kFormItems = 2;  // half-table entries per thread (256 threads)
+
+template <typename T, bool IRREG, bool CHG, bool DIP>
+__global__ __launch_bounds__(256) void k_form2(DevPlan P, const double* __restrict__ fac,
+                                               const double* __restrict__ spos, const double* __restrict__ str,
+                                               int ts, T* __restrict__ expn) {
+  extern __shared__ unsigned char smem_raw[];
+  T* tab = reinterpret_cast<T*>(smem_raw);
+  const int p = P.p, K = P.K;
+  const int pt = IRREG ? p + 1 : p;
+  const int H = (pt + 1) * (pt + 2) / 2;
+  const int ncol = pt / 2 + 1;  // column tasks per source: a and pt - a
+  const int P2 = (pt + 1) * (pt + 1);
+  FacLayout L(p);
+  const int tid = threadIdx.x;
+  const int b = IRREG ? P.p2l_boxes[blockIdx.x] : P.p2m_boxes[blockIdx.x];
+  const double ihd = P.inv_side[P.level[b]];
+  const T ih = (T)ihd;
+  const double cx = P.center[3 * b], cy = P.center[3 * b + 1], cz = P.center[3 * b + 2];
+  T acc[kFormItems][16];
+#pragma unroll
+  for (int q = 0; q < kFormItems; q++)
+#pragma unroll
+    for (int e = 0; e < 16; e++) acc[q][e] = T(0);
+  int jbeg = 0, jend = 1;
+  if (IRREG) { jbeg = P.l4_off[b]; jend = P.l4_off[b + 1]; }
+  for (int jj = jbeg; jj < jend; jj++) {
+    const int j = IRREG ? P.l4_idx[jj] : b;
+    const int s0 = P.isrc[j], sn = P.nsrc[j];
+    for (int t0 = 0; t0 < sn; t0 += ts) {
+      const int tn = min(ts, sn - t0);
+      __syncthreads();
+      for (int task = tid; task < tn * ncol; task += blockDim.x) {
+        const int s = task / ncol, a = task - s * ncol;
+        const double* x = spos + 4 * (size_t)(s0 + t0 + s);
+        const T ux = (T)((x[0] - cx) * ihd), uy = (T)((x[1] - cy) * ihd), uz = (T)((x[2] - cz) * ihd);
+        const T r2 = ux * ux + uy * uy + uz * uz;
+        T* out = tab + 2 * (size_t)s * H;
+        if (IRREG) {
+          const T ir2 = T(1) / r2, ir = sqrt(ir2);
+          irregular_column<T>(fac, L, pt, a, ux, uy, uz, ir2, ir, out, 1);
+          if (pt - a != a) irregular_column<T>(fac, L, pt, pt - a, ux, uy, uz, ir2, ir, out, 1);
+        } else {
+          regular_column<T>(fac, L, pt, a, ux, uy, uz, r2, out, 1);
+          if (pt - a != a) regular_column<T>(fac, L, pt, pt - a, ux, uy, uz, r2, out, 1);
+        }
+      }
+      __syncthreads();
+      for (int s = 0; s < tn; s++) {
+        const double* w = str + 8 * (size_t)(s0 + t0 + s);
+        T wv[8];
+#pragma unroll
+        for (int e = 0; e < 8; e++) wv[e] = (T)w[e];
+#pragma unroll
+        for (int q = 0; q < kFormItems; q++) {
+          const int h = tid + q * (int)blockDim.x;
+          if (h < H) {
+            const T hr = tab[2 * ((size_t)s * H + h)], hi = tab[2 * ((size_t)s * H + h) + 1];
+#pragma unroll
+            for (int c = 0; c < 4; c++) {
+              if ((c == 0 && !CHG) || (c > 0 && !DIP)) continue;
+              const T wr = wv[2 * c], wi = wv[2 * c + 1];
+              T* A = &acc[q][4 * c];
+              A[0] = fma(wr, hr, fma(-wi, hi, A[0]));  // w H^{+a}
+              A[1] = fma(wr, hi, fma(wi, hr, A[1]));
+              A[2] = fma(wr, hr, fma(wi, hi, A[2]));   // w H^{-a} = w conj(H^{a})
+              A[3] = fma(wi, hr, fma(-wr, hi, A[3]));
+            }
+          }
+        }
+      }
+    }
+  }
+  __syncthreads();
+  // sums to LDS: S[c][eidx(n', m')] complex, c = q, v_x, v_y, v_z
+#pragma unroll
+  for (int q = 0; q < kFormItems; q++) {
+    const int h = tid + q * (int)blockDim.x;
+    if (h < H) {
+      int n = (int)((sqrtf(8.0f * h + 1.0f) - 1.0f) * 0.5f);
+      while (n * (n + 1) / 2 > h) n--;
+      while ((n + 1) * (n + 2) / 2 <= h) n++;
+      const int a = h - n * (n + 1) / 2;
+#pragma unroll
+      for (int c = 0; c < 4; c++) {
+        T* S = tab + 2 * (size_t)c * P2;
+        S[2 * eidx(n, a)] = acc[q][4 * c];
+        S[2 * eidx(n, a) + 1] = acc[q][4 * c + 1];
+        S[2 * eidx(n, -a)] = acc[q][4 * c + 2];
+        S[2 * eidx(n, -a) + 1] = acc[q][4 * c + 3];
+      }
+    }
+  }
+  __syncthreads();
+  const T sc = IRREG ? ih : T(1);
+  T* out = expn + 2 * (size_t)b * K;
+  for (int k = tid; k < K; k += blockDim.x) {
+    int n = (int)sqrtf((float)k);
+    while (n * n > k) n--;
+    while ((n + 1) * (n + 1) <= k) n++;
+    const int m = k - n * n - n, km = -m;
+    T ar = 0, ai = 0;
+    if (CHG) {
+      ar = tab[2 * eidx(n, km)];
+      ai = tab[2 * eidx(n, km) + 1];
+    }
+    if (DIP && (IRREG || n >= 1)) {
+      const int nn = IRREG ? n + 1 : n - 1;
+      const T fa = (T)fac[(IRREG ? L.AS : L.AR) + eidx(n, km)];
+      const T fb = (T)fac[(IRREG ? L.BS : L.BR) + eidx(n, km)] * T(0.5);
+      const T fc = (T)fac[(IRREG ? L.CS : L.CR) + eidx(n, km)] * T(0.5);
+      const T* SX = tab + 2 * (size_t)P2;
+      const T* SY = SX + 2 * (size_t)P2;
+      const T* SZ = SY + 2 * (size_t)P2;
+      T gr = 0, gi = 0;
+      if (km <= nn && km >= -nn) {
+        gr += fa * SZ[2 * eidx(nn, km)];
+        gi += fa * SZ[2 * eidx(nn, km) + 1];
+      }
+      if (km + 1 <= nn && km + 1 >= -nn) {  // (v_x - i v_y)/2
+        const int e = eidx(nn, km + 1);
+        gr += fb * (SX[2 * e] + SY[2 * e + 1]);
+        gi += fb * (SX[2 * e + 1] - SY[2 * e]);
+      }
+      if (km - 1 <= nn && km - 1 >= -nn) {  // (v_x + i v_y)/2
+        const int e = eidx(nn, km - 1);
+        gr += fc * (SX[2 * e] - SY[2 * e + 1]);
+        gi += fc * (SX[2 * e + 1] + SY[2 * e]);
+      }
+      ar += gr * ih;
+      ai += gi * ih;
+    }
+    out[2 * k] += ar * sc;
+    out[2 * k + 1] += ai * sc;
+  }
+}
+
+// ---------------------------------------------------------------------------
 // L2P (src/laprouts3d.f:2772) and M2P (src/laprouts3d.f:2432)
 // One workgroup per leaf box; one thread per evaluation point.  All threads
 // walk the coefficients in the same order (LDS broadcast reads) and build
@@ -268,6 +416,142 @@ __global__ void k_eval(DevPlan P, const double* __restrict__ fac, const T* __res
         }
       }
     }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// L2P from a packed expansion.  For every box of the L2P list the local
+// expansion L and the three derived expansions of the gradient,
+//   Gz(n,m) = AR(n+1,m)   L(n+1,m),
+//   Gu(n,m) = BR(n+1,m-1) L(n+1,m-1),
+//   Gd(n,m) = CR(n+1,m+1) L(n+1,m+1)       (n <= p-1; zero at n = p),
+// are stored per half-table entry (n, a >= 0) in the order the evaluation
+// walks them (column a, then n = a..p): 8 complex values
+//   L(n,a) L(n,-a) Gz(n,a) Gz(n,-a) Gu(n,a) Gu(n,-a) Gd(n,a) Gd(n,-a)
+// (the -a slots are zero for a = 0).  The evaluation then reads them with
+// wave-uniform loads and pays four complex products per harmonic:
+//   pot = sum L R, d/dz = sum Gz R, (d/dx + i d/dy) = sum Gu R, (d/dx - i d/dy) = sum Gd R.
+// ---------------------------------------------------------------------------
+__host__ __device__ inline int walk_index(int p, int a, int n) { return a * (p + 1) - a * (a - 1) / 2 + (n - a); }
+
+template <typename T>
+__global__ void k_l2p_pack(DevPlan P, const double* __restrict__ fac, const T* __restrict__ lo,
+                           T* __restrict__ pack) {
+  const int p = P.p, K = P.K;
+  const int H = (p + 1) * (p + 2) / 2;
+  FacLayout L(p);
+  const size_t total = (size_t)P.n_l2p * H;
+  for (size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x; t < total; t += (size_t)gridDim.x * blockDim.x) {
+    const int i = (int)(t / H), w = (int)(t - (size_t)i * H);
+    int a = 0;
+    while (a < p && walk_index(p, a + 1, a + 1) <= w) a++;
+    const int n = a + (w - walk_index(p, a, a));
+    const T* E = lo + 2 * (size_t)P.l2p_boxes[i] * K;
+    T v[16];
+#pragma unroll
+    for (int e = 0; e < 16; e++) v[e] = T(0);
+    v[0] = E[2 * eidx(n, a)];
+    v[1] = E[2 * eidx(n, a) + 1];
+    if (a > 0) { v[2] = E[2 * eidx(n, -a)]; v[3] = E[2 * eidx(n, -a) + 1]; }
+    if (n + 1 <= p) {
+      const int n1 = n + 1;
+      for (int sgn = 0; sgn < (a > 0 ? 2 : 1); sgn++) {
+        const int m = sgn ? -a : a;
+        const T fz = (T)fac[L.AR + eidx(n1, m)], fu = (T)fac[L.BR + eidx(n1, m - 1)], fd = (T)fac[L.CR + eidx(n1, m + 1)];
+        v[4 + 2 * sgn] = fz * E[2 * eidx(n1, m)];
+        v[5 + 2 * sgn] = fz * E[2 * eidx(n1, m) + 1];
+        v[8 + 2 * sgn] = fu * E[2 * eidx(n1, m - 1)];
+        v[9 + 2 * sgn] = fu * E[2 * eidx(n1, m - 1) + 1];
+        v[12 + 2 * sgn] = fd * E[2 * eidx(n1, m + 1)];
+        v[13 + 2 * sgn] = fd * E[2 * eidx(n1, m + 1) + 1];
+      }
+    }
+    T* o = pack + 16 * t;
+#pragma unroll
+    for (int e = 0; e < 16; e++) o[e] = v[e];
+  }
+}
+
+// out += c+ h + c- conj(h)
+template <typename T>
+__device__ __forceinline__ void pack_fma(C2<T>& out, const T* c, const C2<T>& h) {
+  out.re = fma(c[0], h.re, fma(-c[1], h.im, fma(c[2], h.re, fma(c[3], h.im, out.re))));
+  out.im = fma(c[0], h.im, fma(c[1], h.re, fma(-c[2], h.im, fma(c[3], h.re, out.im))));
+}
+
+// One workgroup per L2P work item (box, kind, first point); the loops over
+// the packed entries are wave uniform, so the entries are scalar loads.
+template <typename T>
+__global__ __launch_bounds__(256) void k_l2p2(DevPlan P, const double* __restrict__ fac, const T* __restrict__ pack,
+                                               const double* __restrict__ spos, const double* __restrict__ tpos,
+                                               int wpot0, int wfld0, int wpot1, int wfld1,
+                                               double* __restrict__ pot0, double* __restrict__ fld0,
+                                               double* __restrict__ pot1, double* __restrict__ fld1) {
+  const int p = P.p;
+  const int H = (p + 1) * (p + 2) / 2;
+  FacLayout L(p);
+  const int w = blockIdx.x;
+  const int ib = P.l2p_items[3 * w], kind = P.l2p_items[3 * w + 1], start = P.l2p_items[3 * w + 2];
+  const int b = P.l2p_boxes[ib];
+  const T* pk = pack + (size_t)ib * H * 16;
+  const T ih = (T)P.inv_side[P.level[b]];
+  const double cx = P.center[3 * b], cy = P.center[3 * b + 1], cz = P.center[3 * b + 2];
+  const int wp = kind ? wpot1 : wpot0, wf = kind ? wfld1 : wfld0;
+  const int n0 = kind ? P.itrg[b] : P.isrc[b];
+  const int nn = kind ? P.ntrg[b] : P.nsrc[b];
+  const int i = start + threadIdx.x;
+  const bool active = i < nn;
+  double x = cx, y = cy, z = cz;
+  if (active) {
+    const double* q = (kind ? tpos : spos) + 4 * (size_t)(n0 + i);
+    x = q[0]; y = q[1]; z = q[2];
+  }
+  const T ux = (T)((x - cx) * (double)ih), uy = (T)((y - cy) * (double)ih), uz = (T)((z - cz) * (double)ih);
+  const T r2 = ux * ux + uy * uy + uz * uz;
+  C2<T> pv = {0, 0}, gz = {0, 0}, gu = {0, 0}, gd = {0, 0};
+  C2<T> hmm = {T(1), T(0)};
+  const T* c = pk;
+  for (int a = 0; a <= p; a++) {
+    if (a > 0) {
+      const T f = (T)fac[L.MM + a];
+      const C2<T> v = {-ux * f, -uy * f};
+      hmm = cmul(v, hmm);
+    }
+    C2<T> h0 = hmm, h1 = hmm;
+    for (int n = a; n <= p; n++, c += 16) {
+      C2<T> h;
+      if (n == a) {
+        h = hmm;
+      } else if (n == a + 1) {
+        const T g = (T)fac[L.M1 + a] * uz;
+        h = {g * hmm.re, g * hmm.im};
+        h1 = h;
+      } else {
+        const T aa = (T)(2 * n - 1) * uz, bq = (T)fac[L.RA + hidx(n, a)] * r2, cc = (T)fac[L.RB + hidx(n, a)];
+        h = {(aa * h1.re - bq * h0.re) * cc, (aa * h1.im - bq * h0.im) * cc};
+        h0 = h1;
+        h1 = h;
+      }
+      pack_fma(pv, c, h);
+      if (wf) {
+        pack_fma(gz, c + 4, h);
+        pack_fma(gu, c + 8, h);
+        pack_fma(gd, c + 12, h);
+      }
+    }
+  }
+  if (!active) return;
+  const size_t id = (size_t)(n0 + i);
+  double* po = kind ? pot1 : pot0;
+  double* fo = kind ? fld1 : fld0;
+  if (wp) { po[2 * id] += pv.re; po[2 * id + 1] += pv.im; }
+  if (wf) {
+    // fld = -grad: d/dx = (U+D)/2, d/dy = (U-D)/(2i), d/dz; the gradient in x carries 1/h
+    const C2<T> dx = {T(0.5) * (gu.re + gd.re), T(0.5) * (gu.im + gd.im)};
+    const C2<T> dy = {T(0.5) * (gu.im - gd.im), -T(0.5) * (gu.re - gd.re)};
+    fo[6 * id + 0] -= dx.re * ih; fo[6 * id + 1] -= dx.im * ih;
+    fo[6 * id + 2] -= dy.re * ih; fo[6 * id + 3] -= dy.im * ih;
+    fo[6 * id + 4] -= gz.re * ih; fo[6 * id + 5] -= gz.im * ih;
   }
 }
 
@@ -642,64 +926,66 @@ __global__ __launch_bounds__(256) void k_m2l_grp(DevPlan P, DevTables Tb, int gm
         A[jj * K2 + 2 * k + 1] = xr * ei + xi * er;
       }
       __syncthreads();
-      const int nit = Tb.grp_rcount[pt] << lg;
+      // Work items of a stage (rotation items by degree, z-shift items by
+      // order; the two counts are equal): member jj of item u0 in the first
+      // round and of item NI-1-u0 in the second, which pairs the cheapest
+      // items with the most expensive ones.
+      const int NI = Tb.grp_rcount[pt], U0 = bs >> lg;
+      const int u0 = tid >> lg, jj = tid & (gs - 1);
+      int uq[kM2LItems];
+      uq[0] = u0 < NI ? u0 : -1;
+      uq[1] = NI - 1 - u0 >= U0 ? NI - 1 - u0 : -1;
+      const bool mem = jj < g;
       T r[kM2LItems][16];
       // (b) rotate to the z axis
 #pragma unroll
       for (int q = 0; q < kM2LItems; q++) {
 #pragma unroll
         for (int e = 0; e < 16; e++) r[q][e] = T(0);
-        const int i = tid + q * bs, jj = i & (gs - 1);
-        if (i < nit && jj < g) {
-          const int it = Tb.grp_ritems[i >> lg], n = it >> 8;
+        if (uq[q] >= 0 && mem) {
+          const int it = Tb.grp_ritems[uq[q]], n = it >> 8;
           grp_rot_item<T, false>(Tb.rot_half + Tb.rot_half_off[rc * P1 + n], grp_w(n), n, 4 * (it & 255),
                                  A + jj * K2, r[q]);
         }
       }
       __syncthreads();
 #pragma unroll
-      for (int q = 0; q < kM2LItems; q++) {
-        const int i = tid + q * bs, jj = i & (gs - 1);
-        if (i < nit && jj < g) {
-          const int it = Tb.grp_ritems[i >> lg];
+      for (int q = 0; q < kM2LItems; q++)
+        if (uq[q] >= 0 && mem) {
+          const int it = Tb.grp_ritems[uq[q]];
           grp_rot_store<T>(it >> 8, 4 * (it & 255), r[q], A + jj * K2);
         }
-      }
       __syncthreads();
       // (c) shift along z
-      const int zo = Tb.grp_zoff[pt], nzt = (Tb.grp_zoff[pt + 1] - zo) << lg;
+      const int zo = Tb.grp_zoff[pt];
 #pragma unroll
       for (int q = 0; q < kM2LItems; q++) {
 #pragma unroll
         for (int e = 0; e < 16; e++) r[q][e] = T(0);
-        const int i = tid + q * bs, jj = i & (gs - 1);
-        if (i < nzt && jj < g) {
-          const int it = Tb.grp_zitems[zo + (i >> lg)], am = it >> 8;
+        if (uq[q] >= 0 && mem) {
+          const int it = Tb.grp_zitems[zo + uq[q]], am = it >> 8;
           grp_z_item<T>(Z + Tb.zt_off[am], grp_w(p - am), am, am + 4 * (it & 255), pt, A + jj * K2, r[q]);
         }
       }
       __syncthreads();
 #pragma unroll
-      for (int q = 0; q < kM2LItems; q++) {
-        const int i = tid + q * bs, jj = i & (gs - 1);
-        if (i < nzt && jj < g) {
-          const int it = Tb.grp_zitems[zo + (i >> lg)], am = it >> 8;
+      for (int q = 0; q < kM2LItems; q++)
+        if (uq[q] >= 0 && mem) {
+          const int it = Tb.grp_zitems[zo + uq[q]], am = it >> 8;
           grp_z_store<T>(am, am + 4 * (it & 255), pt, r[q], A + jj * K2);
         }
-      }
       __syncthreads();
       // (d) rotate back, apply e^{-i m phi} and the scale, sum the members
 #pragma unroll
       for (int q = 0; q < kM2LItems; q++) {
 #pragma unroll
         for (int e = 0; e < 16; e++) r[q][e] = T(0);
-        const int i = tid + q * bs, jj = i & (gs - 1);
         int n = 0, mp0 = 0;
-        if (i < nit) {
-          const int it = Tb.grp_ritems[i >> lg];
+        if (uq[q] >= 0) {
+          const int it = Tb.grp_ritems[uq[q]];
           n = it >> 8;
           mp0 = 4 * (it & 255);
-          if (jj < g) {
+          if (mem) {
             grp_rot_item<T, true>(Tb.rot_half + Tb.rot_half_off[rc * P1 + n], grp_w(n), n, mp0, A + jj * K2, r[q]);
             const double* ph = Tb.off_phase + (size_t)t_code[s0 + jj] * 2 * P1;
 #pragma unroll
@@ -721,7 +1007,7 @@ __global__ __launch_bounds__(256) void k_m2l_grp(DevPlan P, DevTables Tb, int gm
           if (gs > 2) v += __shfl_xor(v, 2);
           r[q][e] = v;
         }
-        if (i < nit && jj == 0) {
+        if (uq[q] >= 0 && jj == 0) {
 #pragma unroll
           for (int c = 0; c < 4; c++) {
             const int mq = mp0 + c;

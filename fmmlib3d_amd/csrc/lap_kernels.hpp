@@ -23,6 +23,7 @@ struct DevPlan {
   // work lists
   const int *p2m_boxes;  int n_p2m;
   const int *l2p_boxes;  int n_l2p;
+  const int *l2p_items;  int n_l2pi;  // (position in l2p_boxes, kind, start) triples
   const int *m2p_boxes;  int n_m2p;   // leaves with list 3
   const int *p2l_boxes;  int n_p2l;   // boxes with list 4
   const int *m2l_boxes;  int n_m2l;   // targets with (pruned) list 2

@@ -44,9 +44,9 @@ struct Flat {
 
 struct PlanOffsets {
   size_t level, isrc, nsrc, itrg, ntrg, nchild, child, ijk, l1o, l1i, l2o, l2i, l3o, l3i, l4o, l4i;
-  size_t p2m, l2p, m2p, p2l, m2l, m2m, l2l, p2p0, p2p1, sperm, tperm;
+  size_t p2m, l2p, m2p, p2l, m2l, m2m, l2l, p2p0, p2p1, sperm, tperm, l2pi;
   size_t center, inv_side;
-  int n_p2m, n_l2p, n_m2p, n_p2l, n_m2l, n_p2p0, n_p2p1;
+  int n_p2m, n_l2p, n_m2p, n_p2l, n_m2l, n_p2p0, n_p2p1, n_l2pi;
   std::vector<std::pair<int, int>> m2m_groups, l2l_groups;  // (first, count) in launch order
   std::vector<int> m2m_group_level, l2l_group_level;         // parent level of each group
 };
@@ -105,6 +105,19 @@ inline void build_plan(const Octree& t, const Flags& f, Flat& F, PlanOffsets& O,
   }
   O.p2m = F.i(p2m); O.n_p2m = (int)p2m.size();
   O.l2p = F.i(l2p); O.n_l2p = (int)l2p.size();
+  {
+    // L2P work items (position in the L2P list, kind, first point): one
+    // workgroup per 256 evaluation points of one kind
+    std::vector<int> it;
+    for (int i = 0; i < (int)l2p.size(); i++) {
+      const int b = l2p[i];
+      if (want_src)
+        for (int q = 0; q < t.nsrc[b]; q += kP2PBlock) { it.push_back(i); it.push_back(0); it.push_back(q); }
+      if (want_trg)
+        for (int q = 0; q < t.ntrg[b]; q += kP2PBlock) { it.push_back(i); it.push_back(1); it.push_back(q); }
+    }
+    O.l2pi = F.i(it); O.n_l2pi = (int)it.size() / 3;
+  }
   O.m2p = F.i(m2p); O.n_m2p = (int)m2p.size();
   O.p2l = F.i(p2l); O.n_p2l = (int)p2l.size();
   O.p2p0 = F.i(p2p0); O.n_p2p0 = (int)p2p0.size() / 3;
@@ -155,6 +168,7 @@ inline DevPlan make_devplan(const Octree& t, int p, const PlanOffsets& O, const 
   P.l3_off = I + O.l3o; P.l3_idx = I + O.l3i; P.l4_off = I + O.l4o; P.l4_idx = I + O.l4i;
   P.p2m_boxes = I + O.p2m; P.n_p2m = O.n_p2m;
   P.l2p_boxes = I + O.l2p; P.n_l2p = O.n_l2p;
+  P.l2p_items = I + O.l2pi; P.n_l2pi = O.n_l2pi;
   P.m2p_boxes = I + O.m2p; P.n_m2p = O.n_m2p;
   P.p2l_boxes = I + O.p2l; P.n_p2l = O.n_p2l;
   P.m2l_boxes = I + O.m2l; P.n_m2l = O.n_m2l;

@@ -31,10 +31,33 @@ void* Arena::get(const std::string& name, size_t bytes, int ier_on_fail) {
   return b.p;
 }
 
+void* Arena::get_host(const std::string& name, size_t bytes, int ier_on_fail) {
+  if (bytes == 0) bytes = 256;
+  bytes = (bytes + 255) & ~(size_t)255;
+  Buf& b = hbufs_[name];
+  if (b.n >= bytes) return b.p;
+  if (b.p) {
+    (void)hipHostFree(b.p);
+    b.p = nullptr;
+    b.n = 0;
+  }
+  const size_t want = bytes + bytes / 8;
+  if (hipHostMalloc(&b.p, want, hipHostMallocDefault) != hipSuccess) {
+    (void)hipGetLastError();
+    b.p = nullptr;
+    throw AllocError("pinned host allocation failed for " + name, ier_on_fail);
+  }
+  b.n = want;
+  return b.p;
+}
+
 void Arena::release_all() {
   for (auto& kv : bufs_)
     if (kv.second.p) (void)hipFree(kv.second.p);
   bufs_.clear();
+  for (auto& kv : hbufs_)
+    if (kv.second.p) (void)hipHostFree(kv.second.p);
+  hbufs_.clear();
 }
 
 void Timer::start(hipStream_t stream, int n) {

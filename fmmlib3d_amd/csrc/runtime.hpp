@@ -32,11 +32,13 @@ class Arena {
  public:
   ~Arena();
   void* get(const std::string& name, size_t bytes, int ier_on_fail = 8);
+  // pinned host buffer (hipHostMalloc), same growth rule
+  void* get_host(const std::string& name, size_t bytes, int ier_on_fail = 8);
   void release_all();
 
  private:
   struct Buf { void* p = nullptr; size_t n = 0; };
-  std::map<std::string, Buf> bufs_;
+  std::map<std::string, Buf> bufs_, hbufs_;
 };
 
 struct Timer {
