@@ -23,7 +23,7 @@ HIPCC = os.path.join(ROCM, "bin", "hipcc")
 CXX = os.path.join(ROCM, "llvm", "bin", "clang++")
 FC = os.path.join(ROCM, "bin", "amdflang")
 
-HIP_SOURCES = ["runtime.hip", "fmm3d_lap.hip", "fmm3d_helm.hip", "fmm3d_api.hip", "server.hip"]
+HIP_SOURCES = ["runtime.hip", "tree_gpu.hip", "fmm3d_lap.hip", "fmm3d_helm.hip", "fmm3d_api.hip", "server.hip"]
 CPP_SOURCES = ["octree.cpp", "tables.cpp", "helm_tables.cpp"]
 
 # Packed FP32 instructions are disabled for all device code.

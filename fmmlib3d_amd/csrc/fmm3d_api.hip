@@ -9,6 +9,7 @@
 
 #include "../../include/fmm3d_mi355x.h"
 #include "fmm3d_internal.hpp"
+#include "tree_gpu.hpp"
 #include "octree.hpp"
 
 using namespace fmm3d;
@@ -140,20 +141,24 @@ int fmm_dev(const double* zk, int prec_mode, int iprec, int ns, const double* so
   hipStream_t saved = ctx.stream;
   ctx.stream = (hipStream_t)stream;
   int ier = guarded([&] {
-    // the octree is built on the host: bring the coordinates back into
-    // pinned buffers kept by the context
-    double* hs = (double*)ctx.arena.get_host("coord_src", 3 * sizeof(double) * (size_t)ns);
-    double* ht = (double*)ctx.arena.get_host("coord_trg", 3 * sizeof(double) * (size_t)std::max(nt, 1));
-    FMM_HIP(hipMemcpyAsync(hs, source_dev, 3 * sizeof(double) * (size_t)ns, hipMemcpyDeviceToHost, ctx.stream));
-    if (nt > 0)
-      FMM_HIP(hipMemcpyAsync(ht, target_dev, 3 * sizeof(double) * (size_t)nt, hipMemcpyDeviceToHost, ctx.stream));
-    FMM_HIP(hipStreamSynchronize(ctx.stream));
+    // the octree is built on the device from the device coordinates; the
+    // host builder (FMM3D_TREE=host) needs them back in pinned buffers
+    double* hs = nullptr;
+    double* ht = nullptr;
+    if (!device_tree_enabled()) {
+      hs = (double*)ctx.arena.get_host("coord_src", 3 * sizeof(double) * (size_t)ns);
+      ht = (double*)ctx.arena.get_host("coord_trg", 3 * sizeof(double) * (size_t)std::max(nt, 1));
+      FMM_HIP(hipMemcpyAsync(hs, source_dev, 3 * sizeof(double) * (size_t)ns, hipMemcpyDeviceToHost, ctx.stream));
+      if (nt > 0)
+        FMM_HIP(hipMemcpyAsync(ht, target_dev, 3 * sizeof(double) * (size_t)nt, hipMemcpyDeviceToHost, ctx.stream));
+      FMM_HIP(hipStreamSynchronize(ctx.stream));
+    }
     LapCall c;
     c.iprec = iprec; c.ns = ns; c.nt = nt;
     c.ifcharge = ifcharge == 1; c.ifdipole = ifdipole == 1;
     c.ifpot = ifpot == 1; c.iffld = iffld == 1;
     c.ifpottarg = nt > 0 && ifpottarg == 1; c.iffldtarg = nt > 0 && iffldtarg == 1;
-    c.src_host = hs; c.trg_host = nt > 0 ? ht : nullptr;
+    c.src_host = hs; c.trg_host = nt > 0 ? ht : nullptr;  // null with the device builder
     c.src_dev = source_dev; c.q_dev = charge_dev; c.ds_dev = dipstr_dev; c.dv_dev = dipvec_dev;
     c.trg_dev = target_dev;
     c.pot_dev = pot_dev; c.fld_dev = fld_dev; c.pottarg_dev = pottarg_dev; c.fldtarg_dev = fldtarg_dev;

@@ -15,6 +15,7 @@
 #include "fmm3d_internal.hpp"
 #include "helm_kernels.hip"
 #include "helm_tables.hpp"
+#include "tree_gpu.hpp"
 #include "octree.hpp"
 #include "plan.hpp"
 #include "runtime.hpp"
@@ -120,7 +121,13 @@ int helm_solve(Context& ctx, const LapCall& c) {
   const Flags f{c.ifcharge ? 1 : 0, c.ifdipole ? 1 : 0, c.ifpot ? 1 : 0, c.iffld ? 1 : 0,
                 nt > 0 && c.ifpottarg ? 1 : 0, nt > 0 && c.iffldtarg ? 1 : 0};
   thread_local Octree tree;  // storage reused across calls
-  if (build_octree(c.src_host, ns, c.trg_host, nt, nbox, tree) != 0) return 4;
+  const int* dsperm = nullptr;
+  const int* dtperm = nullptr;
+  if (device_tree_enabled()) {
+    if (build_octree_device(ctx, ctx.stream, c.src_dev, ns, c.trg_dev, nt, nbox, tree, &dsperm, &dtperm) != 0) return 4;
+  } else if (build_octree(c.src_host, ns, c.trg_host, nt, nbox, tree) != 0) {
+    return 4;
+  }
   int ier = 0;
   HelmCache* HC = helm_tables(ctx, zk, tree.size, tree.nlev, eps, ier);
   if (!HC) return ier;
@@ -176,8 +183,10 @@ int helm_solve(Context& ctx, const LapCall& c) {
   double* fld_s = (double*)ctx.arena.get("fld_s", sizeof(double) * 6 * (size_t)ns);
   double* pot_t = (double*)ctx.arena.get("pot_t", sizeof(double) * 2 * (size_t)std::max(nt, 1));
   double* fld_t = (double*)ctx.arena.get("fld_t", sizeof(double) * 6 * (size_t)std::max(nt, 1));
-  const int* sperm = dI + O.sperm;
-  const int* tperm = dI + O.tperm;
+  const int* sperm = dsperm ? dsperm : dI + O.sperm;
+  const int* tperm = dsperm ? dtperm : dI + O.tperm;
+  P.src_perm = sperm;
+  P.trg_perm = tperm;
   hipLaunchKernelGGL(k_gather_sources, dim3(blocks(ns, 256)), dim3(256), 0, s, ns, sperm, c.src_dev,
                      c.q_dev, c.ds_dev, c.dv_dev, f.ifcharge, f.ifdipole, spos, str64, str32);
   if (nt > 0)

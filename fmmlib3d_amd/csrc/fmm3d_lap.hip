@@ -17,6 +17,7 @@
 #include "plan.hpp"
 #include "runtime.hpp"
 #include "tables.hpp"
+#include "tree_gpu.hpp"
 
 namespace fmm3d {
 
@@ -141,7 +142,13 @@ int lap_solve(Context& ctx, const LapCall& c) {
   const Flags f{c.ifcharge ? 1 : 0, c.ifdipole ? 1 : 0, c.ifpot ? 1 : 0, c.iffld ? 1 : 0,
                 nt > 0 && c.ifpottarg ? 1 : 0, nt > 0 && c.iffldtarg ? 1 : 0};
   thread_local Octree tree;  // storage reused across calls
-  if (build_octree(c.src_host, ns, c.trg_host, nt, nbox, tree) != 0) return 4;
+  const int* dsperm = nullptr;
+  const int* dtperm = nullptr;
+  if (device_tree_enabled()) {
+    if (build_octree_device(ctx, ctx.stream, c.src_dev, ns, c.trg_dev, nt, nbox, tree, &dsperm, &dtperm) != 0) return 4;
+  } else if (build_octree(c.src_host, ns, c.trg_host, nt, nbox, tree) != 0) {
+    return 4;
+  }
   const int p = laplace_nterms(eps);
   const int K = (p + 1) * (p + 1);
   if (sizeof(T) * 4 * (size_t)K > max_lds_bytes(ctx)) return 16;
@@ -172,8 +179,10 @@ int lap_solve(Context& ctx, const LapCall& c) {
   double* pot_t = (double*)ctx.arena.get("pot_t", sizeof(double) * 2 * (size_t)std::max(nt, 1));
   double* fld_t = (double*)ctx.arena.get("fld_t", sizeof(double) * 6 * (size_t)std::max(nt, 1));
   const TN* strn = near32 ? (const TN*)(const void*)str32 : (const TN*)(const void*)str64;
-  const int* sperm = dI + O.sperm;
-  const int* tperm = dI + O.tperm;
+  const int* sperm = dsperm ? dsperm : dI + O.sperm;
+  const int* tperm = dsperm ? dtperm : dI + O.tperm;
+  P.src_perm = sperm;
+  P.trg_perm = tperm;
   hipLaunchKernelGGL(k_gather_sources, dim3(blocks(ns, 256)), dim3(256), 0, s, ns, sperm, c.src_dev,
                      c.q_dev, c.ds_dev, c.dv_dev, f.ifcharge, f.ifdipole, spos, str64, str32);
   if (nt > 0)
@@ -240,19 +249,32 @@ int lap_solve(Context& ctx, const LapCall& c) {
     }();
     const bool lds = sizeof(T) * 6 * (size_t)K <= max_lds_bytes(ctx);
     const size_t sh = lds ? sizeof(T) * 6 * K : lds_ps;
-    // group size: work items of one stage must fit kM2LItems per thread, the
-    // group's expansions must fit LDS
+    // launch shape of k_m2l_grp: (largest group, threads, items per thread);
+    // the work items of one stage must fit, the group's expansions must fit LDS
+    static const int m2l_shape = [] {
+      const char* v = std::getenv("FMM3D_M2L_SHAPE");
+      if (v && std::strcmp(v, "g2t256i1") == 0) return 1;
+      if (v && std::strcmp(v, "g4t512i1") == 0) return 2;
+      return 0;
+    }();
+    const int gcap = m2l_shape == 1 ? 2 : kM2LGroup;
+    const int nt = m2l_shape == 2 ? 512 : 256, items = m2l_shape == 0 ? 2 : 1;
     int nI = 0;
     for (int n = 0; n <= p; n++) nI += n / 4 + 1;
     int gmax = 0;
-    for (int g = kM2LGroup; g >= 1 && !gmax; g--) {
+    for (int g = gcap; g >= 1 && !gmax; g--) {
       const int gs = g > 2 ? 4 : g;
       const size_t need = 6 * kM2LChunk * sizeof(int) + sizeof(T) * (size_t)(1 + g) * 2 * K;
-      if (nI * gs <= kM2LItems * 256 && need <= max_lds_bytes(ctx)) gmax = g;
+      if (nI * gs <= items * nt && need <= max_lds_bytes(ctx)) gmax = g;
     }
     if (m2l_form == 0 && gmax > 0) {
       const size_t need = 6 * kM2LChunk * sizeof(int) + sizeof(T) * (size_t)(1 + gmax) * 2 * K;
-      hipLaunchKernelGGL((k_m2l_grp<T>), dim3(P.n_m2l), dim3(256), need, s, P, Tb, gmax, mp, lo);
+      if (m2l_shape == 0)
+        hipLaunchKernelGGL((k_m2l_grp<T, 256, 2>), dim3(P.n_m2l), dim3(256), need, s, P, Tb, gmax, mp, lo);
+      else if (m2l_shape == 1)
+        hipLaunchKernelGGL((k_m2l_grp<T, 256, 1>), dim3(P.n_m2l), dim3(256), need, s, P, Tb, gmax, mp, lo);
+      else
+        hipLaunchKernelGGL((k_m2l_grp<T, 512, 1>), dim3(P.n_m2l), dim3(512), need, s, P, Tb, gmax, mp, lo);
     } else if (m2l_form != 2) {
       if (lds) hipLaunchKernelGGL((k_m2l<T, true>), dim3(P.n_m2l), dim3(256), sh, s, P, Tb, mp, lo);
       else hipLaunchKernelGGL((k_m2l<T, false>), dim3(P.n_m2l), dim3(256), sh, s, P, Tb, mp, lo);

@@ -720,7 +720,6 @@ __global__ void k_m2l(DevPlan P, DevTables Tb, const T* __restrict__ mp, T* __re
 // and adds the sum to the target's local in LDS.
 // ---------------------------------------------------------------------------
 constexpr int kM2LGroup = 4;   // members per group
-constexpr int kM2LItems = 2;   // items per thread and stage (256 threads)
 constexpr int kM2LChunk = 192; // list-2 entries ordered at a time (list 2 has <= 189)
 __device__ __forceinline__ int grp_w(int n) { return 4 * (n / 4 + 1); }  // padded row width
 
@@ -858,8 +857,8 @@ __device__ __forceinline__ void grp_z_store(int am, int l0, int pt, const T (&r)
   }
 }
 
-template <typename T>
-__global__ __launch_bounds__(256) void k_m2l_grp(DevPlan P, DevTables Tb, int gmax, const T* __restrict__ mp,
+template <typename T, int NT, int ITEMS>
+__global__ __launch_bounds__(NT) void k_m2l_grp(DevPlan P, DevTables Tb, int gmax, const T* __restrict__ mp,
                                                   T* __restrict__ lo) {
   extern __shared__ unsigned char smem_raw[];
   int* s_key = reinterpret_cast<int*>(smem_raw);
@@ -932,14 +931,14 @@ __global__ __launch_bounds__(256) void k_m2l_grp(DevPlan P, DevTables Tb, int gm
       // items with the most expensive ones.
       const int NI = Tb.grp_rcount[pt], U0 = bs >> lg;
       const int u0 = tid >> lg, jj = tid & (gs - 1);
-      int uq[kM2LItems];
+      int uq[ITEMS];
       uq[0] = u0 < NI ? u0 : -1;
-      uq[1] = NI - 1 - u0 >= U0 ? NI - 1 - u0 : -1;
+      if (ITEMS > 1) uq[ITEMS - 1] = NI - 1 - u0 >= U0 ? NI - 1 - u0 : -1;
       const bool mem = jj < g;
-      T r[kM2LItems][16];
+      T r[ITEMS][16];
       // (b) rotate to the z axis
 #pragma unroll
-      for (int q = 0; q < kM2LItems; q++) {
+      for (int q = 0; q < ITEMS; q++) {
 #pragma unroll
         for (int e = 0; e < 16; e++) r[q][e] = T(0);
         if (uq[q] >= 0 && mem) {
@@ -950,7 +949,7 @@ __global__ __launch_bounds__(256) void k_m2l_grp(DevPlan P, DevTables Tb, int gm
       }
       __syncthreads();
 #pragma unroll
-      for (int q = 0; q < kM2LItems; q++)
+      for (int q = 0; q < ITEMS; q++)
         if (uq[q] >= 0 && mem) {
           const int it = Tb.grp_ritems[uq[q]];
           grp_rot_store<T>(it >> 8, 4 * (it & 255), r[q], A + jj * K2);
@@ -959,7 +958,7 @@ __global__ __launch_bounds__(256) void k_m2l_grp(DevPlan P, DevTables Tb, int gm
       // (c) shift along z
       const int zo = Tb.grp_zoff[pt];
 #pragma unroll
-      for (int q = 0; q < kM2LItems; q++) {
+      for (int q = 0; q < ITEMS; q++) {
 #pragma unroll
         for (int e = 0; e < 16; e++) r[q][e] = T(0);
         if (uq[q] >= 0 && mem) {
@@ -969,7 +968,7 @@ __global__ __launch_bounds__(256) void k_m2l_grp(DevPlan P, DevTables Tb, int gm
       }
       __syncthreads();
 #pragma unroll
-      for (int q = 0; q < kM2LItems; q++)
+      for (int q = 0; q < ITEMS; q++)
         if (uq[q] >= 0 && mem) {
           const int it = Tb.grp_zitems[zo + uq[q]], am = it >> 8;
           grp_z_store<T>(am, am + 4 * (it & 255), pt, r[q], A + jj * K2);
@@ -977,7 +976,7 @@ __global__ __launch_bounds__(256) void k_m2l_grp(DevPlan P, DevTables Tb, int gm
       __syncthreads();
       // (d) rotate back, apply e^{-i m phi} and the scale, sum the members
 #pragma unroll
-      for (int q = 0; q < kM2LItems; q++) {
+      for (int q = 0; q < ITEMS; q++) {
 #pragma unroll
         for (int e = 0; e < 16; e++) r[q][e] = T(0);
         int n = 0, mp0 = 0;

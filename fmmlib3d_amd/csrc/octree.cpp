@@ -93,6 +93,16 @@ void stable_bin_sort(const Pt* a, Pt* b, int lo, int n, int nbins, int T, const 
   }
 }
 
+void push_box(Octree& t, int level, int i, int j, int k, int par, int is, int n_s, int it, int n_t) {
+  t.level.push_back(level); t.parent.push_back(par); t.nchild.push_back(0);
+  for (int q = 0; q < 8; q++) t.child.push_back(-1);
+  t.ijk.push_back(i); t.ijk.push_back(j); t.ijk.push_back(k);
+  t.isrc.push_back(is); t.nsrc.push_back(n_s); t.itrg.push_back(it); t.ntrg.push_back(n_t);
+  t.bcenter.push_back(box_center_1d(t.center[0], t.size, level, i));
+  t.bcenter.push_back(box_center_1d(t.center[1], t.size, level, j));
+  t.bcenter.push_back(box_center_1d(t.center[2], t.size, level, k));
+}
+
 inline uint64_t ijk_key(int i, int j, int k) {
   return ((uint64_t)(uint32_t)i << 42) | ((uint64_t)(uint32_t)j << 21) | (uint64_t)(uint32_t)k;
 }
@@ -123,243 +133,119 @@ void flatten(std::vector<std::vector<int>>& per, Csr& c) {
 
 }  // namespace
 
-int build_octree(const double* src, int ns, const double* trg, int nt, int nbox, Octree& t) {
-  t.reset();
-  if (ns < 1) return 128;
-  if (nt < 0) nt = 0;
+// ---------------------------------------------------------------------------
+// Box logic shared by the host and device builders
+// ---------------------------------------------------------------------------
+
+void tree_root(Octree& t, const double lo[3], const double hi[3], int ns, int nt, int nbox,
+               std::vector<int>& frontier) {
   // bounding cube of sources and targets (d3tstrcr.f:1357-1395)
-  double lo[3] = {src[0], src[1], src[2]}, hi[3] = {src[0], src[1], src[2]};
-  auto scan = [&](const double* p, int n) {
-    double l0 = lo[0], l1 = lo[1], l2 = lo[2], h0 = hi[0], h1 = hi[1], h2 = hi[2];
-#pragma omp parallel for reduction(min : l0, l1, l2) reduction(max : h0, h1, h2)
-    for (int i = 0; i < n; i++) {
-      l0 = std::min(l0, p[3 * i]); h0 = std::max(h0, p[3 * i]);
-      l1 = std::min(l1, p[3 * i + 1]); h1 = std::max(h1, p[3 * i + 1]);
-      l2 = std::min(l2, p[3 * i + 2]); h2 = std::max(h2, p[3 * i + 2]);
-    }
-    lo[0] = l0; lo[1] = l1; lo[2] = l2; hi[0] = h0; hi[1] = h1; hi[2] = h2;
-  };
-  scan(src, ns);
-  if (nt > 0) scan(trg, nt);
   double size = hi[0] - lo[0];
   if (hi[1] - lo[1] > size) size = hi[1] - lo[1];
   if (hi[2] - lo[2] > size) size = hi[2] - lo[2];
   for (int d = 0; d < 3; d++) t.center[d] = (lo[d] + hi[d]) / 2;
   t.size = size;
-
-  // workspace: positions with their original index
-  Work& WS = g_src;
-  Work& WT = g_trg;
-  if ((int)WS.a.size() < ns) { WS.a.resize(ns); WS.b.resize(ns); }
-  if ((int)WT.a.size() < nt) { WT.a.resize(nt); WT.b.resize(nt); }
-  Pt* SA = WS.a.data();
-  Pt* SB = WS.b.data();
-  Pt* TA = WT.a.data();
-  Pt* TB = WT.b.data();
-#pragma omp parallel for schedule(static)
-  for (int i = 0; i < ns; i++) SA[i] = Pt{src[3 * i], src[3 * i + 1], src[3 * i + 2], i, 0u};
-#pragma omp parallel for schedule(static)
-  for (int i = 0; i < nt; i++) TA[i] = Pt{trg[3 * i], trg[3 * i + 1], trg[3 * i + 2], i, 0u};
-
-  auto push_box = [&](int level, int i, int j, int k, int par, int is, int n_s, int it, int n_t) {
-    t.level.push_back(level); t.parent.push_back(par); t.nchild.push_back(0);
-    for (int q = 0; q < 8; q++) t.child.push_back(-1);
-    t.ijk.push_back(i); t.ijk.push_back(j); t.ijk.push_back(k);
-    t.isrc.push_back(is); t.nsrc.push_back(n_s); t.itrg.push_back(it); t.ntrg.push_back(n_t);
-    t.bcenter.push_back(box_center_1d(t.center[0], size, level, i));
-    t.bcenter.push_back(box_center_1d(t.center[1], size, level, j));
-    t.bcenter.push_back(box_center_1d(t.center[2], size, level, k));
-  };
-  push_box(0, 1, 1, 1, -1, 0, ns, 0, nt);
+  t.ns = ns;
+  t.nt = nt;
+  push_box(t, 0, 1, 1, 1, -1, 0, ns, 0, nt);
   t.ladd = {0, 1};
-  const int nthreads = omp_get_max_threads();
-  constexpr int kMaxLevel = 100;
-  std::vector<int> frontier;
+  frontier.clear();
   if (ns > nbox || nt > nbox) frontier.push_back(0);
-  int L = 0;  // level of the frontier boxes
-  // Each pass resolves up to D levels below the frontier at once: every point
-  // gets the D octant digits of its path (the same centre arithmetic and
-  // strict ">" test as one split per level), a histogram of the digit codes
-  // gives every sub-box's counts and hence which sub-boxes split, and one
-  // stable counting sort by the code truncated at the point's leaf puts the
-  // points in the order D successive stable 8-way partitions would.
-  while (!frontier.empty() && L < kMaxLevel) {
-    const int nf = (int)frontier.size();
-    int D = 4;
-    while (D > 1 && (long)nf << (3 * D) > (1L << 22)) D--;
-    D = std::min(D, kMaxLevel - L);
-    const int NB = 1 << (3 * D);
-    // centre tables per frontier box: ctab[f][dim][depth d][sub-index]
-    const int CT = (1 << D) * 2;  // sum_{d<D} 2^d < 2^D, padded
-    std::vector<double> ctab((size_t)nf * 3 * D * CT);
-    for (int f = 0; f < nf; f++) {
-      const int b = frontier[f];
-      for (int dim = 0; dim < 3; dim++)
-        for (int d = 0; d < D; d++)
-          for (int ii = 0; ii < (1 << d); ii++)
-            ctab[(((size_t)f * 3 + dim) * D + d) * CT + ii] =
-                box_center_1d(t.center[dim], size, L + d, (t.ijk[3 * b + dim] - 1) * (1 << d) + 1 + ii);
-    }
-    auto code_of = [&](const Pt& p, int f) -> uint32_t {
-      const double* cx = &ctab[(((size_t)f * 3 + 0) * D) * CT];
-      const double* cy = &ctab[(((size_t)f * 3 + 1) * D) * CT];
-      const double* cz = &ctab[(((size_t)f * 3 + 2) * D) * CT];
-      uint32_t code = 0;
-      int ii = 0, jj = 0, kk = 0;
-      for (int d = 0; d < D; d++) {
-        const int o = (p.x > cx[d * CT + ii]) | ((p.y > cy[d * CT + jj]) << 1) | ((p.z > cz[d * CT + kk]) << 2);
-        code = (code << 3) | (uint32_t)o;
-        ii = 2 * ii + (o & 1);
-        jj = 2 * jj + ((o >> 1) & 1);
-        kk = 2 * kk + ((o >> 2) & 1);
-      }
-      return code;
-    };
-    // digit codes and per-box histograms (exclusive prefix sums)
-    std::vector<int> hs((size_t)nf * (NB + 1), 0), ht((size_t)nf * (NB + 1), 0);
-    auto codes_and_hist = [&](Pt* P, int lo, int n, int f, int* h, bool par) {
-      if (par) {
-        std::vector<int> part((size_t)nthreads * NB, 0);
-#pragma omp parallel num_threads(nthreads)
-        {
-          const int th = omp_get_thread_num();
-          const int c0 = lo + (int)((long)n * th / nthreads), c1 = lo + (int)((long)n * (th + 1) / nthreads);
-          int* pt = &part[(size_t)th * NB];
-          for (int i = c0; i < c1; i++) {
-            P[i].code = code_of(P[i], f);
-            pt[P[i].code]++;
-          }
-        }
-        for (int th = 0; th < nthreads; th++)
-          for (int k = 0; k < NB; k++) h[k + 1] += part[(size_t)th * NB + k];
-      } else {
-        for (int i = lo; i < lo + n; i++) {
-          P[i].code = code_of(P[i], f);
-          h[P[i].code + 1]++;
-        }
-      }
-      for (int k = 0; k < NB; k++) h[k + 1] += h[k];
-    };
-    const bool few = nf < 2 * nthreads;
-    if (few) {
-      for (int f = 0; f < nf; f++) {
-        const int b = frontier[f];
-        codes_and_hist(SA, t.isrc[b], t.nsrc[b], f, &hs[(size_t)f * (NB + 1)], t.nsrc[b] > (1 << 15));
-        codes_and_hist(TA, t.itrg[b], t.ntrg[b], f, &ht[(size_t)f * (NB + 1)], t.ntrg[b] > (1 << 15));
-      }
-    } else {
-#pragma omp parallel for schedule(dynamic, 1)
-      for (int f = 0; f < nf; f++) {
-        const int b = frontier[f];
-        codes_and_hist(SA, t.isrc[b], t.nsrc[b], f, &hs[(size_t)f * (NB + 1)], false);
-        codes_and_hist(TA, t.itrg[b], t.ntrg[b], f, &ht[(size_t)f * (NB + 1)], false);
-      }
-    }
-    // sub-boxes, level by level across all frontier boxes (breadth-first ids);
-    // node (d, prefix) of frontier box f: id and whether it splits further
-    const int NS = NB * 2;  // sum_{d<=D} 8^d < 2*8^D, stored at [pos(d) + prefix]
-    auto pos = [&](int d) { int o = 0; for (int e = 1; e < d; e++) o += 1 << (3 * e); return o; };
-    std::vector<int> node((size_t)nf * NS, -1);
-    std::vector<char> splits((size_t)nf * NS, 0);
-    std::vector<int> next;
-    for (int d = 1; d <= D; d++) {
-      const int sh = 3 * (D - d), pd = pos(d), pp = pos(d - 1);
-      for (int f = 0; f < nf; f++) {
-        const int fb = frontier[f];
-        const int* hsf = &hs[(size_t)f * (NB + 1)];
-        const int* htf = &ht[(size_t)f * (NB + 1)];
-        const int nparents = d == 1 ? 1 : 1 << (3 * (d - 1));
-        for (int P0 = 0; P0 < nparents; P0++) {
-          int par;
-          if (d == 1) par = fb;
-          else {
-            if (!splits[(size_t)f * NS + pp + P0]) continue;
-            par = node[(size_t)f * NS + pp + P0];
-          }
-          const int bi = t.ijk[3 * par], bj = t.ijk[3 * par + 1], bk = t.ijk[3 * par + 2];
-          for (int o = 0; o < 8; o++) {
-            const int P = P0 * 8 + o;
-            const int s0 = hsf[P << sh], s1 = hsf[(P + 1) << sh];
-            const int u0 = htf[P << sh], u1 = htf[(P + 1) << sh];
-            const int a = s1 - s0, e = u1 - u0;
-            if (a == 0 && e == 0) continue;
-            const int id = (int)t.level.size();
-            push_box(L + d, 2 * (bi - 1) + 1 + (o & 1), 2 * (bj - 1) + 1 + ((o >> 1) & 1),
-                     2 * (bk - 1) + 1 + ((o >> 2) & 1), par, t.isrc[fb] + s0, a, t.itrg[fb] + u0, e);
-            t.child[8 * par + t.nchild[par]++] = id;
-            node[(size_t)f * NS + pd + P] = id;
-            if (a > nbox || e > nbox) {
-              if (d < D) splits[(size_t)f * NS + pd + P] = 1;
-              else next.push_back(id);
-            }
-          }
-        }
-      }
-      t.ladd.push_back((int)t.level.size());
-      t.nlev = L + d;
-      bool more = false;
-      for (int f = 0; f < nf && !more; f++)
-        for (int P = 0; P < (1 << (3 * d)) && !more; P++) more = splits[(size_t)f * NS + pd + P] != 0;
-      if (!more && d < D) {
-        // nothing below this depth: the codes are truncated here for every point
-        D = d;
-        break;
-      }
-    }
-    // stable sort of each frontier box's points by the code truncated at
-    // the depth where the point's box stops splitting
-    const int Dk = D;
-    int posd[8];
-    for (int d = 0; d <= Dk; d++) posd[d] = pos(d);
-    auto eff = [&](uint32_t code, int f, int Dfull) -> uint32_t {
-      // Dfull: digits stored in the code; Dk: depth resolved in this pass
-      const uint32_t c = code >> (3 * (Dfull - Dk));
-      const char* sp = &splits[(size_t)f * NS];
-      for (int d = 1; d < Dk; d++) {
-        const uint32_t pre = c >> (3 * (Dk - d));
-        if (!sp[posd[d] + pre]) return pre << (3 * (Dk - d));
-      }
-      return c;
-    };
-    const int Dfull = [&] { int x = 4; while (x > 1 && (long)nf << (3 * x) > (1L << 22)) x--; return std::min(x, kMaxLevel - L); }();
-    const int NBk = 1 << (3 * Dk);
-    auto sort_box = [&](Pt* A, Pt* Bf, int lo, int n, int f, int T) {
-      stable_bin_sort(A, Bf, lo, n, NBk, T, [&](const Pt& p) { return (int)eff(p.code, f, Dfull); });
-      std::memcpy(A + lo, Bf + lo, sizeof(Pt) * (size_t)n);
-    };
-    if (nf == 1 && frontier[0] == 0) {
-      // the root: every point moves, so sort into the second buffer and swap
-      stable_bin_sort(SA, SB, 0, ns, NBk, nthreads, [&](const Pt& p) { return (int)eff(p.code, 0, Dfull); });
-      stable_bin_sort(TA, TB, 0, nt, NBk, nthreads, [&](const Pt& p) { return (int)eff(p.code, 0, Dfull); });
-      std::swap(SA, SB);
-      std::swap(TA, TB);
-    } else if (few) {
-      for (int f = 0; f < nf; f++) {
-        const int b = frontier[f];
-        sort_box(SA, SB, t.isrc[b], t.nsrc[b], f, nthreads);
-        sort_box(TA, TB, t.itrg[b], t.ntrg[b], f, nthreads);
-      }
-    } else {
-#pragma omp parallel for schedule(dynamic, 1)
-      for (int f = 0; f < nf; f++) {
-        const int b = frontier[f];
-        sort_box(SA, SB, t.isrc[b], t.nsrc[b], f, 1);
-        sort_box(TA, TB, t.itrg[b], t.ntrg[b], f, 1);
-      }
-    }
-    frontier.swap(next);
-    next.clear();
-    L += D;
-  }
-  t.nboxes = (int)t.level.size();
-  t.src_perm.resize(ns);
-  t.trg_perm.resize(nt);
-#pragma omp parallel for schedule(static)
-  for (int i = 0; i < ns; i++) t.src_perm[i] = SA[i].idx;
-#pragma omp parallel for schedule(static)
-  for (int i = 0; i < nt; i++) t.trg_perm[i] = TA[i].idx;
-  const int nb = t.nboxes;
+}
 
+void tree_pass_setup(const Octree& t, int L, TreePass& ps) {
+  const int nf = (int)ps.frontier.size();
+  int D = 4;
+  while (D > 1 && (long)nf << (3 * D) > (1L << 22)) D--;
+  D = std::min(D, kTreeMaxLevel - L);
+  ps.L = L;
+  ps.D = ps.Dfull = D;
+  ps.NB = 1 << (3 * D);
+  ps.CT = (1 << D) * 2;  // sum_{d<D} 2^d < 2^D, padded
+  ps.ctab.assign((size_t)nf * 3 * D * ps.CT, 0.0);
+  for (int f = 0; f < nf; f++) {
+    const int b = ps.frontier[f];
+    for (int dim = 0; dim < 3; dim++)
+      for (int d = 0; d < D; d++)
+        for (int ii = 0; ii < (1 << d); ii++)
+          ps.ctab[(((size_t)f * 3 + dim) * D + d) * ps.CT + ii] =
+              box_center_1d(t.center[dim], t.size, L + d, (t.ijk[3 * b + dim] - 1) * (1 << d) + 1 + ii);
+  }
+  ps.hs.assign((size_t)nf * (ps.NB + 1), 0);
+  ps.ht.assign((size_t)nf * (ps.NB + 1), 0);
+}
+
+int tree_pos(int d) {
+  int o = 0;
+  for (int e = 1; e < d; e++) o += 1 << (3 * e);
+  return o;
+}
+
+void tree_pass_expand(Octree& t, int nbox, TreePass& ps, std::vector<int>& next) {
+  // sub-boxes, level by level across all frontier boxes (breadth-first ids);
+  // node (d, prefix) of frontier box f: id and whether it splits further
+  const int nf = (int)ps.frontier.size();
+  int D = ps.D;
+  const int L = ps.L;
+  ps.NS = ps.NB * 2;  // sum_{d<=D} 8^d < 2*8^D, stored at [pos(d) + prefix]
+  const int NS = ps.NS;
+  std::vector<int> node((size_t)nf * NS, -1);
+  ps.splits.assign((size_t)nf * NS, 0);
+  next.clear();
+  for (int d = 1; d <= D; d++) {
+    const int sh = 3 * (D - d), pd = tree_pos(d), pp = tree_pos(d - 1);
+    for (int f = 0; f < nf; f++) {
+      const int fb = ps.frontier[f];
+      const int* hsf = &ps.hs[(size_t)f * (ps.NB + 1)];
+      const int* htf = &ps.ht[(size_t)f * (ps.NB + 1)];
+      const int nparents = d == 1 ? 1 : 1 << (3 * (d - 1));
+      for (int P0 = 0; P0 < nparents; P0++) {
+        int par;
+        if (d == 1) par = fb;
+        else {
+          if (!ps.splits[(size_t)f * NS + pp + P0]) continue;
+          par = node[(size_t)f * NS + pp + P0];
+        }
+        const int bi = t.ijk[3 * par], bj = t.ijk[3 * par + 1], bk = t.ijk[3 * par + 2];
+        for (int o = 0; o < 8; o++) {
+          const int P = P0 * 8 + o;
+          const int s0 = hsf[P << sh], s1 = hsf[(P + 1) << sh];
+          const int u0 = htf[P << sh], u1 = htf[(P + 1) << sh];
+          const int a = s1 - s0, e = u1 - u0;
+          if (a == 0 && e == 0) continue;
+          const int id = (int)t.level.size();
+          push_box(t, L + d, 2 * (bi - 1) + 1 + (o & 1), 2 * (bj - 1) + 1 + ((o >> 1) & 1),
+                   2 * (bk - 1) + 1 + ((o >> 2) & 1), par, t.isrc[fb] + s0, a, t.itrg[fb] + u0, e);
+          t.child[8 * par + t.nchild[par]++] = id;
+          node[(size_t)f * NS + pd + P] = id;
+          if (a > nbox || e > nbox) {
+            if (d < D) ps.splits[(size_t)f * NS + pd + P] = 1;
+            else next.push_back(id);
+          }
+        }
+      }
+    }
+    t.ladd.push_back((int)t.level.size());
+    t.nlev = L + d;
+    bool more = false;
+    for (int f = 0; f < nf && !more; f++)
+      for (int P = 0; P < (1 << (3 * d)) && !more; P++) more = ps.splits[(size_t)f * NS + pd + P] != 0;
+    if (!more && d < D) {
+      // nothing below this depth: the codes are truncated here for every point
+      D = d;
+      break;
+    }
+  }
+  ps.Dk = D;
+  for (int d = 0; d <= D; d++) ps.posd[d] = tree_pos(d);
+}
+
+// ---------------------------------------------------------------------------
+// Interaction lists from the box structure
+// ---------------------------------------------------------------------------
+void tree_lists(Octree& t) {
+  const int nb = t.nboxes;
+  const int nthreads = omp_get_max_threads();
   // colleagues through a per-level hash of (i,j,k)
   std::vector<std::unordered_map<uint64_t, int>> H(t.nlev + 1);
   for (int l = 0; l <= t.nlev; l++) {
@@ -428,6 +314,153 @@ int build_octree(const double* src, int ns, const double* trg, int nt, int nbox,
   flatten(l2, t.list[1]);
   flatten(l3, t.list[2]);
   flatten(l4, t.list[3]);
+}
+
+// ---------------------------------------------------------------------------
+// Host builder: the points live in host memory
+// ---------------------------------------------------------------------------
+int build_octree(const double* src, int ns, const double* trg, int nt, int nbox, Octree& t) {
+  t.reset();
+  if (ns < 1) return 128;
+  if (nt < 0) nt = 0;
+  double lo[3] = {src[0], src[1], src[2]}, hi[3] = {src[0], src[1], src[2]};
+  auto scan = [&](const double* p, int n) {
+    double l0 = lo[0], l1 = lo[1], l2 = lo[2], h0 = hi[0], h1 = hi[1], h2 = hi[2];
+#pragma omp parallel for reduction(min : l0, l1, l2) reduction(max : h0, h1, h2)
+    for (int i = 0; i < n; i++) {
+      l0 = std::min(l0, p[3 * i]); h0 = std::max(h0, p[3 * i]);
+      l1 = std::min(l1, p[3 * i + 1]); h1 = std::max(h1, p[3 * i + 1]);
+      l2 = std::min(l2, p[3 * i + 2]); h2 = std::max(h2, p[3 * i + 2]);
+    }
+    lo[0] = l0; lo[1] = l1; lo[2] = l2; hi[0] = h0; hi[1] = h1; hi[2] = h2;
+  };
+  scan(src, ns);
+  if (nt > 0) scan(trg, nt);
+  std::vector<int> frontier;
+  tree_root(t, lo, hi, ns, nt, nbox, frontier);
+
+  // workspace: positions with their original index
+  Work& WS = g_src;
+  Work& WT = g_trg;
+  if ((int)WS.a.size() < ns) { WS.a.resize(ns); WS.b.resize(ns); }
+  if ((int)WT.a.size() < nt) { WT.a.resize(nt); WT.b.resize(nt); }
+  Pt* SA = WS.a.data();
+  Pt* SB = WS.b.data();
+  Pt* TA = WT.a.data();
+  Pt* TB = WT.b.data();
+#pragma omp parallel for schedule(static)
+  for (int i = 0; i < ns; i++) SA[i] = Pt{src[3 * i], src[3 * i + 1], src[3 * i + 2], i, 0u};
+#pragma omp parallel for schedule(static)
+  for (int i = 0; i < nt; i++) TA[i] = Pt{trg[3 * i], trg[3 * i + 1], trg[3 * i + 2], i, 0u};
+
+  const int nthreads = omp_get_max_threads();
+  TreePass ps;
+  std::vector<int> next;
+  int L = 0;  // level of the frontier boxes
+  // Each pass resolves up to D levels below the frontier at once: every point
+  // gets the D octant digits of its path (the same centre arithmetic and
+  // strict ">" test as one split per level), a histogram of the digit codes
+  // gives every sub-box's counts and hence which sub-boxes split, and one
+  // stable counting sort by the code truncated at the point's leaf puts the
+  // points in the order D successive stable 8-way partitions would.
+  while (!frontier.empty() && L < kTreeMaxLevel) {
+    ps.frontier = frontier;
+    tree_pass_setup(t, L, ps);
+    const int nf = (int)frontier.size();
+    const int D = ps.D, NB = ps.NB, CT = ps.CT;
+    auto code_of = [&](const Pt& p, int f) -> uint32_t {
+      const double* cx = &ps.ctab[(((size_t)f * 3 + 0) * D) * CT];
+      const double* cy = &ps.ctab[(((size_t)f * 3 + 1) * D) * CT];
+      const double* cz = &ps.ctab[(((size_t)f * 3 + 2) * D) * CT];
+      uint32_t code = 0;
+      int ii = 0, jj = 0, kk = 0;
+      for (int d = 0; d < D; d++) {
+        const int o = (p.x > cx[d * CT + ii]) | ((p.y > cy[d * CT + jj]) << 1) | ((p.z > cz[d * CT + kk]) << 2);
+        code = (code << 3) | (uint32_t)o;
+        ii = 2 * ii + (o & 1);
+        jj = 2 * jj + ((o >> 1) & 1);
+        kk = 2 * kk + ((o >> 2) & 1);
+      }
+      return code;
+    };
+    // digit codes and per-box histograms (exclusive prefix sums)
+    auto codes_and_hist = [&](Pt* P, int lo, int n, int f, int* h, bool par) {
+      if (par) {
+        std::vector<int> part((size_t)nthreads * NB, 0);
+#pragma omp parallel num_threads(nthreads)
+        {
+          const int th = omp_get_thread_num();
+          const int c0 = lo + (int)((long)n * th / nthreads), c1 = lo + (int)((long)n * (th + 1) / nthreads);
+          int* pt = &part[(size_t)th * NB];
+          for (int i = c0; i < c1; i++) {
+            P[i].code = code_of(P[i], f);
+            pt[P[i].code]++;
+          }
+        }
+        for (int th = 0; th < nthreads; th++)
+          for (int k = 0; k < NB; k++) h[k + 1] += part[(size_t)th * NB + k];
+      } else {
+        for (int i = lo; i < lo + n; i++) {
+          P[i].code = code_of(P[i], f);
+          h[P[i].code + 1]++;
+        }
+      }
+      for (int k = 0; k < NB; k++) h[k + 1] += h[k];
+    };
+    const bool few = nf < 2 * nthreads;
+    if (few) {
+      for (int f = 0; f < nf; f++) {
+        const int b = frontier[f];
+        codes_and_hist(SA, t.isrc[b], t.nsrc[b], f, &ps.hs[(size_t)f * (NB + 1)], t.nsrc[b] > (1 << 15));
+        codes_and_hist(TA, t.itrg[b], t.ntrg[b], f, &ps.ht[(size_t)f * (NB + 1)], t.ntrg[b] > (1 << 15));
+      }
+    } else {
+#pragma omp parallel for schedule(dynamic, 1)
+      for (int f = 0; f < nf; f++) {
+        const int b = frontier[f];
+        codes_and_hist(SA, t.isrc[b], t.nsrc[b], f, &ps.hs[(size_t)f * (NB + 1)], false);
+        codes_and_hist(TA, t.itrg[b], t.ntrg[b], f, &ps.ht[(size_t)f * (NB + 1)], false);
+      }
+    }
+    tree_pass_expand(t, nbox, ps, next);
+    // stable sort of each frontier box's points by the code truncated at
+    // the depth where the point's box stops splitting
+    const int NBk = 1 << (3 * ps.Dk);
+    auto sort_box = [&](Pt* A, Pt* Bf, int lo, int n, int f, int T) {
+      stable_bin_sort(A, Bf, lo, n, NBk, T, [&](const Pt& p) { return (int)tree_eff_code(ps, f, p.code); });
+      std::memcpy(A + lo, Bf + lo, sizeof(Pt) * (size_t)n);
+    };
+    if (nf == 1 && frontier[0] == 0) {
+      // the root: every point moves, so sort into the second buffer and swap
+      stable_bin_sort(SA, SB, 0, ns, NBk, nthreads, [&](const Pt& p) { return (int)tree_eff_code(ps, 0, p.code); });
+      stable_bin_sort(TA, TB, 0, nt, NBk, nthreads, [&](const Pt& p) { return (int)tree_eff_code(ps, 0, p.code); });
+      std::swap(SA, SB);
+      std::swap(TA, TB);
+    } else if (few) {
+      for (int f = 0; f < nf; f++) {
+        const int b = frontier[f];
+        sort_box(SA, SB, t.isrc[b], t.nsrc[b], f, nthreads);
+        sort_box(TA, TB, t.itrg[b], t.ntrg[b], f, nthreads);
+      }
+    } else {
+#pragma omp parallel for schedule(dynamic, 1)
+      for (int f = 0; f < nf; f++) {
+        const int b = frontier[f];
+        sort_box(SA, SB, t.isrc[b], t.nsrc[b], f, 1);
+        sort_box(TA, TB, t.itrg[b], t.ntrg[b], f, 1);
+      }
+    }
+    frontier.swap(next);
+    L += ps.Dk;
+  }
+  t.nboxes = (int)t.level.size();
+  t.src_perm.resize(ns);
+  t.trg_perm.resize(nt);
+#pragma omp parallel for schedule(static)
+  for (int i = 0; i < ns; i++) t.src_perm[i] = SA[i].idx;
+#pragma omp parallel for schedule(static)
+  for (int i = 0; i < nt; i++) t.trg_perm[i] = TA[i].idx;
+  tree_lists(t);
   return 0;
 }
 

@@ -5,6 +5,7 @@
 // (lists 1-4).  Built level-synchronously with OpenMP and flattened to CSR so
 // the device receives it in one upload.
 #pragma once
+#include <cstddef>
 #include <cstdint>
 #include <vector>
 
@@ -17,7 +18,7 @@ struct Csr {
 };
 
 struct Octree {
-  int nboxes = 0, nlev = 0;
+  int nboxes = 0, nlev = 0, ns = 0, nt = 0;
   double center[3] = {0, 0, 0}, size = 0;
   // per box
   std::vector<int> level, parent, nchild, child;  // child: 8 per box, -1 unused
@@ -37,7 +38,44 @@ struct Octree {
 };
 
 // Returns 0 on success, 128 when ns < 1 (the value d3tstrcr reports,
-// d3tstrcr.f:141-145).  Coordinates are (n,3) row-major doubles.
+// d3tstrcr.f:141-145).  Coordinates are (n,3) row-major doubles in host memory.
 int build_octree(const double* src, int ns, const double* trg, int nt, int nbox, Octree& t);
+
+// ---- pieces of the level-synchronous build shared with the device builder
+constexpr int kTreeMaxLevel = 100;
+
+// One pass over the points resolves up to D levels below the frontier boxes.
+struct TreePass {
+  std::vector<int> frontier;  // boxes split in this pass
+  int L = 0;                  // their level
+  int D = 1, Dfull = 1, Dk = 1;  // digits per code, digits resolved
+  int NB = 8, CT = 4, NS = 16;
+  std::vector<double> ctab;   // centres: [f][dim][depth][index], width CT
+  std::vector<int> hs, ht;    // per frontier box: NB+1 exclusive prefix counts of the codes
+  std::vector<char> splits;   // per frontier box: NS flags of the sub-boxes that split further
+  int posd[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+};
+
+// root box from the bounding box of all points; frontier = {0} if it splits
+void tree_root(Octree& t, const double lo[3], const double hi[3], int ns, int nt, int nbox,
+               std::vector<int>& frontier);
+// digits, centre table and empty histograms for ps.frontier at level L
+void tree_pass_setup(const Octree& t, int L, TreePass& ps);
+// with ps.hs/ps.ht filled: create the sub-boxes, mark the splits, return the
+// boxes to split in the next pass in `next`; sets ps.Dk
+void tree_pass_expand(Octree& t, int nbox, TreePass& ps, std::vector<int>& next);
+// code of a point truncated at the depth where its box stops splitting
+inline uint32_t tree_eff_code(const TreePass& ps, int f, uint32_t code) {
+  const int Dk = ps.Dk;
+  const uint32_t c = code >> (3 * (ps.Dfull - Dk));
+  const char* sp = &ps.splits[(size_t)f * ps.NS];
+  for (int d = 1; d < Dk; d++) {
+    const uint32_t pre = c >> (3 * (Dk - d));
+    if (!sp[ps.posd[d] + pre]) return pre << (3 * (Dk - d));
+  }
+  return c;
+}
+// colleagues and lists 1-4 from the finished box structure
+void tree_lists(Octree& t);
 
 }  // namespace fmm3d

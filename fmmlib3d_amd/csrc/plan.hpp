@@ -160,7 +160,7 @@ inline void build_plan(const Octree& t, const Flags& f, Flat& F, PlanOffsets& O,
 inline DevPlan make_devplan(const Octree& t, int p, const PlanOffsets& O, const int* I, const double* D) {
   DevPlan P;
   P.nboxes = t.nboxes; P.nlev = t.nlev; P.p = p; P.K = (p + 1) * (p + 1);
-  P.ns = (int)t.src_perm.size(); P.nt = (int)t.trg_perm.size();
+  P.ns = t.ns; P.nt = t.nt;
   P.level = I + O.level; P.isrc = I + O.isrc; P.nsrc = I + O.nsrc; P.itrg = I + O.itrg;
   P.ntrg = I + O.ntrg; P.nchild = I + O.nchild; P.child = I + O.child; P.ijk = I + O.ijk;
   P.center = D + O.center; P.inv_side = D + O.inv_side;

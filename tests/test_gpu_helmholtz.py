@@ -124,6 +124,20 @@ def test_gpu_helm_error_codes():
     assert F.hfmm3dparttarg(1, 1.0, src[:0], q[:0])[0] == 4  # no sources
 
 
+def test_gpu_helm_device_tree_equals_host_tree(rng, monkeypatch):
+    """Device and host octree builders give bitwise equal Helmholtz results."""
+    src = points(rng, 30000, "clusters")
+    trg = points(rng, 6000, "clusters")
+    q, ds, dv = strengths(rng, 30000, True, True)
+    monkeypatch.setenv("FMM3D_TREE", "host")
+    a = F.hfmm3dparttarg(2, 6.0 + 0.2j, src, q, ds, dv, trg)
+    monkeypatch.delenv("FMM3D_TREE")
+    b = F.hfmm3dparttarg(2, 6.0 + 0.2j, src, q, ds, dv, trg)
+    assert a[0] == b[0] == 0
+    for x, y in zip(a[1:], b[1:]):
+        assert np.array_equal(x, y)
+
+
 def test_gpu_helm_deterministic(rng):
     src = points(rng, 20000, "sphere")
     q, _, _ = strengths(rng, 20000, True, False)

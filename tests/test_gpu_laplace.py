@@ -165,6 +165,35 @@ def test_gpu_reduced_precision_within_tolerance(rng, mode):
         assert rel(g[4][idx], fd) < 10 * tol
 
 
+TREE_CASES = [
+    # dist, ns, nt, iprec: one pass (uniform), several passes (clustered,
+    # Plummer), no targets, a root leaf, targets far from the sources
+    ("cube", 30000, 20000, 2), ("clusters", 40000, 0, 1), ("plummer", 30000, 10000, 3),
+    ("sphere", 50000, 5000, 0), ("cube", 100, 50, 1), ("shell", 20000, 20000, -1), ("plane", 20000, 3000, 1),
+]
+
+
+@pytest.mark.parametrize("dist,ns,nt,iprec", TREE_CASES)
+def test_gpu_device_tree_equals_host_tree(rng, monkeypatch, dist, ns, nt, iprec):
+    """The device octree builder yields the host builder's tree and
+    permutation: the FMM results are bitwise equal (FMM3D_TREE=host selects
+    the host builder)."""
+    print("tree case", dist, ns, nt, iprec, flush=True)
+    src = points(rng, ns, dist)
+    trg = points(rng, nt, dist) + (0.5 if dist == "shell" else 0.0) if nt else None
+    q, ds, dv = strengths(rng, ns, True, True)
+    monkeypatch.setenv("FMM3D_TREE", "host")
+    a = F.lfmm3dparttarg(iprec, src, q, ds, dv, trg)
+    monkeypatch.delenv("FMM3D_TREE")
+    b = F.lfmm3dparttarg(iprec, src, q, ds, dv, trg)
+    assert a[0] == b[0] == 0
+    for x, y in zip(a[1:], b[1:]):
+        if y is None:
+            assert x is None
+        else:
+            assert np.array_equal(x, y)
+
+
 def _hip():
     """The HIP runtime the library itself uses (no torch in this process)."""
     import ctypes
