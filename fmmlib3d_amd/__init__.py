@@ -270,9 +270,10 @@ def d3tstrcr(source, target, nbox):
 def last_timings():
     """Per-stage milliseconds of the last call on this thread (see
     fmm3d_last_timings in include/fmm3d_mi355x.h)."""
-    out = np.zeros(11)
-    load().fmm3d_last_timings(_ptr(out), 11)
-    keys = ("tree_plan", "upload", "p2m", "p2l", "m2m", "m2l", "l2l", "l2p", "m2p", "p2p", "total")
+    out = np.zeros(13)
+    load().fmm3d_last_timings(_ptr(out), 13)
+    keys = ("tree_plan", "upload", "p2m", "p2l", "m2m", "m2l", "l2l", "l2p", "m2p", "p2p", "total", "tree",
+            "lists")
     return dict(zip(keys, out.tolist()))
 
 

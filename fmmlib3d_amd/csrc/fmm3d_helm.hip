@@ -156,7 +156,7 @@ int helm_solve(Context& ctx, const LapCall& c) {
   double* dD = (double*)ctx.arena.get("plan_d", F.D.size() * sizeof(double));
   FMM_HIP(hipMemcpyAsync(dI, F.I.data(), F.I.size() * sizeof(int), hipMemcpyHostToDevice, s));
   FMM_HIP(hipMemcpyAsync(dD, F.D.data(), F.D.size() * sizeof(double), hipMemcpyHostToDevice, s));
-  DevPlan P = make_devplan(tree, pmax, O, dI, dD);
+  DevPlan P = make_devplan(tree, pmax, O, dI, dD, dI);
   const int* ti = (const int*)ctx.arena.get("htab_i", HC->I.size() * sizeof(int));
   const double* td = (const double*)ctx.arena.get("htab_d", HC->D.size() * sizeof(double));
   const long* tl = (const long*)ctx.arena.get("htab_l", HC->Lg.size() * sizeof(long));

@@ -8,6 +8,7 @@
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
+#include <string>
 #include <type_traits>
 #include <vector>
 
@@ -144,28 +145,42 @@ int lap_solve(Context& ctx, const LapCall& c) {
   thread_local Octree tree;  // storage reused across calls
   const int* dsperm = nullptr;
   const int* dtperm = nullptr;
+  // the interaction lists are built after the first kernels are queued
   if (device_tree_enabled()) {
-    if (build_octree_device(ctx, ctx.stream, c.src_dev, ns, c.trg_dev, nt, nbox, tree, &dsperm, &dtperm) != 0) return 4;
-  } else if (build_octree(c.src_host, ns, c.trg_host, nt, nbox, tree) != 0) {
+    if (build_octree_device(ctx, ctx.stream, c.src_dev, ns, c.trg_dev, nt, nbox, tree, &dsperm, &dtperm, false) != 0)
+      return 4;
+  } else if (build_octree(c.src_host, ns, c.trg_host, nt, nbox, tree, false) != 0) {
     return 4;
   }
+  auto t_tree = std::chrono::steady_clock::now();
   const int p = laplace_nterms(eps);
   const int K = (p + 1) * (p + 1);
   if (sizeof(T) * 4 * (size_t)K > max_lds_bytes(ctx)) return 16;
-  Flat F;
+  thread_local Flat FA, FB;  // plan parts A (boxes) and B (lists), storage reused
+  FA.I.clear(); FA.D.clear(); FB.I.clear(); FB.D.clear();
   PlanOffsets O;
-  build_plan(tree, f, F, O, 0);
+  build_plan_boxes(tree, f, FA, O, 0);
   auto t1 = std::chrono::steady_clock::now();
   hipStream_t s = ctx.stream;
   Timer tm;
   tm.start(s, 11);
-  // ---- device buffers
-  int* dI = (int*)ctx.arena.get("plan_i", F.I.size() * sizeof(int));
-  double* dD = (double*)ctx.arena.get("plan_d", F.D.size() * sizeof(double));
-  FMM_HIP(hipMemcpyAsync(dI, F.I.data(), F.I.size() * sizeof(int), hipMemcpyHostToDevice, s));
-  FMM_HIP(hipMemcpyAsync(dD, F.D.data(), F.D.size() * sizeof(double), hipMemcpyHostToDevice, s));
+  // ---- device buffers; plan uploads go through pinned staging buffers so
+  // that they do not wait for the kernels already queued
+  auto upload_ints = [&](const char* name, const std::vector<int>& v) {
+    int* h = (int*)ctx.arena.get_host(std::string(name) + "_h", v.size() * sizeof(int));
+    std::memcpy(h, v.data(), v.size() * sizeof(int));
+    int* d = (int*)ctx.arena.get(name, v.size() * sizeof(int));
+    FMM_HIP(hipMemcpyAsync(d, h, v.size() * sizeof(int), hipMemcpyHostToDevice, s));
+    return d;
+  };
+  int* dI = upload_ints("plan_i", FA.I);
+  double* hD = (double*)ctx.arena.get_host("plan_d_h", FA.D.size() * sizeof(double));
+  std::memcpy(hD, FA.D.data(), FA.D.size() * sizeof(double));
+  double* dD = (double*)ctx.arena.get("plan_d", FA.D.size() * sizeof(double));
+  FMM_HIP(hipMemcpyAsync(dD, hD, FA.D.size() * sizeof(double), hipMemcpyHostToDevice, s));
   DevTables Tb = ensure_tables(ctx, p, eps);
   DevPlan P = make_devplan(tree, p, O, dI, dD);
+  // expansion storage for every box: allocated before any kernel is queued
   const size_t nb = tree.nboxes;
   constexpr bool near32 = sizeof(TN) < sizeof(double);
   double* spos = (double*)ctx.arena.get("spos", sizeof(double) * 4 * (size_t)ns);
@@ -229,12 +244,22 @@ int lap_solve(Context& ctx, const LapCall& c) {
   };
   form(std::false_type{}, P.n_p2m, mp);
   tm.mark(2);
-  form(std::true_type{}, P.n_p2l, lo);
-  tm.mark(3);
   // ---- STEP 3: M2M, bottom-up
   const size_t lds_ps = sizeof(T) * 4 * K;
   for (auto& g : O.m2m_groups)
     hipLaunchKernelGGL((k_m2m<T>), dim3(g.second), dim3(256), lds_ps, s, P, Tb, g.first, mp);
+  tm.mark(3);
+  // ---- while the device runs P2M and M2M: lists and plan part B
+  auto t_lists0 = std::chrono::steady_clock::now();
+  tree_lists(tree);
+  build_plan_lists(tree, f, FB, O);
+  const int* dIB = upload_ints("plan_ib", FB.I);  // part B: offsets in O from build_plan_lists
+  devplan_lists(P, O, dIB);
+  // the P2P work items are addressed explicitly: they must lie in part B
+  if (O.p2p0 + 3 * (size_t)O.n_p2p0 > FB.I.size() || O.p2p1 + 3 * (size_t)O.n_p2p1 > FB.I.size())
+    throw HipError("plan: P2P work items outside the list part of the plan");
+  auto t_lists1 = std::chrono::steady_clock::now();
+  form(std::true_type{}, P.n_p2l, lo);
   tm.mark(4);
   // ---- STEP 4: M2L
   if (P.n_m2l) {
@@ -308,9 +333,9 @@ int lap_solve(Context& ctx, const LapCall& c) {
   tm.mark(8);
   // ---- STEP 8: P2P (self + list 1)
   if (f.ifcharge || f.ifdipole) {
-    launch_p2p<TN>(P, dI + O.p2p0, O.n_p2p0, 0, spos, strn, tpos, f.ifcharge, f.ifdipole,
+    launch_p2p<TN>(P, dIB + O.p2p0, O.n_p2p0, 0, spos, strn, tpos, f.ifcharge, f.ifdipole,
                    f.iffld != 0, pot_s, fld_s, s);
-    launch_p2p<TN>(P, dI + O.p2p1, O.n_p2p1, 1, spos, strn, tpos, f.ifcharge, f.ifdipole,
+    launch_p2p<TN>(P, dIB + O.p2p1, O.n_p2p1, 1, spos, strn, tpos, f.ifcharge, f.ifdipole,
                    f.iffldtarg != 0, pot_t, fld_t, s);
   }
   tm.mark(9);
@@ -323,13 +348,18 @@ int lap_solve(Context& ctx, const LapCall& c) {
   FMM_HIP(hipGetLastError());
   FMM_HIP(hipStreamSynchronize(s));
   auto& lt = last_timings();
-  lt.assign(11, 0.0);
-  lt[0] = std::chrono::duration<double, std::milli>(t1 - t0).count();
-  auto el = tm.elapsed();
-  for (int i = 0; i < 9 && i < (int)el.size(); i++) lt[1 + i] = el[i];
+  lt.assign(13, 0.0);
+  auto ms = [](auto a, auto b) { return std::chrono::duration<double, std::milli>(b - a).count(); };
+  lt[0] = ms(t0, t1);
+  auto el = tm.elapsed();  // upload, P2M, M2M, P2L, M2L, L2L, L2P, M2P, P2P, scatter
+  el.resize(10, 0.0);
+  lt[1] = el[0]; lt[2] = el[1]; lt[3] = el[3]; lt[4] = el[2]; lt[5] = el[4];
+  lt[6] = el[5]; lt[7] = el[6]; lt[8] = el[7]; lt[9] = el[8];
   double tot = 0;
   for (auto v : el) tot += v;
   lt[10] = tot + lt[0];
+  lt[11] = ms(t0, t_tree);
+  lt[12] = ms(t_lists0, t_lists1);
   return 0;
 }
 

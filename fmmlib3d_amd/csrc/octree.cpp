@@ -319,7 +319,7 @@ void tree_lists(Octree& t) {
 // ---------------------------------------------------------------------------
 // Host builder: the points live in host memory
 // ---------------------------------------------------------------------------
-int build_octree(const double* src, int ns, const double* trg, int nt, int nbox, Octree& t) {
+int build_octree(const double* src, int ns, const double* trg, int nt, int nbox, Octree& t, bool lists) {
   t.reset();
   if (ns < 1) return 128;
   if (nt < 0) nt = 0;
@@ -460,7 +460,7 @@ int build_octree(const double* src, int ns, const double* trg, int nt, int nbox,
   for (int i = 0; i < ns; i++) t.src_perm[i] = SA[i].idx;
 #pragma omp parallel for schedule(static)
   for (int i = 0; i < nt; i++) t.trg_perm[i] = TA[i].idx;
-  tree_lists(t);
+  if (lists) tree_lists(t);
   return 0;
 }
 

@@ -39,7 +39,8 @@ struct Octree {
 
 // Returns 0 on success, 128 when ns < 1 (the value d3tstrcr reports,
 // d3tstrcr.f:141-145).  Coordinates are (n,3) row-major doubles in host memory.
-int build_octree(const double* src, int ns, const double* trg, int nt, int nbox, Octree& t);
+// With lists = false the interaction lists are left for tree_lists.
+int build_octree(const double* src, int ns, const double* trg, int nt, int nbox, Octree& t, bool lists = true);
 
 // ---- pieces of the level-synchronous build shared with the device builder
 constexpr int kTreeMaxLevel = 100;

@@ -55,19 +55,52 @@ struct Flags {
   int ifcharge, ifdipole, ifpot, iffld, ifpottarg, iffldtarg;
 };
 
-// p2m_min_level: leaves above this level get no multipole (2 for Helmholtz,
-// src/hfmm3dpart.f:681 forms them from level 2 down only)
-inline void build_plan(const Octree& t, const Flags& f, Flat& F, PlanOffsets& O, int p2m_min_level) {
+// Part A of the plan: the boxes, the P2M list and the M2M groups, all that
+// the first FMM steps need.  p2m_min_level: leaves above this level get no
+// multipole (2 for Helmholtz, src/hfmm3dpart.f:681 forms them from level 2
+// down only).  Needs the tree's boxes only, not its lists.
+inline void build_plan_boxes(const Octree& t, const Flags& f, Flat& F, PlanOffsets& O, int p2m_min_level) {
+  (void)f;
   const int nb = t.nboxes;
-  const bool want_src = f.ifpot || f.iffld, want_trg = f.ifpottarg || f.iffldtarg;
   O.level = F.i(t.level); O.isrc = F.i(t.isrc); O.nsrc = F.i(t.nsrc);
   O.itrg = F.i(t.itrg); O.ntrg = F.i(t.ntrg); O.nchild = F.i(t.nchild);
   O.child = F.i(t.child); O.ijk = F.i(t.ijk);
+  std::vector<int> p2m;
+  for (int b = 0; b < nb; b++)
+    if (t.nchild[b] == 0 && t.nsrc[b] > 0 && t.level[b] >= p2m_min_level) p2m.push_back(b);
+  O.p2m = F.i(p2m); O.n_p2m = (int)p2m.size();
+  // M2M parents, levels nlev-1 .. 2 (lfmm3drouts.f:195-258)
+  std::vector<int> m2m;
+  O.m2m_groups.clear();
+  O.m2m_group_level.clear();
+  for (int l = t.nlev - 1; l >= 2; l--) {
+    int first = (int)m2m.size();
+    for (int b = t.ladd[l]; b < t.ladd[l + 1]; b++)
+      if (t.nchild[b] > 0 && t.nsrc[b] > 0) m2m.push_back(b);
+    if ((int)m2m.size() > first) {
+      O.m2m_groups.push_back({first, (int)m2m.size() - first});
+      O.m2m_group_level.push_back(l);
+    }
+  }
+  O.m2m = F.i(m2m);
+  O.sperm = F.i(t.src_perm);
+  O.tperm = F.i(t.trg_perm);
+  O.center = F.d(t.bcenter);
+  std::vector<double> inv(t.nlev + 1);
+  for (int l = 0; l <= t.nlev; l++) inv[l] = 1.0 / t.side(l);
+  O.inv_side = F.d(inv);
+}
+
+// Part B: lists and the work lists of the remaining steps (needs tree_lists).
+inline void build_plan_lists(const Octree& t, const Flags& f, Flat& F, PlanOffsets& O) {
+  const int nb = t.nboxes;
+  const bool want_src = f.ifpot || f.iffld, want_trg = f.ifpottarg || f.iffldtarg;
   O.l1o = F.i(t.list[0].off); O.l1i = F.i(t.list[0].idx);
   // list 2 pruned as lfmm3drouts.f:385-387: sourceless sources, and
   // targetless targets when no source outputs are wanted
   {
     std::vector<int> off(nb + 1, 0), idx;
+    idx.reserve(t.list[1].idx.size());
     for (int b = 0; b < nb; b++) {
       off[b] = (int)idx.size();
       bool skip_b = t.level[b] < 2 || (!want_src && t.ntrg[b] == 0);
@@ -86,11 +119,10 @@ inline void build_plan(const Octree& t, const Flags& f, Flat& F, PlanOffsets& O,
   }
   O.l3o = F.i(t.list[2].off); O.l3i = F.i(t.list[2].idx);
   O.l4o = F.i(t.list[3].off); O.l4i = F.i(t.list[3].idx);
-  std::vector<int> p2m, l2p, m2p, p2l, p2p0, p2p1;
+  std::vector<int> l2p, m2p, p2l, p2p0, p2p1;
   for (int b = 0; b < nb; b++) {
     const bool leaf = t.nchild[b] == 0;
     const bool has_pts = (want_src && t.nsrc[b] > 0) || (want_trg && t.ntrg[b] > 0);
-    if (leaf && t.nsrc[b] > 0 && t.level[b] >= p2m_min_level) p2m.push_back(b);
     if (leaf && t.level[b] >= 2 && has_pts) l2p.push_back(b);
     if (t.list[2].count(b) > 0 && has_pts) m2p.push_back(b);
     if (t.list[3].count(b) > 0) {
@@ -103,7 +135,6 @@ inline void build_plan(const Octree& t, const Flags& f, Flat& F, PlanOffsets& O,
     if (leaf && want_trg)
       for (int s = 0; s < t.ntrg[b]; s += kP2PBlock) { p2p1.push_back(b); p2p1.push_back(1); p2p1.push_back(s); }
   }
-  O.p2m = F.i(p2m); O.n_p2m = (int)p2m.size();
   O.l2p = F.i(l2p); O.n_l2p = (int)l2p.size();
   {
     // L2P work items (position in the L2P list, kind, first point): one
@@ -122,22 +153,10 @@ inline void build_plan(const Octree& t, const Flags& f, Flat& F, PlanOffsets& O,
   O.p2l = F.i(p2l); O.n_p2l = (int)p2l.size();
   O.p2p0 = F.i(p2p0); O.n_p2p0 = (int)p2p0.size() / 3;
   O.p2p1 = F.i(p2p1); O.n_p2p1 = (int)p2p1.size() / 3;
-  // M2M parents, levels nlev-1 .. 2 (lfmm3drouts.f:195-258); L2L parents,
-  // levels 2 .. nlev-1 (lfmm3drouts.f:498-558)
-  std::vector<int> m2m, l2l;
-  O.m2m_groups.clear();
+  // L2L parents, levels 2 .. nlev-1 (lfmm3drouts.f:498-558)
+  std::vector<int> l2l;
   O.l2l_groups.clear();
-  O.m2m_group_level.clear();
   O.l2l_group_level.clear();
-  for (int l = t.nlev - 1; l >= 2; l--) {
-    int first = (int)m2m.size();
-    for (int b = t.ladd[l]; b < t.ladd[l + 1]; b++)
-      if (t.nchild[b] > 0 && t.nsrc[b] > 0) m2m.push_back(b);
-    if ((int)m2m.size() > first) {
-      O.m2m_groups.push_back({first, (int)m2m.size() - first});
-      O.m2m_group_level.push_back(l);
-    }
-  }
   for (int l = 2; l <= t.nlev - 1; l++) {
     int first = (int)l2l.size();
     for (int b = t.ladd[l]; b < t.ladd[l + 1]; b++)
@@ -147,35 +166,42 @@ inline void build_plan(const Octree& t, const Flags& f, Flat& F, PlanOffsets& O,
       O.l2l_group_level.push_back(l);
     }
   }
-  O.m2m = F.i(m2m);
   O.l2l = F.i(l2l);
-  O.sperm = F.i(t.src_perm);
-  O.tperm = F.i(t.trg_perm);
-  O.center = F.d(t.bcenter);
-  std::vector<double> inv(t.nlev + 1);
-  for (int l = 0; l <= t.nlev; l++) inv[l] = 1.0 / t.side(l);
-  O.inv_side = F.d(inv);
 }
 
-inline DevPlan make_devplan(const Octree& t, int p, const PlanOffsets& O, const int* I, const double* D) {
-  DevPlan P;
-  P.nboxes = t.nboxes; P.nlev = t.nlev; P.p = p; P.K = (p + 1) * (p + 1);
-  P.ns = t.ns; P.nt = t.nt;
-  P.level = I + O.level; P.isrc = I + O.isrc; P.nsrc = I + O.nsrc; P.itrg = I + O.itrg;
-  P.ntrg = I + O.ntrg; P.nchild = I + O.nchild; P.child = I + O.child; P.ijk = I + O.ijk;
-  P.center = D + O.center; P.inv_side = D + O.inv_side;
+// Both parts into one flat buffer.
+inline void build_plan(const Octree& t, const Flags& f, Flat& F, PlanOffsets& O, int p2m_min_level) {
+  build_plan_boxes(t, f, F, O, p2m_min_level);
+  build_plan_lists(t, f, F, O);
+}
+
+// Device plan over part A at (IA, DA) and part B at IB (IB = IA when one
+// flat holds both; IB may be null until part B is uploaded, see
+// devplan_lists).
+inline void devplan_lists(DevPlan& P, const PlanOffsets& O, const int* I) {
   P.l1_off = I + O.l1o; P.l1_idx = I + O.l1i; P.l2_off = I + O.l2o; P.l2_idx = I + O.l2i;
   P.l3_off = I + O.l3o; P.l3_idx = I + O.l3i; P.l4_off = I + O.l4o; P.l4_idx = I + O.l4i;
-  P.p2m_boxes = I + O.p2m; P.n_p2m = O.n_p2m;
   P.l2p_boxes = I + O.l2p; P.n_l2p = O.n_l2p;
   P.l2p_items = I + O.l2pi; P.n_l2pi = O.n_l2pi;
   P.m2p_boxes = I + O.m2p; P.n_m2p = O.n_m2p;
   P.p2l_boxes = I + O.p2l; P.n_p2l = O.n_p2l;
   P.m2l_boxes = I + O.m2l; P.n_m2l = O.n_m2l;
-  P.m2m_boxes = I + O.m2m; P.m2m_lev_off = nullptr;
   P.l2l_boxes = I + O.l2l; P.l2l_lev_off = nullptr;
   P.p2p_items = I + O.p2p0; P.n_p2p = O.n_p2p0;
+}
+
+inline DevPlan make_devplan(const Octree& t, int p, const PlanOffsets& O, const int* I, const double* D,
+                            const int* IB = nullptr) {
+  DevPlan P{};
+  P.nboxes = t.nboxes; P.nlev = t.nlev; P.p = p; P.K = (p + 1) * (p + 1);
+  P.ns = t.ns; P.nt = t.nt;
+  P.level = I + O.level; P.isrc = I + O.isrc; P.nsrc = I + O.nsrc; P.itrg = I + O.itrg;
+  P.ntrg = I + O.ntrg; P.nchild = I + O.nchild; P.child = I + O.child; P.ijk = I + O.ijk;
+  P.center = D + O.center; P.inv_side = D + O.inv_side;
+  P.p2m_boxes = I + O.p2m; P.n_p2m = O.n_p2m;
+  P.m2m_boxes = I + O.m2m; P.m2m_lev_off = nullptr;
   P.src_perm = I + O.sperm; P.trg_perm = I + O.tperm;
+  if (IB) devplan_lists(P, O, IB);
   return P;
 }
 

@@ -132,7 +132,7 @@ bool device_tree_enabled() {
 }
 
 int build_octree_device(Context& ctx, hipStream_t s, const double* src, int ns, const double* trg, int nt,
-                        int nbox, Octree& t, const int** sperm, const int** tperm) {
+                        int nbox, Octree& t, const int** sperm, const int** tperm, bool lists) {
   t.reset();
   if (ns < 1) return 128;
   if (nt < 0) nt = 0;
@@ -253,7 +253,7 @@ int build_octree_device(Context& ctx, hipStream_t s, const double* src, int ns, 
   FMM_HIP(hipGetLastError());
   *sperm = P[0].idx;
   *tperm = P[1].idx;
-  tree_lists(t);
+  if (lists) tree_lists(t);
   return 0;
 }
 

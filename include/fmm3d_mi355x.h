@@ -163,8 +163,11 @@ void fmm3d_server_stats(void *server, long long *queued, int *running, long long
 void fmm3d_server_destroy(void *server);
 
 /* Per-stage timings (milliseconds, device time from HIP events) of the last
- * FMM call on this thread: [0] tree+plan (host), [1] upload, [2] P2M,
- * [3] P2L, [4] M2M, [5] M2L, [6] L2L, [7] L2P, [8] M2P, [9] P2P, [10] total.
+ * FMM call on this thread: [0] host time before the first kernel (tree and
+ * box plan), [1] gather/upload, [2] P2M, [3] P2L, [4] M2M, [5] M2L, [6] L2L,
+ * [7] L2P, [8] M2P, [9] P2P, [10] total, [11] tree build (within [0]),
+ * [12] host time of the interaction lists and their plan (Laplace: overlapped
+ * with P2M and M2M on the device).
  * Mirrors the reference's timeinfo(1..8) (src/lfmm3dpart.f:1155). */
 void fmm3d_last_timings(double *ms, int n);
 
