@@ -1214,6 +1214,8 @@ __global__ void k_m2l_fact(DevPlan P, DevTables Tb, const T* __restrict__ mp, T*
 template <typename TN, bool CHG, bool DIP, bool FLD>
 struct PairAcc {
   TN pr = 0, pi = 0, fxr = 0, fxi = 0, fyr = 0, fyi = 0, fzr = 0, fzi = 0;
+  // pot += q/r + (v.d)/r^3 = (q + (v.d)/r^2)/r;
+  // fld += (q + 3 (v.d)/r^2) d/r^3 - v/r^3 = (c d - v)/r^3, d = x_t - x_s
   __device__ __forceinline__ void add(double xt, double yt, double zt, const double* __restrict__ sp,
                                       const TN* __restrict__ st, bool mask) {
     const double dxd = xt - sp[0], dyd = yt - sp[1], dzd = zt - sp[2];
@@ -1221,31 +1223,32 @@ struct PairAcc {
     const TN dx = (TN)dxd, dy = (TN)dyd, dz = (TN)dzd;
     TN ri = rsqrt_t<TN>((TN)r2d);
     if (mask) ri = TN(0);
-    const TN ri2 = ri * ri, ri3 = ri * ri2;
-    TN cr = 0, ci = 0;
-    if (CHG) {
+    const TN ri2 = ri * ri;
+    if (!DIP) {
       pr = fma(st[0], ri, pr);
       pi = fma(st[1], ri, pi);
-      if (FLD) { cr = st[0] * ri3; ci = st[1] * ri3; }
-    }
-    if (DIP) {
-      const TN dotr = fma(st[2], dx, fma(st[4], dy, st[6] * dz));
-      const TN doti = fma(st[3], dx, fma(st[5], dy, st[7] * dz));
-      pr = fma(dotr, ri3, pr);
-      pi = fma(doti, ri3, pi);
       if (FLD) {
-        const TN t = TN(3) * ri2 * ri3;
-        cr = fma(t, dotr, cr);
-        ci = fma(t, doti, ci);
-        fxr = fma(-ri3, st[2], fxr); fxi = fma(-ri3, st[3], fxi);
-        fyr = fma(-ri3, st[4], fyr); fyi = fma(-ri3, st[5], fyi);
-        fzr = fma(-ri3, st[6], fzr); fzi = fma(-ri3, st[7], fzi);
+        const TN ri3 = ri * ri2;
+        const TN cr = st[0] * ri3, ci = st[1] * ri3;
+        fxr = fma(cr, dx, fxr); fxi = fma(ci, dx, fxi);
+        fyr = fma(cr, dy, fyr); fyi = fma(ci, dy, fyi);
+        fzr = fma(cr, dz, fzr); fzi = fma(ci, dz, fzi);
       }
+      return;
     }
+    const TN dotr = fma(st[2], dx, fma(st[4], dy, st[6] * dz));
+    const TN doti = fma(st[3], dx, fma(st[5], dy, st[7] * dz));
+    const TN ar = CHG ? fma(dotr, ri2, st[0]) : dotr * ri2;
+    const TN ai = CHG ? fma(doti, ri2, st[1]) : doti * ri2;
+    pr = fma(ar, ri, pr);
+    pi = fma(ai, ri, pi);
     if (FLD) {
-      fxr = fma(cr, dx, fxr); fxi = fma(ci, dx, fxi);
-      fyr = fma(cr, dy, fyr); fyi = fma(ci, dy, fyi);
-      fzr = fma(cr, dz, fzr); fzi = fma(ci, dz, fzi);
+      const TN ri3 = ri * ri2, t3 = TN(3) * ri2;
+      const TN cr = CHG ? fma(t3, dotr, st[0]) : t3 * dotr;
+      const TN ci = CHG ? fma(t3, doti, st[1]) : t3 * doti;
+      fxr = fma(ri3, fma(cr, dx, -st[2]), fxr); fxi = fma(ri3, fma(ci, dx, -st[3]), fxi);
+      fyr = fma(ri3, fma(cr, dy, -st[4]), fyr); fyi = fma(ri3, fma(ci, dy, -st[5]), fyi);
+      fzr = fma(ri3, fma(cr, dz, -st[6]), fzr); fzi = fma(ri3, fma(ci, dz, -st[7]), fzi);
     }
   }
   __device__ __forceinline__ void fold_into(double* a) {
