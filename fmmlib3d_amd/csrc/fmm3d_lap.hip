@@ -77,6 +77,11 @@ DevTables ensure_tables(Context& ctx, int p, double eps) {
     ctx.tab_off_g[3] = ctx.tab_host_l.size();
     ctx.tab_host_l.insert(ctx.tab_host_l.end(), H.zt_off.begin(), H.zt_off.end());
     ctx.tab_zt_size = H.zt_size;
+    // FP32 copies of the grouped-M2L tables for the FP32 far field
+    ctx.tab_host_f.assign(H.rot_half.begin(), H.rot_half.end());
+    while (ctx.tab_host_f.size() % 4) ctx.tab_host_f.push_back(0.0f);
+    ctx.tab_off_f = ctx.tab_host_f.size();
+    ctx.tab_host_f.insert(ctx.tab_host_f.end(), H.z_m2l_t.begin(), H.z_m2l_t.end());
     ctx.tab_off[7] = o_del;
     ctx.tab_off_re = o_re;
     ctx.tab_off[0] = o_fac; ctx.tab_off[1] = o_ph; ctx.tab_off[2] = o_oph; ctx.tab_off[3] = o_rot;
@@ -87,6 +92,9 @@ DevTables ensure_tables(Context& ctx, int p, double eps) {
     FMM_HIP(hipMemcpyAsync(di, Ii.data(), Ii.size() * sizeof(int), hipMemcpyHostToDevice, ctx.stream));
     FMM_HIP(hipMemcpyAsync(dd, Dd.data(), Dd.size() * sizeof(double), hipMemcpyHostToDevice, ctx.stream));
     FMM_HIP(hipMemcpyAsync(dl, ctx.tab_host_l.data(), ctx.tab_host_l.size() * sizeof(long),
+                           hipMemcpyHostToDevice, ctx.stream));
+    void* df = ctx.arena.get("tab_f", ctx.tab_host_f.size() * sizeof(float));
+    FMM_HIP(hipMemcpyAsync(df, ctx.tab_host_f.data(), ctx.tab_host_f.size() * sizeof(float),
                            hipMemcpyHostToDevice, ctx.stream));
     FMM_HIP(hipStreamSynchronize(ctx.stream));
     ctx.tab_p = p;
@@ -109,6 +117,9 @@ DevTables ensure_tables(Context& ctx, int p, double eps) {
   T.rot_half = dd + ctx.tab_off_g[0]; T.z_m2l_t = dd + ctx.tab_off_g[1];
   T.rot_half_off = dl + ctx.tab_off_g[2]; T.zt_off = dl + ctx.tab_off_g[3];
   T.zt_size = ctx.tab_zt_size;
+  const float* dfl = (const float*)ctx.arena.get("tab_f", ctx.tab_host_f.size() * sizeof(float));
+  T.rot_half_f = dfl;
+  T.z_m2l_t_f = dfl + ctx.tab_off_f;
   return T;
 }
 
@@ -287,13 +298,16 @@ int lap_solve(Context& ctx, const LapCall& c) {
     int nI = 0;
     for (int n = 0; n <= p; n++) nI += n / 4 + 1;
     int gmax = 0;
+    // LDS: six int arrays, the local, a guard of 8 values, g members of (p+4)^2
+    auto m2l_lds = [&](int g) {
+      return 6 * kM2LChunk * sizeof(int) + sizeof(T) * (2 * (size_t)K + 8 + (size_t)g * 2 * (p + 4) * (p + 4));
+    };
     for (int g = gcap; g >= 1 && !gmax; g--) {
       const int gs = g > 2 ? 4 : g;
-      const size_t need = 6 * kM2LChunk * sizeof(int) + sizeof(T) * (size_t)(1 + g) * 2 * K;
-      if (nI * gs <= items * nt && need <= max_lds_bytes(ctx)) gmax = g;
+      if (nI * gs <= items * nt && m2l_lds(g) <= max_lds_bytes(ctx)) gmax = g;
     }
     if (m2l_form == 0 && gmax > 0) {
-      const size_t need = 6 * kM2LChunk * sizeof(int) + sizeof(T) * (size_t)(1 + gmax) * 2 * K;
+      const size_t need = m2l_lds(gmax);
       if (m2l_shape == 0)
         hipLaunchKernelGGL((k_m2l_grp<T, 256, 2>), dim3(P.n_m2l), dim3(256), need, s, P, Tb, gmax, mp, lo);
       else if (m2l_shape == 1)

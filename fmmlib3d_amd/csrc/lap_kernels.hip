@@ -721,69 +721,87 @@ __global__ void k_m2l(DevPlan P, DevTables Tb, const T* __restrict__ mp, T* __re
 // ---------------------------------------------------------------------------
 constexpr int kM2LGroup = 4;   // members per group
 constexpr int kM2LChunk = 192; // list-2 entries ordered at a time (list 2 has <= 189)
-__device__ __forceinline__ int grp_w(int n) { return 4 * (n / 4 + 1); }  // padded row width
 
-// out_{m'} = sum_m D[m][m'] in_m and out_{-m'} = sum_m D[m][m'] in_{-m} for
-// m' = mp0..mp0+3 (columns beyond n are zero in the padded table); BACK
-// multiplies by (-1)^{m+m'} (the transpose).  Dh: half block of degree n,
-// rows of width W; four rows are fetched before they are used.
-template <bool BACK, typename T>
-__device__ __forceinline__ void grp_acc4(T (&r)[16], const double4& d, int m, T xr, T xi, T yr, T yi) {
-  if (BACK && (m & 1)) { xr = -xr; xi = -xi; yr = -yr; yi = -yi; }
-  const T dv[4] = {(T)d.x, (T)d.y, (T)d.z, (T)d.w};
+// Table tiles: 4 rows x 4 columns, read as four 4-vectors (one per row).
+template <typename T>
+struct __attribute__((aligned(4 * sizeof(T)))) V4 {
+  T v[4];
+};
+template <typename T>
+__device__ __forceinline__ void grp_tile(const T* __restrict__ t, V4<T> (&d)[4]) {
+#pragma unroll
+  for (int u = 0; u < 4; u++) d[u] = reinterpret_cast<const V4<T>*>(t)[u];
+}
+template <typename T>
+__device__ __forceinline__ const T* grp_rot_tab(const DevTables& Tb);
+template <>
+__device__ __forceinline__ const double* grp_rot_tab<double>(const DevTables& Tb) { return Tb.rot_half; }
+template <>
+__device__ __forceinline__ const float* grp_rot_tab<float>(const DevTables& Tb) { return Tb.rot_half_f; }
+template <typename T>
+__device__ __forceinline__ const T* grp_z_tab(const DevTables& Tb);
+template <>
+__device__ __forceinline__ const double* grp_z_tab<double>(const DevTables& Tb) { return Tb.z_m2l_t; }
+template <>
+__device__ __forceinline__ const float* grp_z_tab<float>(const DevTables& Tb) { return Tb.z_m2l_t_f; }
+
+// r[4c..4c+3] += d_c * (xr, xi, yr, yi), with the sign (-1)^u of row u when
+// NEG (the transpose's (-1)^m, up to a factor applied afterwards)
+template <bool NEG, typename T>
+__device__ __forceinline__ void grp_fma_row(T (&r)[16], const V4<T>& d, T xr, T xi, T yr, T yi) {
+  if (NEG) { xr = -xr; xi = -xi; yr = -yr; yi = -yi; }
 #pragma unroll
   for (int c = 0; c < 4; c++) {
-    r[4 * c] += dv[c] * xr;
-    r[4 * c + 1] += dv[c] * xi;
-    r[4 * c + 2] += dv[c] * yr;
-    r[4 * c + 3] += dv[c] * yi;
+    r[4 * c] = fma(d.v[c], xr, r[4 * c]);
+    r[4 * c + 1] = fma(d.v[c], xi, r[4 * c + 1]);
+    r[4 * c + 2] = fma(d.v[c], yr, r[4 * c + 2]);
+    r[4 * c + 3] = fma(d.v[c], yi, r[4 * c + 3]);
   }
 }
 
-// one batch of four rows: row r0+u multiplies input degree n, order
-// m = r0+u-n (rows past `last` are clamped and contribute 0)
+// Rotation rows 4rb..4rb+3 (input order m = row - n); xin: in_n^{-n}
+// ascending, yin: in_n^{+n} descending.  Rows past 2n read neighbouring
+// finite LDS values against zero table rows.
 template <typename T, bool BACK>
-__device__ __forceinline__ void grp_rot_batch(T (&r)[16], const double4 (&d)[4], int r0, int n, int last,
-                                              const T* in) {
+__device__ __forceinline__ void grp_rot_batch(T (&r)[16], const V4<T> (&d)[4], int rb, const T* xin, const T* yin) {
 #pragma unroll
   for (int u = 0; u < 4; u++) {
-    const bool ok = r0 + u <= last;
-    const int m = min(r0 + u, last) - n;
-    T xr = in[2 * eidx(n, m)], xi = in[2 * eidx(n, m) + 1];
-    T yr = in[2 * eidx(n, -m)], yi = in[2 * eidx(n, -m) + 1];
-    if (!ok) { xr = 0; xi = 0; yr = 0; yi = 0; }
-    grp_acc4<BACK>(r, d[u], m, xr, xi, yr, yi);
+    const int row = 4 * rb + u;
+    if (BACK && (u & 1))
+      grp_fma_row<true>(r, d[u], xin[2 * row], xin[2 * row + 1], yin[-2 * row], yin[-2 * row + 1]);
+    else
+      grp_fma_row<false>(r, d[u], xin[2 * row], xin[2 * row + 1], yin[-2 * row], yin[-2 * row + 1]);
   }
 }
 
+// out_{m'} = sum_m D[m][m'] in_m, out_{-m'} = sum_m D[m][m'] in_{-m} for the
+// four columns m' = 4q..4q+3 of degree n (tile column q at Db); BACK applies
+// the transpose, D^T[m][m'] = (-1)^{m+m'} D[m'][m].  Two register buffers: the
+// next tile is in flight while the current one is used.
 template <typename T, bool BACK>
-__device__ __forceinline__ void grp_rot_item(const double* __restrict__ Dh, int W, int n, int mp0, const T* in,
+__device__ __forceinline__ void grp_rot_item(const T* __restrict__ Db, int nq, int n, int q, const T* in,
                                              T (&r)[16]) {
-  // rows m = -n..n in batches of four, two register buffers: the next batch
-  // is in flight while the current one is used
-  const double* col = Dh + mp0;
-  const int last = 2 * n;
-  double4 d0[4], d1[4];
-#pragma unroll
-  for (int u = 0; u < 4; u++) d0[u] = *reinterpret_cast<const double4*>(col + (size_t)min(u, last) * W);
-  for (int r0 = 0;;) {
-#pragma unroll
-    for (int u = 0; u < 4; u++) d1[u] = *reinterpret_cast<const double4*>(col + (size_t)min(r0 + 4 + u, last) * W);
+  const T* xin = in + 2 * (n * n);
+  const T* yin = in + 2 * (n * n + 2 * n);
+  const int nrb = (2 * n) / 4 + 1;
+  const size_t stride = (size_t)nq * 16;
+  V4<T> d0[4], d1[4];
+  grp_tile(Db, d0);
+  for (int rb = 0;;) {
+    grp_tile(Db + stride * (size_t)min(rb + 1, nrb - 1), d1);
     __builtin_amdgcn_sched_barrier(0);
-    grp_rot_batch<T, BACK>(r, d0, r0, n, last, in);
-    r0 += 4;
-    if (r0 > last) break;
-#pragma unroll
-    for (int u = 0; u < 4; u++) d0[u] = *reinterpret_cast<const double4*>(col + (size_t)min(r0 + 4 + u, last) * W);
+    grp_rot_batch<T, BACK>(r, d0, rb, xin, yin);
+    if (++rb >= nrb) break;
+    grp_tile(Db + stride * (size_t)min(rb + 1, nrb - 1), d0);
     __builtin_amdgcn_sched_barrier(0);
-    grp_rot_batch<T, BACK>(r, d1, r0, n, last, in);
-    r0 += 4;
-    if (r0 > last) break;
+    grp_rot_batch<T, BACK>(r, d1, rb, xin, yin);
+    if (++rb >= nrb) break;
   }
   if (BACK) {
+    // the row sign's remaining factor (-1)^n and the column sign (-1)^{m'}
 #pragma unroll
     for (int c = 0; c < 4; c++)
-      if ((mp0 + c) & 1) { r[4 * c] = -r[4 * c]; r[4 * c + 1] = -r[4 * c + 1]; r[4 * c + 2] = -r[4 * c + 2]; r[4 * c + 3] = -r[4 * c + 3]; }
+      if ((n + 4 * q + c) & 1) { r[4 * c] = -r[4 * c]; r[4 * c + 1] = -r[4 * c + 1]; r[4 * c + 2] = -r[4 * c + 2]; r[4 * c + 3] = -r[4 * c + 3]; }
   }
 }
 
@@ -806,39 +824,35 @@ __device__ __forceinline__ void grp_rot_store(int n, int mp0, const T (&r)[16], 
 // Zm: rows n of order am in the padded transposed table (width W, column
 // l - am), so one 32-byte load fetches the four degrees of a work item.
 template <typename T>
-__device__ __forceinline__ void grp_z_batch(T (&r)[16], const double4 (&z)[4], int n0, int am, int pt, const T* in) {
+__device__ __forceinline__ void grp_z_batch(T (&r)[16], const V4<T> (&z)[4], int n0, int am, const T* in) {
 #pragma unroll
   for (int u = 0; u < 4; u++) {
-    const bool ok = n0 + u <= pt;
-    const int n = min(n0 + u, pt);
-    T xr = in[2 * eidx(n, am)], xi = in[2 * eidx(n, am) + 1];
-    T yr = in[2 * eidx(n, -am)], yi = in[2 * eidx(n, -am) + 1];
-    if (!ok) { xr = 0; xi = 0; yr = 0; yi = 0; }
-    grp_acc4<false>(r, z[u], 0, xr, xi, yr, yi);
+    const int n = n0 + u;
+    const T* x = in + 2 * (n * n + n + am);
+    grp_fma_row<false>(r, z[u], x[0], x[1], x[-4 * am], x[-4 * am + 1]);
   }
 }
 
+// out_l^{+-am} = sum_{n=am}^{pt} Z[am][l][n] in_n^{+-am}, l = am+4q..am+4q+3;
+// Zm: the tiles of order am (row blocks of n - am, tile column q).  Degrees
+// pt+1..pt+3 of the input are zero, so the last row block needs no mask.
 template <typename T>
-__device__ __forceinline__ void grp_z_item(const double* __restrict__ Zm, int W, int am, int l0, int pt, const T* in,
+__device__ __forceinline__ void grp_z_item(const T* __restrict__ Zm, int nq, int am, int q, int pt, const T* in,
                                            T (&r)[16]) {
-  // rows n = am..pt in batches of four, fetched one batch ahead
-  const double* col = Zm + (l0 - am);
-  double4 z0[4], z1[4];
-#pragma unroll
-  for (int u = 0; u < 4; u++) z0[u] = *reinterpret_cast<const double4*>(col + (size_t)min(am + u, pt) * W);
-  for (int n0 = am;;) {
-#pragma unroll
-    for (int u = 0; u < 4; u++) z1[u] = *reinterpret_cast<const double4*>(col + (size_t)min(n0 + 4 + u, pt) * W);
+  const T* Zq = Zm + 16 * q;
+  const int nrb = (pt - am) / 4 + 1;
+  const size_t stride = (size_t)nq * 16;
+  V4<T> z0[4], z1[4];
+  grp_tile(Zq, z0);
+  for (int rb = 0;;) {
+    grp_tile(Zq + stride * (size_t)min(rb + 1, nrb - 1), z1);
     __builtin_amdgcn_sched_barrier(0);
-    grp_z_batch<T>(r, z0, n0, am, pt, in);
-    n0 += 4;
-    if (n0 > pt) break;
-#pragma unroll
-    for (int u = 0; u < 4; u++) z0[u] = *reinterpret_cast<const double4*>(col + (size_t)min(n0 + 4 + u, pt) * W);
+    grp_z_batch<T>(r, z0, am + 4 * rb, am, in);
+    if (++rb >= nrb) break;
+    grp_tile(Zq + stride * (size_t)min(rb + 1, nrb - 1), z0);
     __builtin_amdgcn_sched_barrier(0);
-    grp_z_batch<T>(r, z1, n0, am, pt, in);
-    n0 += 4;
-    if (n0 > pt) break;
+    grp_z_batch<T>(r, z1, am + 4 * rb, am, in);
+    if (++rb >= nrb) break;
   }
 }
 
@@ -868,12 +882,17 @@ __global__ __launch_bounds__(NT) void k_m2l_grp(DevPlan P, DevTables Tb, int gma
   int* t_box = t_key + kM2LChunk;
   int* t_code = t_box + kM2LChunk;
   T* acc = reinterpret_cast<T*>(t_code + kM2LChunk);
-  T* A = acc + 2 * P.K;
   const int tid = threadIdx.x, bs = blockDim.x;
   const int b = P.m2l_boxes[blockIdx.x];
   const int p = P.p, P1 = p + 1, K2 = 2 * P.K;
+  // members: (p+4)^2 complex each (degrees p+1..p+3 stay zero or hold the
+  // zeros written for a truncated member), after a zero guard of 8 values
+  const int MS = 2 * (p + 4) * (p + 4);
+  T* A = acc + K2 + 8;
   const T ih = (T)P.inv_side[P.level[b]];
-  for (int k = tid; k < K2; k += bs) acc[k] = T(0);
+  for (int k = tid; k < K2 + 8 + gmax * MS; k += bs) acc[k] = T(0);
+  const T* rtab = grp_rot_tab<T>(Tb);
+  const T* ztab = grp_z_tab<T>(Tb);
   const int off0 = P.l2_off[b], n2 = P.l2_off[b + 1] - off0;
   const int bi = P.ijk[3 * b], bj = P.ijk[3 * b + 1], bk = P.ijk[3 * b + 2];
   for (int c0 = 0; c0 < n2; c0 += kM2LChunk) {
@@ -908,21 +927,26 @@ __global__ __launch_bounds__(NT) void k_m2l_grp(DevPlan P, DevTables Tb, int gma
       const int code0 = t_code[s0];
       const int pt = Tb.off_ptrunc[code0];
       const int rc = Tb.off_rot[code0];
-      const double* Z = Tb.z_m2l_t + (size_t)Tb.off_dist[code0] * Tb.zt_size;
-      const int Kp = (pt + 1) * (pt + 1);
-      // (a) members times e^{i m phi}
-      for (int idx = tid; idx < g * Kp; idx += bs) {
-        const int jj = idx / Kp, k = idx - jj * Kp;
-        int n = (int)sqrtf((float)k);
-        while (n * n > k) n--;
-        while ((n + 1) * (n + 1) <= k) n++;
-        const int m = k - n * n - n, am = m < 0 ? -m : m;
-        const double* ph = Tb.off_phase + (size_t)t_code[s0 + jj] * 2 * P1;
-        const T er = (T)ph[2 * am], ei = m < 0 ? -(T)ph[2 * am + 1] : (T)ph[2 * am + 1];
-        const T* src = mp + (size_t)t_box[s0 + jj] * K2;
-        const T xr = src[2 * k], xi = src[2 * k + 1];
-        A[jj * K2 + 2 * k] = xr * er - xi * ei;
-        A[jj * K2 + 2 * k + 1] = xr * ei + xi * er;
+      const T* Z = ztab + (size_t)Tb.off_dist[code0] * Tb.zt_size;
+      const int Kp = (pt + 1) * (pt + 1), Kz = (pt + 4) * (pt + 4);
+      // (a) members times e^{i m phi}; degrees pt+1..pt+3 set to zero
+      for (int idx = tid; idx < g * Kz; idx += bs) {
+        const int jj = idx / Kz, k = idx - jj * Kz;
+        T vr = 0, vi = 0;
+        if (k < Kp) {
+          int n = (int)sqrtf((float)k);
+          while (n * n > k) n--;
+          while ((n + 1) * (n + 1) <= k) n++;
+          const int m = k - n * n - n, am = m < 0 ? -m : m;
+          const double* ph = Tb.off_phase + (size_t)t_code[s0 + jj] * 2 * P1;
+          const T er = (T)ph[2 * am], ei = m < 0 ? -(T)ph[2 * am + 1] : (T)ph[2 * am + 1];
+          const T* src = mp + (size_t)t_box[s0 + jj] * K2;
+          const T xr = src[2 * k], xi = src[2 * k + 1];
+          vr = xr * er - xi * ei;
+          vi = xr * ei + xi * er;
+        }
+        A[jj * MS + 2 * k] = vr;
+        A[jj * MS + 2 * k + 1] = vi;
       }
       __syncthreads();
       // Work items of a stage (rotation items by degree, z-shift items by
@@ -942,9 +966,8 @@ __global__ __launch_bounds__(NT) void k_m2l_grp(DevPlan P, DevTables Tb, int gma
 #pragma unroll
         for (int e = 0; e < 16; e++) r[q][e] = T(0);
         if (uq[q] >= 0 && mem) {
-          const int it = Tb.grp_ritems[uq[q]], n = it >> 8;
-          grp_rot_item<T, false>(Tb.rot_half + Tb.rot_half_off[rc * P1 + n], grp_w(n), n, 4 * (it & 255),
-                                 A + jj * K2, r[q]);
+          const int it = Tb.grp_ritems[uq[q]], n = it >> 8, qc = it & 255;
+          grp_rot_item<T, false>(rtab + Tb.rot_half_off[rc * P1 + n] + 16 * qc, n / 4 + 1, n, qc, A + jj * MS, r[q]);
         }
       }
       __syncthreads();
@@ -952,7 +975,7 @@ __global__ __launch_bounds__(NT) void k_m2l_grp(DevPlan P, DevTables Tb, int gma
       for (int q = 0; q < ITEMS; q++)
         if (uq[q] >= 0 && mem) {
           const int it = Tb.grp_ritems[uq[q]];
-          grp_rot_store<T>(it >> 8, 4 * (it & 255), r[q], A + jj * K2);
+          grp_rot_store<T>(it >> 8, 4 * (it & 255), r[q], A + jj * MS);
         }
       __syncthreads();
       // (c) shift along z
@@ -963,7 +986,7 @@ __global__ __launch_bounds__(NT) void k_m2l_grp(DevPlan P, DevTables Tb, int gma
         for (int e = 0; e < 16; e++) r[q][e] = T(0);
         if (uq[q] >= 0 && mem) {
           const int it = Tb.grp_zitems[zo + uq[q]], am = it >> 8;
-          grp_z_item<T>(Z + Tb.zt_off[am], grp_w(p - am), am, am + 4 * (it & 255), pt, A + jj * K2, r[q]);
+          grp_z_item<T>(Z + Tb.zt_off[am], (p - am) / 4 + 1, am, it & 255, pt, A + jj * MS, r[q]);
         }
       }
       __syncthreads();
@@ -971,7 +994,7 @@ __global__ __launch_bounds__(NT) void k_m2l_grp(DevPlan P, DevTables Tb, int gma
       for (int q = 0; q < ITEMS; q++)
         if (uq[q] >= 0 && mem) {
           const int it = Tb.grp_zitems[zo + uq[q]], am = it >> 8;
-          grp_z_store<T>(am, am + 4 * (it & 255), pt, r[q], A + jj * K2);
+          grp_z_store<T>(am, am + 4 * (it & 255), pt, r[q], A + jj * MS);
         }
       __syncthreads();
       // (d) rotate back, apply e^{-i m phi} and the scale, sum the members
@@ -985,7 +1008,8 @@ __global__ __launch_bounds__(NT) void k_m2l_grp(DevPlan P, DevTables Tb, int gma
           n = it >> 8;
           mp0 = 4 * (it & 255);
           if (mem) {
-            grp_rot_item<T, true>(Tb.rot_half + Tb.rot_half_off[rc * P1 + n], grp_w(n), n, mp0, A + jj * K2, r[q]);
+            grp_rot_item<T, true>(rtab + Tb.rot_half_off[rc * P1 + n] + 4 * mp0, n / 4 + 1, n, mp0 / 4,
+                                  A + jj * MS, r[q]);
             const double* ph = Tb.off_phase + (size_t)t_code[s0 + jj] * 2 * P1;
 #pragma unroll
             for (int c = 0; c < 4; c++) {
@@ -1292,6 +1316,7 @@ __global__ __launch_bounds__(256) void k_p2p(DevPlan P, const double* __restrict
     const int s0 = P.isrc[j], s1 = s0 + P.nsrc[j];
     for (int c = s0; c < s1; c += (fold ? kFold : (1 << 30))) {
       const int ce = fold ? min(c + kFold, s1) : s1;
+#pragma unroll 4
       for (int s = c; s < ce; s++) acc.add(x, y, z, spos + 4 * (size_t)s, str + 8 * (size_t)s, false);
       if (fold) acc.fold_into(tot);
     }

@@ -57,10 +57,12 @@ struct DevTables {
   // grouped M2L work items: rotation (n << 8 | q) by degree, count with n <= pt;
   // z-shift (|m| << 8 | q) per truncation order pt, start of pt's items
   const int *grp_ritems, *grp_rcount, *grp_zitems, *grp_zoff;
-  // padded tables of the grouped M2L: half rotation blocks [m][m' >= 0] per
-  // (class, degree) at rot_half_off[c*(p+1)+n]; z-shift [m][n][l-m] per
-  // distance class (zt_size doubles each), order m at zt_off[m]
+  // tiled tables of the grouped M2L: half rotation blocks [m][m' >= 0] per
+  // (class, degree) at rot_half_off[c*(p+1)+n]; z-shift [n-m][l-m] per
+  // distance class (zt_size values each), order m at zt_off[m]; both in 4x4
+  // tiles [row block][column block][row][column], in FP64 and in FP32
   const double *rot_half, *z_m2l_t;
+  const float *rot_half_f, *z_m2l_t_f;
   const long *rot_half_off, *zt_off;
   long zt_size;
 };

@@ -65,6 +65,8 @@ struct Context {
   size_t tab_off_i[4] = {0, 0, 0, 0};  // grouped M2L items: ritems, rcount, zitems, zoff
   size_t tab_off_g[4] = {0, 0, 0, 0};  // grouped M2L tables: rot_half, z_m2l_t (doubles); their offsets (longs)
   long tab_zt_size = 0;
+  std::vector<float> tab_host_f;  // FP32 copies of the grouped-M2L tables
+  size_t tab_off_f = 0;
   // cached Helmholtz tables (owned by the Helmholtz driver)
   std::shared_ptr<void> helm_tables;
 };

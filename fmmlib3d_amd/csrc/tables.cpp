@@ -130,19 +130,25 @@ void build_tables(int p, double eps, HostTables& H) {
           z[((size_t)m * P1 + l) * P1 + n] = c;
         }
   }
-  // the same coefficients for the grouped M2L, [m][in n][out l - m], rows
-  // padded to a multiple of 4 (one 32-byte load per four output degrees)
+  // the same coefficients for the grouped M2L in 4x4 tiles per order m:
+  // [row block of n - m][column block of l - m][row][column], zero past p
   H.zt_off.assign(P1 + 1, 0);
-  for (int m = 0; m <= p; m++) H.zt_off[m + 1] = H.zt_off[m] + (long)P1 * grp_width(p - m);
+  for (int m = 0; m <= p; m++) {
+    const long nb4 = (p - m) / 4 + 1;
+    H.zt_off[m + 1] = H.zt_off[m] + 16 * nb4 * nb4;
+  }
   H.zt_size = H.zt_off[P1];
   H.z_m2l_t.assign((size_t)H.zt_size * H.ndist, 0.0);
   for (int dc = 0; dc < H.ndist; dc++)
     for (int m = 0; m <= p; m++) {
-      const int W = grp_width(p - m);
+      const int nb4 = (p - m) / 4 + 1;
+      double* zb = &H.z_m2l_t[(size_t)H.zt_size * dc + H.zt_off[m]];
       for (int n = m; n <= p; n++)
-        for (int l = m; l <= p; l++)
-          H.z_m2l_t[(size_t)H.zt_size * dc + H.zt_off[m] + (size_t)n * W + (l - m)] =
+        for (int l = m; l <= p; l++) {
+          const int rr = n - m, cc = l - m;
+          zb[(((size_t)(rr / 4) * nb4 + cc / 4) * 4 + rr % 4) * 4 + cc % 4] =
               H.z_m2l[Z * dc + ((size_t)m * P1 + l) * P1 + n];
+        }
     }
   H.z_m2m.assign(Z, 0.0);
   H.z_l2l.assign(Z, 0.0);
@@ -311,24 +317,27 @@ void build_geometry_tables(int p, HostTables& H) {
       for (int q = 0; am + 4 * q <= pt; q++) H.grp_zitems.push_back((am << 8) | q);
   }
   H.grp_zoff[P1] = (int)H.grp_zitems.size();
-  // half rotation blocks for the grouped M2L: columns m' >= 0 only (the
-  // others follow from D[-m][-m'] = D[m][m']), rows padded to a multiple of 4
-  // so that one 32-byte load fetches the four columns of a work item
+  // rotation blocks for the grouped M2L: columns m' >= 0 only (the others
+  // follow from D[-m][-m'] = D[m][m']), cut into 4x4 tiles [row block][column
+  // block][row][column] so that one work item (four columns) reads one
+  // contiguous 128-byte tile per four rows; rows past 2n and columns past n
+  // are zero
   H.rot_half_off.assign((size_t)nrot * P1, 0);
   long th = 0;
   for (int c = 0; c < nrot; c++)
     for (int n = 0; n <= p; n++) {
       H.rot_half_off[(size_t)c * P1 + n] = th;
-      th += (long)(2 * n + 1) * grp_width(n);
+      th += 16L * (2 * n / 4 + 1) * (n / 4 + 1);
     }
   H.rot_half.assign(th, 0.0);
   for (int c = 0; c < nrot; c++)
     for (int n = 0; n <= p; n++) {
-      const int dim = 2 * n + 1, W = grp_width(n);
+      const int dim = 2 * n + 1, nq = n / 4 + 1;
       const double* D = &H.rot[H.rot_off[(size_t)c * P1 + n]];
-      double* Dh = &H.rot_half[H.rot_half_off[(size_t)c * P1 + n]];
-      for (int m = -n; m <= n; m++)
-        for (int mp = 0; mp <= n; mp++) Dh[(size_t)(m + n) * W + mp] = D[(m + n) * dim + (mp + n)];
+      double* Db = &H.rot_half[H.rot_half_off[(size_t)c * P1 + n]];
+      for (int row = 0; row < dim; row++)
+        for (int mp = 0; mp <= n; mp++)
+          Db[(((size_t)(row / 4) * nq + mp / 4) * 4 + row % 4) * 4 + mp % 4] = D[row * dim + (mp + n)];
     }
 }
 

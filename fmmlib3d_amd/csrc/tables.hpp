@@ -4,9 +4,6 @@
 
 namespace fmm3d {
 
-// row width of the padded tables of the grouped M2L: n+1 rounded up to 4
-inline int grp_width(int n) { return 4 * (n / 4 + 1); }
-
 struct HostTables {
   int p = -1, nrot = 0, ndist = 0;
   std::vector<double> fac;                         // FacLayout(p)
@@ -26,9 +23,9 @@ struct HostTables {
   // grouped M2L work items (see build_geometry_tables)
   std::vector<int> grp_ritems, grp_rcount;         // (n << 8 | q); count of items with n <= pt
   std::vector<int> grp_zitems, grp_zoff;           // (|m| << 8 | q) per pt; start of pt's items
-  std::vector<double> rot_half;                    // per class and degree: [m][m' >= 0], width grp_width(n)
+  std::vector<double> rot_half;                    // per class and degree: 4x4 tiles of [m][m' >= 0]
   std::vector<long> rot_half_off;                  // nrot x (p+1)
-  std::vector<double> z_m2l_t;                     // per distance class: [m][n][l - m], width grp_width(p - m)
+  std::vector<double> z_m2l_t;                     // per distance class and order m: 4x4 tiles of [n-m][l-m]
   std::vector<long> zt_off;                        // start of order m within a class, p+2 entries
   long zt_size = 0;                                // doubles per distance class
 };
