@@ -45,8 +45,8 @@ FP32_PEAK_TFLOPS = 157.3       # MI355X FP32 vector peak (packed)
 FLOPS_PER_PAIR = 65            # charge+dipole, pot+fld, rsqrt counted as 2
 # HBM bytes per k_p2p launch on this workload, from rocprofv3 FETCH_SIZE and
 # WRITE_SIZE runs calibrated against a 1 GiB device copy
-# (profiles/fmm_kernels_config2_config3.md): 2 x FETCH KiB + WRITE KiB.
-P2P_TRAFFIC_BYTES = {"fp64": (2 * 2316467 + 62668) * 1024, "fp32": (2 * 1462139 + 62655) * 1024}
+# (profiles/fmm_kernels_lap1M_helm500k.md): 2 x FETCH KiB + WRITE KiB.
+P2P_TRAFFIC_BYTES = {"fp64": (2 * 2289591 + 62655) * 1024, "fp32": (2 * 1463937 + 62655) * 1024}
 
 
 def make_inputs(n, seed):
