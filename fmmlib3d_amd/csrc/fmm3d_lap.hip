@@ -300,7 +300,7 @@ int lap_solve(Context& ctx, const LapCall& c) {
     int gmax = 0;
     // LDS: six int arrays, the local, a guard of 8 values, g members of (p+4)^2
     auto m2l_lds = [&](int g) {
-      return 6 * kM2LChunk * sizeof(int) + sizeof(T) * (2 * (size_t)K + 8 + (size_t)g * 2 * (p + 4) * (p + 4));
+      return 6 * kM2LChunk * sizeof(int) + sizeof(T) * (2 * (size_t)K + 8 + (size_t)g * grp_member_stride<T>(p));
     };
     for (int g = gcap; g >= 1 && !gmax; g--) {
       const int gs = g > 2 ? 4 : g;

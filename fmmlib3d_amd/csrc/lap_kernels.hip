@@ -722,6 +722,18 @@ __global__ void k_m2l(DevPlan P, DevTables Tb, const T* __restrict__ mp, T* __re
 constexpr int kM2LGroup = 4;   // members per group
 constexpr int kM2LChunk = 192; // list-2 entries ordered at a time (list 2 has <= 189)
 
+// Values per member in LDS: (p+4)^2 complex, padded so that the stride in
+// 4-byte words is 8 mod 32 and the four members of a work item start in
+// different quarters of the 32 LDS banks.
+template <typename T>
+__host__ __device__ inline int grp_member_stride(int p) {
+  const int base = 2 * (p + 4) * (p + 4);
+  const int wpe = (int)sizeof(T) / 4;  // words per value
+  int ms = base;
+  while ((ms * wpe) % 32 != 8) ms++;
+  return ms;
+}
+
 // Table tiles: 4 rows x 4 columns, read as four 4-vectors (one per row).
 template <typename T>
 struct __attribute__((aligned(4 * sizeof(T)))) V4 {
@@ -887,7 +899,7 @@ __global__ __launch_bounds__(NT) void k_m2l_grp(DevPlan P, DevTables Tb, int gma
   const int p = P.p, P1 = p + 1, K2 = 2 * P.K;
   // members: (p+4)^2 complex each (degrees p+1..p+3 stay zero or hold the
   // zeros written for a truncated member), after a zero guard of 8 values
-  const int MS = 2 * (p + 4) * (p + 4);
+  const int MS = grp_member_stride<T>(p);
   T* A = acc + K2 + 8;
   const T ih = (T)P.inv_side[P.level[b]];
   for (int k = tid; k < K2 + 8 + gmax * MS; k += bs) acc[k] = T(0);
