@@ -3,8 +3,9 @@
 // up to four levels are resolved per pass over the points (octant digit
 // codes, a histogram that decides every split, one stable counting sort),
 // child ids are assigned breadth first, and the interaction lists are
-// derived from integer box coordinates and a per-level hash of (i,j,k)
-// instead of d3tstrcr's floating-point corner tests and linked-list store.
+// derived from integer box coordinates (colleagues from the parent's
+// colleagues' children) instead of d3tstrcr's floating-point corner tests and
+// linked-list store.
 #include "octree.hpp"
 
 #include <omp.h>
@@ -12,7 +13,6 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
-#include <unordered_map>
 
 namespace fmm3d {
 
@@ -103,10 +103,6 @@ void push_box(Octree& t, int level, int i, int j, int k, int par, int is, int n_
   t.bcenter.push_back(box_center_1d(t.center[2], t.size, level, k));
 }
 
-inline uint64_t ijk_key(int i, int j, int k) {
-  return ((uint64_t)(uint32_t)i << 42) | ((uint64_t)(uint32_t)j << 21) | (uint64_t)(uint32_t)k;
-}
-
 // closed-interval overlap of two boxes given by level and 1-based (i,j,k)
 inline bool touching(const Octree& t, int a, int b) {
   int la = t.level[a], lb = t.level[b];
@@ -119,8 +115,7 @@ inline bool touching(const Octree& t, int a, int b) {
   return true;
 }
 
-void flatten(std::vector<std::vector<int>>& per, Csr& c) {
-  int nb = (int)per.size();
+void flatten(std::vector<std::vector<int>>& per, int nb, Csr& c) {
   c.off.assign(nb + 1, 0);
   for (int b = 0; b < nb; b++) c.off[b + 1] = c.off[b] + (int)per[b].size();
   c.idx.resize(c.off[nb]);
@@ -246,28 +241,45 @@ void tree_pass_expand(Octree& t, int nbox, TreePass& ps, std::vector<int>& next)
 void tree_lists(Octree& t) {
   const int nb = t.nboxes;
   const int nthreads = omp_get_max_threads();
-  // colleagues through a per-level hash of (i,j,k)
-  std::vector<std::unordered_map<uint64_t, int>> H(t.nlev + 1);
-  for (int l = 0; l <= t.nlev; l++) {
-    H[l].reserve(2 * (t.ladd[l + 1] - t.ladd[l]));
-    for (int b = t.ladd[l]; b < t.ladd[l + 1]; b++)
-      H[l][ijk_key(t.ijk[3 * b], t.ijk[3 * b + 1], t.ijk[3 * b + 2])] = b;
+  // per-box scratch lists kept between calls (no allocation once grown); the
+  // calling thread's copies, bound to references that the OpenMP workers share
+  thread_local std::vector<std::vector<int>> tl_coll, tl_l1, tl_l2, tl_l3, tl_l4;
+  thread_local std::vector<std::vector<std::pair<int, int>>> tl_mirror;
+  auto& coll = tl_coll;
+  auto& l1 = tl_l1;
+  auto& l2 = tl_l2;
+  auto& l3 = tl_l3;
+  auto& l4 = tl_l4;
+  auto& mirror = tl_mirror;
+  for (auto* v : {&coll, &l1, &l2, &l3, &l4}) {
+    if ((int)v->size() < nb) v->resize(nb);
+#pragma omp parallel for schedule(static)
+    for (int b = 0; b < nb; b++) (*v)[b].clear();
   }
-  std::vector<std::vector<int>> coll(nb);
-#pragma omp parallel for schedule(dynamic, 64)
-  for (int b = 1; b < nb; b++) {
-    int l = t.level[b];
-    for (int di = -1; di <= 1; di++)
-      for (int dj = -1; dj <= 1; dj++)
-        for (int dk = -1; dk <= 1; dk++) {
-          if (!di && !dj && !dk) continue;
-          auto it = H[l].find(ijk_key(t.ijk[3 * b] + di, t.ijk[3 * b + 1] + dj, t.ijk[3 * b + 2] + dk));
-          if (it != H[l].end()) coll[b].push_back(it->second);
+  if ((int)mirror.size() < nthreads) mirror.resize(nthreads);
+  for (auto& m : mirror) m.clear();
+  // colleagues, level by level: the same-level boxes touching b are children
+  // of b's parent or of the parent's colleagues
+  for (int l = 1; l <= t.nlev; l++) {
+#pragma omp parallel for schedule(static)
+    for (int b = t.ladd[l]; b < t.ladd[l + 1]; b++) {
+      const int par = t.parent[b];
+      auto visit = [&](int q) {
+        for (int c = 0; c < t.nchild[q]; c++) {
+          const int kid = t.child[8 * q + c];
+          if (kid == b) continue;
+          int d = 0;
+          for (int x = 0; x < 3; x++) d = std::max(d, std::abs(t.ijk[3 * kid + x] - t.ijk[3 * b + x]));
+          if (d <= 1) coll[b].push_back(kid);
         }
+      };
+      visit(par);
+      for (int q : coll[par]) visit(q);
+      std::sort(coll[b].begin(), coll[b].end());
+    }
   }
   // list 2: children of the parent's colleagues (and of the parent) that do
   // not touch the box (d3tstrcr.f:768-821)
-  std::vector<std::vector<int>> l2(nb), l1(nb), l3(nb), l4(nb);
 #pragma omp parallel for schedule(dynamic, 64)
   for (int b = 1; b < nb; b++) {
     int p = t.parent[b];
@@ -285,12 +297,12 @@ void tree_lists(Octree& t) {
   }
   // lists 1 and 3 for leaves below the root (d3tlst31, d3tstrcr.f:912-1069);
   // the coarse-to-fine list-1 entries are mirrored afterwards.
-  std::vector<std::vector<std::pair<int, int>>> mirror(nthreads);
 #pragma omp parallel for schedule(dynamic, 16)
   for (int b = 1; b < nb; b++) {
     if (t.nchild[b]) continue;
     int tid = omp_get_thread_num();
-    std::vector<int> stack;
+    thread_local std::vector<int> stack;
+    stack.clear();
     for (int j0 : coll[b]) {
       stack.push_back(j0);
       while (!stack.empty()) {
@@ -306,14 +318,14 @@ void tree_lists(Octree& t) {
       }
     }
   }
-  for (auto& v : mirror)
-    for (auto& pr : v) l1[pr.first].push_back(pr.second);
+  for (int th = 0; th < nthreads; th++)
+    for (auto& pr : mirror[th]) l1[pr.first].push_back(pr.second);
   for (int b = 0; b < nb; b++)
     for (int j : l3[b]) l4[j].push_back(b);
-  flatten(l1, t.list[0]);
-  flatten(l2, t.list[1]);
-  flatten(l3, t.list[2]);
-  flatten(l4, t.list[3]);
+  flatten(l1, nb, t.list[0]);
+  flatten(l2, nb, t.list[1]);
+  flatten(l3, nb, t.list[2]);
+  flatten(l4, nb, t.list[3]);
 }
 
 // ---------------------------------------------------------------------------
