@@ -260,18 +260,34 @@ int lap_solve(Context& ctx, const LapCall& c) {
   for (auto& g : O.m2m_groups)
     hipLaunchKernelGGL((k_m2m<T>), dim3(g.second), dim3(256), lds_ps, s, P, Tb, g.first, mp);
   tm.mark(3);
-  // ---- while the device runs P2M and M2M: lists and plan part B
+  // ---- while the device runs P2M and M2M: list 1 and the P2P plan (part B1)
   auto t_lists0 = std::chrono::steady_clock::now();
-  tree_lists(tree);
-  build_plan_lists(tree, f, FB, O);
-  const int* dIB = upload_ints("plan_ib", FB.I);  // part B: offsets in O from build_plan_lists
-  devplan_lists(P, O, dIB);
-  // the P2P work items are addressed explicitly: they must lie in part B
+  tree_lists_near(tree);
+  build_plan_near(tree, f, FB, O);
+  const int* dIB = upload_ints("plan_ib", FB.I);  // part B1: offsets in O from build_plan_near
+  devplan_near(P, O, dIB);
+  // the P2P work items are addressed explicitly: they must lie in part B1
   if (O.p2p0 + 3 * (size_t)O.n_p2p0 > FB.I.size() || O.p2p1 + 3 * (size_t)O.n_p2p1 > FB.I.size())
-    throw HipError("plan: P2P work items outside the list part of the plan");
+    throw HipError("plan: P2P work items outside the near-field part of the plan");
   auto t_lists1 = std::chrono::steady_clock::now();
-  form(std::true_type{}, P.n_p2l, lo);
+  // ---- STEP 8: P2P (self + list 1), queued ahead of the far field
+  if (f.ifcharge || f.ifdipole) {
+    launch_p2p<TN>(P, dIB + O.p2p0, O.n_p2p0, 0, spos, strn, tpos, f.ifcharge, f.ifdipole,
+                   f.iffld != 0, pot_s, fld_s, s);
+    launch_p2p<TN>(P, dIB + O.p2p1, O.n_p2p1, 1, spos, strn, tpos, f.ifcharge, f.ifdipole,
+                   f.iffldtarg != 0, pot_t, fld_t, s);
+  }
   tm.mark(4);
+  // ---- while the device runs P2P: lists 2-4 and the far-field plan (part B2)
+  auto t_lists2 = std::chrono::steady_clock::now();
+  tree_lists_far(tree);
+  thread_local Flat FC;
+  FC.I.clear(); FC.D.clear();
+  build_plan_far(tree, f, FC, O);
+  devplan_far(P, O, upload_ints("plan_ic", FC.I));
+  auto t_lists3 = std::chrono::steady_clock::now();
+  form(std::true_type{}, P.n_p2l, lo);
+  tm.mark(5);
   // ---- STEP 4: M2L
   if (P.n_m2l) {
     // Default: pairs grouped by translation class (k_m2l_grp).
@@ -322,11 +338,11 @@ int lap_solve(Context& ctx, const LapCall& c) {
       else hipLaunchKernelGGL((k_m2l_fact<T, false>), dim3(P.n_m2l), dim3(256), sh, s, P, Tb, mp, lo);
     }
   }
-  tm.mark(5);
+  tm.mark(6);
   // ---- STEP 5: L2L, top-down
   for (auto& g : O.l2l_groups)
     hipLaunchKernelGGL((k_l2l<T>), dim3(g.second), dim3(256), lds_ps, s, P, Tb, g.first, lo);
-  tm.mark(6);
+  tm.mark(7);
   // ---- STEP 7: L2P (packed local + gradient expansions, wave-uniform reads)
   const size_t lds_e = sizeof(T) * 2 * K;
   if (P.n_l2p) {
@@ -339,19 +355,11 @@ int lap_solve(Context& ctx, const LapCall& c) {
       hipLaunchKernelGGL((k_l2p2<T>), dim3(P.n_l2pi), dim3(256), 0, s, P, fac, pack, spos, tpos, f.ifpot, f.iffld,
                          f.ifpottarg, f.iffldtarg, pot_s, fld_s, pot_t, fld_t);
   }
-  tm.mark(7);
+  tm.mark(8);
   // ---- STEP 6: M2P (list 3)
   if (P.n_m2p)
     hipLaunchKernelGGL((k_eval<T, true>), dim3(P.n_m2p), dim3(256), lds_e, s, P, fac, mp, spos, tpos,
                        f.ifpot, f.iffld, f.ifpottarg, f.iffldtarg, pot_s, fld_s, pot_t, fld_t);
-  tm.mark(8);
-  // ---- STEP 8: P2P (self + list 1)
-  if (f.ifcharge || f.ifdipole) {
-    launch_p2p<TN>(P, dIB + O.p2p0, O.n_p2p0, 0, spos, strn, tpos, f.ifcharge, f.ifdipole,
-                   f.iffld != 0, pot_s, fld_s, s);
-    launch_p2p<TN>(P, dIB + O.p2p1, O.n_p2p1, 1, spos, strn, tpos, f.ifcharge, f.ifdipole,
-                   f.iffldtarg != 0, pot_t, fld_t, s);
-  }
   tm.mark(9);
   // ---- scatter back to the caller's order (l3dpsort / l3dfsort)
   if (f.ifpot) hipLaunchKernelGGL(k_scatter, dim3(blocks(ns, 256)), dim3(256), 0, s, ns, 1, sperm, pot_s, c.pot_dev);
@@ -365,15 +373,15 @@ int lap_solve(Context& ctx, const LapCall& c) {
   lt.assign(13, 0.0);
   auto ms = [](auto a, auto b) { return std::chrono::duration<double, std::milli>(b - a).count(); };
   lt[0] = ms(t0, t1);
-  auto el = tm.elapsed();  // upload, P2M, M2M, P2L, M2L, L2L, L2P, M2P, P2P, scatter
+  auto el = tm.elapsed();  // upload, P2M, M2M, P2P, P2L, M2L, L2L, L2P, M2P, scatter
   el.resize(10, 0.0);
-  lt[1] = el[0]; lt[2] = el[1]; lt[3] = el[3]; lt[4] = el[2]; lt[5] = el[4];
-  lt[6] = el[5]; lt[7] = el[6]; lt[8] = el[7]; lt[9] = el[8];
+  lt[1] = el[0]; lt[2] = el[1]; lt[3] = el[4]; lt[4] = el[2]; lt[5] = el[5];
+  lt[6] = el[6]; lt[7] = el[7]; lt[8] = el[8]; lt[9] = el[3];
   double tot = 0;
   for (auto v : el) tot += v;
   lt[10] = tot + lt[0];
   lt[11] = ms(t0, t_tree);
-  lt[12] = ms(t_lists0, t_lists1);
+  lt[12] = ms(t_lists0, t_lists1) + ms(t_lists2, t_lists3);
   return 0;
 }
 

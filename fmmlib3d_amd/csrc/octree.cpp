@@ -238,20 +238,25 @@ void tree_pass_expand(Octree& t, int nbox, TreePass& ps, std::vector<int>& next)
 // ---------------------------------------------------------------------------
 // Interaction lists from the box structure
 // ---------------------------------------------------------------------------
-void tree_lists(Octree& t) {
+namespace {
+// per-box scratch lists kept between calls (no allocation once grown); the
+// calling thread's copies, which the OpenMP workers reach through references
+struct ListScratch {
+  std::vector<std::vector<int>> coll, l1, l2, l3, l4;
+  std::vector<std::vector<std::pair<int, int>>> mirror;
+};
+thread_local ListScratch g_lists;
+}  // namespace
+
+void tree_lists_near(Octree& t) {
   const int nb = t.nboxes;
   const int nthreads = omp_get_max_threads();
-  // per-box scratch lists kept between calls (no allocation once grown); the
-  // calling thread's copies, bound to references that the OpenMP workers share
-  thread_local std::vector<std::vector<int>> tl_coll, tl_l1, tl_l2, tl_l3, tl_l4;
-  thread_local std::vector<std::vector<std::pair<int, int>>> tl_mirror;
-  auto& coll = tl_coll;
-  auto& l1 = tl_l1;
-  auto& l2 = tl_l2;
-  auto& l3 = tl_l3;
-  auto& l4 = tl_l4;
-  auto& mirror = tl_mirror;
-  for (auto* v : {&coll, &l1, &l2, &l3, &l4}) {
+  ListScratch& S = g_lists;
+  auto& coll = S.coll;
+  auto& l1 = S.l1;
+  auto& l3 = S.l3;
+  auto& mirror = S.mirror;
+  for (auto* v : {&S.coll, &S.l1, &S.l2, &S.l3, &S.l4}) {
     if ((int)v->size() < nb) v->resize(nb);
 #pragma omp parallel for schedule(static)
     for (int b = 0; b < nb; b++) (*v)[b].clear();
@@ -278,23 +283,6 @@ void tree_lists(Octree& t) {
       std::sort(coll[b].begin(), coll[b].end());
     }
   }
-  // list 2: children of the parent's colleagues (and of the parent) that do
-  // not touch the box (d3tstrcr.f:768-821)
-#pragma omp parallel for schedule(dynamic, 64)
-  for (int b = 1; b < nb; b++) {
-    int p = t.parent[b];
-    auto visit = [&](int q) {
-      for (int c = 0; c < t.nchild[q]; c++) {
-        int kid = t.child[8 * q + c];
-        if (kid == b) continue;
-        int d = 0;
-        for (int x = 0; x < 3; x++) d = std::max(d, std::abs(t.ijk[3 * kid + x] - t.ijk[3 * b + x]));
-        if (d > 1) l2[b].push_back(kid);
-      }
-    };
-    visit(p);
-    for (int q : coll[p]) visit(q);
-  }
   // lists 1 and 3 for leaves below the root (d3tlst31, d3tstrcr.f:912-1069);
   // the coarse-to-fine list-1 entries are mirrored afterwards.
 #pragma omp parallel for schedule(dynamic, 16)
@@ -320,12 +308,43 @@ void tree_lists(Octree& t) {
   }
   for (int th = 0; th < nthreads; th++)
     for (auto& pr : mirror[th]) l1[pr.first].push_back(pr.second);
-  for (int b = 0; b < nb; b++)
-    for (int j : l3[b]) l4[j].push_back(b);
   flatten(l1, nb, t.list[0]);
-  flatten(l2, nb, t.list[1]);
   flatten(l3, nb, t.list[2]);
+}
+
+void tree_lists_far(Octree& t) {
+  const int nb = t.nboxes;
+  ListScratch& S = g_lists;
+  auto& coll = S.coll;
+  auto& l2 = S.l2;
+  auto& l4 = S.l4;
+  // list 2: children of the parent's colleagues (and of the parent) that do
+  // not touch the box (d3tstrcr.f:768-821)
+#pragma omp parallel for schedule(dynamic, 64)
+  for (int b = 1; b < nb; b++) {
+    int p = t.parent[b];
+    auto visit = [&](int q) {
+      for (int c = 0; c < t.nchild[q]; c++) {
+        int kid = t.child[8 * q + c];
+        if (kid == b) continue;
+        int d = 0;
+        for (int x = 0; x < 3; x++) d = std::max(d, std::abs(t.ijk[3 * kid + x] - t.ijk[3 * b + x]));
+        if (d > 1) l2[b].push_back(kid);
+      }
+    };
+    visit(p);
+    for (int q : coll[p]) visit(q);
+  }
+  // list 4 is the transpose of list 3
+  for (int b = 0; b < nb; b++)
+    for (int k = t.list[2].off[b]; k < t.list[2].off[b + 1]; k++) l4[t.list[2].idx[k]].push_back(b);
+  flatten(l2, nb, t.list[1]);
   flatten(l4, nb, t.list[3]);
+}
+
+void tree_lists(Octree& t) {
+  tree_lists_near(t);
+  tree_lists_far(t);
 }
 
 // ---------------------------------------------------------------------------

@@ -78,5 +78,9 @@ inline uint32_t tree_eff_code(const TreePass& ps, int f, uint32_t code) {
 }
 // colleagues and lists 1-4 from the finished box structure
 void tree_lists(Octree& t);
+// the same in two parts on one thread: colleagues, lists 1 and 3; then lists 2
+// and 4 (needs the first part)
+void tree_lists_near(Octree& t);
+void tree_lists_far(Octree& t);
 
 }  // namespace fmm3d

@@ -91,11 +91,27 @@ inline void build_plan_boxes(const Octree& t, const Flags& f, Flat& F, PlanOffse
   O.inv_side = F.d(inv);
 }
 
-// Part B: lists and the work lists of the remaining steps (needs tree_lists).
-inline void build_plan_lists(const Octree& t, const Flags& f, Flat& F, PlanOffsets& O) {
+// Part B1: list 1 and the P2P work items (needs tree_lists_near).
+inline void build_plan_near(const Octree& t, const Flags& f, Flat& F, PlanOffsets& O) {
   const int nb = t.nboxes;
   const bool want_src = f.ifpot || f.iffld, want_trg = f.ifpottarg || f.iffldtarg;
   O.l1o = F.i(t.list[0].off); O.l1i = F.i(t.list[0].idx);
+  std::vector<int> p2p0, p2p1;
+  for (int b = 0; b < nb; b++) {
+    if (t.nchild[b] != 0) continue;
+    if (want_src)
+      for (int s = 0; s < t.nsrc[b]; s += kP2PBlock) { p2p0.push_back(b); p2p0.push_back(0); p2p0.push_back(s); }
+    if (want_trg)
+      for (int s = 0; s < t.ntrg[b]; s += kP2PBlock) { p2p1.push_back(b); p2p1.push_back(1); p2p1.push_back(s); }
+  }
+  O.p2p0 = F.i(p2p0); O.n_p2p0 = (int)p2p0.size() / 3;
+  O.p2p1 = F.i(p2p1); O.n_p2p1 = (int)p2p1.size() / 3;
+}
+
+// Part B2: lists 2-4 and the work lists of the far-field steps (needs all lists).
+inline void build_plan_far(const Octree& t, const Flags& f, Flat& F, PlanOffsets& O) {
+  const int nb = t.nboxes;
+  const bool want_src = f.ifpot || f.iffld, want_trg = f.ifpottarg || f.iffldtarg;
   // list 2 pruned as lfmm3drouts.f:385-387: sourceless sources, and
   // targetless targets when no source outputs are wanted
   {
@@ -119,7 +135,7 @@ inline void build_plan_lists(const Octree& t, const Flags& f, Flat& F, PlanOffse
   }
   O.l3o = F.i(t.list[2].off); O.l3i = F.i(t.list[2].idx);
   O.l4o = F.i(t.list[3].off); O.l4i = F.i(t.list[3].idx);
-  std::vector<int> l2p, m2p, p2l, p2p0, p2p1;
+  std::vector<int> l2p, m2p, p2l;
   for (int b = 0; b < nb; b++) {
     const bool leaf = t.nchild[b] == 0;
     const bool has_pts = (want_src && t.nsrc[b] > 0) || (want_trg && t.ntrg[b] > 0);
@@ -130,10 +146,6 @@ inline void build_plan_lists(const Octree& t, const Flags& f, Flat& F, PlanOffse
       for (int k = t.list[3].off[b]; k < t.list[3].off[b + 1]; k++) any |= t.nsrc[t.list[3].idx[k]] > 0;
       if (any) p2l.push_back(b);
     }
-    if (leaf && want_src)
-      for (int s = 0; s < t.nsrc[b]; s += kP2PBlock) { p2p0.push_back(b); p2p0.push_back(0); p2p0.push_back(s); }
-    if (leaf && want_trg)
-      for (int s = 0; s < t.ntrg[b]; s += kP2PBlock) { p2p1.push_back(b); p2p1.push_back(1); p2p1.push_back(s); }
   }
   O.l2p = F.i(l2p); O.n_l2p = (int)l2p.size();
   {
@@ -151,8 +163,6 @@ inline void build_plan_lists(const Octree& t, const Flags& f, Flat& F, PlanOffse
   }
   O.m2p = F.i(m2p); O.n_m2p = (int)m2p.size();
   O.p2l = F.i(p2l); O.n_p2l = (int)p2l.size();
-  O.p2p0 = F.i(p2p0); O.n_p2p0 = (int)p2p0.size() / 3;
-  O.p2p1 = F.i(p2p1); O.n_p2p1 = (int)p2p1.size() / 3;
   // L2L parents, levels 2 .. nlev-1 (lfmm3drouts.f:498-558)
   std::vector<int> l2l;
   O.l2l_groups.clear();
@@ -169,7 +179,12 @@ inline void build_plan_lists(const Octree& t, const Flags& f, Flat& F, PlanOffse
   O.l2l = F.i(l2l);
 }
 
-// Both parts into one flat buffer.
+inline void build_plan_lists(const Octree& t, const Flags& f, Flat& F, PlanOffsets& O) {
+  build_plan_near(t, f, F, O);
+  build_plan_far(t, f, F, O);
+}
+
+// All parts into one flat buffer.
 inline void build_plan(const Octree& t, const Flags& f, Flat& F, PlanOffsets& O, int p2m_min_level) {
   build_plan_boxes(t, f, F, O, p2m_min_level);
   build_plan_lists(t, f, F, O);
@@ -178,8 +193,12 @@ inline void build_plan(const Octree& t, const Flags& f, Flat& F, PlanOffsets& O,
 // Device plan over part A at (IA, DA) and part B at IB (IB = IA when one
 // flat holds both; IB may be null until part B is uploaded, see
 // devplan_lists).
-inline void devplan_lists(DevPlan& P, const PlanOffsets& O, const int* I) {
-  P.l1_off = I + O.l1o; P.l1_idx = I + O.l1i; P.l2_off = I + O.l2o; P.l2_idx = I + O.l2i;
+inline void devplan_near(DevPlan& P, const PlanOffsets& O, const int* I) {
+  P.l1_off = I + O.l1o; P.l1_idx = I + O.l1i;
+  P.p2p_items = I + O.p2p0; P.n_p2p = O.n_p2p0;
+}
+inline void devplan_far(DevPlan& P, const PlanOffsets& O, const int* I) {
+  P.l2_off = I + O.l2o; P.l2_idx = I + O.l2i;
   P.l3_off = I + O.l3o; P.l3_idx = I + O.l3i; P.l4_off = I + O.l4o; P.l4_idx = I + O.l4i;
   P.l2p_boxes = I + O.l2p; P.n_l2p = O.n_l2p;
   P.l2p_items = I + O.l2pi; P.n_l2pi = O.n_l2pi;
@@ -187,7 +206,10 @@ inline void devplan_lists(DevPlan& P, const PlanOffsets& O, const int* I) {
   P.p2l_boxes = I + O.p2l; P.n_p2l = O.n_p2l;
   P.m2l_boxes = I + O.m2l; P.n_m2l = O.n_m2l;
   P.l2l_boxes = I + O.l2l; P.l2l_lev_off = nullptr;
-  P.p2p_items = I + O.p2p0; P.n_p2p = O.n_p2p0;
+}
+inline void devplan_lists(DevPlan& P, const PlanOffsets& O, const int* I) {
+  devplan_near(P, O, I);
+  devplan_far(P, O, I);
 }
 
 inline DevPlan make_devplan(const Octree& t, int p, const PlanOffsets& O, const int* I, const double* D,
