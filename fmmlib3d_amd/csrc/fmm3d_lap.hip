@@ -239,7 +239,7 @@ int lap_solve(Context& ctx, const LapCall& c) {
       const size_t P2 = (size_t)(pt + 1) * (pt + 1);
       int ts = (int)(65536 / (2 * sizeof(T) * (size_t)H));
       ts = std::max(1, std::min(32, ts));
-      const size_t lds = 2 * sizeof(T) * std::max((size_t)ts * H, 4 * P2);
+      const size_t lds = form2_fac_bytes(p) + 2 * sizeof(T) * std::max((size_t)ts * H, 4 * P2);
 #define FORM2(C, D)                                                                                    \
   if (f.ifcharge == C && f.ifdipole == D)                                                              \
     hipLaunchKernelGGL((k_form2<T, IR, C, D>), dim3(nbox_form), dim3(256), lds, s, P, fac, spos, str64, ts, dst);

@@ -135,13 +135,16 @@ __global__ void k_form(DevPlan P, const double* __restrict__ fac, const double* 
 // (v_x -+ i v_y)/2 formed from S^{v_x}, S^{v_y} by linearity.
 // ---------------------------------------------------------------------------
 constexpr int kFormItems = 2;  // half-table entries per thread (256 threads)
+// LDS bytes of the recurrence factors k_form2 keeps (FacLayout MM..RB), 16-aligned
+__host__ __device__ inline size_t form2_fac_bytes(int p) {
+  return ((size_t)FacLayout(p).AR * sizeof(double) + 15) / 16 * 16;
+}
 
 template <typename T, bool IRREG, bool CHG, bool DIP>
 __global__ __launch_bounds__(256) void k_form2(DevPlan P, const double* __restrict__ fac,
                                                const double* __restrict__ spos, const double* __restrict__ str,
                                                int ts, T* __restrict__ expn) {
   extern __shared__ unsigned char smem_raw[];
-  T* tab = reinterpret_cast<T*>(smem_raw);
   const int p = P.p, K = P.K;
   const int pt = IRREG ? p + 1 : p;
   const int H = (pt + 1) * (pt + 2) / 2;
@@ -149,6 +152,11 @@ __global__ __launch_bounds__(256) void k_form2(DevPlan P, const double* __restri
   const int P2 = (pt + 1) * (pt + 1);
   FacLayout L(p);
   const int tid = threadIdx.x;
+  // the recurrence factors (MM, M1, RA, RB: the head of the factor table) in
+  // LDS, ahead of the harmonic table: each column task reads a different entry
+  double* facl = reinterpret_cast<double*>(smem_raw);
+  for (int k = tid; k < L.AR; k += blockDim.x) facl[k] = fac[k];
+  T* tab = reinterpret_cast<T*>(smem_raw + form2_fac_bytes(p));
   const int b = IRREG ? P.p2l_boxes[blockIdx.x] : P.p2m_boxes[blockIdx.x];
   const double ihd = P.inv_side[P.level[b]];
   const T ih = (T)ihd;
@@ -174,11 +182,11 @@ __global__ __launch_bounds__(256) void k_form2(DevPlan P, const double* __restri
         T* out = tab + 2 * (size_t)s * H;
         if (IRREG) {
           const T ir2 = T(1) / r2, ir = sqrt(ir2);
-          irregular_column<T>(fac, L, pt, a, ux, uy, uz, ir2, ir, out, 1);
-          if (pt - a != a) irregular_column<T>(fac, L, pt, pt - a, ux, uy, uz, ir2, ir, out, 1);
+          irregular_column<T>(facl, L, pt, a, ux, uy, uz, ir2, ir, out, 1);
+          if (pt - a != a) irregular_column<T>(facl, L, pt, pt - a, ux, uy, uz, ir2, ir, out, 1);
         } else {
-          regular_column<T>(fac, L, pt, a, ux, uy, uz, r2, out, 1);
-          if (pt - a != a) regular_column<T>(fac, L, pt, pt - a, ux, uy, uz, r2, out, 1);
+          regular_column<T>(facl, L, pt, a, ux, uy, uz, r2, out, 1);
+          if (pt - a != a) regular_column<T>(facl, L, pt, pt - a, ux, uy, uz, r2, out, 1);
         }
       }
       __syncthreads();
