@@ -782,22 +782,43 @@ __device__ __forceinline__ void grp_fma_row(T (&r)[16], const V4<T>& d, T xr, T 
 // Rotation rows 4rb..4rb+3 (input order m = row - n); xin: in_n^{-n}
 // ascending, yin: in_n^{+n} descending.  Rows past 2n read neighbouring
 // finite LDS values against zero table rows.
-template <typename T, bool BACK>
-__device__ __forceinline__ void grp_rot_batch(T (&r)[16], const V4<T> (&d)[4], int rb, const T* xin, const T* yin) {
+// The LDS inputs of four table rows, read one batch ahead like the tiles:
+// per row u (xr, xi, yr, yi).
+template <typename T>
+struct Rows {
+  T v[16];
+};
+
+// rotation rows 4rb..4rb+3 (input order m = row - n); xin: in_n^{-n}
+// ascending, yin: in_n^{+n} descending.  Rows past 2n read neighbouring
+// finite LDS values against zero table rows.
+template <typename T>
+__device__ __forceinline__ void grp_rot_rows(Rows<T>& x, int rb, const T* xin, const T* yin) {
 #pragma unroll
   for (int u = 0; u < 4; u++) {
     const int row = 4 * rb + u;
+    x.v[4 * u] = xin[2 * row];
+    x.v[4 * u + 1] = xin[2 * row + 1];
+    x.v[4 * u + 2] = yin[-2 * row];
+    x.v[4 * u + 3] = yin[-2 * row + 1];
+  }
+}
+
+template <typename T, bool BACK>
+__device__ __forceinline__ void grp_rows_fma(T (&r)[16], const V4<T> (&d)[4], const Rows<T>& x) {
+#pragma unroll
+  for (int u = 0; u < 4; u++) {
     if (BACK && (u & 1))
-      grp_fma_row<true>(r, d[u], xin[2 * row], xin[2 * row + 1], yin[-2 * row], yin[-2 * row + 1]);
+      grp_fma_row<true>(r, d[u], x.v[4 * u], x.v[4 * u + 1], x.v[4 * u + 2], x.v[4 * u + 3]);
     else
-      grp_fma_row<false>(r, d[u], xin[2 * row], xin[2 * row + 1], yin[-2 * row], yin[-2 * row + 1]);
+      grp_fma_row<false>(r, d[u], x.v[4 * u], x.v[4 * u + 1], x.v[4 * u + 2], x.v[4 * u + 3]);
   }
 }
 
 // out_{m'} = sum_m D[m][m'] in_m, out_{-m'} = sum_m D[m][m'] in_{-m} for the
 // four columns m' = 4q..4q+3 of degree n (tile column q at Db); BACK applies
 // the transpose, D^T[m][m'] = (-1)^{m+m'} D[m'][m].  Two register buffers: the
-// next tile is in flight while the current one is used.
+// next tile and its input rows are in flight while the current ones are used.
 template <typename T, bool BACK>
 __device__ __forceinline__ void grp_rot_item(const T* __restrict__ Db, int nq, int n, int q, const T* in,
                                              T (&r)[16]) {
@@ -806,15 +827,21 @@ __device__ __forceinline__ void grp_rot_item(const T* __restrict__ Db, int nq, i
   const int nrb = (2 * n) / 4 + 1;
   const size_t stride = (size_t)nq * 16;
   V4<T> d0[4], d1[4];
+  Rows<T> x0, x1;
   grp_tile(Db, d0);
+  grp_rot_rows(x0, 0, xin, yin);
   for (int rb = 0;;) {
-    grp_tile(Db + stride * (size_t)min(rb + 1, nrb - 1), d1);
+    const int nx = min(rb + 1, nrb - 1);
+    grp_tile(Db + stride * (size_t)nx, d1);
+    grp_rot_rows(x1, nx, xin, yin);
     __builtin_amdgcn_sched_barrier(0);
-    grp_rot_batch<T, BACK>(r, d0, rb, xin, yin);
+    grp_rows_fma<T, BACK>(r, d0, x0);
     if (++rb >= nrb) break;
-    grp_tile(Db + stride * (size_t)min(rb + 1, nrb - 1), d0);
+    const int ny = min(rb + 1, nrb - 1);
+    grp_tile(Db + stride * (size_t)ny, d0);
+    grp_rot_rows(x0, ny, xin, yin);
     __builtin_amdgcn_sched_barrier(0);
-    grp_rot_batch<T, BACK>(r, d1, rb, xin, yin);
+    grp_rows_fma<T, BACK>(r, d1, x1);
     if (++rb >= nrb) break;
   }
   if (BACK) {
@@ -840,16 +867,17 @@ __device__ __forceinline__ void grp_rot_store(int n, int mp0, const T (&r)[16], 
   }
 }
 
-// out_l^{+-am} = sum_{n=am}^{pt} Z[am][l][n] in_n^{+-am} for l = l0..l0+3;
-// Zm: rows n of order am in the padded transposed table (width W, column
-// l - am), so one 32-byte load fetches the four degrees of a work item.
+// z-shift rows n = n0..n0+3 of order am: (in_n^{am}, in_n^{-am})
 template <typename T>
-__device__ __forceinline__ void grp_z_batch(T (&r)[16], const V4<T> (&z)[4], int n0, int am, const T* in) {
+__device__ __forceinline__ void grp_z_rows(Rows<T>& x, int n0, int am, const T* in) {
 #pragma unroll
   for (int u = 0; u < 4; u++) {
     const int n = n0 + u;
-    const T* x = in + 2 * (n * n + n + am);
-    grp_fma_row<false>(r, z[u], x[0], x[1], x[-4 * am], x[-4 * am + 1]);
+    const T* xp = in + 2 * (n * n + n + am);
+    x.v[4 * u] = xp[0];
+    x.v[4 * u + 1] = xp[1];
+    x.v[4 * u + 2] = xp[-4 * am];
+    x.v[4 * u + 3] = xp[-4 * am + 1];
   }
 }
 
@@ -863,15 +891,21 @@ __device__ __forceinline__ void grp_z_item(const T* __restrict__ Zm, int nq, int
   const int nrb = (pt - am) / 4 + 1;
   const size_t stride = (size_t)nq * 16;
   V4<T> z0[4], z1[4];
+  Rows<T> x0, x1;
   grp_tile(Zq, z0);
+  grp_z_rows(x0, am, am, in);
   for (int rb = 0;;) {
-    grp_tile(Zq + stride * (size_t)min(rb + 1, nrb - 1), z1);
+    const int nx = min(rb + 1, nrb - 1);
+    grp_tile(Zq + stride * (size_t)nx, z1);
+    grp_z_rows(x1, am + 4 * nx, am, in);
     __builtin_amdgcn_sched_barrier(0);
-    grp_z_batch<T>(r, z0, am + 4 * rb, am, in);
+    grp_rows_fma<T, false>(r, z0, x0);
     if (++rb >= nrb) break;
-    grp_tile(Zq + stride * (size_t)min(rb + 1, nrb - 1), z0);
+    const int ny = min(rb + 1, nrb - 1);
+    grp_tile(Zq + stride * (size_t)ny, z0);
+    grp_z_rows(x0, am + 4 * ny, am, in);
     __builtin_amdgcn_sched_barrier(0);
-    grp_z_batch<T>(r, z1, am + 4 * rb, am, in);
+    grp_rows_fma<T, false>(r, z1, x1);
     if (++rb >= nrb) break;
   }
 }
