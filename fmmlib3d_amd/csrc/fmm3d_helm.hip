@@ -141,10 +141,9 @@ int helm_solve(Context& ctx, const LapCall& c) {
   int ts = (int)((ldsmax / sizeof(double) - 4 * (size_t)K) / per_src);
   ts = std::max(1, std::min(32, ts));
   const size_t lds_form = sizeof(double) * (4 * (size_t)K + (size_t)ts * per_src);
-  const size_t lds_eval = sizeof(double) * (2 * (size_t)K + 2 * 64 * (size_t)Fn);
   const size_t lds_ps = sizeof(double) * 4 * (size_t)K;
   const bool m2l_lds = sizeof(double) * 6 * (size_t)K <= ldsmax;
-  if (lds_eval > ldsmax || lds_ps > ldsmax) return 16;
+  if (lds_ps > ldsmax) return 16;
   Flat F;
   PlanOffsets O;
   build_plan(tree, f, F, O, 2);
@@ -232,29 +231,77 @@ int helm_solve(Context& ctx, const LapCall& c) {
     hipLaunchKernelGGL(k_hl2l, dim3(O.l2l_groups[g].second), dim3(256), lds_ps, s, P, Tb, Hd,
                        O.l2l_groups[g].first, HC->l2l_base + O.l2l_group_level[g], lo);
   tm.mark(6);
-  // STEP 7: L2P
-  const size_t Kx = (size_t)(pmax + 2) * (pmax + 2);
-  const size_t lds_eval2 = sizeof(double) * (8 * Kx + 2 * 64 * (size_t)Fn);
-  const bool eval2 = lds_eval2 <= ldsmax;
-  if (P.n_l2p) {
-    if (eval2)
-      hipLaunchKernelGGL((k_heval2<false>), dim3(P.n_l2p), dim3(64), lds_eval2, s, P, Hd, fac, lo, spos, tpos,
-                         f.ifpot, f.iffld, f.ifpottarg, f.iffldtarg, pot_s, fld_s, pot_t, fld_t);
-    else
-      hipLaunchKernelGGL((k_heval<false>), dim3(P.n_l2p), dim3(64), lds_eval, s, P, Hd, fac, lo, spos, tpos,
-                         f.ifpot, f.iffld, f.ifpottarg, f.iffldtarg, pot_s, fld_s, pot_t, fld_t);
+  // STEP 7: L2P and STEP 6: M2P (list 3), from packed expansions
+  {
+    const int grad = (f.iffld || f.iffldtarg) ? 1 : 0;
+    const bool want_src = f.ifpot || f.iffld, want_trg = f.ifpottarg || f.iffldtarg;
+    const int nbx = tree.nboxes;
+    // host side: boxes to pack, their entry offsets, the box -> offset maps and
+    // the work items (box, kind, first point) in chunks of 64 points
+    std::vector<int> hv;                  // m2p slots | l2p slots | m2p pack | l2p pack | items
+    std::vector<int> slot_m(nbx, -1), slot_l(nbx, -1), pk_m, off_m, pk_l, off_l, it_m, it_l;
+    int tot_m = 0, tot_l = 0, pdmax = 1;
+    auto pdof = [&](int lev) { return H.plev[lev] + grad; };
+    auto entries = [&](int lev) { const int pd = pdof(lev); return (pd + 1) * (pd + 2) / 2; };
+    auto items = [&](std::vector<int>& it, int b) {
+      if (want_src)
+        for (int q = 0; q < tree.nsrc[b]; q += 64) { it.push_back(b); it.push_back(0); it.push_back(q); }
+      if (want_trg)
+        for (int q = 0; q < tree.ntrg[b]; q += 64) { it.push_back(b); it.push_back(1); it.push_back(q); }
+    };
+    for (int b = 0; b < nbx; b++) {
+      const bool has_pts = (want_src && tree.nsrc[b] > 0) || (want_trg && tree.ntrg[b] > 0);
+      if (tree.nchild[b] == 0 && tree.level[b] >= 2 && has_pts) {
+        slot_l[b] = tot_l;
+        pk_l.push_back(b);
+        off_l.push_back(tot_l);
+        tot_l += entries(tree.level[b]);
+        pdmax = std::max(pdmax, pdof(tree.level[b]));
+        items(it_l, b);
+      }
+      if (tree.list[2].count(b) > 0 && has_pts) {
+        items(it_m, b);
+        for (int k = tree.list[2].off[b]; k < tree.list[2].off[b + 1]; k++) {
+          const int j = tree.list[2].idx[k];
+          if (tree.level[j] < 2 || tree.nsrc[j] == 0 || slot_m[j] >= 0) continue;
+          slot_m[j] = tot_m;
+          pk_m.push_back(j);
+          off_m.push_back(tot_m);
+          tot_m += entries(tree.level[j]);
+          pdmax = std::max(pdmax, pdof(tree.level[j]));
+        }
+      }
+    }
+    size_t o_sm = 0, o_sl = nbx, o_pm = 2 * (size_t)nbx, o_om = o_pm + pk_m.size(), o_pl = o_om + off_m.size(),
+           o_ol = o_pl + pk_l.size(), o_im = o_ol + off_l.size(), o_il = o_im + it_m.size();
+    hv.reserve(o_il + it_l.size());
+    for (auto* v : {&slot_m, &slot_l, &pk_m, &off_m, &pk_l, &off_l, &it_m, &it_l}) hv.insert(hv.end(), v->begin(), v->end());
+    (void)o_sm;
+    int* hh = (int*)ctx.arena.get_host("heval_i_h", std::max<size_t>(hv.size(), 1) * sizeof(int));
+    std::memcpy(hh, hv.data(), hv.size() * sizeof(int));
+    int* dh = (int*)ctx.arena.get("heval_i", std::max<size_t>(hv.size(), 1) * sizeof(int));
+    FMM_HIP(hipMemcpyAsync(dh, hh, hv.size() * sizeof(int), hipMemcpyHostToDevice, s));
+    const size_t lds3 = sizeof(double) * 2 * 64 * (size_t)(pdmax + 2);
+    if (lds3 > ldsmax) return 16;
+    if (!it_l.empty()) {
+      double* pack = (double*)ctx.arena.get("hpack_l", sizeof(double) * 16 * (size_t)std::max(tot_l, 1));
+      hipLaunchKernelGGL((k_hpack<false>), dim3((unsigned)pk_l.size()), dim3(256), 0, s, P, Hd, fac, lo, dh + o_pl,
+                         dh + o_ol, grad, pack);
+      hipLaunchKernelGGL((k_heval3<false>), dim3((unsigned)(it_l.size() / 3)), dim3(64), lds3, s, P, Hd, fac, pack,
+                         dh + o_sl, dh + o_il, grad, spos, tpos, f.ifpot, f.iffld, f.ifpottarg, f.iffldtarg, pot_s,
+                         fld_s, pot_t, fld_t);
+    }
+    tm.mark(7);
+    if (!it_m.empty() && !pk_m.empty()) {
+      double* pack = (double*)ctx.arena.get("hpack_m", sizeof(double) * 16 * (size_t)std::max(tot_m, 1));
+      hipLaunchKernelGGL((k_hpack<true>), dim3((unsigned)pk_m.size()), dim3(256), 0, s, P, Hd, fac, mp, dh + o_pm,
+                         dh + o_om, grad, pack);
+      hipLaunchKernelGGL((k_heval3<true>), dim3((unsigned)(it_m.size() / 3)), dim3(64), lds3, s, P, Hd, fac, pack,
+                         dh + o_sm, dh + o_im, grad, spos, tpos, f.ifpot, f.iffld, f.ifpottarg, f.iffldtarg, pot_s,
+                         fld_s, pot_t, fld_t);
+    }
+    tm.mark(8);
   }
-  tm.mark(7);
-  // STEP 6: M2P (list 3)
-  if (P.n_m2p) {
-    if (eval2)
-      hipLaunchKernelGGL((k_heval2<true>), dim3(P.n_m2p), dim3(64), lds_eval2, s, P, Hd, fac, mp, spos, tpos,
-                         f.ifpot, f.iffld, f.ifpottarg, f.iffldtarg, pot_s, fld_s, pot_t, fld_t);
-    else
-      hipLaunchKernelGGL((k_heval<true>), dim3(P.n_m2p), dim3(64), lds_eval, s, P, Hd, fac, mp, spos, tpos,
-                         f.ifpot, f.iffld, f.ifpottarg, f.iffldtarg, pot_s, fld_s, pot_t, fld_t);
-  }
-  tm.mark(8);
   // STEP 8: P2P
   if (f.ifcharge || f.ifdipole) {
     launch_hp2p<TN>(P, dI + O.p2p0, O.n_p2p0, 0, Hd.kre, Hd.kim, spos, strn, tpos, f.ifcharge, f.ifdipole,

@@ -535,6 +535,180 @@ __global__ __launch_bounds__(64) void k_heval2(DevPlan P, HelmDev Hd, const doub
 }
 
 // ---------------------------------------------------------------------------
+// M2P / L2P from packed expansions.  For every box whose expansion is
+// evaluated, the expansion Ew = sqrt(2n+1) E and the three gradient
+// expansions Gz, Gu, Gd of k_heval2 are stored per half-table entry (n, a >= 0)
+// in the order of the evaluation walk (column a, then n = a..pd): 8 complex
+//   Ew(n,a) Ew(n,-a) Gz(n,a) Gz(n,-a) Gu(n,a) Gu(n,-a) Gd(n,a) Gd(n,-a)
+// (the -a slots zero for a = 0; Gz, Gu, Gd zero without the gradient).  The
+// evaluation kernel reads them with wave-uniform loads, one thread per point,
+// the point's radial series in its own LDS column.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ int hwalk_index(int pd, int a, int n) { return a * (pd + 1) - a * (a - 1) / 2 + (n - a); }
+
+template <bool IRREG>
+__global__ void k_hpack(DevPlan P, HelmDev Hd, const double* __restrict__ fac, const double* __restrict__ expn,
+                        const int* __restrict__ boxes, const int* __restrict__ offs, int grad,
+                        double* __restrict__ pack) {
+  const int j = boxes[blockIdx.x];
+  const int lev = P.level[j];
+  const int p = Hd.plev[lev];
+  const int pd = grad ? p + 1 : p;
+  const int H = (pd + 1) * (pd + 2) / 2;
+  FacLayout L(P.p);
+  const double sc = Hd.sclev[lev];
+  const double lo = IRREG ? sc : 1.0 / sc, hi = IRREG ? 1.0 / sc : sc;
+  const Cd kk = {Hd.kre, Hd.kim};
+  const double* E = expn + 2 * (size_t)j * P.K;
+  double* out = pack + 16 * (size_t)offs[blockIdx.x];
+  auto ew = [&](int n1, int m1) -> Cd {
+    if (n1 < 0 || n1 > p || m1 > n1 || m1 < -n1) return Cd{0.0, 0.0};
+    return cscale(Cd{E[2 * eidx(n1, m1)], E[2 * eidx(n1, m1) + 1]}, fac[L.M1 + n1]);
+  };
+  auto ek = [&](int n1, int m1) -> Cd {   // k Ew(n1, m1) / (2 n1 + 1)
+    if (n1 < 0 || n1 > p || m1 > n1 || m1 < -n1) return Cd{0.0, 0.0};
+    return cmul(kk, cscale(ew(n1, m1), 1.0 / (2.0 * n1 + 1.0)));
+  };
+  auto fa = [&](int off, int n1, int m1) -> double {
+    if (n1 < 0 || n1 > p || m1 > n1 || m1 < -n1) return 0.0;
+    return fac[off + eidx(n1, m1)];
+  };
+  for (int w = threadIdx.x; w < H; w += blockDim.x) {
+    int a = 0;
+    while (a < pd && hwalk_index(pd, a + 1, a + 1) <= w) a++;
+    const int n = a + (w - hwalk_index(pd, a, a));
+    double v[16];
+#pragma unroll
+    for (int e = 0; e < 16; e++) v[e] = 0.0;
+    for (int sg = 0; sg < (a > 0 ? 2 : 1); sg++) {
+      const int m = sg ? -a : a;
+      const Cd e0 = ew(n, m);
+      v[2 * sg] = e0.re; v[2 * sg + 1] = e0.im;
+      if (grad) {
+        const Cd gz = cadd2(cscale(ek(n + 1, m), fa(L.AR, n + 1, m) * lo), cscale(ek(n - 1, m), fa(L.AS, n - 1, m) * hi));
+        const Cd gu = cadd2(cscale(ek(n + 1, m - 1), fa(L.BR, n + 1, m - 1) * lo),
+                            cscale(ek(n - 1, m - 1), fa(L.BS, n - 1, m - 1) * hi));
+        const Cd gd = cadd2(cscale(ek(n + 1, m + 1), fa(L.CR, n + 1, m + 1) * lo),
+                            cscale(ek(n - 1, m + 1), fa(L.CS, n - 1, m + 1) * hi));
+        v[4 + 2 * sg] = gz.re; v[5 + 2 * sg] = gz.im;
+        v[8 + 2 * sg] = gu.re; v[9 + 2 * sg] = gu.im;
+        v[12 + 2 * sg] = gd.re; v[13 + 2 * sg] = gd.im;
+      }
+    }
+    double* o = out + 16 * (size_t)w;
+#pragma unroll
+    for (int e = 0; e < 16; e++) o[e] = v[e];
+  }
+}
+
+// out += c+ B + c- Bm
+__device__ __forceinline__ void hpack_fma(Cd& out, const double* c, const Cd& B, const Cd& Bm) {
+  out.re = fma(c[0], B.re, fma(-c[1], B.im, fma(c[2], Bm.re, fma(-c[3], Bm.im, out.re))));
+  out.im = fma(c[0], B.im, fma(c[1], B.re, fma(c[2], Bm.im, fma(c[3], Bm.re, out.im))));
+}
+
+// One 64-thread workgroup per work item (box, kind, first point): M2P over
+// list 3 of the box (IRREG) or L2P of the box's own local expansion.  All
+// sums stay in registers until the single store at the end.
+template <bool IRREG>
+__global__ __launch_bounds__(64) void k_heval3(DevPlan P, HelmDev Hd, const double* __restrict__ fac,
+                                               const double* __restrict__ pack, const int* __restrict__ slot,
+                                               const int* __restrict__ items, int grad,
+                                               const double* __restrict__ spos, const double* __restrict__ tpos,
+                                               int wpot0, int wfld0, int wpot1, int wfld1,
+                                               double* __restrict__ pot0, double* __restrict__ fld0,
+                                               double* __restrict__ pot1, double* __restrict__ fld1) {
+  constexpr int NT = 64;
+  extern __shared__ unsigned char smem_raw[];
+  double* fcol = reinterpret_cast<double*>(smem_raw) + 2 * threadIdx.x;  // stride NT complex
+  FacLayout L(P.p);
+  const int b = items[3 * blockIdx.x], kind = items[3 * blockIdx.x + 1], start = items[3 * blockIdx.x + 2];
+  const int wp = kind ? wpot1 : wpot0, wf = kind ? wfld1 : wfld0;
+  const int n0 = kind ? P.itrg[b] : P.isrc[b];
+  const int nn = kind ? P.ntrg[b] : P.nsrc[b];
+  const int i = start + threadIdx.x;
+  const bool active = i < nn;
+  double x = 0.0, y = 0.0, z = 0.0;
+  if (active) {
+    const double* q = (kind ? tpos : spos) + 4 * (size_t)(n0 + i);
+    x = q[0]; y = q[1]; z = q[2];
+  }
+  Cd pot = {0.0, 0.0}, gz = {0.0, 0.0}, gu = {0.0, 0.0}, gd = {0.0, 0.0};
+  int jbeg = 0, jend = 1;
+  if (IRREG) { jbeg = P.l3_off[b]; jend = P.l3_off[b + 1]; }
+  for (int jj = jbeg; jj < jend; jj++) {
+    const int j = IRREG ? P.l3_idx[jj] : b;
+    const int lev = P.level[j];
+    if (lev < 2) continue;
+    if (IRREG && P.nsrc[j] == 0) continue;
+    const int p = Hd.plev[lev];
+    const int pd = grad ? p + 1 : p;
+    const double sc = Hd.sclev[lev];
+    const double* pk = pack + 16 * (size_t)slot[j];
+    const double ux0 = x - P.center[3 * j], uy0 = y - P.center[3 * j + 1], uz0 = z - P.center[3 * j + 2];
+    const double rr = sqrt(ux0 * ux0 + uy0 * uy0 + uz0 * uz0);
+    const Cd zz = {Hd.kre * rr, Hd.kim * rr};
+    // h3dmpeval_trunc0 returns early for a point at the centre
+    const bool skip = !active || (IRREG && cnorm(zz) < 1.0e-30);
+    const double ir = rr > 0.0 ? 1.0 / rr : 0.0;
+    const double ux = rr > 0.0 ? ux0 * ir : 0.0, uy = rr > 0.0 ? uy0 * ir : 0.0, uz = rr > 0.0 ? uz0 * ir : 1.0;
+    if (IRREG) dev_hankel(pd, zz, sc, reinterpret_cast<Cd*>(fcol), NT);
+    else dev_bessel(max(pd, 1), zz, sc, reinterpret_cast<Cd*>(fcol), NT);
+    Cd tp = {0.0, 0.0}, tz = {0.0, 0.0}, tu = {0.0, 0.0}, td = {0.0, 0.0};
+    Cd ymm = {1.0, 0.0};
+    const double* c = pk;
+    for (int m = 0; m <= pd; m++) {
+      if (m > 0) {
+        const double fm = fac[L.MM + m];
+        ymm = cmul(Cd{-ux * fm, -uy * fm}, ymm);
+      }
+      Cd y0 = ymm, y1 = ymm;
+      for (int n = m; n <= pd; n++, c += 16) {
+        Cd yv;
+        if (n == m) {
+          yv = ymm;
+        } else if (n == m + 1) {
+          yv = cscale(ymm, fac[L.M1 + m] * uz);
+          y1 = yv;
+        } else {
+          const double aa = (2.0 * n - 1.0) * uz, bq = fac[L.RA + hidx(n, m)], cc = fac[L.RB + hidx(n, m)];
+          yv = {(aa * y1.re - bq * y0.re) * cc, (aa * y1.im - bq * y0.im) * cc};
+          y0 = y1;
+          y1 = yv;
+        }
+        const Cd fn = {fcol[2 * (size_t)NT * n], fcol[2 * (size_t)NT * n + 1]};
+        const Cd B = {fn.re * yv.re - fn.im * yv.im, fn.re * yv.im + fn.im * yv.re};    // f_n Y_n^m
+        const Cd Bm = {fn.re * yv.re + fn.im * yv.im, fn.im * yv.re - fn.re * yv.im};   // f_n Y_n^{-m}
+        hpack_fma(tp, c, B, Bm);
+        if (grad) {
+          hpack_fma(tz, c + 4, B, Bm);
+          hpack_fma(tu, c + 8, B, Bm);
+          hpack_fma(td, c + 12, B, Bm);
+        }
+      }
+    }
+    if (!skip) {
+      pot = cadd2(pot, tp);
+      gz = cadd2(gz, tz);
+      gu = cadd2(gu, tu);
+      gd = cadd2(gd, td);
+    }
+  }
+  if (!active) return;
+  double* po = kind ? pot1 : pot0;
+  double* fo = kind ? fld1 : fld0;
+  const size_t id = (size_t)(n0 + i);
+  if (wp) { po[2 * id] += pot.re; po[2 * id + 1] += pot.im; }
+  if (wf) {
+    const Cd dx = {0.5 * (gu.re + gd.re), 0.5 * (gu.im + gd.im)};
+    const Cd dy = {0.5 * (gu.im - gd.im), -0.5 * (gu.re - gd.re)};
+    fo[6 * id + 0] -= dx.re; fo[6 * id + 1] -= dx.im;
+    fo[6 * id + 2] -= dy.re; fo[6 * id + 3] -= dy.im;
+    fo[6 * id + 4] -= gz.re; fo[6 * id + 5] -= gz.im;
+  }
+}
+
+// ---------------------------------------------------------------------------
 // Translations: phase, rotate, coaxial shift (complex packed block), rotate
 // back, phase back (src/h3dtrans.f:211, :1229, :1998 apply the same
 // operators by projection).  `acc` accumulates in LDS or global memory.
