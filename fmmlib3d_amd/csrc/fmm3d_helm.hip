@@ -296,7 +296,7 @@ int helm_solve(Context& ctx, const LapCall& c) {
       double* pack = (double*)ctx.arena.get("hpack_m", sizeof(double) * 16 * (size_t)std::max(tot_m, 1));
       hipLaunchKernelGGL((k_hpack<true>), dim3((unsigned)pk_m.size()), dim3(256), 0, s, P, Hd, fac, mp, dh + o_pm,
                          dh + o_om, grad, pack);
-      hipLaunchKernelGGL((k_heval3<true>), dim3((unsigned)(it_m.size() / 3)), dim3(64), lds3, s, P, Hd, fac, pack,
+      hipLaunchKernelGGL((k_heval3<true>), dim3((unsigned)(it_m.size() / 3)), dim3(64), 0, s, P, Hd, fac, pack,
                          dh + o_sm, dh + o_im, grad, spos, tpos, f.ifpot, f.iffld, f.ifpottarg, f.iffldtarg, pot_s,
                          fld_s, pot_t, fld_t);
     }

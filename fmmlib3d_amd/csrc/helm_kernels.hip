@@ -652,8 +652,24 @@ __global__ __launch_bounds__(64) void k_heval3(DevPlan P, HelmDev Hd, const doub
     const bool skip = !active || (IRREG && cnorm(zz) < 1.0e-30);
     const double ir = rr > 0.0 ? 1.0 / rr : 0.0;
     const double ux = rr > 0.0 ? ux0 * ir : 0.0, uy = rr > 0.0 ? uy0 * ir : 0.0, uz = rr > 0.0 ? uz0 * ir : 1.0;
-    if (IRREG) dev_hankel(pd, zz, sc, reinterpret_cast<Cd*>(fcol), NT);
-    else dev_bessel(max(pd, 1), zz, sc, reinterpret_cast<Cd*>(fcol), NT);
+    // radial factors: M2P runs the upward recurrence of h_n(kr) sc^n
+    // (dev_hankel) along each column from the column's first two values, so
+    // nothing is stored; L2P keeps j_n/sc^n (backward recurrence) in LDS
+    Cd cs0 = {0.0, 0.0}, cs1 = {0.0, 0.0}, zinv = {0.0, 0.0};
+    const double sc2 = sc * sc;
+    if (IRREG) {
+      if (!skip) {
+        double sn, cn;
+        sincos(zz.re, &sn, &cn);
+        const double e = exp(-zz.im);
+        cs0 = cdiv(Cd{e * cn, e * sn}, Cd{-zz.im, zz.re});   // e^{iz}/(iz)
+        const Cd zi = cdiv(Cd{1.0, 0.0}, zz);
+        cs1 = cscale(cmul(cs0, Cd{zi.re, zi.im - 1.0}), sc);
+        zinv = cscale(zi, sc);
+      }
+    } else {
+      dev_bessel(max(pd, 1), zz, sc, reinterpret_cast<Cd*>(fcol), NT);
+    }
     Cd tp = {0.0, 0.0}, tz = {0.0, 0.0}, tu = {0.0, 0.0}, td = {0.0, 0.0};
     Cd ymm = {1.0, 0.0};
     const double* c = pk;
@@ -663,6 +679,15 @@ __global__ __launch_bounds__(64) void k_heval3(DevPlan P, HelmDev Hd, const doub
         ymm = cmul(Cd{-ux * fm, -uy * fm}, ymm);
       }
       Cd y0 = ymm, y1 = ymm;
+      Cd hc = cs0, hn1 = cs1;  // h_n, h_{n+1} along the column
+      if (IRREG && m > 0) {
+        // advance the column start: (h_m, h_{m+1}) from (h_{m-1}, h_m)
+        const Cd hnew = csub2(cmul(cscale(zinv, 2.0 * m + 1.0), cs1), cscale(cs0, sc2));
+        cs0 = cs1;
+        cs1 = hnew;
+        hc = cs0;
+        hn1 = cs1;
+      }
       for (int n = m; n <= pd; n++, c += 16) {
         Cd yv;
         if (n == m) {
@@ -676,7 +701,15 @@ __global__ __launch_bounds__(64) void k_heval3(DevPlan P, HelmDev Hd, const doub
           y0 = y1;
           y1 = yv;
         }
-        const Cd fn = {fcol[2 * (size_t)NT * n], fcol[2 * (size_t)NT * n + 1]};
+        Cd fn;
+        if (IRREG) {
+          fn = hc;
+          const Cd hnew = csub2(cmul(cscale(zinv, 2.0 * n + 3.0), hn1), cscale(hc, sc2));
+          hc = hn1;
+          hn1 = hnew;
+        } else {
+          fn = {fcol[2 * (size_t)NT * n], fcol[2 * (size_t)NT * n + 1]};
+        }
         const Cd B = {fn.re * yv.re - fn.im * yv.im, fn.re * yv.im + fn.im * yv.re};    // f_n Y_n^m
         const Cd Bm = {fn.re * yv.re + fn.im * yv.im, fn.im * yv.re - fn.re * yv.im};   // f_n Y_n^{-m}
         hpack_fma(tp, c, B, Bm);
