@@ -239,7 +239,7 @@ int helm_solve(Context& ctx, const LapCall& c) {
     // host side: boxes to pack, their entry offsets, the box -> offset maps and
     // the work items (box, kind, first point) in chunks of 64 points
     std::vector<int> hv;                  // m2p slots | l2p slots | m2p pack | l2p pack | items
-    std::vector<int> slot_m(nbx, -1), slot_l(nbx, -1), pk_m, off_m, pk_l, off_l, it_m, it_l;
+    std::vector<int> slot_m(nbx, -1), slot_l(nbx, -1), pk_m, off_m, pk_l, off_l, it_m, it_l, red_m;
     int tot_m = 0, tot_l = 0, pdmax = 1;
     auto pdof = [&](int lev) { return H.plev[lev] + grad; };
     auto entries = [&](int lev) { const int pd = pdof(lev); return (pd + 1) * (pd + 2) / 2; };
@@ -260,22 +260,45 @@ int helm_solve(Context& ctx, const LapCall& c) {
         items(it_l, b);
       }
       if (tree.list[2].count(b) > 0 && has_pts) {
-        items(it_m, b);
+        // list 3 in slices of about kSlice half-table entries: one workgroup per
+        // (64 points, slice), the slices summed in order by k_hreduce
+        constexpr int kSlice = 1024;
+        std::vector<int> cuts = {tree.list[2].off[b]};
+        int cost = 0;
         for (int k = tree.list[2].off[b]; k < tree.list[2].off[b + 1]; k++) {
           const int j = tree.list[2].idx[k];
-          if (tree.level[j] < 2 || tree.nsrc[j] == 0 || slot_m[j] >= 0) continue;
+          if (tree.level[j] < 2 || tree.nsrc[j] == 0) continue;
+          cost += entries(tree.level[j]);
+          if (cost >= kSlice && k + 1 < tree.list[2].off[b + 1]) { cuts.push_back(k + 1); cost = 0; }
+          if (slot_m[j] >= 0) continue;
           slot_m[j] = tot_m;
           pk_m.push_back(j);
           off_m.push_back(tot_m);
           tot_m += entries(tree.level[j]);
           pdmax = std::max(pdmax, pdof(tree.level[j]));
         }
+        cuts.push_back(tree.list[2].off[b + 1]);
+        const int nsl = (int)cuts.size() - 1;
+        auto slices = [&](int kind, int npts) {
+          for (int q = 0; q < npts; q += 64) {
+            red_m.push_back(b); red_m.push_back(kind); red_m.push_back(q);
+            red_m.push_back((int)(it_m.size() / 5)); red_m.push_back(nsl);
+            for (int c = 0; c < nsl; c++) {
+              it_m.push_back(b); it_m.push_back(kind); it_m.push_back(q);
+              it_m.push_back(cuts[c]); it_m.push_back(cuts[c + 1]);
+            }
+          }
+        };
+        if (want_src) slices(0, tree.nsrc[b]);
+        if (want_trg) slices(1, tree.ntrg[b]);
       }
     }
     size_t o_sm = 0, o_sl = nbx, o_pm = 2 * (size_t)nbx, o_om = o_pm + pk_m.size(), o_pl = o_om + off_m.size(),
-           o_ol = o_pl + pk_l.size(), o_im = o_ol + off_l.size(), o_il = o_im + it_m.size();
-    hv.reserve(o_il + it_l.size());
-    for (auto* v : {&slot_m, &slot_l, &pk_m, &off_m, &pk_l, &off_l, &it_m, &it_l}) hv.insert(hv.end(), v->begin(), v->end());
+           o_ol = o_pl + pk_l.size(), o_im = o_ol + off_l.size(), o_il = o_im + it_m.size(),
+           o_rm = o_il + it_l.size();
+    hv.reserve(o_rm + red_m.size());
+    for (auto* v : {&slot_m, &slot_l, &pk_m, &off_m, &pk_l, &off_l, &it_m, &it_l, &red_m})
+      hv.insert(hv.end(), v->begin(), v->end());
     (void)o_sm;
     int* hh = (int*)ctx.arena.get_host("heval_i_h", std::max<size_t>(hv.size(), 1) * sizeof(int));
     std::memcpy(hh, hv.data(), hv.size() * sizeof(int));
@@ -289,16 +312,20 @@ int helm_solve(Context& ctx, const LapCall& c) {
                          dh + o_ol, grad, pack);
       hipLaunchKernelGGL((k_heval3<false>), dim3((unsigned)(it_l.size() / 3)), dim3(64), lds3, s, P, Hd, fac, pack,
                          dh + o_sl, dh + o_il, grad, spos, tpos, f.ifpot, f.iffld, f.ifpottarg, f.iffldtarg, pot_s,
-                         fld_s, pot_t, fld_t);
+                         fld_s, pot_t, fld_t, (double*)nullptr);
     }
     tm.mark(7);
     if (!it_m.empty() && !pk_m.empty()) {
       double* pack = (double*)ctx.arena.get("hpack_m", sizeof(double) * 16 * (size_t)std::max(tot_m, 1));
       hipLaunchKernelGGL((k_hpack<true>), dim3((unsigned)pk_m.size()), dim3(256), 0, s, P, Hd, fac, mp, dh + o_pm,
                          dh + o_om, grad, pack);
-      hipLaunchKernelGGL((k_heval3<true>), dim3((unsigned)(it_m.size() / 3)), dim3(64), 0, s, P, Hd, fac, pack,
-                         dh + o_sm, dh + o_im, grad, spos, tpos, f.ifpot, f.iffld, f.ifpottarg, f.iffldtarg, pot_s,
-                         fld_s, pot_t, fld_t);
+      const size_t nsub = it_m.size() / 5;
+      double* partial = (double*)ctx.arena.get("hpart_m", sizeof(double) * 8 * 64 * nsub);
+      hipLaunchKernelGGL((k_heval3<true>), dim3((unsigned)nsub), dim3(64), 0, s, P, Hd, fac, pack, dh + o_sm,
+                         dh + o_im, grad, spos, tpos, f.ifpot, f.iffld, f.ifpottarg, f.iffldtarg, pot_s, fld_s, pot_t,
+                         fld_t, partial);
+      hipLaunchKernelGGL(k_hreduce, dim3((unsigned)(red_m.size() / 5)), dim3(64), 0, s, P, dh + o_rm, partial,
+                         f.ifpot, f.iffld, f.ifpottarg, f.iffldtarg, pot_s, fld_s, pot_t, fld_t);
     }
     tm.mark(8);
   }

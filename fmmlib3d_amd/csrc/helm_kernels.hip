@@ -607,9 +607,12 @@ __device__ __forceinline__ void hpack_fma(Cd& out, const double* c, const Cd& B,
   out.im = fma(c[0], B.im, fma(c[1], B.re, fma(c[2], Bm.im, fma(c[3], Bm.re, out.im))));
 }
 
-// One 64-thread workgroup per work item (box, kind, first point): M2P over
-// list 3 of the box (IRREG) or L2P of the box's own local expansion.  All
-// sums stay in registers until the single store at the end.
+// One 64-thread workgroup per work item.  L2P (IRREG = false): item (box,
+// kind, first point), the box's own local expansion, added to the outputs.
+// M2P (IRREG = true): item (box, kind, first point, first and last+1 index
+// into the box's list 3), a slice of list 3 whose sums go to `partial`
+// (8 values per point), added to the outputs in slice order by k_hreduce.
+// All sums stay in registers until the single store at the end.
 template <bool IRREG>
 __global__ __launch_bounds__(64) void k_heval3(DevPlan P, HelmDev Hd, const double* __restrict__ fac,
                                                const double* __restrict__ pack, const int* __restrict__ slot,
@@ -617,12 +620,14 @@ __global__ __launch_bounds__(64) void k_heval3(DevPlan P, HelmDev Hd, const doub
                                                const double* __restrict__ spos, const double* __restrict__ tpos,
                                                int wpot0, int wfld0, int wpot1, int wfld1,
                                                double* __restrict__ pot0, double* __restrict__ fld0,
-                                               double* __restrict__ pot1, double* __restrict__ fld1) {
+                                               double* __restrict__ pot1, double* __restrict__ fld1,
+                                               double* __restrict__ partial) {
   constexpr int NT = 64;
+  constexpr int IS = IRREG ? 5 : 3;
   extern __shared__ unsigned char smem_raw[];
   double* fcol = reinterpret_cast<double*>(smem_raw) + 2 * threadIdx.x;  // stride NT complex
   FacLayout L(P.p);
-  const int b = items[3 * blockIdx.x], kind = items[3 * blockIdx.x + 1], start = items[3 * blockIdx.x + 2];
+  const int b = items[IS * blockIdx.x], kind = items[IS * blockIdx.x + 1], start = items[IS * blockIdx.x + 2];
   const int wp = kind ? wpot1 : wpot0, wf = kind ? wfld1 : wfld0;
   const int n0 = kind ? P.itrg[b] : P.isrc[b];
   const int nn = kind ? P.ntrg[b] : P.nsrc[b];
@@ -635,7 +640,7 @@ __global__ __launch_bounds__(64) void k_heval3(DevPlan P, HelmDev Hd, const doub
   }
   Cd pot = {0.0, 0.0}, gz = {0.0, 0.0}, gu = {0.0, 0.0}, gd = {0.0, 0.0};
   int jbeg = 0, jend = 1;
-  if (IRREG) { jbeg = P.l3_off[b]; jend = P.l3_off[b + 1]; }
+  if (IRREG) { jbeg = items[IS * blockIdx.x + 3]; jend = items[IS * blockIdx.x + 4]; }
   for (int jj = jbeg; jj < jend; jj++) {
     const int j = IRREG ? P.l3_idx[jj] : b;
     const int lev = P.level[j];
@@ -727,12 +732,51 @@ __global__ __launch_bounds__(64) void k_heval3(DevPlan P, HelmDev Hd, const doub
       gd = cadd2(gd, td);
     }
   }
+  if (IRREG) {
+    double* o = partial + 8 * ((size_t)blockIdx.x * NT + threadIdx.x);
+    o[0] = pot.re; o[1] = pot.im; o[2] = gz.re; o[3] = gz.im;
+    o[4] = gu.re; o[5] = gu.im; o[6] = gd.re; o[7] = gd.im;
+    return;
+  }
   if (!active) return;
   double* po = kind ? pot1 : pot0;
   double* fo = kind ? fld1 : fld0;
   const size_t id = (size_t)(n0 + i);
   if (wp) { po[2 * id] += pot.re; po[2 * id + 1] += pot.im; }
   if (wf) {
+    const Cd dx = {0.5 * (gu.re + gd.re), 0.5 * (gu.im + gd.im)};
+    const Cd dy = {0.5 * (gu.im - gd.im), -0.5 * (gu.re - gd.re)};
+    fo[6 * id + 0] -= dx.re; fo[6 * id + 1] -= dx.im;
+    fo[6 * id + 2] -= dy.re; fo[6 * id + 3] -= dy.im;
+    fo[6 * id + 4] -= gz.re; fo[6 * id + 5] -= gz.im;
+  }
+}
+
+// M2P sums of the list-3 slices of one item (box, kind, first point, first
+// slice, number of slices), added in slice order to the outputs.
+__global__ __launch_bounds__(64) void k_hreduce(DevPlan P, const int* __restrict__ red, const double* __restrict__ partial,
+                                                int wpot0, int wfld0, int wpot1, int wfld1, double* __restrict__ pot0,
+                                                double* __restrict__ fld0, double* __restrict__ pot1,
+                                                double* __restrict__ fld1) {
+  const int* r = red + 5 * blockIdx.x;
+  const int b = r[0], kind = r[1], start = r[2], s0 = r[3], ns = r[4];
+  const int wp = kind ? wpot1 : wpot0, wf = kind ? wfld1 : wfld0;
+  const int n0 = kind ? P.itrg[b] : P.isrc[b];
+  const int nn = kind ? P.ntrg[b] : P.nsrc[b];
+  const int i = start + threadIdx.x;
+  if (i >= nn) return;
+  double v[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  for (int c = 0; c < ns; c++) {
+    const double* o = partial + 8 * ((size_t)(s0 + c) * 64 + threadIdx.x);
+#pragma unroll
+    for (int e = 0; e < 8; e++) v[e] += o[e];
+  }
+  double* po = kind ? pot1 : pot0;
+  double* fo = kind ? fld1 : fld0;
+  const size_t id = (size_t)(n0 + i);
+  if (wp) { po[2 * id] += v[0]; po[2 * id + 1] += v[1]; }
+  if (wf) {
+    const Cd gz = {v[2], v[3]}, gu = {v[4], v[5]}, gd = {v[6], v[7]};
     const Cd dx = {0.5 * (gu.re + gd.re), 0.5 * (gu.im + gd.im)};
     const Cd dy = {0.5 * (gu.im - gd.im), -0.5 * (gu.re - gd.re)};
     fo[6 * id + 0] -= dx.re; fo[6 * id + 1] -= dx.im;
