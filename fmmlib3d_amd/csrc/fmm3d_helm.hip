@@ -10,6 +10,7 @@
 #include <cmath>
 #include <cstring>
 #include <memory>
+#include <type_traits>
 #include <vector>
 
 #include "fmm3d_internal.hpp"
@@ -141,6 +142,15 @@ int helm_solve(Context& ctx, const LapCall& c) {
   int ts = (int)((ldsmax / sizeof(double) - 4 * (size_t)K) / per_src);
   ts = std::max(1, std::min(32, ts));
   const size_t lds_form = sizeof(double) * (4 * (size_t)K + (size_t)ts * per_src);
+  // per-thread-sum P2M/P2L: half tables up to degree pmax+1 held by 256
+  // threads x kHFormItems entries, outputs by 256 x kHFormOut
+  // (LDS: 64 KB, or what holds 4 sources at order pmax+1; the kernel picks
+  // its tile from what is left after the factor head)
+  const bool form2 = Hh <= 256 * kHFormItems && K <= 256 * kHFormOut;
+  const size_t lds_form2 = std::max<size_t>((size_t)64 * 1024,
+                                            hform2_fac_bytes(pmax) + sizeof(double) * 2 * ((size_t)Hh + Fn) * 4);
+  const int region2 = (int)((lds_form2 - hform2_fac_bytes(pmax)) / sizeof(double));
+  if (form2 && lds_form2 > ldsmax) return 16;
   const size_t lds_ps = sizeof(double) * 4 * (size_t)K;
   const bool m2l_lds = sizeof(double) * 6 * (size_t)K <= ldsmax;
   if (lds_ps > ldsmax) return 16;
@@ -203,15 +213,25 @@ int helm_solve(Context& ctx, const LapCall& c) {
   }
   tm.mark(1);
   const double* fac = Tb.fac;
+  auto form = [&](auto irreg, int nbx, double* out) {
+    constexpr bool IR = decltype(irreg)::value;
+    if (!nbx) return;
+    if (!form2) {
+      hipLaunchKernelGGL((k_hform<IR>), dim3(nbx), dim3(256), lds_form, s, P, Hd, fac, spos, str64,
+                         f.ifcharge, f.ifdipole, ts, out);
+    } else if (f.ifcharge && f.ifdipole) {
+      hipLaunchKernelGGL((k_hform2<IR, true, true>), dim3(nbx), dim3(256), lds_form2, s, P, Hd, fac, spos, str64, region2, out);
+    } else if (f.ifcharge) {
+      hipLaunchKernelGGL((k_hform2<IR, true, false>), dim3(nbx), dim3(256), lds_form2, s, P, Hd, fac, spos, str64, region2, out);
+    } else if (f.ifdipole) {
+      hipLaunchKernelGGL((k_hform2<IR, false, true>), dim3(nbx), dim3(256), lds_form2, s, P, Hd, fac, spos, str64, region2, out);
+    }
+  };
   // STEP 1: P2M (hfmm3dpart.f:681-757)
-  if (P.n_p2m)
-    hipLaunchKernelGGL((k_hform<false>), dim3(P.n_p2m), dim3(256), lds_form, s, P, Hd, fac, spos, str64,
-                       f.ifcharge, f.ifdipole, ts, mp);
+  form(std::false_type{}, P.n_p2m, mp);
   tm.mark(2);
   // STEP 2: P2L (hfmm3dpart.f:765-830)
-  if (P.n_p2l)
-    hipLaunchKernelGGL((k_hform<true>), dim3(P.n_p2l), dim3(256), lds_form, s, P, Hd, fac, spos, str64,
-                       f.ifcharge, f.ifdipole, ts, lo);
+  form(std::true_type{}, P.n_p2l, lo);
   tm.mark(3);
   // STEP 3: M2M, bottom-up
   for (size_t g = 0; g < O.m2m_groups.size(); g++)

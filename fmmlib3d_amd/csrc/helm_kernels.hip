@@ -535,6 +535,291 @@ __global__ __launch_bounds__(64) void k_heval2(DevPlan P, HelmDev Hd, const doub
 }
 
 // ---------------------------------------------------------------------------
+// P2M / P2L with per-thread sums (the scheme of the Laplace k_form2).  With
+// B_n^mu = f_n(k r) Yhat_n^mu(u) (f = j_n/sc^n for P2M, h_n sc^n for P2L),
+// the 256 threads form G source groups of 256/G threads; each thread owns up
+// to kHFormItems half-table entries (n', a >= 0) and accumulates, over the
+// sources of its group, S^w(n', +-a) = sum_s w_s B_{n'}^{+-a} (Yhat^{-a} =
+// conj Yhat^a) for w = q, v_x, v_y, v_z, the weights read with wave-uniform
+// loads.  G = 4, 2 or 1: the most groups for which the entries of the box's
+// order fit.  Sources are taken in tiles of ts (as many as the LDS holds at
+// the box's order) that run across the boxes of list 4; per tile the
+// unit-vector harmonics and the radial series are built in LDS.  The
+// coefficients are then
+//   out(n,m) += ik sqrt(2n+1) [S^q(n,mu) + k/(2n+1) (AR lo S^z(n-1,mu)
+//     + AS hi S^z(n+1,mu) + BR lo S^u(n-1,mu+1) + BS hi S^u(n+1,mu+1)
+//     + CR lo S^d(n-1,mu-1) + CS hi S^d(n+1,mu-1))],   mu = -m,
+// with S^u, S^d = (S^x -+ i S^y)/2 (the same sums as k_hform, by linearity),
+// combined through LDS in two passes (q and z, then x and y), the groups
+// added in a fixed order.
+// ---------------------------------------------------------------------------
+constexpr int kHFormItems = 4;   // half-table entries per thread of a group
+constexpr int kHFormOut = 8;     // output coefficients per thread in the combine
+constexpr int kHFormTile = 64;   // most sources per tile
+
+// LDS head of the kernel: the factor tables [0, AR) and cm[m] = R_m^m /
+// (-(ux + i uy))^m = prod_{k<=m} sqrt((2k-1)/(2k)), m = 0..pmax+2
+__host__ __device__ inline size_t hform2_fac_bytes(int pmax) {
+  return ((size_t)(FacLayout(pmax).AR + pmax + 3) * sizeof(double) + 15) / 16 * 16;
+}
+
+// Column m of Yhat_n^m(u), n = m..pt, of a unit vector into a half table:
+// Yhat_m^m = cm (-(ux + i uy))^m with the power taken by binary powers, then
+// the three-term recurrence in n of regular_column (with r^2 = 1).
+__device__ __forceinline__ void hform_column(const double* facl, const FacLayout& L, int pt, int m, double cm,
+                                             double ux, double uy, double uz, double* out) {
+  Cd base = {-ux, -uy}, r = {cm, 0.0};
+  for (int e = m; e; e >>= 1) {
+    if (e & 1) r = cmul(r, base);
+    base = cmul(base, base);
+  }
+  cst(out, hidx(m, m), r);
+  if (m + 1 > pt) return;
+  const double f1 = facl[L.M1 + m] * uz;
+  Cd r1 = {f1 * r.re, f1 * r.im};
+  cst(out, hidx(m + 1, m), r1);
+  Cd r0 = r;
+  for (int n = m + 2; n <= pt; n++) {
+    const double a = (double)(2 * n - 1) * uz, b = facl[L.RA + hidx(n, m)], c = facl[L.RB + hidx(n, m)];
+    const Cd rn = {(a * r1.re - b * r0.re) * c, (a * r1.im - b * r0.im) * c};
+    cst(out, hidx(n, m), rn);
+    r0 = r1;
+    r1 = rn;
+  }
+}
+
+template <bool IRREG, bool CHG, bool DIP>
+__global__ __launch_bounds__(256) void k_hform2(DevPlan P, HelmDev Hd, const double* __restrict__ fac,
+                                                const double* __restrict__ spos, const double* __restrict__ str,
+                                                int region, double* __restrict__ expn) {
+  extern __shared__ unsigned char smem_raw[];
+  __shared__ int gidx[kHFormTile];
+  const int b = IRREG ? P.p2l_boxes[blockIdx.x] : P.p2m_boxes[blockIdx.x];
+  const int lev = P.level[b];
+  if (lev < 2) return;
+  const int tid = threadIdx.x;
+  const int pmaxg = P.p, Kg = P.K;
+  const int p = Hd.plev[lev];
+  const int pt = DIP ? p + 1 : p;
+  const int H = (pt + 1) * (pt + 2) / 2;
+  const int P2 = (pt + 1) * (pt + 1);
+  const int G = H <= 64 * kHFormItems ? 4 : (H <= 128 * kHFormItems ? 2 : 1);
+  const int gs = 256 / G, grp = tid / gs, lt = tid - grp * gs;
+  // region = doubles of LDS after the factor head; every order fits 4 sources
+  const int ts = min(kHFormTile, region / (2 * (H + pt + 1)));
+  FacLayout L(pmaxg);
+  double* facl = reinterpret_cast<double*>(smem_raw);
+  double* cm = facl + L.AR;
+  for (int k = tid; k < L.AR; k += 256) facl[k] = fac[k];
+  if (tid == 0) {
+    double c = 1.0;
+    cm[0] = 1.0;
+    for (int k = 1; k <= pt; k++) { c *= fac[L.MM + k]; cm[k] = c; }
+  }
+  double* tab = reinterpret_cast<double*>(smem_raw + hform2_fac_bytes(pmaxg));  // ts x H complex
+  double* fser = tab + 2 * (size_t)ts * H;                                      // ts x (pt+1) complex
+  const double sc = Hd.sclev[lev];
+  const double lo = IRREG ? sc : 1.0 / sc, hi = IRREG ? 1.0 / sc : sc;
+  const double cx = P.center[3 * b], cy = P.center[3 * b + 1], cz = P.center[3 * b + 2];
+  double acc[kHFormItems][16];
+  int hn[kHFormItems];   // degree n' of each owned entry h = n'(n'+1)/2 + a
+#pragma unroll
+  for (int q = 0; q < kHFormItems; q++) {
+    const int h = lt + q * gs;
+    int n = (int)((sqrtf(8.0f * h + 1.0f) - 1.0f) * 0.5f);
+    while (n * (n + 1) / 2 > h) n--;
+    while ((n + 1) * (n + 2) / 2 <= h) n++;
+    hn[q] = n;
+#pragma unroll
+    for (int e = 0; e < 16; e++) acc[q][e] = 0.0;
+  }
+  // the source boxes: list 4 of b for P2L, b itself for P2M
+  const int jbeg = IRREG ? P.l4_off[b] : 0, jend = IRREG ? P.l4_off[b + 1] : 1;
+  auto jbox = [&](int jj) { return IRREG ? P.l4_idx[jj] : b; };
+  int cj = jbeg, co = 0;   // cursor: box position in the list, offset in the box
+  while (cj < jend && P.nsrc[jbox(cj)] == 0) cj++;
+  while (cj < jend) {
+    // this tile: up to ts sources from the cursor on
+    int tn = 0;
+    {
+      int nj = cj, no = co, rem = ts;
+      while (nj < jend) {
+        const int left = P.nsrc[jbox(nj)] - no;
+        if (rem < left) { no += rem; tn += rem; rem = 0; break; }
+        rem -= left; tn += left; nj++; no = 0;
+      }
+      __syncthreads();
+      if (tid < tn) {
+        int wj = cj, wo = co + tid;
+        while (wo >= P.nsrc[jbox(wj)]) { wo -= P.nsrc[jbox(wj)]; wj++; }
+        gidx[tid] = P.isrc[jbox(wj)] + wo;
+      }
+      cj = nj; co = no;
+      while (cj < jend && co >= P.nsrc[jbox(cj)]) { cj++; co = 0; }
+    }
+    __syncthreads();
+    // phase A: the columns m of Yhat on the unit vector (task m * tn + s,
+    // so a wave holds few orders and its loop lengths are close), then the
+    // radial series (tasks after them, in waves of their own)
+    const int ncolt = (pt + 1) * tn;
+    for (int task = tid; task < ncolt + tn; task += 256) {
+      const int m = task < ncolt ? task / tn : -1;
+      const int s = task < ncolt ? task - m * tn : task - ncolt;
+      const double* x = spos + 4 * (size_t)gidx[s];
+      const double ux0 = x[0] - cx, uy0 = x[1] - cy, uz0 = x[2] - cz;
+      const double rr = sqrt(ux0 * ux0 + uy0 * uy0 + uz0 * uz0);
+      if (m >= 0) {
+        const double ir = rr > 0.0 ? 1.0 / rr : 0.0;
+        const double ux = rr > 0.0 ? ux0 * ir : 0.0, uy = rr > 0.0 ? uy0 * ir : 0.0, uz = rr > 0.0 ? uz0 * ir : 1.0;
+        hform_column(facl, L, pt, m, cm[m], ux, uy, uz, tab + 2 * (size_t)s * H);
+      } else {
+        const Cd zz = {Hd.kre * rr, Hd.kim * rr};
+        Cd* fcol = reinterpret_cast<Cd*>(fser + 2 * (size_t)s * (pt + 1));
+        if (IRREG) {
+          if (cnorm(zz) < 1.0e-30) {      // h3dformta_trunc0 skips such a source
+            for (int i = 0; i <= pt; i++) fcol[i] = {0.0, 0.0};
+          } else {
+            dev_hankel(pt, zz, sc, fcol, 1);
+          }
+        } else {
+          // the series needs two entries; the tile row holds pt+1
+          if (pt >= 1) {
+            dev_bessel(pt, zz, sc, fcol, 1);
+          } else {
+            Cd tmp[2];
+            dev_bessel(1, zz, sc, tmp, 1);
+            fcol[0] = tmp[0];
+          }
+        }
+      }
+    }
+    __syncthreads();
+    // phase B: per-thread sums over the group's sources of the tile
+    for (int s = grp; s < tn; s += G) {
+      const double* w = str + 8 * (size_t)gidx[s];
+      const double* Y = tab + 2 * (size_t)s * H;
+      const double* fr = fser + 2 * (size_t)s * (pt + 1);
+#pragma unroll
+      for (int q = 0; q < kHFormItems; q++) {
+        const int h = lt + q * gs;
+        if (h < H) {
+          const int n = hn[q];
+          const double yr = Y[2 * h], yi = Y[2 * h + 1];
+          const double fre = fr[2 * n], fim = fr[2 * n + 1];
+          const double bpr = fre * yr - fim * yi, bpi = fre * yi + fim * yr;   // f Y^a
+          const double bmr = fre * yr + fim * yi, bmi = fim * yr - fre * yi;   // f conj(Y^a)
+#pragma unroll
+          for (int c = 0; c < 4; c++) {
+            if ((c == 0 && !CHG) || (c > 0 && !DIP)) continue;
+            const double wr = w[2 * c], wi = w[2 * c + 1];
+            double* A = &acc[q][4 * c];
+            A[0] = fma(wr, bpr, fma(-wi, bpi, A[0]));
+            A[1] = fma(wr, bpi, fma(wi, bpr, A[1]));
+            A[2] = fma(wr, bmr, fma(-wi, bmi, A[2]));
+            A[3] = fma(wr, bmi, fma(wi, bmr, A[3]));
+          }
+        }
+      }
+    }
+  }
+  // combine through LDS, two weights at a time: c = 0 (q) with 3 (v_z), then 1, 2 (v_x, v_y)
+  const int K = (p + 1) * (p + 1);
+  double* SA = tab;
+  double* SB = tab + 2 * (size_t)P2;
+  Cd part[kHFormOut], dzp[kHFormOut];
+#pragma unroll
+  for (int r = 0; r < kHFormOut; r++) { part[r] = {0.0, 0.0}; dzp[r] = {0.0, 0.0}; }
+  auto sget = [&](const double* S, int n1, int m1) -> Cd {
+    if (n1 < 0 || n1 > pt || m1 > n1 || m1 < -n1) return Cd{0.0, 0.0};
+    return Cd{S[2 * eidx(n1, m1)], S[2 * eidx(n1, m1) + 1]};
+  };
+  auto faf = [&](int off, int n1, int m1) -> double { return fac[off + eidx(n1, m1)]; };
+#pragma unroll
+  for (int pass = 0; pass < 2; pass++) {
+    const int ca = pass == 0 ? 0 : 1, cb = pass == 0 ? 3 : 2;
+    if (pass == 1 && !DIP) break;
+    for (int g = 0; g < G; g++) {
+      __syncthreads();
+      if (grp == g) {
+#pragma unroll
+        for (int q = 0; q < kHFormItems; q++) {
+          const int h = lt + q * gs;
+          if (h < H) {
+            const int n = hn[q];
+            const int a = h - n * (n + 1) / 2;
+            // (a = 0: the two sums are the same entry, taken once)
+            const int ip = 2 * eidx(n, a), im = 2 * eidx(n, -a);
+            if (g == 0) {
+              SA[ip] = acc[q][4 * ca]; SA[ip + 1] = acc[q][4 * ca + 1];
+              SB[ip] = acc[q][4 * cb]; SB[ip + 1] = acc[q][4 * cb + 1];
+              if (a > 0) {
+                SA[im] = acc[q][4 * ca + 2]; SA[im + 1] = acc[q][4 * ca + 3];
+                SB[im] = acc[q][4 * cb + 2]; SB[im + 1] = acc[q][4 * cb + 3];
+              }
+            } else {
+              SA[ip] += acc[q][4 * ca]; SA[ip + 1] += acc[q][4 * ca + 1];
+              SB[ip] += acc[q][4 * cb]; SB[ip + 1] += acc[q][4 * cb + 1];
+              if (a > 0) {
+                SA[im] += acc[q][4 * ca + 2]; SA[im + 1] += acc[q][4 * ca + 3];
+                SB[im] += acc[q][4 * cb + 2]; SB[im + 1] += acc[q][4 * cb + 3];
+              }
+            }
+          }
+        }
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < kHFormOut; r++) {
+      const int k = tid + r * 256;
+      if (k >= K) continue;
+      int n = (int)sqrtf((float)k);
+      while (n * n > k) n--;
+      while ((n + 1) * (n + 1) <= k) n++;
+      const int mu = -(k - n * n - n);
+      if (pass == 0) {
+        if (CHG) part[r] = sget(SA, n, mu);
+        if (DIP) {
+          const Cd z1 = sget(SB, n - 1, mu), z2 = sget(SB, n + 1, mu);
+          dzp[r] = cadd2(cscale(z1, faf(L.AR, n, mu) * lo), cscale(z2, faf(L.AS, n, mu) * hi));
+        }
+      } else if (DIP) {
+        // S^u = (S^x - i S^y)/2, S^d = (S^x + i S^y)/2
+        auto su = [&](int n1, int m1) {
+          const Cd x = sget(SA, n1, m1), y = sget(SB, n1, m1);
+          return Cd{0.5 * (x.re + y.im), 0.5 * (x.im - y.re)};
+        };
+        auto sd = [&](int n1, int m1) {
+          const Cd x = sget(SA, n1, m1), y = sget(SB, n1, m1);
+          return Cd{0.5 * (x.re - y.im), 0.5 * (x.im + y.re)};
+        };
+        const double fbr = n >= 1 ? faf(L.BR, n, mu) : 0.0, fcr = n >= 1 ? faf(L.CR, n, mu) : 0.0;
+        Cd g = dzp[r];
+        g = cadd2(g, cadd2(cscale(su(n - 1, mu + 1), fbr * lo), cscale(su(n + 1, mu + 1), faf(L.BS, n, mu) * hi)));
+        g = cadd2(g, cadd2(cscale(sd(n - 1, mu - 1), fcr * lo), cscale(sd(n + 1, mu - 1), faf(L.CS, n, mu) * hi)));
+        dzp[r] = g;
+      }
+    }
+  }
+  // out += ik sqrt(2n+1) (charge part + k/(2n+1) dipole bracket)
+  double* out = expn + 2 * (size_t)b * Kg;
+  const Cd ik = {-Hd.kim, Hd.kre}, kk = {Hd.kre, Hd.kim};
+#pragma unroll
+  for (int r = 0; r < kHFormOut; r++) {
+    const int k = tid + r * 256;
+    if (k >= K) continue;
+    int n = (int)sqrtf((float)k);
+    while (n * n > k) n--;
+    while ((n + 1) * (n + 1) <= k) n++;
+    Cd v = part[r];
+    if (DIP) v = cadd2(v, cscale(cmul(kk, dzp[r]), 1.0 / (2.0 * n + 1.0)));
+    v = cscale(cmul(ik, v), fac[L.M1 + n]);
+    out[2 * k] += v.re;
+    out[2 * k + 1] += v.im;
+  }
+}
+
+// ---------------------------------------------------------------------------
 // M2P / L2P from packed expansions.  For every box whose expansion is
 // evaluated, the expansion Ew = sqrt(2n+1) E and the three gradient
 // expansions Gz, Gu, Gd of k_heval2 are stored per half-table entry (n, a >= 0)
