@@ -124,11 +124,14 @@ int helm_solve(Context& ctx, const LapCall& c) {
   thread_local Octree tree;  // storage reused across calls
   const int* dsperm = nullptr;
   const int* dtperm = nullptr;
+  // the interaction lists are built after the first kernels are queued
   if (device_tree_enabled()) {
-    if (build_octree_device(ctx, ctx.stream, c.src_dev, ns, c.trg_dev, nt, nbox, tree, &dsperm, &dtperm) != 0) return 4;
-  } else if (build_octree(c.src_host, ns, c.trg_host, nt, nbox, tree) != 0) {
+    if (build_octree_device(ctx, ctx.stream, c.src_dev, ns, c.trg_dev, nt, nbox, tree, &dsperm, &dtperm, false) != 0)
+      return 4;
+  } else if (build_octree(c.src_host, ns, c.trg_host, nt, nbox, tree, false) != 0) {
     return 4;
   }
+  auto t_tree = std::chrono::steady_clock::now();
   int ier = 0;
   HelmCache* HC = helm_tables(ctx, zk, tree.size, tree.nlev, eps, ier);
   if (!HC) return ier;
@@ -154,18 +157,29 @@ int helm_solve(Context& ctx, const LapCall& c) {
   const size_t lds_ps = sizeof(double) * 4 * (size_t)K;
   const bool m2l_lds = sizeof(double) * 6 * (size_t)K <= ldsmax;
   if (lds_ps > ldsmax) return 16;
-  Flat F;
+  thread_local Flat FA, FB, FC;  // plan parts: boxes, near lists, far lists (storage reused)
+  FA.I.clear(); FA.D.clear(); FB.I.clear(); FB.D.clear(); FC.I.clear(); FC.D.clear();
   PlanOffsets O;
-  build_plan(tree, f, F, O, 2);
+  build_plan_boxes(tree, f, FA, O, 2);
   auto t1 = std::chrono::steady_clock::now();
   hipStream_t s = ctx.stream;
   Timer tm;
   tm.start(s, 11);
-  int* dI = (int*)ctx.arena.get("plan_i", F.I.size() * sizeof(int));
-  double* dD = (double*)ctx.arena.get("plan_d", F.D.size() * sizeof(double));
-  FMM_HIP(hipMemcpyAsync(dI, F.I.data(), F.I.size() * sizeof(int), hipMemcpyHostToDevice, s));
-  FMM_HIP(hipMemcpyAsync(dD, F.D.data(), F.D.size() * sizeof(double), hipMemcpyHostToDevice, s));
-  DevPlan P = make_devplan(tree, pmax, O, dI, dD, dI);
+  // plan uploads go through pinned staging buffers so that they do not wait
+  // for the kernels already queued
+  auto upload_ints = [&](const char* name, const std::vector<int>& v) {
+    int* h = (int*)ctx.arena.get_host(std::string(name) + "_h", std::max<size_t>(v.size(), 1) * sizeof(int));
+    std::memcpy(h, v.data(), v.size() * sizeof(int));
+    int* d = (int*)ctx.arena.get(name, std::max<size_t>(v.size(), 1) * sizeof(int));
+    FMM_HIP(hipMemcpyAsync(d, h, v.size() * sizeof(int), hipMemcpyHostToDevice, s));
+    return d;
+  };
+  int* dI = upload_ints("plan_i", FA.I);
+  double* hD = (double*)ctx.arena.get_host("plan_d_h", FA.D.size() * sizeof(double));
+  std::memcpy(hD, FA.D.data(), FA.D.size() * sizeof(double));
+  double* dD = (double*)ctx.arena.get("plan_d", FA.D.size() * sizeof(double));
+  FMM_HIP(hipMemcpyAsync(dD, hD, FA.D.size() * sizeof(double), hipMemcpyHostToDevice, s));
+  DevPlan P = make_devplan(tree, pmax, O, dI, dD);
   const int* ti = (const int*)ctx.arena.get("htab_i", HC->I.size() * sizeof(int));
   const double* td = (const double*)ctx.arena.get("htab_d", HC->D.size() * sizeof(double));
   const long* tl = (const long*)ctx.arena.get("htab_l", HC->Lg.size() * sizeof(long));
@@ -230,14 +244,38 @@ int helm_solve(Context& ctx, const LapCall& c) {
   // STEP 1: P2M (hfmm3dpart.f:681-757)
   form(std::false_type{}, P.n_p2m, mp);
   tm.mark(2);
-  // STEP 2: P2L (hfmm3dpart.f:765-830)
-  form(std::true_type{}, P.n_p2l, lo);
-  tm.mark(3);
   // STEP 3: M2M, bottom-up
   for (size_t g = 0; g < O.m2m_groups.size(); g++)
     hipLaunchKernelGGL(k_hm2m, dim3(O.m2m_groups[g].second), dim3(256), lds_ps, s, P, Tb, Hd,
                        O.m2m_groups[g].first, HC->m2m_base + O.m2m_group_level[g], mp);
+  tm.mark(3);
+  // while the device runs P2M and M2M: list 1 and the P2P plan
+  auto t_lists0 = std::chrono::steady_clock::now();
+  tree_lists_near(tree);
+  build_plan_near(tree, f, FB, O);
+  const int* dIB = upload_ints("plan_ib", FB.I);
+  devplan_near(P, O, dIB);
+  // the P2P work items are addressed explicitly: they must lie in this part
+  if (O.p2p0 + 3 * (size_t)O.n_p2p0 > FB.I.size() || O.p2p1 + 3 * (size_t)O.n_p2p1 > FB.I.size())
+    throw HipError("plan: P2P work items outside the near-field part of the plan");
+  auto t_lists1 = std::chrono::steady_clock::now();
+  // STEP 8: P2P (hfmm3dpart.f:1101-1180), queued ahead of the far field
+  if (f.ifcharge || f.ifdipole) {
+    launch_hp2p<TN>(P, dIB + O.p2p0, O.n_p2p0, 0, Hd.kre, Hd.kim, spos, strn, tpos, f.ifcharge, f.ifdipole,
+                    f.iffld != 0, pot_s, fld_s, s);
+    launch_hp2p<TN>(P, dIB + O.p2p1, O.n_p2p1, 1, Hd.kre, Hd.kim, spos, strn, tpos, f.ifcharge, f.ifdipole,
+                    f.iffldtarg != 0, pot_t, fld_t, s);
+  }
   tm.mark(4);
+  // while the device runs P2P: lists 2-4 and the far-field plan
+  auto t_lists2 = std::chrono::steady_clock::now();
+  tree_lists_far(tree);
+  build_plan_far(tree, f, FC, O);
+  devplan_far(P, O, upload_ints("plan_ic", FC.I));
+  auto t_lists3 = std::chrono::steady_clock::now();
+  // STEP 2: P2L (hfmm3dpart.f:765-830)
+  form(std::true_type{}, P.n_p2l, lo);
+  tm.mark(5);
   // STEP 4: M2L
   if (P.n_m2l) {
     if (m2l_lds)
@@ -245,12 +283,12 @@ int helm_solve(Context& ctx, const LapCall& c) {
     else
       hipLaunchKernelGGL((k_hm2l<false>), dim3(P.n_m2l), dim3(256), lds_ps, s, P, Tb, Hd, mp, lo);
   }
-  tm.mark(5);
+  tm.mark(6);
   // STEP 5: L2L, top-down
   for (size_t g = 0; g < O.l2l_groups.size(); g++)
     hipLaunchKernelGGL(k_hl2l, dim3(O.l2l_groups[g].second), dim3(256), lds_ps, s, P, Tb, Hd,
                        O.l2l_groups[g].first, HC->l2l_base + O.l2l_group_level[g], lo);
-  tm.mark(6);
+  tm.mark(7);
   // STEP 7: L2P and STEP 6: M2P (list 3), from packed expansions
   {
     const int grad = (f.iffld || f.iffldtarg) ? 1 : 0;
@@ -334,7 +372,7 @@ int helm_solve(Context& ctx, const LapCall& c) {
                          dh + o_sl, dh + o_il, grad, spos, tpos, f.ifpot, f.iffld, f.ifpottarg, f.iffldtarg, pot_s,
                          fld_s, pot_t, fld_t, (double*)nullptr);
     }
-    tm.mark(7);
+    tm.mark(8);
     if (!it_m.empty() && !pk_m.empty()) {
       double* pack = (double*)ctx.arena.get("hpack_m", sizeof(double) * 16 * (size_t)std::max(tot_m, 1));
       hipLaunchKernelGGL((k_hpack<true>), dim3((unsigned)pk_m.size()), dim3(256), 0, s, P, Hd, fac, mp, dh + o_pm,
@@ -347,14 +385,6 @@ int helm_solve(Context& ctx, const LapCall& c) {
       hipLaunchKernelGGL(k_hreduce, dim3((unsigned)(red_m.size() / 5)), dim3(64), 0, s, P, dh + o_rm, partial,
                          f.ifpot, f.iffld, f.ifpottarg, f.iffldtarg, pot_s, fld_s, pot_t, fld_t);
     }
-    tm.mark(8);
-  }
-  // STEP 8: P2P
-  if (f.ifcharge || f.ifdipole) {
-    launch_hp2p<TN>(P, dI + O.p2p0, O.n_p2p0, 0, Hd.kre, Hd.kim, spos, strn, tpos, f.ifcharge, f.ifdipole,
-                    f.iffld != 0, pot_s, fld_s, s);
-    launch_hp2p<TN>(P, dI + O.p2p1, O.n_p2p1, 1, Hd.kre, Hd.kim, spos, strn, tpos, f.ifcharge, f.ifdipole,
-                    f.iffldtarg != 0, pot_t, fld_t, s);
   }
   tm.mark(9);
   if (f.ifpot) hipLaunchKernelGGL(k_scatter, dim3(blocks(ns, 256)), dim3(256), 0, s, ns, 1, sperm, pot_s, c.pot_dev);
@@ -365,13 +395,18 @@ int helm_solve(Context& ctx, const LapCall& c) {
   FMM_HIP(hipGetLastError());
   FMM_HIP(hipStreamSynchronize(s));
   auto& lt = last_timings();
-  lt.assign(11, 0.0);
-  lt[0] = std::chrono::duration<double, std::milli>(t1 - t0).count();
-  auto el = tm.elapsed();
-  for (int i = 0; i < 9 && i < (int)el.size(); i++) lt[1 + i] = el[i];
+  lt.assign(13, 0.0);
+  auto ms = [](auto a, auto b) { return std::chrono::duration<double, std::milli>(b - a).count(); };
+  lt[0] = ms(t0, t1);
+  auto el = tm.elapsed();  // upload, P2M, M2M, P2P, P2L, M2L, L2L, L2P, M2P, scatter
+  el.resize(10, 0.0);
+  lt[1] = el[0]; lt[2] = el[1]; lt[3] = el[4]; lt[4] = el[2]; lt[5] = el[5];
+  lt[6] = el[6]; lt[7] = el[7]; lt[8] = el[8]; lt[9] = el[3];
   double tot = 0;
   for (auto v : el) tot += v;
   lt[10] = tot + lt[0];
+  lt[11] = ms(t0, t_tree);
+  lt[12] = ms(t_lists0, t_lists1) + ms(t_lists2, t_lists3);
   return 0;
 }
 

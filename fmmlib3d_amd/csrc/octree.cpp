@@ -150,8 +150,10 @@ void tree_root(Octree& t, const double lo[3], const double hi[3], int ns, int nt
 
 void tree_pass_setup(const Octree& t, int L, TreePass& ps) {
   const int nf = (int)ps.frontier.size();
+  // D levels per pass, with nf * 8^D <= 2^18: the histograms, node ids and
+  // split flags of a pass (copied back and scanned on the host) scale with it
   int D = 4;
-  while (D > 1 && (long)nf << (3 * D) > (1L << 22)) D--;
+  while (D > 1 && (long)nf << (3 * D) > (1L << 18)) D--;
   D = std::min(D, kTreeMaxLevel - L);
   ps.L = L;
   ps.D = ps.Dfull = D;
