@@ -5,8 +5,9 @@ in the unit cube, complex charges and dipole strengths with real and
 imaginary parts uniform in [-1,1], unit dipole vectors uniform on the sphere,
 iprec=2, potential and field at sources and targets.  Inputs are resident in
 HBM before the timed region; a step is one call of the device-resident entry
-point (host octree on copied-back coordinates, upload of the plan, the eight
-FMM stages, scatter back), so `value` = ntarget / seconds per step.
+point (octree on the device, interaction lists and plan on the host while the
+first stages run, the eight FMM stages, scatter back), so `value` = ntarget /
+seconds per step.
 
 Three precision modes run (fp64; mixed_near: FP32 pair arithmetic in the
 near field; mixed: FP32 near field and FP32 expansions/translations, with FP64
@@ -17,10 +18,12 @@ potential and the field.  The P2P roofline uses the exact pair count of the
 tree; the CPU baseline times the reference library (oracle/_ref) on a
 125,000 + 125,000 sample with the same leaf fill (244 points per leaf).
 
-Two more lines ride along in the same JSON object: "helmholtz" (config 3,
-hfmm3dpartself on 500,000 points on the unit sphere, zk = 20, iprec = 2) and
+Three more lines ride along in the same JSON object: "helmholtz"
+(hfmm3dpartself on 500,000 points on the unit sphere, zk = 20, iprec = 2),
 "serving" (a stream of mixed-size Laplace requests through the request server,
-p50/p99 latency and throughput).
+p50/p99 latency and throughput) and "stream5" (Plummer-distributed Laplace and
+Helmholtz requests with charges and dipoles at iprec 2 and 3, each checked
+against the FP64 direct sum).
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
 """
@@ -205,6 +208,67 @@ def serving_leg(F, nreq, nworkers, seed=4):
             "mean_ms": float(np.mean(lat))}
 
 
+def stream5_leg(F, ob, nreq, nworkers, seed=5):
+    """Stream 5 shape: requests with N log-uniform in [5e3, 5e5], Plummer
+    points (a = 0.1, centred in the unit cube, points outside rejected),
+    Laplace or Helmholtz (zk uniform in [1, 30]) per request, iprec in {2, 3},
+    charges and dipoles (U[-1,1] parts, unit dipole vectors).  iprec 2 runs
+    the FP32 near field, iprec 3 FP64 throughout.  All requests are submitted
+    at once; every result is checked against the FP64 direct sum at 200
+    sampled sources."""
+    import numpy as np
+    from fmm_inputs import IPREC_EPS, points, rel
+    rng = np.random.default_rng(seed)
+    reqs = []
+    for _ in range(nreq):
+        n = int(10 ** rng.uniform(np.log10(5e3), np.log10(5e5)))
+        x = points(rng, n, "plummer")
+        zk = complex(rng.uniform(1, 30), 0.0) if rng.integers(0, 2) else None
+        iprec = int(rng.integers(2, 4))
+        q = rng.uniform(-1, 1, n) + 1j * rng.uniform(-1, 1, n)
+        ds = rng.uniform(-1, 1, n) + 1j * rng.uniform(-1, 1, n)
+        z = rng.uniform(-1, 1, n)
+        ph = rng.uniform(0, 2 * np.pi, n)
+        dv = np.stack([np.sqrt(1 - z * z) * np.cos(ph), np.sqrt(1 - z * z) * np.sin(ph), z], axis=1)
+        reqs.append((zk, iprec, x, q, ds, dv))
+
+    def submit(srv, r):
+        zk, iprec, x, q, ds, dv = r
+        prec = F.PREC_MIXED if iprec == 2 else F.PREC_FP64
+        return srv.submit(0 if zk is None else 1, iprec, x, charge=q, dipstr=ds, dipvec=dv, zk=zk, precision=prec)
+
+    npts = sum(r[2].shape[0] for r in reqs)
+    with F.Server(nworkers=nworkers, mem_budget=96 << 30) as srv:
+        for i in [submit(srv, r) for r in reqs[:nworkers]]:
+            srv.wait(i)
+        t0 = time.perf_counter()
+        ids = [submit(srv, r) for r in reqs]
+        out = [srv.wait(i) for i in ids]
+        t_sat = time.perf_counter() - t0
+    worst = {}
+    lat = []
+    for r, o in zip(reqs, out):
+        zk, iprec, x, q, ds, dv = r
+        assert o[0] == 0
+        lat.append(o[5])
+        idx = rng.choice(x.shape[0], 200, replace=False)
+        pd, fd = ob.orc_direct_sample(x, x[idx], idx.astype(np.int32), q, ds, dv, zk=zk)
+        key = "%s_iprec%d" % ("laplace" if zk is None else "helmholtz", iprec)
+        e = max(rel(o[1][idx], pd), rel(o[2][idx], fd))
+        worst[key] = max(worst.get(key, 0.0), e)
+    # two bars: the metric's 1e-6, and the tolerance iprec names (0.5e-6 at
+    # iprec 2, which the FP32 near field can exceed on clustered points)
+    within = all(worst[k] <= IPREC_EPS[int(k[-1])] for k in worst)
+    within_1e6 = all(v <= 1e-6 for v in worst.values())
+    return {"workload": "%d requests, N log-uniform in [5e3, 5e5], Plummer points, Laplace or Helmholtz "
+                        "(zk in [1, 30]), iprec in {2, 3}, charges + dipoles; all submitted at once" % nreq,
+            "workers": nworkers, "points_total": npts, "seconds": t_sat,
+            "req_per_s": nreq / t_sat, "points_per_s": npts / t_sat,
+            "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
+            "max_rel_err": worst, "within_1e-6": within_1e6, "within_iprec_tolerance": within,
+            "sampled_per_request": 200}
+
+
 def run_mode(F, torch, prec, dev_in, dev_out, steps, warmup):
     ts, tq, tds, tdv, tt = dev_in
     pot, fld, pt, ft = dev_out
@@ -244,6 +308,8 @@ def main():
     ap.add_argument("--no-serve", action="store_true", help="skip the serving-stream line")
     ap.add_argument("--serve-requests", type=int, default=48)
     ap.add_argument("--serve-workers", type=int, default=4)
+    ap.add_argument("--no-stream5", action="store_true", help="skip the mixed Laplace/Helmholtz Plummer stream")
+    ap.add_argument("--stream5-requests", type=int, default=32)
     ap.add_argument("--cpu-baseline-child", type=int, default=0, help=argparse.SUPPRESS)
     ap.add_argument("--calibrate-copy-mb", type=int, default=0,
                     help="only run a plain device-to-device copy of this many MiB (HBM counter calibration)")
@@ -317,6 +383,11 @@ def main():
         print("# serving: p50 %.1f ms p99 %.1f ms, saturated %.2f req/s (%.3e points/s)"
               % (serve["p50_ms"], serve["p99_ms"], serve["saturated_req_per_s"], serve["saturated_points_per_s"]),
               file=sys.stderr, flush=True)
+    s5 = None if args.no_stream5 else stream5_leg(F, ob, args.stream5_requests, args.serve_workers)
+    if s5:
+        print("# stream 5: %.2f req/s (%.3e points/s), worst errors %s"
+              % (s5["req_per_s"], s5["points_per_s"], {k: "%.1e" % v for k, v in s5["max_rel_err"].items()}),
+              file=sys.stderr, flush=True)
     cpu = {"value": None, "unit": "targets/s", "cores": 0, "kind": "reference", "sample": "skipped"} \
         if args.no_cpu else cpu_baseline()
     line = {
@@ -340,6 +411,7 @@ def main():
         "modes": {k: {kk: vv for kk, vv in v.items() if kk != "latencies_ms"} for k, v in modes.items()},
         "helmholtz": helm,
         "serving": serve,
+        "stream5": s5,
     }
     print(json.dumps(line))
 
