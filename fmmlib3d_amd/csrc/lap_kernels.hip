@@ -129,15 +129,49 @@ __global__ void k_form(DevPlan P, const double* __restrict__ fac, const double* 
 // box, S^w(n', +-a) = sum_s w_s H_{n'}^{+-a}(u_s) for the weights
 // w = q, v_x, v_y, v_z (H^{-a} = conj(H^a)); the weights are wave-uniform
 // loads.  The harmonic table of a tile of `ts` sources is built in LDS, one
-// thread per (source, column pair a, pt - a).  At the end the sums go to LDS
+// thread per (column m, source), m-major so that a wave holds few orders and
+// its loop lengths are close.  At the end the sums go to LDS
 // and one thread per coefficient combines them: charge S^q(n, -m), dipole
 // through the derivative factors and degree n-1 (P2M) or n+1 (P2L), with
 // (v_x -+ i v_y)/2 formed from S^{v_x}, S^{v_y} by linearity.
 // ---------------------------------------------------------------------------
 constexpr int kFormItems = 2;  // half-table entries per thread (256 threads)
-// LDS bytes of the recurrence factors k_form2 keeps (FacLayout MM..RB), 16-aligned
+// LDS bytes k_form2 keeps ahead of the harmonic table, 16-aligned: the
+// recurrence factors (FacLayout MM..RB) and cm[m] = prod_{k<=m} MM[k],
+// m = 0..p+2
 __host__ __device__ inline size_t form2_fac_bytes(int p) {
-  return ((size_t)FacLayout(p).AR * sizeof(double) + 15) / 16 * 16;
+  return ((size_t)(FacLayout(p).AR + p + 3) * sizeof(double) + 15) / 16 * 16;
+}
+
+// Column m of the regular (R, r2 = |u|^2) or irregular (S, ir2 = 1/|u|^2,
+// ir = 1/|u|) harmonics into a half table, n = m..pt.  The diagonal entry
+// R_m^m = cm w^m, S_m^m = ir cm (w ir2)^m with w = -(ux + i uy) is taken by
+// binary powers; then the three-term recurrences of regular_column and
+// irregular_column.
+template <typename T, bool IRREG>
+__device__ __forceinline__ void form_column(const double* facl, const FacLayout& L, int pt, int m, T cm, T ux,
+                                            T uy, T uz, T r2, T ir2, T ir, T* out) {
+  C2<T> base = IRREG ? C2<T>{-ux * ir2, -uy * ir2} : C2<T>{-ux, -uy};
+  C2<T> r = {IRREG ? cm * ir : cm, T(0)};
+  for (int e = m; e; e >>= 1) {
+    if (e & 1) r = cmul(r, base);
+    base = cmul(base, base);
+  }
+  cst(out, hidx(m, m), r);
+  if (m + 1 > pt) return;
+  const T g = (T)facl[L.M1 + m] * uz * (IRREG ? ir2 : T(1));
+  C2<T> r1 = {g * r.re, g * r.im};
+  cst(out, hidx(m + 1, m), r1);
+  C2<T> r0 = r;
+  for (int n = m + 2; n <= pt; n++) {
+    const T a = (T)(2 * n - 1) * uz;
+    const T b = IRREG ? (T)facl[L.RA + hidx(n, m)] : (T)facl[L.RA + hidx(n, m)] * r2;
+    const T c = IRREG ? (T)facl[L.RB + hidx(n, m)] * ir2 : (T)facl[L.RB + hidx(n, m)];
+    const C2<T> rn = {(a * r1.re - b * r0.re) * c, (a * r1.im - b * r0.im) * c};
+    cst(out, hidx(n, m), rn);
+    r0 = r1;
+    r1 = rn;
+  }
 }
 
 template <typename T, bool IRREG, bool CHG, bool DIP>
@@ -148,14 +182,20 @@ __global__ __launch_bounds__(256) void k_form2(DevPlan P, const double* __restri
   const int p = P.p, K = P.K;
   const int pt = IRREG ? p + 1 : p;
   const int H = (pt + 1) * (pt + 2) / 2;
-  const int ncol = pt / 2 + 1;  // column tasks per source: a and pt - a
   const int P2 = (pt + 1) * (pt + 1);
   FacLayout L(p);
   const int tid = threadIdx.x;
   // the recurrence factors (MM, M1, RA, RB: the head of the factor table) in
-  // LDS, ahead of the harmonic table: each column task reads a different entry
+  // LDS, ahead of the harmonic table: each column task reads a different
+  // entry; then the diagonal constants cm
   double* facl = reinterpret_cast<double*>(smem_raw);
+  double* cmt = facl + L.AR;
   for (int k = tid; k < L.AR; k += blockDim.x) facl[k] = fac[k];
+  if (tid == 0) {
+    double c = 1.0;
+    cmt[0] = 1.0;
+    for (int k = 1; k <= pt; k++) { c *= fac[L.MM + k]; cmt[k] = c; }
+  }
   T* tab = reinterpret_cast<T*>(smem_raw + form2_fac_bytes(p));
   const int b = IRREG ? P.p2l_boxes[blockIdx.x] : P.p2m_boxes[blockIdx.x];
   const double ihd = P.inv_side[P.level[b]];
@@ -174,20 +214,17 @@ __global__ __launch_bounds__(256) void k_form2(DevPlan P, const double* __restri
     for (int t0 = 0; t0 < sn; t0 += ts) {
       const int tn = min(ts, sn - t0);
       __syncthreads();
-      for (int task = tid; task < tn * ncol; task += blockDim.x) {
-        const int s = task / ncol, a = task - s * ncol;
+      for (int task = tid; task < tn * (pt + 1); task += blockDim.x) {
+        const int m = task / tn, s = task - m * tn;
         const double* x = spos + 4 * (size_t)(s0 + t0 + s);
         const T ux = (T)((x[0] - cx) * ihd), uy = (T)((x[1] - cy) * ihd), uz = (T)((x[2] - cz) * ihd);
         const T r2 = ux * ux + uy * uy + uz * uz;
-        T* out = tab + 2 * (size_t)s * H;
+        T ir2 = T(0), ir = T(0);
         if (IRREG) {
-          const T ir2 = T(1) / r2, ir = sqrt(ir2);
-          irregular_column<T>(facl, L, pt, a, ux, uy, uz, ir2, ir, out, 1);
-          if (pt - a != a) irregular_column<T>(facl, L, pt, pt - a, ux, uy, uz, ir2, ir, out, 1);
-        } else {
-          regular_column<T>(facl, L, pt, a, ux, uy, uz, r2, out, 1);
-          if (pt - a != a) regular_column<T>(facl, L, pt, pt - a, ux, uy, uz, r2, out, 1);
+          ir2 = T(1) / r2;
+          ir = sqrt(ir2);
         }
+        form_column<T, IRREG>(facl, L, pt, m, (T)cmt[m], ux, uy, uz, r2, ir2, ir, tab + 2 * (size_t)s * H);
       }
       __syncthreads();
       for (int s = 0; s < tn; s++) {
