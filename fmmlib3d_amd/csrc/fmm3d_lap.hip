@@ -177,14 +177,17 @@ int lap_solve(Context& ctx, const LapCall& c) {
   tm.start(s, 11);
   // ---- device buffers; plan uploads go through pinned staging buffers so
   // that they do not wait for the kernels already queued
-  auto upload_ints = [&](const char* name, const std::vector<int>& v) {
+  auto upload_ints = [&](const char* name, const std::vector<int>& v, hipStream_t st) {
     int* h = (int*)ctx.arena.get_host(std::string(name) + "_h", v.size() * sizeof(int));
     std::memcpy(h, v.data(), v.size() * sizeof(int));
     int* d = (int*)ctx.arena.get(name, v.size() * sizeof(int));
-    FMM_HIP(hipMemcpyAsync(d, h, v.size() * sizeof(int), hipMemcpyHostToDevice, s));
+    FMM_HIP(hipMemcpyAsync(d, h, v.size() * sizeof(int), hipMemcpyHostToDevice, st));
     return d;
   };
-  int* dI = upload_ints("plan_i", FA.I);
+  // the near field (P2P) runs on a second stream into its own sums, which
+  // the final scatter adds to the far-field sums
+  hipStream_t s2 = aux_stream(ctx);
+  int* dI = upload_ints("plan_i", FA.I, s);
   double* hD = (double*)ctx.arena.get_host("plan_d_h", FA.D.size() * sizeof(double));
   std::memcpy(hD, FA.D.data(), FA.D.size() * sizeof(double));
   double* dD = (double*)ctx.arena.get("plan_d", FA.D.size() * sizeof(double));
@@ -204,6 +207,10 @@ int lap_solve(Context& ctx, const LapCall& c) {
   double* fld_s = (double*)ctx.arena.get("fld_s", sizeof(double) * 6 * (size_t)ns);
   double* pot_t = (double*)ctx.arena.get("pot_t", sizeof(double) * 2 * (size_t)std::max(nt, 1));
   double* fld_t = (double*)ctx.arena.get("fld_t", sizeof(double) * 6 * (size_t)std::max(nt, 1));
+  double* pot_sn = (double*)ctx.arena.get("pot_sn", sizeof(double) * 2 * (size_t)ns);
+  double* fld_sn = (double*)ctx.arena.get("fld_sn", sizeof(double) * 6 * (size_t)ns);
+  double* pot_tn = (double*)ctx.arena.get("pot_tn", sizeof(double) * 2 * (size_t)std::max(nt, 1));
+  double* fld_tn = (double*)ctx.arena.get("fld_tn", sizeof(double) * 6 * (size_t)std::max(nt, 1));
   const TN* strn = near32 ? (const TN*)(const void*)str32 : (const TN*)(const void*)str64;
   const int* sperm = dsperm ? dsperm : dI + O.sperm;
   const int* tperm = dsperm ? dtperm : dI + O.tperm;
@@ -221,10 +228,15 @@ int lap_solve(Context& ctx, const LapCall& c) {
   zero(lo, 2 * (size_t)K * nb);
   zero(pot_s, 2 * (size_t)ns);
   zero(fld_s, 6 * (size_t)ns);
+  zero(pot_sn, 2 * (size_t)ns);
+  zero(fld_sn, 6 * (size_t)ns);
   if (nt > 0) {
     zero(pot_t, 2 * (size_t)nt);
     zero(fld_t, 6 * (size_t)nt);
+    zero(pot_tn, 2 * (size_t)nt);
+    zero(fld_tn, 6 * (size_t)nt);
   }
+  FMM_HIP(hipEventRecord(ctx.aux_ev[0], s));  // inputs gathered, sums zeroed
   tm.mark(1);
   const double* fac = Tb.fac;
   // ---- STEP 1: P2M, STEP 2: P2L (list 4)
@@ -264,19 +276,25 @@ int lap_solve(Context& ctx, const LapCall& c) {
   auto t_lists0 = std::chrono::steady_clock::now();
   tree_lists_near(tree);
   build_plan_near(tree, f, FB, O);
-  const int* dIB = upload_ints("plan_ib", FB.I);  // part B1: offsets in O from build_plan_near
-  devplan_near(P, O, dIB);
   // the P2P work items are addressed explicitly: they must lie in part B1
   if (O.p2p0 + 3 * (size_t)O.n_p2p0 > FB.I.size() || O.p2p1 + 3 * (size_t)O.n_p2p1 > FB.I.size())
     throw HipError("plan: P2P work items outside the near-field part of the plan");
   auto t_lists1 = std::chrono::steady_clock::now();
-  // ---- STEP 8: P2P (self + list 1), queued ahead of the far field
+  // ---- STEP 8: P2P (self + list 1) on the second stream, once the inputs
+  // are gathered; its plan part (B1) is uploaded on that stream
+  FMM_HIP(hipStreamWaitEvent(s2, ctx.aux_ev[0], 0));
+  Timer tn;
+  tn.start(s2, 2);
+  const int* dIB = upload_ints("plan_ib", FB.I, s2);  // part B1: offsets in O from build_plan_near
+  devplan_near(P, O, dIB);
   if (f.ifcharge || f.ifdipole) {
     launch_p2p<TN>(P, dIB + O.p2p0, O.n_p2p0, 0, spos, strn, tpos, f.ifcharge, f.ifdipole,
-                   f.iffld != 0, pot_s, fld_s, s);
+                   f.iffld != 0, pot_sn, fld_sn, s2);
     launch_p2p<TN>(P, dIB + O.p2p1, O.n_p2p1, 1, spos, strn, tpos, f.ifcharge, f.ifdipole,
-                   f.iffldtarg != 0, pot_t, fld_t, s);
+                   f.iffldtarg != 0, pot_tn, fld_tn, s2);
   }
+  tn.mark(1);
+  FMM_HIP(hipEventRecord(ctx.aux_ev[1], s2));  // near field done
   tm.mark(4);
   // ---- while the device runs P2P: lists 2-4 and the far-field plan (part B2)
   auto t_lists2 = std::chrono::steady_clock::now();
@@ -284,7 +302,7 @@ int lap_solve(Context& ctx, const LapCall& c) {
   thread_local Flat FC;
   FC.I.clear(); FC.D.clear();
   build_plan_far(tree, f, FC, O);
-  devplan_far(P, O, upload_ints("plan_ic", FC.I));
+  devplan_far(P, O, upload_ints("plan_ic", FC.I, s));
   auto t_lists3 = std::chrono::steady_clock::now();
   form(std::true_type{}, P.n_p2l, lo);
   tm.mark(5);
@@ -361,11 +379,15 @@ int lap_solve(Context& ctx, const LapCall& c) {
     hipLaunchKernelGGL((k_eval<T, true>), dim3(P.n_m2p), dim3(256), lds_e, s, P, fac, mp, spos, tpos,
                        f.ifpot, f.iffld, f.ifpottarg, f.iffldtarg, pot_s, fld_s, pot_t, fld_t);
   tm.mark(9);
-  // ---- scatter back to the caller's order (l3dpsort / l3dfsort)
-  if (f.ifpot) hipLaunchKernelGGL(k_scatter, dim3(blocks(ns, 256)), dim3(256), 0, s, ns, 1, sperm, pot_s, c.pot_dev);
-  if (f.iffld) hipLaunchKernelGGL(k_scatter, dim3(blocks(ns, 256)), dim3(256), 0, s, ns, 3, sperm, fld_s, c.fld_dev);
-  if (f.ifpottarg) hipLaunchKernelGGL(k_scatter, dim3(blocks(nt, 256)), dim3(256), 0, s, nt, 1, tperm, pot_t, c.pottarg_dev);
-  if (f.iffldtarg) hipLaunchKernelGGL(k_scatter, dim3(blocks(nt, 256)), dim3(256), 0, s, nt, 3, tperm, fld_t, c.fldtarg_dev);
+  // ---- far + near, scattered back to the caller's order (l3dpsort / l3dfsort)
+  FMM_HIP(hipStreamWaitEvent(s, ctx.aux_ev[1], 0));
+  auto scatter = [&](int n, int w, const int* perm, const double* far, const double* near, double* out) {
+    hipLaunchKernelGGL(k_scatter_sum, dim3(blocks(n, 256)), dim3(256), 0, s, n, w, perm, far, near, out);
+  };
+  if (f.ifpot) scatter(ns, 1, sperm, pot_s, pot_sn, c.pot_dev);
+  if (f.iffld) scatter(ns, 3, sperm, fld_s, fld_sn, c.fld_dev);
+  if (f.ifpottarg) scatter(nt, 1, tperm, pot_t, pot_tn, c.pottarg_dev);
+  if (f.iffldtarg) scatter(nt, 3, tperm, fld_t, fld_tn, c.fldtarg_dev);
   tm.mark(10);
   FMM_HIP(hipGetLastError());
   FMM_HIP(hipStreamSynchronize(s));
@@ -373,10 +395,12 @@ int lap_solve(Context& ctx, const LapCall& c) {
   lt.assign(13, 0.0);
   auto ms = [](auto a, auto b) { return std::chrono::duration<double, std::milli>(b - a).count(); };
   lt[0] = ms(t0, t1);
-  auto el = tm.elapsed();  // upload, P2M, M2M, P2P, P2L, M2L, L2L, L2P, M2P, scatter
+  // main stream: upload, P2M, M2M, (near-plan gap), P2L, M2L, L2L, L2P, M2P,
+  // wait for the near field + scatter; the P2P time is the second stream's
+  auto el = tm.elapsed();
   el.resize(10, 0.0);
   lt[1] = el[0]; lt[2] = el[1]; lt[3] = el[4]; lt[4] = el[2]; lt[5] = el[5];
-  lt[6] = el[6]; lt[7] = el[7]; lt[8] = el[8]; lt[9] = el[3];
+  lt[6] = el[6]; lt[7] = el[7]; lt[8] = el[8]; lt[9] = tn.elapsed()[0];
   double tot = 0;
   for (auto v : el) tot += v;
   lt[10] = tot + lt[0];

@@ -160,6 +160,17 @@ static __global__ void k_scatter(int n, int w, const int* __restrict__ perm, con
   for (int k = 0; k < 2 * w; k++) out[(size_t)2 * w * o + k] = sorted[(size_t)2 * w * i + k];
 }
 
+// out[perm[i]] = far[i] + near[i]: the far-field and near-field sums of a
+// point, added in that order
+static __global__ void k_scatter_sum(int n, int w, const int* __restrict__ perm, const double* __restrict__ far,
+                                     const double* __restrict__ near, double* __restrict__ out) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  int o = perm[i];
+  for (int k = 0; k < 2 * w; k++)
+    out[(size_t)2 * w * o + k] = far[(size_t)2 * w * i + k] + near[(size_t)2 * w * i + k];
+}
+
 template <typename T>
 __global__ void k_fill_zero(size_t n, T* __restrict__ a) {
   size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;

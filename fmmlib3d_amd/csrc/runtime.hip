@@ -5,6 +5,20 @@ namespace fmm3d {
 
 Arena::~Arena() { release_all(); }
 
+Context::~Context() {
+  for (auto& e : aux_ev)
+    if (e) (void)hipEventDestroy(e);
+  if (aux) (void)hipStreamDestroy(aux);
+}
+
+hipStream_t aux_stream(Context& ctx) {
+  if (!ctx.aux) {
+    FMM_HIP(hipStreamCreateWithFlags(&ctx.aux, hipStreamNonBlocking));
+    for (auto& e : ctx.aux_ev) FMM_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+  }
+  return ctx.aux;
+}
+
 void* Arena::get(const std::string& name, size_t bytes, int ier_on_fail) {
   if (bytes == 0) bytes = 256;
   bytes = (bytes + 255) & ~(size_t)255;

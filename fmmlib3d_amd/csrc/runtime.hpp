@@ -69,7 +69,15 @@ struct Context {
   size_t tab_off_f = 0;
   // cached Helmholtz tables (owned by the Helmholtz driver)
   std::shared_ptr<void> helm_tables;
+  // second stream (non-blocking, default priority) for the near field, and
+  // the two events that order it against `stream`; made on first use
+  hipStream_t aux = nullptr;
+  hipEvent_t aux_ev[2] = {nullptr, nullptr};
+  ~Context();
 };
+
+// the context's second stream, created on first use on the current device
+hipStream_t aux_stream(Context& ctx);
 
 Context& default_context();
 
