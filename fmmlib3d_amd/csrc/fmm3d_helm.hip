@@ -167,14 +167,17 @@ int helm_solve(Context& ctx, const LapCall& c) {
   tm.start(s, 11);
   // plan uploads go through pinned staging buffers so that they do not wait
   // for the kernels already queued
-  auto upload_ints = [&](const char* name, const std::vector<int>& v) {
+  auto upload_ints = [&](const char* name, const std::vector<int>& v, hipStream_t st) {
     int* h = (int*)ctx.arena.get_host(std::string(name) + "_h", std::max<size_t>(v.size(), 1) * sizeof(int));
     std::memcpy(h, v.data(), v.size() * sizeof(int));
     int* d = (int*)ctx.arena.get(name, std::max<size_t>(v.size(), 1) * sizeof(int));
-    FMM_HIP(hipMemcpyAsync(d, h, v.size() * sizeof(int), hipMemcpyHostToDevice, s));
+    FMM_HIP(hipMemcpyAsync(d, h, v.size() * sizeof(int), hipMemcpyHostToDevice, st));
     return d;
   };
-  int* dI = upload_ints("plan_i", FA.I);
+  // the near field (P2P) runs on a second stream into its own sums, which
+  // the final scatter adds to the far-field sums
+  hipStream_t s2 = aux_stream(ctx);
+  int* dI = upload_ints("plan_i", FA.I, s);
   double* hD = (double*)ctx.arena.get_host("plan_d_h", FA.D.size() * sizeof(double));
   std::memcpy(hD, FA.D.data(), FA.D.size() * sizeof(double));
   double* dD = (double*)ctx.arena.get("plan_d", FA.D.size() * sizeof(double));
@@ -206,6 +209,10 @@ int helm_solve(Context& ctx, const LapCall& c) {
   double* fld_s = (double*)ctx.arena.get("fld_s", sizeof(double) * 6 * (size_t)ns);
   double* pot_t = (double*)ctx.arena.get("pot_t", sizeof(double) * 2 * (size_t)std::max(nt, 1));
   double* fld_t = (double*)ctx.arena.get("fld_t", sizeof(double) * 6 * (size_t)std::max(nt, 1));
+  double* pot_sn = (double*)ctx.arena.get("pot_sn", sizeof(double) * 2 * (size_t)ns);
+  double* fld_sn = (double*)ctx.arena.get("fld_sn", sizeof(double) * 6 * (size_t)ns);
+  double* pot_tn = (double*)ctx.arena.get("pot_tn", sizeof(double) * 2 * (size_t)std::max(nt, 1));
+  double* fld_tn = (double*)ctx.arena.get("fld_tn", sizeof(double) * 6 * (size_t)std::max(nt, 1));
   const int* sperm = dsperm ? dsperm : dI + O.sperm;
   const int* tperm = dsperm ? dtperm : dI + O.tperm;
   P.src_perm = sperm;
@@ -221,10 +228,15 @@ int helm_solve(Context& ctx, const LapCall& c) {
   zero(lo, 2 * (size_t)K * nb);
   zero(pot_s, 2 * (size_t)ns);
   zero(fld_s, 6 * (size_t)ns);
+  zero(pot_sn, 2 * (size_t)ns);
+  zero(fld_sn, 6 * (size_t)ns);
   if (nt > 0) {
     zero(pot_t, 2 * (size_t)nt);
     zero(fld_t, 6 * (size_t)nt);
+    zero(pot_tn, 2 * (size_t)nt);
+    zero(fld_tn, 6 * (size_t)nt);
   }
+  FMM_HIP(hipEventRecord(ctx.aux_ev[0], s));  // inputs gathered, sums zeroed
   tm.mark(1);
   const double* fac = Tb.fac;
   auto form = [&](auto irreg, int nbx, double* out) {
@@ -253,25 +265,31 @@ int helm_solve(Context& ctx, const LapCall& c) {
   auto t_lists0 = std::chrono::steady_clock::now();
   tree_lists_near(tree);
   build_plan_near(tree, f, FB, O);
-  const int* dIB = upload_ints("plan_ib", FB.I);
-  devplan_near(P, O, dIB);
   // the P2P work items are addressed explicitly: they must lie in this part
   if (O.p2p0 + 3 * (size_t)O.n_p2p0 > FB.I.size() || O.p2p1 + 3 * (size_t)O.n_p2p1 > FB.I.size())
     throw HipError("plan: P2P work items outside the near-field part of the plan");
   auto t_lists1 = std::chrono::steady_clock::now();
-  // STEP 8: P2P (hfmm3dpart.f:1101-1180), queued ahead of the far field
+  // STEP 8: P2P (hfmm3dpart.f:1101-1180) on the second stream, once the
+  // inputs are gathered; its plan part is uploaded on that stream
+  FMM_HIP(hipStreamWaitEvent(s2, ctx.aux_ev[0], 0));
+  Timer tn;
+  tn.start(s2, 2);
+  const int* dIB = upload_ints("plan_ib", FB.I, s2);
+  devplan_near(P, O, dIB);
   if (f.ifcharge || f.ifdipole) {
     launch_hp2p<TN>(P, dIB + O.p2p0, O.n_p2p0, 0, Hd.kre, Hd.kim, spos, strn, tpos, f.ifcharge, f.ifdipole,
-                    f.iffld != 0, pot_s, fld_s, s);
+                    f.iffld != 0, pot_sn, fld_sn, s2);
     launch_hp2p<TN>(P, dIB + O.p2p1, O.n_p2p1, 1, Hd.kre, Hd.kim, spos, strn, tpos, f.ifcharge, f.ifdipole,
-                    f.iffldtarg != 0, pot_t, fld_t, s);
+                    f.iffldtarg != 0, pot_tn, fld_tn, s2);
   }
+  tn.mark(1);
+  FMM_HIP(hipEventRecord(ctx.aux_ev[1], s2));  // near field done
   tm.mark(4);
   // while the device runs P2P: lists 2-4 and the far-field plan
   auto t_lists2 = std::chrono::steady_clock::now();
   tree_lists_far(tree);
   build_plan_far(tree, f, FC, O);
-  devplan_far(P, O, upload_ints("plan_ic", FC.I));
+  devplan_far(P, O, upload_ints("plan_ic", FC.I, s));
   auto t_lists3 = std::chrono::steady_clock::now();
   // STEP 2: P2L (hfmm3dpart.f:765-830)
   form(std::true_type{}, P.n_p2l, lo);
@@ -387,10 +405,15 @@ int helm_solve(Context& ctx, const LapCall& c) {
     }
   }
   tm.mark(9);
-  if (f.ifpot) hipLaunchKernelGGL(k_scatter, dim3(blocks(ns, 256)), dim3(256), 0, s, ns, 1, sperm, pot_s, c.pot_dev);
-  if (f.iffld) hipLaunchKernelGGL(k_scatter, dim3(blocks(ns, 256)), dim3(256), 0, s, ns, 3, sperm, fld_s, c.fld_dev);
-  if (f.ifpottarg) hipLaunchKernelGGL(k_scatter, dim3(blocks(nt, 256)), dim3(256), 0, s, nt, 1, tperm, pot_t, c.pottarg_dev);
-  if (f.iffldtarg) hipLaunchKernelGGL(k_scatter, dim3(blocks(nt, 256)), dim3(256), 0, s, nt, 3, tperm, fld_t, c.fldtarg_dev);
+  // far + near, scattered back to the caller's order (h3dpsort / h3dfsort)
+  FMM_HIP(hipStreamWaitEvent(s, ctx.aux_ev[1], 0));
+  auto scatter = [&](int n, int w, const int* perm, const double* far, const double* near, double* out) {
+    hipLaunchKernelGGL(k_scatter_sum, dim3(blocks(n, 256)), dim3(256), 0, s, n, w, perm, far, near, out);
+  };
+  if (f.ifpot) scatter(ns, 1, sperm, pot_s, pot_sn, c.pot_dev);
+  if (f.iffld) scatter(ns, 3, sperm, fld_s, fld_sn, c.fld_dev);
+  if (f.ifpottarg) scatter(nt, 1, tperm, pot_t, pot_tn, c.pottarg_dev);
+  if (f.iffldtarg) scatter(nt, 3, tperm, fld_t, fld_tn, c.fldtarg_dev);
   tm.mark(10);
   FMM_HIP(hipGetLastError());
   FMM_HIP(hipStreamSynchronize(s));
@@ -398,10 +421,12 @@ int helm_solve(Context& ctx, const LapCall& c) {
   lt.assign(13, 0.0);
   auto ms = [](auto a, auto b) { return std::chrono::duration<double, std::milli>(b - a).count(); };
   lt[0] = ms(t0, t1);
-  auto el = tm.elapsed();  // upload, P2M, M2M, P2P, P2L, M2L, L2L, L2P, M2P, scatter
+  // main stream: upload, P2M, M2M, (near-plan gap), P2L, M2L, L2L, L2P, M2P,
+  // wait for the near field + scatter; the P2P time is the second stream's
+  auto el = tm.elapsed();
   el.resize(10, 0.0);
   lt[1] = el[0]; lt[2] = el[1]; lt[3] = el[4]; lt[4] = el[2]; lt[5] = el[5];
-  lt[6] = el[6]; lt[7] = el[7]; lt[8] = el[8]; lt[9] = el[3];
+  lt[6] = el[6]; lt[7] = el[7]; lt[8] = el[8]; lt[9] = tn.elapsed()[0];
   double tot = 0;
   for (auto v : el) tot += v;
   lt[10] = tot + lt[0];
