@@ -166,8 +166,11 @@ void fmm3d_server_destroy(void *server);
  * FMM call on this thread: [0] host time before the first kernel (tree and
  * box plan), [1] gather/upload, [2] P2M, [3] P2L, [4] M2M, [5] M2L, [6] L2L,
  * [7] L2P, [8] M2P, [9] P2P, [10] total, [11] tree build (within [0]),
- * [12] host time of the interaction lists and their plan (Laplace: overlapped
- * with P2M and M2M on the device).
+ * [12] host time of the interaction lists and their plan (overlapped with
+ * P2M and M2M, then with P2P, on the device).  P2P runs on a second stream
+ * at the same time as the far-field stages, so [9] overlaps [2]-[8], and
+ * those stage times include the sharing of the GPU; [10] is the end-to-end
+ * time of the call.
  * Mirrors the reference's timeinfo(1..8) (src/lfmm3dpart.f:1155). */
 void fmm3d_last_timings(double *ms, int n);
 
