@@ -49,7 +49,7 @@ FLOPS_PER_PAIR = 65            # charge+dipole, pot+fld, rsqrt counted as 2
 # HBM bytes per k_p2p launch on this workload, from rocprofv3 FETCH_SIZE and
 # WRITE_SIZE runs calibrated against a 1 GiB device copy
 # (profiles/fmm_kernels_lap1M_helm500k.md): 2 x FETCH KiB + WRITE KiB.
-P2P_TRAFFIC_BYTES = {"fp64": (2 * 2289591 + 62655) * 1024, "fp32": (2 * 1463937 + 62655) * 1024}
+P2P_TRAFFIC_BYTES = {"fp64": (2 * 2293902 + 62655) * 1024, "fp32": (2 * 1468419 + 62655) * 1024}
 
 
 def make_inputs(n, seed):
@@ -405,7 +405,9 @@ def main():
         "roofline": {"kernel": "k_p2p (near field, both launches)", "bound": "mfma", "achieved": achieved,
                      "peak": peak, "unit": "TFLOP/s", "frac": achieved / peak,
                      "traffic": P2P_TRAFFIC_BYTES["fp64" if best == "fp64" else "fp32"],
-                     "note": "P2P is FP64 VALU bound; MI355X FP64 vector peak equals its FP64 matrix peak. "
+                     "note": "P2P is VALU bound (no matrix-core form); peak = the vector peak of the pair "
+                             "arithmetic's type. Both launches run on the second stream beside the far-field "
+                             "kernels and are timed with HIP events on that stream. "
                              "algorithmic flops = %d pairs x %d" % (pairs, FLOPS_PER_PAIR)},
         "cpu_baseline": cpu,
         "modes": {k: {kk: vv for kk, vv in v.items() if kk != "latencies_ms"} for k, v in modes.items()},

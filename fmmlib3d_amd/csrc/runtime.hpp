@@ -73,6 +73,9 @@ struct Context {
   // the two events that order it against `stream`; made on first use
   hipStream_t aux = nullptr;
   hipEvent_t aux_ev[2] = {nullptr, nullptr};
+  // false: the near field stays on `stream` (the request server, whose
+  // workers already overlap whole requests)
+  bool near_on_aux = true;
   ~Context();
 };
 

@@ -51,7 +51,8 @@ size_t estimate_bytes(const Request& r) {
   static const int pl[] = {2, 5, 9, 14, 26, 39, 51, 64, 1};
   const int k = std::min(std::max(r.iprec + 2, 0), 8);
   const size_t npts = (size_t)r.ns + r.nt;
-  const size_t per_point = 24 + 32 + 64 + 32 + 16 + 48 + 16 + 48;   // in, sorted, strengths, outputs
+  // in, sorted, strengths, outputs, far and near sums
+  const size_t per_point = 24 + 32 + 64 + 32 + 16 + 48 + 16 + 48 + 16 + 48;
   size_t leaves = nb[k] ? npts / nb[k] + 1 : 1;
   size_t boxes = leaves * 8 / 7 + 1;
   int p = pl[k];
@@ -182,6 +183,7 @@ void* fmm3d_server_create(int nworkers, long long mem_budget_bytes) {
   for (int i = 0; i < nworkers; i++) {
     auto w = std::make_unique<fmm3d::Worker>();
     w->ctx.device = dev;
+    w->ctx.near_on_aux = false;   // the workers overlap whole requests instead
     if (hipStreamCreateWithFlags(&w->ctx.stream, hipStreamNonBlocking) != hipSuccess) {
       (void)hipGetLastError();
       w->ctx.stream = nullptr;
