@@ -94,7 +94,8 @@ void hp2p(const DevPlan& P, const int* items, int n, int kind, double kre, doubl
   DevPlan Q = P;
   Q.p2p_items = items;
   Q.n_p2p = n;
-  hipLaunchKernelGGL((k_hp2p<TN, CHG, DIP, FLD>), dim3(n), dim3(kP2PBlock), 0, s, Q, kre, kim, spos, str, tpos, kind, pot, fo);
+  hipLaunchKernelGGL((k_hp2p<TN, CHG, DIP, FLD>), dim3(item_groups(n)), dim3(kP2PBlock * kItemWaves), 0, s, Q, kre, kim,
+                     spos, str, tpos, kind, pot, fo);
 }
 
 template <typename TN>

@@ -131,7 +131,7 @@ void launch_p2p(const DevPlan& P, const int* items, int n, int kind, const doubl
   DevPlan Q = P;
   Q.p2p_items = items;
   Q.n_p2p = n;
-  dim3 g(n), b(kP2PBlock);
+  dim3 g(item_groups(n)), b(kP2PBlock * kItemWaves);
 #define P2P_CASE(C, Dp, F)                                                                        \
   if (ifcharge == C && ifdipole == Dp && fld == F) {                                             \
     hipLaunchKernelGGL((k_p2p<TN, C, Dp, F>), g, b, 0, s, Q, spos, str, tpos, kind, pot, fo);    \
@@ -371,7 +371,7 @@ int lap_solve(Context& ctx, const LapCall& c) {
     hipLaunchKernelGGL((k_l2p_pack<T>), dim3((unsigned)std::min<size_t>(blocks(tot, 256), 65536)), dim3(256), 0, s,
                        P, fac, lo, pack);
     if (P.n_l2pi)
-      hipLaunchKernelGGL((k_l2p2<T>), dim3(P.n_l2pi), dim3(256), 0, s, P, fac, pack, spos, tpos, f.ifpot, f.iffld,
+      hipLaunchKernelGGL((k_l2p2<T>), dim3(item_groups(P.n_l2pi)), dim3(kP2PBlock * kItemWaves), 0, s, P, fac, pack, spos, tpos, f.ifpot, f.iffld,
                          f.ifpottarg, f.iffldtarg, pot_s, fld_s, pot_t, fld_t);
   }
   tm.mark(8);

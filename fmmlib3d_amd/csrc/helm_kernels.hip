@@ -1301,11 +1301,13 @@ template <typename TN, bool CHG, bool DIP, bool FLD>
 __global__ __launch_bounds__(256) void k_hp2p(DevPlan P, double kre, double kim, const double* __restrict__ spos,
                                               const TN* __restrict__ str, const double* __restrict__ tpos,
                                               int kind, double* __restrict__ pot, double* __restrict__ fld) {
-  const int w = blockIdx.x;
+  // one work item (64 points) per wave
+  const int w = __builtin_amdgcn_readfirstlane((int)(blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)));
+  if (w >= P.n_p2p) return;
   const int b = P.p2p_items[3 * w], start = P.p2p_items[3 * w + 2];
   const int n0 = kind ? P.itrg[b] : P.isrc[b];
   const int nn = kind ? P.ntrg[b] : P.nsrc[b];
-  const int i = start + threadIdx.x;
+  const int i = start + (int)(threadIdx.x & 63);
   const bool active = i < nn;
   const int me = n0 + (active ? i : 0);
   const double* mp = (kind ? tpos : spos) + 4 * (size_t)me;

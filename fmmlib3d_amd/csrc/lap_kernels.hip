@@ -535,7 +535,9 @@ __global__ __launch_bounds__(256) void k_l2p2(DevPlan P, const double* __restric
   const int p = P.p;
   const int H = (p + 1) * (p + 2) / 2;
   FacLayout L(p);
-  const int w = blockIdx.x;
+  // one work item (64 points) per wave
+  const int w = __builtin_amdgcn_readfirstlane((int)(blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)));
+  if (w >= P.n_l2pi) return;
   const int ib = P.l2p_items[3 * w], kind = P.l2p_items[3 * w + 1], start = P.l2p_items[3 * w + 2];
   const int b = P.l2p_boxes[ib];
   const T* pk = pack + (size_t)ib * H * 16;
@@ -544,7 +546,7 @@ __global__ __launch_bounds__(256) void k_l2p2(DevPlan P, const double* __restric
   const int wp = kind ? wpot1 : wpot0, wf = kind ? wfld1 : wfld0;
   const int n0 = kind ? P.itrg[b] : P.isrc[b];
   const int nn = kind ? P.ntrg[b] : P.nsrc[b];
-  const int i = start + threadIdx.x;
+  const int i = start + (int)(threadIdx.x & 63);
   const bool active = i < nn;
   double x = cx, y = cy, z = cz;
   if (active) {
@@ -1380,11 +1382,13 @@ template <typename TN, bool CHG, bool DIP, bool FLD>
 __global__ __launch_bounds__(256) void k_p2p(DevPlan P, const double* __restrict__ spos,
                                              const TN* __restrict__ str, const double* __restrict__ tpos,
                                              int kind, double* __restrict__ pot, double* __restrict__ fld) {
-  const int w = blockIdx.x;
+  // one work item (64 points) per wave
+  const int w = __builtin_amdgcn_readfirstlane((int)(blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)));
+  if (w >= P.n_p2p) return;
   const int b = P.p2p_items[3 * w], start = P.p2p_items[3 * w + 2];
   const int n0 = kind ? P.itrg[b] : P.isrc[b];
   const int nn = kind ? P.ntrg[b] : P.nsrc[b];
-  const int i = start + threadIdx.x;
+  const int i = start + (int)(threadIdx.x & 63);
   const bool active = i < nn;
   const int me = n0 + (active ? i : 0);
   const double* mp = (kind ? tpos : spos) + 4 * (size_t)me;

@@ -13,7 +13,15 @@
 
 namespace fmm3d {
 
-constexpr int kP2PBlock = 256;
+// Evaluation points per P2P / L2P work item and threads per workgroup: one
+// wave per item, so a leaf of n points keeps ceil(n/64) waves busy (with
+// 256-point items a 244-point leaf left 12 lanes idle and a 300-point leaf
+// 212)
+constexpr int kP2PBlock = 64;
+// items per workgroup (one per wave): the workgroups stay 256 threads wide,
+// like those of the far-field kernels that run beside the near field
+constexpr int kItemWaves = 4;
+inline unsigned item_groups(int n) { return (unsigned)((n + kItemWaves - 1) / kItemWaves); }
 
 // iprec -> tolerance and leaf size (src/lfmm3dpart.f:277-300)
 inline bool iprec_params(int iprec, int ns, int nt, double& eps, int& nbox) {
@@ -150,7 +158,7 @@ inline void build_plan_far(const Octree& t, const Flags& f, Flat& F, PlanOffsets
   O.l2p = F.i(l2p); O.n_l2p = (int)l2p.size();
   {
     // L2P work items (position in the L2P list, kind, first point): one
-    // workgroup per 256 evaluation points of one kind
+    // workgroup per kP2PBlock evaluation points of one kind
     std::vector<int> it;
     for (int i = 0; i < (int)l2p.size(); i++) {
       const int b = l2p[i];
