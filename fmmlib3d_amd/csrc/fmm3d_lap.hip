@@ -335,7 +335,8 @@ int lap_solve(Context& ctx, const LapCall& c) {
     int gmax = 0;
     // LDS: six int arrays, the local, a guard of 8 values, g members of (p+4)^2
     auto m2l_lds = [&](int g) {
-      return 6 * kM2LChunk * sizeof(int) + sizeof(T) * (2 * (size_t)K + 8 + (size_t)g * grp_member_stride<T>(p));
+      return (6 * kM2LChunk + (size_t)grp_kn_ints(p)) * sizeof(int) +
+             sizeof(T) * (2 * (size_t)kM2LGroup * (p + 1) + 2 * (size_t)K + 8 + (size_t)g * grp_member_stride<T>(p));
     };
     for (int g = gcap; g >= 1 && !gmax; g--) {
       const int gs = g > 2 ? 4 : g;

@@ -772,6 +772,10 @@ constexpr int kM2LChunk = 192; // list-2 entries ordered at a time (list 2 has <
 // Values per member in LDS: (p+4)^2 complex, padded so that the stride in
 // 4-byte words is 8 mod 32 and the four members of a work item start in
 // different quarters of the 32 LDS banks.
+// ints of the per-index degree table of k_m2l_grp, kept a multiple of 4 so
+// that what follows stays 16-byte aligned
+__host__ __device__ inline int grp_kn_ints(int p) { return ((p + 4) * (p + 4) + 3) / 4 * 4; }
+
 template <typename T>
 __host__ __device__ inline int grp_member_stride(int p) {
   const int base = 2 * (p + 4) * (p + 4);
@@ -974,16 +978,26 @@ __global__ __launch_bounds__(NT) void k_m2l_grp(DevPlan P, DevTables Tb, int gma
   int* t_key = s_code + kM2LChunk;
   int* t_box = t_key + kM2LChunk;
   int* t_code = t_box + kM2LChunk;
-  T* acc = reinterpret_cast<T*>(t_code + kM2LChunk);
   const int tid = threadIdx.x, bs = blockDim.x;
   const int b = P.m2l_boxes[blockIdx.x];
   const int p = P.p, P1 = p + 1, K2 = 2 * P.K;
+  // degree of each packed index k < (p+4)^2, and the members' phases
+  // e^{i am phi}, am = 0..p, of the group in hand
+  int* s_kn = t_code + kM2LChunk;
+  T* s_ph = reinterpret_cast<T*>(s_kn + grp_kn_ints(p));
+  T* acc = s_ph + 2 * kM2LGroup * P1;
   // members: (p+4)^2 complex each (degrees p+1..p+3 stay zero or hold the
   // zeros written for a truncated member), after a zero guard of 8 values
   const int MS = grp_member_stride<T>(p);
   T* A = acc + K2 + 8;
   const T ih = (T)P.inv_side[P.level[b]];
   for (int k = tid; k < K2 + 8 + gmax * MS; k += bs) acc[k] = T(0);
+  for (int k = tid; k < (p + 4) * (p + 4); k += bs) {
+    int n = (int)sqrtf((float)k);
+    while (n * n > k) n--;
+    while ((n + 1) * (n + 1) <= k) n++;
+    s_kn[k] = n;
+  }
   const T* rtab = grp_rot_tab<T>(Tb);
   const T* ztab = grp_z_tab<T>(Tb);
   const int off0 = P.l2_off[b], n2 = P.l2_off[b + 1] - off0;
@@ -1022,24 +1036,31 @@ __global__ __launch_bounds__(NT) void k_m2l_grp(DevPlan P, DevTables Tb, int gma
       const int rc = Tb.off_rot[code0];
       const T* Z = ztab + (size_t)Tb.off_dist[code0] * Tb.zt_size;
       const int Kp = (pt + 1) * (pt + 1), Kz = (pt + 4) * (pt + 4);
+      for (int idx = tid; idx < g * (pt + 1); idx += bs) {
+        const int jj = idx / (pt + 1), am = idx - jj * (pt + 1);
+        const double* ph = Tb.off_phase + (size_t)t_code[s0 + jj] * 2 * P1;
+        s_ph[2 * (jj * P1 + am)] = (T)ph[2 * am];
+        s_ph[2 * (jj * P1 + am) + 1] = (T)ph[2 * am + 1];
+      }
+      __syncthreads();
       // (a) members times e^{i m phi}; degrees pt+1..pt+3 set to zero
-      for (int idx = tid; idx < g * Kz; idx += bs) {
-        const int jj = idx / Kz, k = idx - jj * Kz;
-        T vr = 0, vi = 0;
-        if (k < Kp) {
-          int n = (int)sqrtf((float)k);
-          while (n * n > k) n--;
-          while ((n + 1) * (n + 1) <= k) n++;
-          const int m = k - n * n - n, am = m < 0 ? -m : m;
-          const double* ph = Tb.off_phase + (size_t)t_code[s0 + jj] * 2 * P1;
-          const T er = (T)ph[2 * am], ei = m < 0 ? -(T)ph[2 * am + 1] : (T)ph[2 * am + 1];
-          const T* src = mp + (size_t)t_box[s0 + jj] * K2;
-          const T xr = src[2 * k], xi = src[2 * k + 1];
-          vr = xr * er - xi * ei;
-          vi = xr * ei + xi * er;
+      for (int jj = 0; jj < g; jj++) {
+        const T* src = mp + (size_t)t_box[s0 + jj] * K2;
+        const T* phj = s_ph + 2 * jj * P1;
+        T* Aj = A + jj * MS;
+        for (int k = tid; k < Kz; k += bs) {
+          T vr = 0, vi = 0;
+          if (k < Kp) {
+            const int n = s_kn[k];
+            const int m = k - n * n - n, am = m < 0 ? -m : m;
+            const T er = phj[2 * am], ei = m < 0 ? -phj[2 * am + 1] : phj[2 * am + 1];
+            const T xr = src[2 * k], xi = src[2 * k + 1];
+            vr = xr * er - xi * ei;
+            vi = xr * ei + xi * er;
+          }
+          Aj[2 * k] = vr;
+          Aj[2 * k + 1] = vi;
         }
-        A[jj * MS + 2 * k] = vr;
-        A[jj * MS + 2 * k + 1] = vi;
       }
       __syncthreads();
       // Work items of a stage (rotation items by degree, z-shift items by
@@ -1103,11 +1124,11 @@ __global__ __launch_bounds__(NT) void k_m2l_grp(DevPlan P, DevTables Tb, int gma
           if (mem) {
             grp_rot_item<T, true>(rtab + Tb.rot_half_off[rc * P1 + n] + 4 * mp0, n / 4 + 1, n, mp0 / 4,
                                   A + jj * MS, r[q]);
-            const double* ph = Tb.off_phase + (size_t)t_code[s0 + jj] * 2 * P1;
+            const T* ph = s_ph + 2 * jj * P1;
 #pragma unroll
             for (int c = 0; c < 4; c++) {
               const int mq = mp0 + c <= n ? mp0 + c : n;
-              const T er = (T)ph[2 * mq] * ih, ei = (T)ph[2 * mq + 1] * ih;
+              const T er = ph[2 * mq] * ih, ei = ph[2 * mq + 1] * ih;
               const T pr = r[q][4 * c], pi = r[q][4 * c + 1], qr = r[q][4 * c + 2], qi = r[q][4 * c + 3];
               r[q][4 * c] = pr * er + pi * ei;       // (pr + i pi) e^{-i m' phi}
               r[q][4 * c + 1] = pi * er - pr * ei;
