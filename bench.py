@@ -373,6 +373,16 @@ def main():
     flops = pairs * FLOPS_PER_PAIR
     achieved = flops / (M["p2p_ms"] * 1e-3) / 1e12
     peak = FP64_PEAK_TFLOPS if best == "fp64" else FP32_PEAK_TFLOPS
+    # the same kernel with the near field kept on the caller's stream (no
+    # far-field kernels beside it): a few extra steps, outside the timed ones
+    prec_of = {"fp64": F.PREC_FP64, "mixed_near": F.PREC_MIXED, "mixed": F.PREC_FP32}
+    os.environ["FMM3D_NEAR_STREAM"] = "0"
+    try:
+        _, _, st_alone = run_mode(F, torch, prec_of[best], dev_in, dev_out, 2, 1)
+    finally:
+        del os.environ["FMM3D_NEAR_STREAM"]
+    p2p_alone_ms = float(np.mean([s_["p2p"] for s_ in st_alone]))
+    achieved_alone = flops / (p2p_alone_ms * 1e-3) / 1e12
     helm = None if args.no_helm else helmholtz_leg(F, torch, ob, steps=max(2, min(args.steps, 3)), warmup=1)
     if helm:
         print("# helmholtz: %.1f ms/step, err pot %.2e fld %.2e, stages %s"
@@ -404,10 +414,13 @@ def main():
         "rel_err_vs_fp64_direct": {"pottarg": M["err_pottarg"], "fldtarg": M["err_fldtarg"], "sampled_targets": 1000},
         "roofline": {"kernel": "k_p2p (near field, both launches)", "bound": "mfma", "achieved": achieved,
                      "peak": peak, "unit": "TFLOP/s", "frac": achieved / peak,
+                     "achieved_alone": achieved_alone, "frac_alone": achieved_alone / peak,
+                     "p2p_ms": M["p2p_ms"], "p2p_ms_alone": p2p_alone_ms,
                      "traffic": P2P_TRAFFIC_BYTES["fp64" if best == "fp64" else "fp32"],
                      "note": "P2P is VALU bound (no matrix-core form); peak = the vector peak of the pair "
                              "arithmetic's type. Both launches run on the second stream beside the far-field "
-                             "kernels and are timed with HIP events on that stream. "
+                             "kernels and are timed with HIP events on that stream; *_alone: the same launches "
+                             "with the near field on the caller's stream (FMM3D_NEAR_STREAM=0), 2 extra steps. "
                              "algorithmic flops = %d pairs x %d" % (pairs, FLOPS_PER_PAIR)},
         "cpu_baseline": cpu,
         "modes": {k: {kk: vv for kk, vv in v.items() if kk != "latencies_ms"} for k, v in modes.items()},

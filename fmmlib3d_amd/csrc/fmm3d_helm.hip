@@ -177,8 +177,7 @@ int helm_solve(Context& ctx, const LapCall& c) {
   };
   // the near field (P2P) runs on a second stream into its own sums, which
   // the final scatter adds to the far-field sums
-  hipStream_t s2 = aux_stream(ctx);
-  if (!ctx.near_on_aux) s2 = s;
+  hipStream_t s2 = near_stream(ctx);
   int* dI = upload_ints("plan_i", FA.I, s);
   double* hD = (double*)ctx.arena.get_host("plan_d_h", FA.D.size() * sizeof(double));
   std::memcpy(hD, FA.D.data(), FA.D.size() * sizeof(double));

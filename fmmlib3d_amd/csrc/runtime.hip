@@ -1,6 +1,9 @@
 // Device runtime: arena, timers, default context.
 #include "runtime.hpp"
 
+#include <cstdlib>
+#include <cstring>
+
 namespace fmm3d {
 
 Arena::~Arena() { release_all(); }
@@ -17,6 +20,13 @@ hipStream_t aux_stream(Context& ctx) {
     for (auto& e : ctx.aux_ev) FMM_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
   }
   return ctx.aux;
+}
+
+hipStream_t near_stream(Context& ctx) {
+  hipStream_t s2 = aux_stream(ctx);
+  const char* v = std::getenv("FMM3D_NEAR_STREAM");
+  if (!ctx.near_on_aux || (v && std::strcmp(v, "0") == 0)) return ctx.stream;
+  return s2;
 }
 
 void* Arena::get(const std::string& name, size_t bytes, int ier_on_fail) {

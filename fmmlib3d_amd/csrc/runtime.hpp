@@ -82,6 +82,10 @@ struct Context {
 // the context's second stream, created on first use on the current device
 hipStream_t aux_stream(Context& ctx);
 
+// the stream the near field of this call runs on: the second stream, or
+// `ctx.stream` when the context keeps it there or FMM3D_NEAR_STREAM=0
+hipStream_t near_stream(Context& ctx);
+
 Context& default_context();
 
 // last per-stage timings of this thread
