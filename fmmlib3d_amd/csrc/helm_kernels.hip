@@ -1248,7 +1248,8 @@ struct HPairAcc {
     const TN dx = (TN)dxd, dy = (TN)dyd, dz = (TN)dzd, r2 = (TN)r2d;
     const TN kre = (TN)kre_d, kim = (TN)kim_d;
     TN ir = rsqrt_t<TN>(r2);
-    ir = ir * (TN(1.5) - TN(0.5) * r2 * ir * ir);   // one Newton step
+    // FP32: one Newton step on the hardware estimate (FP64 rsqrt_t is refined already)
+    if (sizeof(TN) == sizeof(float)) ir = ir * (TN(1.5) - TN(0.5) * r2 * ir * ir);
     if (mask) ir = TN(0);                            // a source never acts on itself
     const TN r = r2 * ir;
     TN s, c;
