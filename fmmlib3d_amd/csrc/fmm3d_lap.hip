@@ -1,6 +1,8 @@
-// Laplace particle FMM driver: host octree -> flat plan -> one upload ->
-// the eight FMM steps of lfmm3dparttargmain (src/lfmm3dpart.f:528-1169) as
-// HIP kernels on one stream -> scatter back to the caller's order.
+// Laplace particle FMM driver: device octree -> box plan -> the eight FMM
+// steps of lfmm3dparttargmain (src/lfmm3dpart.f:528-1169) as HIP kernels,
+// the interaction lists built on the host while the first kernels run, the
+// near field on a second stream -> far + near scattered back to the
+// caller's order.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -8,6 +10,9 @@
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
+#include <map>
+#include <memory>
+#include <mutex>
 #include <string>
 #include <type_traits>
 #include <vector>
@@ -24,102 +29,115 @@ namespace fmm3d {
 
 namespace {
 
-// Upload (once per order p) the translation tables.
-DevTables ensure_tables(Context& ctx, int p, double eps) {
-  if (ctx.tab_p != p) {
-    HostTables H;
-    build_tables(p, eps, H);
-    // ints: off_rot | off_dist | off_ptrunc | oct_rot ; doubles: fac | off_phase |
-    // oct_phase | rot | z_m2l | z_m2m | z_l2l ; longs: rot_off
-    ctx.tab_host_i.clear();
-    ctx.tab_host_d.clear();
-    auto& Ii = ctx.tab_host_i;
-    auto& Dd = ctx.tab_host_d;
-    Ii.insert(Ii.end(), H.off_rot.begin(), H.off_rot.end());
-    Ii.insert(Ii.end(), H.off_dist.begin(), H.off_dist.end());
-    Ii.insert(Ii.end(), H.off_ptrunc.begin(), H.off_ptrunc.end());
-    Ii.insert(Ii.end(), H.oct_rot.begin(), H.oct_rot.end());
-    const std::vector<int>* gi[4] = {&H.grp_ritems, &H.grp_rcount, &H.grp_zitems, &H.grp_zoff};
-    for (int q = 0; q < 4; q++) {
-      ctx.tab_off_i[q] = Ii.size();
-      Ii.insert(Ii.end(), gi[q]->begin(), gi[q]->end());
-    }
-    size_t o_fac = 0;
-    Dd.insert(Dd.end(), H.fac.begin(), H.fac.end());
-    size_t o_ph = Dd.size();
-    Dd.insert(Dd.end(), H.off_phase.begin(), H.off_phase.end());
-    size_t o_oph = Dd.size();
-    Dd.insert(Dd.end(), H.oct_phase.begin(), H.oct_phase.end());
-    size_t o_rot = Dd.size();
-    Dd.insert(Dd.end(), H.rot.begin(), H.rot.end());
-    size_t o_zl = Dd.size();
-    Dd.insert(Dd.end(), H.z_m2l.begin(), H.z_m2l.end());
-    size_t o_zm = Dd.size();
-    Dd.insert(Dd.end(), H.z_m2m.begin(), H.z_m2m.end());
-    size_t o_zll = Dd.size();
-    Dd.insert(Dd.end(), H.z_l2l.begin(), H.z_l2l.end());
-    size_t o_del = Dd.size();
-    Dd.insert(Dd.end(), H.delta.begin(), H.delta.end());
-    size_t o_re = Dd.size();
-    Dd.insert(Dd.end(), H.rot_e.begin(), H.rot_e.end());
-    // 32-byte aligned starts for the padded tables of the grouped M2L
-    while (Dd.size() % 4) Dd.push_back(0.0);
-    ctx.tab_off_g[0] = Dd.size();
-    Dd.insert(Dd.end(), H.rot_half.begin(), H.rot_half.end());
-    while (Dd.size() % 4) Dd.push_back(0.0);
-    ctx.tab_off_g[1] = Dd.size();
-    Dd.insert(Dd.end(), H.z_m2l_t.begin(), H.z_m2l_t.end());
-    ctx.tab_host_l = H.rot_off;
-    ctx.tab_off_l = ctx.tab_host_l.size();
-    ctx.tab_host_l.insert(ctx.tab_host_l.end(), H.delta_off.begin(), H.delta_off.end());
-    ctx.tab_off_g[2] = ctx.tab_host_l.size();
-    ctx.tab_host_l.insert(ctx.tab_host_l.end(), H.rot_half_off.begin(), H.rot_half_off.end());
-    ctx.tab_off_g[3] = ctx.tab_host_l.size();
-    ctx.tab_host_l.insert(ctx.tab_host_l.end(), H.zt_off.begin(), H.zt_off.end());
-    ctx.tab_zt_size = H.zt_size;
-    // FP32 copies of the grouped-M2L tables for the FP32 far field
-    ctx.tab_host_f.assign(H.rot_half.begin(), H.rot_half.end());
-    while (ctx.tab_host_f.size() % 4) ctx.tab_host_f.push_back(0.0f);
-    ctx.tab_off_f = ctx.tab_host_f.size();
-    ctx.tab_host_f.insert(ctx.tab_host_f.end(), H.z_m2l_t.begin(), H.z_m2l_t.end());
-    ctx.tab_off[7] = o_del;
-    ctx.tab_off_re = o_re;
-    ctx.tab_off[0] = o_fac; ctx.tab_off[1] = o_ph; ctx.tab_off[2] = o_oph; ctx.tab_off[3] = o_rot;
-    ctx.tab_off[4] = o_zl; ctx.tab_off[5] = o_zm; ctx.tab_off[6] = o_zll;
-    void* di = ctx.arena.get("tab_i", Ii.size() * sizeof(int));
-    void* dd = ctx.arena.get("tab_d", Dd.size() * sizeof(double));
-    void* dl = ctx.arena.get("tab_l", ctx.tab_host_l.size() * sizeof(long));
-    FMM_HIP(hipMemcpyAsync(di, Ii.data(), Ii.size() * sizeof(int), hipMemcpyHostToDevice, ctx.stream));
-    FMM_HIP(hipMemcpyAsync(dd, Dd.data(), Dd.size() * sizeof(double), hipMemcpyHostToDevice, ctx.stream));
-    FMM_HIP(hipMemcpyAsync(dl, ctx.tab_host_l.data(), ctx.tab_host_l.size() * sizeof(long),
-                           hipMemcpyHostToDevice, ctx.stream));
-    void* df = ctx.arena.get("tab_f", ctx.tab_host_f.size() * sizeof(float));
-    FMM_HIP(hipMemcpyAsync(df, ctx.tab_host_f.data(), ctx.tab_host_f.size() * sizeof(float),
-                           hipMemcpyHostToDevice, ctx.stream));
-    FMM_HIP(hipStreamSynchronize(ctx.stream));
-    ctx.tab_p = p;
+// Translation tables of one order, flattened for upload: ints, doubles, longs
+// and the FP32 copies of the grouped-M2L tables, with the offsets of each part.
+struct LapTabSet {
+  std::vector<int> I;
+  std::vector<double> D;
+  std::vector<long> L;
+  std::vector<float> F;
+  size_t off[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  size_t off_l = 0, off_re = 0;
+  size_t off_i[4] = {0, 0, 0, 0};  // grouped M2L items: ritems, rcount, zitems, zoff
+  size_t off_g[4] = {0, 0, 0, 0};  // grouped M2L tables: rot_half, z_m2l_t (doubles); their offsets (longs)
+  size_t off_f = 0;
+  long zt_size = 0;
+};
+
+// The host tables of order p, built once per process (tens to a hundred ms
+// at p = 26..39) and shared by every context.
+std::shared_ptr<const LapTabSet> lap_host_tables(int p, double eps) {
+  static std::mutex mu;
+  static std::map<std::pair<int, double>, std::shared_ptr<const LapTabSet>> cache;
+  std::lock_guard<std::mutex> lk(mu);
+  auto it = cache.find({p, eps});
+  if (it != cache.end()) return it->second;
+  HostTables H;
+  build_tables(p, eps, H);
+  auto T = std::make_shared<LapTabSet>();
+  // ints: off_rot | off_dist | off_ptrunc | oct_rot | grouped items ; doubles:
+  // fac | off_phase | oct_phase | rot | z_m2l | z_m2m | z_l2l | delta | rot_e |
+  // rot_half | z_m2l_t ; longs: rot_off | delta_off | rot_half_off | zt_off
+  auto& Ii = T->I;
+  auto& Dd = T->D;
+  Ii.insert(Ii.end(), H.off_rot.begin(), H.off_rot.end());
+  Ii.insert(Ii.end(), H.off_dist.begin(), H.off_dist.end());
+  Ii.insert(Ii.end(), H.off_ptrunc.begin(), H.off_ptrunc.end());
+  Ii.insert(Ii.end(), H.oct_rot.begin(), H.oct_rot.end());
+  const std::vector<int>* gi[4] = {&H.grp_ritems, &H.grp_rcount, &H.grp_zitems, &H.grp_zoff};
+  for (int q = 0; q < 4; q++) {
+    T->off_i[q] = Ii.size();
+    Ii.insert(Ii.end(), gi[q]->begin(), gi[q]->end());
   }
-  const int* di = (const int*)ctx.arena.get("tab_i", ctx.tab_host_i.size() * sizeof(int));
-  const double* dd = (const double*)ctx.arena.get("tab_d", ctx.tab_host_d.size() * sizeof(double));
-  const long* dl = (const long*)ctx.arena.get("tab_l", ctx.tab_host_l.size() * sizeof(long));
+  const std::vector<double>* dparts[9] = {&H.fac, &H.off_phase, &H.oct_phase, &H.rot, &H.z_m2l,
+                                          &H.z_m2m, &H.z_l2l, &H.delta, &H.rot_e};
+  size_t o[9];
+  for (int q = 0; q < 9; q++) {
+    o[q] = Dd.size();
+    Dd.insert(Dd.end(), dparts[q]->begin(), dparts[q]->end());
+  }
+  // 32-byte aligned starts for the padded tables of the grouped M2L
+  while (Dd.size() % 4) Dd.push_back(0.0);
+  T->off_g[0] = Dd.size();
+  Dd.insert(Dd.end(), H.rot_half.begin(), H.rot_half.end());
+  while (Dd.size() % 4) Dd.push_back(0.0);
+  T->off_g[1] = Dd.size();
+  Dd.insert(Dd.end(), H.z_m2l_t.begin(), H.z_m2l_t.end());
+  T->L = H.rot_off;
+  T->off_l = T->L.size();
+  T->L.insert(T->L.end(), H.delta_off.begin(), H.delta_off.end());
+  T->off_g[2] = T->L.size();
+  T->L.insert(T->L.end(), H.rot_half_off.begin(), H.rot_half_off.end());
+  T->off_g[3] = T->L.size();
+  T->L.insert(T->L.end(), H.zt_off.begin(), H.zt_off.end());
+  T->zt_size = H.zt_size;
+  // FP32 copies of the grouped-M2L tables for the FP32 far field
+  T->F.assign(H.rot_half.begin(), H.rot_half.end());
+  while (T->F.size() % 4) T->F.push_back(0.0f);
+  T->off_f = T->F.size();
+  T->F.insert(T->F.end(), H.z_m2l_t.begin(), H.z_m2l_t.end());
+  for (int q = 0; q < 7; q++) T->off[q] = o[q];
+  T->off[7] = o[7];
+  T->off_re = o[8];
+  cache[{p, eps}] = T;
+  return T;
+}
+
+// The device tables of order p for this context: uploaded on first use of
+// the order and kept (each order under its own arena names).
+DevTables ensure_tables(Context& ctx, int p, double eps) {
+  const auto hs = lap_host_tables(p, eps);
+  const LapTabSet& H = *hs;
+  const std::string sfx = "." + std::to_string(p);
+  int* di = (int*)ctx.arena.get("tab_i" + sfx, H.I.size() * sizeof(int));
+  double* dd = (double*)ctx.arena.get("tab_d" + sfx, H.D.size() * sizeof(double));
+  long* dl = (long*)ctx.arena.get("tab_l" + sfx, H.L.size() * sizeof(long));
+  float* df = (float*)ctx.arena.get("tab_f" + sfx, H.F.size() * sizeof(float));
+  if (!ctx.lap_tabs.count(p)) {
+    FMM_HIP(hipMemcpyAsync(di, H.I.data(), H.I.size() * sizeof(int), hipMemcpyHostToDevice, ctx.stream));
+    FMM_HIP(hipMemcpyAsync(dd, H.D.data(), H.D.size() * sizeof(double), hipMemcpyHostToDevice, ctx.stream));
+    FMM_HIP(hipMemcpyAsync(dl, H.L.data(), H.L.size() * sizeof(long), hipMemcpyHostToDevice, ctx.stream));
+    FMM_HIP(hipMemcpyAsync(df, H.F.data(), H.F.size() * sizeof(float), hipMemcpyHostToDevice, ctx.stream));
+    FMM_HIP(hipStreamSynchronize(ctx.stream));
+    ctx.lap_tabs.insert(p);
+  }
   DevTables T;
   T.p = p;
   T.off_rot = di; T.off_dist = di + 343; T.off_ptrunc = di + 686; T.oct_rot = di + 1029;
-  T.fac = dd + ctx.tab_off[0]; T.off_phase = dd + ctx.tab_off[1]; T.oct_phase = dd + ctx.tab_off[2];
-  T.rot = dd + ctx.tab_off[3]; T.z_m2l = dd + ctx.tab_off[4]; T.z_m2m = dd + ctx.tab_off[5];
-  T.z_l2l = dd + ctx.tab_off[6];
+  T.fac = dd + H.off[0]; T.off_phase = dd + H.off[1]; T.oct_phase = dd + H.off[2];
+  T.rot = dd + H.off[3]; T.z_m2l = dd + H.off[4]; T.z_m2m = dd + H.off[5];
+  T.z_l2l = dd + H.off[6];
   T.rot_off = dl;
-  T.delta = dd + ctx.tab_off[7];
-  T.delta_off = dl + ctx.tab_off_l;
-  T.rot_e = dd + ctx.tab_off_re;
-  T.grp_ritems = di + ctx.tab_off_i[0]; T.grp_rcount = di + ctx.tab_off_i[1];
-  T.grp_zitems = di + ctx.tab_off_i[2]; T.grp_zoff = di + ctx.tab_off_i[3];
-  T.rot_half = dd + ctx.tab_off_g[0]; T.z_m2l_t = dd + ctx.tab_off_g[1];
-  T.rot_half_off = dl + ctx.tab_off_g[2]; T.zt_off = dl + ctx.tab_off_g[3];
-  T.zt_size = ctx.tab_zt_size;
-  const float* dfl = (const float*)ctx.arena.get("tab_f", ctx.tab_host_f.size() * sizeof(float));
-  T.rot_half_f = dfl;
-  T.z_m2l_t_f = dfl + ctx.tab_off_f;
+  T.delta = dd + H.off[7];
+  T.delta_off = dl + H.off_l;
+  T.rot_e = dd + H.off_re;
+  T.grp_ritems = di + H.off_i[0]; T.grp_rcount = di + H.off_i[1];
+  T.grp_zitems = di + H.off_i[2]; T.grp_zoff = di + H.off_i[3];
+  T.rot_half = dd + H.off_g[0]; T.z_m2l_t = dd + H.off_g[1];
+  T.rot_half_off = dl + H.off_g[2]; T.zt_off = dl + H.off_g[3];
+  T.zt_size = H.zt_size;
+  T.rot_half_f = df;
+  T.z_m2l_t_f = df + H.off_f;
   return T;
 }
 

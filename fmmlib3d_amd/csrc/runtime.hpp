@@ -6,6 +6,7 @@
 #include <map>
 #include <memory>
 #include <mutex>
+#include <set>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -55,18 +56,9 @@ struct Context {
   hipStream_t stream = nullptr;  // null = legacy default stream
   Arena arena;
   std::mutex mu;
-  // cached host-built translation tables (by order p)
-  int tab_p = -1;
-  std::vector<int> tab_host_i;
-  std::vector<double> tab_host_d;
-  std::vector<long> tab_host_l;
-  size_t tab_off[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-  size_t tab_off_l = 0, tab_off_re = 0;
-  size_t tab_off_i[4] = {0, 0, 0, 0};  // grouped M2L items: ritems, rcount, zitems, zoff
-  size_t tab_off_g[4] = {0, 0, 0, 0};  // grouped M2L tables: rot_half, z_m2l_t (doubles); their offsets (longs)
-  long tab_zt_size = 0;
-  std::vector<float> tab_host_f;  // FP32 copies of the grouped-M2L tables
-  size_t tab_off_f = 0;
+  // orders p whose Laplace translation tables are in this context's arena
+  // (the host tables are built once per process and order)
+  std::set<int> lap_tabs;
   // cached Helmholtz tables (owned by the Helmholtz driver)
   std::shared_ptr<void> helm_tables;
   // second stream (non-blocking, default priority) for the near field, and
