@@ -138,6 +138,22 @@ def test_gpu_helm_device_tree_equals_host_tree(rng, monkeypatch):
         assert np.array_equal(x, y)
 
 
+@pytest.mark.parametrize("mode", [F.PREC_FP64, F.PREC_MIXED])
+def test_gpu_helm_near_field_stream_does_not_change_results(rng, monkeypatch, mode):
+    """Near field on the second stream or on the caller's stream
+    (FMM3D_NEAR_STREAM=0): bitwise equal results."""
+    src = points(rng, 25000, "sphere")
+    trg = points(rng, 5000, "cube")
+    q, ds, dv = strengths(rng, 25000, True, True)
+    monkeypatch.setenv("FMM3D_NEAR_STREAM", "0")
+    a = F.hfmm3dparttarg(2, 8.0, src, q, ds, dv, trg, precision=mode)
+    monkeypatch.delenv("FMM3D_NEAR_STREAM")
+    b = F.hfmm3dparttarg(2, 8.0, src, q, ds, dv, trg, precision=mode)
+    assert a[0] == b[0] == 0
+    for x, y in zip(a[1:], b[1:]):
+        assert np.array_equal(x, y)
+
+
 def test_gpu_helm_deterministic(rng):
     src = points(rng, 20000, "sphere")
     q, _, _ = strengths(rng, 20000, True, False)

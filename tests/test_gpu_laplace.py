@@ -194,6 +194,42 @@ def test_gpu_device_tree_equals_host_tree(rng, monkeypatch, dist, ns, nt, iprec)
             assert np.array_equal(x, y)
 
 
+@pytest.mark.parametrize("mode", [F.PREC_FP64, F.PREC_MIXED, F.PREC_FP32])
+def test_gpu_near_field_stream_does_not_change_results(rng, monkeypatch, mode):
+    """The near field runs on a second stream beside the far field and the
+    scatter adds the two sums; with FMM3D_NEAR_STREAM=0 everything runs on
+    one stream.  The results are bitwise equal (same sums, same order), so
+    the events that order the two streams leave nothing to chance."""
+    src = points(rng, 30000, "clusters")
+    trg = points(rng, 20000, "cube")
+    q, ds, dv = strengths(rng, 30000, True, True)
+    monkeypatch.setenv("FMM3D_NEAR_STREAM", "0")
+    a = F.lfmm3dparttarg(2, src, q, ds, dv, trg, precision=mode)
+    monkeypatch.delenv("FMM3D_NEAR_STREAM")
+    for _ in range(2):
+        b = F.lfmm3dparttarg(2, src, q, ds, dv, trg, precision=mode)
+        assert a[0] == b[0] == 0
+        for x, y in zip(a[1:], b[1:]):
+            assert np.array_equal(x, y)
+
+
+def test_gpu_tables_of_several_orders(rng):
+    """Translation tables of every order used stay on the device: calls
+    alternating between iprec 1, 2 and 3 give the results of the first call
+    at each iprec, bitwise."""
+    src = points(rng, 12000, "cube")
+    q, ds, dv = strengths(rng, 12000, True, True)
+    first = {}
+    for iprec in (1, 2, 3, 2, 1, 3, 2):
+        r = F.lfmm3dparttarg(iprec, src, q, ds, dv)
+        assert r[0] == 0
+        if iprec not in first:
+            first[iprec] = r
+        else:
+            for x, y in zip(first[iprec][1:3], r[1:3]):
+                assert np.array_equal(x, y)
+
+
 def _hip():
     """The HIP runtime the library itself uses (no torch in this process)."""
     import ctypes
