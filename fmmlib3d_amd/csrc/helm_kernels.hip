@@ -262,279 +262,6 @@ __global__ void k_hform(DevPlan P, HelmDev Hd, const double* __restrict__ fac,
 }
 
 // ---------------------------------------------------------------------------
-// M2P (src/helmrouts3d.f:2211) and L2P (:2550).  One workgroup of NT threads
-// per box, one point per thread; the radial series of every thread live in
-// LDS (stride NT), the expansion premultiplied by sqrt(2n+1) (potential) and
-// by sqrt(2n+1) k/(2n+1) (gradient) is broadcast from LDS.
-// ---------------------------------------------------------------------------
-template <bool IRREG>
-__global__ __launch_bounds__(64) void k_heval(DevPlan P, HelmDev Hd, const double* __restrict__ fac,
-                                              const double* __restrict__ expn,
-                                              const double* __restrict__ spos,
-                                              const double* __restrict__ tpos, int wpot0, int wfld0,
-                                              int wpot1, int wfld1, double* __restrict__ pot0,
-                                              double* __restrict__ fld0, double* __restrict__ pot1,
-                                              double* __restrict__ fld1) {
-  constexpr int NT = 64;
-  extern __shared__ unsigned char smem_raw[];
-  double* Ew = reinterpret_cast<double*>(smem_raw);   // 2*Kg
-  double* fb = Ew + 2 * (size_t)P.K;                  // NT * (pmax+2) complex
-  const int pmaxg = P.p;
-  FacLayout L(pmaxg);
-  const int b = IRREG ? P.m2p_boxes[blockIdx.x] : P.l2p_boxes[blockIdx.x];
-  const bool grad = (wfld0 != 0) || (wfld1 != 0);
-  int jbeg = 0, jend = 1;
-  if (IRREG) { jbeg = P.l3_off[b]; jend = P.l3_off[b + 1]; }
-  for (int jj = jbeg; jj < jend; jj++) {
-    const int j = IRREG ? P.l3_idx[jj] : b;
-    const int lev = P.level[j];
-    if (lev < 2) continue;
-    if (IRREG && P.nsrc[j] == 0) continue;
-    const int p = Hd.plev[lev], K = (p + 1) * (p + 1);
-    const double sc = Hd.sclev[lev];
-    const double lo = IRREG ? sc : 1.0 / sc, hi = IRREG ? 1.0 / sc : sc;
-    const double cx = P.center[3 * j], cy = P.center[3 * j + 1], cz = P.center[3 * j + 2];
-    __syncthreads();
-    const Cd kk = {Hd.kre, Hd.kim};
-    for (int k = threadIdx.x; k < K; k += NT) {
-      int n = (int)sqrtf((float)k);
-      while (n * n > k) n--;
-      while ((n + 1) * (n + 1) <= k) n++;
-      Cd e = cscale(Cd{expn[2 * ((size_t)j * P.K + k)], expn[2 * ((size_t)j * P.K + k) + 1]}, (double)fac[L.M1 + n]);
-      Ew[2 * k] = e.re; Ew[2 * k + 1] = e.im;
-    }
-    __syncthreads();
-    const int pd = grad ? p + 1 : p;
-    for (int kind = 0; kind < 2; kind++) {
-      const int wp = kind ? wpot1 : wpot0, wf = kind ? wfld1 : wfld0;
-      if (!wp && !wf) continue;
-      const int n0 = kind ? P.itrg[b] : P.isrc[b];
-      const int nn = kind ? P.ntrg[b] : P.nsrc[b];
-      for (int i0 = 0; i0 < nn; i0 += NT) {
-        const int i = i0 + threadIdx.x;
-        if (i >= nn) continue;
-        const double* q = (kind ? tpos : spos) + 4 * (size_t)(n0 + i);
-        const double ux0 = q[0] - cx, uy0 = q[1] - cy, uz0 = q[2] - cz;
-        const double rr = sqrt(ux0 * ux0 + uy0 * uy0 + uz0 * uz0);
-        const Cd zz = {Hd.kre * rr, Hd.kim * rr};
-        if (IRREG && cnorm(zz) < 1.0e-30) continue;   // h3dmpeval_trunc0 returns early
-        const double ir = rr > 0.0 ? 1.0 / rr : 0.0;
-        const double ux = rr > 0.0 ? ux0 * ir : 0.0, uy = rr > 0.0 ? uy0 * ir : 0.0, uz = rr > 0.0 ? uz0 * ir : 1.0;
-        double* f = fb + 2 * (size_t)threadIdx.x;   // f_n at f[2*NT*n]
-        if (IRREG) dev_hankel(pd, zz, sc, reinterpret_cast<Cd*>(f), NT);
-        else dev_bessel(max(pd, 1), zz, sc, reinterpret_cast<Cd*>(f), NT);
-        Cd pot = {0.0, 0.0}, gz = {0.0, 0.0}, gu = {0.0, 0.0}, gd = {0.0, 0.0};
-        auto E = [&](const double* A, int n, int m) -> Cd { return cld(A, eidx(n, m)); };
-        // gradient coefficient of degree n: sqrt(2n+1) E_n^m k/(2n+1)
-        auto Ekv = [&](int n, int m) -> Cd { return cscale(cmul(kk, cld(Ew, eidx(n, m))), 1.0 / (2.0 * n + 1.0)); };
-        Cd ymm = {1.0, 0.0};
-        for (int m = 0; m <= pd; m++) {
-          if (m > 0) {
-            const double fm = fac[L.MM + m];
-            ymm = cmul(Cd{-ux * fm, -uy * fm}, ymm);
-          }
-          Cd y0 = ymm, y1 = {0.0, 0.0};
-          for (int n = m; n <= pd; n++) {
-            Cd y;
-            if (n == m) y = ymm;
-            else if (n == m + 1) { y = cscale(ymm, fac[L.M1 + m] * uz); y1 = y; }
-            else {
-              const double a = (2.0 * n - 1.0) * uz, bq = fac[L.RA + hidx(n, m)], c = fac[L.RB + hidx(n, m)];
-              y = {(a * y1.re - bq * y0.re) * c, (a * y1.im - bq * y0.im) * c};
-              y0 = y1;
-              y1 = y;
-            }
-            if (n == m + 1) y0 = ymm;
-            const Cd fn = {f[2 * (size_t)NT * n], f[2 * (size_t)NT * n + 1]};
-            const Cd B = cmul(fn, y), Bm = cmul(fn, cconj(y));
-            if (n <= p) {
-              cfma(pot, E(Ew, n, m), B);
-              if (m > 0) cfma(pot, E(Ew, n, -m), Bm);
-            }
-            if (!grad) continue;
-            for (int sgn = 0; sgn < (m > 0 ? 2 : 1); sgn++) {
-              const int mm = sgn ? -m : m;
-              const Cd Bv = sgn ? Bm : B;
-              // contributions from coefficient degree n+1 (lo terms) and n-1 (hi terms)
-              Cd cz_ = {0.0, 0.0}, cu_ = {0.0, 0.0}, cd_ = {0.0, 0.0};
-              if (n + 1 <= p) {
-                const int n1 = n + 1;
-                cz_ = cadd2(cz_, cscale(Ekv(n1, mm), fac[L.AR + eidx(n1, mm)] * lo));
-                if (mm - 1 >= -n1) cu_ = cadd2(cu_, cscale(Ekv(n1, mm - 1), fac[L.BR + eidx(n1, mm - 1)] * lo));
-                if (mm + 1 <= n1) cd_ = cadd2(cd_, cscale(Ekv(n1, mm + 1), fac[L.CR + eidx(n1, mm + 1)] * lo));
-              }
-              if (n >= 1 && n - 1 <= p) {
-                const int n1 = n - 1;
-                if (mm <= n1 && mm >= -n1) cz_ = cadd2(cz_, cscale(Ekv(n1, mm), fac[L.AS + eidx(n1, mm)] * hi));
-                if (mm - 1 <= n1 && mm - 1 >= -n1) cu_ = cadd2(cu_, cscale(Ekv(n1, mm - 1), fac[L.BS + eidx(n1, mm - 1)] * hi));
-                if (mm + 1 <= n1 && mm + 1 >= -n1) cd_ = cadd2(cd_, cscale(Ekv(n1, mm + 1), fac[L.CS + eidx(n1, mm + 1)] * hi));
-              }
-              cfma(gz, cz_, Bv);
-              cfma(gu, cu_, Bv);
-              cfma(gd, cd_, Bv);
-            }
-          }
-        }
-        double* po = kind ? pot1 : pot0;
-        double* fo = kind ? fld1 : fld0;
-        const size_t id = (size_t)(n0 + i);
-        if (wp) { po[2 * id] += pot.re; po[2 * id + 1] += pot.im; }
-        if (wf) {
-          const Cd dx = {0.5 * (gu.re + gd.re), 0.5 * (gu.im + gd.im)};
-          const Cd dy = {0.5 * (gu.im - gd.im), -0.5 * (gu.re - gd.re)};
-          fo[6 * id + 0] -= dx.re; fo[6 * id + 1] -= dx.im;
-          fo[6 * id + 2] -= dy.re; fo[6 * id + 3] -= dy.im;
-          fo[6 * id + 4] -= gz.re; fo[6 * id + 5] -= gz.im;
-        }
-      }
-    }
-  }
-}
-
-// Same evaluation with the gradient folded into three derived expansions of
-// degree p+1, built once per expansion in LDS:
-//   Gz(n,m) = lo AR(n+1,m) Ek(n+1,m) + hi AS(n-1,m) Ek(n-1,m)
-//   GU(n,m) = lo BR(n+1,m-1) Ek(n+1,m-1) + hi BS(n-1,m-1) Ek(n-1,m-1)
-//   GD(n,m) = lo CR(n+1,m+1) Ek(n+1,m+1) + hi CS(n-1,m+1) Ek(n-1,m+1)
-// (Ek = sqrt(2n+1) E k/(2n+1)), so a point costs four complex FMAs per basis
-// function.  LDS: 4 (pmax+2)^2 complex + the radial series of 64 points.
-template <bool IRREG>
-__global__ __launch_bounds__(64) void k_heval2(DevPlan P, HelmDev Hd, const double* __restrict__ fac,
-                                               const double* __restrict__ expn,
-                                               const double* __restrict__ spos,
-                                               const double* __restrict__ tpos, int wpot0, int wfld0,
-                                               int wpot1, int wfld1, double* __restrict__ pot0,
-                                               double* __restrict__ fld0, double* __restrict__ pot1,
-                                               double* __restrict__ fld1) {
-  constexpr int NT = 64;
-  extern __shared__ unsigned char smem_raw[];
-  const int pmaxg = P.p;
-  const int Kx = (pmaxg + 2) * (pmaxg + 2);
-  double* Ew = reinterpret_cast<double*>(smem_raw);  // 2*Kx each
-  double* Gz = Ew + 2 * (size_t)Kx;
-  double* Gu = Gz + 2 * (size_t)Kx;
-  double* Gd = Gu + 2 * (size_t)Kx;
-  double* fb = Gd + 2 * (size_t)Kx;                  // NT * (pmax+2) complex
-  FacLayout L(pmaxg);
-  const int b = IRREG ? P.m2p_boxes[blockIdx.x] : P.l2p_boxes[blockIdx.x];
-  const bool grad = (wfld0 != 0) || (wfld1 != 0);
-  const Cd kk = {Hd.kre, Hd.kim};
-  int jbeg = 0, jend = 1;
-  if (IRREG) { jbeg = P.l3_off[b]; jend = P.l3_off[b + 1]; }
-  for (int jj = jbeg; jj < jend; jj++) {
-    const int j = IRREG ? P.l3_idx[jj] : b;
-    const int lev = P.level[j];
-    if (lev < 2) continue;
-    if (IRREG && P.nsrc[j] == 0) continue;
-    const int p = Hd.plev[lev];
-    const int pd = grad ? p + 1 : p;
-    const int Kd = (pd + 1) * (pd + 1);
-    const double sc = Hd.sclev[lev];
-    const double lo = IRREG ? sc : 1.0 / sc, hi = IRREG ? 1.0 / sc : sc;
-    const double cx = P.center[3 * j], cy = P.center[3 * j + 1], cz = P.center[3 * j + 2];
-    const double* E = expn + 2 * (size_t)j * P.K;
-    __syncthreads();
-    for (int k = threadIdx.x; k < Kd; k += NT) {
-      int n = (int)sqrtf((float)k);
-      while (n * n > k) n--;
-      while ((n + 1) * (n + 1) <= k) n++;
-      const int m = k - n * n - n;
-      Cd e = {0.0, 0.0};
-      if (n <= p) e = cscale(Cd{E[2 * k], E[2 * k + 1]}, fac[L.M1 + n]);
-      Ew[2 * k] = e.re; Ew[2 * k + 1] = e.im;
-      if (!grad) continue;
-      auto ek = [&](int n1, int m1) -> Cd {   // Ek(n1, m1), zero when out of range
-        if (n1 < 0 || n1 > p || m1 > n1 || m1 < -n1) return Cd{0.0, 0.0};
-        const Cd v = cscale(Cd{E[2 * eidx(n1, m1)], E[2 * eidx(n1, m1) + 1]}, fac[L.M1 + n1] / (2.0 * n1 + 1.0));
-        return cmul(kk, v);
-      };
-      auto fa = [&](int off, int n1, int m1) -> double {
-        if (n1 < 0 || n1 > p || m1 > n1 || m1 < -n1) return 0.0;
-        return fac[off + eidx(n1, m1)];
-      };
-      Cd gz = cadd2(cscale(ek(n + 1, m), fa(L.AR, n + 1, m) * lo), cscale(ek(n - 1, m), fa(L.AS, n - 1, m) * hi));
-      Cd gu = cadd2(cscale(ek(n + 1, m - 1), fa(L.BR, n + 1, m - 1) * lo), cscale(ek(n - 1, m - 1), fa(L.BS, n - 1, m - 1) * hi));
-      Cd gd = cadd2(cscale(ek(n + 1, m + 1), fa(L.CR, n + 1, m + 1) * lo), cscale(ek(n - 1, m + 1), fa(L.CS, n - 1, m + 1) * hi));
-      Gz[2 * k] = gz.re; Gz[2 * k + 1] = gz.im;
-      Gu[2 * k] = gu.re; Gu[2 * k + 1] = gu.im;
-      Gd[2 * k] = gd.re; Gd[2 * k + 1] = gd.im;
-    }
-    __syncthreads();
-    for (int kind = 0; kind < 2; kind++) {
-      const int wp = kind ? wpot1 : wpot0, wf = kind ? wfld1 : wfld0;
-      if (!wp && !wf) continue;
-      const int n0 = kind ? P.itrg[b] : P.isrc[b];
-      const int nn = kind ? P.ntrg[b] : P.nsrc[b];
-      for (int i0 = 0; i0 < nn; i0 += NT) {
-        const int i = i0 + threadIdx.x;
-        if (i >= nn) continue;
-        const double* q = (kind ? tpos : spos) + 4 * (size_t)(n0 + i);
-        const double ux0 = q[0] - cx, uy0 = q[1] - cy, uz0 = q[2] - cz;
-        const double rr = sqrt(ux0 * ux0 + uy0 * uy0 + uz0 * uz0);
-        const Cd zz = {Hd.kre * rr, Hd.kim * rr};
-        if (IRREG && cnorm(zz) < 1.0e-30) continue;   // h3dmpeval_trunc0 returns early
-        const double ir = rr > 0.0 ? 1.0 / rr : 0.0;
-        const double ux = rr > 0.0 ? ux0 * ir : 0.0, uy = rr > 0.0 ? uy0 * ir : 0.0, uz = rr > 0.0 ? uz0 * ir : 1.0;
-        double* f = fb + 2 * (size_t)threadIdx.x;
-        if (IRREG) dev_hankel(pd, zz, sc, reinterpret_cast<Cd*>(f), NT);
-        else dev_bessel(max(pd, 1), zz, sc, reinterpret_cast<Cd*>(f), NT);
-        Cd pot = {0.0, 0.0}, gz = {0.0, 0.0}, gu = {0.0, 0.0}, gd = {0.0, 0.0};
-        Cd ymm = {1.0, 0.0};
-        for (int m = 0; m <= pd; m++) {
-          if (m > 0) {
-            const double fm = fac[L.MM + m];
-            ymm = cmul(Cd{-ux * fm, -uy * fm}, ymm);
-          }
-          Cd y0 = ymm, y1 = {0.0, 0.0};
-          for (int n = m; n <= pd; n++) {
-            Cd y;
-            if (n == m) y = ymm;
-            else if (n == m + 1) { y = cscale(ymm, fac[L.M1 + m] * uz); y1 = y; }
-            else {
-              const double a = (2.0 * n - 1.0) * uz, bq = fac[L.RA + hidx(n, m)], c = fac[L.RB + hidx(n, m)];
-              y = {(a * y1.re - bq * y0.re) * c, (a * y1.im - bq * y0.im) * c};
-              y0 = y1;
-              y1 = y;
-            }
-            if (n == m + 1) y0 = ymm;
-            const Cd fn = {f[2 * (size_t)NT * n], f[2 * (size_t)NT * n + 1]};
-            const Cd B = cmul(fn, y);
-            const int kp = eidx(n, m);
-            cfma(pot, cld(Ew, kp), B);
-            if (grad) {
-              cfma(gz, cld(Gz, kp), B);
-              cfma(gu, cld(Gu, kp), B);
-              cfma(gd, cld(Gd, kp), B);
-            }
-            if (m > 0) {
-              const Cd Bm = cmul(fn, cconj(y));
-              const int km = eidx(n, -m);
-              cfma(pot, cld(Ew, km), Bm);
-              if (grad) {
-                cfma(gz, cld(Gz, km), Bm);
-                cfma(gu, cld(Gu, km), Bm);
-                cfma(gd, cld(Gd, km), Bm);
-              }
-            }
-          }
-        }
-        double* po = kind ? pot1 : pot0;
-        double* fo = kind ? fld1 : fld0;
-        const size_t id = (size_t)(n0 + i);
-        if (wp) { po[2 * id] += pot.re; po[2 * id + 1] += pot.im; }
-        if (wf) {
-          const Cd dx = {0.5 * (gu.re + gd.re), 0.5 * (gu.im + gd.im)};
-          const Cd dy = {0.5 * (gu.im - gd.im), -0.5 * (gu.re - gd.re)};
-          fo[6 * id + 0] -= dx.re; fo[6 * id + 1] -= dx.im;
-          fo[6 * id + 2] -= dy.re; fo[6 * id + 3] -= dy.im;
-          fo[6 * id + 4] -= gz.re; fo[6 * id + 5] -= gz.im;
-        }
-      }
-    }
-  }
-}
-
-// ---------------------------------------------------------------------------
 // P2M / P2L with per-thread sums (the scheme of the Laplace k_form2).  With
 // B_n^mu = f_n(k r) Yhat_n^mu(u) (f = j_n/sc^n for P2M, h_n sc^n for P2L),
 // the 256 threads form G source groups of 256/G threads; each thread owns up
@@ -822,7 +549,9 @@ __global__ __launch_bounds__(256) void k_hform2(DevPlan P, HelmDev Hd, const dou
 // ---------------------------------------------------------------------------
 // M2P / L2P from packed expansions.  For every box whose expansion is
 // evaluated, the expansion Ew = sqrt(2n+1) E and the three gradient
-// expansions Gz, Gu, Gd of k_heval2 are stored per half-table entry (n, a >= 0)
+// expansions Gz, Gu, Gd (the z-derivative and U, D of the field written as
+// expansions of degree pd + 1 through the identities at the top of this
+// file) are stored per half-table entry (n, a >= 0)
 // in the order of the evaluation walk (column a, then n = a..pd): 8 complex
 //   Ew(n,a) Ew(n,-a) Gz(n,a) Gz(n,-a) Gu(n,a) Gu(n,-a) Gd(n,a) Gd(n,-a)
 // (the -a slots zero for a = 0; Gz, Gu, Gd zero without the gradient).  The
