@@ -338,7 +338,7 @@ int helm_solve(Context& ctx, const LapCall& c) {
       }
       if (tree.list[2].count(b) > 0 && has_pts) {
         // list 3 in slices of about kSlice half-table entries: one workgroup per
-        // (64 points, slice), the slices summed in order by k_hreduce
+        // (64 points, slice), the slices summed in order by k_slice_reduce
         constexpr int kSlice = 1024;
         std::vector<int> cuts = {tree.list[2].off[b]};
         int cost = 0;
@@ -401,8 +401,9 @@ int helm_solve(Context& ctx, const LapCall& c) {
       hipLaunchKernelGGL((k_heval3<true>), dim3((unsigned)nsub), dim3(64), 0, s, P, Hd, fac, pack, dh + o_sm,
                          dh + o_im, grad, spos, tpos, f.ifpot, f.iffld, f.ifpottarg, f.iffldtarg, pot_s, fld_s, pot_t,
                          fld_t, partial);
-      hipLaunchKernelGGL(k_hreduce, dim3((unsigned)(red_m.size() / 5)), dim3(64), 0, s, P, dh + o_rm, partial,
-                         f.ifpot, f.iffld, f.ifpottarg, f.iffldtarg, pot_s, fld_s, pot_t, fld_t);
+      hipLaunchKernelGGL(k_slice_reduce, dim3((unsigned)(red_m.size() / 5)), dim3(64), 0, s, dh + o_rm, partial,
+                         P.isrc, P.nsrc, P.itrg, P.ntrg, f.ifpot, f.iffld, f.ifpottarg, f.iffldtarg, pot_s, fld_s,
+                         pot_t, fld_t);
     }
   }
   tm.mark(9);

@@ -381,7 +381,6 @@ int lap_solve(Context& ctx, const LapCall& c) {
     hipLaunchKernelGGL((k_l2l<T>), dim3(g.second), dim3(256), lds_ps, s, P, Tb, g.first, lo);
   tm.mark(7);
   // ---- STEP 7: L2P (packed local + gradient expansions, wave-uniform reads)
-  const size_t lds_e = sizeof(T) * 2 * K;
   if (P.n_l2p) {
     const size_t H = (size_t)(p + 1) * (p + 2) / 2;
     T* pack = (T*)ctx.arena.get("l2p_pack", sizeof(T) * 16 * H * (size_t)P.n_l2p);
@@ -393,10 +392,70 @@ int lap_solve(Context& ctx, const LapCall& c) {
                          f.ifpottarg, f.iffldtarg, pot_s, fld_s, pot_t, fld_t);
   }
   tm.mark(8);
-  // ---- STEP 6: M2P (list 3)
-  if (P.n_m2p)
-    hipLaunchKernelGGL((k_eval<T, true>), dim3(P.n_m2p), dim3(256), lds_e, s, P, fac, mp, spos, tpos,
-                       f.ifpot, f.iffld, f.ifpottarg, f.iffldtarg, pot_s, fld_s, pot_t, fld_t);
+  // ---- STEP 6: M2P (list 3) from packed multipoles; list 3 of each box is
+  // cut into slices of about kSlice packed entries, one wave per (64 points,
+  // slice), and the slices' sums are added in order (k_slice_reduce)
+  if (P.n_m2p && (f.ifcharge || f.ifdipole)) {
+    const int grad = (f.iffld || f.iffldtarg) ? 1 : 0;
+    const bool want_src = f.ifpot || f.iffld, want_trg = f.ifpottarg || f.iffldtarg;
+    const int pd = grad ? p + 1 : p;
+    const int Hd = (pd + 1) * (pd + 2) / 2;
+    constexpr int kSlice = 1024;
+    const int nbx = tree.nboxes;
+    thread_local std::vector<int> slot, pk, off, it, red, hv;
+    slot.assign(nbx, -1);
+    pk.clear(); off.clear(); it.clear(); red.clear(); hv.clear();
+    int tot = 0;
+    std::vector<int> cuts;
+    for (int b = 0; b < nbx; b++) {
+      const int l0 = tree.list[2].off[b], l1 = tree.list[2].off[b + 1];
+      if (l0 == l1) continue;
+      if (!((want_src && tree.nsrc[b] > 0) || (want_trg && tree.ntrg[b] > 0))) continue;
+      cuts.assign(1, l0);
+      int cost = 0;
+      for (int k = l0; k < l1; k++) {
+        const int j = tree.list[2].idx[k];
+        if (tree.nsrc[j] == 0) continue;
+        cost += Hd;
+        if (cost >= kSlice && k + 1 < l1) { cuts.push_back(k + 1); cost = 0; }
+        if (slot[j] >= 0) continue;
+        slot[j] = tot;
+        pk.push_back(j);
+        off.push_back(tot);
+        tot += Hd;
+      }
+      cuts.push_back(l1);
+      const int nsl = (int)cuts.size() - 1;
+      auto slices = [&](int kind, int npts) {
+        for (int q = 0; q < npts; q += 64) {
+          red.push_back(b); red.push_back(kind); red.push_back(q);
+          red.push_back((int)(it.size() / 5)); red.push_back(nsl);
+          for (int c = 0; c < nsl; c++) {
+            it.push_back(b); it.push_back(kind); it.push_back(q);
+            it.push_back(cuts[c]); it.push_back(cuts[c + 1]);
+          }
+        }
+      };
+      if (want_src) slices(0, tree.nsrc[b]);
+      if (want_trg) slices(1, tree.ntrg[b]);
+    }
+    if (!pk.empty() && !it.empty()) {
+      const size_t o_pk = nbx, o_off = o_pk + pk.size(), o_it = o_off + off.size(), o_red = o_it + it.size();
+      for (auto* v : {&slot, &pk, &off, &it, &red}) hv.insert(hv.end(), v->begin(), v->end());
+      const int* dh = upload_ints("m2p_i", hv, s);
+      T* pack = (T*)ctx.arena.get("m2p_pack", sizeof(T) * 16 * (size_t)tot);
+      const size_t nit = it.size() / 5;
+      double* partial = (double*)ctx.arena.get("m2p_part", sizeof(double) * 8 * 64 * nit);
+      const size_t tp = (size_t)pk.size() * Hd;
+      hipLaunchKernelGGL((k_m2p_pack<T>), dim3((unsigned)std::min<size_t>(blocks(tp, 256), 65536)), dim3(256), 0, s,
+                         P, fac, mp, dh + o_pk, dh + o_off, (int)pk.size(), grad, pack);
+      hipLaunchKernelGGL((k_m2p_eval<T>), dim3(item_groups((int)nit)), dim3(kP2PBlock * kItemWaves), 0, s, P, fac,
+                         pack, dh, dh + o_it, (int)nit, grad, spos, tpos, partial);
+      hipLaunchKernelGGL(k_slice_reduce, dim3((unsigned)(red.size() / 5)), dim3(64), 0, s, dh + o_red, partial,
+                         P.isrc, P.nsrc, P.itrg, P.ntrg, f.ifpot, f.iffld, f.ifpottarg, f.iffldtarg, pot_s, fld_s,
+                         pot_t, fld_t);
+    }
+  }
   tm.mark(9);
   // ---- far + near, scattered back to the caller's order (l3dpsort / l3dfsort)
   FMM_HIP(hipStreamWaitEvent(s, ctx.aux_ev[1], 0));

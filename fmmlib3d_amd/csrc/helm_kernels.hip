@@ -625,7 +625,7 @@ __device__ __forceinline__ void hpack_fma(Cd& out, const double* c, const Cd& B,
 // kind, first point), the box's own local expansion, added to the outputs.
 // M2P (IRREG = true): item (box, kind, first point, first and last+1 index
 // into the box's list 3), a slice of list 3 whose sums go to `partial`
-// (8 values per point), added to the outputs in slice order by k_hreduce.
+// (8 values per point), added to the outputs in slice order by k_slice_reduce.
 // All sums stay in registers until the single store at the end.
 template <bool IRREG>
 __global__ __launch_bounds__(64) void k_heval3(DevPlan P, HelmDev Hd, const double* __restrict__ fac,
@@ -758,39 +758,6 @@ __global__ __launch_bounds__(64) void k_heval3(DevPlan P, HelmDev Hd, const doub
   const size_t id = (size_t)(n0 + i);
   if (wp) { po[2 * id] += pot.re; po[2 * id + 1] += pot.im; }
   if (wf) {
-    const Cd dx = {0.5 * (gu.re + gd.re), 0.5 * (gu.im + gd.im)};
-    const Cd dy = {0.5 * (gu.im - gd.im), -0.5 * (gu.re - gd.re)};
-    fo[6 * id + 0] -= dx.re; fo[6 * id + 1] -= dx.im;
-    fo[6 * id + 2] -= dy.re; fo[6 * id + 3] -= dy.im;
-    fo[6 * id + 4] -= gz.re; fo[6 * id + 5] -= gz.im;
-  }
-}
-
-// M2P sums of the list-3 slices of one item (box, kind, first point, first
-// slice, number of slices), added in slice order to the outputs.
-__global__ __launch_bounds__(64) void k_hreduce(DevPlan P, const int* __restrict__ red, const double* __restrict__ partial,
-                                                int wpot0, int wfld0, int wpot1, int wfld1, double* __restrict__ pot0,
-                                                double* __restrict__ fld0, double* __restrict__ pot1,
-                                                double* __restrict__ fld1) {
-  const int* r = red + 5 * blockIdx.x;
-  const int b = r[0], kind = r[1], start = r[2], s0 = r[3], ns = r[4];
-  const int wp = kind ? wpot1 : wpot0, wf = kind ? wfld1 : wfld0;
-  const int n0 = kind ? P.itrg[b] : P.isrc[b];
-  const int nn = kind ? P.ntrg[b] : P.nsrc[b];
-  const int i = start + threadIdx.x;
-  if (i >= nn) return;
-  double v[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-  for (int c = 0; c < ns; c++) {
-    const double* o = partial + 8 * ((size_t)(s0 + c) * 64 + threadIdx.x);
-#pragma unroll
-    for (int e = 0; e < 8; e++) v[e] += o[e];
-  }
-  double* po = kind ? pot1 : pot0;
-  double* fo = kind ? fld1 : fld0;
-  const size_t id = (size_t)(n0 + i);
-  if (wp) { po[2 * id] += v[0]; po[2 * id + 1] += v[1]; }
-  if (wf) {
-    const Cd gz = {v[2], v[3]}, gu = {v[4], v[5]}, gd = {v[6], v[7]};
     const Cd dx = {0.5 * (gu.re + gd.re), 0.5 * (gu.im + gd.im)};
     const Cd dy = {0.5 * (gu.im - gd.im), -0.5 * (gu.re - gd.re)};
     fo[6 * id + 0] -= dx.re; fo[6 * id + 1] -= dx.im;

@@ -171,6 +171,40 @@ static __global__ void k_scatter_sum(int n, int w, const int* __restrict__ perm,
     out[(size_t)2 * w * o + k] = far[(size_t)2 * w * i + k] + near[(size_t)2 * w * i + k];
 }
 
+// Sums of the list-3 slices of one evaluation item (box, kind, first point,
+// first slice, number of slices), 8 values per point (pot, d/dz, U, D as
+// complex), added in slice order to the outputs: pot += pot, fld -= grad
+// with d/dx = (U+D)/2, d/dy = (U-D)/(2i).  One 64-thread workgroup per item.
+static __global__ void k_slice_reduce(const int* __restrict__ red, const double* __restrict__ partial,
+                                      const int* __restrict__ isrc, const int* __restrict__ nsrc,
+                                      const int* __restrict__ itrg, const int* __restrict__ ntrg, int wpot0,
+                                      int wfld0, int wpot1, int wfld1, double* __restrict__ pot0,
+                                      double* __restrict__ fld0, double* __restrict__ pot1,
+                                      double* __restrict__ fld1) {
+  const int* r = red + 5 * blockIdx.x;
+  const int b = r[0], kind = r[1], start = r[2], s0 = r[3], ns = r[4];
+  const int wp = kind ? wpot1 : wpot0, wf = kind ? wfld1 : wfld0;
+  const int n0 = kind ? itrg[b] : isrc[b];
+  const int nn = kind ? ntrg[b] : nsrc[b];
+  const int i = start + threadIdx.x;
+  if (i >= nn) return;
+  double v[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  for (int c = 0; c < ns; c++) {
+    const double* o = partial + 8 * ((size_t)(s0 + c) * 64 + threadIdx.x);
+    for (int e = 0; e < 8; e++) v[e] += o[e];
+  }
+  double* po = kind ? pot1 : pot0;
+  double* fo = kind ? fld1 : fld0;
+  const size_t id = (size_t)(n0 + i);
+  if (wp) { po[2 * id] += v[0]; po[2 * id + 1] += v[1]; }
+  if (wf) {
+    // v[2..7] = d/dz, U, D (complex)
+    fo[6 * id + 0] -= 0.5 * (v[4] + v[6]); fo[6 * id + 1] -= 0.5 * (v[5] + v[7]);
+    fo[6 * id + 2] -= 0.5 * (v[5] - v[7]); fo[6 * id + 3] -= -0.5 * (v[4] - v[6]);
+    fo[6 * id + 4] -= v[2]; fo[6 * id + 5] -= v[3];
+  }
+}
+
 template <typename T>
 __global__ void k_fill_zero(size_t n, T* __restrict__ a) {
   size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;

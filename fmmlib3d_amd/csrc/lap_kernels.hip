@@ -317,155 +317,8 @@ __global__ __launch_bounds__(256) void k_form2(DevPlan P, const double* __restri
 }
 
 // ---------------------------------------------------------------------------
-// L2P (src/laprouts3d.f:2772) and M2P (src/laprouts3d.f:2432)
-// One workgroup per leaf box; one thread per evaluation point.  All threads
-// walk the coefficients in the same order (LDS broadcast reads) and build
-// their own harmonics column by column.
-// ---------------------------------------------------------------------------
-template <typename T, bool IRREG>
-__device__ __forceinline__ void eval_expansion(const double* fac, const FacLayout& L, int p,
-                                               const T* E, T ux, T uy, T uz, C2<T>& pot,
-                                               C2<T>& gz, C2<T>& gu, C2<T>& gd, bool grad) {
-  T r2 = ux * ux + uy * uy + uz * uz;
-  T ir2 = IRREG ? T(1) / r2 : T(0);
-  T ir = IRREG ? sqrt(ir2) : T(0);
-  const int pmax = IRREG ? p + 1 : p;  // S needs one more degree for the gradient
-  // Walk columns m >= 0; in each column the harmonic H_n^m for n = m..pmax.
-  C2<T> hmm = {IRREG ? ir : T(1), T(0)};
-  for (int m = 0; m <= pmax; m++) {
-    if (m > 0) {
-      T f = (T)fac[L.MM + m] * (IRREG ? ir2 : T(1));
-      C2<T> w = {-ux * f, -uy * f};
-      hmm = cmul(w, hmm);
-    }
-    C2<T> h0 = hmm, h1 = {T(0), T(0)};
-    for (int n = m; n <= pmax; n++) {
-      C2<T> h;
-      if (n == m) h = hmm;
-      else if (n == m + 1) {
-        T g = (T)fac[L.M1 + m] * uz * (IRREG ? ir2 : T(1));
-        h = {g * hmm.re, g * hmm.im};
-        h1 = h;
-      } else {
-        T a = (T)(2 * n - 1) * uz;
-        T bq = (T)fac[L.RA + hidx(n, m)] * (IRREG ? T(1) : r2);
-        T c = (T)fac[L.RB + hidx(n, m)] * (IRREG ? ir2 : T(1));
-        h = {(a * h1.re - bq * h0.re) * c, (a * h1.im - bq * h0.im) * c};
-        h0 = h1;
-        h1 = h;
-      }
-      if (n == m + 1) h0 = hmm;
-      C2<T> hc = cconj(h);
-      // potential
-      if (n <= p) {
-        cfma(pot, cld(E, eidx(n, m)), h);
-        if (m > 0) cfma(pot, cld(E, eidx(n, -m)), hc);
-      }
-      if (!grad) continue;
-      if (!IRREG) {
-        // R_n^m appears in the derivatives of R_{n+1}^{m}, R_{n+1}^{m-1}, R_{n+1}^{m+1}
-        if (n + 1 <= p) {
-          int n1 = n + 1;
-          cfma(gz, C2<T>{(T)fac[L.AR + eidx(n1, m)] * E[2 * eidx(n1, m)], (T)fac[L.AR + eidx(n1, m)] * E[2 * eidx(n1, m) + 1]}, h);
-          cfma(gu, C2<T>{(T)fac[L.BR + eidx(n1, m - 1)] * E[2 * eidx(n1, m - 1)], (T)fac[L.BR + eidx(n1, m - 1)] * E[2 * eidx(n1, m - 1) + 1]}, h);
-          cfma(gd, C2<T>{(T)fac[L.CR + eidx(n1, m + 1)] * E[2 * eidx(n1, m + 1)], (T)fac[L.CR + eidx(n1, m + 1)] * E[2 * eidx(n1, m + 1) + 1]}, h);
-          if (m > 0) {
-            cfma(gz, C2<T>{(T)fac[L.AR + eidx(n1, -m)] * E[2 * eidx(n1, -m)], (T)fac[L.AR + eidx(n1, -m)] * E[2 * eidx(n1, -m) + 1]}, hc);
-            cfma(gu, C2<T>{(T)fac[L.BR + eidx(n1, -m - 1)] * E[2 * eidx(n1, -m - 1)], (T)fac[L.BR + eidx(n1, -m - 1)] * E[2 * eidx(n1, -m - 1) + 1]}, hc);
-            cfma(gd, C2<T>{(T)fac[L.CR + eidx(n1, 1 - m)] * E[2 * eidx(n1, 1 - m)], (T)fac[L.CR + eidx(n1, 1 - m)] * E[2 * eidx(n1, 1 - m) + 1]}, hc);
-          }
-        }
-      } else {
-        // S_n^m appears in the derivatives of S_{n-1}^{m}, S_{n-1}^{m+1}, S_{n-1}^{m-1}
-        if (n >= 1) {
-          int n1 = n - 1;
-          if (m <= n1) {
-            cfma(gz, C2<T>{(T)fac[L.AS + eidx(n1, m)] * E[2 * eidx(n1, m)], (T)fac[L.AS + eidx(n1, m)] * E[2 * eidx(n1, m) + 1]}, h);
-          }
-          if (m - 1 <= n1 && m - 1 >= -n1) {
-            cfma(gu, C2<T>{(T)fac[L.BS + eidx(n1, m - 1)] * E[2 * eidx(n1, m - 1)], (T)fac[L.BS + eidx(n1, m - 1)] * E[2 * eidx(n1, m - 1) + 1]}, h);
-          }
-          if (m + 1 <= n1) {
-            cfma(gd, C2<T>{(T)fac[L.CS + eidx(n1, m + 1)] * E[2 * eidx(n1, m + 1)], (T)fac[L.CS + eidx(n1, m + 1)] * E[2 * eidx(n1, m + 1) + 1]}, h);
-          }
-          if (m > 0) {
-            if (m <= n1) {
-              cfma(gz, C2<T>{(T)fac[L.AS + eidx(n1, -m)] * E[2 * eidx(n1, -m)], (T)fac[L.AS + eidx(n1, -m)] * E[2 * eidx(n1, -m) + 1]}, hc);
-            }
-            if (-m - 1 >= -n1) {
-              cfma(gu, C2<T>{(T)fac[L.BS + eidx(n1, -m - 1)] * E[2 * eidx(n1, -m - 1)], (T)fac[L.BS + eidx(n1, -m - 1)] * E[2 * eidx(n1, -m - 1) + 1]}, hc);
-            }
-            if (-m + 1 <= n1 && -m + 1 >= -n1) {
-              cfma(gd, C2<T>{(T)fac[L.CS + eidx(n1, -m + 1)] * E[2 * eidx(n1, -m + 1)], (T)fac[L.CS + eidx(n1, -m + 1)] * E[2 * eidx(n1, -m + 1) + 1]}, hc);
-            }
-          }
-        }
-      }
-    }
-  }
-}
-
-// kind 0: sources of the box as evaluation points, kind 1: targets
-template <typename T, bool IRREG>
-__global__ void k_eval(DevPlan P, const double* __restrict__ fac, const T* __restrict__ expn,
-                       const double* __restrict__ spos, const double* __restrict__ tpos, int wpot0,
-                       int wfld0, int wpot1, int wfld1, double* __restrict__ pot0,
-                       double* __restrict__ fld0, double* __restrict__ pot1,
-                       double* __restrict__ fld1) {
-  extern __shared__ unsigned char smem_raw[];
-  T* E = reinterpret_cast<T*>(smem_raw);
-  const int p = P.p, K = P.K;
-  FacLayout L(p);
-  const int b = IRREG ? P.m2p_boxes[blockIdx.x] : P.l2p_boxes[blockIdx.x];
-  int jbeg = 0, jend = 1;
-  if (IRREG) { jbeg = P.l3_off[b]; jend = P.l3_off[b + 1]; }
-  for (int jj = jbeg; jj < jend; jj++) {
-    const int j = IRREG ? P.l3_idx[jj] : b;
-    const T ih = (T)P.inv_side[P.level[j]];
-    const double cx = P.center[3 * j], cy = P.center[3 * j + 1], cz = P.center[3 * j + 2];
-    __syncthreads();
-    for (int k = threadIdx.x; k < 2 * K; k += blockDim.x) E[k] = expn[2 * (size_t)j * K + k];
-    __syncthreads();
-    for (int kind = 0; kind < 2; kind++) {
-      const int wp = kind ? wpot1 : wpot0, wf = kind ? wfld1 : wfld0;
-      if (!wp && !wf) continue;
-      const int n0 = kind ? P.itrg[b] : P.isrc[b];
-      const int nn = kind ? P.ntrg[b] : P.nsrc[b];
-      for (int i0 = 0; i0 < nn; i0 += blockDim.x) {
-        const int i = i0 + threadIdx.x;
-        const bool active = i < nn;
-        double x = 0, y = 0, z = 0;
-        if (active) {
-          const double* q = (kind ? tpos : spos) + 4 * (size_t)(n0 + i);
-          x = q[0]; y = q[1]; z = q[2];
-        }
-        T ux = active ? (T)((x - cx) * (double)ih) : T(1);
-        T uy = active ? (T)((y - cy) * (double)ih) : T(1);
-        T uz = active ? (T)((z - cz) * (double)ih) : T(1);
-        C2<T> pv = {0, 0}, gz = {0, 0}, gu = {0, 0}, gd = {0, 0};
-        eval_expansion<T, IRREG>(fac, L, p, E, ux, uy, uz, pv, gz, gu, gd, wf != 0);
-        if (!active) continue;
-        // M2P: pot = sum M S(u)/h ; L2P: pot = sum L R(u); gradient w.r.t. x adds 1/h
-        T sp = IRREG ? ih : T(1), sg = IRREG ? ih * ih : ih;
-        double* po = kind ? pot1 : pot0;
-        double* fo = kind ? fld1 : fld0;
-        size_t id = (size_t)(n0 + i);
-        if (wp) { po[2 * id] += pv.re * sp; po[2 * id + 1] += pv.im * sp; }
-        if (wf) {
-          // fld = -grad: d/dx = (U+D)/2, d/dy = (U-D)/(2i), d/dz
-          C2<T> dx = {T(0.5) * (gu.re + gd.re), T(0.5) * (gu.im + gd.im)};
-          C2<T> dy = {T(0.5) * (gu.im - gd.im), -T(0.5) * (gu.re - gd.re)};
-          fo[6 * id + 0] -= dx.re * sg; fo[6 * id + 1] -= dx.im * sg;
-          fo[6 * id + 2] -= dy.re * sg; fo[6 * id + 3] -= dy.im * sg;
-          fo[6 * id + 4] -= gz.re * sg; fo[6 * id + 5] -= gz.im * sg;
-        }
-      }
-    }
-  }
-}
-
-// ---------------------------------------------------------------------------
-// L2P from a packed expansion.  For every box of the L2P list the local
+// L2P (src/laprouts3d.f:2772 l3dtaevalall_trunc) from a packed expansion.
+// For every box of the L2P list the local
 // expansion L and the three derived expansions of the gradient,
 //   Gz(n,m) = AR(n+1,m)   L(n+1,m),
 //   Gu(n,m) = BR(n+1,m-1) L(n+1,m-1),
@@ -600,6 +453,146 @@ __global__ __launch_bounds__(256) void k_l2p2(DevPlan P, const double* __restric
     fo[6 * id + 2] -= dy.re * ih; fo[6 * id + 3] -= dy.im * ih;
     fo[6 * id + 4] -= gz.re * ih; fo[6 * id + 5] -= gz.im * ih;
   }
+}
+
+// ---------------------------------------------------------------------------
+// M2P (list 3) from packed multipoles.  For every box whose multipole is
+// evaluated, the multipole M and the three derived expansions of the
+// gradient over the irregular harmonics of degree n <= p+1,
+//   Gz(n,m) = AS(n-1,m)   M(n-1,m),
+//   Gu(n,m) = BS(n-1,m-1) M(n-1,m-1),
+//   Gd(n,m) = CS(n-1,m+1) M(n-1,m+1)      (zero where (n-1, m') is outside),
+// are stored per half-table entry (n, a >= 0) in the order of the walk
+// (column a, then n = a..pd, pd = p+1 with the gradient, else p), 8 complex
+// values as for L2P.  One wave evaluates 64 points of a box against a slice
+// of its list 3 (src/laprouts3d.f:2432 l3dmpevalall_trunc): per source box
+// the irregular harmonics S_n^a(u) by the column recurrence of
+// irregular_column, four complex products per harmonic, scaled
+// by 1/h (potential) and 1/h^2 (gradient) of the source box; the slice's sums
+// go to `partial` and k_slice_reduce adds the slices in order.
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ void k_m2p_pack(DevPlan P, const double* __restrict__ fac, const T* __restrict__ mp,
+                           const int* __restrict__ boxes, const int* __restrict__ offs, int nbox, int grad,
+                           T* __restrict__ pack) {
+  const int p = P.p, K = P.K;
+  const int pd = grad ? p + 1 : p;
+  const int Hd = (pd + 1) * (pd + 2) / 2;
+  FacLayout L(p);
+  const size_t total = (size_t)nbox * Hd;
+  for (size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x; t < total; t += (size_t)gridDim.x * blockDim.x) {
+    const int i = (int)(t / Hd), w = (int)(t - (size_t)i * Hd);
+    int a = 0;
+    while (a < pd && walk_index(pd, a + 1, a + 1) <= w) a++;
+    const int n = a + (w - walk_index(pd, a, a));
+    const T* E = mp + 2 * (size_t)boxes[i] * K;
+    auto ef = [&](int off, int n1, int m1, int e) -> T {  // factor * E(n1, m1), zero outside
+      if (n1 < 0 || m1 > n1 || m1 < -n1) return T(0);
+      return (T)fac[off + eidx(n1, m1)] * E[2 * eidx(n1, m1) + e];
+    };
+    T v[16];
+#pragma unroll
+    for (int e = 0; e < 16; e++) v[e] = T(0);
+    if (n <= p) {
+      v[0] = E[2 * eidx(n, a)];
+      v[1] = E[2 * eidx(n, a) + 1];
+      if (a > 0) { v[2] = E[2 * eidx(n, -a)]; v[3] = E[2 * eidx(n, -a) + 1]; }
+    }
+    if (grad && n >= 1) {
+      const int n1 = n - 1;
+      for (int sgn = 0; sgn < (a > 0 ? 2 : 1); sgn++) {
+        const int m = sgn ? -a : a;
+        for (int e = 0; e < 2; e++) {
+          v[4 + 2 * sgn + e] = ef(L.AS, n1, m, e);
+          v[8 + 2 * sgn + e] = ef(L.BS, n1, m - 1, e);
+          v[12 + 2 * sgn + e] = ef(L.CS, n1, m + 1, e);
+        }
+      }
+    }
+    T* o = pack + 16 * ((size_t)offs[i] + w);
+#pragma unroll
+    for (int e = 0; e < 16; e++) o[e] = v[e];
+  }
+}
+
+// items: 5 ints per wave (box, kind, first point, first and end position in
+// list 3); slot[j]: the pack offset (in entries) of source box j
+template <typename T>
+__global__ __launch_bounds__(256) void k_m2p_eval(DevPlan P, const double* __restrict__ fac, const T* __restrict__ pack,
+                                                  const int* __restrict__ slot, const int* __restrict__ items, int nitems,
+                                                  int grad, const double* __restrict__ spos,
+                                                  const double* __restrict__ tpos, double* __restrict__ partial) {
+  const int p = P.p;
+  const int pd = grad ? p + 1 : p;
+  FacLayout L(p);
+  const int w = __builtin_amdgcn_readfirstlane((int)(blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)));
+  if (w >= nitems) return;
+  const int lane = (int)(threadIdx.x & 63);
+  const int* it = items + 5 * w;
+  const int b = it[0], kind = it[1], start = it[2], jbeg = it[3], jend = it[4];
+  const int n0 = kind ? P.itrg[b] : P.isrc[b];
+  const int nn = kind ? P.ntrg[b] : P.nsrc[b];
+  const int i = start + lane;
+  const bool active = i < nn;
+  double x = 0.0, y = 0.0, z = 0.0;
+  if (active) {
+    const double* q = (kind ? tpos : spos) + 4 * (size_t)(n0 + i);
+    x = q[0]; y = q[1]; z = q[2];
+  }
+  double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  for (int jj = jbeg; jj < jend; jj++) {
+    const int j = P.l3_idx[jj];
+    if (P.nsrc[j] == 0) continue;
+    const double ihd = P.inv_side[P.level[j]];
+    const T ih = (T)ihd;
+    // inactive lanes evaluate at a harmless point and are not written
+    const T ux = active ? (T)((x - P.center[3 * j]) * ihd) : T(1);
+    const T uy = active ? (T)((y - P.center[3 * j + 1]) * ihd) : T(1);
+    const T uz = active ? (T)((z - P.center[3 * j + 2]) * ihd) : T(1);
+    const T r2 = ux * ux + uy * uy + uz * uz;
+    const T ir2 = T(1) / r2, ir = sqrt(ir2);
+    C2<T> pv = {0, 0}, gz = {0, 0}, gu = {0, 0}, gd = {0, 0};
+    C2<T> hmm = {ir, T(0)};
+    const T* c = pack + 16 * (size_t)slot[j];
+    for (int a = 0; a <= pd; a++) {
+      if (a > 0) {
+        const T f = (T)fac[L.MM + a] * ir2;
+        const C2<T> v = {-ux * f, -uy * f};
+        hmm = cmul(v, hmm);
+      }
+      C2<T> h0 = hmm, h1 = hmm;
+      for (int n = a; n <= pd; n++, c += 16) {
+        C2<T> h;
+        if (n == a) {
+          h = hmm;
+        } else if (n == a + 1) {
+          const T g = (T)fac[L.M1 + a] * uz * ir2;
+          h = {g * hmm.re, g * hmm.im};
+          h1 = h;
+        } else {
+          const T aa = (T)(2 * n - 1) * uz, bq = (T)fac[L.RA + hidx(n, a)], cc = (T)fac[L.RB + hidx(n, a)] * ir2;
+          h = {(aa * h1.re - bq * h0.re) * cc, (aa * h1.im - bq * h0.im) * cc};
+          h0 = h1;
+          h1 = h;
+        }
+        pack_fma(pv, c, h);
+        if (grad) {
+          pack_fma(gz, c + 4, h);
+          pack_fma(gu, c + 8, h);
+          pack_fma(gd, c + 12, h);
+        }
+      }
+    }
+    // pot = sum M S(u)/h ; the gradient w.r.t. x adds another 1/h
+    const T sg = ih * ih;
+    acc[0] += (double)(pv.re * ih); acc[1] += (double)(pv.im * ih);
+    acc[2] += (double)(gz.re * sg); acc[3] += (double)(gz.im * sg);
+    acc[4] += (double)(gu.re * sg); acc[5] += (double)(gu.im * sg);
+    acc[6] += (double)(gd.re * sg); acc[7] += (double)(gd.im * sg);
+  }
+  double* o = partial + 8 * ((size_t)w * 64 + lane);
+#pragma unroll
+  for (int e = 0; e < 8; e++) o[e] = acc[e];
 }
 
 // ---------------------------------------------------------------------------
