@@ -258,24 +258,26 @@ int lap_solve(Context& ctx, const LapCall& c) {
   tm.mark(1);
   const double* fac = Tb.fac;
   // ---- STEP 1: P2M, STEP 2: P2L (list 4)
-  // k_form2 (per-thread sums) when its half table fits two entries per
-  // thread, else k_form (per-coefficient contraction)
+  // k_form2 (per-thread sums) when its half table fits two (or four) entries
+  // per thread, else k_form (per-coefficient contraction)
   auto form = [&](auto irreg_tag, int nbox_form, T* dst) {
     constexpr bool IR = decltype(irreg_tag)::value;
     if (nbox_form == 0 || (!f.ifcharge && !f.ifdipole)) return;
     const int pt = IR ? p + 1 : p;
     const int H = (pt + 1) * (pt + 2) / 2;
-    if (H <= kFormItems * 256) {
-      const size_t P2 = (size_t)(pt + 1) * (pt + 1);
-      int ts = (int)(65536 / (2 * sizeof(T) * (size_t)H));
-      ts = std::max(1, std::min(32, ts));
-      const size_t lds = form2_fac_bytes(p) + 2 * sizeof(T) * std::max((size_t)ts * H, 4 * P2);
-#define FORM2(C, D)                                                                                    \
+    const size_t P2 = (size_t)(pt + 1) * (pt + 1);
+    int ts = (int)(65536 / (2 * sizeof(T) * (size_t)H));
+    ts = std::max(1, std::min(32, ts));
+    const size_t lds2 = form2_fac_bytes(p) + 2 * sizeof(T) * std::max((size_t)ts * H, 4 * P2);
+#define FORM2(C, D, I)                                                                                 \
   if (f.ifcharge == C && f.ifdipole == D)                                                              \
-    hipLaunchKernelGGL((k_form2<T, IR, C, D>), dim3(nbox_form), dim3(256), lds, s, P, fac, spos, str64, ts, dst);
-      FORM2(1, 0) FORM2(0, 1) FORM2(1, 1)
-#undef FORM2
+    hipLaunchKernelGGL((k_form2<T, IR, C, D, I>), dim3(nbox_form), dim3(256), lds2, s, P, fac, spos, str64, ts, dst);
+    if (H <= kFormItems * 256) {
+      FORM2(1, 0, kFormItems) FORM2(0, 1, kFormItems) FORM2(1, 1, kFormItems)
+    } else if (H <= 2 * kFormItems * 256 && lds2 <= max_lds_bytes(ctx)) {
+      FORM2(1, 0, 2 * kFormItems) FORM2(0, 1, 2 * kFormItems) FORM2(1, 1, 2 * kFormItems)
     } else {
+#undef FORM2
       int ts = (int)((65536 / sizeof(T) - 2 * K) / (2 * H + 11));
       ts = std::max(1, std::min(32, ts));
       const size_t lds = sizeof(T) * (2 * K + (size_t)ts * (2 * H + 11));

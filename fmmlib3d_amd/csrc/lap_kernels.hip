@@ -135,7 +135,7 @@ __global__ void k_form(DevPlan P, const double* __restrict__ fac, const double* 
 // through the derivative factors and degree n-1 (P2M) or n+1 (P2L), with
 // (v_x -+ i v_y)/2 formed from S^{v_x}, S^{v_y} by linearity.
 // ---------------------------------------------------------------------------
-constexpr int kFormItems = 2;  // half-table entries per thread (256 threads)
+constexpr int kFormItems = 2;  // half-table entries per thread (256 threads); 4 for larger orders
 // LDS bytes k_form2 keeps ahead of the harmonic table, 16-aligned: the
 // recurrence factors (FacLayout MM..RB) and cm[m] = prod_{k<=m} MM[k],
 // m = 0..p+2
@@ -174,7 +174,7 @@ __device__ __forceinline__ void form_column(const double* facl, const FacLayout&
   }
 }
 
-template <typename T, bool IRREG, bool CHG, bool DIP>
+template <typename T, bool IRREG, bool CHG, bool DIP, int ITEMS>
 __global__ __launch_bounds__(256) void k_form2(DevPlan P, const double* __restrict__ fac,
                                                const double* __restrict__ spos, const double* __restrict__ str,
                                                int ts, T* __restrict__ expn) {
@@ -201,9 +201,9 @@ __global__ __launch_bounds__(256) void k_form2(DevPlan P, const double* __restri
   const double ihd = P.inv_side[P.level[b]];
   const T ih = (T)ihd;
   const double cx = P.center[3 * b], cy = P.center[3 * b + 1], cz = P.center[3 * b + 2];
-  T acc[kFormItems][16];
+  T acc[ITEMS][16];
 #pragma unroll
-  for (int q = 0; q < kFormItems; q++)
+  for (int q = 0; q < ITEMS; q++)
 #pragma unroll
     for (int e = 0; e < 16; e++) acc[q][e] = T(0);
   int jbeg = 0, jend = 1;
@@ -233,7 +233,7 @@ __global__ __launch_bounds__(256) void k_form2(DevPlan P, const double* __restri
 #pragma unroll
         for (int e = 0; e < 8; e++) wv[e] = (T)w[e];
 #pragma unroll
-        for (int q = 0; q < kFormItems; q++) {
+        for (int q = 0; q < ITEMS; q++) {
           const int h = tid + q * (int)blockDim.x;
           if (h < H) {
             const T hr = tab[2 * ((size_t)s * H + h)], hi = tab[2 * ((size_t)s * H + h) + 1];
@@ -255,7 +255,7 @@ __global__ __launch_bounds__(256) void k_form2(DevPlan P, const double* __restri
   __syncthreads();
   // sums to LDS: S[c][eidx(n', m')] complex, c = q, v_x, v_y, v_z
 #pragma unroll
-  for (int q = 0; q < kFormItems; q++) {
+  for (int q = 0; q < ITEMS; q++) {
     const int h = tid + q * (int)blockDim.x;
     if (h < H) {
       int n = (int)((sqrtf(8.0f * h + 1.0f) - 1.0f) * 0.5f);
