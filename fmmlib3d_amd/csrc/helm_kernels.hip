@@ -1002,6 +1002,7 @@ __global__ __launch_bounds__(256) void k_hp2p(DevPlan P, double kre, double kim,
   const int w = __builtin_amdgcn_readfirstlane((int)(blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)));
   if (w >= P.n_p2p) return;
   const int b = P.p2p_items[3 * w], start = P.p2p_items[3 * w + 2];
+  if (b < 0) return;   // padding item
   const int n0 = kind ? P.itrg[b] : P.isrc[b];
   const int nn = kind ? P.ntrg[b] : P.nsrc[b];
   const int i = start + (int)(threadIdx.x & 63);

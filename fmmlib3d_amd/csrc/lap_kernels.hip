@@ -392,6 +392,7 @@ __global__ __launch_bounds__(256) void k_l2p2(DevPlan P, const double* __restric
   const int w = __builtin_amdgcn_readfirstlane((int)(blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)));
   if (w >= P.n_l2pi) return;
   const int ib = P.l2p_items[3 * w], kind = P.l2p_items[3 * w + 1], start = P.l2p_items[3 * w + 2];
+  if (ib < 0) return;   // padding item
   const int b = P.l2p_boxes[ib];
   const T* pk = pack + (size_t)ib * H * 16;
   const T ih = (T)P.inv_side[P.level[b]];
@@ -1400,6 +1401,7 @@ __global__ __launch_bounds__(256) void k_p2p(DevPlan P, const double* __restrict
   const int w = __builtin_amdgcn_readfirstlane((int)(blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)));
   if (w >= P.n_p2p) return;
   const int b = P.p2p_items[3 * w], start = P.p2p_items[3 * w + 2];
+  if (b < 0) return;   // padding item
   const int n0 = kind ? P.itrg[b] : P.isrc[b];
   const int nn = kind ? P.ntrg[b] : P.nsrc[b];
   const int i = start + (int)(threadIdx.x & 63);

@@ -99,6 +99,20 @@ inline void build_plan_boxes(const Octree& t, const Flags& f, Flat& F, PlanOffse
   O.inv_side = F.d(inv);
 }
 
+// Work items (box, kind, first point) of kP2PBlock points for one box.  A
+// box's items go into one workgroup (kItemWaves items) when they fit, so
+// its waves share the source data in cache: the list is padded with null
+// items (box -1, skipped by the kernels) up to the next workgroup when the
+// box's items would straddle one.
+inline void push_items(std::vector<int>& it, int b, int kind, int npts) {
+  const int c = (npts + kP2PBlock - 1) / kP2PBlock;
+  if (c == 0) return;
+  const int used = (int)(it.size() / 3) % kItemWaves;
+  if (used > 0 && used + c > kItemWaves)
+    for (int k = used; k < kItemWaves; k++) { it.push_back(-1); it.push_back(kind); it.push_back(0); }
+  for (int q = 0; q < npts; q += kP2PBlock) { it.push_back(b); it.push_back(kind); it.push_back(q); }
+}
+
 // Part B1: list 1 and the P2P work items (needs tree_lists_near).
 inline void build_plan_near(const Octree& t, const Flags& f, Flat& F, PlanOffsets& O) {
   const int nb = t.nboxes;
@@ -107,10 +121,8 @@ inline void build_plan_near(const Octree& t, const Flags& f, Flat& F, PlanOffset
   std::vector<int> p2p0, p2p1;
   for (int b = 0; b < nb; b++) {
     if (t.nchild[b] != 0) continue;
-    if (want_src)
-      for (int s = 0; s < t.nsrc[b]; s += kP2PBlock) { p2p0.push_back(b); p2p0.push_back(0); p2p0.push_back(s); }
-    if (want_trg)
-      for (int s = 0; s < t.ntrg[b]; s += kP2PBlock) { p2p1.push_back(b); p2p1.push_back(1); p2p1.push_back(s); }
+    if (want_src) push_items(p2p0, b, 0, t.nsrc[b]);
+    if (want_trg) push_items(p2p1, b, 1, t.ntrg[b]);
   }
   O.p2p0 = F.i(p2p0); O.n_p2p0 = (int)p2p0.size() / 3;
   O.p2p1 = F.i(p2p1); O.n_p2p1 = (int)p2p1.size() / 3;
@@ -162,10 +174,8 @@ inline void build_plan_far(const Octree& t, const Flags& f, Flat& F, PlanOffsets
     std::vector<int> it;
     for (int i = 0; i < (int)l2p.size(); i++) {
       const int b = l2p[i];
-      if (want_src)
-        for (int q = 0; q < t.nsrc[b]; q += kP2PBlock) { it.push_back(i); it.push_back(0); it.push_back(q); }
-      if (want_trg)
-        for (int q = 0; q < t.ntrg[b]; q += kP2PBlock) { it.push_back(i); it.push_back(1); it.push_back(q); }
+      if (want_src) push_items(it, i, 0, t.nsrc[b]);
+      if (want_trg) push_items(it, i, 1, t.ntrg[b]);
     }
     O.l2pi = F.i(it); O.n_l2pi = (int)it.size() / 3;
   }
