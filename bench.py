@@ -49,7 +49,7 @@ FLOPS_PER_PAIR = 65            # charge+dipole, pot+fld, rsqrt counted as 2
 # HBM bytes per k_p2p launch on this workload, from rocprofv3 FETCH_SIZE and
 # WRITE_SIZE runs calibrated against a 1 GiB device copy
 # (profiles/fmm_kernels_lap1M_helm500k.md): 2 x FETCH KiB + WRITE KiB.
-P2P_TRAFFIC_BYTES = {"fp64": (2 * 4033882 + 62847) * 1024, "fp32": (2 * 2589495 + 62853) * 1024}
+P2P_TRAFFIC_BYTES = {"fp64": (2 * 4062612 + 62846) * 1024, "fp32": (2 * 2585573 + 62852) * 1024}
 
 
 def make_inputs(n, seed):
