@@ -36,6 +36,9 @@ struct HelmCache {
   size_t o_fac = 0, o_offph = 0, o_octph = 0, o_rot = 0, o_sclev = 0, o_z = 0;
   size_t o_rotoff = 0, o_coax = 0;
   int m2m_base = 0, l2l_base = 0;
+  // grouped M2L: rotation tiles and items at pmax, complex coaxial tiles
+  size_t o_rothalf = 0, o_rothalf_off = 0, o_gri = 0, o_grc = 0, o_gzi = 0, o_gzo = 0, o_zt = 0, o_ztoff = 0;
+  bool grouped_m2l = false;   // every list-2 order fits its table's tiles
 };
 
 HelmCache* helm_tables(Context& ctx, cplx zk, double size, int nlev, double eps, int& ier) {
@@ -75,6 +78,62 @@ HelmCache* helm_tables(Context& ctx, cplx zk, double size, int nlev, double eps,
     for (const auto& d : H.m2m) put_desc(d);
     nc->l2l_base = nc->m2m_base + nlev + 1;
     for (const auto& d : H.l2l) put_desc(d);
+    // grouped M2L: the geometry tables' rotation tiles and work items (order
+    // pmax), and each M2L table's coaxial blocks as complex 4x4 tiles per
+    // order am: [row block of n - am][column block of l - am], 16 real parts
+    // then 16 imaginary parts of [row][column]
+    {
+      const int P1 = H.pmax + 1;
+      nc->o_gri = put_i(H.geo.grp_ritems);
+      nc->o_grc = put_i(H.geo.grp_rcount);
+      nc->o_gzi = put_i(H.geo.grp_zitems);
+      nc->o_gzo = put_i(H.geo.grp_zoff);
+      nc->o_rothalf = put_d(H.geo.rot_half);
+      const size_t ntab = H.m2l.size();
+      std::vector<long> zt_off(ntab * P1, 0);
+      std::vector<double> zt;
+      bool ok = true;
+      const size_t nzc = H.z.size() / 2;   // complex entries
+      for (size_t t = 0; t < ntab; t++) {
+        // M2L runs from level 2 on; the descriptors of levels 0 and 1 are empty
+        if (t < 2 * (size_t)H.geo.ndist) continue;
+        const CoaxDesc& d = H.m2l[t];
+        if (d.pin != d.pout || d.pout < 0 || d.pout > H.pmax) { ok = false; continue; }
+        const int pz = d.pout;
+        // the table's last block ends inside the coaxial data
+        if ((size_t)(d.base + coax_block(pz, pz, pz) + 1) > nzc) { ok = false; continue; }
+        for (int am = 0; am <= pz; am++) {
+          const int nb4 = (pz - am) / 4 + 1;
+          const size_t b0 = zt.size();
+          zt_off[t * P1 + am] = (long)b0;
+          zt.resize(b0 + 32 * (size_t)nb4 * nb4, 0.0);
+          const long mb = d.base + coax_block(am, d.pin, d.pout);
+          for (int n = am; n <= pz; n++)
+            for (int l = am; l <= pz; l++) {
+              const long ix = mb + (long)(l - am) * (d.pin + 1 - am) + (n - am);
+              const int rr = n - am, cc = l - am;
+              const size_t tile = b0 + 32 * ((size_t)(rr / 4) * nb4 + cc / 4);
+              zt[tile + (rr % 4) * 4 + cc % 4] = H.z[2 * ix];
+              zt[tile + 16 + (rr % 4) * 4 + cc % 4] = H.z[2 * ix + 1];
+            }
+        }
+      }
+      // every list-2 order must lie within its table's tiles
+      const int nd2 = H.geo.ndist;
+      for (int l = 2; l <= nlev && ok; l++)
+        for (int code = 0; code < 343 && ok; code++) {
+          const int pt = H.ptrunc[(size_t)l * 343 + code];
+          if (pt <= 0) continue;
+          const CoaxDesc& d = H.m2l[(size_t)l * nd2 + H.geo.off_dist[code]];
+          ok = d.pin == d.pout && pt <= d.pout;
+        }
+      nc->grouped_m2l = ok;
+      nc->o_zt = put_d(zt);
+      nc->o_rothalf_off = Lg.size();
+      Lg.insert(Lg.end(), H.geo.rot_half_off.begin(), H.geo.rot_half_off.end());
+      nc->o_ztoff = Lg.size();
+      Lg.insert(Lg.end(), zt_off.begin(), zt_off.end());
+    }
     void* di = ctx.arena.get("htab_i", I.size() * sizeof(int));
     void* dd = ctx.arena.get("htab_d", D.size() * sizeof(double));
     void* dl = ctx.arena.get("htab_l", Lg.size() * sizeof(long));
@@ -197,6 +256,10 @@ int helm_solve(Context& ctx, const LapCall& c) {
   Hd.nd = H.geo.ndist;
   Hd.plev = ti + HC->o_plev; Hd.sclev = td + HC->o_sclev; Hd.ptrunc = ti + HC->o_ptrunc;
   Hd.coax = tl + HC->o_coax; Hd.z = td + HC->o_z;
+  Hd.rot_half = td + HC->o_rothalf; Hd.rot_half_off = tl + HC->o_rothalf_off;
+  Hd.grp_ritems = ti + HC->o_gri; Hd.grp_rcount = ti + HC->o_grc;
+  Hd.grp_zitems = ti + HC->o_gzi; Hd.grp_zoff = ti + HC->o_gzo;
+  Hd.zt = td + HC->o_zt; Hd.zt_off = tl + HC->o_ztoff;
   const size_t nb = tree.nboxes;
   double* spos = (double*)ctx.arena.get("spos", sizeof(double) * 4 * (size_t)ns);
   double* str64 = (double*)ctx.arena.get("str64", sizeof(double) * 8 * (size_t)ns);
@@ -295,9 +358,27 @@ int helm_solve(Context& ctx, const LapCall& c) {
   // STEP 2: P2L (hfmm3dpart.f:765-830)
   form(std::true_type{}, P.n_p2l, lo);
   tm.mark(5);
-  // STEP 4: M2L
+  // STEP 4: M2L, grouped by translation class (k_hm2l_grp) when the group's
+  // expansions fit LDS and a stage's work items fit one pass, else pair by
+  // pair (k_hm2l)
   if (P.n_m2l) {
-    if (m2l_lds)
+    int nI = 0;
+    for (int n = 0; n <= pmax; n++) nI += n / 4 + 1;
+    auto grp_lds = [&](int g) {
+      return (6 * kM2LChunk + (size_t)grp_kn_ints(pmax)) * sizeof(int) +
+             sizeof(double) * (2 * (size_t)kM2LGroup * (pmax + 1) + 2 * (size_t)K + 8 +
+                               (size_t)g * grp_member_stride<double>(pmax));
+    };
+    int gmax = 0;
+    for (int g = kM2LGroup; g >= 1 && !gmax && HC->grouped_m2l; g--) {
+      const int gs = g > 2 ? 4 : g;
+      if (nI * gs <= 256 && grp_lds(g) <= ldsmax) gmax = g;
+    }
+    // (one work item per thread: with two, four members' expansions take
+    // ~92 KB of LDS at pmax 23 and leave one workgroup per CU)
+    if (gmax > 0)
+      hipLaunchKernelGGL((k_hm2l_grp<256, 1>), dim3(P.n_m2l), dim3(256), grp_lds(gmax), s, P, Tb, Hd, gmax, mp, lo);
+    else if (m2l_lds)
       hipLaunchKernelGGL((k_hm2l<true>), dim3(P.n_m2l), dim3(256), sizeof(double) * 6 * (size_t)K, s, P, Tb, Hd, mp, lo);
     else
       hipLaunchKernelGGL((k_hm2l<false>), dim3(P.n_m2l), dim3(256), lds_ps, s, P, Tb, Hd, mp, lo);

@@ -14,6 +14,7 @@
 #include <hip/hip_runtime.h>
 
 #include "kern_common.hpp"
+#include "m2l_grp.hpp"
 
 namespace fmm3d {
 
@@ -28,6 +29,13 @@ struct HelmDev {
   const int* ptrunc;       // (nlev+1) x 343 list-2 orders
   const long* coax;        // per table: base, pin, pout
   const double* z;         // coaxial blocks, complex
+  // grouped M2L (k_hm2l_grp): rotation tiles and work items at pmax, and the
+  // coaxial M2L tables in complex 4x4 tiles, (pmax+1) offsets per table
+  const double* rot_half;
+  const long* rot_half_off;
+  const int *grp_ritems, *grp_rcount, *grp_zitems, *grp_zoff;
+  const double* zt;
+  const long* zt_off;
 };
 
 __device__ __forceinline__ Cd cadd2(Cd a, Cd b) { return {a.re + b.re, a.im + b.im}; }
@@ -908,6 +916,217 @@ __global__ void k_hm2l(DevPlan P, DevTables Tb, HelmDev Hd, const double* __rest
     double* out = lo + 2 * (size_t)b * P.K;
     for (int k = threadIdx.x; k < 2 * K; k += blockDim.x) out[k] += acc[k];
   }
+}
+
+// ---------------------------------------------------------------------------
+// M2L grouped by translation class (the scheme of the Laplace k_m2l_grp):
+// the list-2 sources of a target box with the same (dx^2+dy^2, dz) share the
+// rotation class, the coaxial table (per level and distance class) and the
+// truncation order; up to kM2LGroup of them are translated together, the
+// lanes of a work item holding the members.  The coaxial shift is a complex
+// table (grp_z_item_c); no final scale (the tables carry the level's sc).
+// src/hfmm3drouts.f:330-477 applies the same operators pair by pair.
+// ---------------------------------------------------------------------------
+template <int NT, int ITEMS>
+__global__ __launch_bounds__(NT) void k_hm2l_grp(DevPlan P, DevTables Tb, HelmDev Hd, int gmax,
+                                                   const double* __restrict__ mp, double* __restrict__ lo) {
+  using T = double;
+  extern __shared__ unsigned char smem_raw[];
+  int* s_key = reinterpret_cast<int*>(smem_raw);
+  int* s_box = s_key + kM2LChunk;
+  int* s_code = s_box + kM2LChunk;
+  int* t_key = s_code + kM2LChunk;
+  int* t_box = t_key + kM2LChunk;
+  int* t_code = t_box + kM2LChunk;
+  const int tid = threadIdx.x, bs = blockDim.x;
+  const int b = P.m2l_boxes[blockIdx.x];
+  const int p = P.p, P1 = p + 1, K2 = 2 * P.K;
+  // degree of each packed index k < (p+4)^2, and the members' phases
+  // e^{i am phi}, am = 0..p, of the group in hand
+  int* s_kn = t_code + kM2LChunk;
+  T* s_ph = reinterpret_cast<T*>(s_kn + grp_kn_ints(p));
+  T* acc = s_ph + 2 * kM2LGroup * P1;
+  // members: (p+4)^2 complex each (degrees p+1..p+3 stay zero or hold the
+  // zeros written for a truncated member), after a zero guard of 8 values
+  const int MS = grp_member_stride<T>(p);
+  T* A = acc + K2 + 8;
+  const T ih = T(1);   // the coaxial tables carry the level scales
+  const int lev = P.level[b];
+  for (int k = tid; k < K2 + 8 + gmax * MS; k += bs) acc[k] = T(0);
+  for (int k = tid; k < (p + 4) * (p + 4); k += bs) {
+    int n = (int)sqrtf((float)k);
+    while (n * n > k) n--;
+    while ((n + 1) * (n + 1) <= k) n++;
+    s_kn[k] = n;
+  }
+  const T* rtab = Hd.rot_half;
+  const int off0 = P.l2_off[b], n2 = P.l2_off[b + 1] - off0;
+  const int bi = P.ijk[3 * b], bj = P.ijk[3 * b + 1], bk = P.ijk[3 * b + 2];
+  for (int c0 = 0; c0 < n2; c0 += kM2LChunk) {
+    const int cn = min(kM2LChunk, n2 - c0);
+    __syncthreads();
+    if (tid < cn) {
+      const int j = P.l2_idx[off0 + c0 + tid];
+      const int di = P.ijk[3 * j] - bi, dj = P.ijk[3 * j + 1] - bj, dk = P.ijk[3 * j + 2] - bk;
+      s_box[tid] = j;
+      s_code[tid] = ((di + 3) * 7 + (dj + 3)) * 7 + (dk + 3);
+      s_key[tid] = (di * di + dj * dj) * 7 + (dk + 3);
+    }
+    __syncthreads();
+    if (tid < cn) {  // stable rank by key
+      const int key = s_key[tid];
+      int rank = 0;
+      for (int t = 0; t < cn; t++) {
+        const int kt = s_key[t];
+        rank += (kt < key) || (kt == key && t < tid);
+      }
+      t_key[rank] = key;
+      t_box[rank] = s_box[tid];
+      t_code[rank] = s_code[tid];
+    }
+    __syncthreads();
+    for (int s0 = 0; s0 < cn;) {
+      const int key0 = t_key[s0];
+      int g = 1;
+      while (g < gmax && s0 + g < cn && t_key[s0 + g] == key0) g++;
+      const int gs = g > 2 ? 4 : g;  // lanes per item
+      const int lg = gs == 4 ? 2 : gs - 1;
+      const int code0 = t_code[s0];
+      const int pt = Hd.ptrunc[lev * 343 + code0];
+      const int rc = Tb.off_rot[code0];
+      const int tz = lev * Hd.nd + Tb.off_dist[code0];
+      const long* zto = Hd.zt_off + (size_t)tz * P1;
+      const int pz = (int)Hd.coax[3 * tz + 2];   // order of the table's tiles
+      const int Kp = (pt + 1) * (pt + 1), Kz = (pt + 4) * (pt + 4);
+      for (int idx = tid; idx < g * (pt + 1); idx += bs) {
+        const int jj = idx / (pt + 1), am = idx - jj * (pt + 1);
+        const double* ph = Tb.off_phase + (size_t)t_code[s0 + jj] * 2 * P1;
+        s_ph[2 * (jj * P1 + am)] = (T)ph[2 * am];
+        s_ph[2 * (jj * P1 + am) + 1] = (T)ph[2 * am + 1];
+      }
+      __syncthreads();
+      // (a) members times e^{i m phi}; degrees pt+1..pt+3 set to zero
+      for (int jj = 0; jj < g; jj++) {
+        const T* src = mp + (size_t)t_box[s0 + jj] * K2;
+        const T* phj = s_ph + 2 * jj * P1;
+        T* Aj = A + jj * MS;
+        for (int k = tid; k < Kz; k += bs) {
+          T vr = 0, vi = 0;
+          if (k < Kp) {
+            const int n = s_kn[k];
+            const int m = k - n * n - n, am = m < 0 ? -m : m;
+            const T er = phj[2 * am], ei = m < 0 ? -phj[2 * am + 1] : phj[2 * am + 1];
+            const T xr = src[2 * k], xi = src[2 * k + 1];
+            vr = xr * er - xi * ei;
+            vi = xr * ei + xi * er;
+          }
+          Aj[2 * k] = vr;
+          Aj[2 * k + 1] = vi;
+        }
+      }
+      __syncthreads();
+      // Work items of a stage (rotation items by degree, z-shift items by
+      // order; the two counts are equal): member jj of item u0 in the first
+      // round and of item NI-1-u0 in the second, which pairs the cheapest
+      // items with the most expensive ones.
+      const int NI = Hd.grp_rcount[pt], U0 = bs >> lg;
+      const int u0 = tid >> lg, jj = tid & (gs - 1);
+      int uq[ITEMS];
+      uq[0] = u0 < NI ? u0 : -1;
+      if (ITEMS > 1) uq[ITEMS - 1] = NI - 1 - u0 >= U0 ? NI - 1 - u0 : -1;
+      const bool mem = jj < g;
+      T r[ITEMS][16];
+      // (b) rotate to the z axis
+#pragma unroll
+      for (int q = 0; q < ITEMS; q++) {
+#pragma unroll
+        for (int e = 0; e < 16; e++) r[q][e] = T(0);
+        if (uq[q] >= 0 && mem) {
+          const int it = Hd.grp_ritems[uq[q]], n = it >> 8, qc = it & 255;
+          grp_rot_item<T, false>(rtab + Hd.rot_half_off[rc * P1 + n] + 16 * qc, n / 4 + 1, n, qc, A + jj * MS, r[q]);
+        }
+      }
+      __syncthreads();
+#pragma unroll
+      for (int q = 0; q < ITEMS; q++)
+        if (uq[q] >= 0 && mem) {
+          const int it = Hd.grp_ritems[uq[q]];
+          grp_rot_store<T>(it >> 8, 4 * (it & 255), r[q], A + jj * MS);
+        }
+      __syncthreads();
+      // (c) shift along z
+      const int zo = Hd.grp_zoff[pt];
+#pragma unroll
+      for (int q = 0; q < ITEMS; q++) {
+#pragma unroll
+        for (int e = 0; e < 16; e++) r[q][e] = T(0);
+        if (uq[q] >= 0 && mem) {
+          const int it = Hd.grp_zitems[zo + uq[q]], am = it >> 8;
+          grp_z_item_c<T>(Hd.zt + zto[am], (pz - am) / 4 + 1, am, it & 255, pt, A + jj * MS, r[q]);
+        }
+      }
+      __syncthreads();
+#pragma unroll
+      for (int q = 0; q < ITEMS; q++)
+        if (uq[q] >= 0 && mem) {
+          const int it = Hd.grp_zitems[zo + uq[q]], am = it >> 8;
+          grp_z_store<T>(am, am + 4 * (it & 255), pt, r[q], A + jj * MS);
+        }
+      __syncthreads();
+      // (d) rotate back, apply e^{-i m phi} and the scale, sum the members
+#pragma unroll
+      for (int q = 0; q < ITEMS; q++) {
+#pragma unroll
+        for (int e = 0; e < 16; e++) r[q][e] = T(0);
+        int n = 0, mp0 = 0;
+        if (uq[q] >= 0) {
+          const int it = Hd.grp_ritems[uq[q]];
+          n = it >> 8;
+          mp0 = 4 * (it & 255);
+          if (mem) {
+            grp_rot_item<T, true>(rtab + Hd.rot_half_off[rc * P1 + n] + 4 * mp0, n / 4 + 1, n, mp0 / 4,
+                                  A + jj * MS, r[q]);
+            const T* ph = s_ph + 2 * jj * P1;
+#pragma unroll
+            for (int c = 0; c < 4; c++) {
+              const int mq = mp0 + c <= n ? mp0 + c : n;
+              const T er = ph[2 * mq] * ih, ei = ph[2 * mq + 1] * ih;
+              const T pr = r[q][4 * c], pi = r[q][4 * c + 1], qr = r[q][4 * c + 2], qi = r[q][4 * c + 3];
+              r[q][4 * c] = pr * er + pi * ei;       // (pr + i pi) e^{-i m' phi}
+              r[q][4 * c + 1] = pi * er - pr * ei;
+              r[q][4 * c + 2] = qr * er - qi * ei;   // (qr + i qi) e^{+i m' phi}
+              r[q][4 * c + 3] = qi * er + qr * ei;
+            }
+          }
+        }
+#pragma unroll
+        for (int e = 0; e < 16; e++) {
+          T v = r[q][e];
+          if (gs > 1) v += __shfl_xor(v, 1);
+          if (gs > 2) v += __shfl_xor(v, 2);
+          r[q][e] = v;
+        }
+        if (uq[q] >= 0 && jj == 0) {
+#pragma unroll
+          for (int c = 0; c < 4; c++) {
+            const int mq = mp0 + c;
+            if (mq > n) break;
+            acc[2 * eidx(n, mq)] += r[q][4 * c];
+            acc[2 * eidx(n, mq) + 1] += r[q][4 * c + 1];
+            if (mq > 0) {
+              acc[2 * eidx(n, -mq)] += r[q][4 * c + 2];
+              acc[2 * eidx(n, -mq) + 1] += r[q][4 * c + 3];
+            }
+          }
+        }
+      }
+      __syncthreads();
+      s0 += g;
+    }
+  }
+  __syncthreads();
+  T* out = lo + (size_t)b * K2;
+  for (int k = tid; k < K2; k += bs) out[k] += acc[k];
 }
 
 // ---------------------------------------------------------------------------
