@@ -140,6 +140,18 @@ constexpr int kFormItems = 2;  // half-table entries per thread (256 threads); 4
 // LDS bytes k_form2 keeps ahead of the harmonic table, 16-aligned: the
 // recurrence factors (FacLayout MM..RB) and cm[m] = prod_{k<=m} MM[k],
 // m = 0..p+2
+// Tile row stride (entries) of k_form2: the half table of one source, padded
+// so that consecutive sources start one entry apart modulo the 32 LDS banks
+// (the harmonic columns of neighbouring sources are written by neighbouring
+// lanes)
+template <typename T>
+__host__ __device__ inline int form2_row_stride(int H) {
+  const int words = 2 * (int)sizeof(T) / 4;  // per complex entry
+  int hs = H;
+  while ((hs * words) % 32 != words) hs++;
+  return hs;
+}
+
 __host__ __device__ inline size_t form2_fac_bytes(int p) {
   return ((size_t)(FacLayout(p).AR + p + 3) * sizeof(double) + 15) / 16 * 16;
 }
@@ -183,6 +195,7 @@ __global__ __launch_bounds__(256) void k_form2(DevPlan P, const double* __restri
   const int p = P.p, K = P.K;
   const int pt = IRREG ? p + 1 : p;
   const int H = (pt + 1) * (pt + 2) / 2;
+  const int HS = form2_row_stride<T>(H);
   const int P2 = (pt + 1) * (pt + 1);
   FacLayout L(p);
   const int tid = threadIdx.x;
@@ -225,7 +238,7 @@ __global__ __launch_bounds__(256) void k_form2(DevPlan P, const double* __restri
           ir2 = T(1) / r2;
           ir = sqrt(ir2);
         }
-        form_column<T, IRREG>(facl, L, pt, m, (T)cmt[m], ux, uy, uz, r2, ir2, ir, tab + 2 * (size_t)s * H);
+        form_column<T, IRREG>(facl, L, pt, m, (T)cmt[m], ux, uy, uz, r2, ir2, ir, tab + 2 * (size_t)s * HS);
       }
       __syncthreads();
       for (int s = 0; s < tn; s++) {
@@ -237,7 +250,7 @@ __global__ __launch_bounds__(256) void k_form2(DevPlan P, const double* __restri
         for (int q = 0; q < ITEMS; q++) {
           const int h = tid + q * (int)blockDim.x;
           if (h < H) {
-            const T hr = tab[2 * ((size_t)s * H + h)], hi = tab[2 * ((size_t)s * H + h) + 1];
+            const T hr = tab[2 * ((size_t)s * HS + h)], hi = tab[2 * ((size_t)s * HS + h) + 1];
 #pragma unroll
             for (int c = 0; c < 4; c++) {
               if ((c == 0 && !CHG) || (c > 0 && !DIP)) continue;
