@@ -258,17 +258,23 @@ void tree_lists_near(Octree& t) {
   auto& l1 = S.l1;
   auto& l3 = S.l3;
   auto& mirror = S.mirror;
-  for (auto* v : {&S.coll, &S.l1, &S.l2, &S.l3, &S.l4}) {
+  for (auto* v : {&S.coll, &S.l1, &S.l2, &S.l3, &S.l4})
     if ((int)v->size() < nb) v->resize(nb);
-#pragma omp parallel for schedule(static)
-    for (int b = 0; b < nb; b++) (*v)[b].clear();
-  }
   if ((int)mirror.size() < nthreads) mirror.resize(nthreads);
   for (auto& m : mirror) m.clear();
+  auto& l2s = S.l2;
+  auto& l4s = S.l4;
+  // one parallel region (a fork/join costs more than a level's work here)
+#pragma omp parallel
+  {
+#pragma omp for schedule(static)
+    for (int b = 0; b < nb; b++) {
+      coll[b].clear(); l1[b].clear(); l2s[b].clear(); l3[b].clear(); l4s[b].clear();
+    }
   // colleagues, level by level: the same-level boxes touching b are children
   // of b's parent or of the parent's colleagues
   for (int l = 1; l <= t.nlev; l++) {
-#pragma omp parallel for schedule(static)
+#pragma omp for schedule(static)
     for (int b = t.ladd[l]; b < t.ladd[l + 1]; b++) {
       const int par = t.parent[b];
       auto visit = [&](int q) {
@@ -287,7 +293,7 @@ void tree_lists_near(Octree& t) {
   }
   // lists 1 and 3 for leaves below the root (d3tlst31, d3tstrcr.f:912-1069);
   // the coarse-to-fine list-1 entries are mirrored afterwards.
-#pragma omp parallel for schedule(dynamic, 16)
+#pragma omp for schedule(dynamic, 16)
   for (int b = 1; b < nb; b++) {
     if (t.nchild[b]) continue;
     int tid = omp_get_thread_num();
@@ -308,6 +314,7 @@ void tree_lists_near(Octree& t) {
       }
     }
   }
+  }  // parallel
   for (int th = 0; th < nthreads; th++)
     for (auto& pr : mirror[th]) l1[pr.first].push_back(pr.second);
   flatten(l1, nb, t.list[0]);
