@@ -119,9 +119,12 @@ def test_gpu_scaled_and_shifted_coordinates(rng):
     assert base[0] == 0
 
 
-def test_gpu_deterministic(rng):
-    src = points(rng, 30000, "cube")
-    trg = points(rng, 10000, "cube")
+@pytest.mark.parametrize("dist", ["cube", "clusters"])
+def test_gpu_deterministic(rng, dist):
+    """Bitwise equal results on repeated calls (clustered points exercise the
+    list-3 slices summed in order and list 4)."""
+    src = points(rng, 30000, dist)
+    trg = points(rng, 10000, dist)
     q, ds, dv = strengths(rng, 30000, True, True)
     a = F.lfmm3dparttarg(2, src, q, ds, dv, trg)
     b = F.lfmm3dparttarg(2, src, q, ds, dv, trg)
