@@ -131,7 +131,9 @@ __global__ void k_form(DevPlan P, const double* __restrict__ fac, const double* 
 // w = q, v_x, v_y, v_z (H^{-a} = conj(H^a)); the weights are wave-uniform
 // loads.  The harmonic table of a tile of `ts` sources is built in LDS, one
 // thread per (column m, source), m-major so that a wave holds few orders and
-// its loop lengths are close.  At the end the sums go to LDS
+// its loop lengths are close; each source's scalars (form_scalars) are
+// computed once, during the previous tile's sums, from positions read while
+// the previous tile's table was built.  At the end the sums go to LDS
 // and one thread per coefficient combines them: charge S^q(n, -m), dipole
 // through the derivative factors and degree n-1 (P2M) or n+1 (P2L), with
 // (v_x -+ i v_y)/2 formed from S^{v_x}, S^{v_y} by linearity.
@@ -156,34 +158,75 @@ __host__ __device__ inline size_t form2_fac_bytes(int p) {
   return ((size_t)(FacLayout(p).AR + p + 3) * sizeof(double) + 15) / 16 * 16;
 }
 
-// Column m of the regular (R, r2 = |u|^2) or irregular (S, ir2 = 1/|u|^2,
-// ir = 1/|u|) harmonics into a half table, n = m..pt.  The diagonal entry
-// R_m^m = cm w^m, S_m^m = ir cm (w ir2)^m with w = -(ux + i uy) is taken by
-// binary powers; then the three-term recurrences of regular_column and
-// irregular_column.
+// Per-source scalars of the column tasks, 8 T per source: the power base
+// w = -(ux + i uy) (times 1/|u|^2 for S), uz, the scale of the recurrence's
+// second term (|u|^2 for R, 1 for S), the scale of each new entry (1 for R,
+// 1/|u|^2 for S) and of the diagonal (1 for R, 1/|u| for S).
 template <typename T, bool IRREG>
-__device__ __forceinline__ void form_column(const double* facl, const FacLayout& L, int pt, int m, T cm, T ux,
-                                            T uy, T uz, T r2, T ir2, T ir, T* out) {
-  C2<T> base = IRREG ? C2<T>{-ux * ir2, -uy * ir2} : C2<T>{-ux, -uy};
-  C2<T> r = {IRREG ? cm * ir : cm, T(0)};
+__device__ __forceinline__ void form_scalars(double x, double y, double z, double cx, double cy, double cz,
+                                             double ihd, T* sc) {
+  const T ux = (T)((x - cx) * ihd), uy = (T)((y - cy) * ihd), uz = (T)((z - cz) * ihd);
+  const T r2 = ux * ux + uy * uy + uz * uz;
+  T ir2 = T(1), ir = T(1);
+  if (IRREG) {
+    ir2 = T(1) / r2;
+    ir = sqrt(ir2);
+  }
+  sc[0] = IRREG ? -ux * ir2 : -ux;
+  sc[1] = IRREG ? -uy * ir2 : -uy;
+  sc[2] = uz;
+  sc[3] = IRREG ? T(1) : r2;
+  sc[4] = ir2;
+  sc[5] = ir;
+  sc[6] = T(0);
+  sc[7] = T(0);
+}
+
+// Column m of the regular (R) or irregular (S) harmonics of one source into
+// a half table, n = m..pt, from that source's scalars `sc` (form_scalars).
+// The diagonal entry R_m^m = cm w^m, S_m^m = ir cm (w ir2)^m is taken by
+// binary powers; then the three-term recurrences of regular_column and
+// irregular_column.  The entry index advances by n + 1 per step, and the
+// next step's two recurrence factors are read from LDS before this step's
+// arithmetic, so the LDS latency overlaps the dependent chain.
+template <typename T, bool IRREG>
+__device__ __forceinline__ void form_column(const double* facl, const FacLayout& L, int pt, int m, T cm,
+                                            const T* sc, T* out) {
+  C2<T> base = {sc[0], sc[1]};
+  const T uz = sc[2], bsc = sc[3], csc = sc[4];
+  C2<T> r = {IRREG ? cm * sc[5] : cm, T(0)};
   for (int e = m; e; e >>= 1) {
     if (e & 1) r = cmul(r, base);
     base = cmul(base, base);
   }
-  cst(out, hidx(m, m), r);
+  int k = hidx(m, m);
+  cst(out, k, r);
   if (m + 1 > pt) return;
-  const T g = (T)facl[L.M1 + m] * uz * (IRREG ? ir2 : T(1));
+  const T g = (T)facl[L.M1 + m] * uz * csc;
   C2<T> r1 = {g * r.re, g * r.im};
-  cst(out, hidx(m + 1, m), r1);
+  k += m + 1;  // hidx(m + 1, m)
+  cst(out, k, r1);
+  if (m + 2 > pt) return;
   C2<T> r0 = r;
+  k += m + 2;  // hidx(m + 2, m)
+  double fa = facl[L.RA + k], fb = facl[L.RB + k];
+  T cn = (T)(2 * m + 3);  // 2n - 1, exact
+#pragma unroll 2
   for (int n = m + 2; n <= pt; n++) {
-    const T a = (T)(2 * n - 1) * uz;
-    const T b = IRREG ? (T)facl[L.RA + hidx(n, m)] : (T)facl[L.RA + hidx(n, m)] * r2;
-    const T c = IRREG ? (T)facl[L.RB + hidx(n, m)] * ir2 : (T)facl[L.RB + hidx(n, m)];
+    const T b = IRREG ? (T)fa : (T)fa * bsc;
+    const T c = IRREG ? (T)fb * csc : (T)fb;
+    const int kn = k + n + 1;  // hidx(n + 1, m)
+    // at n = pt this reads past the column (still inside the factor and cm
+    // tables in LDS); the values go unused
+    fa = facl[L.RA + kn];
+    fb = facl[L.RB + kn];
+    const T a = cn * uz;
+    cn += T(2);
     const C2<T> rn = {(a * r1.re - b * r0.re) * c, (a * r1.im - b * r0.im) * c};
-    cst(out, hidx(n, m), rn);
+    cst(out, k, rn);
     r0 = r1;
     r1 = rn;
+    k = kn;
   }
 }
 
@@ -220,51 +263,92 @@ __global__ __launch_bounds__(256) void k_form2(DevPlan P, const double* __restri
   for (int q = 0; q < ITEMS; q++)
 #pragma unroll
     for (int e = 0; e < 16; e++) acc[q][e] = T(0);
-  int jbeg = 0, jend = 1;
-  if (IRREG) { jbeg = P.l4_off[b]; jend = P.l4_off[b + 1]; }
-  for (int jj = jbeg; jj < jend; jj++) {
-    const int j = IRREG ? P.l4_idx[jj] : b;
-    const int s0 = P.isrc[j], sn = P.nsrc[j];
-    for (int t0 = 0; t0 < sn; t0 += ts) {
-      const int tn = min(ts, sn - t0);
-      __syncthreads();
-      for (int task = tid; task < tn * (pt + 1); task += blockDim.x) {
-        const int m = task / tn, s = task - m * tn;
-        const double* x = spos + 4 * (size_t)(s0 + t0 + s);
-        const T ux = (T)((x[0] - cx) * ihd), uy = (T)((x[1] - cy) * ihd), uz = (T)((x[2] - cz) * ihd);
-        const T r2 = ux * ux + uy * uy + uz * uz;
-        T ir2 = T(0), ir = T(0);
-        if (IRREG) {
-          ir2 = T(1) / r2;
-          ir = sqrt(ir2);
+  // per-source scalars of the current tile, after the tile region
+  T* scs = tab + 2 * (size_t)max(ts * HS, 4 * P2);
+  T* wsh = scs + 8 * ts;  // the tile's source weights
+  int rot = blockIdx.x;
+  using V2 = typename std::conditional<std::is_same<T, double>::value, double2, float2>::type;
+  // tile cursor over the sources of the list-4 boxes (P2L) or of box b (P2M):
+  // (jj, t0) moves to the first tile holding sources, s0/sn of its box
+  const int jend = IRREG ? P.l4_off[b + 1] : 1;
+  auto settle = [&](int& jj, int& t0, int& s0, int& sn) {
+    for (; jj < jend; jj++, t0 = 0) {
+      const int j = IRREG ? P.l4_idx[jj] : b;
+      s0 = P.isrc[j];
+      sn = P.nsrc[j];
+      if (t0 < sn) return;
+    }
+  };
+  int jj = IRREG ? P.l4_off[b] : 0, t0 = 0, s0 = 0, sn = 0;
+  settle(jj, t0, s0, sn);
+  if (jj < jend && tid < min(ts, sn - t0)) {
+    const double* x = spos + 4 * (size_t)(s0 + t0 + tid);
+    form_scalars<T, IRREG>(x[0], x[1], x[2], cx, cy, cz, ihd, scs + 8 * tid);
+  }
+  while (jj < jend) {
+    const int tn = min(ts, sn - t0), cs0 = s0 + t0;
+    int njj = jj, nt0 = t0 + tn, ns0 = s0, nsn = sn;
+    settle(njj, nt0, ns0, nsn);
+    const int ntn = njj < jend ? min(ts, nsn - nt0) : 0;
+    __syncthreads();  // this tile's scalars written; the last tile's sums done with the table
+    // the next tile's positions, read while this tile's table is built
+    double nx = 0, ny = 0, nz = 0;
+    if (tid < ntn) {
+      const double* x = spos + 4 * (size_t)(ns0 + nt0 + tid);
+      nx = x[0];
+      ny = x[1];
+      nz = x[2];
+    }
+    // this tile's weights, one value per thread (8 ts <= 256), staged in LDS
+    // for the sums
+    T wreg = T(0);
+    if (tid < 8 * tn) wreg = (T)str[8 * (size_t)cs0 + tid];
+    // column tasks, m-major, in blocks of 64; block k of round r goes to wave
+    // (k - r - rot) mod nw: the wave with the shortest columns of one round
+    // takes the longest of the next, and the long low-m columns visit every
+    // wave (and SIMD) in turn from tile to tile
+    {
+      const int ntask = tn * (pt + 1), nw = (int)blockDim.x >> 6;
+      int vw = ((tid >> 6) + rot) % nw;
+      for (int r0 = 0; r0 < ntask; r0 += blockDim.x) {
+        const int task = r0 + 64 * vw + (tid & 63);
+        if (task < ntask) {
+          const int m = task / tn, s = task - m * tn;
+          form_column<T, IRREG>(facl, L, pt, m, (T)cmt[m], scs + 8 * s, tab + 2 * (size_t)s * HS);
         }
-        form_column<T, IRREG>(facl, L, pt, m, (T)cmt[m], ux, uy, uz, r2, ir2, ir, tab + 2 * (size_t)s * HS);
+        vw = vw + 1 == nw ? 0 : vw + 1;
       }
-      __syncthreads();
-      for (int s = 0; s < tn; s++) {
-        const double* w = str + 8 * (size_t)(s0 + t0 + s);
-        T wv[8];
+      rot++;
+    }
+    if (tid < 8 * tn) wsh[tid] = wreg;
+    __syncthreads();
+    if (tid < ntn) form_scalars<T, IRREG>(nx, ny, nz, cx, cy, cz, ihd, scs + 8 * tid);
+#pragma unroll 2
+    for (int s = 0; s < tn; s++) {
+      const T* wv = wsh + 8 * s;
 #pragma unroll
-        for (int e = 0; e < 8; e++) wv[e] = (T)w[e];
+      for (int q = 0; q < ITEMS; q++) {
+        const int h = tid + q * (int)blockDim.x;
+        if (h < H) {
+          const V2 hv = *reinterpret_cast<const V2*>(tab + 2 * ((size_t)s * HS + h));
+          const T hr = hv.x, hi = hv.y;
 #pragma unroll
-        for (int q = 0; q < ITEMS; q++) {
-          const int h = tid + q * (int)blockDim.x;
-          if (h < H) {
-            const T hr = tab[2 * ((size_t)s * HS + h)], hi = tab[2 * ((size_t)s * HS + h) + 1];
-#pragma unroll
-            for (int c = 0; c < 4; c++) {
-              if ((c == 0 && !CHG) || (c > 0 && !DIP)) continue;
-              const T wr = wv[2 * c], wi = wv[2 * c + 1];
-              T* A = &acc[q][4 * c];
-              A[0] = fma(wr, hr, fma(-wi, hi, A[0]));  // w H^{+a}
-              A[1] = fma(wr, hi, fma(wi, hr, A[1]));
-              A[2] = fma(wr, hr, fma(wi, hi, A[2]));   // w H^{-a} = w conj(H^{a})
-              A[3] = fma(wi, hr, fma(-wr, hi, A[3]));
-            }
+          for (int c = 0; c < 4; c++) {
+            if ((c == 0 && !CHG) || (c > 0 && !DIP)) continue;
+            const T wr = wv[2 * c], wi = wv[2 * c + 1];
+            T* A = &acc[q][4 * c];
+            A[0] = fma(wr, hr, fma(-wi, hi, A[0]));  // w H^{+a}
+            A[1] = fma(wr, hi, fma(wi, hr, A[1]));
+            A[2] = fma(wr, hr, fma(wi, hi, A[2]));   // w H^{-a} = w conj(H^{a})
+            A[3] = fma(wi, hr, fma(-wr, hi, A[3]));
           }
         }
       }
     }
+    jj = njj;
+    t0 = nt0;
+    s0 = ns0;
+    sn = nsn;
   }
   __syncthreads();
   // sums to LDS: S[c][eidx(n', m')] complex, c = q, v_x, v_y, v_z
