@@ -269,7 +269,7 @@ int lap_solve(Context& ctx, const LapCall& c) {
     const int HS = form2_row_stride<T>(H);
     int ts = (int)(65536 / (2 * sizeof(T) * (size_t)HS));
     ts = std::max(1, std::min(32, ts));
-    const size_t lds2 = form2_fac_bytes(p) + 2 * sizeof(T) * std::max((size_t)ts * HS, 4 * P2) + 16 * sizeof(T) * ts;
+    const size_t lds2 = form2_fac_bytes(p) + 2 * sizeof(T) * std::max((size_t)ts * HS, 4 * P2) + (kFormScStride + 8) * sizeof(T) * ts;
 #define FORM2(C, D, I)                                                                                 \
   if (f.ifcharge == C && f.ifdipole == D)                                                              \
     hipLaunchKernelGGL((k_form2<T, IR, C, D, I>), dim3(nbox_form), dim3(256), lds2, s, P, fac, spos, str64, ts, dst);

@@ -158,6 +158,10 @@ __host__ __device__ inline size_t form2_fac_bytes(int p) {
   return ((size_t)(FacLayout(p).AR + p + 3) * sizeof(double) + 15) / 16 * 16;
 }
 
+// LDS stride (T) of the per-source scalars: 8 used, padded so that the rows
+// of neighbouring sources start in different banks
+constexpr int kFormScStride = 10;
+
 // Per-source scalars of the column tasks, 8 T per source: the power base
 // w = -(ux + i uy) (times 1/|u|^2 for S), uz, the scale of the recurrence's
 // second term (|u|^2 for R, 1 for S), the scale of each new entry (1 for R,
@@ -182,6 +186,17 @@ __device__ __forceinline__ void form_scalars(double x, double y, double z, doubl
   sc[7] = T(0);
 }
 
+// One complex table entry as a single LDS store (16 bytes for double, 8 for
+// float; the k_form2 tile rows keep entries at that alignment)
+template <typename T>
+__device__ __forceinline__ void cst_vec(T* p, int k, C2<T> v) {
+  using V2 = typename std::conditional<std::is_same<T, double>::value, double2, float2>::type;
+  V2 t;
+  t.x = v.re;
+  t.y = v.im;
+  *reinterpret_cast<V2*>(p + 2 * k) = t;
+}
+
 // Column m of the regular (R) or irregular (S) harmonics of one source into
 // a half table, n = m..pt, from that source's scalars `sc` (form_scalars).
 // The diagonal entry R_m^m = cm w^m, S_m^m = ir cm (w ir2)^m is taken by
@@ -200,12 +215,12 @@ __device__ __forceinline__ void form_column(const double* facl, const FacLayout&
     base = cmul(base, base);
   }
   int k = hidx(m, m);
-  cst(out, k, r);
+  cst_vec(out, k, r);
   if (m + 1 > pt) return;
   const T g = (T)facl[L.M1 + m] * uz * csc;
   C2<T> r1 = {g * r.re, g * r.im};
   k += m + 1;  // hidx(m + 1, m)
-  cst(out, k, r1);
+  cst_vec(out, k, r1);
   if (m + 2 > pt) return;
   C2<T> r0 = r;
   k += m + 2;  // hidx(m + 2, m)
@@ -223,7 +238,7 @@ __device__ __forceinline__ void form_column(const double* facl, const FacLayout&
     const T a = cn * uz;
     cn += T(2);
     const C2<T> rn = {(a * r1.re - b * r0.re) * c, (a * r1.im - b * r0.im) * c};
-    cst(out, k, rn);
+    cst_vec(out, k, rn);
     r0 = r1;
     r1 = rn;
     k = kn;
@@ -265,7 +280,7 @@ __global__ __launch_bounds__(256) void k_form2(DevPlan P, const double* __restri
     for (int e = 0; e < 16; e++) acc[q][e] = T(0);
   // per-source scalars of the current tile, after the tile region
   T* scs = tab + 2 * (size_t)max(ts * HS, 4 * P2);
-  T* wsh = scs + 8 * ts;  // the tile's source weights
+  T* wsh = scs + kFormScStride * ts;  // the tile's source weights
   int rot = blockIdx.x;
   using V2 = typename std::conditional<std::is_same<T, double>::value, double2, float2>::type;
   // tile cursor over the sources of the list-4 boxes (P2L) or of box b (P2M):
@@ -283,7 +298,7 @@ __global__ __launch_bounds__(256) void k_form2(DevPlan P, const double* __restri
   settle(jj, t0, s0, sn);
   if (jj < jend && tid < min(ts, sn - t0)) {
     const double* x = spos + 4 * (size_t)(s0 + t0 + tid);
-    form_scalars<T, IRREG>(x[0], x[1], x[2], cx, cy, cz, ihd, scs + 8 * tid);
+    form_scalars<T, IRREG>(x[0], x[1], x[2], cx, cy, cz, ihd, scs + kFormScStride * tid);
   }
   while (jj < jend) {
     const int tn = min(ts, sn - t0), cs0 = s0 + t0;
@@ -314,7 +329,7 @@ __global__ __launch_bounds__(256) void k_form2(DevPlan P, const double* __restri
         const int task = r0 + 64 * vw + (tid & 63);
         if (task < ntask) {
           const int m = task / tn, s = task - m * tn;
-          form_column<T, IRREG>(facl, L, pt, m, (T)cmt[m], scs + 8 * s, tab + 2 * (size_t)s * HS);
+          form_column<T, IRREG>(facl, L, pt, m, (T)cmt[m], scs + kFormScStride * s, tab + 2 * (size_t)s * HS);
         }
         vw = vw + 1 == nw ? 0 : vw + 1;
       }
@@ -322,7 +337,7 @@ __global__ __launch_bounds__(256) void k_form2(DevPlan P, const double* __restri
     }
     if (tid < 8 * tn) wsh[tid] = wreg;
     __syncthreads();
-    if (tid < ntn) form_scalars<T, IRREG>(nx, ny, nz, cx, cy, cz, ihd, scs + 8 * tid);
+    if (tid < ntn) form_scalars<T, IRREG>(nx, ny, nz, cx, cy, cz, ihd, scs + kFormScStride * tid);
 #pragma unroll 2
     for (int s = 0; s < tn; s++) {
       const T* wv = wsh + 8 * s;
