@@ -298,6 +298,14 @@ __host__ __device__ inline size_t hform2_fac_bytes(int pmax) {
   return ((size_t)(FacLayout(pmax).AR + pmax + 3) * sizeof(double) + 15) / 16 * 16;
 }
 
+// One complex table entry as a single 16-byte LDS store
+__device__ __forceinline__ void cst_v2(double* p, int k, Cd v) {
+  double2 t;
+  t.x = v.re;
+  t.y = v.im;
+  *reinterpret_cast<double2*>(p + 2 * k) = t;
+}
+
 // Column m of Yhat_n^m(u), n = m..pt, of a unit vector into a half table:
 // Yhat_m^m = cm (-(ux + i uy))^m with the power taken by binary powers, then
 // the three-term recurrence in n of regular_column (with r^2 = 1).
@@ -308,16 +316,16 @@ __device__ __forceinline__ void hform_column(const double* facl, const FacLayout
     if (e & 1) r = cmul(r, base);
     base = cmul(base, base);
   }
-  cst(out, hidx(m, m), r);
+  cst_v2(out, hidx(m, m), r);
   if (m + 1 > pt) return;
   const double f1 = facl[L.M1 + m] * uz;
   Cd r1 = {f1 * r.re, f1 * r.im};
-  cst(out, hidx(m + 1, m), r1);
+  cst_v2(out, hidx(m + 1, m), r1);
   Cd r0 = r;
   for (int n = m + 2; n <= pt; n++) {
     const double a = (double)(2 * n - 1) * uz, b = facl[L.RA + hidx(n, m)], c = facl[L.RB + hidx(n, m)];
     const Cd rn = {(a * r1.re - b * r0.re) * c, (a * r1.im - b * r0.im) * c};
-    cst(out, hidx(n, m), rn);
+    cst_v2(out, hidx(n, m), rn);
     r0 = r1;
     r1 = rn;
   }
@@ -341,7 +349,11 @@ __global__ __launch_bounds__(256) void k_hform2(DevPlan P, HelmDev Hd, const dou
   const int G = H <= 64 * kHFormItems ? 4 : (H <= 128 * kHFormItems ? 2 : 1);
   const int gs = 256 / G, grp = tid / gs, lt = tid - grp * gs;
   // region = doubles of LDS after the factor head; every order fits 4 sources
-  const int ts = min(kHFormTile, region / (2 * (H + pt + 1)));
+  // tile row stride: H entries padded so that consecutive sources start one
+  // entry (4 banks) apart modulo the 32 LDS banks
+  int HS = H;
+  while ((HS * 4) % 32 != 4) HS++;
+  const int ts = min(kHFormTile, region / (2 * (HS + pt + 1)));
   FacLayout L(pmaxg);
   double* facl = reinterpret_cast<double*>(smem_raw);
   double* cm = facl + L.AR;
@@ -352,7 +364,7 @@ __global__ __launch_bounds__(256) void k_hform2(DevPlan P, HelmDev Hd, const dou
     for (int k = 1; k <= pt; k++) { c *= fac[L.MM + k]; cm[k] = c; }
   }
   double* tab = reinterpret_cast<double*>(smem_raw + hform2_fac_bytes(pmaxg));  // ts x H complex
-  double* fser = tab + 2 * (size_t)ts * H;                                      // ts x (pt+1) complex
+  double* fser = tab + 2 * (size_t)ts * HS;                                     // ts x (pt+1) complex
   const double sc = Hd.sclev[lev];
   const double lo = IRREG ? sc : 1.0 / sc, hi = IRREG ? 1.0 / sc : sc;
   const double cx = P.center[3 * b], cy = P.center[3 * b + 1], cz = P.center[3 * b + 2];
@@ -406,7 +418,7 @@ __global__ __launch_bounds__(256) void k_hform2(DevPlan P, HelmDev Hd, const dou
       if (m >= 0) {
         const double ir = rr > 0.0 ? 1.0 / rr : 0.0;
         const double ux = rr > 0.0 ? ux0 * ir : 0.0, uy = rr > 0.0 ? uy0 * ir : 0.0, uz = rr > 0.0 ? uz0 * ir : 1.0;
-        hform_column(facl, L, pt, m, cm[m], ux, uy, uz, tab + 2 * (size_t)s * H);
+        hform_column(facl, L, pt, m, cm[m], ux, uy, uz, tab + 2 * (size_t)s * HS);
       } else {
         const Cd zz = {Hd.kre * rr, Hd.kim * rr};
         Cd* fcol = reinterpret_cast<Cd*>(fser + 2 * (size_t)s * (pt + 1));
@@ -432,14 +444,15 @@ __global__ __launch_bounds__(256) void k_hform2(DevPlan P, HelmDev Hd, const dou
     // phase B: per-thread sums over the group's sources of the tile
     for (int s = grp; s < tn; s += G) {
       const double* w = str + 8 * (size_t)gidx[s];
-      const double* Y = tab + 2 * (size_t)s * H;
+      const double* Y = tab + 2 * (size_t)s * HS;
       const double* fr = fser + 2 * (size_t)s * (pt + 1);
 #pragma unroll
       for (int q = 0; q < kHFormItems; q++) {
         const int h = lt + q * gs;
         if (h < H) {
           const int n = hn[q];
-          const double yr = Y[2 * h], yi = Y[2 * h + 1];
+          const double2 yv = *reinterpret_cast<const double2*>(Y + 2 * h);
+          const double yr = yv.x, yi = yv.y;
           const double fre = fr[2 * n], fim = fr[2 * n + 1];
           const double bpr = fre * yr - fim * yi, bpi = fre * yi + fim * yr;   // f Y^a
           const double bmr = fre * yr + fim * yi, bmi = fim * yr - fre * yi;   // f conj(Y^a)
