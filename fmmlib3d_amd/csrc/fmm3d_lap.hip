@@ -258,8 +258,8 @@ int lap_solve(Context& ctx, const LapCall& c) {
   tm.mark(1);
   const double* fac = Tb.fac;
   // ---- STEP 1: P2M, STEP 2: P2L (list 4)
-  // k_form2 (per-thread sums) when its half table fits two (or four) entries
-  // per thread, else k_form (per-coefficient contraction)
+  // k_form2 (per-thread sums) when its half table fits one (or two) entries
+  // per thread of kFormThreads, else k_form (per-coefficient contraction)
   auto form = [&](auto irreg_tag, int nbox_form, T* dst) {
     constexpr bool IR = decltype(irreg_tag)::value;
     if (nbox_form == 0 || (!f.ifcharge && !f.ifdipole)) return;
@@ -272,10 +272,10 @@ int lap_solve(Context& ctx, const LapCall& c) {
     const size_t lds2 = form2_fac_bytes(p) + 2 * sizeof(T) * std::max((size_t)ts * HS, 4 * P2) + (kFormScStride + 8) * sizeof(T) * ts;
 #define FORM2(C, D, I)                                                                                 \
   if (f.ifcharge == C && f.ifdipole == D)                                                              \
-    hipLaunchKernelGGL((k_form2<T, IR, C, D, I>), dim3(nbox_form), dim3(256), lds2, s, P, fac, spos, str64, ts, dst);
-    if (H <= kFormItems * 256) {
+    hipLaunchKernelGGL((k_form2<T, IR, C, D, I>), dim3(nbox_form), dim3(kFormThreads), lds2, s, P, fac, spos, str64, ts, dst);
+    if (H <= kFormItems * kFormThreads) {
       FORM2(1, 0, kFormItems) FORM2(0, 1, kFormItems) FORM2(1, 1, kFormItems)
-    } else if (H <= 2 * kFormItems * 256 && lds2 <= max_lds_bytes(ctx)) {
+    } else if (H <= 2 * kFormItems * kFormThreads && lds2 <= max_lds_bytes(ctx)) {
       FORM2(1, 0, 2 * kFormItems) FORM2(0, 1, 2 * kFormItems) FORM2(1, 1, 2 * kFormItems)
     } else {
 #undef FORM2

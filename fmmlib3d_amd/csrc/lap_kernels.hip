@@ -138,7 +138,9 @@ __global__ void k_form(DevPlan P, const double* __restrict__ fac, const double* 
 // through the derivative factors and degree n-1 (P2M) or n+1 (P2L), with
 // (v_x -+ i v_y)/2 formed from S^{v_x}, S^{v_y} by linearity.
 // ---------------------------------------------------------------------------
-constexpr int kFormItems = 2;  // half-table entries per thread (256 threads); 4 for larger orders
+constexpr int kFormThreads = 512;  // threads of k_form2: the tile's LDS holds few sources, so twice the
+                                   // waves of 256-thread groups per CU hide the column latency
+constexpr int kFormItems = 1;      // half-table entries per thread; 2 for larger orders
 // LDS bytes k_form2 keeps ahead of the harmonic table, 16-aligned: the
 // recurrence factors (FacLayout MM..RB) and cm[m] = prod_{k<=m} MM[k],
 // m = 0..p+2
@@ -246,7 +248,7 @@ __device__ __forceinline__ void form_column(const double* facl, const FacLayout&
 }
 
 template <typename T, bool IRREG, bool CHG, bool DIP, int ITEMS>
-__global__ __launch_bounds__(256) void k_form2(DevPlan P, const double* __restrict__ fac,
+__global__ __launch_bounds__(kFormThreads) void k_form2(DevPlan P, const double* __restrict__ fac,
                                                const double* __restrict__ spos, const double* __restrict__ str,
                                                int ts, T* __restrict__ expn) {
   extern __shared__ unsigned char smem_raw[];
@@ -314,7 +316,8 @@ __global__ __launch_bounds__(256) void k_form2(DevPlan P, const double* __restri
       ny = x[1];
       nz = x[2];
     }
-    // this tile's weights, one value per thread (8 ts <= 256), staged in LDS
+    // this tile's weights, one value per thread (8 ts <= 256 <= kFormThreads),
+    // staged in LDS
     // for the sums
     T wreg = T(0);
     if (tid < 8 * tn) wreg = (T)str[8 * (size_t)cs0 + tid];
