@@ -258,10 +258,11 @@ int lap_solve(Context& ctx, const LapCall& c) {
   tm.mark(1);
   const double* fac = Tb.fac;
   // ---- STEP 1: P2M, STEP 2: P2L (list 4)
-  // k_form2 (per-thread sums) when its half table fits one (or two) entries
-  // per thread of kFormThreads, else k_form (per-coefficient contraction)
+  // k_form2 (per-thread sums) when its half table fits I1 (or 2 I1) entries
+  // per thread, else k_form (per-coefficient contraction)
   auto form = [&](auto irreg_tag, int nbox_form, T* dst) {
     constexpr bool IR = decltype(irreg_tag)::value;
+    constexpr int NT = form2_threads<IR>(), I1 = 512 / NT;
     if (nbox_form == 0 || (!f.ifcharge && !f.ifdipole)) return;
     const int pt = IR ? p + 1 : p;
     const int H = (pt + 1) * (pt + 2) / 2;
@@ -272,11 +273,11 @@ int lap_solve(Context& ctx, const LapCall& c) {
     const size_t lds2 = form2_fac_bytes(p) + 2 * sizeof(T) * std::max((size_t)ts * HS, 4 * P2) + (kFormScStride + 8) * sizeof(T) * ts;
 #define FORM2(C, D, I)                                                                                 \
   if (f.ifcharge == C && f.ifdipole == D)                                                              \
-    hipLaunchKernelGGL((k_form2<T, IR, C, D, I>), dim3(nbox_form), dim3(kFormThreads), lds2, s, P, fac, spos, str64, ts, dst);
-    if (H <= kFormItems * kFormThreads) {
-      FORM2(1, 0, kFormItems) FORM2(0, 1, kFormItems) FORM2(1, 1, kFormItems)
-    } else if (H <= 2 * kFormItems * kFormThreads && lds2 <= max_lds_bytes(ctx)) {
-      FORM2(1, 0, 2 * kFormItems) FORM2(0, 1, 2 * kFormItems) FORM2(1, 1, 2 * kFormItems)
+    hipLaunchKernelGGL((k_form2<T, IR, C, D, I>), dim3(nbox_form), dim3(NT), lds2, s, P, fac, spos, str64, ts, dst);
+    if (H <= I1 * NT) {
+      FORM2(1, 0, I1) FORM2(0, 1, I1) FORM2(1, 1, I1)
+    } else if (H <= 2 * I1 * NT && lds2 <= max_lds_bytes(ctx)) {
+      FORM2(1, 0, 2 * I1) FORM2(0, 1, 2 * I1) FORM2(1, 1, 2 * I1)
     } else {
 #undef FORM2
       int ts = (int)((65536 / sizeof(T) - 2 * K) / (2 * H + 11));

@@ -138,9 +138,15 @@ __global__ void k_form(DevPlan P, const double* __restrict__ fac, const double* 
 // through the derivative factors and degree n-1 (P2M) or n+1 (P2L), with
 // (v_x -+ i v_y)/2 formed from S^{v_x}, S^{v_y} by linearity.
 // ---------------------------------------------------------------------------
-constexpr int kFormThreads = 512;  // threads of k_form2: the tile's LDS holds few sources, so twice the
-                                   // waves of 256-thread groups per CU hide the column latency
-constexpr int kFormItems = 1;      // half-table entries per thread; 2 for larger orders
+// Threads of k_form2.  P2L: 512, since the tile's LDS admits two workgroups per
+// CU and twice the waves of 256-thread groups hide the column recurrence.
+// P2M: 256, since it runs beside the near field, where a 512-thread workgroup
+// waits for two near-field workgroups to leave a CU.  Each thread owns
+// 512 / threads half-table entries, twice that for larger orders.
+template <bool IRREG>
+__host__ __device__ constexpr int form2_threads() {
+  return IRREG ? 512 : 256;
+}
 // LDS bytes k_form2 keeps ahead of the harmonic table, 16-aligned: the
 // recurrence factors (FacLayout MM..RB) and cm[m] = prod_{k<=m} MM[k],
 // m = 0..p+2
@@ -248,7 +254,7 @@ __device__ __forceinline__ void form_column(const double* facl, const FacLayout&
 }
 
 template <typename T, bool IRREG, bool CHG, bool DIP, int ITEMS>
-__global__ __launch_bounds__(kFormThreads) void k_form2(DevPlan P, const double* __restrict__ fac,
+__global__ __launch_bounds__(form2_threads<IRREG>()) void k_form2(DevPlan P, const double* __restrict__ fac,
                                                const double* __restrict__ spos, const double* __restrict__ str,
                                                int ts, T* __restrict__ expn) {
   extern __shared__ unsigned char smem_raw[];
@@ -316,7 +322,7 @@ __global__ __launch_bounds__(kFormThreads) void k_form2(DevPlan P, const double*
       ny = x[1];
       nz = x[2];
     }
-    // this tile's weights, one value per thread (8 ts <= 256 <= kFormThreads),
+    // this tile's weights, one value per thread (8 ts <= 256 <= blockDim.x),
     // staged in LDS
     // for the sums
     T wreg = T(0);
