@@ -301,3 +301,21 @@ def test_gpu_headline_config_accuracy_and_linearity():
     b = F.lfmm3dparttarg(2, src, q + q2, ds, dv, trg, ifpot=0, iffld=0)
     assert rel(pt + a[3], b[3]) < 1e-12
     assert rel(ft + a[4], b[4]) < 1e-12
+
+
+@pytest.mark.slow
+@pytest.mark.parametrize("dist", ["clusters", "plummer"])
+def test_gpu_nonuniform_1M_accuracy(dist):
+    """1M clustered or Plummer points, charges + dipoles, iprec=2, FP64: the
+    list-4 (P2L) and list-3 (M2P) paths carry much of the work here; potential
+    and field at sampled sources against the FP64 direct sum."""
+    rng = np.random.default_rng(11)
+    n = 1_000_000
+    src = points(rng, n, dist)
+    q, ds, dv = strengths(rng, n, True, True)
+    ier, pot, fld = F.lfmm3dpartself(2, src, q, ds, dv)
+    assert ier == 0
+    idx = rng.choice(n, 200, replace=False).astype(np.int32)
+    pd, fd = ob.orc_direct_sample(src, src[idx], idx, q, ds, dv)
+    assert rel(pot[idx], pd) < IPREC_EPS[2]
+    assert rel(fld[idx], fd) < 10 * IPREC_EPS[2]
