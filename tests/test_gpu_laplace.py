@@ -58,7 +58,7 @@ CASES = [
     (6000, 0, 2, "sphere", 1, 1),
     (8000, 4000, 1, "clusters", 1, 1),
     (3000, 2000, 3, "clusters", 0, 1),
-    (6000, 2000, 3, "plummer", 1, 1),    # p = 39: P2M/P2L with four entries per thread
+    (6000, 2000, 3, "plummer", 1, 1),    # p = 39: P2M/P2L with twice the entries per thread
     (5000, 0, 3, "shell", 1, 0),
     (4000, 4000, 0, "plummer", 1, 1),
     (3000, 1000, -1, "shell", 1, 0),
