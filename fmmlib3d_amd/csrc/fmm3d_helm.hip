@@ -211,7 +211,7 @@ int helm_solve(Context& ctx, const LapCall& c) {
   // its tile from what is left after the factor head)
   const bool form2 = Hh <= 256 * kHFormItems && K <= 256 * kHFormOut;
   const size_t lds_form2 = std::max<size_t>((size_t)64 * 1024,
-                                            hform2_fac_bytes(pmax) + sizeof(double) * 2 * ((size_t)Hh + Fn) * 4);
+                                            hform2_fac_bytes(pmax) + sizeof(double) * (2 * ((size_t)Hh + Fn) + 8) * 4);
   const int region2 = (int)((lds_form2 - hform2_fac_bytes(pmax)) / sizeof(double));
   if (form2 && lds_form2 > ldsmax) return 16;
   const size_t lds_ps = sizeof(double) * 4 * (size_t)K;

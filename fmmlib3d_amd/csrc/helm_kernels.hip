@@ -353,7 +353,7 @@ __global__ __launch_bounds__(256) void k_hform2(DevPlan P, HelmDev Hd, const dou
   // entry (4 banks) apart modulo the 32 LDS banks
   int HS = H;
   while ((HS * 4) % 32 != 4) HS++;
-  const int ts = min(kHFormTile, region / (2 * (HS + pt + 1)));
+  const int ts = min(kHFormTile, region / (2 * (HS + pt + 1) + 8));
   FacLayout L(pmaxg);
   double* facl = reinterpret_cast<double*>(smem_raw);
   double* cm = facl + L.AR;
@@ -365,6 +365,7 @@ __global__ __launch_bounds__(256) void k_hform2(DevPlan P, HelmDev Hd, const dou
   }
   double* tab = reinterpret_cast<double*>(smem_raw + hform2_fac_bytes(pmaxg));  // ts x H complex
   double* fser = tab + 2 * (size_t)ts * HS;                                     // ts x (pt+1) complex
+  double* wsh = fser + 2 * (size_t)ts * (pt + 1);                               // ts x 8 source weights
   const double sc = Hd.sclev[lev];
   const double lo = IRREG ? sc : 1.0 / sc, hi = IRREG ? 1.0 / sc : sc;
   const double cx = P.center[3 * b], cy = P.center[3 * b + 1], cz = P.center[3 * b + 2];
@@ -408,6 +409,11 @@ __global__ __launch_bounds__(256) void k_hform2(DevPlan P, HelmDev Hd, const dou
     // phase A: the columns m of Yhat on the unit vector (task m * tn + s,
     // so a wave holds few orders and its loop lengths are close), then the
     // radial series (tasks after them, in waves of their own)
+    // the tile's weights (8 per source, two values per thread), read while
+    // the tables are built and staged in LDS for the sums
+    double wr0 = 0.0, wr1 = 0.0;
+    if (tid < 8 * tn) wr0 = str[8 * (size_t)gidx[tid >> 3] + (tid & 7)];
+    if (tid + 256 < 8 * tn) wr1 = str[8 * (size_t)gidx[(tid + 256) >> 3] + (tid & 7)];
     const int ncolt = (pt + 1) * tn;
     for (int task = tid; task < ncolt + tn; task += 256) {
       const int m = task < ncolt ? task / tn : -1;
@@ -440,10 +446,12 @@ __global__ __launch_bounds__(256) void k_hform2(DevPlan P, HelmDev Hd, const dou
         }
       }
     }
+    if (tid < 8 * tn) wsh[tid] = wr0;
+    if (tid + 256 < 8 * tn) wsh[tid + 256] = wr1;
     __syncthreads();
     // phase B: per-thread sums over the group's sources of the tile
     for (int s = grp; s < tn; s += G) {
-      const double* w = str + 8 * (size_t)gidx[s];
+      const double* w = wsh + 8 * (size_t)s;
       const double* Y = tab + 2 * (size_t)s * HS;
       const double* fr = fser + 2 * (size_t)s * (pt + 1);
 #pragma unroll
