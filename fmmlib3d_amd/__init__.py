@@ -30,7 +30,7 @@ PREC_FP32 = 3     # FP32 near and far field
 EXPORTED = (
     "fmm3d_lfmm3dparttarg", "fmm3d_lfmm3dpartself", "fmm3d_l3dpartdirect",
     "fmm3d_lfmm3dparttarg_ex", "fmm3d_lfmm3dparttarg_dev", "fmm3d_last_timings",
-    "fmm3d_version", "fmm3d_d3tstrcr",
+    "fmm3d_last_precision", "fmm3d_version", "fmm3d_d3tstrcr",
     "fmm3d_hfmm3dparttarg", "fmm3d_hfmm3dpartself", "fmm3d_h3dpartdirect",
     "fmm3d_hfmm3dparttarg_ex", "fmm3d_hfmm3dparttarg_dev",
     "fmm3d_server_create", "fmm3d_server_submit", "fmm3d_server_wait", "fmm3d_server_stats",
@@ -93,12 +93,20 @@ def load():
         lib.fmm3d_server_destroy.restype = None
         lib.fmm3d_last_timings.argtypes = [vp, i]
         lib.fmm3d_last_timings.restype = None
+        lib.fmm3d_last_precision.argtypes = []
+        lib.fmm3d_last_precision.restype = i
         _lib = lib
     return _lib
 
 
 def version():
     return load().fmm3d_version().decode()
+
+
+def last_precision():
+    """The PREC_* mode the last FMM call of this thread ran in (PREC_FP64
+    when the requested mode fell back to FP64 at iprec >= 3)."""
+    return int(load().fmm3d_last_precision())
 
 
 def _ptr(a):

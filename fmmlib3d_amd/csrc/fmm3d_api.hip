@@ -60,11 +60,35 @@ int check_args(const double* zk, int prec_mode, int iprec, int ns, int nt) {
   return 0;
 }
 
+// 128 when an array the flags ask for is NULL (the reference would fault;
+// on the device path a NULL array would fault the GPU): source with ns > 0,
+// charge with ifcharge = 1, dipstr/dipvec with ifdipole = 1, target with
+// nt > 0, and each output whose flag is 1.
+int check_ptrs(int ns, const double* source, int ifcharge, const double* charge, int ifdipole,
+               const double* dipstr, const double* dipvec, int ifpot, const double* pot, int iffld,
+               const double* fld, int nt, const double* target, int ifpottarg, const double* pottarg,
+               int iffldtarg, const double* fldtarg) {
+  if (ns > 0 && !source) return 128;
+  if (ifcharge == 1 && ns > 0 && !charge) return 128;
+  if (ifdipole == 1 && ns > 0 && (!dipstr || !dipvec)) return 128;
+  if (ifpot == 1 && ns > 0 && !pot) return 128;
+  if (iffld == 1 && ns > 0 && !fld) return 128;
+  if (nt > 0) {
+    if (!target) return 128;
+    if (ifpottarg == 1 && !pottarg) return 128;
+    if (iffldtarg == 1 && !fldtarg) return 128;
+  }
+  return 0;
+}
+
 int fmm_host(const double* zk, int prec_mode, int iprec, int ns, const double* source, int ifcharge,
              const double* charge, int ifdipole, const double* dipstr, const double* dipvec,
              int ifpot, double* pot, int iffld, double* fld, int nt, const double* target,
              int ifpottarg, double* pottarg, int iffldtarg, double* fldtarg) {
   if (int e = check_args(zk, prec_mode, iprec, ns, nt)) return e;
+  if (int e = check_ptrs(ns, source, ifcharge, charge, ifdipole, dipstr, dipvec, ifpot, pot, iffld, fld, nt,
+                         target, ifpottarg, pottarg, iffldtarg, fldtarg))
+    return e;
   Context& ctx = default_context();
   std::lock_guard<std::mutex> lock(ctx.mu);
   return guarded([&] {
@@ -136,6 +160,9 @@ int fmm_dev(const double* zk, int prec_mode, int iprec, int ns, const double* so
             const double* target_dev, int ifpottarg, double* pottarg_dev, int iffldtarg,
             double* fldtarg_dev, void* stream) {
   if (int e = check_args(zk, prec_mode, iprec, ns, nt)) return e;
+  if (int e = check_ptrs(ns, source_dev, ifcharge, charge_dev, ifdipole, dipstr_dev, dipvec_dev, ifpot, pot_dev,
+                         iffld, fld_dev, nt, target_dev, ifpottarg, pottarg_dev, iffldtarg, fldtarg_dev))
+    return e;
   Context& ctx = default_context();
   std::lock_guard<std::mutex> lock(ctx.mu);
   hipStream_t saved = ctx.stream;
@@ -233,6 +260,9 @@ void direct_host(const double* zk, int nsource, const double* source, int ifchar
                  int iffld, double* fld, int ntarget, const double* target, int ifpottarg,
                  double* pottarg, int iffldtarg, double* fldtarg) {
   if (nsource < 0 || ntarget < 0) return;
+  if (check_ptrs(nsource, source, ifcharge, charge, ifdipole, dipstr, dipvec, ifpot, pot, iffld, fld, ntarget,
+                 target, ifpottarg, pottarg, iffldtarg, fldtarg))
+    return;  // no error channel: a call with a missing array computes nothing
   Context& ctx = default_context();
   std::lock_guard<std::mutex> lock(ctx.mu);
   (void)guarded([&] {
@@ -288,7 +318,9 @@ void fmm3d_last_timings(double* ms, int n) {
   for (int i = 0; i < n; i++) ms[i] = i < (int)t.size() ? t[i] : 0.0;
 }
 
-const char* fmm3d_version(void) { return "fmm3d-mi355x 0.1 (gfx950)"; }
+int fmm3d_last_precision(void) { return last_precision(); }
+
+const char* fmm3d_version(void) { return "fmm3d-mi355x 0.2 (gfx950)"; }
 
 }  // extern "C"
 

@@ -519,6 +519,8 @@ int helm_solve(Context& ctx, const LapCall& c) {
 }
 
 int helm_solve_mode(Context& ctx, const LapCall& c, int mode) {
+  mode = effective_precision(c.iprec, mode);
+  last_precision() = mode;
   switch (mode) {
     case 0: return helm_solve<double>(ctx, c);
     case 1: return helm_solve<float>(ctx, c);

@@ -17,6 +17,18 @@ struct LapCall {
   double zk_re = 0.0, zk_im = 0.0;  // Helmholtz wavenumber (unused for Laplace)
 };
 
+// The precision mode a call runs in: the requested FMM3D_PREC_* mode while
+// FP32 can meet the tolerance of iprec, FP64 throughout at iprec >= 3 (a
+// tolerance of 0.5e-9 or tighter, src/lfmm3dpart.f:277-285, which FP32
+// pair arithmetic and FP32 translations cannot reach).
+inline int effective_precision(int iprec, int mode) { return iprec >= 3 ? 0 : mode; }
+
+// the mode the last FMM call of this thread ran in
+inline int& last_precision() {
+  static thread_local int m = 0;
+  return m;
+}
+
 // T: far-field arithmetic and expansion storage; TN: near-field arithmetic
 template <typename T, typename TN>
 int lap_solve(Context& ctx, const LapCall& c);

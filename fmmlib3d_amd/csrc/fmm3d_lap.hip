@@ -274,15 +274,20 @@ int lap_solve(Context& ctx, const LapCall& c) {
 #define FORM2(C, D, I)                                                                                 \
   if (f.ifcharge == C && f.ifdipole == D)                                                              \
     hipLaunchKernelGGL((k_form2<T, IR, C, D, I>), dim3(nbox_form), dim3(NT), lds2, s, P, fac, spos, str64, ts, dst);
-    if (H <= I1 * NT) {
+    const bool lds2_fits = lds2 <= max_lds_bytes(ctx);
+    if (H <= I1 * NT && lds2_fits) {
       FORM2(1, 0, I1) FORM2(0, 1, I1) FORM2(1, 1, I1)
-    } else if (H <= 2 * I1 * NT && lds2 <= max_lds_bytes(ctx)) {
+    } else if (H <= 2 * I1 * NT && lds2_fits) {
       FORM2(1, 0, 2 * I1) FORM2(0, 1, 2 * I1) FORM2(1, 1, 2 * I1)
     } else {
 #undef FORM2
-      int ts = (int)((65536 / sizeof(T) - 2 * K) / (2 * H + 11));
-      ts = std::max(1, std::min(32, ts));
-      const size_t lds = sizeof(T) * (2 * K + (size_t)ts * (2 * H + 11));
+      // tile of ts sources: 2K accumulators + ts (2H + 11) table, positions
+      // and strengths, sized in signed arithmetic against the device limit
+      const long avail = (long)(max_lds_bytes(ctx) / sizeof(T)) - 2L * K;
+      const long ts1 = avail / (2L * H + 11);
+      if (ts1 < 1) throw AllocError("P2M/P2L tile does not fit LDS", 16);
+      const int ts = (int)std::min<long>(32, ts1);
+      const size_t lds = sizeof(T) * (2 * (size_t)K + (size_t)ts * (2 * H + 11));
       hipLaunchKernelGGL((k_form<T, IR>), dim3(nbox_form), dim3(256), lds, s, P, fac, spos, str64,
                          f.ifcharge, f.ifdipole, ts, dst);
     }
@@ -551,6 +556,8 @@ int lap_direct(Context& ctx, const LapCall& c) {
 
 namespace fmm3d {
 int lap_solve_mode(Context& ctx, const LapCall& c, int mode) {
+  mode = effective_precision(c.iprec, mode);
+  last_precision() = mode;
   switch (mode) {
     case 0: return lap_solve<double, double>(ctx, c);
     case 1: return lap_solve<double, float>(ctx, c);

@@ -1187,9 +1187,29 @@ struct HPairAcc {
     // FP32: one Newton step on the hardware estimate (FP64 rsqrt_t is refined already)
     if (sizeof(TN) == sizeof(float)) ir = ir * (TN(1.5) - TN(0.5) * r2 * ir * ir);
     if (mask) ir = TN(0);                            // a source never acts on itself
-    const TN r = r2 * ir;
-    TN s, c;
-    sincos_t<TN>(kre * r, &s, &c);
+    TN r, s, c;
+    if (sizeof(TN) == sizeof(float)) {
+      // The phase k r needs an absolute accuracy that FP32 cannot give once
+      // k r is large (its rounding is k r 2^-24), so r comes from one FP64
+      // Newton step on r^2 and the phase is reduced to [-pi, pi] in FP64,
+      // then split into an FP32 head and tail: sin/cos of the head, first
+      // order in the tail.  (Masked pair: ir = 0 gives r = 0.)
+      const double ird = (double)ir;
+      const double r0 = r2d * ird;
+      const double rd = fma(0.5 * ird, fma(-r0, r0, r2d), r0);
+      const double th = kre_d * rd;
+      const double nrev = rint(th * 0.15915494309189535);     // th / (2 pi)
+      const double thr = fma(-nrev, 2.4492935982947064e-16, fma(-nrev, 6.283185307179586, th));
+      const TN hi = (TN)thr, lo = (TN)(thr - (double)hi);
+      TN s0, c0;
+      sincos_t<TN>(hi, &s0, &c0);
+      s = fma(lo, c0, s0);
+      c = fma(-lo, s0, c0);
+      r = (TN)rd;
+    } else {
+      r = r2 * ir;
+      sincos_t<TN>(kre * r, &s, &c);
+    }
     const TN damp = kim == TN(0) ? TN(1) : exp_t<TN>(-kim * r);
     const TN gr = c * damp * ir, gi = s * damp * ir;   // G = e^{ikr}/r
     const TN ir2 = ir * ir;

@@ -38,6 +38,7 @@ struct Request {
   std::vector<double> src, q, ds, dv, trg;        // inputs (copied at submission)
   std::vector<double> pot, fld, pt, ft;          // outputs
   size_t bytes = 0;                              // device-memory estimate
+  int bypassed = 0;                              // times a later request was admitted first
   int ier = -1;
   bool done = false;
   Clock::time_point t_submit, t_start, t_end;
@@ -66,6 +67,9 @@ struct Worker {
   std::thread th;
 };
 
+// admissions of later requests before the queue head is served first
+constexpr int kMaxBypass = 8;
+
 }  // namespace
 
 struct Server {
@@ -85,17 +89,30 @@ struct Server {
       std::shared_ptr<Request> r;
       {
         std::unique_lock<std::mutex> lk(mu);
-        // admit the oldest request when its estimate fits what is left of the
-        // budget (a request larger than the whole budget runs alone)
+        // admit the oldest queued request whose estimate fits what is left of
+        // the budget (backfill: smaller requests behind a large one start
+        // when it does not fit); a request larger than the whole budget runs
+        // alone; once the head has been passed over kMaxBypass times, only
+        // the head is admitted, so a large request is not starved
+        auto pick = [&]() -> std::deque<std::shared_ptr<Request>>::iterator {
+          if (queue.empty()) return queue.end();
+          const auto head = queue.begin();
+          if (in_use + (*head)->bytes <= budget || in_use == 0) return head;
+          if ((*head)->bypassed >= kMaxBypass) return queue.end();
+          for (auto it = std::next(head); it != queue.end(); ++it)
+            if (in_use + (*it)->bytes <= budget) return it;
+          return queue.end();
+        };
+        std::deque<std::shared_ptr<Request>>::iterator sel;
         cv_work.wait(lk, [&] {
-          if (stop) return true;
-          if (queue.empty()) return false;
-          const size_t b = queue.front()->bytes;
-          return in_use + b <= budget || in_use == 0;
+          if (stop && queue.empty()) return true;
+          sel = pick();
+          return sel != queue.end();
         });
         if (stop && queue.empty()) return;
-        r = queue.front();
-        queue.pop_front();
+        if (sel != queue.begin()) queue.front()->bypassed++;
+        r = *sel;
+        queue.erase(sel);
         in_use += r->bytes;
         busy++;
         r->t_start = Clock::now();
@@ -213,6 +230,9 @@ long long fmm3d_server_submit(void* server, int kernel, int prec_mode, int iprec
   if (iprec < -2 || iprec > 6) return -32;
   if (kernel == 1 && (!zk || (zk[0] == 0.0 && zk[1] == 0.0) || (prec_mode != 0 && prec_mode != 1))) return -128;
   if (kernel == 0 && (prec_mode < 0 || prec_mode > 3)) return -128;
+  if (!source || (ifcharge == 1 && !charge) || (ifdipole == 1 && (!dipstr || !dipvec)) ||
+      (ntarget > 0 && !target))
+    return -128;
   auto r = std::make_shared<fmm3d::Request>();
   r->kernel = kernel; r->prec = prec_mode; r->iprec = iprec;
   if (zk) { r->zk[0] = zk[0]; r->zk[1] = zk[1]; }
@@ -252,7 +272,8 @@ int fmm3d_server_wait(void* server, long long id, int block, double* pot, double
       if (!block) return 0;
       S->cv_done.wait(lk, [&] { return r->done; });
     }
-    S->all.erase(it);
+    // another waiter on the same id may have released it while this one slept
+    if (S->all.erase(id) == 0) return -1;
   }
   auto cp = [](double* dst, const std::vector<double>& v) {
     if (dst && !v.empty()) std::memcpy(dst, v.data(), v.size() * sizeof(double));

@@ -10,7 +10,9 @@
  *    (x,y,z components of a point consecutive);
  *  - flags are 0/1 exactly as in the reference; outputs are written only when
  *    their flag is 1 and are fully overwritten (not accumulated);
- *  - arrays whose flag is 0 may be NULL;
+ *  - arrays whose flag is 0 may be NULL; a NULL array that the flags ask
+ *    for (source, charge with ifcharge=1, dipstr/dipvec with ifdipole=1,
+ *    target with ntarget>0, an output whose flag is 1) returns 128;
  *  - the return value is the reference's `ier`: 0 ok, 4 tree failure (also
  *    returned for nsource < 1, as the reference does through d3tstrcr's
  *    ier=128 and ten retries), 8 bulk workspace, 16 expansion workspace.
@@ -27,7 +29,10 @@ extern "C" {
 #endif
 
 /* Precision modes for the *_ex / *_dev entry points (a bit mask).  Point
- * positions, offsets and output accumulation stay FP64 in every mode. */
+ * positions, offsets and output accumulation stay FP64 in every mode.  A
+ * mode is honoured while FP32 can meet the tolerance of iprec: at iprec >= 3
+ * (0.5e-9 and tighter, src/lfmm3dpart.f:277-285) every mode runs FP64
+ * throughout; fmm3d_last_precision() tells which mode a call ran in. */
 #define FMM3D_PREC_FP64 0      /* every stage in FP64 (the default)                 */
 #define FMM3D_PREC_MIXED 1     /* FP32 near-field pair arithmetic, FP64 far field   */
 #define FMM3D_PREC_FAR32 2     /* FP32 expansions and translations, FP64 near field */
@@ -67,7 +72,9 @@ int fmm3d_lfmm3dparttarg_ex(int prec_mode, int iprec, int nsource, const double 
 /* Device-resident variant of lfmm3dparttarg: every array is a device pointer
  * in the same layout; `stream` is a hipStream_t (NULL = default stream).
  * The call returns when the results are complete on the device.  The octree
- * is still built on the host (the coordinates are copied back for it).
+ * is built on the device from the device coordinates (only per-box
+ * histograms come back to the host, which decides the boxes and builds the
+ * interaction lists); the coordinates themselves never leave HBM.
  * Extension with no reference counterpart, for callers whose data already
  * lives in HBM (benchmarks, the serving loop). */
 int fmm3d_lfmm3dparttarg_dev(int prec_mode, int iprec, int nsource, const double *source_dev,
@@ -141,9 +148,11 @@ int fmm3d_d3tstrcr(int nsource, const double *source, int ntarget, const double 
  * the drop-in contract).  A persistent loop that runs several evaluations at
  * once on the GPU: `nworkers` slots, each with its own HIP stream and
  * device arena (trees, expansions and tables stay resident between
- * requests); a queued request starts when a slot is free and its
- * device-memory estimate fits what is left of `mem_budget_bytes`.  Inputs are
- * copied at submission.  All functions are thread safe. */
+ * requests); when a slot is free, the oldest queued request whose
+ * device-memory estimate fits what is left of `mem_budget_bytes` starts
+ * (smaller requests may pass a large one that does not fit yet, at most 8
+ * times, after which the large one is served first).  Inputs are copied at
+ * submission.  All functions are thread safe. */
 void *fmm3d_server_create(int nworkers, long long mem_budget_bytes);
 /* kernel 0 = Laplace (zk ignored), 1 = Helmholtz (zk = (re, im)); other
  * arguments as fmm3d_lfmm3dparttarg_ex.  Returns a request id >= 0, or
@@ -173,6 +182,11 @@ void fmm3d_server_destroy(void *server);
  * time of the call.
  * Mirrors the reference's timeinfo(1..8) (src/lfmm3dpart.f:1155). */
 void fmm3d_last_timings(double *ms, int n);
+
+/* The FMM3D_PREC_* mode the last FMM call of the calling thread ran in
+ * (FMM3D_PREC_FP64 when the requested mode fell back at iprec >= 3).
+ * Extension with no reference counterpart. */
+int fmm3d_last_precision(void);
 
 /* Library version string. */
 const char *fmm3d_version(void);

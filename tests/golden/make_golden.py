@@ -5,7 +5,7 @@ The reference Fortran (fmmlib3d, compiled from its own sources by
 seeded inputs; inputs and outputs are stored as .npz (no pickles).  Run in a
 container where the reference sources are present:
 
-    make -C oracle ref && python tests/golden/make_golden.py
+    make -C oracle ref && python tests/golden/make_golden.py [case names]
 """
 import ctypes
 import os
@@ -44,6 +44,10 @@ FMM_CASES = [
     ("lap_targ_iprecm1_clusters", 105, -1, 1000, 1000, "clusters", 1, 1),
     ("lap_targ_rootleaf", 106, 1, 120, 30, "cube", 1, 1),
     ("lap_targ_iprecm2_plane", 107, -2, 900, 300, "plane", 1, 1),
+    # p = 51 and 64 (nbox 1200, 1400): enough points for leaves at level 2,
+    # so list 2 and the far field exist
+    ("lap_targ_iprec4_cube", 108, 4, 12000, 2000, "cube", 1, 1),
+    ("lap_self_iprec5_cube", 109, 5, 14000, 0, "cube", 1, 0),
 ]
 
 
@@ -173,6 +177,9 @@ HELM_CASES = [
     ("helm_targ_iprec0_cube_dip", 504, 0, 12.0, 1500, 700, "cube", 0, 1),
     ("helm_self_iprec3_cube", 505, 3, 2.0, 1800, 0, "cube", 1, 1),
     ("helm_targ_rootleaf", 506, 1, 1.0, 100, 40, "cube", 1, 1),
+    # zk = 30 at iprec 3: orders of 30 and more at levels 2-3, where the
+    # reference switches to rotations by projection (h3dtrans.f:318)
+    ("helm_targ_iprec3_zk30", 507, 3, 30.0, 8000, 1000, "cube", 1, 1),
 ]
 
 
@@ -300,6 +307,16 @@ def make_helm_orders():
 
 
 def main():
+    only = set(sys.argv[1:])   # names of the FMM cases to (re)write; none = everything
+    if only:
+        for c in FMM_CASES:
+            if c[0] in only:
+                make_fmm(*c)
+        for c in HELM_CASES:
+            if c[0] in only:
+                make_helm(*c)
+        print("fixtures written to", HERE)
+        return
     for c in FMM_CASES:
         make_fmm(*c)
     make_direct()

@@ -183,18 +183,25 @@ def test_gpu_helm_sphere_500k_accuracy():
     assert rel(pot + a[1], b[1]) < 1e-12
 
 
-def test_gpu_helm_fp32_near_field_within_tolerance(rng):
-    """FMM3D_PREC_MIXED: FP32 pair arithmetic in the near field, FP64
-    translations; against the FP64 direct sum at the iprec tolerance (never
-    tighter than 1e-6) for the potential and a digit less for the field."""
-    for iprec, dist, zk in ((1, "cube", 6.0), (2, "sphere", 20.0), (2, "clusters", 2.0 + 0.5j)):
-        src = points(rng, 20000, dist)
-        trg = points(rng, 4000, dist)
-        q, ds, dv = strengths(rng, 20000, True, True)
-        g = F.hfmm3dparttarg(iprec, zk, src, q, ds, dv, trg, precision=F.PREC_MIXED)
-        assert g[0] == 0
-        idx = rng.choice(4000, 300, replace=False)
-        pd, fd = ob.orc_direct_sample(src, trg[idx], -np.ones(300, np.int32), q, ds, dv, zk=zk)
-        tol = max(IPREC_EPS[iprec], 1e-6)
-        assert rel(g[3][idx], pd) < tol
-        assert rel(g[4][idx], fd) < 10 * tol
+HPREC_CASES = [(1, "cube", 6.0), (1, "plummer", 25.0), (2, "sphere", 20.0), (2, "clusters", 2.0 + 0.5j),
+               (2, "plummer", 30.0), (2, "plummer", 4.0), (3, "cube", 10.0), (3, "plummer", 25.0)]
+
+
+@pytest.mark.parametrize("iprec,dist,zk", HPREC_CASES)
+def test_gpu_helm_fp32_near_field_within_tolerance(rng, iprec, dist, zk):
+    """FMM3D_PREC_MIXED: FP32 pair arithmetic in the near field (phase k r
+    reduced in FP64), FP64 translations; against the FP64 direct sum at the
+    tolerance iprec names for the potential and a digit less for the field.
+    At iprec 3 the library runs FP64 throughout."""
+    src = points(rng, 20000, dist)
+    trg = points(rng, 4000, dist)
+    q, ds, dv = strengths(rng, 20000, True, True)
+    g = F.hfmm3dparttarg(iprec, zk, src, q, ds, dv, trg, precision=F.PREC_MIXED)
+    assert g[0] == 0
+    assert F.last_precision() == (F.PREC_FP64 if iprec >= 3 else F.PREC_MIXED)
+    idx = rng.choice(4000, 300, replace=False)
+    pd, fd = ob.orc_direct_sample(src, trg[idx], -np.ones(300, np.int32), q, ds, dv, zk=zk)
+    ep, ef = rel(g[3][idx], pd), rel(g[4][idx], fd)
+    print("helm iprec %d %s zk %s: pot %.2e fld %.2e" % (iprec, dist, zk, ep, ef))
+    assert ep < IPREC_EPS[iprec]
+    assert ef < 10 * IPREC_EPS[iprec]

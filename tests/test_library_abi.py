@@ -32,6 +32,45 @@ def test_library_exports_every_declared_symbol():
     assert set(F.EXPORTED) == names
 
 
+def test_null_arrays_rejected_before_device_use():
+    """An array the flags ask for that is NULL returns 128 (host and device
+    entry points; the direct sums compute nothing) before any device call."""
+    lib = F.load()
+    src = np.zeros((4, 3))
+    q = np.ones(4, np.complex128)
+    out = np.zeros(4, np.complex128)
+    fo = np.zeros((4, 3), np.complex128)
+    p = F._ptr
+    # source NULL, charge NULL with ifcharge=1, dipoles NULL with ifdipole=1
+    assert lib.fmm3d_lfmm3dparttarg(1, 4, None, 1, p(q), 0, None, None, 1, p(out), 0, None,
+                                    0, None, 0, None, 0, None) == 128
+    assert lib.fmm3d_lfmm3dparttarg(1, 4, p(src), 1, None, 0, None, None, 1, p(out), 0, None,
+                                    0, None, 0, None, 0, None) == 128
+    assert lib.fmm3d_lfmm3dparttarg(1, 4, p(src), 0, None, 1, p(q), None, 1, p(out), 0, None,
+                                    0, None, 0, None, 0, None) == 128
+    # outputs NULL with their flag set, target NULL with ntarget > 0
+    assert lib.fmm3d_lfmm3dparttarg(1, 4, p(src), 1, p(q), 0, None, None, 1, None, 0, None,
+                                    0, None, 0, None, 0, None) == 128
+    assert lib.fmm3d_lfmm3dparttarg(1, 4, p(src), 1, p(q), 0, None, None, 0, None, 1, None,
+                                    0, None, 0, None, 0, None) == 128
+    assert lib.fmm3d_lfmm3dparttarg(1, 4, p(src), 1, p(q), 0, None, None, 1, p(out), 1, p(fo),
+                                    4, None, 0, None, 0, None) == 128
+    assert lib.fmm3d_lfmm3dparttarg(1, 4, p(src), 1, p(q), 0, None, None, 1, p(out), 1, p(fo),
+                                    4, p(src), 1, None, 0, None) == 128
+    # the device entry points check the same before touching the device
+    assert lib.fmm3d_lfmm3dparttarg_dev(0, 1, 4, p(src), 1, p(q), 0, None, None, 1, None, 0, None,
+                                        0, None, 0, None, 0, None, None) == 128
+    zk = np.array([1.0, 0.0])
+    assert lib.fmm3d_hfmm3dparttarg_dev(0, 1, p(zk), 4, p(src), 1, None, 0, None, None, 1, p(out), 0, None,
+                                        0, None, 0, None, 0, None, None) == 128
+    assert lib.fmm3d_hfmm3dparttarg(1, p(zk), 4, p(src), 0, None, 1, p(q), None, 1, p(out), 0, None,
+                                    0, None, 0, None, 0, None) == 128
+    # the direct sum has no error channel: it leaves the outputs untouched
+    out[:] = 7
+    lib.fmm3d_l3dpartdirect(4, p(src), 1, None, 0, None, None, 1, p(out), 0, None, 0, None, 0, None, 0, None)
+    assert np.all(out == 7)
+
+
 def test_version_string():
     assert "gfx950" in F.version()
 

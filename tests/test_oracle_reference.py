@@ -10,7 +10,9 @@ pytestmark = pytest.mark.skipif(not ob.ref_available(), reason="oracle/_ref not 
 
 CASES = [(3000, 2000, 1, "cube", 1, 1), (4000, 0, 2, "sphere", 1, 0), (3000, 3000, 0, "clusters", 1, 1),
          (2000, 1000, 3, "cube", 0, 1), (1500, 500, -1, "plummer", 1, 1), (2500, 700, -2, "shell", 1, 1),
-         (1, 5, 1, "cube", 1, 1), (200, 0, 6, "cube", 1, 1)]
+         (1, 5, 1, "cube", 1, 1), (200, 0, 6, "cube", 1, 1),
+         # p = 51 and 64, with leaves at level 2 (a far field)
+         (12000, 2000, 4, "cube", 1, 1), (15000, 1000, 5, "clusters", 1, 1)]
 
 
 @pytest.mark.parametrize("ns,nt,iprec,dist,ifc,ifd", CASES)
