@@ -15,15 +15,19 @@ positions, offsets and accumulation in every mode); `value` is the fastest
 mode whose relative L2
 error against an FP64 direct sum on 1000 sampled targets is <= 1e-6 for the
 potential and the field.  The P2P roofline uses the exact pair count of the
-tree; the CPU baseline times the reference library (oracle/_ref) on a
-125,000 + 125,000 sample with the same leaf fill (244 points per leaf).
+tree; the CPU baseline times the reference library (oracle/_ref, OpenMP on
+the host's threads) on the same 1M + 1M inputs: one warm-up call, then the
+median of three, with the reference's own per-step timeinfo.  The drop-in
+line times the same workload through the host-array entry point (H2D and
+D2H included).
 
 Three more lines ride along in the same JSON object: "helmholtz"
 (hfmm3dpartself on 500,000 points on the unit sphere, zk = 20, iprec = 2),
-"serving" (a stream of mixed-size Laplace requests through the request server,
-p50/p99 latency and throughput) and "stream5" (Plummer-distributed Laplace and
-Helmholtz requests with charges and dipoles at iprec 2 and 3, each checked
-against the FP64 direct sum).
+"serving" (512 mixed-size Laplace requests through the request server,
+saturated and then with Poisson arrivals, p50/p99 latency and throughput) and
+"stream5" (256 Plummer-distributed Laplace and Helmholtz requests with
+charges and dipoles at iprec 2 and 3, the same two phases, each result
+checked against the FP64 direct sum).
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
 """
@@ -77,38 +81,96 @@ def p2p_pairs(src, trg, nbox=400):
     return int(total)
 
 
-def cpu_baseline_child(n):
-    """Runs in a child process: time the reference library (or the oracle)."""
+def cpu_baseline_child(n, runs):
+    """Runs in a child process: time the reference library (or the oracle)
+    on the headline inputs (the same seed as the GPU run), one warm-up call
+    and `runs` timed calls of lfmm3dparttarg; the reference prints its
+    per-step timeinfo (prini on unit 6), which the parent parses."""
     import numpy as np
     import oracle_bindings as ob
-    _, src, trg, q, ds, dv = make_inputs(n, SEED + 100)
+    _, src, trg, q, ds, dv = make_inputs(n, SEED)
     kind = "reference" if ob.ref_available() else "port"
+    if kind == "reference":
+        six, zero = ctypes.c_int(6), ctypes.c_int(0)
+        ob.ref().prini_(ctypes.byref(six), ctypes.byref(zero))
     run = ob.ref_lfmm3dparttarg if kind == "reference" else ob.orc_lfmm3dparttarg
-    run(IPREC, src[:2000], q[:2000], ds[:2000], dv[:2000], trg[:2000])  # warm up the runtime
     times = []
-    for _ in range(2):
+    for k in range(runs + 1):
         t = time.perf_counter()
         ier, *_ = run(IPREC, src, q, ds, dv, trg)
-        times.append(time.perf_counter() - t)
+        dt = time.perf_counter() - t
         assert ier == 0
-    t = min(times)
-    print(json.dumps({"kind": kind, "seconds": t, "value": n / t}))
+        if k > 0:                      # the first call is the warm-up
+            times.append(dt)
+        print("#cpu-run %d %.3f s" % (k, dt), flush=True)
+    print(json.dumps({"kind": kind, "times": times, "median": float(np.median(times)),
+                      "min": min(times)}), flush=True)
 
 
-def cpu_baseline(n=125_000):
-    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-    env = dict(os.environ, OMP_NUM_THREADS=str(threads), OMP_STACKSIZE="1024M")
+def _lscpu():
+    info = {}
     try:
-        out = subprocess.run([sys.executable, os.path.abspath(__file__), "--cpu-baseline-child", str(n)],
-                             env=env, stdout=subprocess.PIPE, stderr=subprocess.PIPE, text=True, timeout=300)
-        res = json.loads(out.stdout.strip().splitlines()[-1])
+        out = subprocess.run(["lscpu"], stdout=subprocess.PIPE, text=True, timeout=30).stdout
+        for line in out.splitlines():
+            k, _, v = line.partition(":")
+            info[k.strip()] = v.strip()
+    except Exception:
+        pass
+    sockets = int(info.get("Socket(s)", "0") or 0)
+    cores = int(info.get("Core(s) per socket", "0") or 0)
+    return {"model": info.get("Model name"), "sockets": sockets, "physical_cores": sockets * cores,
+            "logical_cpus": int(info.get("CPU(s)", "0") or 0),
+            "cpus_available_to_process": len(os.sched_getaffinity(0))}
+
+
+def _timeinfo(text):
+    """The last `timeinfo=` block the reference printed (8 values: P2M, P2L,
+    M2M, M2L, L2L, M2P, L2P, P2P seconds, src/lfmm3dpart.f:1155) and its
+    `time in d3tstrcr=`."""
+    lines = text.splitlines()
+    out = {}
+    for i, line in enumerate(lines):
+        if line.strip() == "timeinfo=":
+            vals = []
+            for nxt in lines[i + 1:i + 4]:
+                try:
+                    vals += [float(x) for x in nxt.split()]
+                except ValueError:
+                    break
+            if len(vals) >= 8:
+                names = ("p2m", "p2l", "m2m", "m2l", "l2l", "m2p", "l2p", "p2p")
+                out["timeinfo_s"] = dict(zip(names, vals[:8]))
+        if line.strip() == "time in d3tstrcr=" and i + 1 < len(lines):
+            try:
+                out["tree_s"] = float(lines[i + 1].split()[0])
+            except (ValueError, IndexError):
+                pass
+    return out
+
+
+def cpu_baseline(n=N, runs=3):
+    """The reference (oracle/_ref, OpenMP) on the headline workload itself,
+    on this host's CPU share: OMP_NUM_THREADS threads (16 on the GPU host),
+    1 warm-up + `runs` timed calls, median reported."""
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    env = dict(os.environ, OMP_NUM_THREADS=str(threads), OMP_STACKSIZE="1024M", OMP_PROC_BIND="close")
+    cpu = _lscpu()
+    try:
+        out = subprocess.run([sys.executable, "-u", os.path.abspath(__file__), "--cpu-baseline-child", str(n),
+                              "--cpu-runs", str(runs)],
+                             env=env, stdout=subprocess.PIPE, stderr=subprocess.PIPE, text=True, timeout=900)
+        res = json.loads([ln for ln in out.stdout.splitlines() if ln.startswith('{"kind"')][-1])
     except Exception as e:  # the baseline is reported, never required
         return {"value": None, "unit": "targets/s", "cores": threads, "kind": "reference",
-                "sample": "failed: %s" % e}
-    return {"value": res["value"], "unit": "targets/s", "cores": threads, "kind": res["kind"],
-            "sample": "lfmm3dparttarg %d sources + %d targets uniform cube, charges+dipoles, iprec=2 "
-                      "(same 244-point leaf fill as the 1M case), best of 2 after warm-up, %.2f s"
-                      % (n, n, res["seconds"])}
+                "sample": "failed: %s" % e, "cpu": cpu}
+    r = {"value": n / res["median"], "unit": "targets/s", "cores": threads, "kind": res["kind"],
+         "sample": "the headline workload itself: lfmm3dparttarg, %d sources + %d targets uniform in the cube, "
+                   "charges + unit dipoles, iprec=2, all four outputs (same seeded inputs as the GPU run); "
+                   "1 warm-up + %d timed calls, median %.2f s (min %.2f s), %d OpenMP threads"
+                   % (n, n, runs, res["median"], res["min"], threads),
+         "seconds": res["times"], "cpu": cpu}
+    r.update(_timeinfo(out.stdout))
+    return r
 
 
 def helmholtz_leg(F, torch, ob, steps, warmup):
@@ -156,52 +218,62 @@ def helmholtz_leg(F, torch, ob, steps, warmup):
             "stages_ms": {k: float(np.mean([s_[k] for s_ in stages])) for k in stages[0]}}
 
 
-def serving_leg(F, nreq, nworkers, seed=4):
-    """Config 4 shape: lfmm3dpartself requests with N log-uniform in
-    [1e4, 2e6], points from the unit cube, a sphere shell (r in [0.995, 1.005])
-    or 8 Gaussian clusters (sigma 0.02), iprec uniform in {1,2,3}, charges in
-    [-1,1].  Phase 1 submits all at once (saturated throughput); phase 2
-    submits the same requests at Poisson arrival times at 70% of that rate and
-    reports latency from submission to completion."""
+def _poisson_stream(F, reqs, submit, nworkers, seed):
+    """Phase 1: every request submitted at once (saturated throughput).
+    Phase 2: the same requests submitted at Poisson arrival times at 70% of
+    that rate; latency = submission to completion.  Returns the phase-2
+    results (in request order), the latencies and the rates."""
     import numpy as np
-    from fmm_inputs import points
     rng = np.random.default_rng(seed)
-    reqs = []
-    for _ in range(nreq):
-        n = int(10 ** rng.uniform(4, np.log10(2e6)))
-        dist = ["cube", "shell", "clusters"][rng.integers(0, 3)]
-        x = points(rng, n, dist)
-        q = rng.uniform(-1, 1, n) + 1j * rng.uniform(-1, 1, n)
-        reqs.append((int(rng.integers(1, 4)), x, q))
-    npts = sum(r[1].shape[0] for r in reqs)
     with F.Server(nworkers=nworkers, mem_budget=96 << 30) as srv:
         # warm every worker (tables, arenas)
-        for i in [srv.submit(0, it, x, q) for (it, x, q) in reqs[:nworkers]]:
+        for i in [submit(srv, r) for r in reqs[:nworkers]]:
             srv.wait(i)
         t0 = time.perf_counter()
-        ids = [srv.submit(0, it, x, q) for (it, x, q) in reqs]
+        ids = [submit(srv, r) for r in reqs]
         for i in ids:
             assert srv.wait(i)[0] == 0
         t_sat = time.perf_counter() - t0
-        rate = 0.7 * nreq / t_sat
-        gaps = rng.exponential(1.0 / rate, nreq)
+        print("#   saturated: %d requests in %.2f s" % (len(reqs), t_sat), file=sys.stderr, flush=True)
+        rate = 0.7 * len(reqs) / t_sat
+        gaps = rng.exponential(1.0 / rate, len(reqs))
         t0 = time.perf_counter()
         ids, tnext = [], 0.0
-        for (it, x, q), g in zip(reqs, gaps):
+        for r, g in zip(reqs, gaps):
             tnext += g
             d = t0 + tnext - time.perf_counter()
             if d > 0:
                 time.sleep(d)
-            ids.append(srv.submit(0, it, x, q))
-        lat = []
+            ids.append(submit(srv, r))
+        outs = []
         for i in ids:
-            r = srv.wait(i)
-            assert r[0] == 0
-            lat.append(r[5])
+            o = srv.wait(i)
+            assert o[0] == 0
+            outs.append(o)
         t_stream = time.perf_counter() - t0
+    lat = [o[5] for o in outs]
+    return outs, lat, t_sat, rate, t_stream
+
+
+def serving_leg(F, nreq, nworkers, seed=4):
+    """Laplace serving stream: `nreq` lfmm3dpartself requests (N log-uniform
+    in [1e4, 2e6]; unit cube, sphere shell or 8 Gaussian clusters; iprec
+    uniform in {1,2,3}; charges in [-1,1]; FP64), saturated and then with
+    Poisson arrivals at 70% of the saturated rate."""
+    import numpy as np
+    from fmm_inputs import stream4_requests
+    reqs = stream4_requests(seed, nreq)
+    npts = sum(r[1].shape[0] for r in reqs)
+
+    def submit(srv, r):
+        it, x, q = r
+        return srv.submit(0, it, x, q)
+
+    outs, lat, t_sat, rate, t_stream = _poisson_stream(F, reqs, submit, nworkers, seed + 1000)
+    del outs
     return {"workload": "%d lfmm3dpartself requests, N log-uniform in [1e4, 2e6], cube/shell/clusters, "
-                        "iprec in {1,2,3}; Poisson arrivals at 70%% of the saturated rate" % nreq,
-            "workers": nworkers, "points_total": npts,
+                        "iprec in {1,2,3}, FP64; Poisson arrivals at 70%% of the saturated rate" % nreq,
+            "workers": nworkers, "requests": nreq, "points_total": npts,
             "saturated_req_per_s": nreq / t_sat, "saturated_points_per_s": npts / t_sat,
             "arrival_rate_hz": rate, "stream_s": t_stream,
             "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
@@ -209,64 +281,62 @@ def serving_leg(F, nreq, nworkers, seed=4):
 
 
 def stream5_leg(F, ob, nreq, nworkers, seed=5):
-    """Stream 5 shape: requests with N log-uniform in [5e3, 5e5], Plummer
-    points (a = 0.1, centred in the unit cube, points outside rejected),
-    Laplace or Helmholtz (zk uniform in [1, 30]) per request, iprec in {2, 3},
-    charges and dipoles (U[-1,1] parts, unit dipole vectors).  iprec 2 runs
-    the FP32 near field, iprec 3 FP64 throughout.  All requests are submitted
-    at once; every result is checked against the FP64 direct sum at 200
-    sampled sources."""
+    """Mixed stream: `nreq` requests (N log-uniform in [5e3, 5e5], Plummer
+    points, Laplace or Helmholtz with zk in [1, 30], iprec in {2, 3},
+    charges and dipoles), all asking for the mixed mode (FP32 near-field pair
+    arithmetic), which the library runs in FP64 at iprec 3; saturated and
+    then with Poisson arrivals at 70% of the saturated rate.  Every Poisson-
+    phase result is checked against the FP64 direct sum at 200 sampled
+    sources: potential and field against the tolerance iprec names."""
     import numpy as np
-    from fmm_inputs import IPREC_EPS, points, rel
-    rng = np.random.default_rng(seed)
-    reqs = []
-    for _ in range(nreq):
-        n = int(10 ** rng.uniform(np.log10(5e3), np.log10(5e5)))
-        x = points(rng, n, "plummer")
-        zk = complex(rng.uniform(1, 30), 0.0) if rng.integers(0, 2) else None
-        iprec = int(rng.integers(2, 4))
-        q = rng.uniform(-1, 1, n) + 1j * rng.uniform(-1, 1, n)
-        ds = rng.uniform(-1, 1, n) + 1j * rng.uniform(-1, 1, n)
-        z = rng.uniform(-1, 1, n)
-        ph = rng.uniform(0, 2 * np.pi, n)
-        dv = np.stack([np.sqrt(1 - z * z) * np.cos(ph), np.sqrt(1 - z * z) * np.sin(ph), z], axis=1)
-        reqs.append((zk, iprec, x, q, ds, dv))
+    from fmm_inputs import IPREC_EPS, rel, stream5_requests
+    reqs = stream5_requests(seed, nreq)
 
     def submit(srv, r):
         zk, iprec, x, q, ds, dv = r
-        prec = F.PREC_MIXED if iprec == 2 else F.PREC_FP64
-        return srv.submit(0 if zk is None else 1, iprec, x, charge=q, dipstr=ds, dipvec=dv, zk=zk, precision=prec)
+        return srv.submit(0 if zk is None else 1, iprec, x, charge=q, dipstr=ds, dipvec=dv, zk=zk,
+                          precision=F.PREC_MIXED)
 
     npts = sum(r[2].shape[0] for r in reqs)
-    with F.Server(nworkers=nworkers, mem_budget=96 << 30) as srv:
-        for i in [submit(srv, r) for r in reqs[:nworkers]]:
-            srv.wait(i)
-        t0 = time.perf_counter()
-        ids = [submit(srv, r) for r in reqs]
-        out = [srv.wait(i) for i in ids]
-        t_sat = time.perf_counter() - t0
+    outs, lat, t_sat, rate, t_stream = _poisson_stream(F, reqs, submit, nworkers, seed + 1000)
+    rng = np.random.default_rng(seed + 2000)
     worst = {}
-    lat = []
-    for r, o in zip(reqs, out):
+    for r, o in zip(reqs, outs):
         zk, iprec, x, q, ds, dv = r
-        assert o[0] == 0
-        lat.append(o[5])
         idx = rng.choice(x.shape[0], 200, replace=False)
         pd, fd = ob.orc_direct_sample(x, x[idx], idx.astype(np.int32), q, ds, dv, zk=zk)
         key = "%s_iprec%d" % ("laplace" if zk is None else "helmholtz", iprec)
-        e = max(rel(o[1][idx], pd), rel(o[2][idx], fd))
-        worst[key] = max(worst.get(key, 0.0), e)
-    # two bars: the metric's 1e-6, and the tolerance iprec names (0.5e-6 at
-    # iprec 2, which the FP32 near field can exceed on clustered points)
-    within = all(worst[k] <= IPREC_EPS[int(k[-1])] for k in worst)
+        for out, ref, name in ((o[1], pd, "pot"), (o[2], fd, "fld")):
+            k2 = key + "_" + name
+            worst[k2] = max(worst.get(k2, 0.0), rel(out[idx], ref))
+    within = all(v <= IPREC_EPS[int(k.split("_")[1][5:])] for k, v in worst.items())
     within_1e6 = all(v <= 1e-6 for v in worst.values())
     return {"workload": "%d requests, N log-uniform in [5e3, 5e5], Plummer points, Laplace or Helmholtz "
-                        "(zk in [1, 30]), iprec in {2, 3}, charges + dipoles; all submitted at once" % nreq,
-            "workers": nworkers, "points_total": npts, "seconds": t_sat,
-            "req_per_s": nreq / t_sat, "points_per_s": npts / t_sat,
+                        "(zk in [1, 30]), iprec in {2, 3}, charges + dipoles, mixed precision (FP64 at iprec 3); "
+                        "Poisson arrivals at 70%% of the saturated rate" % nreq,
+            "workers": nworkers, "requests": nreq, "points_total": npts,
+            "saturated_req_per_s": nreq / t_sat, "saturated_points_per_s": npts / t_sat,
+            "arrival_rate_hz": rate, "stream_s": t_stream,
             "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
+            "mean_ms": float(np.mean(lat)),
             "max_rel_err": worst, "within_1e-6": within_1e6, "within_iprec_tolerance": within,
             "sampled_per_request": 200}
+
+
+def dropin_leg(F, prec, src, q, ds, dv, trg, steps=3):
+    """The drop-in call: fmm3d_lfmm3dparttarg_ex with host arrays in and out
+    (upload, device tree, solve, download), on the headline inputs."""
+    import numpy as np
+    F.lfmm3dparttarg(IPREC, src, q, ds, dv, trg, precision=prec)   # warm-up
+    lat = []
+    for _ in range(steps):
+        a = time.perf_counter()
+        r = F.lfmm3dparttarg(IPREC, src, q, ds, dv, trg, precision=prec)
+        lat.append(time.perf_counter() - a)
+        assert r[0] == 0
+    ms = float(np.median(lat)) * 1e3
+    return {"entry": "fmm3d_lfmm3dparttarg_ex (host arrays in and out, H2D and D2H included)",
+            "ms_per_call": ms, "targets_per_s": N / (ms * 1e-3), "calls": steps}
 
 
 def run_mode(F, torch, prec, dev_in, dev_out, steps, warmup):
@@ -306,16 +376,17 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-helm", action="store_true", help="skip the Helmholtz (config 3) line")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving-stream line")
-    ap.add_argument("--serve-requests", type=int, default=48)
+    ap.add_argument("--serve-requests", type=int, default=512)
     ap.add_argument("--serve-workers", type=int, default=4)
     ap.add_argument("--no-stream5", action="store_true", help="skip the mixed Laplace/Helmholtz Plummer stream")
-    ap.add_argument("--stream5-requests", type=int, default=32)
+    ap.add_argument("--stream5-requests", type=int, default=256)
     ap.add_argument("--cpu-baseline-child", type=int, default=0, help=argparse.SUPPRESS)
+    ap.add_argument("--cpu-runs", type=int, default=3, help="timed reference calls after the warm-up")
     ap.add_argument("--calibrate-copy-mb", type=int, default=0,
                     help="only run a plain device-to-device copy of this many MiB (HBM counter calibration)")
     args = ap.parse_args()
     if args.cpu_baseline_child:
-        cpu_baseline_child(args.cpu_baseline_child)
+        cpu_baseline_child(args.cpu_baseline_child, args.cpu_runs)
         return
     if args.calibrate_copy_mb:
         import torch
@@ -383,6 +454,9 @@ def main():
         del os.environ["FMM3D_NEAR_STREAM"]
     p2p_alone_ms = float(np.mean([s_["p2p"] for s_ in st_alone]))
     achieved_alone = flops / (p2p_alone_ms * 1e-3) / 1e12
+    dropin = dropin_leg(F, prec_of[best], src, q, ds, dv, trg)
+    print("# drop-in (host arrays): %.1f ms/call, %.3e targets/s" % (dropin["ms_per_call"], dropin["targets_per_s"]),
+          file=sys.stderr, flush=True)
     helm = None if args.no_helm else helmholtz_leg(F, torch, ob, steps=max(2, min(args.steps, 3)), warmup=1)
     if helm:
         print("# helmholtz: %.1f ms/step, err pot %.2e fld %.2e, stages %s"
@@ -395,11 +469,11 @@ def main():
               file=sys.stderr, flush=True)
     s5 = None if args.no_stream5 else stream5_leg(F, ob, args.stream5_requests, args.serve_workers)
     if s5:
-        print("# stream 5: %.2f req/s (%.3e points/s), worst errors %s"
-              % (s5["req_per_s"], s5["points_per_s"], {k: "%.1e" % v for k, v in s5["max_rel_err"].items()}),
-              file=sys.stderr, flush=True)
+        print("# stream 5: p50 %.1f ms p99 %.1f ms, saturated %.2f req/s (%.3e points/s), worst errors %s"
+              % (s5["p50_ms"], s5["p99_ms"], s5["saturated_req_per_s"], s5["saturated_points_per_s"],
+                 {k: "%.1e" % v for k, v in s5["max_rel_err"].items()}), file=sys.stderr, flush=True)
     cpu = {"value": None, "unit": "targets/s", "cores": 0, "kind": "reference", "sample": "skipped"} \
-        if args.no_cpu else cpu_baseline()
+        if args.no_cpu else cpu_baseline(runs=args.cpu_runs)
     line = {
         "metric": METRIC, "value": M["targets_per_s"], "unit": "targets/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": M["ms_per_step"],
@@ -423,6 +497,7 @@ def main():
                              "with the near field on the caller's stream (FMM3D_NEAR_STREAM=0), 2 extra steps. "
                              "algorithmic flops = %d pairs x %d" % (pairs, FLOPS_PER_PAIR)},
         "cpu_baseline": cpu,
+        "dropin_host_arrays": dropin,
         "modes": {k: {kk: vv for kk, vv in v.items() if kk != "latencies_ms"} for k, v in modes.items()},
         "helmholtz": helm,
         "serving": serve,

@@ -1172,15 +1172,28 @@ template <>
 __device__ __forceinline__ float exp_t<float>(float x) { return expf(x); }
 
 // Offsets and r^2 in FP64 always; with TN = float the rest of the pair is
-// FP32 (sums folded into FP64 by the caller every kFold sources).
+// FP32 (sums folded into FP64 by the caller every kFold sources), except for
+// the closest pairs (r^2 < rc2, in practice r below an eighth of the leaf
+// side): their terms dominate the sums at the points they are near, and the
+// few FP32 roundings of such a pair would be the largest error of the
+// result, so they are computed in FP64 and added to the FP64 totals `tot`.
 template <typename TN, bool CHG, bool DIP, bool FLD>
 struct HPairAcc {
   TN a[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  template <typename TS>
   __device__ __forceinline__ void add(double kre_d, double kim_d, double xt, double yt, double zt,
-                                      const double* __restrict__ sp, const TN* __restrict__ st,
-                                      bool mask) {
+                                      const double* __restrict__ sp, const TS* __restrict__ sts,
+                                      bool mask, double rc2 = 0.0, double* tot = nullptr) {
     const double dxd = xt - sp[0], dyd = yt - sp[1], dzd = zt - sp[2];
     const double r2d = fma(dxd, dxd, fma(dyd, dyd, dzd * dzd));
+    if (sizeof(TN) == sizeof(float) && r2d < rc2 && !mask) {
+      HPairAcc<double, CHG, DIP, FLD> c64;
+      c64.add(kre_d, kim_d, xt, yt, zt, sp, sts, false);
+      for (int k = 0; k < 8; k++) tot[k] += c64.a[k];
+      return;
+    }
+    TN st[8];
+    for (int k = 0; k < 8; k++) st[k] = (TN)sts[k];
     const TN dx = (TN)dxd, dy = (TN)dyd, dz = (TN)dzd, r2 = (TN)r2d;
     const TN kre = (TN)kre_d, kim = (TN)kim_d;
     TN ir = rsqrt_t<TN>(r2);
@@ -1273,12 +1286,16 @@ __global__ __launch_bounds__(256) void k_hp2p(DevPlan P, double kre, double kim,
   HPairAcc<TN, CHG, DIP, FLD> acc;
   double tot[8] = {0, 0, 0, 0, 0, 0, 0, 0};
   constexpr bool fold = sizeof(TN) < sizeof(double);
+  // FP32 pairs: pairs closer than an eighth of this leaf's side run in FP64
+  const double hside = 1.0 / P.inv_side[P.level[b]];
+  const double rc2 = fold ? hside * hside * (1.0 / 64.0) : 0.0;
   {
     const int s0 = P.isrc[b], s1 = s0 + P.nsrc[b];
     const int self = kind ? -1 : me;
     for (int c = s0; c < s1; c += (fold ? kFold : (1 << 30))) {
       const int ce = fold ? min(c + kFold, s1) : s1;
-      for (int s = c; s < ce; s++) acc.add(kre, kim, x, y, z, spos + 4 * (size_t)s, str + 8 * (size_t)s, s == self);
+      for (int s = c; s < ce; s++)
+        acc.add(kre, kim, x, y, z, spos + 4 * (size_t)s, str + 8 * (size_t)s, s == self, rc2, tot);
       if (fold) acc.fold_into(tot);
     }
   }
@@ -1287,7 +1304,8 @@ __global__ __launch_bounds__(256) void k_hp2p(DevPlan P, double kre, double kim,
     const int s0 = P.isrc[j], s1 = s0 + P.nsrc[j];
     for (int c = s0; c < s1; c += (fold ? kFold : (1 << 30))) {
       const int ce = fold ? min(c + kFold, s1) : s1;
-      for (int s = c; s < ce; s++) acc.add(kre, kim, x, y, z, spos + 4 * (size_t)s, str + 8 * (size_t)s, false);
+      for (int s = c; s < ce; s++)
+        acc.add(kre, kim, x, y, z, spos + 4 * (size_t)s, str + 8 * (size_t)s, false, rc2, tot);
       if (fold) acc.fold_into(tot);
     }
   }

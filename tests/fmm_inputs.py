@@ -58,3 +58,42 @@ def rel(a, b):
 
 # tolerance implied by iprec (src/lfmm3dpart.f:277-285)
 IPREC_EPS = {-2: .5e0, -1: .5e-1, 0: .5e-2, 1: .5e-3, 2: .5e-6, 3: .5e-9, 4: .5e-12, 5: .5e-15}
+
+
+def stream4_requests(seed=4, nreq=512):
+    """The Laplace serving stream: lfmm3dpartself requests with N
+    log-uniform in [1e4, 2e6], points from the unit cube, a sphere shell
+    (r in [0.995, 1.005]) or 8 Gaussian clusters (sigma 0.02) chosen
+    uniformly, iprec uniform in {1, 2, 3}, charges with U[-1,1] parts.
+    Returns (iprec, points, charges) per request."""
+    rng = np.random.default_rng(seed)
+    reqs = []
+    for _ in range(nreq):
+        n = int(10 ** rng.uniform(4, np.log10(2e6)))
+        dist = ["cube", "shell", "clusters"][rng.integers(0, 3)]
+        x = points(rng, n, dist)
+        q = rng.uniform(-1, 1, n) + 1j * rng.uniform(-1, 1, n)
+        reqs.append((int(rng.integers(1, 4)), x, q))
+    return reqs
+
+
+def stream5_requests(seed=5, nreq=256):
+    """The mixed serving stream: N log-uniform in [5e3, 5e5], Plummer points
+    (a = 0.1, centred in the unit cube, points outside rejected), Laplace or
+    Helmholtz (zk uniform in [1, 30]) per request, iprec in {2, 3}, charges
+    and dipoles (U[-1,1] parts, unit dipole vectors).  Returns (zk or None,
+    iprec, points, charge, dipstr, dipvec) per request."""
+    rng = np.random.default_rng(seed)
+    reqs = []
+    for _ in range(nreq):
+        n = int(10 ** rng.uniform(np.log10(5e3), np.log10(5e5)))
+        x = points(rng, n, "plummer")
+        zk = complex(rng.uniform(1, 30), 0.0) if rng.integers(0, 2) else None
+        iprec = int(rng.integers(2, 4))
+        q = rng.uniform(-1, 1, n) + 1j * rng.uniform(-1, 1, n)
+        ds = rng.uniform(-1, 1, n) + 1j * rng.uniform(-1, 1, n)
+        z = rng.uniform(-1, 1, n)
+        ph = rng.uniform(0, 2 * np.pi, n)
+        dv = np.stack([np.sqrt(1 - z * z) * np.cos(ph), np.sqrt(1 - z * z) * np.sin(ph), z], axis=1)
+        reqs.append((zk, iprec, x, q, ds, dv))
+    return reqs
