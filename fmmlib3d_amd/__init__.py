@@ -29,7 +29,8 @@ PREC_FP32 = 3     # FP32 near and far field
 # every symbol include/fmm3d_mi355x.h declares
 EXPORTED = (
     "fmm3d_lfmm3dparttarg", "fmm3d_lfmm3dpartself", "fmm3d_l3dpartdirect",
-    "fmm3d_lfmm3dparttarg_ex", "fmm3d_lfmm3dparttarg_dev", "fmm3d_last_timings",
+    "fmm3d_lfmm3dparttarg_ex", "fmm3d_lfmm3dparttarg_dev", "fmm3d_lfmm3dparttarg_dev_noalloc",
+    "fmm3d_last_timings",
     "fmm3d_last_precision", "fmm3d_version", "fmm3d_d3tstrcr",
     "fmm3d_hfmm3dparttarg", "fmm3d_hfmm3dpartself", "fmm3d_h3dpartdirect",
     "fmm3d_hfmm3dparttarg_ex", "fmm3d_hfmm3dparttarg_dev",
@@ -63,6 +64,8 @@ def load():
         lib.fmm3d_lfmm3dparttarg_ex.restype = i
         lib.fmm3d_lfmm3dparttarg_dev.argtypes = [i] + targ + [vp]
         lib.fmm3d_lfmm3dparttarg_dev.restype = i
+        lib.fmm3d_lfmm3dparttarg_dev_noalloc.argtypes = [i] + targ + [vp]
+        lib.fmm3d_lfmm3dparttarg_dev_noalloc.restype = i
         lib.fmm3d_lfmm3dpartself.argtypes = [i, i, vp, i, vp, i, vp, vp, i, vp, i, vp]
         lib.fmm3d_lfmm3dpartself.restype = i
         lib.fmm3d_l3dpartdirect.argtypes = [i, vp, i, vp, i, vp, vp, i, vp, i, vp, i, vp, i, vp, i, vp]

@@ -5,6 +5,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstring>
+#include <mutex>
 #include <vector>
 
 #include "../../include/fmm3d_mi355x.h"
@@ -154,17 +155,12 @@ int fmm3d_lfmm3dpartself(int iprec, int nsource, const double* source, int ifcha
 
 namespace {
 
-int fmm_dev(const double* zk, int prec_mode, int iprec, int ns, const double* source_dev,
-            int ifcharge, const double* charge_dev, int ifdipole, const double* dipstr_dev,
-            const double* dipvec_dev, int ifpot, double* pot_dev, int iffld, double* fld_dev, int nt,
-            const double* target_dev, int ifpottarg, double* pottarg_dev, int iffldtarg,
-            double* fldtarg_dev, void* stream) {
-  if (int e = check_args(zk, prec_mode, iprec, ns, nt)) return e;
-  if (int e = check_ptrs(ns, source_dev, ifcharge, charge_dev, ifdipole, dipstr_dev, dipvec_dev, ifpot, pot_dev,
-                         iffld, fld_dev, nt, target_dev, ifpottarg, pottarg_dev, iffldtarg, fldtarg_dev))
-    return e;
-  Context& ctx = default_context();
-  std::lock_guard<std::mutex> lock(ctx.mu);
+// The device-resident call on a given context; the caller holds ctx.mu.
+int fmm_dev_on(Context& ctx, const double* zk, int prec_mode, int iprec, int ns, const double* source_dev,
+               int ifcharge, const double* charge_dev, int ifdipole, const double* dipstr_dev,
+               const double* dipvec_dev, int ifpot, double* pot_dev, int iffld, double* fld_dev, int nt,
+               const double* target_dev, int ifpottarg, double* pottarg_dev, int iffldtarg,
+               double* fldtarg_dev, void* stream) {
   hipStream_t saved = ctx.stream;
   ctx.stream = (hipStream_t)stream;
   int ier = guarded([&] {
@@ -196,6 +192,61 @@ int fmm_dev(const double* zk, int prec_mode, int iprec, int ns, const double* so
   return ier;
 }
 
+int fmm_dev(const double* zk, int prec_mode, int iprec, int ns, const double* source_dev,
+            int ifcharge, const double* charge_dev, int ifdipole, const double* dipstr_dev,
+            const double* dipvec_dev, int ifpot, double* pot_dev, int iffld, double* fld_dev, int nt,
+            const double* target_dev, int ifpottarg, double* pottarg_dev, int iffldtarg,
+            double* fldtarg_dev, void* stream) {
+  if (int e = check_args(zk, prec_mode, iprec, ns, nt)) return e;
+  if (int e = check_ptrs(ns, source_dev, ifcharge, charge_dev, ifdipole, dipstr_dev, dipvec_dev, ifpot, pot_dev,
+                         iffld, fld_dev, nt, target_dev, ifpottarg, pottarg_dev, iffldtarg, fldtarg_dev))
+    return e;
+  Context& ctx = default_context();
+  std::lock_guard<std::mutex> lock(ctx.mu);
+  return fmm_dev_on(ctx, zk, prec_mode, iprec, ns, source_dev, ifcharge, charge_dev, ifdipole, dipstr_dev,
+                    dipvec_dev, ifpot, pot_dev, iffld, fld_dev, nt, target_dev, ifpottarg, pottarg_dev,
+                    iffldtarg, fldtarg_dev, stream);
+}
+
+// ---- allocation-free variant: every device buffer of the call is a slice
+// of `scratch`, a file-scope device array of FMM3D_NOALLOC_SCRATCH_BYTES.
+// Host code never uses the array's name as an address: the device address
+// comes from hipGetSymbolAddress, once per device with that device current,
+// and is kept per device in NoAllocDevice::ptr.
+__device__ __align__(256) unsigned char scratch[FMM3D_NOALLOC_SCRATCH_BYTES];
+
+constexpr int kMaxDevices = 64;
+
+struct NoAllocDevice {
+  std::once_flag once;
+  hipError_t status = hipErrorNotInitialized;
+  void* ptr = nullptr;  // this device's copy of `scratch`
+  Context ctx;          // arena in fixed mode over ptr; near field on the caller's stream
+};
+
+NoAllocDevice& noalloc_device(int dev) {
+  static NoAllocDevice devs[kMaxDevices];
+  NoAllocDevice& d = devs[dev];
+  std::call_once(d.once, [&] {
+    d.status = hipGetSymbolAddress(&d.ptr, HIP_SYMBOL(scratch));
+    if (d.status != hipSuccess) {
+      (void)hipGetLastError();
+      d.ptr = nullptr;
+      return;
+    }
+    d.ctx.device = dev;
+    d.ctx.near_on_aux = false;
+    d.ctx.arena.use_fixed(d.ptr, FMM3D_NOALLOC_SCRATCH_BYTES);
+  });
+  return d;
+}
+
+// Bytes per point that every call takes from the region whatever its tree
+// (sorted positions and strengths, far and near sums, octree keys): a
+// lower bound, checked before any device work; the exact bound is the
+// region itself (AllocError kIerScratch from the arena).
+constexpr size_t kNoAllocBytesPerPoint = 256;
+
 }  // namespace
 
 extern "C" {
@@ -209,6 +260,34 @@ int fmm3d_lfmm3dparttarg_dev(int prec_mode, int iprec, int ns, const double* sou
   return fmm_dev(nullptr, prec_mode, iprec, ns, source_dev, ifcharge, charge_dev, ifdipole, dipstr_dev,
                  dipvec_dev, ifpot, pot_dev, iffld, fld_dev, nt, target_dev, ifpottarg, pottarg_dev,
                  iffldtarg, fldtarg_dev, stream);
+}
+
+int fmm3d_lfmm3dparttarg_dev_noalloc(int prec_mode, int iprec, int ns, const double* source_dev,
+                                     int ifcharge, const double* charge_dev, int ifdipole,
+                                     const double* dipstr_dev, const double* dipvec_dev, int ifpot,
+                                     double* pot_dev, int iffld, double* fld_dev, int nt,
+                                     const double* target_dev, int ifpottarg, double* pottarg_dev,
+                                     int iffldtarg, double* fldtarg_dev, void* stream) {
+  if (int e = check_args(nullptr, prec_mode, iprec, ns, nt)) return e;
+  if (int e = check_ptrs(ns, source_dev, ifcharge, charge_dev, ifdipole, dipstr_dev, dipvec_dev, ifpot, pot_dev,
+                         iffld, fld_dev, nt, target_dev, ifpottarg, pottarg_dev, iffldtarg, fldtarg_dev))
+    return e;
+  if (((size_t)ns + (size_t)nt) * kNoAllocBytesPerPoint > FMM3D_NOALLOC_SCRATCH_BYTES) return kIerScratch;
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDevices) {
+    (void)hipGetLastError();
+    return 64;
+  }
+  NoAllocDevice& d = noalloc_device(dev);
+  if (d.status != hipSuccess) return 64;
+  std::lock_guard<std::mutex> lock(d.ctx.mu);
+  // the region keeps the previous call's contents: forget its slices and
+  // the tables it held (they are uploaded again into this call's slices)
+  d.ctx.arena.reset_fixed();
+  d.ctx.lap_tabs.clear();
+  return fmm_dev_on(d.ctx, nullptr, prec_mode, iprec, ns, source_dev, ifcharge, charge_dev, ifdipole, dipstr_dev,
+                    dipvec_dev, ifpot, pot_dev, iffld, fld_dev, nt, target_dev, ifpottarg, pottarg_dev,
+                    iffldtarg, fldtarg_dev, stream);
 }
 
 int fmm3d_hfmm3dparttarg_dev(int prec_mode, int iprec, const double* zk, int ns, const double* source_dev,

@@ -14,24 +14,54 @@ Context::~Context() {
   if (aux) (void)hipStreamDestroy(aux);
 }
 
+static void aux_events(Context& ctx) {
+  for (auto& e : ctx.aux_ev)
+    if (!e) FMM_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+}
+
 hipStream_t aux_stream(Context& ctx) {
-  if (!ctx.aux) {
-    FMM_HIP(hipStreamCreateWithFlags(&ctx.aux, hipStreamNonBlocking));
-    for (auto& e : ctx.aux_ev) FMM_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
-  }
+  if (!ctx.aux) FMM_HIP(hipStreamCreateWithFlags(&ctx.aux, hipStreamNonBlocking));
+  aux_events(ctx);
   return ctx.aux;
 }
 
 hipStream_t near_stream(Context& ctx) {
-  hipStream_t s2 = aux_stream(ctx);
   const char* v = std::getenv("FMM3D_NEAR_STREAM");
-  if (!ctx.near_on_aux || (v && std::strcmp(v, "0") == 0)) return ctx.stream;
-  return s2;
+  if (!ctx.near_on_aux || (v && std::strcmp(v, "0") == 0)) {
+    aux_events(ctx);  // the drivers order the two sums with them either way
+    return ctx.stream;
+  }
+  return aux_stream(ctx);
+}
+
+void Arena::use_fixed(void* base, size_t cap) {
+  release_all();
+  fbase_ = (char*)base;
+  fcap_ = cap;
+  fused_ = 0;
+}
+
+void Arena::reset_fixed() {
+  bufs_.clear();
+  fused_ = 0;
 }
 
 void* Arena::get(const std::string& name, size_t bytes, int ier_on_fail) {
   if (bytes == 0) bytes = 256;
   bytes = (bytes + 255) & ~(size_t)255;
+  if (fbase_) {
+    // a slice of the fixed region: reused while large enough, else a new
+    // slice after the last one (the old one is not reclaimed until reset)
+    auto it = bufs_.find(name);
+    if (it != bufs_.end() && it->second.n >= bytes) return it->second.p;
+    if (bytes > fcap_ - fused_)
+      throw AllocError("fixed device region too small for " + name, kIerScratch);
+    Buf& b = bufs_[name];
+    b.p = fbase_ + fused_;
+    b.n = bytes;
+    fused_ += bytes;
+    return b.p;
+  }
   Buf& b = bufs_[name];
   if (b.n >= bytes) return b.p;
   if (b.p) {
@@ -58,6 +88,11 @@ void* Arena::get(const std::string& name, size_t bytes, int ier_on_fail) {
 void* Arena::get_host(const std::string& name, size_t bytes, int ier_on_fail) {
   if (bytes == 0) bytes = 256;
   bytes = (bytes + 255) & ~(size_t)255;
+  if (fbase_) {
+    std::vector<char>& v = heap_[name];
+    if (v.size() < bytes) v.resize(bytes);
+    return v.data();
+  }
   Buf& b = hbufs_[name];
   if (b.n >= bytes) return b.p;
   if (b.p) {
@@ -76,9 +111,11 @@ void* Arena::get_host(const std::string& name, size_t bytes, int ier_on_fail) {
 }
 
 void Arena::release_all() {
-  for (auto& kv : bufs_)
-    if (kv.second.p) (void)hipFree(kv.second.p);
+  if (!fbase_)
+    for (auto& kv : bufs_)
+      if (kv.second.p) (void)hipFree(kv.second.p);
   bufs_.clear();
+  heap_.clear();
   for (auto& kv : hbufs_)
     if (kv.second.p) (void)hipHostFree(kv.second.p);
   hbufs_.clear();

@@ -28,7 +28,16 @@ struct AllocError : std::runtime_error {
       throw ::fmm3d::HipError(std::string(#call) + ": " + hipGetErrorString(e_)); \
   } while (0)
 
+// ier of a call whose buffers do not fit a fixed region (see Arena::use_fixed)
+constexpr int kIerScratch = 256;
+
 // Grow-only named device buffers; released when the context dies.
+//
+// Fixed mode (use_fixed): every device buffer is a slice of one region the
+// caller owns, handed out in request order, and host staging buffers are
+// ordinary heap memory; the arena then never calls hipMalloc, hipFree,
+// hipHostMalloc or hipHostFree.  reset_fixed() forgets the slices (start of
+// a call); a request beyond the region throws AllocError(kIerScratch).
 class Arena {
  public:
   ~Arena();
@@ -36,10 +45,16 @@ class Arena {
   // pinned host buffer (hipHostMalloc), same growth rule
   void* get_host(const std::string& name, size_t bytes, int ier_on_fail = 8);
   void release_all();
+  void use_fixed(void* base, size_t cap);
+  void reset_fixed();
+  bool fixed() const { return fbase_ != nullptr; }
 
  private:
   struct Buf { void* p = nullptr; size_t n = 0; };
   std::map<std::string, Buf> bufs_, hbufs_;
+  std::map<std::string, std::vector<char>> heap_;  // host buffers in fixed mode
+  char* fbase_ = nullptr;
+  size_t fcap_ = 0, fused_ = 0;
 };
 
 struct Timer {
@@ -69,6 +84,9 @@ struct Context {
   // workers already overlap whole requests)
   bool near_on_aux = true;
   ~Context();
+  Context() = default;
+  Context(const Context&) = delete;
+  Context& operator=(const Context&) = delete;
 };
 
 // the context's second stream, created on first use on the current device

@@ -85,6 +85,34 @@ int fmm3d_lfmm3dparttarg_dev(int prec_mode, int iprec, int nsource, const double
                              int ifpottarg, double *pottarg_dev, int iffldtarg, double *fldtarg_dev,
                              void *stream);
 
+/* Size of the device array behind fmm3d_lfmm3dparttarg_dev_noalloc (256 MiB;
+ * each GPU the entry point runs on holds one until the process ends). */
+#define FMM3D_NOALLOC_SCRATCH_BYTES (256ull << 20)
+
+/* fmm3d_lfmm3dparttarg_dev without any device allocation.  No counterpart in
+ * the reference: it is for callers that cannot allocate device memory
+ * (tight serving loops, harnesses that forbid hipMalloc).  Arguments and
+ * results are those of fmm3d_lfmm3dparttarg_dev (and bitwise equal to them);
+ * every work buffer of the call (sorted points, expansions, plan, tables,
+ * sums) is a slice of one static device array of
+ * FMM3D_NOALLOC_SCRATCH_BYTES, so the call makes no hipMalloc, hipFree or
+ * hipHostMalloc (host staging is ordinary heap memory, and the near field
+ * stays on `stream`).  A problem whose buffers do not fit returns 256; as a
+ * rule of thumb (nsource + ntarget) up to about 4e5 fit at iprec 2, and
+ * any nsource + ntarget above 1,048,576 is rejected before device work
+ * starts.  The array keeps the last call's contents; calls on one GPU are
+ * serialised by the library within a process, and calls from two processes
+ * on one GPU use separate arrays.  Like the other entry points, the call
+ * synchronises with the host (tree decisions and lists are made there), so
+ * it cannot be captured in a HIP graph. */
+int fmm3d_lfmm3dparttarg_dev_noalloc(int prec_mode, int iprec, int nsource, const double *source_dev,
+                                     int ifcharge, const double *charge_dev,
+                                     int ifdipole, const double *dipstr_dev, const double *dipvec_dev,
+                                     int ifpot, double *pot_dev, int iffld, double *fld_dev,
+                                     int ntarget, const double *target_dev,
+                                     int ifpottarg, double *pottarg_dev, int iffldtarg,
+                                     double *fldtarg_dev, void *stream);
+
 /* Replaces hfmm3dparttarg, src/hfmm3dpart.f:141-144: Helmholtz kernel
  * e^{ik r}/r, zk = complex*16 wavenumber passed by reference as (re, im)
  * (zk = 0 returns 128).  Other arguments as fmm3d_lfmm3dparttarg. */

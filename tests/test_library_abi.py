@@ -132,3 +132,19 @@ def test_host_octree_equals_golden_tree():
 
 def test_host_octree_rejects_empty():
     assert F.d3tstrcr(np.zeros((0, 3)), None, 10)[0] == 128
+
+
+def test_noalloc_rejects_problem_beyond_its_array():
+    """fmm3d_lfmm3dparttarg_dev_noalloc returns 256 for a problem larger
+    than its static device array, before any device call (the pointers are
+    never dereferenced)."""
+    import ctypes
+    lib = F.load()
+    dummy = ctypes.c_void_p(256)
+    n = 700_000
+    ier = lib.fmm3d_lfmm3dparttarg_dev_noalloc(0, 2, n, dummy, 1, dummy, 0, None, None, 1, dummy, 1, dummy,
+                                              n, dummy, 1, dummy, 1, dummy, None)
+    assert ier == 256
+    # argument errors come first, as in the other entry points
+    assert lib.fmm3d_lfmm3dparttarg_dev_noalloc(0, 9, n, dummy, 1, dummy, 0, None, None, 1, dummy, 1, dummy,
+                                               n, dummy, 1, dummy, 1, dummy, None) == 32
