@@ -53,7 +53,14 @@ FLOPS_PER_PAIR = 65            # charge+dipole, pot+fld, rsqrt counted as 2
 # HBM bytes per k_p2p launch on this workload, from rocprofv3 FETCH_SIZE and
 # WRITE_SIZE runs calibrated against a 1 GiB device copy
 # (profiles/fmm_kernels_lap1M_helm500k.md): 2 x FETCH KiB + WRITE KiB.
-P2P_TRAFFIC_BYTES = {"fp64": (2 * 4062612 + 62846) * 1024, "fp32": (2 * 2585573 + 62852) * 1024}
+P2P_TRAFFIC_BYTES = {"fp64": (2 * 4238407 + 62846) * 1024, "fp32": (2 * 2568393 + 88269) * 1024}
+# VALU lane instructions per pair in k_p2p on this workload, from rocprofv3
+# SQ_INSTS_VALU (wave64 instructions x 64 / pairs; profiles/valu_mfma_counters_lap1M.md).
+# A wave64 VALU instruction occupies its SIMD for 4 cycles; the chip has
+# 256 CUs x 4 SIMDs at 2.4 GHz, so the issue-time ceiling of a launch is
+# pairs x this / 64 x 4 / (1024 x 2.4e9) seconds.
+P2P_VALU_INSTR_PER_PAIR = {"fp64": 43.2, "fp32": 44.3}
+SIMD_CYCLES_PER_S = 1024 * 2.4e9
 
 
 def make_inputs(n, seed):
@@ -454,6 +461,7 @@ def main():
         del os.environ["FMM3D_NEAR_STREAM"]
     p2p_alone_ms = float(np.mean([s_["p2p"] for s_ in st_alone]))
     achieved_alone = flops / (p2p_alone_ms * 1e-3) / 1e12
+    issue_s = pairs * P2P_VALU_INSTR_PER_PAIR["fp64" if best == "fp64" else "fp32"] / 64 * 4 / SIMD_CYCLES_PER_S
     dropin = dropin_leg(F, prec_of[best], src, q, ds, dv, trg)
     print("# drop-in (host arrays): %.1f ms/call, %.3e targets/s" % (dropin["ms_per_call"], dropin["targets_per_s"]),
           file=sys.stderr, flush=True)
@@ -491,11 +499,17 @@ def main():
                      "achieved_alone": achieved_alone, "frac_alone": achieved_alone / peak,
                      "p2p_ms": M["p2p_ms"], "p2p_ms_alone": p2p_alone_ms,
                      "traffic": P2P_TRAFFIC_BYTES["fp64" if best == "fp64" else "fp32"],
+                     "valu_issue_frac": issue_s / (M["p2p_ms"] * 1e-3),
+                     "valu_issue_frac_alone": issue_s / (p2p_alone_ms * 1e-3),
                      "note": "P2P is VALU bound (no matrix-core form); peak = the vector peak of the pair "
                              "arithmetic's type. Both launches run on the second stream beside the far-field "
                              "kernels and are timed with HIP events on that stream; *_alone: the same launches "
                              "with the near field on the caller's stream (FMM3D_NEAR_STREAM=0), 2 extra steps. "
-                             "algorithmic flops = %d pairs x %d" % (pairs, FLOPS_PER_PAIR)},
+                             "algorithmic flops = %d pairs x %d. valu_issue_frac: time the launches' VALU "
+                             "instructions (%.1f per pair, rocprofv3 SQ_INSTS_VALU) need at one wave64 instruction "
+                             "per 4 cycles per SIMD, 1024 SIMDs, 2.4 GHz, over the measured time; FP32 peak "
+                             "counts packed FP32, which this build does not emit"
+                             % (pairs, FLOPS_PER_PAIR, P2P_VALU_INSTR_PER_PAIR["fp64" if best == "fp64" else "fp32"])},
         "cpu_baseline": cpu,
         "dropin_host_arrays": dropin,
         "modes": {k: {kk: vv for kk, vv in v.items() if kk != "latencies_ms"} for k, v in modes.items()},

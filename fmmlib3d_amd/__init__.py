@@ -30,6 +30,7 @@ PREC_FP32 = 3     # FP32 near and far field
 EXPORTED = (
     "fmm3d_lfmm3dparttarg", "fmm3d_lfmm3dpartself", "fmm3d_l3dpartdirect",
     "fmm3d_lfmm3dparttarg_ex", "fmm3d_lfmm3dparttarg_dev", "fmm3d_lfmm3dparttarg_dev_noalloc",
+    "fmm3d_lfmm3dparttarg_hess",
     "fmm3d_last_timings",
     "fmm3d_last_precision", "fmm3d_version", "fmm3d_d3tstrcr",
     "fmm3d_hfmm3dparttarg", "fmm3d_hfmm3dpartself", "fmm3d_h3dpartdirect",
@@ -66,6 +67,9 @@ def load():
         lib.fmm3d_lfmm3dparttarg_dev.restype = i
         lib.fmm3d_lfmm3dparttarg_dev_noalloc.argtypes = [i] + targ + [vp]
         lib.fmm3d_lfmm3dparttarg_dev_noalloc.restype = i
+        lib.fmm3d_lfmm3dparttarg_hess.argtypes = [i, i, vp, i, vp, i, vp, vp, i, vp, i, vp, i, vp, i, vp, i, vp,
+                                                  i, vp, i, vp]
+        lib.fmm3d_lfmm3dparttarg_hess.restype = i
         lib.fmm3d_lfmm3dpartself.argtypes = [i, i, vp, i, vp, i, vp, vp, i, vp, i, vp]
         lib.fmm3d_lfmm3dpartself.restype = i
         lib.fmm3d_l3dpartdirect.argtypes = [i, vp, i, vp, i, vp, vp, i, vp, i, vp, i, vp, i, vp, i, vp]
@@ -90,7 +94,7 @@ def load():
         lib.fmm3d_server_submit.restype = ll
         lib.fmm3d_server_wait.argtypes = [vp, ll, i, vp, vp, vp, vp, vp, vp]
         lib.fmm3d_server_wait.restype = i
-        lib.fmm3d_server_stats.argtypes = [vp, vp, vp, vp]
+        lib.fmm3d_server_stats.argtypes = [vp, vp, vp, vp, vp]
         lib.fmm3d_server_stats.restype = None
         lib.fmm3d_server_destroy.argtypes = [vp]
         lib.fmm3d_server_destroy.restype = None
@@ -160,6 +164,29 @@ def lfmm3dparttarg(iprec, source, charge=None, dipstr=None, dipvec=None, target=
         int(ds is not None), _ptr(ds), _ptr(dv), int(bool(ifpot)), _ptr(pot), int(bool(iffld)),
         _ptr(fld), nt, _ptr(trg), int(pt is not None), _ptr(pt), int(ft is not None), _ptr(ft))
     return ier, pot, fld, pt, ft
+
+
+def lfmm3dparttarg_hess(iprec, source, charge=None, dipstr=None, dipvec=None, target=None,
+                        ifpot=1, iffld=1, ifhess=1, ifpottarg=1, iffldtarg=1, ifhesstarg=1):
+    """Laplace FMM with Hessians (FP64).  Returns (ier, pot, fld, hess,
+    pottarg, fldtarg, hesstarg); hess is (n, 6) complex, per point (pot_xx,
+    pot_yy, pot_zz, pot_xy, pot_xz, pot_yz), the order of the reference's
+    lpotfld3dhess (src/laprouts3d.f:5442); outputs whose flag is 0 are None."""
+    lib = load()
+    src, q, ds, dv, trg = _prep(source, charge, dipstr, dipvec, target)
+    ns = src.shape[0]
+    nt = 0 if trg is None else trg.shape[0]
+    pot = np.zeros(ns, np.complex128) if ifpot else None
+    fld = np.zeros((ns, 3), np.complex128) if iffld else None
+    hs = np.zeros((ns, 6), np.complex128) if ifhess else None
+    pt = np.zeros(nt, np.complex128) if (ifpottarg and nt) else None
+    ft = np.zeros((nt, 3), np.complex128) if (iffldtarg and nt) else None
+    ht = np.zeros((nt, 6), np.complex128) if (ifhesstarg and nt) else None
+    ier = lib.fmm3d_lfmm3dparttarg_hess(
+        int(iprec), ns, _ptr(src), int(q is not None), _ptr(q), int(ds is not None), _ptr(ds), _ptr(dv),
+        int(bool(ifpot)), _ptr(pot), int(bool(iffld)), _ptr(fld), int(hs is not None), _ptr(hs),
+        nt, _ptr(trg), int(pt is not None), _ptr(pt), int(ft is not None), _ptr(ft), int(ht is not None), _ptr(ht))
+    return ier, pot, fld, hs, pt, ft, ht
 
 
 def lfmm3dpartself(iprec, source, charge=None, dipstr=None, dipvec=None, ifpot=1, iffld=1,
@@ -344,8 +371,9 @@ class Server:
         q = ctypes.c_longlong(0)
         r = ctypes.c_int(0)
         b = ctypes.c_longlong(0)
-        self._lib.fmm3d_server_stats(self._h, ctypes.byref(q), ctypes.byref(r), ctypes.byref(b))
-        return {"queued": q.value, "running": r.value, "bytes_in_use": b.value}
+        fz = ctypes.c_longlong(0)
+        self._lib.fmm3d_server_stats(self._h, ctypes.byref(q), ctypes.byref(r), ctypes.byref(b), ctypes.byref(fz))
+        return {"queued": q.value, "running": r.value, "bytes_in_use": b.value, "fused_requests": fz.value}
 
     def close(self):
         if self._h:

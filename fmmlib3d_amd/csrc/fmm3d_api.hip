@@ -85,11 +85,13 @@ int check_ptrs(int ns, const double* source, int ifcharge, const double* charge,
 int fmm_host(const double* zk, int prec_mode, int iprec, int ns, const double* source, int ifcharge,
              const double* charge, int ifdipole, const double* dipstr, const double* dipvec,
              int ifpot, double* pot, int iffld, double* fld, int nt, const double* target,
-             int ifpottarg, double* pottarg, int iffldtarg, double* fldtarg) {
+             int ifpottarg, double* pottarg, int iffldtarg, double* fldtarg, int ifhess = 0,
+             double* hess = nullptr, int ifhesstarg = 0, double* hesstarg = nullptr) {
   if (int e = check_args(zk, prec_mode, iprec, ns, nt)) return e;
   if (int e = check_ptrs(ns, source, ifcharge, charge, ifdipole, dipstr, dipvec, ifpot, pot, iffld, fld, nt,
                          target, ifpottarg, pottarg, iffldtarg, fldtarg))
     return e;
+  if ((ifhess == 1 && !hess) || (nt > 0 && ifhesstarg == 1 && !hesstarg)) return 128;
   Context& ctx = default_context();
   std::lock_guard<std::mutex> lock(ctx.mu);
   return guarded([&] {
@@ -110,12 +112,18 @@ int fmm_host(const double* zk, int prec_mode, int iprec, int ns, const double* s
     if (c.iffld) c.fld_dev = (double*)ctx.arena.get("out_fld", 6 * sizeof(double) * ns);
     if (c.ifpottarg) c.pottarg_dev = (double*)ctx.arena.get("out_pt", 2 * sizeof(double) * nt);
     if (c.iffldtarg) c.fldtarg_dev = (double*)ctx.arena.get("out_ft", 6 * sizeof(double) * nt);
+    c.ifhess = ifhess == 1;
+    c.ifhesstarg = nt > 0 && ifhesstarg == 1;
+    if (c.ifhess) c.hess_dev = (double*)ctx.arena.get("out_hess", 12 * sizeof(double) * ns);
+    if (c.ifhesstarg) c.hesstarg_dev = (double*)ctx.arena.get("out_ht", 12 * sizeof(double) * nt);
     int ier = run_solve(ctx, zk, c, prec_mode);
     if (ier) return ier;
     if (c.ifpot) download(ctx, pot, c.pot_dev, 2 * (size_t)ns);
     if (c.iffld) download(ctx, fld, c.fld_dev, 6 * (size_t)ns);
     if (c.ifpottarg) download(ctx, pottarg, c.pottarg_dev, 2 * (size_t)nt);
     if (c.iffldtarg) download(ctx, fldtarg, c.fldtarg_dev, 6 * (size_t)nt);
+    if (c.ifhess) download(ctx, hess, c.hess_dev, 12 * (size_t)ns);
+    if (c.ifhesstarg) download(ctx, hesstarg, c.hesstarg_dev, 12 * (size_t)nt);
     FMM_HIP(hipStreamSynchronize(ctx.stream));
     return 0;
   });
@@ -133,6 +141,17 @@ int fmm3d_lfmm3dparttarg(int iprec, int nsource, const double* source, int ifcha
   return fmm_host(nullptr, FMM3D_PREC_FP64, iprec, nsource, source, ifcharge, charge, ifdipole, dipstr,
                   dipvec, ifpot, pot, iffld, fld, ntarget, target, ifpottarg, pottarg, iffldtarg,
                   fldtarg);
+}
+
+int fmm3d_lfmm3dparttarg_hess(int iprec, int nsource, const double* source, int ifcharge,
+                              const double* charge, int ifdipole, const double* dipstr,
+                              const double* dipvec, int ifpot, double* pot, int iffld, double* fld,
+                              int ifhess, double* hess, int ntarget, const double* target,
+                              int ifpottarg, double* pottarg, int iffldtarg, double* fldtarg,
+                              int ifhesstarg, double* hesstarg) {
+  return fmm_host(nullptr, 0, iprec, nsource, source, ifcharge, charge, ifdipole, dipstr, dipvec, ifpot, pot,
+                  iffld, fld, ntarget, target, ifpottarg, pottarg, iffldtarg, fldtarg, ifhess, hess, ifhesstarg,
+                  hesstarg);
 }
 
 int fmm3d_lfmm3dparttarg_ex(int prec_mode, int iprec, int nsource, const double* source,

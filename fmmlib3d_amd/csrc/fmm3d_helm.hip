@@ -149,21 +149,21 @@ HelmCache* helm_tables(Context& ctx, cplx zk, double size, int nlev, double eps,
 
 template <typename TN, bool CHG, bool DIP, bool FLD>
 void hp2p(const DevPlan& P, const int* items, int n, int kind, double kre, double kim, const double* spos,
-          const TN* str, const double* tpos, double* pot, double* fo, hipStream_t s) {
+          const TN* str, const double* str64, const double* tpos, double* pot, double* fo, hipStream_t s) {
   DevPlan Q = P;
   Q.p2p_items = items;
   Q.n_p2p = n;
   hipLaunchKernelGGL((k_hp2p<TN, CHG, DIP, FLD>), dim3(item_groups(n)), dim3(kP2PBlock * kItemWaves), 0, s, Q, kre, kim,
-                     spos, str, tpos, kind, pot, fo);
+                     spos, str, str64, tpos, kind, pot, fo);
 }
 
 template <typename TN>
 void launch_hp2p(const DevPlan& P, const int* items, int n, int kind, double kre, double kim,
-                 const double* spos, const TN* str, const double* tpos, int ifcharge, int ifdipole,
+                 const double* spos, const TN* str, const double* str64, const double* tpos, int ifcharge, int ifdipole,
                  bool fld, double* pot, double* fo, hipStream_t s) {
   if (n == 0) return;
 #define HCASE(C, Dp, F) \
-  if (ifcharge == C && ifdipole == Dp && fld == F) return hp2p<TN, C, Dp, F>(P, items, n, kind, kre, kim, spos, str, tpos, pot, fo, s);
+  if (ifcharge == C && ifdipole == Dp && fld == F) return hp2p<TN, C, Dp, F>(P, items, n, kind, kre, kim, spos, str, str64, tpos, pot, fo, s);
   HCASE(1, 0, false) HCASE(1, 0, true) HCASE(0, 1, false) HCASE(0, 1, true) HCASE(1, 1, false) HCASE(1, 1, true)
 #undef HCASE
 }
@@ -341,9 +341,9 @@ int helm_solve(Context& ctx, const LapCall& c) {
   const int* dIB = upload_ints("plan_ib", FB.I, s2);
   devplan_near(P, O, dIB);
   if (f.ifcharge || f.ifdipole) {
-    launch_hp2p<TN>(P, dIB + O.p2p0, O.n_p2p0, 0, Hd.kre, Hd.kim, spos, strn, tpos, f.ifcharge, f.ifdipole,
+    launch_hp2p<TN>(P, dIB + O.p2p0, O.n_p2p0, 0, Hd.kre, Hd.kim, spos, strn, str64, tpos, f.ifcharge, f.ifdipole,
                     f.iffld != 0, pot_sn, fld_sn, s2);
-    launch_hp2p<TN>(P, dIB + O.p2p1, O.n_p2p1, 1, Hd.kre, Hd.kim, spos, strn, tpos, f.ifcharge, f.ifdipole,
+    launch_hp2p<TN>(P, dIB + O.p2p1, O.n_p2p1, 1, Hd.kre, Hd.kim, spos, strn, str64, tpos, f.ifcharge, f.ifdipole,
                     f.iffldtarg != 0, pot_tn, fld_tn, s2);
   }
   tn.mark(1);

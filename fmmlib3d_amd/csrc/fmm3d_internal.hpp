@@ -4,6 +4,8 @@
 
 namespace fmm3d {
 
+struct Octree;
+
 // One Laplace evaluation.  *_host pointers are host copies of the
 // coordinates (the octree is built on the host); *_dev pointers are device
 // buffers in the reference layouts (complex as interleaved doubles).
@@ -15,12 +17,29 @@ struct LapCall {
                *trg_dev = nullptr;
   double *pot_dev = nullptr, *fld_dev = nullptr, *pottarg_dev = nullptr, *fldtarg_dev = nullptr;
   double zk_re = 0.0, zk_im = 0.0;  // Helmholtz wavenumber (unused for Laplace)
+  // Laplace only: a finished tree with its lists (a forest of several
+  // requests, merge_forest) used instead of building one from the points;
+  // and whether the launches from the gather to the scatter are captured in
+  // a HIP graph and launched as one (no stage timings then)
+  const Octree* tree_in = nullptr;
+  bool graph = false;
+  // with tree_in: whether its lists are still to be built (a tree built for
+  // this same call, handed on to another precision), and the device
+  // permutations of a tree built on the device (else its host ones are used)
+  bool tree_lists_todo = false;
+  const int *sperm_in = nullptr, *tperm_in = nullptr;
+  // Laplace, FP64 only: Hessians (pot_xx, pot_yy, pot_zz, pot_xy, pot_xz,
+  // pot_yz; 6 complex per point) at the sources and at the targets
+  int ifhess = 0, ifhesstarg = 0;
+  double *hess_dev = nullptr, *hesstarg_dev = nullptr;
 };
 
 // The precision mode a call runs in: the requested FMM3D_PREC_* mode while
 // FP32 can meet the tolerance of iprec, FP64 throughout at iprec >= 3 (a
 // tolerance of 0.5e-9 or tighter, src/lfmm3dpart.f:277-285, which FP32
-// pair arithmetic and FP32 translations cannot reach).
+// pair arithmetic and FP32 translations cannot reach).  At iprec 2 the
+// Laplace driver also drops FP32 expansions (FMM3D_PREC_FAR32 bit) on a
+// tree whose leaves lie on more than one level, once the tree is built.
 inline int effective_precision(int iprec, int mode) { return iprec >= 3 ? 0 : mode; }
 
 // the mode the last FMM call of this thread ran in

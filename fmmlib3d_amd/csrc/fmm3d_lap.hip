@@ -8,6 +8,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <cstdio>
 #include <cstdlib>
 #include <cstring>
 #include <map>
@@ -19,6 +20,8 @@
 
 #include "fmm3d_internal.hpp"
 #include "lap_kernels.hip"
+#include "m2l_mfma.hpp"
+#include "hess_kernels.hpp"
 #include "octree.hpp"
 #include "plan.hpp"
 #include "runtime.hpp"
@@ -42,6 +45,9 @@ struct LapTabSet {
   size_t off_g[4] = {0, 0, 0, 0};  // grouped M2L tables: rot_half, z_m2l_t (doubles); their offsets (longs)
   size_t off_f = 0;
   long zt_size = 0;
+  size_t off_m[4] = {0, 0, 0, 0};    // MFMA tables: mf_rot, mf_z (doubles); mf_rot_off, mf_z_off (longs)
+  size_t off_mf[2] = {0, 0};         // their FP32 copies
+  long mf_rot_back = 0, mf_z_size = 0;
 };
 
 // The host tables of order p, built once per process (tens to a hundred ms
@@ -91,11 +97,30 @@ std::shared_ptr<const LapTabSet> lap_host_tables(int p, double eps) {
   T->off_g[3] = T->L.size();
   T->L.insert(T->L.end(), H.zt_off.begin(), H.zt_off.end());
   T->zt_size = H.zt_size;
+  // MFMA-ordered M2L tables (64-value blocks, 512-byte aligned starts)
+  while (Dd.size() % 64) Dd.push_back(0.0);
+  T->off_m[0] = Dd.size();
+  Dd.insert(Dd.end(), H.mf_rot.begin(), H.mf_rot.end());
+  while (Dd.size() % 64) Dd.push_back(0.0);
+  T->off_m[1] = Dd.size();
+  Dd.insert(Dd.end(), H.mf_z.begin(), H.mf_z.end());
+  T->off_m[2] = T->L.size();
+  T->L.insert(T->L.end(), H.mf_rot_off.begin(), H.mf_rot_off.end());
+  T->off_m[3] = T->L.size();
+  T->L.insert(T->L.end(), H.mf_z_off.begin(), H.mf_z_off.end());
+  T->mf_rot_back = H.mf_rot_back;
+  T->mf_z_size = H.mf_z_size;
   // FP32 copies of the grouped-M2L tables for the FP32 far field
   T->F.assign(H.rot_half.begin(), H.rot_half.end());
   while (T->F.size() % 4) T->F.push_back(0.0f);
   T->off_f = T->F.size();
   T->F.insert(T->F.end(), H.z_m2l_t.begin(), H.z_m2l_t.end());
+  while (T->F.size() % 64) T->F.push_back(0.0f);
+  T->off_mf[0] = T->F.size();
+  T->F.insert(T->F.end(), H.mf_rot.begin(), H.mf_rot.end());
+  while (T->F.size() % 64) T->F.push_back(0.0f);
+  T->off_mf[1] = T->F.size();
+  T->F.insert(T->F.end(), H.mf_z.begin(), H.mf_z.end());
   for (int q = 0; q < 7; q++) T->off[q] = o[q];
   T->off[7] = o[7];
   T->off_re = o[8];
@@ -103,23 +128,25 @@ std::shared_ptr<const LapTabSet> lap_host_tables(int p, double eps) {
   return T;
 }
 
-// The device tables of order p for this context: uploaded on first use of
-// the order and kept (each order under its own arena names).
+// The device tables of (order p, tolerance eps) for this context: uploaded
+// on first use of the pair and kept (each pair under its own arena names).
 DevTables ensure_tables(Context& ctx, int p, double eps) {
   const auto hs = lap_host_tables(p, eps);
   const LapTabSet& H = *hs;
-  const std::string sfx = "." + std::to_string(p);
+  char key[48];
+  std::snprintf(key, sizeof key, ".%d.%a", p, eps);
+  const std::string sfx = key;
   int* di = (int*)ctx.arena.get("tab_i" + sfx, H.I.size() * sizeof(int));
   double* dd = (double*)ctx.arena.get("tab_d" + sfx, H.D.size() * sizeof(double));
   long* dl = (long*)ctx.arena.get("tab_l" + sfx, H.L.size() * sizeof(long));
   float* df = (float*)ctx.arena.get("tab_f" + sfx, H.F.size() * sizeof(float));
-  if (!ctx.lap_tabs.count(p)) {
+  if (!ctx.lap_tabs.count({p, eps})) {
     FMM_HIP(hipMemcpyAsync(di, H.I.data(), H.I.size() * sizeof(int), hipMemcpyHostToDevice, ctx.stream));
     FMM_HIP(hipMemcpyAsync(dd, H.D.data(), H.D.size() * sizeof(double), hipMemcpyHostToDevice, ctx.stream));
     FMM_HIP(hipMemcpyAsync(dl, H.L.data(), H.L.size() * sizeof(long), hipMemcpyHostToDevice, ctx.stream));
     FMM_HIP(hipMemcpyAsync(df, H.F.data(), H.F.size() * sizeof(float), hipMemcpyHostToDevice, ctx.stream));
     FMM_HIP(hipStreamSynchronize(ctx.stream));
-    ctx.lap_tabs.insert(p);
+    ctx.lap_tabs.insert({p, eps});
   }
   DevTables T;
   T.p = p;
@@ -138,13 +165,17 @@ DevTables ensure_tables(Context& ctx, int p, double eps) {
   T.zt_size = H.zt_size;
   T.rot_half_f = df;
   T.z_m2l_t_f = df + H.off_f;
+  T.mf_rot = dd + H.off_m[0]; T.mf_z = dd + H.off_m[1];
+  T.mf_rot_f = df + H.off_mf[0]; T.mf_z_f = df + H.off_mf[1];
+  T.mf_rot_off = dl + H.off_m[2]; T.mf_z_off = dl + H.off_m[3];
+  T.mf_rot_back = H.mf_rot_back; T.mf_z_size = H.mf_z_size;
   return T;
 }
 
 template <typename TN>
 void launch_p2p(const DevPlan& P, const int* items, int n, int kind, const double* spos,
-                const TN* str, const double* tpos, int ifcharge, int ifdipole, bool fld, double* pot,
-                double* fo, hipStream_t s) {
+                const TN* str, const double* str64, const double* tpos, int ifcharge, int ifdipole, bool fld,
+                double* pot, double* fo, hipStream_t s) {
   if (n == 0) return;
   DevPlan Q = P;
   Q.p2p_items = items;
@@ -152,12 +183,34 @@ void launch_p2p(const DevPlan& P, const int* items, int n, int kind, const doubl
   dim3 g(item_groups(n)), b(kP2PBlock * kItemWaves);
 #define P2P_CASE(C, Dp, F)                                                                        \
   if (ifcharge == C && ifdipole == Dp && fld == F) {                                             \
-    hipLaunchKernelGGL((k_p2p<TN, C, Dp, F>), g, b, 0, s, Q, spos, str, tpos, kind, pot, fo);    \
+    hipLaunchKernelGGL((k_p2p<TN, C, Dp, F>), g, b, 0, s, Q, spos, str, str64, tpos, kind, pot, fo); \
     return;                                                                                       \
   }
   P2P_CASE(1, 0, false) P2P_CASE(1, 0, true) P2P_CASE(0, 1, false) P2P_CASE(0, 1, true)
   P2P_CASE(1, 1, false) P2P_CASE(1, 1, true)
 #undef P2P_CASE
+}
+
+// Factor table of order p + 1 for the Hessian kernels (degrees up to p + 2),
+// built once per process and order, uploaded once per context and order.
+const double* hess_factors(Context& ctx, int p) {
+  static std::mutex mu;
+  static std::map<int, std::vector<double>> cache;
+  const std::vector<double>* h;
+  {
+    std::lock_guard<std::mutex> lk(mu);
+    auto it = cache.find(p);
+    if (it == cache.end()) {
+      std::vector<double> v;
+      build_fac_table(p + 1, v);
+      it = cache.emplace(p, std::move(v)).first;
+    }
+    h = &it->second;
+  }
+  const std::string name = "hess_fac." + std::to_string(p);
+  double* d = (double*)ctx.arena.get(name, h->size() * sizeof(double));
+  FMM_HIP(hipMemcpyAsync(d, h->data(), h->size() * sizeof(double), hipMemcpyHostToDevice, ctx.stream));
+  return d;
 }
 
 }  // namespace
@@ -171,15 +224,37 @@ int lap_solve(Context& ctx, const LapCall& c) {
   const int ns = c.ns, nt = c.nt;
   const Flags f{c.ifcharge ? 1 : 0, c.ifdipole ? 1 : 0, c.ifpot ? 1 : 0, c.iffld ? 1 : 0,
                 nt > 0 && c.ifpottarg ? 1 : 0, nt > 0 && c.iffldtarg ? 1 : 0};
-  thread_local Octree tree;  // storage reused across calls
+  thread_local Octree own_tree;  // storage reused across calls
+  Octree& tree = c.tree_in ? const_cast<Octree&>(*c.tree_in) : own_tree;
+  const bool prebuilt = c.tree_in != nullptr;
   const int* dsperm = nullptr;
   const int* dtperm = nullptr;
   // the interaction lists are built after the first kernels are queued
-  if (device_tree_enabled()) {
+  if (prebuilt) {
+    if (tree.ns != ns || tree.nt != nt) return 128;
+    dsperm = c.sperm_in;
+    dtperm = c.tperm_in;
+  } else if (device_tree_enabled()) {
     if (build_octree_device(ctx, ctx.stream, c.src_dev, ns, c.trg_dev, nt, nbox, tree, &dsperm, &dtperm, false) != 0)
       return 4;
   } else if (build_octree(c.src_host, ns, c.trg_host, nt, nbox, tree, false) != 0) {
     return 4;
+  }
+  const bool lists_done = prebuilt && !c.tree_lists_todo;
+  // FP32 expansions at iprec 2 (tolerance 0.5e-6) only where every leaf lies
+  // on one level.  Boxes of different sizes (lists 3 and 4, adaptive trees of
+  // clustered points) evaluate expansions near their radius of convergence,
+  // where the FP32 rounding of the large high-degree terms reached 6e-7 of
+  // the potential on clustered inputs; such a tree runs the far field in
+  // FP64 (the same tree, handed on).
+  if (std::is_same<T, float>::value && c.iprec == 2 && !tree.leaves_on_one_level()) {
+    LapCall c2 = c;
+    c2.tree_in = &tree;
+    c2.tree_lists_todo = !lists_done;
+    c2.sperm_in = dsperm;
+    c2.tperm_in = dtperm;
+    last_precision() &= ~2;
+    return lap_solve<double, TN>(ctx, c2);
   }
   auto t_tree = std::chrono::steady_clock::now();
   const int p = laplace_nterms(eps);
@@ -191,8 +266,13 @@ int lap_solve(Context& ctx, const LapCall& c) {
   build_plan_boxes(tree, f, FA, O, 0);
   auto t1 = std::chrono::steady_clock::now();
   hipStream_t s = ctx.stream;
+  // stage timers (event records), not while the launches are captured
+  const bool timed = !c.graph;
   Timer tm;
   tm.start(s, 11);
+  auto mark = [&](int i) {
+    if (timed) tm.mark(i);
+  };
   // ---- device buffers; plan uploads go through pinned staging buffers so
   // that they do not wait for the kernels already queued
   auto upload_ints = [&](const char* name, const std::vector<int>& v, hipStream_t st) {
@@ -234,6 +314,25 @@ int lap_solve(Context& ctx, const LapCall& c) {
   const int* tperm = dsperm ? dtperm : dI + O.tperm;
   P.src_perm = sperm;
   P.trg_perm = tperm;
+  // graph mode: every launch and copy from here to the scatter, on both
+  // streams (the near-field stream joins through the events), is captured
+  // and then launched as one graph; buffers are zeroed by kernels, not
+  // memset nodes
+  hipGraph_t graph = nullptr;
+  struct CaptureGuard {  // ends a capture that an error left open
+    hipStream_t s;
+    bool on = false;
+    ~CaptureGuard() {
+      if (!on) return;
+      hipGraph_t g = nullptr;
+      if (hipStreamEndCapture(s, &g) == hipSuccess && g) (void)hipGraphDestroy(g);
+      (void)hipGetLastError();
+    }
+  } capture{s};
+  if (c.graph) {
+    FMM_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeRelaxed));
+    capture.on = true;
+  }
   hipLaunchKernelGGL(k_gather_sources, dim3(blocks(ns, 256)), dim3(256), 0, s, ns, sperm, c.src_dev,
                      c.q_dev, c.ds_dev, c.dv_dev, f.ifcharge, f.ifdipole, spos, str64, str32);
   if (nt > 0)
@@ -255,7 +354,7 @@ int lap_solve(Context& ctx, const LapCall& c) {
     zero(fld_tn, 6 * (size_t)nt);
   }
   FMM_HIP(hipEventRecord(ctx.aux_ev[0], s));  // inputs gathered, sums zeroed
-  tm.mark(1);
+  mark(1);
   const double* fac = Tb.fac;
   // ---- STEP 1: P2M, STEP 2: P2L (list 4)
   // k_form2 (per-thread sums) when its half table fits I1 (or 2 I1) entries
@@ -293,15 +392,15 @@ int lap_solve(Context& ctx, const LapCall& c) {
     }
   };
   form(std::false_type{}, P.n_p2m, mp);
-  tm.mark(2);
+  mark(2);
   // ---- STEP 3: M2M, bottom-up
   const size_t lds_ps = sizeof(T) * 4 * K;
   for (auto& g : O.m2m_groups)
     hipLaunchKernelGGL((k_m2m<T>), dim3(g.second), dim3(256), lds_ps, s, P, Tb, g.first, mp);
-  tm.mark(3);
+  mark(3);
   // ---- while the device runs P2M and M2M: list 1 and the P2P plan (part B1)
   auto t_lists0 = std::chrono::steady_clock::now();
-  tree_lists_near(tree);
+  if (!lists_done) tree_lists_near(tree);
   build_plan_near(tree, f, FB, O);
   // the P2P work items are addressed explicitly: they must lie in part B1
   if (O.p2p0 + 3 * (size_t)O.n_p2p0 > FB.I.size() || O.p2p1 + 3 * (size_t)O.n_p2p1 > FB.I.size())
@@ -311,84 +410,107 @@ int lap_solve(Context& ctx, const LapCall& c) {
   // are gathered; its plan part (B1) is uploaded on that stream
   FMM_HIP(hipStreamWaitEvent(s2, ctx.aux_ev[0], 0));
   Timer tn;
-  tn.start(s2, 2);
+  if (timed) tn.start(s2, 2);
   const int* dIB = upload_ints("plan_ib", FB.I, s2);  // part B1: offsets in O from build_plan_near
   devplan_near(P, O, dIB);
+  const int* dIB_near = dIB;
   if (f.ifcharge || f.ifdipole) {
-    launch_p2p<TN>(P, dIB + O.p2p0, O.n_p2p0, 0, spos, strn, tpos, f.ifcharge, f.ifdipole,
+    launch_p2p<TN>(P, dIB + O.p2p0, O.n_p2p0, 0, spos, strn, str64, tpos, f.ifcharge, f.ifdipole,
                    f.iffld != 0, pot_sn, fld_sn, s2);
-    launch_p2p<TN>(P, dIB + O.p2p1, O.n_p2p1, 1, spos, strn, tpos, f.ifcharge, f.ifdipole,
+    launch_p2p<TN>(P, dIB + O.p2p1, O.n_p2p1, 1, spos, strn, str64, tpos, f.ifcharge, f.ifdipole,
                    f.iffldtarg != 0, pot_tn, fld_tn, s2);
   }
-  tn.mark(1);
+  if (timed) tn.mark(1);
   FMM_HIP(hipEventRecord(ctx.aux_ev[1], s2));  // near field done
-  tm.mark(4);
+  mark(4);
   // ---- while the device runs P2P: lists 2-4 and the far-field plan (part B2)
   auto t_lists2 = std::chrono::steady_clock::now();
-  tree_lists_far(tree);
+  if (!lists_done) tree_lists_far(tree);
   thread_local Flat FC;
   FC.I.clear(); FC.D.clear();
   build_plan_far(tree, f, FC, O);
   devplan_far(P, O, upload_ints("plan_ic", FC.I, s));
   auto t_lists3 = std::chrono::steady_clock::now();
   form(std::true_type{}, P.n_p2l, lo);
-  tm.mark(5);
+  mark(5);
   // ---- STEP 4: M2L
   if (P.n_m2l) {
     // Default: pairs grouped by translation class (k_m2l_grp).
     // FMM3D_M2L_ROTATION=pairs selects one pair at a time with per-class
-    // rotation blocks, =factored the factored rotations.
+    // rotation blocks, =factored the factored rotations, =mfma the grouped
+    // kernel on the matrix cores.
     static const int m2l_form = [] {
       const char* v = std::getenv("FMM3D_M2L_ROTATION");
       if (v && std::strcmp(v, "factored") == 0) return 2;
       if (v && std::strcmp(v, "pairs") == 0) return 1;
+      if (v && std::strcmp(v, "mfma") == 0) return 3;
       return 0;
     }();
-    const bool lds = sizeof(T) * 6 * (size_t)K <= max_lds_bytes(ctx);
-    const size_t sh = lds ? sizeof(T) * 6 * K : lds_ps;
-    // launch shape of k_m2l_grp: (largest group, threads, items per thread);
-    // the work items of one stage must fit, the group's expansions must fit LDS
-    static const int m2l_shape = [] {
-      const char* v = std::getenv("FMM3D_M2L_SHAPE");
-      if (v && std::strcmp(v, "g2t256i1") == 0) return 1;
-      if (v && std::strcmp(v, "g4t512i1") == 0) return 2;
-      return 0;
+    // FMM3D_M2L_ROTATION=mfma: the grouped M2L on the matrix cores
+    // (k_m2l_mfma) wherever its LDS fits and p <= 47 (three row tiles);
+    // FMM3D_M2L_MFMA_THREADS=512 runs eight waves per workgroup.
+    static const int mf_nt = [] {
+      const char* v = std::getenv("FMM3D_M2L_MFMA_THREADS");
+      return (v && std::strcmp(v, "512") == 0) ? 512 : 256;
     }();
-    const int gcap = m2l_shape == 1 ? 2 : kM2LGroup;
-    const int nt = m2l_shape == 2 ? 512 : 256, items = m2l_shape == 0 ? 2 : 1;
-    int nI = 0;
-    for (int n = 0; n <= p; n++) nI += n / 4 + 1;
-    int gmax = 0;
-    // LDS: six int arrays, the local, a guard of 8 values, g members of (p+4)^2
-    auto m2l_lds = [&](int g) {
-      return (6 * kM2LChunk + (size_t)grp_kn_ints(p)) * sizeof(int) +
-             sizeof(T) * (2 * (size_t)kM2LGroup * (p + 1) + 2 * (size_t)K + 8 + (size_t)g * grp_member_stride<T>(p));
-    };
-    for (int g = gcap; g >= 1 && !gmax; g--) {
-      const int gs = g > 2 ? 4 : g;
-      if (nI * gs <= items * nt && m2l_lds(g) <= max_lds_bytes(ctx)) gmax = g;
-    }
-    if (m2l_form == 0 && gmax > 0) {
-      const size_t need = m2l_lds(gmax);
-      if (m2l_shape == 0)
-        hipLaunchKernelGGL((k_m2l_grp<T, 256, 2>), dim3(P.n_m2l), dim3(256), need, s, P, Tb, gmax, mp, lo);
-      else if (m2l_shape == 1)
-        hipLaunchKernelGGL((k_m2l_grp<T, 256, 1>), dim3(P.n_m2l), dim3(256), need, s, P, Tb, gmax, mp, lo);
-      else
-        hipLaunchKernelGGL((k_m2l_grp<T, 512, 1>), dim3(P.n_m2l), dim3(512), need, s, P, Tb, gmax, mp, lo);
-    } else if (m2l_form != 2) {
-      if (lds) hipLaunchKernelGGL((k_m2l<T, true>), dim3(P.n_m2l), dim3(256), sh, s, P, Tb, mp, lo);
-      else hipLaunchKernelGGL((k_m2l<T, false>), dim3(P.n_m2l), dim3(256), sh, s, P, Tb, mp, lo);
+    const size_t mf_lds = m2l_mfma_lds<T>(p);
+    if (m2l_form == 3 && p <= 47 && mf_lds <= max_lds_bytes(ctx)) {
+      const int mrt = p / 16 + 1;
+#define M2L_MFMA(NT_, R_) \
+  hipLaunchKernelGGL((k_m2l_mfma<T, NT_, R_>), dim3(P.n_m2l), dim3(NT_), mf_lds, s, P, Tb, mp, lo)
+      if (mf_nt == 512) {
+        if (mrt == 1) M2L_MFMA(512, 1); else if (mrt == 2) M2L_MFMA(512, 2); else M2L_MFMA(512, 3);
+      } else {
+        if (mrt == 1) M2L_MFMA(256, 1); else if (mrt == 2) M2L_MFMA(256, 2); else M2L_MFMA(256, 3);
+      }
+#undef M2L_MFMA
     } else {
-      if (lds) hipLaunchKernelGGL((k_m2l_fact<T, true>), dim3(P.n_m2l), dim3(256), sh, s, P, Tb, mp, lo);
-      else hipLaunchKernelGGL((k_m2l_fact<T, false>), dim3(P.n_m2l), dim3(256), sh, s, P, Tb, mp, lo);
+      const bool lds = sizeof(T) * 6 * (size_t)K <= max_lds_bytes(ctx);
+      const size_t sh = lds ? sizeof(T) * 6 * K : lds_ps;
+      // launch shape of k_m2l_grp: (largest group, threads, items per thread);
+      // the work items of one stage must fit, the group's expansions must fit LDS
+      static const int m2l_shape = [] {
+        const char* v = std::getenv("FMM3D_M2L_SHAPE");
+        if (v && std::strcmp(v, "g2t256i1") == 0) return 1;
+        if (v && std::strcmp(v, "g4t512i1") == 0) return 2;
+        return 0;
+      }();
+      const int gcap = m2l_shape == 1 ? 2 : kM2LGroup;
+      const int nt = m2l_shape == 2 ? 512 : 256, items = m2l_shape == 0 ? 2 : 1;
+      int nI = 0;
+      for (int n = 0; n <= p; n++) nI += n / 4 + 1;
+      int gmax = 0;
+      // LDS: six int arrays, the local, a guard of 8 values, g members of (p+4)^2
+      auto m2l_lds = [&](int g) {
+        return (6 * kM2LChunk + (size_t)grp_kn_ints(p)) * sizeof(int) +
+               sizeof(T) * (2 * (size_t)kM2LGroup * (p + 1) + 2 * (size_t)K + 8 + (size_t)g * grp_member_stride<T>(p));
+      };
+      for (int g = gcap; g >= 1 && !gmax; g--) {
+        const int gs = g > 2 ? 4 : g;
+        if (nI * gs <= items * nt && m2l_lds(g) <= max_lds_bytes(ctx)) gmax = g;
+      }
+      if ((m2l_form == 0 || m2l_form == 3) && gmax > 0) {
+        const size_t need = m2l_lds(gmax);
+        if (m2l_shape == 0)
+          hipLaunchKernelGGL((k_m2l_grp<T, 256, 2>), dim3(P.n_m2l), dim3(256), need, s, P, Tb, gmax, mp, lo);
+        else if (m2l_shape == 1)
+          hipLaunchKernelGGL((k_m2l_grp<T, 256, 1>), dim3(P.n_m2l), dim3(256), need, s, P, Tb, gmax, mp, lo);
+        else
+          hipLaunchKernelGGL((k_m2l_grp<T, 512, 1>), dim3(P.n_m2l), dim3(512), need, s, P, Tb, gmax, mp, lo);
+      } else if (m2l_form != 2) {
+        if (lds) hipLaunchKernelGGL((k_m2l<T, true>), dim3(P.n_m2l), dim3(256), sh, s, P, Tb, mp, lo);
+        else hipLaunchKernelGGL((k_m2l<T, false>), dim3(P.n_m2l), dim3(256), sh, s, P, Tb, mp, lo);
+      } else {
+        if (lds) hipLaunchKernelGGL((k_m2l_fact<T, true>), dim3(P.n_m2l), dim3(256), sh, s, P, Tb, mp, lo);
+        else hipLaunchKernelGGL((k_m2l_fact<T, false>), dim3(P.n_m2l), dim3(256), sh, s, P, Tb, mp, lo);
+      }
     }
   }
-  tm.mark(6);
+  mark(6);
   // ---- STEP 5: L2L, top-down
   for (auto& g : O.l2l_groups)
     hipLaunchKernelGGL((k_l2l<T>), dim3(g.second), dim3(256), lds_ps, s, P, Tb, g.first, lo);
-  tm.mark(7);
+  mark(7);
   // ---- STEP 7: L2P (packed local + gradient expansions, wave-uniform reads)
   if (P.n_l2p) {
     const size_t H = (size_t)(p + 1) * (p + 2) / 2;
@@ -400,7 +522,7 @@ int lap_solve(Context& ctx, const LapCall& c) {
       hipLaunchKernelGGL((k_l2p2<T>), dim3(item_groups(P.n_l2pi)), dim3(kP2PBlock * kItemWaves), 0, s, P, fac, pack, spos, tpos, f.ifpot, f.iffld,
                          f.ifpottarg, f.iffldtarg, pot_s, fld_s, pot_t, fld_t);
   }
-  tm.mark(8);
+  mark(8);
   // ---- STEP 6: M2P (list 3) from packed multipoles; list 3 of each box is
   // cut into slices of about kSlice packed entries, one wave per (64 points,
   // slice), and the slices' sums are added in order (k_slice_reduce)
@@ -465,9 +587,62 @@ int lap_solve(Context& ctx, const LapCall& c) {
                          pot_t, fld_t);
     }
   }
-  tm.mark(9);
+  mark(9);
   // ---- far + near, scattered back to the caller's order (l3dpsort / l3dfsort)
   FMM_HIP(hipStreamWaitEvent(s, ctx.aux_ev[1], 0));
+  // ---- Hessians (FP64): near field over the P2P items, far field from the
+  // locals (L2P items) and the list-3 multipoles, then scattered like the field
+  if (c.ifhess || c.ifhesstarg) {
+    if (!(std::is_same<T, double>::value && std::is_same<TN, double>::value))
+      throw AllocError("Hessians are computed in FP64 only", 128);
+    const int h0 = c.ifhess ? 1 : 0, h1 = c.ifhesstarg ? 1 : 0;
+    const double* fac2 = hess_factors(ctx, p);
+    double* hs_f = (double*)ctx.arena.get("hess_sf", sizeof(double) * 12 * (size_t)ns);
+    double* hs_n = (double*)ctx.arena.get("hess_sn", sizeof(double) * 12 * (size_t)ns);
+    double* ht_f = (double*)ctx.arena.get("hess_tf", sizeof(double) * 12 * (size_t)std::max(nt, 1));
+    double* ht_n = (double*)ctx.arena.get("hess_tn", sizeof(double) * 12 * (size_t)std::max(nt, 1));
+    if (h0) { zero(hs_f, 12 * (size_t)ns); zero(hs_n, 12 * (size_t)ns); }
+    if (h1) { zero(ht_f, 12 * (size_t)nt); zero(ht_n, 12 * (size_t)nt); }
+    const double* str_h = reinterpret_cast<const double*>(str64);
+    auto near_hess = [&](const int* items, int n, int kind, double* out) {
+      if (n == 0 || !(f.ifcharge || f.ifdipole)) return;
+      DevPlan Q = P;
+      Q.p2p_items = items;
+      Q.n_p2p = n;
+      dim3 g(item_groups(n)), bl(kP2PBlock * kItemWaves);
+      if (f.ifcharge && f.ifdipole)
+        hipLaunchKernelGGL((k_p2p_hess<true, true>), g, bl, 0, s, Q, spos, str_h, tpos, kind, out);
+      else if (f.ifcharge)
+        hipLaunchKernelGGL((k_p2p_hess<true, false>), g, bl, 0, s, Q, spos, str_h, tpos, kind, out);
+      else
+        hipLaunchKernelGGL((k_p2p_hess<false, true>), g, bl, 0, s, Q, spos, str_h, tpos, kind, out);
+    };
+    if (h0) near_hess(dIB_near + O.p2p0, O.n_p2p0, 0, hs_n);
+    if (h1) near_hess(dIB_near + O.p2p1, O.n_p2p1, 1, ht_n);
+    if (P.n_l2pi && (f.ifcharge || f.ifdipole))
+      hipLaunchKernelGGL(k_l2p_hess, dim3(item_groups(P.n_l2pi)), dim3(kP2PBlock * kItemWaves), 0, s, P, fac2,
+                         reinterpret_cast<const double*>(lo), spos, tpos, h0, h1, hs_f, ht_f);
+    // list-3 work items: (box, kind, first point) for every box with list 3
+    thread_local std::vector<int> hit;
+    hit.clear();
+    for (int b = 0; b < tree.nboxes; b++) {
+      if (tree.list[2].count(b) == 0) continue;
+      if (h0)
+        for (int q = 0; q < tree.nsrc[b]; q += kP2PBlock) { hit.push_back(b); hit.push_back(0); hit.push_back(q); }
+      if (h1)
+        for (int q = 0; q < tree.ntrg[b]; q += kP2PBlock) { hit.push_back(b); hit.push_back(1); hit.push_back(q); }
+    }
+    if (!hit.empty() && (f.ifcharge || f.ifdipole)) {
+      const int* dh = upload_ints("hess_m2p", hit, s);
+      const int nit = (int)(hit.size() / 3);
+      hipLaunchKernelGGL(k_m2p_hess, dim3(item_groups(nit)), dim3(kP2PBlock * kItemWaves), 0, s, P, fac2,
+                         reinterpret_cast<const double*>(mp), dh, nit, spos, tpos, hs_f, ht_f);
+    }
+    if (h0) hipLaunchKernelGGL(k_scatter_sum, dim3(blocks(ns, 256)), dim3(256), 0, s, ns, 6, sperm, hs_f, hs_n,
+                               c.hess_dev);
+    if (h1) hipLaunchKernelGGL(k_scatter_sum, dim3(blocks(nt, 256)), dim3(256), 0, s, nt, 6, tperm, ht_f, ht_n,
+                               c.hesstarg_dev);
+  }
   auto scatter = [&](int n, int w, const int* perm, const double* far, const double* near, double* out) {
     hipLaunchKernelGGL(k_scatter_sum, dim3(blocks(n, 256)), dim3(256), 0, s, n, w, perm, far, near, out);
   };
@@ -475,7 +650,17 @@ int lap_solve(Context& ctx, const LapCall& c) {
   if (f.iffld) scatter(ns, 3, sperm, fld_s, fld_sn, c.fld_dev);
   if (f.ifpottarg) scatter(nt, 1, tperm, pot_t, pot_tn, c.pottarg_dev);
   if (f.iffldtarg) scatter(nt, 3, tperm, fld_t, fld_tn, c.fldtarg_dev);
-  tm.mark(10);
+  mark(10);
+  if (c.graph) {
+    capture.on = false;
+    FMM_HIP(hipStreamEndCapture(s, &graph));
+    hipGraphExec_t exec = nullptr;
+    const hipError_t e1 = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+    const hipError_t e2 = e1 == hipSuccess ? hipGraphLaunch(exec, s) : e1;
+    if (exec) (void)hipGraphExecDestroy(exec);
+    (void)hipGraphDestroy(graph);
+    FMM_HIP(e2);
+  }
   FMM_HIP(hipGetLastError());
   FMM_HIP(hipStreamSynchronize(s));
   auto& lt = last_timings();
@@ -484,10 +669,10 @@ int lap_solve(Context& ctx, const LapCall& c) {
   lt[0] = ms(t0, t1);
   // main stream: upload, P2M, M2M, (near-plan gap), P2L, M2L, L2L, L2P, M2P,
   // wait for the near field + scatter; the P2P time is the second stream's
-  auto el = tm.elapsed();
+  auto el = timed ? tm.elapsed() : std::vector<double>();
   el.resize(10, 0.0);
   lt[1] = el[0]; lt[2] = el[1]; lt[3] = el[4]; lt[4] = el[2]; lt[5] = el[5];
-  lt[6] = el[6]; lt[7] = el[7]; lt[8] = el[8]; lt[9] = tn.elapsed()[0];
+  lt[6] = el[6]; lt[7] = el[7]; lt[8] = el[8]; lt[9] = timed ? tn.elapsed()[0] : 0.0;
   double tot = 0;
   for (auto v : el) tot += v;
   lt[10] = tot + lt[0];

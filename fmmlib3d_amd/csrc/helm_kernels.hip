@@ -1175,23 +1175,21 @@ __device__ __forceinline__ float exp_t<float>(float x) { return expf(x); }
 // FP32 (sums folded into FP64 by the caller every kFold sources), except for
 // the closest pairs (r^2 < rc2, in practice r below an eighth of the leaf
 // side): their terms dominate the sums at the points they are near, and the
-// few FP32 roundings of such a pair would be the largest error of the
-// result, so they are computed in FP64 and added to the FP64 totals `tot`.
+// few FP32 roundings of such a pair (of v and d too, which a cancelling v.d
+// multiplies) would be the largest error of the result.  add() leaves them
+// out and reports them; the caller adds the close pairs of a flagged run in
+// FP64 from the FP64 strengths (close_hpairs).
 template <typename TN, bool CHG, bool DIP, bool FLD>
 struct HPairAcc {
   TN a[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  // returns whether the pair is a close one (r^2 < rc2), which is then left out
   template <typename TS>
-  __device__ __forceinline__ void add(double kre_d, double kim_d, double xt, double yt, double zt,
+  __device__ __forceinline__ bool add(double kre_d, double kim_d, double xt, double yt, double zt,
                                       const double* __restrict__ sp, const TS* __restrict__ sts,
-                                      bool mask, double rc2 = 0.0, double* tot = nullptr) {
+                                      bool mask, double rc2 = 0.0) {
     const double dxd = xt - sp[0], dyd = yt - sp[1], dzd = zt - sp[2];
     const double r2d = fma(dxd, dxd, fma(dyd, dyd, dzd * dzd));
-    if (sizeof(TN) == sizeof(float) && r2d < rc2 && !mask) {
-      HPairAcc<double, CHG, DIP, FLD> c64;
-      c64.add(kre_d, kim_d, xt, yt, zt, sp, sts, false);
-      for (int k = 0; k < 8; k++) tot[k] += c64.a[k];
-      return;
-    }
+    const bool close = sizeof(TN) == sizeof(float) && r2d < rc2;
     TN st[8];
     for (int k = 0; k < 8; k++) st[k] = (TN)sts[k];
     const TN dx = (TN)dxd, dy = (TN)dyd, dz = (TN)dzd, r2 = (TN)r2d;
@@ -1199,7 +1197,7 @@ struct HPairAcc {
     TN ir = rsqrt_t<TN>(r2);
     // FP32: one Newton step on the hardware estimate (FP64 rsqrt_t is refined already)
     if (sizeof(TN) == sizeof(float)) ir = ir * (TN(1.5) - TN(0.5) * r2 * ir * ir);
-    if (mask) ir = TN(0);                            // a source never acts on itself
+    if (mask || close) ir = TN(0);                   // a source never acts on itself
     TN r, s, c;
     if (sizeof(TN) == sizeof(float)) {
       // The phase k r needs an absolute accuracy that FP32 cannot give once
@@ -1258,6 +1256,7 @@ struct HPairAcc {
         a[7] -= (st[6] * g_i + st[7] * g_r) + ui * dz;
       }
     }
+    return close && !mask;
   }
   __device__ __forceinline__ void fold_into(double* t) {
     for (int k = 0; k < 8; k++) {
@@ -1267,10 +1266,27 @@ struct HPairAcc {
   }
 };
 
+// The close pairs (r^2 < rc2) of sources s0..s1-1 at (xt, yt, zt), except
+// source `self`, in FP64 from the FP64 strengths, added to tot.
+template <bool CHG, bool DIP, bool FLD>
+__device__ __forceinline__ void close_hpairs(double kre, double kim, double xt, double yt, double zt,
+                                             const double* __restrict__ spos, const double* __restrict__ str64,
+                                             int s0, int s1, int self, double rc2, double* tot) {
+  HPairAcc<double, CHG, DIP, FLD> c64;
+  for (int s = s0; s < s1; s++) {
+    const double* sp = spos + 4 * (size_t)s;
+    const double dx = xt - sp[0], dy = yt - sp[1], dz = zt - sp[2];
+    if (s != self && fma(dx, dx, fma(dy, dy, dz * dz)) < rc2)
+      c64.add(kre, kim, xt, yt, zt, sp, str64 + 8 * (size_t)s, false);
+  }
+  c64.fold_into(tot);
+}
+
 template <typename TN, bool CHG, bool DIP, bool FLD>
 __global__ __launch_bounds__(256) void k_hp2p(DevPlan P, double kre, double kim, const double* __restrict__ spos,
-                                              const TN* __restrict__ str, const double* __restrict__ tpos,
-                                              int kind, double* __restrict__ pot, double* __restrict__ fld) {
+                                              const TN* __restrict__ str, const double* __restrict__ str64,
+                                              const double* __restrict__ tpos, int kind, double* __restrict__ pot,
+                                              double* __restrict__ fld) {
   // one work item (64 points) per wave
   const int w = __builtin_amdgcn_readfirstlane((int)(blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)));
   if (w >= P.n_p2p) return;
@@ -1294,9 +1310,13 @@ __global__ __launch_bounds__(256) void k_hp2p(DevPlan P, double kre, double kim,
     const int self = kind ? -1 : me;
     for (int c = s0; c < s1; c += (fold ? kFold : (1 << 30))) {
       const int ce = fold ? min(c + kFold, s1) : s1;
+      bool close = false;
       for (int s = c; s < ce; s++)
-        acc.add(kre, kim, x, y, z, spos + 4 * (size_t)s, str + 8 * (size_t)s, s == self, rc2, tot);
-      if (fold) acc.fold_into(tot);
+        close |= acc.add(kre, kim, x, y, z, spos + 4 * (size_t)s, str + 8 * (size_t)s, s == self, rc2);
+      if (fold) {
+        acc.fold_into(tot);
+        if (close) close_hpairs<CHG, DIP, FLD>(kre, kim, x, y, z, spos, str64, c, ce, self, rc2, tot);
+      }
     }
   }
   for (int jj = P.l1_off[b]; jj < P.l1_off[b + 1]; jj++) {
@@ -1304,9 +1324,13 @@ __global__ __launch_bounds__(256) void k_hp2p(DevPlan P, double kre, double kim,
     const int s0 = P.isrc[j], s1 = s0 + P.nsrc[j];
     for (int c = s0; c < s1; c += (fold ? kFold : (1 << 30))) {
       const int ce = fold ? min(c + kFold, s1) : s1;
+      bool close = false;
       for (int s = c; s < ce; s++)
-        acc.add(kre, kim, x, y, z, spos + 4 * (size_t)s, str + 8 * (size_t)s, false, rc2, tot);
-      if (fold) acc.fold_into(tot);
+        close |= acc.add(kre, kim, x, y, z, spos + 4 * (size_t)s, str + 8 * (size_t)s, false, rc2);
+      if (fold) {
+        acc.fold_into(tot);
+        if (close) close_hpairs<CHG, DIP, FLD>(kre, kim, x, y, z, spos, str64, c, ce, -1, rc2, tot);
+      }
     }
   }
   if (!fold) acc.fold_into(tot);

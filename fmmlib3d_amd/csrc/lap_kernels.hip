@@ -1273,7 +1273,14 @@ __global__ void k_m2l_fact(DevPlan P, DevTables Tb, const T* __restrict__ mp, T*
 // the scalar cache and every lane reuses them from SGPRs.  Offsets and r^2
 // are formed in FP64 for every precision; with TN = float the rest of the
 // pair is FP32, summed in FP32 over runs of at most kFold sources and folded
-// into FP64 accumulators.
+// into FP64 accumulators, except for the closest pairs (r^2 < rc2, r below
+// 1/32 of the leaf side): such a pair dominates the sums at its point, and a
+// dipole's v.d can cancel, which multiplies the FP32 rounding of v and d by
+// |v||d|/|v.d|.  The FP32 loop only keeps the smallest r^2 of the run (one
+// min per pair, no branch, so the unrolled source loads stay batched); a
+// point whose run held a close pair drops that run's FP32 sums and runs it
+// again with the close pairs in FP64 from the FP64 strengths (redo_run),
+// which happens for a few runs only.
 // ---------------------------------------------------------------------------
 
 template <typename TN, bool CHG, bool DIP, bool FLD>
@@ -1281,12 +1288,13 @@ struct PairAcc {
   TN pr = 0, pi = 0, fxr = 0, fxi = 0, fyr = 0, fyi = 0, fzr = 0, fzi = 0;
   // pot += q/r + (v.d)/r^3 = (q + (v.d)/r^2)/r;
   // fld += (q + 3 (v.d)/r^2) d/r^3 - v/r^3 = (c d - v)/r^3, d = x_t - x_s
-  __device__ __forceinline__ void add(double xt, double yt, double zt, const double* __restrict__ sp,
-                                      const TN* __restrict__ st, bool mask) {
+  // returns r^2 in TN
+  __device__ __forceinline__ TN add(double xt, double yt, double zt, const double* __restrict__ sp,
+                                    const TN* __restrict__ st, bool mask) {
     const double dxd = xt - sp[0], dyd = yt - sp[1], dzd = zt - sp[2];
     const double r2d = fma(dxd, dxd, fma(dyd, dyd, dzd * dzd));
-    const TN dx = (TN)dxd, dy = (TN)dyd, dz = (TN)dzd;
-    TN ri = rsqrt_t<TN>((TN)r2d);
+    const TN dx = (TN)dxd, dy = (TN)dyd, dz = (TN)dzd, r2 = (TN)r2d;
+    TN ri = rsqrt_t<TN>(r2);
     if (mask) ri = TN(0);
     const TN ri2 = ri * ri;
     if (!DIP) {
@@ -1299,7 +1307,7 @@ struct PairAcc {
         fyr = fma(cr, dy, fyr); fyi = fma(ci, dy, fyi);
         fzr = fma(cr, dz, fzr); fzi = fma(ci, dz, fzi);
       }
-      return;
+      return r2;
     }
     const TN dotr = fma(st[2], dx, fma(st[4], dy, st[6] * dz));
     const TN doti = fma(st[3], dx, fma(st[5], dy, st[7] * dz));
@@ -1315,6 +1323,7 @@ struct PairAcc {
       fyr = fma(ri3, fma(cr, dy, -st[4]), fyr); fyi = fma(ri3, fma(ci, dy, -st[5]), fyi);
       fzr = fma(ri3, fma(cr, dz, -st[6]), fzr); fzi = fma(ri3, fma(ci, dz, -st[7]), fzi);
     }
+    return r2;
   }
   __device__ __forceinline__ void fold_into(double* a) {
     a[0] += (double)pr; a[1] += (double)pi;
@@ -1326,10 +1335,32 @@ struct PairAcc {
   }
 };
 
+// A run of sources s0..s1-1 again at (xt, yt, zt) after its FP32 sums were
+// dropped: the close pairs (r^2 < rc2, not source `self`) in FP64 from the
+// FP64 strengths into tot, the rest in FP32 into acc as before.
 template <typename TN, bool CHG, bool DIP, bool FLD>
-__global__ __launch_bounds__(256) void k_p2p(DevPlan P, const double* __restrict__ spos,
-                                             const TN* __restrict__ str, const double* __restrict__ tpos,
-                                             int kind, double* __restrict__ pot, double* __restrict__ fld) {
+__device__ __forceinline__ void redo_run(PairAcc<TN, CHG, DIP, FLD>& acc, double xt, double yt, double zt,
+                                         const double* __restrict__ spos, const TN* __restrict__ str,
+                                         const double* __restrict__ str64, int s0, int s1, int self, double rc2,
+                                         double* tot) {
+  acc = PairAcc<TN, CHG, DIP, FLD>();
+  PairAcc<double, CHG, DIP, FLD> c64;
+  for (int s = s0; s < s1; s++) {
+    const double* sp = spos + 4 * (size_t)s;
+    const double dx = xt - sp[0], dy = yt - sp[1], dz = zt - sp[2];
+    if (s != self && fma(dx, dx, fma(dy, dy, dz * dz)) < rc2)
+      c64.add(xt, yt, zt, sp, str64 + 8 * (size_t)s, false);
+    else
+      acc.add(xt, yt, zt, sp, str + 8 * (size_t)s, s == self);
+  }
+  c64.fold_into(tot);
+}
+
+template <typename TN, bool CHG, bool DIP, bool FLD>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) void k_p2p(DevPlan P, const double* __restrict__ spos,
+                                             const TN* __restrict__ str, const double* __restrict__ str64,
+                                             const double* __restrict__ tpos, int kind, double* __restrict__ pot,
+                                             double* __restrict__ fld) {
   // one work item (64 points) per wave
   const int w = __builtin_amdgcn_readfirstlane((int)(blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)));
   if (w >= P.n_p2p) return;
@@ -1345,14 +1376,28 @@ __global__ __launch_bounds__(256) void k_p2p(DevPlan P, const double* __restrict
   double tot[8] = {0, 0, 0, 0, 0, 0, 0, 0};
   PairAcc<TN, CHG, DIP, FLD> acc;
   constexpr bool fold = sizeof(TN) < sizeof(double);
+  // FP32 pairs: pairs closer than 1/32 of this leaf's side run in FP64 (on a
+  // uniform 1M cube 0.05 such pairs per point; the near-field error of the
+  // FP32 pairs falls from 1.5e-7 to 8e-8 of the potential, and on a sphere
+  // with a target next to a dipole from 7e-7 to 3e-9)
+  const double hside = 1.0 / P.inv_side[P.level[b]];
+  const double rc2 = fold ? hside * hside * (1.0 / 1024.0) : 0.0;
+  const TN rc2n = (TN)rc2, kFar = TN(1e30);
   // own box first: a source never acts on itself (index test, lfmm3dpart.f:1193)
   {
     const int s0 = P.isrc[b], s1 = s0 + P.nsrc[b];
     const int self = kind ? -1 : me;
     for (int c = s0; c < s1; c += (fold ? kFold : (1 << 30))) {
       const int ce = fold ? min(c + kFold, s1) : s1;
-      for (int s = c; s < ce; s++) acc.add(x, y, z, spos + 4 * (size_t)s, str + 8 * (size_t)s, s == self);
-      if (fold) acc.fold_into(tot);
+      TN mn = kFar;
+      for (int s = c; s < ce; s++) {
+        const TN r2 = acc.add(x, y, z, spos + 4 * (size_t)s, str + 8 * (size_t)s, s == self);
+        if (fold) mn = fmin(mn, s == self ? kFar : r2);
+      }
+      if (fold) {
+        if (mn < rc2n) redo_run(acc, x, y, z, spos, str, str64, c, ce, self, rc2, tot);
+        acc.fold_into(tot);
+      }
     }
   }
   for (int jj = P.l1_off[b]; jj < P.l1_off[b + 1]; jj++) {
@@ -1360,9 +1405,16 @@ __global__ __launch_bounds__(256) void k_p2p(DevPlan P, const double* __restrict
     const int s0 = P.isrc[j], s1 = s0 + P.nsrc[j];
     for (int c = s0; c < s1; c += (fold ? kFold : (1 << 30))) {
       const int ce = fold ? min(c + kFold, s1) : s1;
+      TN mn = kFar;
 #pragma unroll 4
-      for (int s = c; s < ce; s++) acc.add(x, y, z, spos + 4 * (size_t)s, str + 8 * (size_t)s, false);
-      if (fold) acc.fold_into(tot);
+      for (int s = c; s < ce; s++) {
+        const TN r2 = acc.add(x, y, z, spos + 4 * (size_t)s, str + 8 * (size_t)s, false);
+        if (fold) mn = fmin(mn, r2);
+      }
+      if (fold) {
+        if (mn < rc2n) redo_run(acc, x, y, z, spos, str, str64, c, ce, -1, rc2, tot);
+        acc.fold_into(tot);
+      }
     }
   }
   if (!fold) acc.fold_into(tot);

@@ -17,7 +17,7 @@ struct DevPlan {
   // boxes
   const int *level, *isrc, *nsrc, *itrg, *ntrg, *nchild, *child, *ijk;
   const double *center;   // 3 per box
-  const double *inv_side; // 1/side per level
+  const double *inv_side; // 1/side per level (per scale index of a forest, see merge_forest)
   // lists (CSR over boxes)
   const int *l1_off, *l1_idx, *l2_off, *l2_idx, *l3_off, *l3_idx, *l4_off, *l4_idx;
   // work lists
@@ -65,6 +65,14 @@ struct DevTables {
   const float *rot_half_f, *z_m2l_t_f;
   const long *rot_half_off, *zt_off;
   long zt_size;
+  // MFMA-ordered M2L tables (FP64 and FP32): forward rotation blocks of
+  // (class, degree) at mf_rot_off[c*(p+1)+n], back blocks mf_rot_back
+  // further; z-shift blocks of order m at mf_z_off[m] within a distance
+  // class of mf_z_size values
+  const double *mf_rot, *mf_z;
+  const float *mf_rot_f, *mf_z_f;
+  const long *mf_rot_off, *mf_z_off;
+  long mf_rot_back, mf_z_size;
 };
 
 }  // namespace fmm3d

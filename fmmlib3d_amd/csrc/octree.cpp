@@ -19,7 +19,7 @@ namespace fmm3d {
 double Octree::side(int l) const { return size / std::ldexp(1.0, l); }
 
 void Octree::reset() {
-  nboxes = nlev = 0;
+  nboxes = nlev = ns = nt = 0;
   center[0] = center[1] = center[2] = 0.0;
   size = 0.0;
   for (auto* v : {&level, &parent, &nchild, &child, &ijk, &isrc, &nsrc, &itrg, &ntrg, &ladd, &src_perm, &trg_perm})
@@ -28,6 +28,88 @@ void Octree::reset() {
   for (auto& c : list) {
     c.off.clear();
     c.idx.clear();
+  }
+  scale_id.clear();
+  scale_inv.clear();
+}
+
+void merge_forest(const std::vector<const Octree*>& trees, Octree& f) {
+  f.reset();
+  const int nt = (int)trees.size();
+  int maxlev = 0, nb = 0;
+  for (const Octree* t : trees) {
+    maxlev = std::max(maxlev, t->nlev);
+    nb += t->nboxes;
+    f.ns += t->ns;
+    f.nt += t->nt;
+  }
+  f.nboxes = nb;
+  f.nlev = maxlev;
+  // new ids: level-major, then tree, then the tree's own (breadth-first) order
+  std::vector<std::vector<int>> nid(nt);
+  std::vector<int> so(nt), to(nt), sbase(nt);
+  for (int r = 0, s0 = 0, t0 = 0, b0 = 0; r < nt; r++) {
+    nid[r].assign(trees[r]->nboxes, -1);
+    so[r] = s0; to[r] = t0; sbase[r] = b0;
+    s0 += trees[r]->ns; t0 += trees[r]->nt; b0 += trees[r]->nlev + 1;
+  }
+  f.ladd.assign(maxlev + 2, 0);
+  std::vector<std::pair<int, int>> order;  // (tree, old box) by new id
+  order.reserve(nb);
+  for (int l = 0; l <= maxlev; l++) {
+    f.ladd[l] = (int)order.size();
+    for (int r = 0; r < nt; r++) {
+      const Octree& t = *trees[r];
+      if (l > t.nlev) continue;
+      for (int b = t.ladd[l]; b < t.ladd[l + 1]; b++) {
+        nid[r][b] = (int)order.size();
+        order.push_back({r, b});
+      }
+    }
+  }
+  f.ladd[maxlev + 1] = nb;
+  f.level.resize(nb); f.parent.resize(nb); f.nchild.resize(nb); f.child.resize(8 * (size_t)nb);
+  f.ijk.resize(3 * (size_t)nb); f.isrc.resize(nb); f.nsrc.resize(nb); f.itrg.resize(nb); f.ntrg.resize(nb);
+  f.bcenter.resize(3 * (size_t)nb); f.scale_id.resize(nb);
+  for (int b = 0; b < nb; b++) {
+    const int r = order[b].first, o = order[b].second;
+    const Octree& t = *trees[r];
+    f.level[b] = t.level[o];
+    f.parent[b] = t.parent[o] < 0 ? t.parent[o] : nid[r][t.parent[o]];
+    f.nchild[b] = t.nchild[o];
+    for (int c = 0; c < 8; c++) {
+      const int ch = t.child[8 * (size_t)o + c];
+      f.child[8 * (size_t)b + c] = ch < 0 ? ch : nid[r][ch];
+    }
+    for (int d = 0; d < 3; d++) {
+      f.ijk[3 * (size_t)b + d] = t.ijk[3 * (size_t)o + d];
+      f.bcenter[3 * (size_t)b + d] = t.bcenter[3 * (size_t)o + d];
+    }
+    f.isrc[b] = t.isrc[o] + so[r];
+    f.nsrc[b] = t.nsrc[o];
+    f.itrg[b] = t.itrg[o] + to[r];
+    f.ntrg[b] = t.ntrg[o];
+    f.scale_id[b] = sbase[r] + t.level[o];
+  }
+  f.scale_inv.clear();
+  for (const Octree* t : trees)
+    for (int l = 0; l <= t->nlev; l++) f.scale_inv.push_back(1.0 / t->side(l));
+  f.src_perm.clear();
+  f.trg_perm.clear();
+  for (int r = 0; r < nt; r++) {
+    for (int v : trees[r]->src_perm) f.src_perm.push_back(v + so[r]);
+    for (int v : trees[r]->trg_perm) f.trg_perm.push_back(v + to[r]);
+  }
+  for (int k = 0; k < 4; k++) {
+    Csr& L = f.list[k];
+    L.off.assign(nb + 1, 0);
+    L.idx.clear();
+    for (int b = 0; b < nb; b++) {
+      const int r = order[b].first, o = order[b].second;
+      const Csr& S = trees[r]->list[k];
+      for (int q = S.off[o]; q < S.off[o + 1]; q++) L.idx.push_back(nid[r][S.idx[q]]);
+      L.off[b + 1] = (int)L.idx.size();
+    }
   }
 }
 

@@ -30,8 +30,24 @@ struct Octree {
   std::vector<int> src_perm, trg_perm;
   // interaction lists (index 0..3 = reference lists 1..4), ascending per box
   Csr list[4];
+  // Forest of several trees (merge_forest): per box the index of its
+  // (tree, level) scale and 1/side per index; empty for a single tree, whose
+  // scale is 1/side(level)
+  std::vector<int> scale_id;
+  std::vector<double> scale_inv;
 
   bool is_leaf(int b) const { return nchild[b] == 0; }
+  // every leaf that holds points lies on one level (then lists 3 and 4 are
+  // empty: no interactions between boxes of different sizes)
+  bool leaves_on_one_level() const {
+    int lv = -1;
+    for (int b = 0; b < nboxes; b++) {
+      if (nchild[b] != 0 || nsrc[b] + ntrg[b] == 0) continue;
+      if (lv < 0) lv = level[b];
+      else if (level[b] != lv) return false;
+    }
+    return true;
+  }
   // empties every array but keeps its storage (repeated builds reuse it)
   void reset();
   double side(int l) const;
@@ -76,6 +92,15 @@ inline uint32_t tree_eff_code(const TreePass& ps, int f, uint32_t code) {
   }
   return c;
 }
+// Disjoint union of finished trees (lists built) as one Octree, for running
+// several small problems through one set of launches: boxes ordered by level,
+// then tree, then their own order (so every tree keeps its numbering order
+// and a level's boxes stay contiguous in ladd); point ranges and
+// permutations shifted by src_off / trg_off (the trees' points concatenated
+// in tree order); lists renumbered; scale_id / scale_inv carry each tree's
+// box sizes.
+void merge_forest(const std::vector<const Octree*>& trees, Octree& f);
+
 // colleagues and lists 1-4 from the finished box structure
 void tree_lists(Octree& t);
 // the same in two parts on one thread: colleagues, lists 1 and 3; then lists 2

@@ -70,7 +70,9 @@ struct Flags {
 inline void build_plan_boxes(const Octree& t, const Flags& f, Flat& F, PlanOffsets& O, int p2m_min_level) {
   (void)f;
   const int nb = t.nboxes;
-  O.level = F.i(t.level); O.isrc = F.i(t.isrc); O.nsrc = F.i(t.nsrc);
+  // the Laplace kernels read a box's level only to find its scale 1/side: a
+  // forest (Laplace only) uploads its per-tree scale index instead
+  O.level = F.i(t.scale_id.empty() ? t.level : t.scale_id); O.isrc = F.i(t.isrc); O.nsrc = F.i(t.nsrc);
   O.itrg = F.i(t.itrg); O.ntrg = F.i(t.ntrg); O.nchild = F.i(t.nchild);
   O.child = F.i(t.child); O.ijk = F.i(t.ijk);
   std::vector<int> p2m;
@@ -94,9 +96,13 @@ inline void build_plan_boxes(const Octree& t, const Flags& f, Flat& F, PlanOffse
   O.sperm = F.i(t.src_perm);
   O.tperm = F.i(t.trg_perm);
   O.center = F.d(t.bcenter);
-  std::vector<double> inv(t.nlev + 1);
-  for (int l = 0; l <= t.nlev; l++) inv[l] = 1.0 / t.side(l);
-  O.inv_side = F.d(inv);
+  if (t.scale_inv.empty()) {
+    std::vector<double> inv(t.nlev + 1);
+    for (int l = 0; l <= t.nlev; l++) inv[l] = 1.0 / t.side(l);
+    O.inv_side = F.d(inv);
+  } else {
+    O.inv_side = F.d(t.scale_inv);
+  }
 }
 
 // Work items (box, kind, first point) of kP2PBlock points for one box.  A

@@ -7,6 +7,7 @@
 #include <memory>
 #include <mutex>
 #include <set>
+#include <utility>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -71,9 +72,10 @@ struct Context {
   hipStream_t stream = nullptr;  // null = legacy default stream
   Arena arena;
   std::mutex mu;
-  // orders p whose Laplace translation tables are in this context's arena
-  // (the host tables are built once per process and order)
-  std::set<int> lap_tabs;
+  // (order p, tolerance eps) of the Laplace translation tables in this
+  // context's arena (the host tables are built once per process and key;
+  // eps sets the per-offset truncation orders, so it is part of the key)
+  std::set<std::pair<int, double>> lap_tabs;
   // cached Helmholtz tables (owned by the Helmholtz driver)
   std::shared_ptr<void> helm_tables;
   // second stream (non-blocking, default priority) for the near field, and

@@ -7,6 +7,14 @@
 // that arrive while others run start as soon as a slot and memory free up and
 // their kernels overlap on the device.  Latency is measured from submission
 // to completion.
+//
+// Small Laplace requests (at most kFuseMaxPoints points) that are queued
+// together with the same tolerance, precision and output flags are fused: a
+// worker takes up to kFuseMaxBatch of them, builds their trees on the host,
+// joins them into one forest (merge_forest) and runs all of them through one
+// set of launches, captured in a HIP graph and launched as one.  Every
+// request's boxes, lists and per-box work are those of the same call made
+// alone, so its results are bitwise the same.
 #include <hip/hip_runtime.h>
 #include <omp.h>
 
@@ -14,7 +22,9 @@
 #include <atomic>
 #include <chrono>
 #include <condition_variable>
+#include <cstdlib>
 #include <cstring>
+#include <string>
 #include <deque>
 #include <map>
 #include <memory>
@@ -24,6 +34,8 @@
 
 #include "../../include/fmm3d_mi355x.h"
 #include "fmm3d_internal.hpp"
+#include "octree.hpp"
+#include "plan.hpp"
 
 namespace fmm3d {
 namespace {
@@ -70,6 +82,31 @@ struct Worker {
 // admissions of later requests before the queue head is served first
 constexpr int kMaxBypass = 8;
 
+// fusion of small Laplace requests: per-request size, requests per batch,
+// points per batch
+constexpr int kFuseMaxPoints = 65536;
+constexpr int kFuseMaxBatch = 16;
+constexpr int kFuseMaxTotal = 262144;
+
+bool fusable(const Request& r) {
+  return r.kernel == 0 && r.iprec <= 5 && (long)r.ns + r.nt <= kFuseMaxPoints;
+}
+// requests that can share one forest: same tolerance (hence order and leaf
+// size), precision mode and output flags
+bool same_batch(const Request& a, const Request& b) {
+  return a.iprec == b.iprec && a.prec == b.prec && a.ifcharge == b.ifcharge && a.ifdipole == b.ifdipole &&
+         a.ifpot == b.ifpot && a.iffld == b.iffld && (a.nt > 0) == (b.nt > 0) && a.ifpottarg == b.ifpottarg &&
+         a.iffldtarg == b.iffldtarg;
+}
+
+bool fusion_enabled() {
+  static const bool on = [] {
+    const char* v = std::getenv("FMM3D_SERVER_FUSE");
+    return !(v && std::strcmp(v, "0") == 0);
+  }();
+  return on;
+}
+
 }  // namespace
 
 struct Server {
@@ -82,11 +119,13 @@ struct Server {
   long long next_id = 0;
   bool stop = false;
   int busy = 0;
+  long long fused = 0;  // requests served in fused batches
 
   void run(Worker* w) {
     if (hipSetDevice(w->ctx.device) != hipSuccess) (void)hipGetLastError();
     for (;;) {
       std::shared_ptr<Request> r;
+      std::vector<std::shared_ptr<Request>> batch;
       {
         std::unique_lock<std::mutex> lk(mu);
         // admit the oldest queued request whose estimate fits what is left of
@@ -115,11 +154,29 @@ struct Server {
         queue.erase(sel);
         in_use += r->bytes;
         busy++;
-        r->t_start = Clock::now();
+        batch.push_back(r);
+        // small Laplace requests queued with the same parameters join it
+        if (fusion_enabled() && fusable(*r)) {
+          long pts = (long)r->ns + r->nt;
+          for (auto it = queue.begin(); it != queue.end() && (int)batch.size() < kFuseMaxBatch;) {
+            const Request& q = **it;
+            if (fusable(q) && same_batch(*r, q) && pts + q.ns + q.nt <= kFuseMaxTotal &&
+                in_use + q.bytes <= budget) {
+              pts += q.ns + q.nt;
+              in_use += q.bytes;
+              batch.push_back(*it);
+              it = queue.erase(it);
+            } else {
+              ++it;
+            }
+          }
+        }
+        const auto now = Clock::now();
+        for (auto& b : batch) b->t_start = now;
       }
       int ier = 0;
       try {
-        ier = solve(w->ctx, *r);
+        ier = batch.size() > 1 ? solve_fused(w->ctx, batch) : solve(w->ctx, *r);
       } catch (const AllocError& e) {
         ier = e.ier;
       } catch (const HipError&) {
@@ -130,10 +187,14 @@ struct Server {
       }
       {
         std::lock_guard<std::mutex> lk(mu);
-        r->ier = ier;
-        r->t_end = Clock::now();
-        r->done = true;
-        in_use -= r->bytes;
+        const auto now = Clock::now();
+        for (auto& b : batch) {
+          b->ier = ier;
+          b->t_end = now;
+          b->done = true;
+          in_use -= b->bytes;
+        }
+        if (batch.size() > 1) fused += (long long)batch.size();
         busy--;
       }
       cv_done.notify_all();
@@ -175,6 +236,82 @@ struct Server {
     down(r.fld, c.fld_dev, 6 * (size_t)r.ns);
     down(r.pt, c.pottarg_dev, 2 * (size_t)r.nt);
     down(r.ft, c.fldtarg_dev, 6 * (size_t)r.nt);
+    FMM_HIP(hipStreamSynchronize(ctx.stream));
+    return 0;
+  }
+
+  // Several small Laplace requests (same_batch) as one forest: host trees,
+  // merged; inputs concatenated and uploaded once; one graph-captured solve;
+  // outputs split back.  A tree that cannot be built fails the batch (4).
+  static int solve_fused(Context& ctx, std::vector<std::shared_ptr<Request>>& batch) {
+    const Request& r0 = *batch[0];
+    double eps;
+    int nbox;
+    if (!iprec_params(r0.iprec, r0.ns, r0.nt, eps, nbox)) return 32;
+    const int nb = (int)batch.size();
+    thread_local std::vector<Octree> trees;
+    thread_local Octree forest;
+    if ((int)trees.size() < nb) trees.resize(nb);
+    std::vector<const Octree*> tp(nb);
+    int ns = 0, nt = 0;
+    for (int k = 0; k < nb; k++) {
+      const Request& q = *batch[k];
+      trees[k].reset();
+      if (build_octree(q.src.data(), q.ns, q.nt > 0 ? q.trg.data() : nullptr, q.nt, nbox, trees[k]) != 0) return 4;
+      tp[k] = &trees[k];
+      ns += q.ns;
+      nt += q.nt;
+    }
+    merge_forest(tp, forest);
+    // concatenated inputs through one pinned staging buffer
+    auto up = [&](const char* name, size_t per, bool use, const std::vector<double> Request::*field,
+                  int Request::*count) -> double* {
+      if (!use) return nullptr;
+      size_t tot = 0;
+      for (auto& q : batch) tot += per * (size_t)((*q).*count);
+      if (tot == 0) return nullptr;
+      double* h = (double*)ctx.arena.get_host(std::string(name) + "_h", tot * sizeof(double));
+      size_t o = 0;
+      for (auto& q : batch) {
+        const std::vector<double>& v = (*q).*field;
+        std::memcpy(h + o, v.data(), v.size() * sizeof(double));
+        o += per * (size_t)((*q).*count);
+      }
+      double* d = (double*)ctx.arena.get(name, tot * sizeof(double));
+      FMM_HIP(hipMemcpyAsync(d, h, tot * sizeof(double), hipMemcpyHostToDevice, ctx.stream));
+      return d;
+    };
+    LapCall c;
+    c.iprec = r0.iprec; c.ns = ns; c.nt = nt;
+    c.ifcharge = r0.ifcharge; c.ifdipole = r0.ifdipole; c.ifpot = r0.ifpot; c.iffld = r0.iffld;
+    c.ifpottarg = nt > 0 && r0.ifpottarg; c.iffldtarg = nt > 0 && r0.iffldtarg;
+    c.src_dev = up("fin_src", 3, true, &Request::src, &Request::ns);
+    c.q_dev = up("fin_q", 2, c.ifcharge, &Request::q, &Request::ns);
+    c.ds_dev = up("fin_ds", 2, c.ifdipole, &Request::ds, &Request::ns);
+    c.dv_dev = up("fin_dv", 3, c.ifdipole, &Request::dv, &Request::ns);
+    c.trg_dev = up("fin_trg", 3, nt > 0, &Request::trg, &Request::nt);
+    if (c.ifpot) c.pot_dev = (double*)ctx.arena.get("fout_pot", 2 * sizeof(double) * ns);
+    if (c.iffld) c.fld_dev = (double*)ctx.arena.get("fout_fld", 6 * sizeof(double) * ns);
+    if (c.ifpottarg) c.pottarg_dev = (double*)ctx.arena.get("fout_pt", 2 * sizeof(double) * nt);
+    if (c.iffldtarg) c.fldtarg_dev = (double*)ctx.arena.get("fout_ft", 6 * sizeof(double) * nt);
+    c.tree_in = &forest;
+    c.graph = true;
+    if (int ier = lap_solve_mode(ctx, c, r0.prec)) return ier;
+    auto down = [&](const double* d, size_t per, std::vector<double> Request::*field, int Request::*count) {
+      if (!d) return;
+      size_t o = 0;
+      for (auto& q : batch) {
+        const size_t n = per * (size_t)((*q).*count);
+        std::vector<double>& v = (*q).*field;
+        v.resize(n);
+        if (n) FMM_HIP(hipMemcpyAsync(v.data(), d + o, n * sizeof(double), hipMemcpyDeviceToHost, ctx.stream));
+        o += n;
+      }
+    };
+    down(c.pot_dev, 2, &Request::pot, &Request::ns);
+    down(c.fld_dev, 6, &Request::fld, &Request::ns);
+    down(c.pottarg_dev, 2, &Request::pt, &Request::nt);
+    down(c.fldtarg_dev, 6, &Request::ft, &Request::nt);
     FMM_HIP(hipStreamSynchronize(ctx.stream));
     return 0;
   }
@@ -290,13 +427,15 @@ int fmm3d_server_wait(void* server, long long id, int block, double* pot, double
   return 1;
 }
 
-void fmm3d_server_stats(void* server, long long* queued, int* running, long long* bytes_in_use) {
+void fmm3d_server_stats(void* server, long long* queued, int* running, long long* bytes_in_use,
+                        long long* fused_requests) {
   auto* S = static_cast<Server*>(server);
   if (!S) return;
   std::lock_guard<std::mutex> lk(S->mu);
   if (queued) *queued = (long long)S->queue.size();
   if (running) *running = S->busy;
   if (bytes_in_use) *bytes_in_use = (long long)S->in_use;
+  if (fused_requests) *fused_requests = S->fused;
 }
 
 void fmm3d_server_destroy(void* server) {

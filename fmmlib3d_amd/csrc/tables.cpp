@@ -101,6 +101,65 @@ struct SqrtBinom {
   double operator()(int a, int b) const { return (b < 0 || b > a) ? 0.0 : t[(size_t)a * N + b]; }
 };
 
+// A-operand order of a 16x16x4 MFMA: lane l holds A[l % 16][l / 16] of a
+// 16 x 4 block (row tile rt, column block kb of the matrix).
+//
+// Rotation of degree n, forward: out_{m'} = sum_m F[m'][m] in_m with
+// F[m'][m] = rot block (row m + n, column m' + n); only rows m' >= 0 are
+// stored, the negative orders follow from F[-m'][-m] = F[m'][m] (the kernel
+// applies the same rows to the reversed input).  Back: the transpose,
+// out_m = sum_{m'} F[m'][m] in_{m'}, which has the same symmetry.
+// z-shift of order m: out_l = sum_n Z[m][l][n] in_n for l, n = m..p.
+static void build_mfma_tables(int p, HostTables& H) {
+  const int P1 = p + 1;
+  H.mf_rot_off.assign((size_t)H.nrot * P1, 0);
+  long tot = 0;
+  for (int c = 0; c < H.nrot; c++)
+    for (int n = 0; n <= p; n++) {
+      H.mf_rot_off[(size_t)c * P1 + n] = tot;
+      tot += 64L * mf_rot_tiles(n) * mf_rot_kblocks(n);
+    }
+  H.mf_rot_back = tot;
+  H.mf_rot.assign(2 * (size_t)tot, 0.0);
+  for (int c = 0; c < H.nrot; c++)
+    for (int n = 0; n <= p; n++) {
+      const int dim = 2 * n + 1, nkb = mf_rot_kblocks(n);
+      const double* D = &H.rot[H.rot_off[(size_t)c * P1 + n]];
+      double* fw = &H.mf_rot[H.mf_rot_off[(size_t)c * P1 + n]];
+      double* bk = fw + tot;
+      for (int rt = 0; rt < mf_rot_tiles(n); rt++)
+        for (int kb = 0; kb < nkb; kb++)
+          for (int l = 0; l < 64; l++) {
+            const int r = 16 * rt + l % 16, kap = 4 * kb + l / 16;
+            if (r > n || kap >= dim) continue;
+            const size_t at = ((size_t)rt * nkb + kb) * 64 + l;
+            fw[at] = D[(size_t)kap * dim + (r + n)];   // F[r][kap - n]
+            bk[at] = D[(size_t)(r + n) * dim + kap];   // F[kap - n][r]
+          }
+    }
+  H.mf_z_off.assign(P1, 0);
+  long zs = 0;
+  for (int m = 0; m <= p; m++) {
+    H.mf_z_off[m] = zs;
+    zs += 64L * mf_z_tiles(p, m) * mf_z_kblocks(p, m);
+  }
+  H.mf_z_size = zs;
+  H.mf_z.assign((size_t)zs * H.ndist, 0.0);
+  const size_t Z = (size_t)P1 * P1 * P1;
+  for (int dc = 0; dc < H.ndist; dc++)
+    for (int m = 0; m <= p; m++) {
+      const int nkb = mf_z_kblocks(p, m);
+      double* zb = &H.mf_z[(size_t)zs * dc + H.mf_z_off[m]];
+      for (int lt = 0; lt < mf_z_tiles(p, m); lt++)
+        for (int kb = 0; kb < nkb; kb++)
+          for (int l = 0; l < 64; l++) {
+            const int lo = m + 16 * lt + l % 16, ni = m + 4 * kb + l / 16;
+            if (lo > p || ni > p) continue;
+            zb[((size_t)lt * nkb + kb) * 64 + l] = H.z_m2l[Z * dc + ((size_t)m * P1 + lo) * P1 + ni];
+          }
+    }
+}
+
 void build_tables(int p, double eps, HostTables& H) {
   build_geometry_tables(p, H);
   const int P1 = p + 1;
@@ -150,6 +209,7 @@ void build_tables(int p, double eps, HostTables& H) {
               H.z_m2l[Z * dc + ((size_t)m * P1 + l) * P1 + n];
         }
     }
+  build_mfma_tables(p, H);
   H.z_m2m.assign(Z, 0.0);
   H.z_l2l.assign(Z, 0.0);
   const double r = std::sqrt(3.0) / 4.0;
@@ -165,35 +225,37 @@ void build_tables(int p, double eps, HostTables& H) {
 
 // Factor table, rotation classes, rotation blocks and phases for order p:
 // the kernel-independent half of the translation tables.
+void build_fac_table(int p, std::vector<double>& fac) {
+  const int P = p + 2, Hh = P * (P + 1) / 2, E = P * P;
+  fac.assign((size_t)2 * (P + 1) + 2 * Hh + 6 * E, 0.0);
+  const int MM = 0, M1 = MM + P + 1, RA = M1 + P + 1, RB = RA + Hh, AR = RB + Hh, BR = AR + E,
+            CR = BR + E, AS = CR + E, BS = AS + E, CS = BS + E;
+  for (int m = 1; m <= P; m++) fac[MM + m] = std::sqrt((2.0 * m - 1.0) / (2.0 * m));
+  for (int m = 0; m <= P; m++) fac[M1 + m] = std::sqrt(2.0 * m + 1.0);
+  for (int n = 0; n < P; n++)
+    for (int m = 0; m <= n; m++) {
+      int h = n * (n + 1) / 2 + m;
+      fac[RA + h] = (n >= m + 2) ? std::sqrt((n + m - 1.0) * (n - m - 1.0)) : 0.0;
+      fac[RB + h] = (n > m) ? 1.0 / std::sqrt((double)(n - m) * (n + m)) : 0.0;
+    }
+  for (int n = 0; n < P; n++)
+    for (int m = -n; m <= n; m++) {
+      int e = n * n + n + m;
+      fac[AR + e] = std::sqrt((double)(n - m) * (n + m));
+      fac[BR + e] = std::sqrt(std::max(0.0, (double)(n - m) * (n - m - 1))) * (m >= 0 ? 1.0 : -1.0);
+      fac[CR + e] = std::sqrt(std::max(0.0, (double)(n + m) * (n + m - 1))) * (m >= 1 ? -1.0 : 1.0);
+      fac[AS + e] = -std::sqrt((double)(n + 1 - m) * (n + 1 + m));
+      fac[BS + e] = std::sqrt((double)(n + m + 1) * (n + m + 2)) * (m >= 0 ? 1.0 : -1.0);
+      fac[CS + e] = std::sqrt((double)(n - m + 1) * (n - m + 2)) * (m >= 1 ? -1.0 : 1.0);
+    }
+}
+
 void build_geometry_tables(int p, HostTables& H) {
   H = HostTables();
   H.p = p;
   const int P1 = p + 1;
   // --- factor table (layout mirrored by FacLayout in lap_kernels.hip)
-  {
-    const int P = p + 2, Hh = P * (P + 1) / 2, E = P * P;
-    H.fac.assign((size_t)2 * (P + 1) + 2 * Hh + 6 * E, 0.0);
-    const int MM = 0, M1 = MM + P + 1, RA = M1 + P + 1, RB = RA + Hh, AR = RB + Hh, BR = AR + E,
-              CR = BR + E, AS = CR + E, BS = AS + E, CS = BS + E;
-    for (int m = 1; m <= P; m++) H.fac[MM + m] = std::sqrt((2.0 * m - 1.0) / (2.0 * m));
-    for (int m = 0; m <= P; m++) H.fac[M1 + m] = std::sqrt(2.0 * m + 1.0);
-    for (int n = 0; n < P; n++)
-      for (int m = 0; m <= n; m++) {
-        int h = n * (n + 1) / 2 + m;
-        H.fac[RA + h] = (n >= m + 2) ? std::sqrt((n + m - 1.0) * (n - m - 1.0)) : 0.0;
-        H.fac[RB + h] = (n > m) ? 1.0 / std::sqrt((double)(n - m) * (n + m)) : 0.0;
-      }
-    for (int n = 0; n < P; n++)
-      for (int m = -n; m <= n; m++) {
-        int e = n * n + n + m;
-        H.fac[AR + e] = std::sqrt((double)(n - m) * (n + m));
-        H.fac[BR + e] = std::sqrt(std::max(0.0, (double)(n - m) * (n - m - 1))) * (m >= 0 ? 1.0 : -1.0);
-        H.fac[CR + e] = std::sqrt(std::max(0.0, (double)(n + m) * (n + m - 1))) * (m >= 1 ? -1.0 : 1.0);
-        H.fac[AS + e] = -std::sqrt((double)(n + 1 - m) * (n + 1 + m));
-        H.fac[BS + e] = std::sqrt((double)(n + m + 1) * (n + m + 2)) * (m >= 0 ? 1.0 : -1.0);
-        H.fac[CS + e] = std::sqrt((double)(n - m + 1) * (n - m + 2)) * (m >= 1 ? -1.0 : 1.0);
-      }
-  }
+  build_fac_table(p, H.fac);
   // --- rotation classes: one per distinct polar angle of the list-2 offsets
   //     and of the child-parent directions (+-1,+-1,+-1)
   std::map<long long, int> theta_class;

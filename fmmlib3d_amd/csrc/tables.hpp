@@ -28,13 +28,34 @@ struct HostTables {
   std::vector<double> z_m2l_t;                     // per distance class and order m: 4x4 tiles of [n-m][l-m]
   std::vector<long> zt_off;                        // start of order m within a class, p+2 entries
   long zt_size = 0;                                // doubles per distance class
+  // M2L on the matrix cores (16x16x4 MFMA operand order, see
+  // build_mfma_tables): per rotation class and degree n the forward and back
+  // rotation rows m' = 0..n in blocks (row tile of 16, column block of 4),
+  // each block the 64 A-operand values in lane order; per distance class and
+  // order m the z-shift rows l = m..p likewise
+  std::vector<double> mf_rot;                      // forward blocks, then back blocks at + mf_rot_back
+  std::vector<long> mf_rot_off;                    // nrot x (p+1)
+  long mf_rot_back = 0;
+  std::vector<double> mf_z;                        // ndist x mf_z_size
+  std::vector<long> mf_z_off;                      // start of order m within a class, p+1 entries
+  long mf_z_size = 0;
 };
+
+// Blocks of a degree-n rotation in the MFMA tables: row tiles and column
+// blocks of 4 (rows m' = 0..n, columns = the 2n+1 input orders).
+inline int mf_rot_tiles(int n) { return n / 16 + 1; }
+inline int mf_rot_kblocks(int n) { return (2 * n) / 4 + 1; }
+// z-shift of order m at order p: row tiles of l - m and column blocks of n - m
+inline int mf_z_tiles(int p, int m) { return (p - m) / 16 + 1; }
+inline int mf_z_kblocks(int p, int m) { return (p - m) / 4 + 1; }
 
 // l3dterms (src/l3dterms.f:34): expansion order for tolerance eps
 int laplace_nterms(double eps);
 // l3dterms_list2 (src/l3dterms.f:136): 7^3 table indexed by offset code
 std::vector<int> laplace_itable(double eps);
 void build_tables(int p, double eps, HostTables& H);
+// the factor table alone (FacLayout(p): degrees up to p+1), as built for order p
+void build_fac_table(int p, std::vector<double>& fac);
 // the kernel-independent part (factors, rotation classes and blocks, phases,
 // distance classes) that build_tables starts from
 void build_geometry_tables(int p, HostTables& H);

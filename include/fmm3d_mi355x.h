@@ -32,7 +32,13 @@ extern "C" {
  * positions, offsets and output accumulation stay FP64 in every mode.  A
  * mode is honoured while FP32 can meet the tolerance of iprec: at iprec >= 3
  * (0.5e-9 and tighter, src/lfmm3dpart.f:277-285) every mode runs FP64
- * throughout; fmm3d_last_precision() tells which mode a call ran in. */
+ * throughout; at iprec 2 (0.5e-6) the FP32 expansions of FAR32 and FP32
+ * run only on an octree whose leaves all lie on one level, and in FP64 on
+ * an adaptive one (boxes of different sizes evaluate expansions near their
+ * radius of convergence, where FP32 rounding reached 6e-7).  In the FP32
+ * near field the closest pairs (under 1/32 of the leaf side for Laplace,
+ * 1/8 for Helmholtz) run in FP64.
+ * fmm3d_last_precision() tells which mode a call ran in. */
 #define FMM3D_PREC_FP64 0      /* every stage in FP64 (the default)                 */
 #define FMM3D_PREC_MIXED 1     /* FP32 near-field pair arithmetic, FP64 far field   */
 #define FMM3D_PREC_FAR32 2     /* FP32 expansions and translations, FP64 near field */
@@ -59,6 +65,22 @@ void fmm3d_l3dpartdirect(int nsource, const double *source,
                          int ifpot, double *pot, int iffld, double *fld,
                          int ntarget, const double *target,
                          int ifpottarg, double *pottarg, int iffldtarg, double *fldtarg);
+
+/* fmm3d_lfmm3dparttarg with Hessians: hess(6,nsource) and hesstarg(6,ntarget)
+ * complex*16, per point (pot_xx, pot_yy, pot_zz, pot_xy, pot_xz, pot_yz) of the
+ * same potential, written when ifhess / ifhesstarg is 1 (other outputs as
+ * fmm3d_lfmm3dparttarg).  FP64 throughout.  Extension: the reference's
+ * particle FMM returns pot and fld only; the quantities and their order are
+ * those of its library routines lpotfld3dhess / lpotfld3dhess_dp
+ * (src/laprouts3d.f:5442, :5619) and l3dtaevalhess / l3dmpevalhess (:5297,
+ * :5229). */
+int fmm3d_lfmm3dparttarg_hess(int iprec, int nsource, const double *source,
+                              int ifcharge, const double *charge,
+                              int ifdipole, const double *dipstr, const double *dipvec,
+                              int ifpot, double *pot, int iffld, double *fld, int ifhess, double *hess,
+                              int ntarget, const double *target,
+                              int ifpottarg, double *pottarg, int iffldtarg, double *fldtarg,
+                              int ifhesstarg, double *hesstarg);
 
 /* Same as fmm3d_lfmm3dparttarg with an explicit precision mode (FMM3D_PREC_*).
  * Extension with no reference counterpart: the reference has FP64 only. */
@@ -195,7 +217,12 @@ long long fmm3d_server_submit(void *server, int kernel, int prec_mode, int iprec
  * pending (block = 0), -1 for an unknown id. */
 int fmm3d_server_wait(void *server, long long id, int block, double *pot, double *fld,
                       double *pottarg, double *fldtarg, int *ier, double *latency_ms);
-void fmm3d_server_stats(void *server, long long *queued, int *running, long long *bytes_in_use);
+/* Queue length, busy workers, device-memory estimate in use, and the number
+ * of requests served so far in fused batches (several small Laplace requests
+ * with the same tolerance, precision and output flags, run as one forest in
+ * one graph-captured launch sequence).  Any pointer may be NULL. */
+void fmm3d_server_stats(void *server, long long *queued, int *running, long long *bytes_in_use,
+                        long long *fused_requests);
 /* Finishes the queued requests, then releases the workers and their memory. */
 void fmm3d_server_destroy(void *server);
 

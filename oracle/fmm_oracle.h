@@ -92,6 +92,11 @@ void orc_l3dpartdirect(int ns, const double *src, int ifcharge, const zdouble *q
                        int ifpot, zdouble *pot, int iffld, zdouble *fld,
                        int nt, const double *trg, int ifpottarg, zdouble *pottarg,
                        int iffldtarg, zdouble *fldtarg);
+/* Hessians (pot_xx, pot_yy, pot_zz, pot_xy, pot_xz, pot_yz) of the direct sum
+ * at nt points (laprouts3d.f:5442 lpotfld3dhess, :5619 lpotfld3dhess_dp) */
+void orc_l3ddirect_hess_sample(int ns, const double *src, int ifcharge, const zdouble *q,
+                               int ifdipole, const zdouble *ds, const double *dv,
+                               int nt, const double *trg, const int *self_index, zdouble *hess);
 /* direct sum evaluated at a subset of targets/sources given by index lists */
 void orc_l3ddirect_sample(int ns, const double *src, int ifcharge, const zdouble *q,
                           int ifdipole, const zdouble *ds, const double *dv,

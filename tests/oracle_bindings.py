@@ -153,6 +153,49 @@ def orc_direct_sample(source, points, self_index, charge=None, dipstr=None, dipv
     return pot, fld
 
 
+def orc_direct_hess_sample(source, points, self_index, charge=None, dipstr=None, dipvec=None):
+    """FP64 direct-sum Hessians (n, 6) at `points` (order pot_xx, pot_yy,
+    pot_zz, pot_xy, pot_xz, pot_yz); self_index[j] >= 0 skips that source."""
+    o = oracle()
+    src = _c(source, np.float64)
+    pts = _c(points, np.float64)
+    si = _c(self_index, np.int32)
+    q = _c(charge, np.complex128)
+    ds = _c(dipstr, np.complex128)
+    dv = _c(dipvec, np.float64)
+    hs = np.zeros((pts.shape[0], 6), np.complex128)
+    o.orc_l3ddirect_hess_sample(src.shape[0], _vp(src), int(q is not None), _vp(q), int(ds is not None), _vp(ds),
+                                _vp(dv), pts.shape[0], _vp(pts), _vp(si), _vp(hs))
+    return hs
+
+
+def ref_lpotfld3dhess(source, charge, target):
+    """The reference's lpotfld3dhess (one charge, one target): pot, fld, hess."""
+    r = ref()
+    s, t = _c(source, np.float64), _c(target, np.float64)
+    q = np.array([charge], np.complex128)
+    pot = np.zeros(1, np.complex128)
+    fld = np.zeros(3, np.complex128)
+    hs = np.zeros(6, np.complex128)
+    one = ctypes.c_int(1)
+    r.lpotfld3dhess_(ctypes.byref(one), ctypes.byref(one), _vp(s), _vp(q), _vp(t), _vp(pot), _vp(fld), _vp(hs))
+    return pot[0], fld, hs
+
+
+def ref_lpotfld3dhess_dp(source, dipstr, dipvec, target):
+    """The reference's lpotfld3dhess_dp (one dipole, one target): pot, fld, hess."""
+    r = ref()
+    s, t, dv = _c(source, np.float64), _c(target, np.float64), _c(dipvec, np.float64)
+    ds = np.array([dipstr], np.complex128)
+    pot = np.zeros(1, np.complex128)
+    fld = np.zeros(3, np.complex128)
+    hs = np.zeros(6, np.complex128)
+    one = ctypes.c_int(1)
+    r.lpotfld3dhess_dp_(ctypes.byref(one), ctypes.byref(one), _vp(s), _vp(ds), _vp(dv), _vp(t), _vp(pot), _vp(fld),
+                        _vp(hs))
+    return pot[0], fld, hs
+
+
 class ZC(ctypes.Structure):
     """C `double _Complex` passed by value (two doubles, SysV classes SSE,SSE)."""
     _fields_ = [("re", ctypes.c_double), ("im", ctypes.c_double)]

@@ -64,3 +64,29 @@ def test_server_rejects_bad_requests(rng):
             srv.submit(1, 1, src, q)            # Helmholtz without zk
         rid = srv.submit(0, 1, src, q)
         assert srv.wait(rid)[0] == 0
+
+
+@pytest.mark.parametrize("iprec,ifd,with_targets", [(1, 0, False), (2, 1, True)])
+def test_server_fuses_small_requests(rng, iprec, ifd, with_targets):
+    """Small Laplace requests queued together with the same parameters run as
+    one forest in one graph-captured launch sequence; each result is bitwise
+    the result of the same call made alone.  Two rounds on the same server,
+    so the workers reuse their forest storage."""
+    with F.Server(nworkers=2, mem_budget=16 << 30) as srv:
+        for rnd in range(2):
+            reqs = []
+            for k in range(24):
+                n = int(rng.integers(1000, 20000))
+                src = points(rng, n, "clusters" if k % 2 else "cube")
+                trg = points(rng, int(rng.integers(100, 3000)), "cube") if with_targets else None
+                q, ds, dv = strengths(rng, n, True, bool(ifd))
+                reqs.append((iprec, None, src, q, ds, dv, trg))
+            alone = [_alone(*r) for r in reqs]
+            ids = [srv.submit(0, iprec, src, q, ds, dv, trg) for (iprec, zk, src, q, ds, dv, trg) in reqs]
+            outs = [srv.wait(i) for i in ids]
+            st = srv.stats()
+            print("round", rnd, "fused requests so far:", st["fused_requests"])
+            assert st["fused_requests"] >= 4 * (rnd + 1)
+            for o, a in zip(outs, alone):
+                assert o[0] == 0 and a[0] == 0
+                _same(o[1:5], a[1:5])
