@@ -346,6 +346,42 @@ def dropin_leg(F, prec, src, q, ds, dv, trg, steps=3):
             "ms_per_call": ms, "targets_per_s": N / (ms * 1e-3), "calls": steps}
 
 
+def triangle_leg(F, ob, level=7, ntarget=100_000, iprec=2, steps=3):
+    """Triangle FMM (lfmm3dtriatarg): a triangulated unit sphere (icosahedron
+    refined `level` times: 327,680 triangles at 7), single and dipole layer
+    with N(0,1) complex densities, pot+fld at the centroids and at targets in
+    the shell 0.9 < r < 1.1, iprec 2, FP64, host arrays in and out; checked
+    against the oracle's analytic direct sum at 100 centroids and 100 targets."""
+    import numpy as np
+    from fmm_inputs import icosphere, rel
+    rng = np.random.default_rng(77)
+    T = icosphere(level)
+    n = T.shape[0]
+    q = rng.normal(size=n) + 1j * rng.normal(size=n)
+    mu = rng.normal(size=n) + 1j * rng.normal(size=n)
+    x = rng.normal(size=(ntarget, 3))
+    trg = x / np.linalg.norm(x, axis=1)[:, None] * rng.uniform(0.9, 1.1, ntarget)[:, None]
+    F.lfmm3dtriatarg(iprec, T, q, mu, trg)   # warm-up
+    lat, stages = [], []
+    for _ in range(steps):
+        a = time.perf_counter()
+        ier, pot, fld, pt, ft = F.lfmm3dtriatarg(iprec, T, q, mu, trg)
+        lat.append(time.perf_counter() - a)
+        stages.append(F.last_timings())
+        assert ier == 0
+    idx = rng.choice(n, 100, replace=False)
+    it = rng.choice(ntarget, 100, replace=False)
+    po, fo = ob.orc_tria_direct_sample(T, T.mean(axis=1)[idx], idx.astype(np.int32), q, mu)
+    pto, fto = ob.orc_tria_direct_sample(T, trg[it], None, q, mu)
+    ms = float(np.median(lat)) * 1e3
+    return {"workload": "lfmm3dtriatarg: %d triangles (icosphere level %d) + %d targets, single + dipole layer, "
+                        "iprec %d, FP64, host arrays" % (n, level, ntarget, iprec),
+            "ms_per_call": ms, "points_per_s": (n + ntarget) / (ms * 1e-3),
+            "err_pot": rel(pot[idx], po), "err_fld": rel(fld[idx], fo),
+            "err_pottarg": rel(pt[it], pto), "err_fldtarg": rel(ft[it], fto),
+            "stages_ms": {k: float(np.mean([s_[k] for s_ in stages])) for k in stages[0]}}
+
+
 def run_mode(F, torch, prec, dev_in, dev_out, steps, warmup):
     ts, tq, tds, tdv, tt = dev_in
     pot, fld, pt, ft = dev_out
@@ -382,6 +418,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-helm", action="store_true", help="skip the Helmholtz (config 3) line")
+    ap.add_argument("--no-tria", action="store_true", help="skip the triangle FMM line")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving-stream line")
     ap.add_argument("--serve-requests", type=int, default=512)
     ap.add_argument("--serve-workers", type=int, default=4)
@@ -470,6 +507,11 @@ def main():
         print("# helmholtz: %.1f ms/step, err pot %.2e fld %.2e, stages %s"
               % (helm["ms_per_step"], helm["err_pot"], helm["err_fld"],
                  {k: round(v, 2) for k, v in helm["stages_ms"].items()}), file=sys.stderr, flush=True)
+    tria = None if args.no_tria else triangle_leg(F, ob)
+    if tria:
+        print("# triangles: %.1f ms/call, err pot %.2e fld %.2e pottarg %.2e fldtarg %.2e, stages %s"
+              % (tria["ms_per_call"], tria["err_pot"], tria["err_fld"], tria["err_pottarg"], tria["err_fldtarg"],
+                 {k: round(v, 2) for k, v in tria["stages_ms"].items()}), file=sys.stderr, flush=True)
     serve = None if args.no_serve else serving_leg(F, args.serve_requests, args.serve_workers)
     if serve:
         print("# serving: p50 %.1f ms p99 %.1f ms, saturated %.2f req/s (%.3e points/s)"
@@ -514,6 +556,7 @@ def main():
         "dropin_host_arrays": dropin,
         "modes": {k: {kk: vv for kk, vv in v.items() if kk != "latencies_ms"} for k, v in modes.items()},
         "helmholtz": helm,
+        "triangles": tria,
         "serving": serve,
         "stream5": s5,
     }

@@ -37,6 +37,7 @@ EXPORTED = (
     "fmm3d_hfmm3dparttarg_ex", "fmm3d_hfmm3dparttarg_dev",
     "fmm3d_server_create", "fmm3d_server_submit", "fmm3d_server_wait", "fmm3d_server_stats",
     "fmm3d_server_destroy",
+    "fmm3d_lfmm3dtriatarg", "fmm3d_lfmm3dtriaself", "fmm3d_l3dtriadirect",
 )
 
 _lib = None
@@ -102,6 +103,13 @@ def load():
         lib.fmm3d_last_timings.restype = None
         lib.fmm3d_last_precision.argtypes = []
         lib.fmm3d_last_precision.restype = i
+        ttarg = [i, i, vp, vp, vp, i, vp, i, vp, vp, i, vp, i, vp, i, vp, i, vp, i, vp]
+        lib.fmm3d_lfmm3dtriatarg.argtypes = ttarg
+        lib.fmm3d_lfmm3dtriatarg.restype = i
+        lib.fmm3d_lfmm3dtriaself.argtypes = ttarg[:14]
+        lib.fmm3d_lfmm3dtriaself.restype = i
+        lib.fmm3d_l3dtriadirect.argtypes = ttarg[1:]
+        lib.fmm3d_l3dtriadirect.restype = None
         _lib = lib
     return _lib
 
@@ -187,6 +195,63 @@ def lfmm3dparttarg_hess(iprec, source, charge=None, dipstr=None, dipvec=None, ta
         int(bool(ifpot)), _ptr(pot), int(bool(iffld)), _ptr(fld), int(hs is not None), _ptr(hs),
         nt, _ptr(trg), int(pt is not None), _ptr(pt), int(ft is not None), _ptr(ft), int(ht is not None), _ptr(ht))
     return ier, pot, fld, hs, pt, ft, ht
+
+
+def _tri_prep(triaflat, charge, dipstr, target, source):
+    tri = np.ascontiguousarray(triaflat, np.float64).reshape(-1, 3, 3)   # (n, vertex, xyz)
+    n = tri.shape[0]
+    cen = tri.mean(axis=1) if source is None else np.ascontiguousarray(source, np.float64).reshape(n, 3)
+    nrm = np.cross(tri[:, 1] - tri[:, 0], tri[:, 2] - tri[:, 0])
+    nrm = np.ascontiguousarray(nrm / np.linalg.norm(nrm, axis=1)[:, None])
+    q = None if charge is None else np.ascontiguousarray(charge, np.complex128).reshape(n)
+    ds = None if dipstr is None else np.ascontiguousarray(dipstr, np.complex128).reshape(n)
+    trg = None if target is None else np.ascontiguousarray(target, np.float64).reshape(-1, 3)
+    return np.ascontiguousarray(tri), cen, nrm, q, ds, trg
+
+
+def lfmm3dtriatarg(iprec, triaflat, charge=None, dipstr=None, target=None, source=None,
+                   ifpot=1, iffld=1, ifpottarg=1, iffldtarg=1):
+    """Triangle FMM (lfmm3dtriatarg, src/lfmm3dtria.f:121): triaflat is
+    (n, 3, 3) = per triangle its three vertices; charge / dipstr the
+    single- and dipole-layer densities (the dipole layer along the normal
+    (v2-v1)x(v3-v1)); pot/fld at `source` (default: the centroids) and at
+    target.  Returns (ier, pot, fld, pottarg, fldtarg)."""
+    lib = load()
+    tri, cen, nrm, q, ds, trg = _tri_prep(triaflat, charge, dipstr, target, source)
+    n = tri.shape[0]
+    nt = 0 if trg is None else trg.shape[0]
+    pot = np.zeros(n, np.complex128)
+    fld = np.zeros((n, 3), np.complex128)
+    pt = np.zeros(max(nt, 1), np.complex128)
+    ft = np.zeros((max(nt, 1), 3), np.complex128)
+    ier = lib.fmm3d_lfmm3dtriatarg(
+        int(iprec), n, _ptr(tri), _ptr(nrm), _ptr(cen), int(q is not None), _ptr(q), int(ds is not None), _ptr(ds),
+        _ptr(nrm), int(bool(ifpot)), _ptr(pot), int(bool(iffld)), _ptr(fld), nt, _ptr(trg),
+        int(bool(ifpottarg) and nt > 0), _ptr(pt), int(bool(iffldtarg) and nt > 0), _ptr(ft))
+    return ier, pot, fld, pt[:nt], ft[:nt]
+
+
+def lfmm3dtriaself(iprec, triaflat, charge=None, dipstr=None, source=None, ifpot=1, iffld=1):
+    """Triangle FMM without targets (lfmm3dtriaself): (ier, pot, fld)."""
+    g = lfmm3dtriatarg(iprec, triaflat, charge, dipstr, None, source, ifpot, iffld, 0, 0)
+    return g[0], g[1], g[2]
+
+
+def l3dtriadirect(triaflat, charge=None, dipstr=None, target=None, source=None):
+    """Direct triangle sum on the device (l3dtriadirect, src/lfmm3dtria.f:1385):
+    (pot, fld, pottarg, fldtarg)."""
+    lib = load()
+    tri, cen, nrm, q, ds, trg = _tri_prep(triaflat, charge, dipstr, target, source)
+    n = tri.shape[0]
+    nt = 0 if trg is None else trg.shape[0]
+    pot = np.zeros(n, np.complex128)
+    fld = np.zeros((n, 3), np.complex128)
+    pt = np.zeros(max(nt, 1), np.complex128)
+    ft = np.zeros((max(nt, 1), 3), np.complex128)
+    lib.fmm3d_l3dtriadirect(n, _ptr(tri), _ptr(nrm), _ptr(cen), int(q is not None), _ptr(q), int(ds is not None),
+                            _ptr(ds), _ptr(nrm), 1, _ptr(pot), 1, _ptr(fld), nt, _ptr(trg), int(nt > 0), _ptr(pt),
+                            int(nt > 0), _ptr(ft))
+    return pot, fld, pt[:nt], ft[:nt]
 
 
 def lfmm3dpartself(iprec, source, charge=None, dipstr=None, dipvec=None, ifpot=1, iffld=1,

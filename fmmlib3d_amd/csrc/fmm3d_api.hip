@@ -86,8 +86,10 @@ int fmm_host(const double* zk, int prec_mode, int iprec, int ns, const double* s
              const double* charge, int ifdipole, const double* dipstr, const double* dipvec,
              int ifpot, double* pot, int iffld, double* fld, int nt, const double* target,
              int ifpottarg, double* pottarg, int iffldtarg, double* fldtarg, int ifhess = 0,
-             double* hess = nullptr, int ifhesstarg = 0, double* hesstarg = nullptr) {
+             double* hess = nullptr, int ifhesstarg = 0, double* hesstarg = nullptr,
+             const double* tria = nullptr, bool tri_mode = false) {
   if (int e = check_args(zk, prec_mode, iprec, ns, nt)) return e;
+  if (tri_mode && !tria) return 128;
   if (int e = check_ptrs(ns, source, ifcharge, charge, ifdipole, dipstr, dipvec, ifpot, pot, iffld, fld, nt,
                          target, ifpottarg, pottarg, iffldtarg, fldtarg))
     return e;
@@ -108,6 +110,7 @@ int fmm_host(const double* zk, int prec_mode, int iprec, int ns, const double* s
     c.ds_dev = c.ifdipole ? upload(ctx, "in_ds", dipstr, 2 * (size_t)ns) : nullptr;
     c.dv_dev = c.ifdipole ? upload(ctx, "in_dv", dipvec, 3 * (size_t)ns) : nullptr;
     c.trg_dev = nt > 0 ? upload(ctx, "in_trg", target, 3 * (size_t)nt) : nullptr;
+    if (tri_mode) c.tri_dev = upload(ctx, "in_tri", tria, 9 * (size_t)ns);
     if (c.ifpot) c.pot_dev = (double*)ctx.arena.get("out_pot", 2 * sizeof(double) * ns);
     if (c.iffld) c.fld_dev = (double*)ctx.arena.get("out_fld", 6 * sizeof(double) * ns);
     if (c.ifpottarg) c.pottarg_dev = (double*)ctx.arena.get("out_pt", 2 * sizeof(double) * nt);
@@ -356,8 +359,10 @@ namespace {
 void direct_host(const double* zk, int nsource, const double* source, int ifcharge, const double* charge,
                  int ifdipole, const double* dipstr, const double* dipvec, int ifpot, double* pot,
                  int iffld, double* fld, int ntarget, const double* target, int ifpottarg,
-                 double* pottarg, int iffldtarg, double* fldtarg) {
+                 double* pottarg, int iffldtarg, double* fldtarg, const double* tria = nullptr,
+                 bool tri_mode = false) {
   if (nsource < 0 || ntarget < 0) return;
+  if (tri_mode && nsource > 0 && !tria) return;
   if (check_ptrs(nsource, source, ifcharge, charge, ifdipole, dipstr, dipvec, ifpot, pot, iffld, fld, ntarget,
                  target, ifpottarg, pottarg, iffldtarg, fldtarg))
     return;  // no error channel: a call with a missing array computes nothing
@@ -379,7 +384,8 @@ void direct_host(const double* zk, int nsource, const double* source, int ifchar
     if (c.iffld) c.fld_dev = (double*)ctx.arena.get("out_fld", 6 * sizeof(double) * std::max(nsource, 1));
     if (c.ifpottarg) c.pottarg_dev = (double*)ctx.arena.get("out_pt", 2 * sizeof(double) * std::max(ntarget, 1));
     if (c.iffldtarg) c.fldtarg_dev = (double*)ctx.arena.get("out_ft", 6 * sizeof(double) * std::max(ntarget, 1));
-    int ier = zk ? helm_direct(ctx, c) : lap_direct(ctx, c);
+    if (tri_mode) c.tri_dev = upload(ctx, "in_tri", tria, 9 * (size_t)nsource);
+    int ier = tri_mode ? tri_direct(ctx, c) : zk ? helm_direct(ctx, c) : lap_direct(ctx, c);
     if (ier) return ier;
     if (c.ifpot) download(ctx, pot, c.pot_dev, 2 * (size_t)nsource);
     if (c.iffld) download(ctx, fld, c.fld_dev, 6 * (size_t)nsource);
@@ -409,6 +415,34 @@ void fmm3d_h3dpartdirect(const double* zk, int nsource, const double* source, in
   if (!zk) return;
   direct_host(zk, nsource, source, ifcharge, charge, ifdipole, dipstr, dipvec, ifpot, pot, iffld,
               fld, ntarget, target, ifpottarg, pottarg, iffldtarg, fldtarg);
+}
+
+int fmm3d_lfmm3dtriatarg(int iprec, int nsource, const double* triaflat, const double* trianorm,
+                         const double* source, int ifcharge, const double* charge, int ifdipole,
+                         const double* dipstr, const double* dipvec, int ifpot, double* pot, int iffld, double* fld,
+                         int ntarget, const double* target, int ifpottarg, double* pottarg, int iffldtarg,
+                         double* fldtarg) {
+  (void)trianorm;  // the dipole layer is along the normal of the vertex order, as in the reference
+  return fmm_host(nullptr, FMM3D_PREC_FP64, iprec, nsource, source, ifcharge, charge, ifdipole, dipstr, dipvec,
+                  ifpot, pot, iffld, fld, ntarget, target, ifpottarg, pottarg, iffldtarg, fldtarg, 0, nullptr, 0,
+                  nullptr, triaflat, true);
+}
+
+int fmm3d_lfmm3dtriaself(int iprec, int nsource, const double* triaflat, const double* trianorm,
+                         const double* source, int ifcharge, const double* charge, int ifdipole,
+                         const double* dipstr, const double* dipvec, int ifpot, double* pot, int iffld,
+                         double* fld) {
+  return fmm3d_lfmm3dtriatarg(iprec, nsource, triaflat, trianorm, source, ifcharge, charge, ifdipole, dipstr,
+                              dipvec, ifpot, pot, iffld, fld, 0, nullptr, 0, nullptr, 0, nullptr);
+}
+
+void fmm3d_l3dtriadirect(int nsource, const double* triaflat, const double* trianorm, const double* source,
+                         int ifcharge, const double* charge, int ifdipole, const double* dipstr,
+                         const double* dipvec, int ifpot, double* pot, int iffld, double* fld, int ntarget,
+                         const double* target, int ifpottarg, double* pottarg, int iffldtarg, double* fldtarg) {
+  (void)trianorm;
+  direct_host(nullptr, nsource, source, ifcharge, charge, ifdipole, dipstr, dipvec, ifpot, pot, iffld, fld,
+              ntarget, target, ifpottarg, pottarg, iffldtarg, fldtarg, triaflat, true);
 }
 
 void fmm3d_last_timings(double* ms, int n) {

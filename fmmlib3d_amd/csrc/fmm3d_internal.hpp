@@ -32,6 +32,10 @@ struct LapCall {
   // pot_yz; 6 complex per point) at the sources and at the targets
   int ifhess = 0, ifhesstarg = 0;
   double *hess_dev = nullptr, *hesstarg_dev = nullptr;
+  // Laplace triangle FMM (FP64): the triangles' vertices (9 doubles each, in
+  // the caller's order); src_dev then holds the evaluation points on them
+  // (centroids), q_dev / ds_dev the single- and dipole-layer densities
+  const double* tri_dev = nullptr;
 };
 
 // The precision mode a call runs in: the requested FMM3D_PREC_* mode while
@@ -55,6 +59,8 @@ int lap_solve(Context& ctx, const LapCall& c);
 // precision mode bitmask (FMM3D_PREC_* in include/fmm3d_mi355x.h)
 int lap_solve_mode(Context& ctx, const LapCall& c, int mode);
 int lap_direct(Context& ctx, const LapCall& c);
+// l3dtriadirect on the device (c.tri_dev set)
+int tri_direct(Context& ctx, const LapCall& c);
 
 // Helmholtz (hfmm3dparttarg / h3dpartdirect); mode 0 = FP64, 1 = FP32 near field
 int helm_solve_mode(Context& ctx, const LapCall& c, int mode);

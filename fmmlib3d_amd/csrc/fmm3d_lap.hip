@@ -22,6 +22,7 @@
 #include "lap_kernels.hip"
 #include "m2l_mfma.hpp"
 #include "hess_kernels.hpp"
+#include "tria_kernels.hpp"
 #include "octree.hpp"
 #include "plan.hpp"
 #include "runtime.hpp"
@@ -213,6 +214,60 @@ const double* hess_factors(Context& ctx, int p) {
   return d;
 }
 
+// Leaf size of the triangle FMM (src/lfmm3dtria.f:286-301): 8/3, 15/3,
+// 30/3, 60/3, 120/3, 240/3, 480/3, 700/3 in integer arithmetic for iprec
+// -2..5, every triangle and target in one leaf at iprec 6.
+int tri_nbox(int iprec, int ns, int nt) {
+  static const int nb[] = {2, 5, 10, 20, 40, 80, 160, 233};
+  return iprec == 6 ? ns + nt : nb[iprec + 2];
+}
+
+// Gauss-Legendre nodes and weights on [0, 1], q points (Newton on P_q)
+void gauss01(int q, std::vector<double>& x, std::vector<double>& w) {
+  x.assign(q, 0.0);
+  w.assign(q, 0.0);
+  const double pi = 3.14159265358979323846;
+  for (int i = 0; i < q; i++) {
+    double t = std::cos(pi * (i + 0.75) / (q + 0.5)), dp = 1.0;
+    for (int it = 0; it < 100; it++) {
+      double p0 = 1.0, p1 = t;
+      for (int k = 2; k <= q; k++) {
+        const double p2 = ((2.0 * k - 1.0) * t * p1 - (k - 1.0) * p0) / k;
+        p0 = p1;
+        p1 = p2;
+      }
+      const double pq = q == 1 ? t : p1, pm = q == 1 ? 1.0 : p0;
+      dp = q * (t * pq - pm) / (t * t - 1.0);
+      const double dt = pq / dp;
+      t -= dt;
+      if (std::fabs(dt) < 1e-16) break;
+    }
+    x[i] = 0.5 * (1.0 - t);
+    w[i] = 1.0 / ((1.0 - t * t) * dp * dp);  // 2/((1-t^2) P'^2) on [-1,1], halved
+  }
+}
+
+// Quadrature on the reference triangle (xi, eta >= 0, xi + eta <= 1; weights
+// summing to its area 1/2) used to turn a triangle into weighted points for
+// P2M / P2L: the square [0,1]^2 collapsed onto it (xi = u(1-v), eta = u v,
+// Jacobian u) with q-point Gauss rules, exact for polynomials of degree
+// 2q - 2.  q by iprec: 2 (iprec <= -1), 3 (iprec 0), 4 (iprec 1, 2), 5
+// (iprec >= 3): degrees 2, 4, 6, 8, the reference's symmetric rules being of
+// order 2, 4 and 6 (lfmm3dtria.f:680-701).
+std::vector<double> tri_rule(int iprec) {
+  const int q = iprec <= -1 ? 2 : iprec == 0 ? 3 : iprec <= 2 ? 4 : 5;
+  std::vector<double> x, w, r;
+  gauss01(q, x, w);
+  for (int i = 0; i < q; i++)
+    for (int j = 0; j < q; j++) {
+      const double u = x[i], v = x[j];
+      r.push_back(u * (1.0 - v));
+      r.push_back(u * v);
+      r.push_back(w[i] * w[j] * u);
+    }
+  return r;
+}
+
 }  // namespace
 
 template <typename T, typename TN>
@@ -221,6 +276,12 @@ int lap_solve(Context& ctx, const LapCall& c) {
   double eps;
   int nbox;
   if (!iprec_params(c.iprec, c.ns, c.nt, eps, nbox)) return 32;
+  const bool tri = c.tri_dev != nullptr;
+  if (tri) {
+    if (!(std::is_same<T, double>::value && std::is_same<TN, double>::value) || c.ifhess || c.ifhesstarg)
+      return 128;
+    nbox = tri_nbox(c.iprec, c.ns, c.nt);
+  }
   const int ns = c.ns, nt = c.nt;
   const Flags f{c.ifcharge ? 1 : 0, c.ifdipole ? 1 : 0, c.ifpot ? 1 : 0, c.iffld ? 1 : 0,
                 nt > 0 && c.ifpottarg ? 1 : 0, nt > 0 && c.iffldtarg ? 1 : 0};
@@ -314,6 +375,16 @@ int lap_solve(Context& ctx, const LapCall& c) {
   const int* tperm = dsperm ? dtperm : dI + O.tperm;
   P.src_perm = sperm;
   P.trg_perm = tperm;
+  // triangle FMM: frames and densities in sorted order, and the quadrature
+  // points that stand for the triangles in P2M / P2L (PF: the plan with each
+  // box's source range counted in points)
+  const double* fpos = spos;
+  const double* fstr = str64;
+  double *tgeo = nullptr, *tstr = nullptr;
+  int* tri_rng = nullptr;
+  std::vector<double> rule;
+  if (tri) rule = tri_rule(c.iprec);
+  const int nq = (int)rule.size() / 3;
   // graph mode: every launch and copy from here to the scatter, on both
   // streams (the near-field stream joins through the events), is captured
   // and then launched as one graph; buffers are zeroed by kernels, not
@@ -353,6 +424,23 @@ int lap_solve(Context& ctx, const LapCall& c) {
     zero(pot_tn, 2 * (size_t)nt);
     zero(fld_tn, 6 * (size_t)nt);
   }
+  if (tri) {
+    tgeo = (double*)ctx.arena.get("tri_geo", sizeof(double) * kTriGeo * (size_t)ns);
+    tstr = (double*)ctx.arena.get("tri_str", sizeof(double) * 4 * (size_t)ns);
+    double* npos = (double*)ctx.arena.get("tri_npos", sizeof(double) * 4 * (size_t)ns * nq);
+    double* nstr = (double*)ctx.arena.get("tri_nstr", sizeof(double) * 8 * (size_t)ns * nq);
+    double* hr = (double*)ctx.arena.get_host("tri_rule_h", sizeof(double) * rule.size());
+    std::memcpy(hr, rule.data(), sizeof(double) * rule.size());
+    double* dr = (double*)ctx.arena.get("tri_rule", sizeof(double) * rule.size());
+    FMM_HIP(hipMemcpyAsync(dr, hr, sizeof(double) * rule.size(), hipMemcpyHostToDevice, s));
+    hipLaunchKernelGGL(k_tri_prep, dim3(blocks(ns, 256)), dim3(256), 0, s, ns, sperm, c.tri_dev, c.q_dev,
+                       c.ds_dev, f.ifcharge, f.ifdipole, nq, dr, tgeo, tstr, npos, nstr);
+    tri_rng = (int*)ctx.arena.get("tri_ranges", sizeof(int) * 2 * nb);
+    hipLaunchKernelGGL(k_scale_ranges, dim3(blocks(nb, 256)), dim3(256), 0, s, (int)nb, P.isrc, P.nsrc, nq, tri_rng,
+                       tri_rng + nb);
+    fpos = npos;
+    fstr = nstr;
+  }
   FMM_HIP(hipEventRecord(ctx.aux_ev[0], s));  // inputs gathered, sums zeroed
   mark(1);
   const double* fac = Tb.fac;
@@ -367,12 +455,17 @@ int lap_solve(Context& ctx, const LapCall& c) {
     const int H = (pt + 1) * (pt + 2) / 2;
     const size_t P2 = (size_t)(pt + 1) * (pt + 1);
     const int HS = form2_row_stride<T>(H);
+    DevPlan PF = P;  // the current plan (P2L needs list 4), source ranges in points for triangles
+    if (tri) {
+      PF.isrc = tri_rng;
+      PF.nsrc = tri_rng + nb;
+    }
     int ts = (int)(65536 / (2 * sizeof(T) * (size_t)HS));
     ts = std::max(1, std::min(32, ts));
     const size_t lds2 = form2_fac_bytes(p) + 2 * sizeof(T) * std::max((size_t)ts * HS, 4 * P2) + (kFormScStride + 8) * sizeof(T) * ts;
 #define FORM2(C, D, I)                                                                                 \
   if (f.ifcharge == C && f.ifdipole == D)                                                              \
-    hipLaunchKernelGGL((k_form2<T, IR, C, D, I>), dim3(nbox_form), dim3(NT), lds2, s, P, fac, spos, str64, ts, dst);
+    hipLaunchKernelGGL((k_form2<T, IR, C, D, I>), dim3(nbox_form), dim3(NT), lds2, s, PF, fac, fpos, fstr, ts, dst);
     const bool lds2_fits = lds2 <= max_lds_bytes(ctx);
     if (H <= I1 * NT && lds2_fits) {
       FORM2(1, 0, I1) FORM2(0, 1, I1) FORM2(1, 1, I1)
@@ -387,7 +480,7 @@ int lap_solve(Context& ctx, const LapCall& c) {
       if (ts1 < 1) throw AllocError("P2M/P2L tile does not fit LDS", 16);
       const int ts = (int)std::min<long>(32, ts1);
       const size_t lds = sizeof(T) * (2 * (size_t)K + (size_t)ts * (2 * H + 11));
-      hipLaunchKernelGGL((k_form<T, IR>), dim3(nbox_form), dim3(256), lds, s, P, fac, spos, str64,
+      hipLaunchKernelGGL((k_form<T, IR>), dim3(nbox_form), dim3(256), lds, s, PF, fac, fpos, fstr,
                          f.ifcharge, f.ifdipole, ts, dst);
     }
   };
@@ -414,7 +507,23 @@ int lap_solve(Context& ctx, const LapCall& c) {
   const int* dIB = upload_ints("plan_ib", FB.I, s2);  // part B1: offsets in O from build_plan_near
   devplan_near(P, O, dIB);
   const int* dIB_near = dIB;
-  if (f.ifcharge || f.ifdipole) {
+  if (tri && (f.ifcharge || f.ifdipole)) {
+    auto tri_near = [&](const int* items, int n, int kind, double* po, double* fo) {
+      if (n == 0) return;
+      DevPlan Q = P;
+      Q.p2p_items = items;
+      Q.n_p2p = n;
+      dim3 g(item_groups(n)), bl(kP2PBlock * kItemWaves);
+      if (f.ifcharge && f.ifdipole)
+        hipLaunchKernelGGL((k_tri_p2p<true, true>), g, bl, 0, s2, Q, tgeo, tstr, spos, tpos, kind, po, fo);
+      else if (f.ifcharge)
+        hipLaunchKernelGGL((k_tri_p2p<true, false>), g, bl, 0, s2, Q, tgeo, tstr, spos, tpos, kind, po, fo);
+      else
+        hipLaunchKernelGGL((k_tri_p2p<false, true>), g, bl, 0, s2, Q, tgeo, tstr, spos, tpos, kind, po, fo);
+    };
+    tri_near(dIB + O.p2p0, O.n_p2p0, 0, pot_sn, f.iffld ? fld_sn : nullptr);
+    tri_near(dIB + O.p2p1, O.n_p2p1, 1, pot_tn, f.iffldtarg ? fld_tn : nullptr);
+  } else if (f.ifcharge || f.ifdipole) {
     launch_p2p<TN>(P, dIB + O.p2p0, O.n_p2p0, 0, spos, strn, str64, tpos, f.ifcharge, f.ifdipole,
                    f.iffld != 0, pot_sn, fld_sn, s2);
     launch_p2p<TN>(P, dIB + O.p2p1, O.n_p2p1, 1, spos, strn, str64, tpos, f.ifcharge, f.ifdipole,
@@ -737,11 +846,55 @@ int lap_direct(Context& ctx, const LapCall& c) {
   return 0;
 }
 
+// l3dtriadirect (src/lfmm3dtria.f:1385) on the device: caller's order, one
+// pass over all triangles per point.
+int tri_direct(Context& ctx, const LapCall& c) {
+  const int n = c.ns, nt = c.nt;
+  hipStream_t s = ctx.stream;
+  std::vector<int> id((size_t)std::max(n, 1));
+  for (size_t i = 0; i < id.size(); i++) id[i] = (int)i;
+  int* dperm = (int*)ctx.arena.get("plan_i", id.size() * sizeof(int));
+  FMM_HIP(hipMemcpyAsync(dperm, id.data(), id.size() * sizeof(int), hipMemcpyHostToDevice, s));
+  double* geo = (double*)ctx.arena.get("tri_geo", sizeof(double) * kTriGeo * (size_t)std::max(n, 1));
+  double* tstr = (double*)ctx.arena.get("tri_str", sizeof(double) * 4 * (size_t)std::max(n, 1));
+  const int ifc = c.ifcharge ? 1 : 0, ifd = c.ifdipole ? 1 : 0;
+  hipLaunchKernelGGL(k_tri_prep, dim3(blocks(n, 256)), dim3(256), 0, s, n, dperm, c.tri_dev, c.q_dev, c.ds_dev, ifc,
+                     ifd, 0, (const double*)nullptr, geo, tstr, (double*)nullptr, (double*)nullptr);
+  const size_t nmax = (size_t)std::max(std::max(n, nt), 1);
+  double* pot = (double*)ctx.arena.get("pot_s", sizeof(double) * 2 * nmax);
+  double* fld = (double*)ctx.arena.get("fld_s", sizeof(double) * 6 * nmax);
+  auto run = [&](int npts, const double* pts, int centroids, int wpot, double* po, int wfld, double* fo) {
+    if (npts <= 0 || !(wpot || wfld)) return;
+    if (!ifc && !ifd) {
+      hipLaunchKernelGGL((k_fill_zero<double>), dim3(blocks(2 * (size_t)npts, 256)), dim3(256), 0, s,
+                         2 * (size_t)npts, pot);
+      hipLaunchKernelGGL((k_fill_zero<double>), dim3(blocks(6 * (size_t)npts, 256)), dim3(256), 0, s,
+                         6 * (size_t)npts, fld);
+    } else if (ifc && ifd) {
+      hipLaunchKernelGGL((k_tri_direct<true, true>), dim3(blocks(npts, 256)), dim3(256), 0, s, n, geo, tstr, npts,
+                         pts, centroids, pot, fld);
+    } else if (ifc) {
+      hipLaunchKernelGGL((k_tri_direct<true, false>), dim3(blocks(npts, 256)), dim3(256), 0, s, n, geo, tstr, npts,
+                         pts, centroids, pot, fld);
+    } else {
+      hipLaunchKernelGGL((k_tri_direct<false, true>), dim3(blocks(npts, 256)), dim3(256), 0, s, n, geo, tstr, npts,
+                         pts, centroids, pot, fld);
+    }
+    if (wpot) FMM_HIP(hipMemcpyAsync(po, pot, sizeof(double) * 2 * npts, hipMemcpyDeviceToDevice, s));
+    if (wfld) FMM_HIP(hipMemcpyAsync(fo, fld, sizeof(double) * 6 * npts, hipMemcpyDeviceToDevice, s));
+  };
+  run(n, c.src_dev, 1, c.ifpot, c.pot_dev, c.iffld, c.fld_dev);
+  run(nt, c.trg_dev, 0, c.ifpottarg, c.pottarg_dev, c.iffldtarg, c.fldtarg_dev);
+  FMM_HIP(hipGetLastError());
+  FMM_HIP(hipStreamSynchronize(s));
+  return 0;
+}
+
 }  // namespace fmm3d
 
 namespace fmm3d {
 int lap_solve_mode(Context& ctx, const LapCall& c, int mode) {
-  mode = effective_precision(c.iprec, mode);
+  mode = c.tri_dev ? 0 : effective_precision(c.iprec, mode);
   last_precision() = mode;
   switch (mode) {
     case 0: return lap_solve<double, double>(ctx, c);

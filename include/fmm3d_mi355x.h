@@ -82,6 +82,41 @@ int fmm3d_lfmm3dparttarg_hess(int iprec, int nsource, const double *source,
                               int ifpottarg, double *pottarg, int iffldtarg, double *fldtarg,
                               int ifhesstarg, double *hesstarg);
 
+/* Replaces lfmm3dtriatarg (src/lfmm3dtria.f:121-141): the triangle FMM.
+ * Piecewise-constant single layer (charge) and dipole layer (dipstr) on
+ * nsource flat triangles, Green's function 1/r without 1/(4 pi),
+ * self-interactions included; pot/fld (= -gradient) at the points `source`
+ * (the triangles' centroids, one per triangle, evaluated on their own
+ * triangle in the plane) and at ntarget targets.  triaflat is (3,3,nsource)
+ * in Fortran order (x,y,z of vertex 1, 2, 3 per triangle); the dipole layer
+ * points along the normal of the vertex order, (v2-v1)x(v3-v1) -- the
+ * reference requires dipvec (and trianorm) to be that normal and so do
+ * these entries, which read neither.  Near field by the analytic
+ * quadratures (direct3dtritarglaps2 / lapd2, direct3dtrilaps2 / lapd2,
+ * src/triquadflatlib.f:147-818), far field through quadrature points of
+ * each triangle.  FP64; leaf size as the reference's (lfmm3dtria.f:286-301).
+ * Returns ier as fmm3d_lfmm3dparttarg. */
+int fmm3d_lfmm3dtriatarg(int iprec, int nsource, const double *triaflat, const double *trianorm,
+                         const double *source, int ifcharge, const double *charge,
+                         int ifdipole, const double *dipstr, const double *dipvec,
+                         int ifpot, double *pot, int iffld, double *fld,
+                         int ntarget, const double *target,
+                         int ifpottarg, double *pottarg, int iffldtarg, double *fldtarg);
+
+/* Replaces lfmm3dtriaself (src/lfmm3dtria.f:80): no targets. */
+int fmm3d_lfmm3dtriaself(int iprec, int nsource, const double *triaflat, const double *trianorm,
+                         const double *source, int ifcharge, const double *charge,
+                         int ifdipole, const double *dipstr, const double *dipvec,
+                         int ifpot, double *pot, int iffld, double *fld);
+
+/* Replaces l3dtriadirect (src/lfmm3dtria.f:1385): the O(N^2) triangle sum. */
+void fmm3d_l3dtriadirect(int nsource, const double *triaflat, const double *trianorm,
+                         const double *source, int ifcharge, const double *charge,
+                         int ifdipole, const double *dipstr, const double *dipvec,
+                         int ifpot, double *pot, int iffld, double *fld,
+                         int ntarget, const double *target,
+                         int ifpottarg, double *pottarg, int iffldtarg, double *fldtarg);
+
 /* Same as fmm3d_lfmm3dparttarg with an explicit precision mode (FMM3D_PREC_*).
  * Extension with no reference counterpart: the reference has FP64 only. */
 int fmm3d_lfmm3dparttarg_ex(int prec_mode, int iprec, int nsource, const double *source,

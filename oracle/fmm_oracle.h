@@ -149,6 +149,18 @@ int orc_hfmm3dparttarg(int iprec, zdouble zk, int ns, const double *src,
                        int nt, const double *trg, int ifpottarg, zdouble *pottarg,
                        int iffldtarg, zdouble *fldtarg);
 
+/* ---- Laplace layer potentials of flat triangles (fmm_oracle_tria.c) ---- */
+/* unit-density SLP pot, SLP fld[3], DLP pot, DLP fld[3] of one triangle
+ * (vertices xyz, 9 doubles) at x; self != 0 for the triangle's own centroid */
+void orc_tri_unit(const double *tri, const double *x, int self, double *out);
+void orc_l3dtriadirect(int ntri, const double *tri, const double *cent, int ifcharge,
+                       const zdouble *charge, int ifdipole, const zdouble *dipstr, int ifpot,
+                       zdouble *pot, int iffld, zdouble *fld, int ntarget, const double *target,
+                       int ifpottarg, zdouble *pottarg, int iffldtarg, zdouble *fldtarg);
+void orc_l3dtriadirect_sample(int ntri, const double *tri, int ifcharge, const zdouble *charge,
+                              int ifdipole, const zdouble *dipstr, int npts, const double *pts,
+                              const int *self_index, zdouble *pot, zdouble *fld);
+
 #ifdef __cplusplus
 }
 #endif

@@ -408,3 +408,108 @@ def orc_tree(source, target, nbox, maxboxes=200000, maxlist=20000000):
             out[k][b] = tuple(lists[o_: o_ + c].tolist())
             o_ += c
     return boxes[:nb], isrc, itrg[:nt], out, nlev.value
+
+
+# ---- Laplace layer potentials of flat triangles ----
+
+def orc_tri_unit(tri, x, self=False):
+    """The oracle's unit-density integrals of one triangle (3x3 vertices) at x:
+    (SLP pot, SLP fld[3], DLP pot, DLP fld[3])."""
+    o = oracle()
+    t = _c(tri, np.float64).reshape(9)
+    p = _c(x, np.float64).reshape(3)
+    out = np.zeros(8)
+    o.orc_tri_unit(_vp(t), _vp(p), ctypes.c_int(1 if self else 0), _vp(out))
+    return out[0], out[1:4].copy(), out[4], out[5:8].copy()
+
+
+def orc_tria_direct_sample(tri, points, self_index, charge=None, dipstr=None):
+    """The oracle's direct triangle sum at `points`; self_index[j] >= 0 names
+    the triangle whose centroid point j is."""
+    o = oracle()
+    t = _c(tri, np.float64).reshape(-1, 9)
+    pts = _c(points, np.float64).reshape(-1, 3)
+    n, npts = t.shape[0], pts.shape[0]
+    q = _c(charge, np.complex128) if charge is not None else np.zeros(1, np.complex128)
+    ds = _c(dipstr, np.complex128) if dipstr is not None else np.zeros(1, np.complex128)
+    si = _c(self_index, np.int32) if self_index is not None else None
+    pot = np.zeros(npts, np.complex128)
+    fld = np.zeros((npts, 3), np.complex128)
+    o.orc_l3dtriadirect_sample(ctypes.c_int(n), _vp(t), ctypes.c_int(charge is not None), _vp(q),
+                               ctypes.c_int(dipstr is not None), _vp(ds), ctypes.c_int(npts), _vp(pts),
+                               _vp(si), _vp(pot), _vp(fld))
+    return pot, fld
+
+
+def ref_tri_unit(tri, x, self=False):
+    """The reference's analytic triangle quadratures for one triangle at x
+    (direct3dtritarglaps2 / direct3dtritarglapd2, or at the triangle's own
+    centroid direct3dtrilaps2 / direct3dtrilapd2): unit-density (SLP pot,
+    SLP fld, DLP pot, DLP fld)."""
+    r = ref()
+    t = np.asfortranarray(_c(tri, np.float64).reshape(3, 3).T)   # triang(3,3): column j = vertex j
+    p = _c(x, np.float64).reshape(3)
+    e1, e2 = t[:, 1] - t[:, 0], t[:, 2] - t[:, 0]
+    nrm = np.cross(e1, e2)
+    nrm = nrm / np.linalg.norm(nrm)
+    one = np.array([1.0 + 0j])
+    pot = np.zeros(1, np.complex128)
+    fld = np.zeros(3, np.complex128)
+    out = []
+    if self:
+        r.direct3dtrilaps2_(_i(1), _i(1), _vp(p), _vp(one), _vp(t), _i(1), _vp(pot), _i(1), _vp(fld))
+    else:
+        r.direct3dtritarglaps2_(_i(1), _vp(p), _vp(one), _vp(t), _i(1), _vp(pot), _i(1), _vp(fld))
+    out += [pot[0].real, fld.real.copy()]
+    pot[:] = 0
+    fld[:] = 0
+    if self:
+        r.direct3dtrilapd2_(_i(1), _i(1), _vp(p), _vp(one), _vp(t), _vp(nrm), _i(1), _vp(pot), _i(1), _vp(fld))
+    else:
+        r.direct3dtritarglapd2_(_i(1), _vp(p), _vp(one), _vp(t), _vp(nrm), _i(1), _vp(pot), _i(1), _vp(fld))
+    out += [pot[0].real, fld.real.copy()]
+    return tuple(out)
+
+
+def _tri_args(tri, cent, charge, dipstr, target):
+    t = np.asfortranarray(np.transpose(_c(tri, np.float64).reshape(-1, 3, 3), (0, 2, 1)))  # (n, 3xyz, 3vert) C
+    t = np.ascontiguousarray(_c(tri, np.float64).reshape(-1, 9))     # vertex-major = Fortran triaflat(3,3,n)
+    n = t.shape[0]
+    v = t.reshape(n, 3, 3)
+    nrm = np.cross(v[:, 1] - v[:, 0], v[:, 2] - v[:, 0])
+    nrm = np.ascontiguousarray(nrm / np.linalg.norm(nrm, axis=1)[:, None])
+    c = _c(cent, np.float64)
+    q = _c(charge, np.complex128) if charge is not None else np.zeros(1, np.complex128)
+    ds = _c(dipstr, np.complex128) if dipstr is not None else np.zeros(1, np.complex128)
+    trg = _c(target, np.float64) if target is not None else np.zeros((1, 3))
+    nt = 0 if target is None else trg.shape[0]
+    return t, nrm, c, q, ds, trg, nt, n
+
+
+def ref_l3dtriadirect(tri, cent, charge=None, dipstr=None, target=None):
+    """The reference's l3dtriadirect (src/lfmm3dtria.f:1385)."""
+    r = ref()
+    t, nrm, c, q, ds, trg, nt, n = _tri_args(tri, cent, charge, dipstr, target)
+    pot = np.zeros(n, np.complex128)
+    fld = np.zeros((n, 3), np.complex128)
+    pt = np.zeros(max(nt, 1), np.complex128)
+    ft = np.zeros((max(nt, 1), 3), np.complex128)
+    r.l3dtriadirect_(_i(n), _vp(t), _vp(nrm), _vp(c), _i(charge is not None), _vp(q),
+                     _i(dipstr is not None), _vp(ds), _vp(nrm), _i(1), _vp(pot), _i(1), _vp(fld),
+                     _i(nt), _vp(trg), _i(1 if nt else 0), _vp(pt), _i(1 if nt else 0), _vp(ft))
+    return pot, fld, pt[:nt], ft[:nt]
+
+
+def ref_lfmm3dtriatarg(iprec, tri, cent, charge=None, dipstr=None, target=None):
+    """The reference's triangle FMM lfmm3dtriatarg (src/lfmm3dtria.f:121)."""
+    r = ref()
+    t, nrm, c, q, ds, trg, nt, n = _tri_args(tri, cent, charge, dipstr, target)
+    pot = np.zeros(n, np.complex128)
+    fld = np.zeros((n, 3), np.complex128)
+    pt = np.zeros(max(nt, 1), np.complex128)
+    ft = np.zeros((max(nt, 1), 3), np.complex128)
+    ier = ctypes.c_int(0)
+    r.lfmm3dtriatarg_(ctypes.byref(ier), _i(iprec), _i(n), _vp(t), _vp(nrm), _vp(c), _i(charge is not None),
+                      _vp(q), _i(dipstr is not None), _vp(ds), _vp(nrm), _i(1), _vp(pot), _i(1), _vp(fld),
+                      _i(nt), _vp(trg), _i(1 if nt else 0), _vp(pt), _i(1 if nt else 0), _vp(ft))
+    return ier.value, pot, fld, pt[:nt], ft[:nt]
