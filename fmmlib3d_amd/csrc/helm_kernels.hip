@@ -960,7 +960,7 @@ __global__ __launch_bounds__(NT) void k_hm2l_grp(DevPlan P, DevTables Tb, HelmDe
   int* t_box = t_key + kM2LChunk;
   int* t_code = t_box + kM2LChunk;
   const int tid = threadIdx.x, bs = blockDim.x;
-  const int b = P.m2l_boxes[blockIdx.x];
+  const int b = P.m2l_boxes[xcd_block()];
   const int p = P.p, P1 = p + 1, K2 = 2 * P.K;
   // degree of each packed index k < (p+4)^2, and the members' phases
   // e^{i am phi}, am = 0..p, of the group in hand
@@ -1288,7 +1288,7 @@ __global__ __launch_bounds__(256) void k_hp2p(DevPlan P, double kre, double kim,
                                               const double* __restrict__ tpos, int kind, double* __restrict__ pot,
                                               double* __restrict__ fld) {
   // one work item (64 points) per wave
-  const int w = __builtin_amdgcn_readfirstlane((int)(blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)));
+  const int w = __builtin_amdgcn_readfirstlane((int)(xcd_block() * (blockDim.x >> 6) + (threadIdx.x >> 6)));
   if (w >= P.n_p2p) return;
   const int b = P.p2p_items[3 * w], start = P.p2p_items[3 * w + 2];
   if (b < 0) return;   // padding item

@@ -250,6 +250,23 @@ __device__ void rotate_block(const DevTables& Tb, int rc, int pmax, const T* in,
   }
 }
 
+// The block index remapped so that each XCD runs contiguous chunks of the
+// grid: workgroups are handed to the 8 XCDs of an MI355X round-robin in
+// block order, each XCD with its own L2, so block b runs on XCD b % 8.  In
+// each group of 8 * kXcdChunk blocks, XCD x gets the kXcdChunk consecutive
+// blocks x * kXcdChunk ..; a last, partial group keeps its order.  Work
+// items in box (Morton) order then keep runs of neighbouring boxes -- which
+// read the same sources or multipoles -- on one L2, while chunks of every
+// part of the domain still go to every XCD (work per box varies in space).
+constexpr int kXcdChunk = 64;
+__device__ __forceinline__ int xcd_block() {
+  const int b = (int)blockIdx.x, gsz = 8 * kXcdChunk;
+  const int g = b / gsz;
+  if ((g + 1) * gsz > (int)gridDim.x) return b;
+  const int l = b - g * gsz;
+  return g * gsz + (l & 7) * kXcdChunk + (l >> 3);
+}
+
 // Near-field precision helpers: the pair arithmetic after r^2 runs in TN
 // (double or float); FP32 partial sums are folded into FP64 every kFold sources.
 constexpr int kFold = 32;

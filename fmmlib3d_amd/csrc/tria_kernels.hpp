@@ -178,7 +178,7 @@ __global__ __launch_bounds__(256) void k_tri_p2p(DevPlan P, const double* __rest
                                                  const double* __restrict__ tstr, const double* __restrict__ cpos,
                                                  const double* __restrict__ tpos, int kind, double* __restrict__ pot,
                                                  double* __restrict__ fld) {
-  const int w = __builtin_amdgcn_readfirstlane((int)(blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)));
+  const int w = __builtin_amdgcn_readfirstlane((int)(xcd_block() * (blockDim.x >> 6) + (threadIdx.x >> 6)));
   if (w >= P.n_p2p) return;
   const int b = P.p2p_items[3 * w], start = P.p2p_items[3 * w + 2];
   if (b < 0) return;  // padding item
