@@ -53,7 +53,7 @@ FLOPS_PER_PAIR = 65            # charge+dipole, pot+fld, rsqrt counted as 2
 # HBM bytes per k_p2p launch on this workload, from rocprofv3 FETCH_SIZE and
 # WRITE_SIZE runs calibrated against a 1 GiB device copy
 # (profiles/fmm_kernels_lap1M_helm500k.md): 2 x FETCH KiB + WRITE KiB.
-P2P_TRAFFIC_BYTES = {"fp64": (2 * 4238407 + 62846) * 1024, "fp32": (2 * 2568393 + 88269) * 1024}
+P2P_TRAFFIC_BYTES = {"fp64": (2 * 2674172 + 62877) * 1024, "fp32": (2 * 1212368 + 88665) * 1024}
 # VALU lane instructions per pair in k_p2p on this workload, from rocprofv3
 # SQ_INSTS_VALU (wave64 instructions x 64 / pairs; profiles/valu_mfma_counters_lap1M.md).
 # A wave64 VALU instruction occupies its SIMD for 4 cycles; the chip has

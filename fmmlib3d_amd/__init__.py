@@ -37,7 +37,7 @@ EXPORTED = (
     "fmm3d_hfmm3dparttarg_ex", "fmm3d_hfmm3dparttarg_dev",
     "fmm3d_server_create", "fmm3d_server_submit", "fmm3d_server_wait", "fmm3d_server_stats",
     "fmm3d_server_destroy",
-    "fmm3d_lfmm3dtriatarg", "fmm3d_lfmm3dtriaself", "fmm3d_l3dtriadirect",
+    "fmm3d_lfmm3dtriatarg", "fmm3d_lfmm3dtriaself", "fmm3d_lfmm3dtriampftarg", "fmm3d_l3dtriadirect",
 )
 
 _lib = None
@@ -108,6 +108,8 @@ def load():
         lib.fmm3d_lfmm3dtriatarg.restype = i
         lib.fmm3d_lfmm3dtriaself.argtypes = ttarg[:14]
         lib.fmm3d_lfmm3dtriaself.restype = i
+        lib.fmm3d_lfmm3dtriampftarg.argtypes = ttarg[:10] + ttarg[14:]
+        lib.fmm3d_lfmm3dtriampftarg.restype = i
         lib.fmm3d_l3dtriadirect.argtypes = ttarg[1:]
         lib.fmm3d_l3dtriadirect.restype = None
         _lib = lib
@@ -235,6 +237,21 @@ def lfmm3dtriaself(iprec, triaflat, charge=None, dipstr=None, source=None, ifpot
     """Triangle FMM without targets (lfmm3dtriaself): (ier, pot, fld)."""
     g = lfmm3dtriatarg(iprec, triaflat, charge, dipstr, None, source, ifpot, iffld, 0, 0)
     return g[0], g[1], g[2]
+
+
+def lfmm3dtriampftarg(iprec, triaflat, charge=None, dipstr=None, target=None, source=None):
+    """Triangle FMM with outputs at the targets only (lfmm3dtriampftarg):
+    (ier, pottarg, fldtarg)."""
+    lib = load()
+    tri, cen, nrm, q, ds, trg = _tri_prep(triaflat, charge, dipstr, target, source)
+    n = tri.shape[0]
+    nt = 0 if trg is None else trg.shape[0]
+    pt = np.zeros(max(nt, 1), np.complex128)
+    ft = np.zeros((max(nt, 1), 3), np.complex128)
+    ier = lib.fmm3d_lfmm3dtriampftarg(int(iprec), n, _ptr(tri), _ptr(nrm), _ptr(cen), int(q is not None), _ptr(q),
+                                      int(ds is not None), _ptr(ds), _ptr(nrm), nt, _ptr(trg), int(nt > 0), _ptr(pt),
+                                      int(nt > 0), _ptr(ft))
+    return ier, pt[:nt], ft[:nt]
 
 
 def l3dtriadirect(triaflat, charge=None, dipstr=None, target=None, source=None):

@@ -436,6 +436,15 @@ int fmm3d_lfmm3dtriaself(int iprec, int nsource, const double* triaflat, const d
                               dipvec, ifpot, pot, iffld, fld, 0, nullptr, 0, nullptr, 0, nullptr);
 }
 
+int fmm3d_lfmm3dtriampftarg(int iprec, int nsource, const double* triaflat, const double* trianorm,
+                            const double* source, int ifcharge, const double* charge, int ifdipole,
+                            const double* dipstr, const double* dipvec, int ntarget, const double* target,
+                            int ifpottarg, double* pottarg, int iffldtarg, double* fldtarg) {
+  return fmm3d_lfmm3dtriatarg(iprec, nsource, triaflat, trianorm, source, ifcharge, charge, ifdipole, dipstr,
+                              dipvec, 0, nullptr, 0, nullptr, ntarget, target, ifpottarg, pottarg, iffldtarg,
+                              fldtarg);
+}
+
 void fmm3d_l3dtriadirect(int nsource, const double* triaflat, const double* trianorm, const double* source,
                          int ifcharge, const double* charge, int ifdipole, const double* dipstr,
                          const double* dipvec, int ifpot, double* pot, int iffld, double* fld, int ntarget,

@@ -109,6 +109,14 @@ int fmm3d_lfmm3dtriaself(int iprec, int nsource, const double *triaflat, const d
                          int ifdipole, const double *dipstr, const double *dipvec,
                          int ifpot, double *pot, int iffld, double *fld);
 
+/* Replaces lfmm3dtriampftarg (src/lfmm3dtria.f:1566): the triangle FMM with
+ * outputs at the targets only. */
+int fmm3d_lfmm3dtriampftarg(int iprec, int nsource, const double *triaflat, const double *trianorm,
+                            const double *source, int ifcharge, const double *charge,
+                            int ifdipole, const double *dipstr, const double *dipvec,
+                            int ntarget, const double *target,
+                            int ifpottarg, double *pottarg, int iffldtarg, double *fldtarg);
+
 /* Replaces l3dtriadirect (src/lfmm3dtria.f:1385): the O(N^2) triangle sum. */
 void fmm3d_l3dtriadirect(int nsource, const double *triaflat, const double *trianorm,
                          const double *source, int ifcharge, const double *charge,

@@ -110,6 +110,8 @@ def test_gpu_tria_self_and_one_leaf():
     assert rel(pt, ptd) < 1e-12 and rel(ft, ftd) < 1e-12
     ier2, pot2, fld2 = F.lfmm3dtriaself(6, T, q, mu)
     assert ier2 == 0 and np.array_equal(pot2, pot) and np.array_equal(fld2, fld)
+    ier3, pt3, ft3 = F.lfmm3dtriampftarg(6, T, q, mu, trg)
+    assert ier3 == 0 and rel(pt3, ptd) < 1e-12 and rel(ft3, ftd) < 1e-12
 
 
 @pytest.mark.skipif(not ob.ref_available(), reason="reference library not built here")
