@@ -96,6 +96,41 @@ def test_gpu_tria_fmm_soup_adaptive(iprec, tsz):
     assert e[1] < 10 * bar and e[3] < 10 * bar
 
 
+def _plate(m):
+    """A flat square plate z = 0.3 in [0,1]^2 cut into 2 m^2 triangles
+    (every pair of triangles coplanar)."""
+    x = np.linspace(0, 1, m + 1)
+    tris = []
+    for i in range(m):
+        for j in range(m):
+            a, b, c, d = (x[i], x[j]), (x[i + 1], x[j]), (x[i + 1], x[j + 1]), (x[i], x[j + 1])
+            tris.append([a, b, c])
+            tris.append([a, c, d])
+    t = np.array(tris)
+    return np.concatenate([t, np.full(t.shape[:2] + (1,), 0.3)], axis=2)
+
+
+def test_gpu_tria_fmm_coplanar_plate():
+    """All triangles in one plane: every near pair is evaluated in the plane
+    (h = 0, principal values), and targets above, below and in the plane."""
+    rng = np.random.default_rng(78)
+    T = _plate(90)                                    # 16,200 triangles
+    n = T.shape[0]
+    q, mu = _densities(rng, n, 1, 1)
+    trg = rng.random((2000, 3))
+    trg[:500, 2] = 0.3                                 # in the plate's plane
+    ier, pot, fld, pt, ft = F.lfmm3dtriatarg(2, T, q, mu, trg)
+    assert ier == 0
+    idx = rng.choice(n, 300, replace=False)
+    it = rng.choice(2000, 300, replace=False)
+    po, fo = ob.orc_tria_direct_sample(T, T.mean(axis=1)[idx], idx.astype(np.int32), q, mu)
+    pto, fto = ob.orc_tria_direct_sample(T, trg[it], None, q, mu)
+    e = [rel(pot[idx], po), rel(fld[idx], fo), rel(pt[it], pto), rel(ft[it], fto)]
+    print("plate iprec 2: pot %.2e fld %.2e pottarg %.2e fldtarg %.2e" % tuple(e))
+    assert e[0] < IPREC_EPS[2] and e[2] < IPREC_EPS[2]
+    assert e[1] < 10 * IPREC_EPS[2] and e[3] < 10 * IPREC_EPS[2]
+
+
 def test_gpu_tria_self_and_one_leaf():
     """iprec 6 puts everything in one leaf: the FMM is the direct sum."""
     rng = np.random.default_rng(75)
