@@ -38,6 +38,7 @@ EXPORTED = (
     "fmm3d_server_create", "fmm3d_server_submit", "fmm3d_server_wait", "fmm3d_server_stats",
     "fmm3d_server_destroy",
     "fmm3d_lfmm3dtriatarg", "fmm3d_lfmm3dtriaself", "fmm3d_lfmm3dtriampftarg", "fmm3d_l3dtriadirect",
+    "fmm3d_hfmm3dtriatarg", "fmm3d_hfmm3dtriaself", "fmm3d_hfmm3dtriampftarg", "fmm3d_h3dtriadirect",
 )
 
 _lib = None
@@ -112,6 +113,15 @@ def load():
         lib.fmm3d_lfmm3dtriampftarg.restype = i
         lib.fmm3d_l3dtriadirect.argtypes = ttarg[1:]
         lib.fmm3d_l3dtriadirect.restype = None
+        htt = [i, vp] + ttarg[1:]
+        lib.fmm3d_hfmm3dtriatarg.argtypes = htt
+        lib.fmm3d_hfmm3dtriatarg.restype = i
+        lib.fmm3d_hfmm3dtriaself.argtypes = htt[:15]
+        lib.fmm3d_hfmm3dtriaself.restype = i
+        lib.fmm3d_hfmm3dtriampftarg.argtypes = htt[:11] + htt[15:]
+        lib.fmm3d_hfmm3dtriampftarg.restype = i
+        lib.fmm3d_h3dtriadirect.argtypes = htt
+        lib.fmm3d_h3dtriadirect.restype = None
         _lib = lib
     return _lib
 
@@ -266,6 +276,75 @@ def l3dtriadirect(triaflat, charge=None, dipstr=None, target=None, source=None):
     pt = np.zeros(max(nt, 1), np.complex128)
     ft = np.zeros((max(nt, 1), 3), np.complex128)
     lib.fmm3d_l3dtriadirect(n, _ptr(tri), _ptr(nrm), _ptr(cen), int(q is not None), _ptr(q), int(ds is not None),
+                            _ptr(ds), _ptr(nrm), 1, _ptr(pot), 1, _ptr(fld), nt, _ptr(trg), int(nt > 0), _ptr(pt),
+                            int(nt > 0), _ptr(ft))
+    return pot, fld, pt[:nt], ft[:nt]
+
+
+def hfmm3dtriatarg(iprec, zk, triaflat, charge=None, dipstr=None, target=None, source=None,
+                   ifpot=1, iffld=1, ifpottarg=1, iffldtarg=1, ifcharge=1, ifdipole=1):
+    """Helmholtz triangle FMM (hfmm3dtriatarg, src/hfmm3dtria.f:133), the
+    arguments as lfmm3dtriatarg plus the wavenumber zk; ifcharge / ifdipole
+    = 2 subtract that density's Laplace part (the difference kernel).
+    Returns (ier, pot, fld, pottarg, fldtarg)."""
+    lib = load()
+    tri, cen, nrm, q, ds, trg = _tri_prep(triaflat, charge, dipstr, target, source)
+    n = tri.shape[0]
+    nt = 0 if trg is None else trg.shape[0]
+    pot = np.zeros(n, np.complex128)
+    fld = np.zeros((n, 3), np.complex128)
+    pt = np.zeros(max(nt, 1), np.complex128)
+    ft = np.zeros((max(nt, 1), 3), np.complex128)
+    z = _zk(zk)
+    ier = lib.fmm3d_hfmm3dtriatarg(
+        int(iprec), _ptr(z), n, _ptr(tri), _ptr(nrm), _ptr(cen), int(ifcharge) if q is not None else 0, _ptr(q),
+        int(ifdipole) if ds is not None else 0, _ptr(ds), _ptr(nrm), int(bool(ifpot)), _ptr(pot),
+        int(bool(iffld)), _ptr(fld), nt, _ptr(trg), int(bool(ifpottarg) and nt > 0), _ptr(pt),
+        int(bool(iffldtarg) and nt > 0), _ptr(ft))
+    return ier, pot, fld, pt[:nt], ft[:nt]
+
+
+def hfmm3dtriaself(iprec, zk, triaflat, charge=None, dipstr=None, source=None, ifpot=1, iffld=1, ifcharge=1,
+                   ifdipole=1):
+    """Helmholtz triangle FMM without targets (hfmm3dtriaself): (ier, pot, fld)."""
+    g = hfmm3dtriatarg(iprec, zk, triaflat, charge, dipstr, None, source, ifpot, iffld, 0, 0, ifcharge, ifdipole)
+    return g[0], g[1], g[2]
+
+
+def hfmm3dtriampftarg(iprec, zk, triaflat, charge=None, dipstr=None, target=None, source=None, ifcharge=1,
+                      ifdipole=1):
+    """Helmholtz triangle FMM with outputs at the targets only
+    (hfmm3dtriampftarg): (ier, pottarg, fldtarg)."""
+    lib = load()
+    tri, cen, nrm, q, ds, trg = _tri_prep(triaflat, charge, dipstr, target, source)
+    n = tri.shape[0]
+    nt = 0 if trg is None else trg.shape[0]
+    pt = np.zeros(max(nt, 1), np.complex128)
+    ft = np.zeros((max(nt, 1), 3), np.complex128)
+    z = _zk(zk)
+    ier = lib.fmm3d_hfmm3dtriampftarg(int(iprec), _ptr(z), n, _ptr(tri), _ptr(nrm), _ptr(cen),
+                                      int(ifcharge) if q is not None else 0, _ptr(q),
+                                      int(ifdipole) if ds is not None else 0, _ptr(ds), _ptr(nrm), nt, _ptr(trg),
+                                      int(nt > 0), _ptr(pt), int(nt > 0), _ptr(ft))
+    return ier, pt[:nt], ft[:nt]
+
+
+def h3dtriadirect(nqtri, zk, triaflat, charge=None, dipstr=None, target=None, source=None, ifcharge=1,
+                  ifdipole=1):
+    """Direct Helmholtz triangle sum on the device (h3dtriadirect,
+    src/hfmm3dtria.f:1456) with the symmetric rule of degree nqtri:
+    (pot, fld, pottarg, fldtarg)."""
+    lib = load()
+    tri, cen, nrm, q, ds, trg = _tri_prep(triaflat, charge, dipstr, target, source)
+    n = tri.shape[0]
+    nt = 0 if trg is None else trg.shape[0]
+    pot = np.zeros(n, np.complex128)
+    fld = np.zeros((n, 3), np.complex128)
+    pt = np.zeros(max(nt, 1), np.complex128)
+    ft = np.zeros((max(nt, 1), 3), np.complex128)
+    z = _zk(zk)
+    lib.fmm3d_h3dtriadirect(int(nqtri), _ptr(z), n, _ptr(tri), _ptr(nrm), _ptr(cen),
+                            int(ifcharge) if q is not None else 0, _ptr(q), int(ifdipole) if ds is not None else 0,
                             _ptr(ds), _ptr(nrm), 1, _ptr(pot), 1, _ptr(fld), nt, _ptr(trg), int(nt > 0), _ptr(pt),
                             int(nt > 0), _ptr(ft))
     return pot, fld, pt[:nt], ft[:nt]

@@ -24,7 +24,7 @@ CXX = os.path.join(ROCM, "llvm", "bin", "clang++")
 FC = os.path.join(ROCM, "bin", "amdflang")
 
 HIP_SOURCES = ["runtime.hip", "tree_gpu.hip", "fmm3d_lap.hip", "fmm3d_helm.hip", "fmm3d_api.hip", "server.hip"]
-CPP_SOURCES = ["octree.cpp", "tables.cpp", "helm_tables.cpp"]
+CPP_SOURCES = ["octree.cpp", "tables.cpp", "helm_tables.cpp", "tria_rules.cpp"]
 
 # Packed FP32 instructions are disabled for all device code.
 HIP_FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-fopenmp",

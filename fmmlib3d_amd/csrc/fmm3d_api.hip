@@ -11,6 +11,7 @@
 #include "../../include/fmm3d_mi355x.h"
 #include "fmm3d_internal.hpp"
 #include "tree_gpu.hpp"
+#include "tria_rules.hpp"
 #include "octree.hpp"
 
 using namespace fmm3d;
@@ -396,6 +397,130 @@ void direct_host(const double* zk, int nsource, const double* source, int ifchar
   });
 }
 
+// out -= sub, n doubles
+__global__ void k_subtract(size_t n, double* __restrict__ out, const double* __restrict__ sub) {
+  const size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+  if (i < n) out[i] -= sub[i];
+}
+
+// The Helmholtz triangle FMM (hfmm3dtriatarg, src/hfmm3dtria.f:133-647):
+// density flags 0, 1, or 2 (that density's Laplace part subtracted: the
+// near field with the difference kernel, the far field as the Helmholtz far
+// field minus the Laplace one, as the reference's second, far-only Laplace
+// pass does, src/hfmm3dtria.f:552-640).
+int htri_host(const double* zk, int iprec, int ns, const double* tria, const double* source, int ifcharge,
+              const double* charge, int ifdipole, const double* dipstr, int ifpot, double* pot, int iffld,
+              double* fld, int nt, const double* target, int ifpottarg, double* pottarg, int iffldtarg,
+              double* fldtarg) {
+  if (!zk) return 128;
+  if (ifcharge < 0 || ifcharge > 2 || ifdipole < 0 || ifdipole > 2) return 128;
+  if (int e = check_args(zk, 0, iprec, ns, nt)) return e;
+  if (!tria) return 128;
+  const int ifc = ifcharge ? 1 : 0, ifd = ifdipole ? 1 : 0;
+  // the dipole orientation is the triangle normal: no dipvec is read
+  if (int e = check_ptrs(ns, source, ifc, charge, 0, nullptr, nullptr, ifpot, pot, iffld, fld, nt, target,
+                         ifpottarg, pottarg, iffldtarg, fldtarg))
+    return e;
+  if (ifd && !dipstr) return 128;
+  Context& ctx = default_context();
+  std::lock_guard<std::mutex> lock(ctx.mu);
+  return guarded([&] {
+    LapCall c;
+    c.iprec = iprec; c.ns = ns; c.nt = nt;
+    c.zk_re = zk[0]; c.zk_im = zk[1];
+    c.ifcharge = ifc; c.ifdipole = ifd;
+    c.ifpot = ifpot == 1; c.iffld = iffld == 1;
+    c.ifpottarg = nt > 0 && ifpottarg == 1; c.iffldtarg = nt > 0 && iffldtarg == 1;
+    c.tri_lapc = ifcharge != 2; c.tri_lapd = ifdipole != 2;
+    c.src_host = source;
+    c.trg_host = nt > 0 ? target : nullptr;
+    c.src_dev = upload(ctx, "in_src", source, 3 * (size_t)ns);
+    c.q_dev = ifc ? upload(ctx, "in_q", charge, 2 * (size_t)ns) : nullptr;
+    c.ds_dev = ifd ? upload(ctx, "in_ds", dipstr, 2 * (size_t)ns) : nullptr;
+    c.trg_dev = nt > 0 ? upload(ctx, "in_trg", target, 3 * (size_t)nt) : nullptr;
+    c.tri_dev = upload(ctx, "in_tri", tria, 9 * (size_t)ns);
+    if (c.ifpot) c.pot_dev = (double*)ctx.arena.get("out_pot", 2 * sizeof(double) * ns);
+    if (c.iffld) c.fld_dev = (double*)ctx.arena.get("out_fld", 6 * sizeof(double) * ns);
+    if (c.ifpottarg) c.pottarg_dev = (double*)ctx.arena.get("out_pt", 2 * sizeof(double) * nt);
+    if (c.iffldtarg) c.fldtarg_dev = (double*)ctx.arena.get("out_ft", 6 * sizeof(double) * nt);
+    int ier = helm_solve_mode(ctx, c, 0);
+    if (ier) return ier;
+    if (ifcharge == 2 || ifdipole == 2) {
+      LapCall cl = c;
+      cl.zk_re = cl.zk_im = 0.0;
+      cl.ifcharge = ifcharge == 2; cl.ifdipole = ifdipole == 2;
+      cl.far_only = true;
+      if (c.ifpot) cl.pot_dev = (double*)ctx.arena.get("out_pot_lap", 2 * sizeof(double) * ns);
+      if (c.iffld) cl.fld_dev = (double*)ctx.arena.get("out_fld_lap", 6 * sizeof(double) * ns);
+      if (c.ifpottarg) cl.pottarg_dev = (double*)ctx.arena.get("out_pt_lap", 2 * sizeof(double) * nt);
+      if (c.iffldtarg) cl.fldtarg_dev = (double*)ctx.arena.get("out_ft_lap", 6 * sizeof(double) * nt);
+      ier = lap_solve_mode(ctx, cl, 0);
+      if (ier) return ier;
+      auto sub = [&](double* out, const double* l, size_t n) {
+        if (out && n)
+          hipLaunchKernelGGL(k_subtract, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, ctx.stream, n, out, l);
+      };
+      sub(c.pot_dev, cl.pot_dev, 2 * (size_t)ns);
+      sub(c.fld_dev, cl.fld_dev, 6 * (size_t)ns);
+      sub(c.pottarg_dev, cl.pottarg_dev, 2 * (size_t)nt);
+      sub(c.fldtarg_dev, cl.fldtarg_dev, 6 * (size_t)nt);
+      FMM_HIP(hipGetLastError());
+    }
+    if (c.ifpot) download(ctx, pot, c.pot_dev, 2 * (size_t)ns);
+    if (c.iffld) download(ctx, fld, c.fld_dev, 6 * (size_t)ns);
+    if (c.ifpottarg) download(ctx, pottarg, c.pottarg_dev, 2 * (size_t)nt);
+    if (c.iffldtarg) download(ctx, fldtarg, c.fldtarg_dev, 6 * (size_t)nt);
+    FMM_HIP(hipStreamSynchronize(ctx.stream));
+    return 0;
+  });
+}
+
+// h3dtriadirect (src/hfmm3dtria.f:1456): no error channel, as the
+// reference; a call with a missing array or an unsupported rule degree
+// computes nothing.
+void htri_direct_host(int nqtri, const double* zk, int ns, const double* tria, const double* source, int ifcharge,
+                      const double* charge, int ifdipole, const double* dipstr, int ifpot, double* pot, int iffld,
+                      double* fld, int nt, const double* target, int ifpottarg, double* pottarg, int iffldtarg,
+                      double* fldtarg) {
+  if (!zk || ns < 0 || nt < 0 || tri_sym_rule(nqtri).empty()) return;
+  if (ifcharge < 0 || ifcharge > 2 || ifdipole < 0 || ifdipole > 2) return;
+  if (ns > 0 && !tria) return;
+  const int ifc = ifcharge ? 1 : 0, ifd = ifdipole ? 1 : 0;
+  if (check_ptrs(ns, source, ifc, charge, 0, nullptr, nullptr, ifpot, pot, iffld, fld, nt, target, ifpottarg,
+                 pottarg, iffldtarg, fldtarg))
+    return;
+  if (ifd && ns > 0 && !dipstr) return;
+  Context& ctx = default_context();
+  std::lock_guard<std::mutex> lock(ctx.mu);
+  (void)guarded([&] {
+    LapCall c;
+    c.ns = ns; c.nt = nt;
+    c.zk_re = zk[0]; c.zk_im = zk[1];
+    c.ifcharge = ifc; c.ifdipole = ifd;
+    c.ifpot = ifpot == 1; c.iffld = iffld == 1;
+    c.ifpottarg = ifpottarg == 1; c.iffldtarg = iffldtarg == 1;
+    c.tri_lapc = ifcharge != 2; c.tri_lapd = ifdipole != 2;
+    c.tri_nqtri = nqtri;
+    c.src_dev = upload(ctx, "in_src", source, 3 * (size_t)ns);
+    c.q_dev = ifc ? upload(ctx, "in_q", charge, 2 * (size_t)ns) : nullptr;
+    c.ds_dev = ifd ? upload(ctx, "in_ds", dipstr, 2 * (size_t)ns) : nullptr;
+    c.trg_dev = nt > 0 ? upload(ctx, "in_trg", target, 3 * (size_t)nt) : nullptr;
+    c.tri_dev = upload(ctx, "in_tri", tria, 9 * (size_t)ns);
+    if (c.ifpot) c.pot_dev = (double*)ctx.arena.get("out_pot", 2 * sizeof(double) * std::max(ns, 1));
+    if (c.iffld) c.fld_dev = (double*)ctx.arena.get("out_fld", 6 * sizeof(double) * std::max(ns, 1));
+    if (c.ifpottarg) c.pottarg_dev = (double*)ctx.arena.get("out_pt", 2 * sizeof(double) * std::max(nt, 1));
+    if (c.iffldtarg) c.fldtarg_dev = (double*)ctx.arena.get("out_ft", 6 * sizeof(double) * std::max(nt, 1));
+    int ier = htri_direct(ctx, c);
+    if (ier) return ier;
+    if (c.ifpot) download(ctx, pot, c.pot_dev, 2 * (size_t)ns);
+    if (c.iffld) download(ctx, fld, c.fld_dev, 6 * (size_t)ns);
+    if (c.ifpottarg) download(ctx, pottarg, c.pottarg_dev, 2 * (size_t)nt);
+    if (c.iffldtarg) download(ctx, fldtarg, c.fldtarg_dev, 6 * (size_t)nt);
+    FMM_HIP(hipStreamSynchronize(ctx.stream));
+    return 0;
+  });
+}
+
 }  // namespace
 
 extern "C" {
@@ -452,6 +577,44 @@ void fmm3d_l3dtriadirect(int nsource, const double* triaflat, const double* tria
   (void)trianorm;
   direct_host(nullptr, nsource, source, ifcharge, charge, ifdipole, dipstr, dipvec, ifpot, pot, iffld, fld,
               ntarget, target, ifpottarg, pottarg, iffldtarg, fldtarg, triaflat, true);
+}
+
+int fmm3d_hfmm3dtriatarg(int iprec, const double* zk, int nsource, const double* triaflat,
+                         const double* trianorm, const double* source, int ifcharge, const double* charge,
+                         int ifdipole, const double* dipstr, const double* dipvec, int ifpot, double* pot, int iffld,
+                         double* fld, int ntarget, const double* target, int ifpottarg, double* pottarg,
+                         int iffldtarg, double* fldtarg) {
+  (void)trianorm;  // the dipole layer is along the normal of the vertex order, as in the reference
+  (void)dipvec;
+  return htri_host(zk, iprec, nsource, triaflat, source, ifcharge, charge, ifdipole, dipstr, ifpot, pot, iffld, fld,
+                   ntarget, target, ifpottarg, pottarg, iffldtarg, fldtarg);
+}
+
+int fmm3d_hfmm3dtriaself(int iprec, const double* zk, int nsource, const double* triaflat,
+                         const double* trianorm, const double* source, int ifcharge, const double* charge,
+                         int ifdipole, const double* dipstr, const double* dipvec, int ifpot, double* pot, int iffld,
+                         double* fld) {
+  return fmm3d_hfmm3dtriatarg(iprec, zk, nsource, triaflat, trianorm, source, ifcharge, charge, ifdipole, dipstr,
+                              dipvec, ifpot, pot, iffld, fld, 0, nullptr, 0, nullptr, 0, nullptr);
+}
+
+int fmm3d_hfmm3dtriampftarg(int iprec, const double* zk, int nsource, const double* triaflat,
+                            const double* trianorm, const double* source, int ifcharge, const double* charge,
+                            int ifdipole, const double* dipstr, const double* dipvec, int ntarget,
+                            const double* target, int ifpottarg, double* pottarg, int iffldtarg, double* fldtarg) {
+  return fmm3d_hfmm3dtriatarg(iprec, zk, nsource, triaflat, trianorm, source, ifcharge, charge, ifdipole, dipstr,
+                              dipvec, 0, nullptr, 0, nullptr, ntarget, target, ifpottarg, pottarg, iffldtarg,
+                              fldtarg);
+}
+
+void fmm3d_h3dtriadirect(int nqtri, const double* zk, int nsource, const double* triaflat, const double* trianorm,
+                         const double* source, int ifcharge, const double* charge, int ifdipole, const double* dipstr,
+                         const double* dipvec, int ifpot, double* pot, int iffld, double* fld, int ntarget,
+                         const double* target, int ifpottarg, double* pottarg, int iffldtarg, double* fldtarg) {
+  (void)trianorm;
+  (void)dipvec;
+  htri_direct_host(nqtri, zk, nsource, triaflat, source, ifcharge, charge, ifdipole, dipstr, ifpot, pot, iffld, fld,
+                   ntarget, target, ifpottarg, pottarg, iffldtarg, fldtarg);
 }
 
 void fmm3d_last_timings(double* ms, int n) {

@@ -16,6 +16,8 @@
 #include "fmm3d_internal.hpp"
 #include "helm_kernels.hip"
 #include "helm_tables.hpp"
+#include "htria_kernels.hpp"
+#include "tria_rules.hpp"
 #include "tree_gpu.hpp"
 #include "octree.hpp"
 #include "plan.hpp"
@@ -177,8 +179,17 @@ int helm_solve(Context& ctx, const LapCall& c) {
   double eps;
   int nbox;
   if (!iprec_params(c.iprec, c.ns, c.nt, eps, nbox)) return 32;
-  const int ns = c.ns, nt = c.nt;
+  const bool tri = c.tri_dev != nullptr;
   const cplx zk(c.zk_re, c.zk_im);
+  // triangle FMM: the near-field rule (of iprec unless the caller names one)
+  const std::vector<double>* sym = nullptr;
+  if (tri) {
+    if (!std::is_same<TN, double>::value) return 128;
+    nbox = tri_nbox(c.iprec, c.ns, c.nt);
+    sym = &tri_sym_rule(c.tri_nqtri > 0 ? c.tri_nqtri : htri_nqtri(c.iprec));
+    if (sym->empty() || sym->size() / 3 > (size_t)kSymRuleMax) return 128;
+  }
+  const int ns = c.ns, nt = c.nt;
   const Flags f{c.ifcharge ? 1 : 0, c.ifdipole ? 1 : 0, c.ifpot ? 1 : 0, c.iffld ? 1 : 0,
                 nt > 0 && c.ifpottarg ? 1 : 0, nt > 0 && c.iffldtarg ? 1 : 0};
   thread_local Octree tree;  // storage reused across calls
@@ -281,8 +292,10 @@ int helm_solve(Context& ctx, const LapCall& c) {
   const int* tperm = dsperm ? dtperm : dI + O.tperm;
   P.src_perm = sperm;
   P.trg_perm = tperm;
+  // (triangles: the point strengths are not used -- k_tri_prep forms the
+  // triangle densities and quadrature points -- and no dipole vectors are read)
   hipLaunchKernelGGL(k_gather_sources, dim3(blocks(ns, 256)), dim3(256), 0, s, ns, sperm, c.src_dev,
-                     c.q_dev, c.ds_dev, c.dv_dev, f.ifcharge, f.ifdipole, spos, str64, str32);
+                     c.q_dev, c.ds_dev, c.dv_dev, tri ? 0 : f.ifcharge, tri ? 0 : f.ifdipole, spos, str64, str32);
   if (nt > 0)
     hipLaunchKernelGGL(k_gather_targets, dim3(blocks(nt, 256)), dim3(256), 0, s, nt, tperm, c.trg_dev, tpos);
   auto zero = [&](double* a, size_t n) {
@@ -300,21 +313,52 @@ int helm_solve(Context& ctx, const LapCall& c) {
     zero(pot_tn, 2 * (size_t)nt);
     zero(fld_tn, 6 * (size_t)nt);
   }
+  // triangle FMM: frames and densities in sorted order, and the quadrature
+  // points that stand for the triangles in P2M / P2L (their box ranges
+  // counted in points)
+  const double* fpos = spos;
+  const double* fstr = str64;
+  double *tgeo = nullptr, *tstr = nullptr;
+  int* tri_rng = nullptr;
+  if (tri) {
+    const std::vector<double> rule = tri_rule(c.iprec, true);
+    const int nq = (int)rule.size() / 3;
+    tgeo = (double*)ctx.arena.get("tri_geo", sizeof(double) * kTriGeo * (size_t)ns);
+    tstr = (double*)ctx.arena.get("tri_str", sizeof(double) * 4 * (size_t)ns);
+    double* npos = (double*)ctx.arena.get("tri_npos", sizeof(double) * 4 * (size_t)ns * nq);
+    double* nstr = (double*)ctx.arena.get("tri_nstr", sizeof(double) * 8 * (size_t)ns * nq);
+    double* hr = (double*)ctx.arena.get_host("tri_rule_h", sizeof(double) * rule.size());
+    std::memcpy(hr, rule.data(), sizeof(double) * rule.size());
+    double* dr = (double*)ctx.arena.get("tri_rule", sizeof(double) * rule.size());
+    FMM_HIP(hipMemcpyAsync(dr, hr, sizeof(double) * rule.size(), hipMemcpyHostToDevice, s));
+    hipLaunchKernelGGL(k_tri_prep, dim3(blocks(ns, 256)), dim3(256), 0, s, ns, sperm, c.tri_dev, c.q_dev,
+                       c.ds_dev, f.ifcharge, f.ifdipole, nq, dr, tgeo, tstr, npos, nstr);
+    tri_rng = (int*)ctx.arena.get("tri_ranges", sizeof(int) * 2 * nb);
+    hipLaunchKernelGGL(k_scale_ranges, dim3(blocks(nb, 256)), dim3(256), 0, s, (int)nb, P.isrc, P.nsrc, nq, tri_rng,
+                       tri_rng + nb);
+    fpos = npos;
+    fstr = nstr;
+  }
   FMM_HIP(hipEventRecord(ctx.aux_ev[0], s));  // inputs gathered, sums zeroed
   tm.mark(1);
   const double* fac = Tb.fac;
   auto form = [&](auto irreg, int nbx, double* out) {
     constexpr bool IR = decltype(irreg)::value;
     if (!nbx) return;
+    DevPlan PF = P;  // the current plan (P2L needs list 4), source ranges in points for triangles
+    if (tri) {
+      PF.isrc = tri_rng;
+      PF.nsrc = tri_rng + nb;
+    }
     if (!form2) {
-      hipLaunchKernelGGL((k_hform<IR>), dim3(nbx), dim3(256), lds_form, s, P, Hd, fac, spos, str64,
+      hipLaunchKernelGGL((k_hform<IR>), dim3(nbx), dim3(256), lds_form, s, PF, Hd, fac, fpos, fstr,
                          f.ifcharge, f.ifdipole, ts, out);
     } else if (f.ifcharge && f.ifdipole) {
-      hipLaunchKernelGGL((k_hform2<IR, true, true>), dim3(nbx), dim3(256), lds_form2, s, P, Hd, fac, spos, str64, region2, out);
+      hipLaunchKernelGGL((k_hform2<IR, true, true>), dim3(nbx), dim3(256), lds_form2, s, PF, Hd, fac, fpos, fstr, region2, out);
     } else if (f.ifcharge) {
-      hipLaunchKernelGGL((k_hform2<IR, true, false>), dim3(nbx), dim3(256), lds_form2, s, P, Hd, fac, spos, str64, region2, out);
+      hipLaunchKernelGGL((k_hform2<IR, true, false>), dim3(nbx), dim3(256), lds_form2, s, PF, Hd, fac, fpos, fstr, region2, out);
     } else if (f.ifdipole) {
-      hipLaunchKernelGGL((k_hform2<IR, false, true>), dim3(nbx), dim3(256), lds_form2, s, P, Hd, fac, spos, str64, region2, out);
+      hipLaunchKernelGGL((k_hform2<IR, false, true>), dim3(nbx), dim3(256), lds_form2, s, PF, Hd, fac, fpos, fstr, region2, out);
     }
   };
   // STEP 1: P2M (hfmm3dpart.f:681-757)
@@ -340,7 +384,30 @@ int helm_solve(Context& ctx, const LapCall& c) {
   tn.start(s2, 2);
   const int* dIB = upload_ints("plan_ib", FB.I, s2);
   devplan_near(P, O, dIB);
-  if (f.ifcharge || f.ifdipole) {
+  if (tri && c.far_only) {
+    // the far field alone
+  } else if (tri && (f.ifcharge || f.ifdipole)) {
+    double* hr = (double*)ctx.arena.get_host("htri_rule_h", sizeof(double) * 3 * kSymRuleMax);
+    std::memcpy(hr, sym->data(), sizeof(double) * sym->size());
+    double* dr = (double*)ctx.arena.get("htri_rule", sizeof(double) * 3 * kSymRuleMax);
+    FMM_HIP(hipMemcpyAsync(dr, hr, sizeof(double) * sym->size(), hipMemcpyHostToDevice, s2));
+    const HtriParams hp{zk.real(), zk.imag(), std::abs(zk), c.tri_lapc, c.tri_lapd, (int)sym->size() / 3, dr};
+    auto tri_near = [&](const int* items, int n, int kind, double* po, double* fo) {
+      if (n == 0) return;
+      DevPlan Q = P;
+      Q.p2p_items = items;
+      Q.n_p2p = n;
+      dim3 g(item_groups(n)), bl(kP2PBlock * kItemWaves);
+      if (f.ifcharge && f.ifdipole)
+        hipLaunchKernelGGL((k_htri_p2p<true, true>), g, bl, 0, s2, Q, tgeo, tstr, spos, tpos, kind, hp, po, fo);
+      else if (f.ifcharge)
+        hipLaunchKernelGGL((k_htri_p2p<true, false>), g, bl, 0, s2, Q, tgeo, tstr, spos, tpos, kind, hp, po, fo);
+      else
+        hipLaunchKernelGGL((k_htri_p2p<false, true>), g, bl, 0, s2, Q, tgeo, tstr, spos, tpos, kind, hp, po, fo);
+    };
+    tri_near(dIB + O.p2p0, O.n_p2p0, 0, pot_sn, f.iffld ? fld_sn : nullptr);
+    tri_near(dIB + O.p2p1, O.n_p2p1, 1, pot_tn, f.iffldtarg ? fld_tn : nullptr);
+  } else if (f.ifcharge || f.ifdipole) {
     launch_hp2p<TN>(P, dIB + O.p2p0, O.n_p2p0, 0, Hd.kre, Hd.kim, spos, strn, str64, tpos, f.ifcharge, f.ifdipole,
                     f.iffld != 0, pot_sn, fld_sn, s2);
     launch_hp2p<TN>(P, dIB + O.p2p1, O.n_p2p1, 1, Hd.kre, Hd.kim, spos, strn, str64, tpos, f.ifcharge, f.ifdipole,
@@ -519,7 +586,7 @@ int helm_solve(Context& ctx, const LapCall& c) {
 }
 
 int helm_solve_mode(Context& ctx, const LapCall& c, int mode) {
-  mode = effective_precision(c.iprec, mode);
+  mode = c.tri_dev ? 0 : effective_precision(c.iprec, mode);
   last_precision() = mode;
   switch (mode) {
     case 0: return helm_solve<double>(ctx, c);
@@ -573,6 +640,60 @@ int helm_direct(Context& ctx, const LapCall& c) {
     run(nt, tpos, 0);
     copy_out(nt, c.ifpottarg, c.pottarg_dev, c.iffldtarg, c.fldtarg_dev);
   }
+  FMM_HIP(hipGetLastError());
+  FMM_HIP(hipStreamSynchronize(s));
+  return 0;
+}
+
+// h3dtriadirect (src/hfmm3dtria.f:1456) on the device: caller's order, one
+// pass over all triangles per point, the near-field rule of degree
+// c.tri_nqtri.
+int htri_direct(Context& ctx, const LapCall& c) {
+  const int n = c.ns, nt = c.nt;
+  const std::vector<double>& sr = tri_sym_rule(c.tri_nqtri);
+  if (sr.empty() || sr.size() / 3 > (size_t)kSymRuleMax) return 128;
+  hipStream_t s = ctx.stream;
+  double* hr = (double*)ctx.arena.get_host("htri_rule_h", sizeof(double) * 3 * kSymRuleMax);
+  std::memcpy(hr, sr.data(), sizeof(double) * sr.size());
+  double* dr = (double*)ctx.arena.get("htri_rule", sizeof(double) * 3 * kSymRuleMax);
+  FMM_HIP(hipMemcpyAsync(dr, hr, sizeof(double) * sr.size(), hipMemcpyHostToDevice, s));
+  const cplx zk(c.zk_re, c.zk_im);
+  const HtriParams hp{zk.real(), zk.imag(), std::abs(zk), c.tri_lapc, c.tri_lapd, (int)sr.size() / 3, dr};
+  std::vector<int> id((size_t)std::max(n, 1));
+  for (size_t i = 0; i < id.size(); i++) id[i] = (int)i;
+  int* dperm = (int*)ctx.arena.get("plan_i", id.size() * sizeof(int));
+  FMM_HIP(hipMemcpyAsync(dperm, id.data(), id.size() * sizeof(int), hipMemcpyHostToDevice, s));
+  double* geo = (double*)ctx.arena.get("tri_geo", sizeof(double) * kTriGeo * (size_t)std::max(n, 1));
+  double* tstr = (double*)ctx.arena.get("tri_str", sizeof(double) * 4 * (size_t)std::max(n, 1));
+  const int ifc = c.ifcharge ? 1 : 0, ifd = c.ifdipole ? 1 : 0;
+  if (n > 0)
+    hipLaunchKernelGGL(k_tri_prep, dim3(blocks(n, 256)), dim3(256), 0, s, n, dperm, c.tri_dev, c.q_dev, c.ds_dev,
+                       ifc, ifd, 0, (const double*)nullptr, geo, tstr, (double*)nullptr, (double*)nullptr);
+  const size_t nmax = (size_t)std::max(std::max(n, nt), 1);
+  double* pot = (double*)ctx.arena.get("pot_s", sizeof(double) * 2 * nmax);
+  double* fld = (double*)ctx.arena.get("fld_s", sizeof(double) * 6 * nmax);
+  auto run = [&](int npts, const double* pts, int centroids, int wpot, double* po, int wfld, double* fo) {
+    if (npts <= 0 || !(wpot || wfld)) return;
+    if (n == 0 || (!ifc && !ifd)) {
+      hipLaunchKernelGGL((k_fill_zero<double>), dim3(blocks(2 * (size_t)npts, 256)), dim3(256), 0, s,
+                         2 * (size_t)npts, pot);
+      hipLaunchKernelGGL((k_fill_zero<double>), dim3(blocks(6 * (size_t)npts, 256)), dim3(256), 0, s,
+                         6 * (size_t)npts, fld);
+    } else if (ifc && ifd) {
+      hipLaunchKernelGGL((k_htri_direct<true, true>), dim3(blocks(npts, 256)), dim3(256), 0, s, n, geo, tstr, npts,
+                         pts, centroids, hp, pot, fld);
+    } else if (ifc) {
+      hipLaunchKernelGGL((k_htri_direct<true, false>), dim3(blocks(npts, 256)), dim3(256), 0, s, n, geo, tstr, npts,
+                         pts, centroids, hp, pot, fld);
+    } else {
+      hipLaunchKernelGGL((k_htri_direct<false, true>), dim3(blocks(npts, 256)), dim3(256), 0, s, n, geo, tstr, npts,
+                         pts, centroids, hp, pot, fld);
+    }
+    if (wpot) FMM_HIP(hipMemcpyAsync(po, pot, sizeof(double) * 2 * npts, hipMemcpyDeviceToDevice, s));
+    if (wfld) FMM_HIP(hipMemcpyAsync(fo, fld, sizeof(double) * 6 * npts, hipMemcpyDeviceToDevice, s));
+  };
+  run(n, c.src_dev, 1, c.ifpot, c.pot_dev, c.iffld, c.fld_dev);
+  run(nt, c.trg_dev, 0, c.ifpottarg, c.pottarg_dev, c.iffldtarg, c.fldtarg_dev);
   FMM_HIP(hipGetLastError());
   FMM_HIP(hipStreamSynchronize(s));
   return 0;

@@ -36,6 +36,13 @@ struct LapCall {
   // the caller's order); src_dev then holds the evaluation points on them
   // (centroids), q_dev / ds_dev the single- and dipole-layer densities
   const double* tri_dev = nullptr;
+  // Helmholtz triangle FMM: whether the single- / dipole-layer densities
+  // take the Laplace part of the kernel (0: the difference kernel
+  // e^{ikr}/r - 1/r, a density flag of 2 in the reference), and the degree
+  // of the near-field rule (0: the one of iprec)
+  int tri_lapc = 1, tri_lapd = 1, tri_nqtri = 0;
+  // triangle FMMs: leave out the near field (the far field alone)
+  bool far_only = false;
 };
 
 // The precision mode a call runs in: the requested FMM3D_PREC_* mode while
@@ -65,5 +72,7 @@ int tri_direct(Context& ctx, const LapCall& c);
 // Helmholtz (hfmm3dparttarg / h3dpartdirect); mode 0 = FP64, 1 = FP32 near field
 int helm_solve_mode(Context& ctx, const LapCall& c, int mode);
 int helm_direct(Context& ctx, const LapCall& c);
+// h3dtriadirect on the device (c.tri_dev set; c.tri_nqtri the rule's degree)
+int htri_direct(Context& ctx, const LapCall& c);
 
 }  // namespace fmm3d

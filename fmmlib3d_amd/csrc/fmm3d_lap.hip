@@ -23,6 +23,7 @@
 #include "m2l_mfma.hpp"
 #include "hess_kernels.hpp"
 #include "tria_kernels.hpp"
+#include "tria_rules.hpp"
 #include "octree.hpp"
 #include "plan.hpp"
 #include "runtime.hpp"
@@ -214,60 +215,6 @@ const double* hess_factors(Context& ctx, int p) {
   return d;
 }
 
-// Leaf size of the triangle FMM (src/lfmm3dtria.f:286-301): 8/3, 15/3,
-// 30/3, 60/3, 120/3, 240/3, 480/3, 700/3 in integer arithmetic for iprec
-// -2..5, every triangle and target in one leaf at iprec 6.
-int tri_nbox(int iprec, int ns, int nt) {
-  static const int nb[] = {2, 5, 10, 20, 40, 80, 160, 233};
-  return iprec == 6 ? ns + nt : nb[iprec + 2];
-}
-
-// Gauss-Legendre nodes and weights on [0, 1], q points (Newton on P_q)
-void gauss01(int q, std::vector<double>& x, std::vector<double>& w) {
-  x.assign(q, 0.0);
-  w.assign(q, 0.0);
-  const double pi = 3.14159265358979323846;
-  for (int i = 0; i < q; i++) {
-    double t = std::cos(pi * (i + 0.75) / (q + 0.5)), dp = 1.0;
-    for (int it = 0; it < 100; it++) {
-      double p0 = 1.0, p1 = t;
-      for (int k = 2; k <= q; k++) {
-        const double p2 = ((2.0 * k - 1.0) * t * p1 - (k - 1.0) * p0) / k;
-        p0 = p1;
-        p1 = p2;
-      }
-      const double pq = q == 1 ? t : p1, pm = q == 1 ? 1.0 : p0;
-      dp = q * (t * pq - pm) / (t * t - 1.0);
-      const double dt = pq / dp;
-      t -= dt;
-      if (std::fabs(dt) < 1e-16) break;
-    }
-    x[i] = 0.5 * (1.0 - t);
-    w[i] = 1.0 / ((1.0 - t * t) * dp * dp);  // 2/((1-t^2) P'^2) on [-1,1], halved
-  }
-}
-
-// Quadrature on the reference triangle (xi, eta >= 0, xi + eta <= 1; weights
-// summing to its area 1/2) used to turn a triangle into weighted points for
-// P2M / P2L: the square [0,1]^2 collapsed onto it (xi = u(1-v), eta = u v,
-// Jacobian u) with q-point Gauss rules, exact for polynomials of degree
-// 2q - 2.  q by iprec: 2 (iprec <= -1), 3 (iprec 0), 4 (iprec 1, 2), 5
-// (iprec >= 3): degrees 2, 4, 6, 8, the reference's symmetric rules being of
-// order 2, 4 and 6 (lfmm3dtria.f:680-701).
-std::vector<double> tri_rule(int iprec) {
-  const int q = iprec <= -1 ? 2 : iprec == 0 ? 3 : iprec <= 2 ? 4 : 5;
-  std::vector<double> x, w, r;
-  gauss01(q, x, w);
-  for (int i = 0; i < q; i++)
-    for (int j = 0; j < q; j++) {
-      const double u = x[i], v = x[j];
-      r.push_back(u * (1.0 - v));
-      r.push_back(u * v);
-      r.push_back(w[i] * w[j] * u);
-    }
-  return r;
-}
-
 }  // namespace
 
 template <typename T, typename TN>
@@ -383,7 +330,7 @@ int lap_solve(Context& ctx, const LapCall& c) {
   double *tgeo = nullptr, *tstr = nullptr;
   int* tri_rng = nullptr;
   std::vector<double> rule;
-  if (tri) rule = tri_rule(c.iprec);
+  if (tri) rule = tri_rule(c.iprec, false);
   const int nq = (int)rule.size() / 3;
   // graph mode: every launch and copy from here to the scatter, on both
   // streams (the near-field stream joins through the events), is captured
@@ -404,8 +351,10 @@ int lap_solve(Context& ctx, const LapCall& c) {
     FMM_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeRelaxed));
     capture.on = true;
   }
+  // (triangles: the point strengths are not used -- k_tri_prep forms the
+  // triangle densities and quadrature points -- and no dipole vectors are read)
   hipLaunchKernelGGL(k_gather_sources, dim3(blocks(ns, 256)), dim3(256), 0, s, ns, sperm, c.src_dev,
-                     c.q_dev, c.ds_dev, c.dv_dev, f.ifcharge, f.ifdipole, spos, str64, str32);
+                     c.q_dev, c.ds_dev, c.dv_dev, tri ? 0 : f.ifcharge, tri ? 0 : f.ifdipole, spos, str64, str32);
   if (nt > 0)
     hipLaunchKernelGGL(k_gather_targets, dim3(blocks(nt, 256)), dim3(256), 0, s, nt, tperm, c.trg_dev, tpos);
   auto zero = [&](auto* a, size_t n) {
@@ -507,7 +456,9 @@ int lap_solve(Context& ctx, const LapCall& c) {
   const int* dIB = upload_ints("plan_ib", FB.I, s2);  // part B1: offsets in O from build_plan_near
   devplan_near(P, O, dIB);
   const int* dIB_near = dIB;
-  if (tri && (f.ifcharge || f.ifdipole)) {
+  if (tri && c.far_only) {
+    // the far field alone
+  } else if (tri && (f.ifcharge || f.ifdipole)) {
     auto tri_near = [&](const int* items, int n, int kind, double* po, double* fo) {
       if (n == 0) return;
       DevPlan Q = P;

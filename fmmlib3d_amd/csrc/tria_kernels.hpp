@@ -35,7 +35,7 @@ constexpr int kTriGeo = 12;   // doubles per triangle frame
 
 // frames, strengths (q, mu: 4 doubles) and quadrature points (4 doubles of
 // position, 8 of strengths each) of the triangles in sorted order
-__global__ __launch_bounds__(256) void k_tri_prep(int n, const int* __restrict__ perm, const double* __restrict__ tri,
+static __global__ __launch_bounds__(256) void k_tri_prep(int n, const int* __restrict__ perm, const double* __restrict__ tri,
                                                   const double* __restrict__ q, const double* __restrict__ mu, int ifc,
                                                   int ifd, int nq, const double* __restrict__ rule,
                                                   double* __restrict__ geo, double* __restrict__ tstr,
@@ -79,7 +79,7 @@ __global__ __launch_bounds__(256) void k_tri_prep(int n, const int* __restrict__
 }
 
 // each box's source range counted in quadrature points (nq per triangle)
-__global__ void k_scale_ranges(int nb, const int* __restrict__ isrc, const int* __restrict__ nsrc, int nq,
+static __global__ void k_scale_ranges(int nb, const int* __restrict__ isrc, const int* __restrict__ nsrc, int nq,
                                int* __restrict__ isrc_q, int* __restrict__ nsrc_q) {
   const int b = blockIdx.x * blockDim.x + threadIdx.x;
   if (b >= nb) return;
@@ -96,17 +96,32 @@ __device__ __forceinline__ double tri_inv_rrs(double s, double R, double R02) {
   return s >= 0.0 ? 1.0 / (R * (R + s)) : (R - s) / (R * R02);
 }
 
-// Unit-density single- and dipole-layer pot and fld of one triangle at
-// (x, y, z); self: the point is the triangle's own centroid.  Adds
-// cq * (single layer) + cm * (dipole layer) into acc[8] (pot, fld; complex).
-template <bool CHG, bool DIP>
-__device__ __forceinline__ void tri_pair(const double* __restrict__ g, const double* __restrict__ ts, double x,
-                                         double y, double z, bool self, double* acc) {
+// The point (x, y, z) in the frame of triangle g: in-plane coordinates
+// (px, py), height h along the normal (0 at the triangle's own centroid),
+// and the frame's second axis e2 = n x e1.
+struct TriPoint {
+  double px, py, h, e2x, e2y, e2z;
+};
+
+__device__ __forceinline__ TriPoint tri_point(const double* __restrict__ g, double x, double y, double z, bool self) {
   const double rx = x - g[0], ry = y - g[1], rz = z - g[2];
   const double e1x = g[3], e1y = g[4], e1z = g[5], nx = g[6], ny = g[7], nz = g[8];
-  const double e2x = ny * e1z - nz * e1y, e2y = nz * e1x - nx * e1z, e2z = nx * e1y - ny * e1x;
-  const double px = rx * e1x + ry * e1y + rz * e1z, py = rx * e2x + ry * e2y + rz * e2z;
-  const double h = self ? 0.0 : rx * nx + ry * ny + rz * nz;
+  TriPoint t;
+  t.e2x = ny * e1z - nz * e1y;
+  t.e2y = nz * e1x - nx * e1z;
+  t.e2z = nx * e1y - ny * e1x;
+  t.px = rx * e1x + ry * e1y + rz * e1z;
+  t.py = rx * t.e2x + ry * t.e2y + rz * t.e2z;
+  t.h = self ? 0.0 : rx * nx + ry * ny + rz * nz;
+  return t;
+}
+
+// Unit-density Laplace integrals of triangle g at the point t: u[0] single-
+// layer pot, u[1..3] its fld, u[4] dipole-layer pot, u[5..7] its fld, the
+// fields in global coordinates (CHG / DIP: which of the two are formed).
+template <bool CHG, bool DIP>
+__device__ __forceinline__ void tri_unit(const double* __restrict__ g, const TriPoint& t, double* u) {
+  const double px = t.px, py = t.py, h = t.h;
   const double ah = fabs(h), h2 = h * h;
   const double vx[3] = {0.0, g[9], g[10]}, vy[3] = {0.0, 0.0, g[11]};
   double R[3];
@@ -149,24 +164,41 @@ __device__ __forceinline__ void tri_pair(const double* __restrict__ g, const dou
     }
   }
   const double sg = h > 0.0 ? 1.0 : (h < 0.0 ? -1.0 : 0.0);
+  const double e1x = g[3], e1y = g[4], e1z = g[5], nx = g[6], ny = g[7], nz = g[8];
   if (CHG) {
-    spot = fma(-ah, beta, spot);
     const double sfz = sg * beta;
-    const double gx = sfx * e1x + sfy * e2x + sfz * nx, gy = sfx * e1y + sfy * e2y + sfz * ny,
-                 gz = sfx * e1z + sfy * e2z + sfz * nz;
-    acc[0] = fma(ts[0], spot, acc[0]); acc[1] = fma(ts[1], spot, acc[1]);
-    acc[2] = fma(ts[0], gx, acc[2]); acc[3] = fma(ts[1], gx, acc[3]);
-    acc[4] = fma(ts[0], gy, acc[4]); acc[5] = fma(ts[1], gy, acc[5]);
-    acc[6] = fma(ts[0], gz, acc[6]); acc[7] = fma(ts[1], gz, acc[7]);
+    u[0] = fma(-ah, beta, spot);
+    u[1] = sfx * e1x + sfy * t.e2x + sfz * nx;
+    u[2] = sfx * e1y + sfy * t.e2y + sfz * ny;
+    u[3] = sfx * e1z + sfy * t.e2z + sfz * nz;
   }
   if (DIP) {
-    const double dpot = sg * beta;
-    const double gx = dfx * e1x + dfy * e2x + dfz * nx, gy = dfx * e1y + dfy * e2y + dfz * ny,
-                 gz = dfx * e1z + dfy * e2z + dfz * nz;
-    acc[0] = fma(ts[2], dpot, acc[0]); acc[1] = fma(ts[3], dpot, acc[1]);
-    acc[2] = fma(ts[2], gx, acc[2]); acc[3] = fma(ts[3], gx, acc[3]);
-    acc[4] = fma(ts[2], gy, acc[4]); acc[5] = fma(ts[3], gy, acc[5]);
-    acc[6] = fma(ts[2], gz, acc[6]); acc[7] = fma(ts[3], gz, acc[7]);
+    u[4] = sg * beta;
+    u[5] = dfx * e1x + dfy * t.e2x + dfz * nx;
+    u[6] = dfx * e1y + dfy * t.e2y + dfz * ny;
+    u[7] = dfx * e1z + dfy * t.e2z + dfz * nz;
+  }
+}
+
+// Unit-density single- and dipole-layer pot and fld of one triangle at
+// (x, y, z); self: the point is the triangle's own centroid.  Adds
+// cq * (single layer) + cm * (dipole layer) into acc[8] (pot, fld; complex).
+template <bool CHG, bool DIP>
+__device__ __forceinline__ void tri_pair(const double* __restrict__ g, const double* __restrict__ ts, double x,
+                                         double y, double z, bool self, double* acc) {
+  double u[8];
+  tri_unit<CHG, DIP>(g, tri_point(g, x, y, z, self), u);
+  if (CHG) {
+    acc[0] = fma(ts[0], u[0], acc[0]); acc[1] = fma(ts[1], u[0], acc[1]);
+    acc[2] = fma(ts[0], u[1], acc[2]); acc[3] = fma(ts[1], u[1], acc[3]);
+    acc[4] = fma(ts[0], u[2], acc[4]); acc[5] = fma(ts[1], u[2], acc[5]);
+    acc[6] = fma(ts[0], u[3], acc[6]); acc[7] = fma(ts[1], u[3], acc[7]);
+  }
+  if (DIP) {
+    acc[0] = fma(ts[2], u[4], acc[0]); acc[1] = fma(ts[3], u[4], acc[1]);
+    acc[2] = fma(ts[2], u[5], acc[2]); acc[3] = fma(ts[3], u[5], acc[3]);
+    acc[4] = fma(ts[2], u[6], acc[4]); acc[5] = fma(ts[3], u[6], acc[5]);
+    acc[6] = fma(ts[2], u[7], acc[6]); acc[7] = fma(ts[3], u[7], acc[7]);
   }
 }
 

@@ -94,7 +94,7 @@ int fmm3d_lfmm3dparttarg_hess(int iprec, int nsource, const double *source,
  * these entries, which read neither.  Near field by the analytic
  * quadratures (direct3dtritarglaps2 / lapd2, direct3dtrilaps2 / lapd2,
  * src/triquadflatlib.f:147-818), far field through quadrature points of
- * each triangle.  FP64; leaf size as the reference's (lfmm3dtria.f:286-301).
+ * each triangle.  FP64; leaf size as the reference's (lfmm3dtria.f:297-304).
  * Returns ier as fmm3d_lfmm3dparttarg. */
 int fmm3d_lfmm3dtriatarg(int iprec, int nsource, const double *triaflat, const double *trianorm,
                          const double *source, int ifcharge, const double *charge,
@@ -120,6 +120,49 @@ int fmm3d_lfmm3dtriampftarg(int iprec, int nsource, const double *triaflat, cons
 /* Replaces l3dtriadirect (src/lfmm3dtria.f:1385): the O(N^2) triangle sum. */
 void fmm3d_l3dtriadirect(int nsource, const double *triaflat, const double *trianorm,
                          const double *source, int ifcharge, const double *charge,
+                         int ifdipole, const double *dipstr, const double *dipvec,
+                         int ifpot, double *pot, int iffld, double *fld,
+                         int ntarget, const double *target,
+                         int ifpottarg, double *pottarg, int iffldtarg, double *fldtarg);
+
+/* Replaces hfmm3dtriatarg (src/hfmm3dtria.f:133-144): the Helmholtz
+ * triangle FMM, Green's function e^{ikr}/r without 1/(4 pi), zk = (re, im),
+ * arguments as fmm3d_lfmm3dtriatarg.  ifcharge / ifdipole = 2 subtract that
+ * density's Laplace part (the difference kernel e^{ikr}/r - 1/r), as the
+ * reference's flag 2.  Near field: the Laplace analytic quadratures plus a
+ * D3-symmetric rule of degree 2, 4 or 6 by iprec on (e^{ikr}-1)/r
+ * (direct3dtritarghelms3 / helmd3, direct3dtrihelms2 / helmd2,
+ * src/triquadflatlib.f:819-1358; hfmm3dtria.f:730-751); far field through
+ * quadrature points of each triangle.  FP64.  Returns ier as
+ * fmm3d_hfmm3dparttarg (128 also for a density flag outside 0..2). */
+int fmm3d_hfmm3dtriatarg(int iprec, const double *zk, int nsource, const double *triaflat,
+                         const double *trianorm, const double *source, int ifcharge, const double *charge,
+                         int ifdipole, const double *dipstr, const double *dipvec,
+                         int ifpot, double *pot, int iffld, double *fld,
+                         int ntarget, const double *target,
+                         int ifpottarg, double *pottarg, int iffldtarg, double *fldtarg);
+
+/* Replaces hfmm3dtriaself (src/hfmm3dtria.f:92): no targets. */
+int fmm3d_hfmm3dtriaself(int iprec, const double *zk, int nsource, const double *triaflat,
+                         const double *trianorm, const double *source, int ifcharge, const double *charge,
+                         int ifdipole, const double *dipstr, const double *dipvec,
+                         int ifpot, double *pot, int iffld, double *fld);
+
+/* Replaces hfmm3dtriampftarg (src/hfmm3dtria.f:1668): outputs at the
+ * targets only. */
+int fmm3d_hfmm3dtriampftarg(int iprec, const double *zk, int nsource, const double *triaflat,
+                            const double *trianorm, const double *source, int ifcharge, const double *charge,
+                            int ifdipole, const double *dipstr, const double *dipvec,
+                            int ntarget, const double *target,
+                            int ifpottarg, double *pottarg, int iffldtarg, double *fldtarg);
+
+/* Replaces h3dtriadirect (src/hfmm3dtria.f:1456): the O(N^2) Helmholtz
+ * triangle sum with the symmetric rule of degree nqtri (1..6; the
+ * reference's triasymq rules, degree 3 being its degree-4 rule).  Like the
+ * reference it has no error channel: an unsupported nqtri, a density flag
+ * outside 0..2 or a missing array computes nothing. */
+void fmm3d_h3dtriadirect(int nqtri, const double *zk, int nsource, const double *triaflat,
+                         const double *trianorm, const double *source, int ifcharge, const double *charge,
                          int ifdipole, const double *dipstr, const double *dipvec,
                          int ifpot, double *pot, int iffld, double *fld,
                          int ntarget, const double *target,

@@ -161,6 +161,17 @@ void orc_l3dtriadirect_sample(int ntri, const double *tri, int ifcharge, const z
                               int ifdipole, const zdouble *dipstr, int npts, const double *pts,
                               const int *self_index, zdouble *pot, zdouble *fld);
 
+/* ---- Helmholtz layer potentials of flat triangles (fmm_oracle_htria.c) ---- */
+/* D3-symmetric rule of degree 1..6 on the reference triangle (weights sum
+ * to 1/2); returns the node count (at most 12), 0 for other degrees */
+int  orc_tri_symrule(int degree, double *xi, double *eta, double *w);
+/* unit-density SLP pot, fld[3], DLP pot, fld[3] (complex) of one triangle at x */
+void orc_htri_unit(const double *tri, const double *x, int self, zdouble zk, int nq, const double *rxi,
+                   const double *reta, const double *rw, int lap_c, int lap_d, zdouble *out);
+void orc_h3dtriadirect_sample(int ntri, const double *tri, zdouble zk, int nqtri, int ifcharge,
+                              const zdouble *charge, int ifdipole, const zdouble *dipstr, int npts,
+                              const double *pts, const int *self_index, zdouble *pot, zdouble *fld);
+
 #ifdef __cplusplus
 }
 #endif

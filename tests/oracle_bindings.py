@@ -513,3 +513,127 @@ def ref_lfmm3dtriatarg(iprec, tri, cent, charge=None, dipstr=None, target=None):
                       _vp(q), _i(dipstr is not None), _vp(ds), _vp(nrm), _i(1), _vp(pot), _i(1), _vp(fld),
                       _i(nt), _vp(trg), _i(1 if nt else 0), _vp(pt), _i(1 if nt else 0), _vp(ft))
     return ier.value, pot, fld, pt[:nt], ft[:nt]
+
+
+# ---- Helmholtz layer potentials of flat triangles ----
+
+def orc_tri_symrule(degree):
+    """The oracle's D3-symmetric rule of the given degree on the reference
+    triangle: (xi, eta, w), weights summing to 1/2."""
+    o = oracle()
+    xi, eta, w = np.zeros(16), np.zeros(16), np.zeros(16)
+    n = o.orc_tri_symrule(ctypes.c_int(degree), _vp(xi), _vp(eta), _vp(w))
+    return xi[:n].copy(), eta[:n].copy(), w[:n].copy()
+
+
+def orc_htri_unit(tri, x, zk, degree, self=False, lap_c=True, lap_d=True):
+    """The oracle's unit-density Helmholtz integrals of one triangle at x:
+    (SLP pot, SLP fld[3], DLP pot, DLP fld[3]), complex."""
+    o = oracle()
+    xi, eta, w = orc_tri_symrule(degree)
+    t = _c(tri, np.float64).reshape(9)
+    p = _c(x, np.float64).reshape(3)
+    out = np.zeros(8, np.complex128)
+    o.orc_htri_unit(_vp(t), _vp(p), ctypes.c_int(1 if self else 0), ctypes.c_double(complex(zk).real),
+                    ctypes.c_double(complex(zk).imag), ctypes.c_int(len(w)), _vp(xi), _vp(eta), _vp(w),
+                    ctypes.c_int(1 if lap_c else 0), ctypes.c_int(1 if lap_d else 0), _vp(out))
+    return out[0], out[1:4].copy(), out[4], out[5:8].copy()
+
+
+def orc_htria_direct_sample(tri, zk, nqtri, points, self_index, charge=None, dipstr=None, ifcharge=1,
+                            ifdipole=1):
+    """The oracle's direct Helmholtz triangle sum (h3dtriadirect) at
+    `points`; flag 2 subtracts the Laplace part of that density."""
+    o = oracle()
+    t = _c(tri, np.float64).reshape(-1, 9)
+    pts = _c(points, np.float64).reshape(-1, 3)
+    n, npts = t.shape[0], pts.shape[0]
+    q = _c(charge, np.complex128) if charge is not None else np.zeros(1, np.complex128)
+    ds = _c(dipstr, np.complex128) if dipstr is not None else np.zeros(1, np.complex128)
+    si = _c(self_index, np.int32) if self_index is not None else None
+    pot = np.zeros(npts, np.complex128)
+    fld = np.zeros((npts, 3), np.complex128)
+    z = np.array([complex(zk)], np.complex128)
+    o.orc_h3dtriadirect_sample(ctypes.c_int(n), _vp(t), ctypes.c_double(z[0].real), ctypes.c_double(z[0].imag),
+                               ctypes.c_int(nqtri), ctypes.c_int(ifcharge if charge is not None else 0), _vp(q),
+                               ctypes.c_int(ifdipole if dipstr is not None else 0), _vp(ds), ctypes.c_int(npts),
+                               _vp(pts), _vp(si), _vp(pot), _vp(fld))
+    return pot, fld
+
+
+def ref_triasymq(degree):
+    """The reference's triasymq rule on the triangle (0,0), (1,0), (0,1)."""
+    r = ref()
+    v1, v2, v3 = np.array([0.0, 0.0]), np.array([1.0, 0.0]), np.array([0.0, 1.0])
+    rn = np.zeros((500, 2))
+    w = np.zeros(500)
+    nn = ctypes.c_int(0)
+    r.triasymq_(_i(degree), _vp(v1), _vp(v2), _vp(v3), _vp(rn), _vp(w), ctypes.byref(nn))
+    return rn[:nn.value, 0].copy(), rn[:nn.value, 1].copy(), w[:nn.value].copy()
+
+
+def ref_htri_unit(tri, x, zk, nqtri, self=False, ifl_c=0, ifl_d=0):
+    """The reference's Helmholtz triangle quadratures for one triangle at x
+    (direct3dtritarghelms3 / direct3dtritarghelmd3, or at the triangle's own
+    centroid direct3dtrihelms2 / direct3dtrihelmd2), unit densities; ifl 1
+    for the difference kernel."""
+    r = ref()
+    t = np.asfortranarray(_c(tri, np.float64).reshape(3, 3).T)
+    p = _c(x, np.float64).reshape(3)
+    e1, e2 = t[:, 1] - t[:, 0], t[:, 2] - t[:, 0]
+    nrm = np.cross(e1, e2)
+    nrm = nrm / np.linalg.norm(nrm)
+    z = np.array([complex(zk)], np.complex128)
+    one = np.array([1.0 + 0j])
+    pot = np.zeros(1, np.complex128)
+    fld = np.zeros(3, np.complex128)
+    if self:
+        r.direct3dtrihelms2_(_i(ifl_c), _i(1), _i(1), _vp(z), _i(nqtri), _vp(p), _vp(one), _vp(t), _vp(pot), _vp(fld))
+    else:
+        r.direct3dtritarghelms3_(_i(ifl_c), _i(1), _vp(p), _vp(z), _i(nqtri), _vp(one), _vp(t), _i(1), _vp(pot),
+                                 _i(1), _vp(fld))
+    out = [pot[0], fld.copy()]
+    pot[:] = 0
+    fld[:] = 0
+    if self:
+        r.direct3dtrihelmd2_(_i(ifl_d), _i(1), _i(1), _vp(z), _i(nqtri), _vp(p), _vp(one), _vp(t), _vp(nrm),
+                             _vp(pot), _vp(fld))
+    else:
+        r.direct3dtritarghelmd3_(_i(ifl_d), _i(1), _vp(p), _vp(z), _i(nqtri), _vp(one), _vp(t), _vp(nrm), _i(1),
+                                 _vp(pot), _i(1), _vp(fld))
+    out += [pot[0], fld.copy()]
+    return tuple(out)
+
+
+def ref_h3dtriadirect(nqtri, zk, tri, cent, charge=None, dipstr=None, target=None, ifcharge=1, ifdipole=1):
+    """The reference's h3dtriadirect (src/hfmm3dtria.f:1456)."""
+    r = ref()
+    t, nrm, c, q, ds, trg, nt, n = _tri_args(tri, cent, charge, dipstr, target)
+    z = np.array([complex(zk)], np.complex128)
+    pot = np.zeros(n, np.complex128)
+    fld = np.zeros((n, 3), np.complex128)
+    pt = np.zeros(max(nt, 1), np.complex128)
+    ft = np.zeros((max(nt, 1), 3), np.complex128)
+    r.h3dtriadirect_(_i(nqtri), _vp(z), _i(n), _vp(t), _vp(nrm), _vp(c),
+                     _i(ifcharge if charge is not None else 0), _vp(q),
+                     _i(ifdipole if dipstr is not None else 0), _vp(ds), _vp(nrm), _i(1), _vp(pot), _i(1),
+                     _vp(fld), _i(nt), _vp(trg), _i(1 if nt else 0), _vp(pt), _i(1 if nt else 0), _vp(ft))
+    return pot, fld, pt[:nt], ft[:nt]
+
+
+def ref_hfmm3dtriatarg(iprec, zk, tri, cent, charge=None, dipstr=None, target=None, ifcharge=1, ifdipole=1):
+    """The reference's Helmholtz triangle FMM hfmm3dtriatarg
+    (src/hfmm3dtria.f:133)."""
+    r = ref()
+    t, nrm, c, q, ds, trg, nt, n = _tri_args(tri, cent, charge, dipstr, target)
+    z = np.array([complex(zk)], np.complex128)
+    pot = np.zeros(n, np.complex128)
+    fld = np.zeros((n, 3), np.complex128)
+    pt = np.zeros(max(nt, 1), np.complex128)
+    ft = np.zeros((max(nt, 1), 3), np.complex128)
+    ier = ctypes.c_int(0)
+    r.hfmm3dtriatarg_(ctypes.byref(ier), _i(iprec), _vp(z), _i(n), _vp(t), _vp(nrm), _vp(c),
+                      _i(ifcharge if charge is not None else 0), _vp(q),
+                      _i(ifdipole if dipstr is not None else 0), _vp(ds), _vp(nrm), _i(1), _vp(pot), _i(1),
+                      _vp(fld), _i(nt), _vp(trg), _i(1 if nt else 0), _vp(pt), _i(1 if nt else 0), _vp(ft))
+    return ier.value, pot, fld, pt[:nt], ft[:nt]
