@@ -346,12 +346,13 @@ def dropin_leg(F, prec, src, q, ds, dv, trg, steps=3):
             "ms_per_call": ms, "targets_per_s": N / (ms * 1e-3), "calls": steps}
 
 
-def triangle_leg(F, ob, level=7, ntarget=100_000, iprec=2, steps=3):
-    """Triangle FMM (lfmm3dtriatarg): a triangulated unit sphere (icosahedron
-    refined `level` times: 327,680 triangles at 7), single and dipole layer
-    with N(0,1) complex densities, pot+fld at the centroids and at targets in
-    the shell 0.9 < r < 1.1, iprec 2, FP64, host arrays in and out; checked
-    against the oracle's analytic direct sum at 100 centroids and 100 targets."""
+def triangle_leg(F, ob, level=7, ntarget=100_000, iprec=2, steps=3, zk=None):
+    """Triangle FMM (lfmm3dtriatarg, or hfmm3dtriatarg with wavenumber zk):
+    a triangulated unit sphere (icosahedron refined `level` times: 327,680
+    triangles at 7), single and dipole layer with N(0,1) complex densities,
+    pot+fld at the centroids and at targets in the shell 0.9 < r < 1.1,
+    iprec 2, FP64, host arrays in and out; checked against the oracle's
+    direct sum at 100 centroids and 100 targets."""
     import numpy as np
     from fmm_inputs import icosphere, rel
     rng = np.random.default_rng(77)
@@ -361,21 +362,32 @@ def triangle_leg(F, ob, level=7, ntarget=100_000, iprec=2, steps=3):
     mu = rng.normal(size=n) + 1j * rng.normal(size=n)
     x = rng.normal(size=(ntarget, 3))
     trg = x / np.linalg.norm(x, axis=1)[:, None] * rng.uniform(0.9, 1.1, ntarget)[:, None]
-    F.lfmm3dtriatarg(iprec, T, q, mu, trg)   # warm-up
+    if zk is None:
+        call = lambda: F.lfmm3dtriatarg(iprec, T, q, mu, trg)  # noqa: E731
+    else:
+        call = lambda: F.hfmm3dtriatarg(iprec, zk, T, q, mu, trg)  # noqa: E731
+    call()   # warm-up
     lat, stages = [], []
     for _ in range(steps):
         a = time.perf_counter()
-        ier, pot, fld, pt, ft = F.lfmm3dtriatarg(iprec, T, q, mu, trg)
+        ier, pot, fld, pt, ft = call()
         lat.append(time.perf_counter() - a)
         stages.append(F.last_timings())
         assert ier == 0
     idx = rng.choice(n, 100, replace=False)
     it = rng.choice(ntarget, 100, replace=False)
-    po, fo = ob.orc_tria_direct_sample(T, T.mean(axis=1)[idx], idx.astype(np.int32), q, mu)
-    pto, fto = ob.orc_tria_direct_sample(T, trg[it], None, q, mu)
+    cen = T.mean(axis=1)
+    if zk is None:
+        po, fo = ob.orc_tria_direct_sample(T, cen[idx], idx.astype(np.int32), q, mu)
+        pto, fto = ob.orc_tria_direct_sample(T, trg[it], None, q, mu)
+    else:
+        nq = 2 if iprec < 0 else 4 if iprec == 0 else 6
+        po, fo = ob.orc_htria_direct_sample(T, zk, nq, cen[idx], idx.astype(np.int32), q, mu)
+        pto, fto = ob.orc_htria_direct_sample(T, zk, nq, trg[it], None, q, mu)
     ms = float(np.median(lat)) * 1e3
-    return {"workload": "lfmm3dtriatarg: %d triangles (icosphere level %d) + %d targets, single + dipole layer, "
-                        "iprec %d, FP64, host arrays" % (n, level, ntarget, iprec),
+    name = "lfmm3dtriatarg" if zk is None else "hfmm3dtriatarg (zk = %s)" % zk
+    return {"workload": "%s: %d triangles (icosphere level %d) + %d targets, single + dipole layer, "
+                        "iprec %d, FP64, host arrays" % (name, n, level, ntarget, iprec),
             "ms_per_call": ms, "points_per_s": (n + ntarget) / (ms * 1e-3),
             "err_pot": rel(pot[idx], po), "err_fld": rel(fld[idx], fo),
             "err_pottarg": rel(pt[it], pto), "err_fldtarg": rel(ft[it], fto),
@@ -508,10 +520,12 @@ def main():
               % (helm["ms_per_step"], helm["err_pot"], helm["err_fld"],
                  {k: round(v, 2) for k, v in helm["stages_ms"].items()}), file=sys.stderr, flush=True)
     tria = None if args.no_tria else triangle_leg(F, ob)
-    if tria:
-        print("# triangles: %.1f ms/call, err pot %.2e fld %.2e pottarg %.2e fldtarg %.2e, stages %s"
-              % (tria["ms_per_call"], tria["err_pot"], tria["err_fld"], tria["err_pottarg"], tria["err_fldtarg"],
-                 {k: round(v, 2) for k, v in tria["stages_ms"].items()}), file=sys.stderr, flush=True)
+    htria = None if args.no_tria else triangle_leg(F, ob, zk=10.0)
+    for name, t in (("triangles", tria), ("triangles (helmholtz)", htria)):
+        if t:
+            print("# %s: %.1f ms/call, err pot %.2e fld %.2e pottarg %.2e fldtarg %.2e, stages %s"
+                  % (name, t["ms_per_call"], t["err_pot"], t["err_fld"], t["err_pottarg"], t["err_fldtarg"],
+                     {k: round(v, 2) for k, v in t["stages_ms"].items()}), file=sys.stderr, flush=True)
     serve = None if args.no_serve else serving_leg(F, args.serve_requests, args.serve_workers)
     if serve:
         print("# serving: p50 %.1f ms p99 %.1f ms, saturated %.2f req/s (%.3e points/s)"
@@ -557,6 +571,7 @@ def main():
         "modes": {k: {kk: vv for kk, vv in v.items() if kk != "latencies_ms"} for k, v in modes.items()},
         "helmholtz": helm,
         "triangles": tria,
+        "triangles_helmholtz": htria,
         "serving": serve,
         "stream5": s5,
     }

@@ -95,7 +95,7 @@ __device__ __forceinline__ void htri_pair(const double* __restrict__ g, const do
     const double dx = t.px - a, dy = t.py - b;
     const double r2 = fma(dx, dx, fma(dy, dy, h2)), r = sqrt(r2);
     const Zd uu = zscale(zz, r);
-    const double ex = exp(uu.r);
+    const double ex = hp.kim == 0.0 ? 1.0 : exp(uu.r);   // real k: |e^u| = 1
     double sn, cs;
     sincos(uu.i, &sn, &cs);
     const Zd e = {ex * cs, ex * sn};

@@ -19,8 +19,9 @@
 //                 fld = (-h sum m [1/(R_b(R_b+s_b)) - 1/(R_a(R_a+s_a))], sum (d/R0^2)(s_b/R_b - s_a/R_a))
 // At a triangle's own centroid h = 0 (principal values: single-layer normal
 // field 0, dipole-layer potential 0, its normal field the finite part).
-// Far field: each triangle becomes nq weighted points of a product Gauss rule
-// (collapsed square, exact to degree 2q-2) with the particle strengths
+// Far field: each triangle becomes the nq weighted nodes of a rule on it
+// (symmetric up to degree 6, a collapsed-square product Gauss rule beyond)
+// with the particle strengths
 // layout (charge, dipole vector = mu w 2A n), and the particle P2M/P2L
 // kernels form the expansions of the leaf boxes from those points.
 #pragma once

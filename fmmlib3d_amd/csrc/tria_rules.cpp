@@ -1,8 +1,9 @@
 // Leaf sizes and quadrature rules of the triangle FMMs.
 //
-// Far field: each triangle becomes the nodes of a product Gauss rule (the
-// square [0,1]^2 collapsed onto the reference triangle), exact to degree
-// 2q - 2.
+// Far field: each triangle becomes the nodes of a rule on it -- up to
+// degree 6 the symmetric rules below, beyond that a product Gauss rule (the
+// square [0,1]^2 collapsed onto the reference triangle, exact to degree
+// 2q - 2).
 //
 // Near field of the Helmholtz triangle FMM: the smooth part
 // (e^{ikr} - 1)/r of the kernel is integrated with the D3-symmetric rules
@@ -146,15 +147,17 @@ std::vector<double> solve_rule(const std::vector<Orbit>& orb, std::vector<double
 }  // namespace
 
 std::vector<double> tri_rule(int iprec, bool helmholtz) {
-  // q by iprec: 2 (iprec <= -1), 3 (iprec 0), 4 (iprec 1, 2), 5, 6, 7 (iprec
-  // 3, 4, 5) and one more from iprec 3 on for the Helmholtz kernel, whose
-  // expansions of a triangle also carry its phase e^{ik.y}: degrees 2, 4, 6,
-  // 8 (10), 10 (12), 12 (14).  The reference's own far-field rules are of
-  // order 2, 4, 6 (Laplace, src/lfmm3dtria.f:680-701) and 2, 4 (Helmholtz,
-  // src/hfmm3dtria.f:730-751).
-  // (iprec 6 has no far field: every point in one leaf)
-  int q = iprec <= -1 || iprec >= 6 ? 2 : iprec == 0 ? 3 : iprec <= 2 ? 4 : iprec + 2;
-  if (helmholtz && iprec >= 3 && iprec <= 5) q += 1;
+  // Degree by iprec: 2 (iprec <= -1), 4 (iprec 0), 6 (iprec 1, 2) by the
+  // symmetric rules (3, 6, 12 nodes); from iprec 3 on the product rule with
+  // q = iprec + 2 (degrees 8, 10, 12 for iprec 3, 4, 5), one more for the
+  // Helmholtz kernel, whose expansions of a triangle also carry its phase
+  // e^{ik.y}.  The reference's own far-field rules are of order 2, 4, 6
+  // (Laplace, src/lfmm3dtria.f:680-701) and 2, 4 (Helmholtz,
+  // src/hfmm3dtria.f:730-751).  iprec 6 has no far field (every point in
+  // one leaf): the smallest rule.
+  if (iprec <= 2 || iprec >= 6) return tri_sym_rule(iprec <= -1 || iprec >= 6 ? 2 : iprec == 0 ? 4 : 6);
+  int q = iprec + 2;
+  if (helmholtz) q += 1;
   std::vector<double> x, w, r;
   gauss01(q, x, w);
   for (int i = 0; i < q; i++)
