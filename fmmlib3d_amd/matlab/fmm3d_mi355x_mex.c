@@ -12,6 +12,19 @@
  *   [pot,fld,pottarg,fldtarg] = fmm3d_mi355x_mex('l3dpartdirect', source,
  *        charge, dipstr, dipvec, ifpot, iffld, target, ifpottarg, iffldtarg)
  *   [pot,...] = fmm3d_mi355x_mex('h3dpartdirect', zk, source, ...)
+ *   [ier,pot,fld,pottarg,fldtarg] = fmm3d_mi355x_mex('lfmm3dtriatarg', iprec,
+ *        triaflat, source, ifcharge, charge, ifdipole, dipstr, ifpot, iffld,
+ *        target, ifpottarg, iffldtarg)
+ *   [ier,...] = fmm3d_mi355x_mex('hfmm3dtriatarg', iprec, zk, triaflat, ...)
+ *   [pot,fld,pottarg,fldtarg] = fmm3d_mi355x_mex('l3dtriadirect', triaflat,
+ *        source, ifcharge, charge, ifdipole, dipstr, ifpot, iffld, target,
+ *        ifpottarg, iffldtarg)
+ *   [pot,...] = fmm3d_mi355x_mex('h3dtriadirect', nqtri, zk, triaflat, ...)
+ *
+ * (the triangle FMMs, matlab/fmm3d.mw of the reference: triaflat is 9 x n --
+ * the three vertices of each triangle -- and the dipole layer is along the
+ * normal of the vertex order; ifcharge / ifdipole 2 give the Helmholtz
+ * difference kernel)
  *
  * source/target: real 3 x n; charge/dipstr: complex (real accepted) vectors
  * of n; dipvec: real 3 x n; an empty charge/dipstr turns that term off and an
@@ -118,6 +131,70 @@ static void particle_call(int kind, int nlhs, mxArray *plhs[], const mxArray *pr
   }
 }
 
+/* arguments after the routine name (and iprec / zk / nqtri): triaflat,
+ * source, ifcharge, charge, ifdipole, dipstr, ifpot, iffld, target,
+ * ifpottarg, iffldtarg */
+static void triangle_call(int kind, int nlhs, mxArray *plhs[], const mxArray *prhs[], int iprec,
+                          const double *zk) {
+  mwSize ns, nt;
+  if (!mxIsDouble(prhs[0]) || mxIsComplex(prhs[0]) || mxGetM(prhs[0]) != 9)
+    mexErrMsgIdAndTxt("fmm3d:shape", "triaflat must be a real 9 x n array");
+  ns = mxGetN(prhs[0]);
+  if (ns < 1) mexErrMsgIdAndTxt("fmm3d:shape", "triaflat must hold at least one triangle");
+  const double *tri = mxGetDoubles(prhs[0]);
+  mwSize nsrc;
+  const double *src = points_in(prhs[1], "source", &nsrc);
+  if (nsrc != ns) mexErrMsgIdAndTxt("fmm3d:shape", "source must be 3 x (number of triangles)");
+  const int ifcharge = (int)mxGetScalar(prhs[2]), ifdipole = (int)mxGetScalar(prhs[4]);
+  const double *q = ifcharge ? complex_in(prhs[3], ns, "charge") : NULL;
+  const double *ds = ifdipole ? complex_in(prhs[5], ns, "dipstr") : NULL;
+  /* the unit normals of the vertex order (the reference's trianorm and dipvec) */
+  double *nrm = (double *)mxMalloc(3 * ns * sizeof(double));
+  for (mwSize i = 0; i < ns; i++) {
+    const double *v = tri + 9 * i;
+    const double u[3] = {v[3] - v[0], v[4] - v[1], v[5] - v[2]}, w[3] = {v[6] - v[0], v[7] - v[1], v[8] - v[2]};
+    double n[3] = {u[1] * w[2] - u[2] * w[1], u[2] * w[0] - u[0] * w[2], u[0] * w[1] - u[1] * w[0]};
+    double l = n[0] * n[0] + n[1] * n[1] + n[2] * n[2];
+    l = l > 0 ? 1.0 / __builtin_sqrt(l) : 0.0;
+    for (int k = 0; k < 3; k++) nrm[3 * i + k] = n[k] * l;
+  }
+  const int ifpot = flag(prhs[6]), iffld = flag(prhs[7]);
+  const double *trg = points_in(prhs[8], "target", &nt);
+  const int ifpottarg = nt > 0 && flag(prhs[9]), iffldtarg = nt > 0 && flag(prhs[10]);
+  double *pot, *fld, *pt, *ft;
+  const int direct = kind >= 2, off = direct ? 0 : 1;
+  mxArray *out[5];
+  out[off + 0] = complex_out(1, ns, ifpot, &pot);
+  out[off + 1] = complex_out(3, ns, iffld, &fld);
+  out[off + 2] = complex_out(1, nt, ifpottarg, &pt);
+  out[off + 3] = complex_out(3, nt, iffldtarg, &ft);
+  int ier = 0;
+  switch (kind) {
+    case 0:
+      ier = fmm3d_lfmm3dtriatarg(iprec, (int)ns, tri, nrm, src, ifcharge, q, ifdipole, ds, nrm, ifpot, pot, iffld,
+                                 fld, (int)nt, trg, ifpottarg, pt, iffldtarg, ft);
+      break;
+    case 1:
+      ier = fmm3d_hfmm3dtriatarg(iprec, zk, (int)ns, tri, nrm, src, ifcharge, q, ifdipole, ds, nrm, ifpot, pot,
+                                 iffld, fld, (int)nt, trg, ifpottarg, pt, iffldtarg, ft);
+      break;
+    case 2:
+      fmm3d_l3dtriadirect((int)ns, tri, nrm, src, ifcharge, q, ifdipole, ds, nrm, ifpot, pot, iffld, fld, (int)nt,
+                          trg, ifpottarg, pt, iffldtarg, ft);
+      break;
+    default: /* iprec carries nqtri */
+      fmm3d_h3dtriadirect(iprec, zk, (int)ns, tri, nrm, src, ifcharge, q, ifdipole, ds, nrm, ifpot, pot, iffld,
+                          fld, (int)nt, trg, ifpottarg, pt, iffldtarg, ft);
+      break;
+  }
+  if (!direct) out[0] = mxCreateDoubleScalar((double)ier);
+  const int nout = direct ? 4 : 5;
+  for (int k = 0; k < nout; k++) {
+    if (k < (nlhs > 0 ? nlhs : 1)) plhs[k] = out[k];
+    else mxDestroyArray(out[k]);
+  }
+}
+
 void mexFunction(int nlhs, mxArray *plhs[], int nrhs, const mxArray *prhs[]) {
   char name[32];
   if (nrhs < 1 || !mxIsChar(prhs[0]) || mxGetString(prhs[0], name, sizeof(name)) != 0)
@@ -139,6 +216,22 @@ void mexFunction(int nlhs, mxArray *plhs[], int nrhs, const mxArray *prhs[]) {
     zk[0] = mxGetScalar(prhs[1]);
     if (mxIsComplex(prhs[1])) zk[1] = ((const double *)mxGetComplexDoubles(prhs[1]))[1];
     particle_call(3, nlhs, plhs, prhs + 2, 0, zk);
+  } else if (!strcmp(name, "lfmm3dtriatarg")) {
+    if (nrhs != 13 || nlhs > 5) mexErrMsgIdAndTxt("fmm3d:usage", "lfmm3dtriatarg: 12 arguments after the name");
+    triangle_call(0, nlhs, plhs, prhs + 2, (int)mxGetScalar(prhs[1]), NULL);
+  } else if (!strcmp(name, "hfmm3dtriatarg")) {
+    if (nrhs != 14 || nlhs > 5) mexErrMsgIdAndTxt("fmm3d:usage", "hfmm3dtriatarg: 13 arguments after the name");
+    zk[0] = mxGetScalar(prhs[2]);
+    if (mxIsComplex(prhs[2])) zk[1] = ((const double *)mxGetComplexDoubles(prhs[2]))[1];
+    triangle_call(1, nlhs, plhs, prhs + 3, (int)mxGetScalar(prhs[1]), zk);
+  } else if (!strcmp(name, "l3dtriadirect")) {
+    if (nrhs != 12 || nlhs > 4) mexErrMsgIdAndTxt("fmm3d:usage", "l3dtriadirect: 11 arguments after the name");
+    triangle_call(2, nlhs, plhs, prhs + 1, 0, NULL);
+  } else if (!strcmp(name, "h3dtriadirect")) {
+    if (nrhs != 14 || nlhs > 4) mexErrMsgIdAndTxt("fmm3d:usage", "h3dtriadirect: 13 arguments after the name");
+    zk[0] = mxGetScalar(prhs[2]);
+    if (mxIsComplex(prhs[2])) zk[1] = ((const double *)mxGetComplexDoubles(prhs[2]))[1];
+    triangle_call(3, nlhs, plhs, prhs + 3, (int)mxGetScalar(prhs[1]), zk);
   } else {
     mexErrMsgIdAndTxt("fmm3d:usage", "unknown routine %s", name);
   }
