@@ -95,10 +95,19 @@ def build_oracle(verbose=False):
         print("built oracle")
 
 
+def build_test_gateway(verbose=False):
+    """The MATLAB gateway compiled against the mx API test double in
+    tests/matlab (test infrastructure; needs the HIP library built)."""
+    _run(["make", "-s", "-B", "-C", os.path.join(ROOT, "tests", "matlab")])
+    if verbose:
+        print("built MATLAB gateway test library")
+
+
 def main():
     build_hip(verbose=True)
     build_fortran(verbose=True)
     build_oracle(verbose=True)
+    build_test_gateway(verbose=True)
 
 
 if __name__ == "__main__":
