@@ -167,3 +167,32 @@ def test_gpu_htria_fmm_vs_reference_fmm():
     print("vs reference FMM: pot %.2e fld %.2e pottarg %.2e fldtarg %.2e" % tuple(e))
     assert e[0] < 2 * IPREC_EPS[2] and e[2] < 2 * IPREC_EPS[2]
     assert e[1] < 20 * IPREC_EPS[2] and e[3] < 20 * IPREC_EPS[2]
+
+
+@pytest.mark.parametrize("iprec", [4, 5])
+def test_gpu_htria_fmm_high_iprec(iprec):
+    """iprec 4 and 5: within the tolerance at iprec 4; at iprec 5 the FP64
+    rounding of the sums, about 4e-14 here, is the floor."""
+    rng = np.random.default_rng(101 + iprec)
+    T = icosphere(4)
+    n = T.shape[0]
+    q, mu = _densities(rng, n, 1, 1)
+    trg = _targets(rng, 1000)
+    zk = 2.0 + 0.1j
+    out = F.hfmm3dtriatarg(iprec, zk, T, q, mu, trg)
+    e = _check(T, zk, iprec, q, mu, trg, out, rng, 1, 1, "sphere")
+    assert max(e) < max(IPREC_EPS[iprec], 2e-13)
+
+
+@pytest.mark.parametrize("m", [1, 2])
+def test_gpu_htria_fmm_one_or_two_triangles(m):
+    rng = np.random.default_rng(105 + m)
+    T = icosphere(1)[:m]
+    q, mu = _densities(rng, m, 1, 1)
+    trg = _targets(rng, 50)
+    zk = 3.0
+    ier, pot, fld, pt, ft = F.hfmm3dtriatarg(2, zk, T, q, mu, trg)
+    pd, fd, ptd, ftd = F.h3dtriadirect(6, zk, T, q, mu, trg)
+    assert ier == 0
+    for a, b in zip((pot, fld, pt, ft), (pd, fd, ptd, ftd)):
+        assert rel(a, b) < 1e-14

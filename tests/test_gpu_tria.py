@@ -181,3 +181,39 @@ def test_gpu_tria_fmm_large_sphere():
     e = [rel(pot[idx], po), rel(fld[idx], fo), rel(pt[it], pto), rel(ft[it], fto)]
     print("large sphere: pot %.2e fld %.2e pottarg %.2e fldtarg %.2e, %s" % (*e, F.last_timings()))
     assert e[0] < IPREC_EPS[2] and e[2] < IPREC_EPS[2]
+
+
+@pytest.mark.parametrize("iprec", [4, 5])
+def test_gpu_tria_fmm_high_iprec(iprec):
+    """iprec 4 and 5 (far-field product rules of 6 and 7 points per
+    direction): within the tolerance at iprec 4; at iprec 5 (0.5e-15) the
+    FP64 rounding of the sums themselves, about 4e-14 here, is the floor."""
+    rng = np.random.default_rng(79 + iprec)
+    T = icosphere(4)
+    n = T.shape[0]
+    q, mu = _densities(rng, n, 1, 1)
+    trg = _targets(rng, 1000)
+    ier, pot, fld, pt, ft = F.lfmm3dtriatarg(iprec, T, q, mu, trg)
+    assert ier == 0
+    idx = rng.choice(n, 200, replace=False)
+    it = rng.choice(1000, 200, replace=False)
+    po, fo = ob.orc_tria_direct_sample(T, T.mean(axis=1)[idx], idx.astype(np.int32), q, mu)
+    pto, fto = ob.orc_tria_direct_sample(T, trg[it], None, q, mu)
+    e = [rel(pot[idx], po), rel(fld[idx], fo), rel(pt[it], pto), rel(ft[it], fto)]
+    print("iprec %d: pot %.2e fld %.2e pottarg %.2e fldtarg %.2e" % (iprec, *e))
+    bar = max(IPREC_EPS[iprec], 2e-13)
+    assert max(e) < bar
+
+
+@pytest.mark.parametrize("m", [1, 2])
+def test_gpu_tria_fmm_one_or_two_triangles(m):
+    """A tree of one leaf: the FMM is the direct sum."""
+    rng = np.random.default_rng(90 + m)
+    T = icosphere(1)[:m]
+    q, mu = _densities(rng, m, 1, 1)
+    trg = _targets(rng, 50)
+    ier, pot, fld, pt, ft = F.lfmm3dtriatarg(2, T, q, mu, trg)
+    pd, fd, ptd, ftd = F.l3dtriadirect(T, q, mu, trg)
+    assert ier == 0
+    for a, b in zip((pot, fld, pt, ft), (pd, fd, ptd, ftd)):
+        assert rel(a, b) < 1e-14
