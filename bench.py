@@ -180,6 +180,72 @@ def cpu_baseline(n=N, runs=3):
     return r
 
 
+def tri_inputs(level, ntarget):
+    """The triangle legs' inputs: icosphere `level`, N(0,1) complex
+    densities, targets in the shell 0.9 < r < 1.1 (seed 77)."""
+    import numpy as np
+    from fmm_inputs import icosphere
+    rng = np.random.default_rng(77)
+    T = icosphere(level)
+    n = T.shape[0]
+    q = rng.normal(size=n) + 1j * rng.normal(size=n)
+    mu = rng.normal(size=n) + 1j * rng.normal(size=n)
+    x = rng.normal(size=(ntarget, 3))
+    trg = x / np.linalg.norm(x, axis=1)[:, None] * rng.uniform(0.9, 1.1, ntarget)[:, None]
+    return rng, T, q, mu, trg
+
+
+TRI_SAMPLE = (5, 6250)   # icosphere level 5 (20,480 triangles) + 6,250 targets
+
+
+def tri_baseline_child(zk):
+    """Runs in a child process: the reference triangle FMM (lfmm3dtriatarg,
+    or hfmm3dtriatarg when zk is given) on the bounded sample, one call."""
+    import oracle_bindings as ob
+    _, T, q, mu, trg = tri_inputs(*TRI_SAMPLE)
+    t = time.perf_counter()
+    if zk is None:
+        ier, *_ = ob.ref_lfmm3dtriatarg(2, T, T.mean(axis=1), q, mu, trg)
+    else:
+        ier, *_ = ob.ref_hfmm3dtriatarg(2, zk, T, T.mean(axis=1), q, mu, trg)
+    dt = time.perf_counter() - t
+    print(json.dumps({"ier": ier, "seconds": dt}), flush=True)
+
+
+def tri_baseline(F, ob, zk):
+    """The reference triangle FMM on the bounded sample (OMP_NUM_THREADS
+    threads, a child process: the reference prints to unit 6) beside this
+    library on the same sample (median of 3 after a warm-up)."""
+    import numpy as np
+    if not ob.ref_available():
+        return None
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    env = dict(os.environ, OMP_NUM_THREADS=str(threads), OMP_STACKSIZE="1024M", OMP_PROC_BIND="close")
+    args = [sys.executable, "-u", os.path.abspath(__file__), "--tri-baseline-child"]
+    if zk is not None:
+        args += ["--tri-zk", repr(zk)]
+    try:
+        out = subprocess.run(args, env=env, stdout=subprocess.PIPE, stderr=subprocess.PIPE, text=True, timeout=600)
+        res = json.loads([ln for ln in out.stdout.splitlines() if ln.startswith('{"ier"')][-1])
+    except Exception as e:  # reported, never required
+        return {"sample": "failed: %s" % e}
+    _, T, q, mu, trg = tri_inputs(*TRI_SAMPLE)
+    call = (lambda: F.lfmm3dtriatarg(2, T, q, mu, trg)) if zk is None else \
+        (lambda: F.hfmm3dtriatarg(2, zk, T, q, mu, trg))
+    call()
+    lat = []
+    for _ in range(3):
+        a = time.perf_counter()
+        call()
+        lat.append(time.perf_counter() - a)
+    gpu = float(np.median(lat))
+    return {"sample": "%d triangles (icosphere level %d) + %d targets, iprec 2, same seeds; reference: one call, "
+                      "%d OpenMP threads; this library: median of 3 after a warm-up, host arrays"
+                      % (T.shape[0], TRI_SAMPLE[0], TRI_SAMPLE[1], threads),
+            "reference_s": res["seconds"], "gpu_s": gpu, "ratio": res["seconds"] / gpu, "cores": threads,
+            "kind": "reference"}
+
+
 def helmholtz_leg(F, torch, ob, steps, warmup):
     """Config 3: hfmm3dpartself on 500,000 points on the unit sphere (normalised
     Gaussian vectors), zk = 20, iprec = 2, charges with N(0,1) parts, FP64;
@@ -354,14 +420,9 @@ def triangle_leg(F, ob, level=7, ntarget=100_000, iprec=2, steps=3, zk=None):
     iprec 2, FP64, host arrays in and out; checked against the oracle's
     direct sum at 100 centroids and 100 targets."""
     import numpy as np
-    from fmm_inputs import icosphere, rel
-    rng = np.random.default_rng(77)
-    T = icosphere(level)
+    from fmm_inputs import rel
+    rng, T, q, mu, trg = tri_inputs(level, ntarget)
     n = T.shape[0]
-    q = rng.normal(size=n) + 1j * rng.normal(size=n)
-    mu = rng.normal(size=n) + 1j * rng.normal(size=n)
-    x = rng.normal(size=(ntarget, 3))
-    trg = x / np.linalg.norm(x, axis=1)[:, None] * rng.uniform(0.9, 1.1, ntarget)[:, None]
     if zk is None:
         call = lambda: F.lfmm3dtriatarg(iprec, T, q, mu, trg)  # noqa: E731
     else:
@@ -437,12 +498,17 @@ def main():
     ap.add_argument("--no-stream5", action="store_true", help="skip the mixed Laplace/Helmholtz Plummer stream")
     ap.add_argument("--stream5-requests", type=int, default=256)
     ap.add_argument("--cpu-baseline-child", type=int, default=0, help=argparse.SUPPRESS)
+    ap.add_argument("--tri-baseline-child", action="store_true", help=argparse.SUPPRESS)
+    ap.add_argument("--tri-zk", type=complex, default=None, help=argparse.SUPPRESS)
     ap.add_argument("--cpu-runs", type=int, default=3, help="timed reference calls after the warm-up")
     ap.add_argument("--calibrate-copy-mb", type=int, default=0,
                     help="only run a plain device-to-device copy of this many MiB (HBM counter calibration)")
     args = ap.parse_args()
     if args.cpu_baseline_child:
         cpu_baseline_child(args.cpu_baseline_child, args.cpu_runs)
+        return
+    if args.tri_baseline_child:
+        tri_baseline_child(args.tri_zk)
         return
     if args.calibrate_copy_mb:
         import torch
@@ -521,11 +587,18 @@ def main():
                  {k: round(v, 2) for k, v in helm["stages_ms"].items()}), file=sys.stderr, flush=True)
     tria = None if args.no_tria else triangle_leg(F, ob)
     htria = None if args.no_tria else triangle_leg(F, ob, zk=10.0)
-    for name, t in (("triangles", tria), ("triangles (helmholtz)", htria)):
+    for name, t, zk in (("triangles", tria, None), ("triangles (helmholtz)", htria, 10.0)):
         if t:
             print("# %s: %.1f ms/call, err pot %.2e fld %.2e pottarg %.2e fldtarg %.2e, stages %s"
                   % (name, t["ms_per_call"], t["err_pot"], t["err_fld"], t["err_pottarg"], t["err_fldtarg"],
                      {k: round(v, 2) for k, v in t["stages_ms"].items()}), file=sys.stderr, flush=True)
+            if not args.no_cpu:
+                t["cpu_reference"] = tri_baseline(F, ob, zk)
+                c = t["cpu_reference"]
+                if c and "reference_s" in c:
+                    print("#   reference on %s: %.2f s, this library %.1f ms (%.0fx)"
+                          % (c["sample"].split(",")[0], c["reference_s"], 1e3 * c["gpu_s"], c["ratio"]),
+                          file=sys.stderr, flush=True)
     serve = None if args.no_serve else serving_leg(F, args.serve_requests, args.serve_workers)
     if serve:
         print("# serving: p50 %.1f ms p99 %.1f ms, saturated %.2f req/s (%.3e points/s)"
