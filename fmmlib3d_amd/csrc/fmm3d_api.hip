@@ -32,16 +32,26 @@ int guarded(F&& f) {
   }
 }
 
+// Host arrays of a few MB and more go through the context's pinned staging
+// buffers (host copy and DMA overlapped); smaller ones are copied directly.
+constexpr size_t kStagedMin = (size_t)4 << 20;
+
 double* upload(Context& ctx, const char* name, const double* h, size_t n) {
   if (!h || n == 0) return nullptr;
   double* d = (double*)ctx.arena.get(name, n * sizeof(double));
-  FMM_HIP(hipMemcpyAsync(d, h, n * sizeof(double), hipMemcpyHostToDevice, ctx.stream));
+  if (n * sizeof(double) >= kStagedMin && !ctx.arena.fixed())
+    staged_upload(ctx, d, h, n * sizeof(double));
+  else
+    FMM_HIP(hipMemcpyAsync(d, h, n * sizeof(double), hipMemcpyHostToDevice, ctx.stream));
   return d;
 }
 
 void download(Context& ctx, double* h, const double* d, size_t n) {
   if (!h || !d || n == 0) return;
-  FMM_HIP(hipMemcpyAsync(h, d, n * sizeof(double), hipMemcpyDeviceToHost, ctx.stream));
+  if (n * sizeof(double) >= kStagedMin && !ctx.arena.fixed())
+    staged_download(ctx, h, d, n * sizeof(double));
+  else
+    FMM_HIP(hipMemcpyAsync(h, d, n * sizeof(double), hipMemcpyDeviceToHost, ctx.stream));
 }
 
 // kernel: nullptr = Laplace, else the Helmholtz wavenumber (re, im)

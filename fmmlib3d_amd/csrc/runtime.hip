@@ -11,6 +11,8 @@ Arena::~Arena() { release_all(); }
 Context::~Context() {
   for (auto& e : aux_ev)
     if (e) (void)hipEventDestroy(e);
+  for (auto& e : stage_ev)
+    if (e) (void)hipEventDestroy(e);
   if (aux) (void)hipStreamDestroy(aux);
 }
 
@@ -164,6 +166,75 @@ Context& default_context() {
 std::vector<double>& last_timings() {
   static thread_local std::vector<double> t(11, 0.0);
   return t;
+}
+
+// ---- staged host copies ----
+
+static void par_copy(void* dst, const void* src, size_t n) {
+  constexpr size_t kBlock = (size_t)1 << 20;
+  const long nb = (long)((n + kBlock - 1) / kBlock);
+  if (nb <= 2) {
+    std::memcpy(dst, src, n);
+    return;
+  }
+#pragma omp parallel for schedule(static)
+  for (long b = 0; b < nb; b++) {
+    const size_t o = (size_t)b * kBlock, len = o + kBlock <= n ? kBlock : n - o;
+    std::memcpy((char*)dst + o, (const char*)src + o, len);
+  }
+}
+
+static char* stage_buffer(Context& ctx, int k) {
+  for (auto& e : ctx.stage_ev)
+    if (!e) FMM_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+  return (char*)ctx.arena.get_host(k ? "stage_1" : "stage_0", kStageBytes);
+}
+
+// the staging buffer k is free once its last DMA has finished
+static void stage_wait(Context& ctx, int k) {
+  if (ctx.stage_pending[k]) {
+    FMM_HIP(hipEventSynchronize(ctx.stage_ev[k]));
+    ctx.stage_pending[k] = false;
+  }
+}
+
+void staged_upload(Context& ctx, void* dev, const void* host, size_t bytes) {
+  for (size_t off = 0; off < bytes; off += kStageBytes) {
+    const size_t len = off + kStageBytes <= bytes ? kStageBytes : bytes - off;
+    const int k = ctx.stage_next;
+    ctx.stage_next ^= 1;
+    char* pin = stage_buffer(ctx, k);
+    stage_wait(ctx, k);
+    par_copy(pin, (const char*)host + off, len);
+    FMM_HIP(hipMemcpyAsync((char*)dev + off, pin, len, hipMemcpyHostToDevice, ctx.stream));
+    FMM_HIP(hipEventRecord(ctx.stage_ev[k], ctx.stream));
+    ctx.stage_pending[k] = true;
+  }
+}
+
+void staged_download(Context& ctx, void* host, const void* dev, size_t bytes) {
+  const size_t nchunk = (bytes + kStageBytes - 1) / kStageBytes;
+  int kk[2] = {0, 0};
+  auto issue = [&](size_t c) {
+    const size_t off = c * kStageBytes, len = off + kStageBytes <= bytes ? kStageBytes : bytes - off;
+    const int k = ctx.stage_next;
+    ctx.stage_next ^= 1;
+    char* pin = stage_buffer(ctx, k);
+    stage_wait(ctx, k);
+    FMM_HIP(hipMemcpyAsync(pin, (const char*)dev + off, len, hipMemcpyDeviceToHost, ctx.stream));
+    FMM_HIP(hipEventRecord(ctx.stage_ev[k], ctx.stream));
+    ctx.stage_pending[k] = true;
+    kk[c & 1] = k;
+  };
+  if (nchunk == 0) return;
+  issue(0);
+  for (size_t c = 0; c < nchunk; c++) {
+    if (c + 1 < nchunk) issue(c + 1);  // the next chunk's DMA runs while this one is copied out
+    const int k = kk[c & 1];
+    const size_t off = c * kStageBytes, len = off + kStageBytes <= bytes ? kStageBytes : bytes - off;
+    stage_wait(ctx, k);
+    par_copy((char*)host + off, stage_buffer(ctx, k), len);
+  }
 }
 
 }  // namespace fmm3d

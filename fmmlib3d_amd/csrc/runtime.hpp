@@ -85,6 +85,11 @@ struct Context {
   // false: the near field stays on `stream` (the request server, whose
   // workers already overlap whole requests)
   bool near_on_aux = true;
+  // staged host copies (staged_upload / staged_download): the pinned
+  // buffer in turn and, per buffer, the event after its last DMA
+  int stage_next = 0;
+  hipEvent_t stage_ev[2] = {nullptr, nullptr};
+  bool stage_pending[2] = {false, false};
   ~Context();
   Context() = default;
   Context(const Context&) = delete;
@@ -93,6 +98,15 @@ struct Context {
 
 // the context's second stream, created on first use on the current device
 hipStream_t aux_stream(Context& ctx);
+
+// Copies between pageable host memory and the device on ctx.stream through
+// two pinned staging buffers of kStageBytes: the host-side copy of one chunk
+// (OpenMP threads) overlaps the DMA of the other.  staged_upload returns
+// once every chunk is queued (later work on ctx.stream is ordered after
+// it); staged_download returns with the host array filled.
+constexpr size_t kStageBytes = (size_t)16 << 20;
+void staged_upload(Context& ctx, void* dev, const void* host, size_t bytes);
+void staged_download(Context& ctx, void* host, const void* dev, size_t bytes);
 
 // the stream the near field of this call runs on: the second stream, or
 // `ctx.stream` when the context keeps it there or FMM3D_NEAR_STREAM=0
