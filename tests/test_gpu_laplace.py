@@ -301,10 +301,13 @@ def _hip():
     return rt
 
 
-def test_gpu_device_api_matches_host_api(rng):
+@pytest.mark.parametrize("ns,nt", [(20000, 8000), (700001, 300007)])
+def test_gpu_device_api_matches_host_api(rng, ns, nt):
+    """The host-array entry point equals the device-pointer one bitwise; at
+    700,001 + 300,007 points its arrays (11-34 MB) cross the pinned staging
+    buffers in several chunks with a partial last one."""
     import ctypes
     rt = _hip()
-    ns, nt = 20000, 8000
     src = points(rng, ns, "cube")
     trg = points(rng, nt, "cube")
     q, ds, dv = strengths(rng, ns, True, True)
