@@ -174,3 +174,52 @@ def test_oracle_htria_direct_matches_reference(zk, nq, fc, fd):
     pt, ft = ob.orc_htria_direct_sample(T, zk, nq, trg, None, q, mu, fc, fd)
     assert rel(po, rp) < 1e-13 and rel(fo, rf) < 1e-13
     assert rel(pt, rpt) < 1e-13 and rel(ft, rft) < 1e-13
+
+
+def _green_data(level, zk, x0=(0.0, 0.2, 0.0)):
+    """The reference drivers' check (examples/hfmm3dtria_driver.f, and its
+    Laplace twin): u = G(x - x0) for x0 inside a triangulated unit sphere,
+    single-layer density du/dn / (4 pi) (as fld . n), dipole-layer density
+    u / (4 pi); Green's identity gives pot = u/2 on the surface and
+    fld . n / (2 pi) = the single-layer density, to discretisation
+    accuracy."""
+    T = icosphere(level)
+    cen = T.mean(axis=1)
+    nrm = np.cross(T[:, 1] - T[:, 0], T[:, 2] - T[:, 0])
+    nrm /= np.linalg.norm(nrm, axis=1)[:, None]
+    d = cen - np.asarray(x0)
+    r = np.linalg.norm(d, axis=1)
+    if zk is None:
+        u, grad = 1 / r, -d / r[:, None] ** 3
+    else:
+        e = np.exp(1j * zk * r)
+        u, grad = e / r, ((1j * zk * r - 1) * e / r ** 3)[:, None] * d
+    charge = np.sum(-grad * nrm, 1) / (4 * np.pi)
+    dipstr = u / (4 * np.pi) + 0j
+    return T, cen, nrm, u, charge, dipstr
+
+
+def _green_errors(pot, fld, nrm, u, charge):
+    e_pot = np.linalg.norm(pot - u / 2) / np.linalg.norm(u / 2)
+    e_dn = np.linalg.norm(np.sum(fld * nrm, 1) / (2 * np.pi) - charge) / np.linalg.norm(charge)
+    return e_pot, e_dn
+
+
+@pytest.mark.parametrize("zk", [None, 1.0 + 0.1j])
+def test_oracle_tria_green_identity_converges(zk):
+    """Green's identity on the sphere with the oracle's direct sums: the
+    errors are discretisation errors and fall with refinement (about 4x for
+    the potential, 2x for the normal field per level); a wrong sign or a
+    missing self term leaves O(1) errors."""
+    errs = []
+    for level in (2, 3):
+        T, cen, nrm, u, charge, dipstr = _green_data(level, zk)
+        idx = np.arange(T.shape[0], dtype=np.int32)
+        if zk is None:
+            pot, fld = ob.orc_tria_direct_sample(T, cen, idx, charge, dipstr)
+        else:
+            pot, fld = ob.orc_htria_direct_sample(T, zk, 6, cen, idx, charge, dipstr)
+        errs.append(_green_errors(pot, fld, nrm, u, charge))
+    (p2, n2), (p3, n3) = errs
+    assert p3 < 6e-3 and p3 < p2 / 3
+    assert n3 < 6e-2 and n3 < 0.7 * n2
