@@ -148,3 +148,50 @@ def test_noalloc_rejects_problem_beyond_its_array():
     # argument errors come first, as in the other entry points
     assert lib.fmm3d_lfmm3dparttarg_dev_noalloc(0, 9, n, dummy, 1, dummy, 0, None, None, 1, dummy, 1, dummy,
                                                n, dummy, 1, dummy, 1, dummy, None) == 32
+
+
+def test_triangle_entries_reject_bad_arguments_before_device_use():
+    """The triangle FMMs check their arguments on the host: NULL triangles or
+    densities the flags ask for, a density flag outside 0..2 (Helmholtz) or
+    outside 0..1 (Laplace), iprec outside -2..6, zk = 0; the direct sums,
+    with no error channel, leave their outputs untouched."""
+    from fmm_inputs import icosphere
+    lib = F.load()
+    p = F._ptr
+    T = np.ascontiguousarray(icosphere(0).reshape(-1, 9))
+    n = T.shape[0]
+    cen = np.ascontiguousarray(T.reshape(n, 3, 3).mean(axis=1))
+    q = np.ones(n, np.complex128)
+    pot = np.zeros(n, np.complex128)
+    zk = np.array([2.0, 0.0])
+    # Helmholtz
+    assert lib.fmm3d_hfmm3dtriatarg(1, p(zk), n, None, None, p(cen), 1, p(q), 0, None, None, 1, p(pot), 0, None,
+                                    0, None, 0, None, 0, None) == 128
+    assert lib.fmm3d_hfmm3dtriatarg(1, p(zk), n, p(T), None, p(cen), 2, None, 0, None, None, 1, p(pot), 0, None,
+                                    0, None, 0, None, 0, None) == 128
+    assert lib.fmm3d_hfmm3dtriatarg(1, p(zk), n, p(T), None, p(cen), 0, None, 1, None, None, 1, p(pot), 0, None,
+                                    0, None, 0, None, 0, None) == 128
+    assert lib.fmm3d_hfmm3dtriatarg(1, p(zk), n, p(T), None, p(cen), 3, p(q), 0, None, None, 1, p(pot), 0, None,
+                                    0, None, 0, None, 0, None) == 128
+    assert lib.fmm3d_hfmm3dtriatarg(9, p(zk), n, p(T), None, p(cen), 1, p(q), 0, None, None, 1, p(pot), 0, None,
+                                    0, None, 0, None, 0, None) == 32
+    z0 = np.zeros(2)
+    assert lib.fmm3d_hfmm3dtriatarg(1, p(z0), n, p(T), None, p(cen), 1, p(q), 0, None, None, 1, p(pot), 0, None,
+                                    0, None, 0, None, 0, None) == 128
+    assert lib.fmm3d_hfmm3dtriatarg(1, None, n, p(T), None, p(cen), 1, p(q), 0, None, None, 1, p(pot), 0, None,
+                                    0, None, 0, None, 0, None) == 128
+    assert F.hfmm3dtriatarg(1, 2.0, np.zeros((0, 3, 3)), np.zeros(0, np.complex128))[0] == 4
+    # Laplace
+    assert lib.fmm3d_lfmm3dtriatarg(1, n, None, None, p(cen), 1, p(q), 0, None, None, 1, p(pot), 0, None,
+                                    0, None, 0, None, 0, None) == 128
+    assert lib.fmm3d_lfmm3dtriatarg(-3, n, p(T), None, p(cen), 1, p(q), 0, None, None, 1, p(pot), 0, None,
+                                    0, None, 0, None, 0, None) == 32
+    # the direct sums compute nothing on bad input
+    pot[:] = 7
+    lib.fmm3d_h3dtriadirect(7, p(zk), n, p(T), None, p(cen), 1, p(q), 0, None, None, 1, p(pot), 0, None, 0, None,
+                            0, None, 0, None)                                   # no rule of degree 7
+    lib.fmm3d_h3dtriadirect(6, p(zk), n, p(T), None, p(cen), 1, None, 0, None, None, 1, p(pot), 0, None, 0, None,
+                            0, None, 0, None)                                   # charge NULL
+    lib.fmm3d_l3dtriadirect(n, None, None, p(cen), 1, p(q), 0, None, None, 1, p(pot), 0, None, 0, None, 0, None,
+                            0, None)                                            # triangles NULL
+    assert np.all(pot == 7)
