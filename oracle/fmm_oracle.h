@@ -97,6 +97,16 @@ void orc_l3dpartdirect(int ns, const double *src, int ifcharge, const zdouble *q
 void orc_l3ddirect_hess_sample(int ns, const double *src, int ifcharge, const zdouble *q,
                                int ifdipole, const zdouble *ds, const double *dv,
                                int nt, const double *trg, const int *self_index, zdouble *hess);
+/* ---- quadrupole sources (fmm_oracle_qp.c; laprouts3d.f:8691 lpotfld3dhess_qp,
+ * :7914 l3dformmp_qp_trunc, :8261 l3dformta_qp_trunc).  quadvec(6,n) orders
+ * (xx, yy, zz, xy, xz, yz). ---- */
+void orc_qp_ladder_regular(int n, int m, double *Lj);
+void orc_qp_ladder_irregular(int n, int m, double *Lj);
+void orc_qp_channels(zdouble quadstr, const double *qv, zdouble *qj);
+void orc_l3dform_qp(int irreg, double rscale, const double *src, const zdouble *quadstr, const double *quadvec,
+                    int ns, const double *center, int p, zdouble *out);
+void orc_l3ddirect_qp_sample(int ns, const double *src, const zdouble *quadstr, const double *quadvec, int npts,
+                             const double *pts, const int *self_index, zdouble *pot, zdouble *fld);
 /* direct sum evaluated at a subset of targets/sources given by index lists */
 void orc_l3ddirect_sample(int ns, const double *src, int ifcharge, const zdouble *q,
                           int ifdipole, const zdouble *ds, const double *dv,

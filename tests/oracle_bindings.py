@@ -637,3 +637,77 @@ def ref_hfmm3dtriatarg(iprec, zk, tri, cent, charge=None, dipstr=None, target=No
                       _i(ifdipole if dipstr is not None else 0), _vp(ds), _vp(nrm), _i(1), _vp(pot), _i(1),
                       _vp(fld), _i(nt), _vp(trg), _i(1 if nt else 0), _vp(pt), _i(1 if nt else 0), _vp(ft))
     return ier.value, pot, fld, pt[:nt], ft[:nt]
+
+
+# --------------------------------------------------------------------------
+# quadrupole sources (oracle/fmm_oracle_qp.c; laprouts3d.f:7914, :8261, :8691)
+# --------------------------------------------------------------------------
+def orc_qp_ladder(n, m, irregular):
+    o = oracle()
+    L = np.zeros(5)
+    (o.orc_qp_ladder_irregular if irregular else o.orc_qp_ladder_regular)(int(n), int(m), _vp(L))
+    return L
+
+
+def orc_l3dform_qp(irregular, rscale, source, quadstr, quadvec, center, p):
+    """Packed multipole (irregular=False) or local (True) expansion of order p
+    of quadrupoles; quadvec (n, 6) ordered xx, yy, zz, xy, xz, yz."""
+    o = oracle()
+    src = _c(np.atleast_2d(source), np.float64)
+    qs = _c(np.atleast_1d(quadstr), np.complex128)
+    qv = _c(np.atleast_2d(quadvec), np.float64)
+    c = _c(center, np.float64)
+    out = np.zeros((p + 1) ** 2, np.complex128)
+    o.orc_l3dform_qp(int(bool(irregular)), ctypes.c_double(rscale), _vp(src), _vp(qs), _vp(qv), src.shape[0],
+                     _vp(c), int(p), _vp(out))
+    return out
+
+
+def orc_direct_qp_sample(source, quadstr, quadvec, points, self_index):
+    """FP64 pot and fld at `points` of quadrupole sources; self_index[j] >= 0
+    skips that source."""
+    o = oracle()
+    src = _c(source, np.float64)
+    qs = _c(quadstr, np.complex128)
+    qv = _c(quadvec, np.float64)
+    pts = _c(points, np.float64)
+    si = _c(self_index, np.int32)
+    pot = np.zeros(pts.shape[0], np.complex128)
+    fld = np.zeros((pts.shape[0], 3), np.complex128)
+    o.orc_l3ddirect_qp_sample(src.shape[0], _vp(src), _vp(qs), _vp(qv), pts.shape[0], _vp(pts), _vp(si), _vp(pot),
+                              _vp(fld))
+    return pot, fld
+
+
+def ref_lpotfld3dhess_qp(source, quadstr, quadvec, target):
+    """The reference's lpotfld3dhess_qp (one quadrupole, one target): pot, fld, hess."""
+    r = ref()
+    s, t, qv = _c(source, np.float64), _c(target, np.float64), _c(quadvec, np.float64)
+    qs = np.array([quadstr], np.complex128)
+    pot = np.zeros(1, np.complex128)
+    fld = np.zeros(3, np.complex128)
+    hs = np.zeros(6, np.complex128)
+    one = ctypes.c_int(1)
+    r.lpotfld3dhess_qp_(ctypes.byref(one), ctypes.byref(one), _vp(s), _vp(qs), _vp(qv), _vp(t), _vp(pot), _vp(fld),
+                        _vp(hs))
+    return pot[0], fld, hs
+
+
+def ref_l3dform_qp(irregular, rscale, source, quadstr, quadvec, center, p):
+    """The reference's l3dformmp_qp_trunc / l3dformta_qp_trunc, packed."""
+    r = ref()
+    src = _c(np.atleast_2d(source), np.float64)
+    qs = _c(np.atleast_1d(quadstr), np.complex128)
+    qv = _c(np.atleast_2d(quadvec), np.float64)
+    c = _c(center, np.float64)
+    nlege = p + 10
+    lw = 4 * (nlege + 1) ** 2 + 100
+    w = np.zeros(lw)
+    lused = ctypes.c_int(0)
+    r.ylgndrfwini_(_i(nlege), _vp(w), _i(lw), ctypes.byref(lused))
+    A = np.zeros((2 * p + 1) * (p + 1), np.complex128)
+    ier = ctypes.c_int(0)
+    fn = r.l3dformta_qp_trunc_ if irregular else r.l3dformmp_qp_trunc_
+    fn(ctypes.byref(ier), _d(rscale), _vp(src), _vp(qs), _vp(qv), _i(src.shape[0]), _vp(c), _i(p), _vp(A), _vp(w),
+       _i(nlege))
+    return ref_to_packed(A, p)
