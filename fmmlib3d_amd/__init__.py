@@ -30,7 +30,7 @@ PREC_FP32 = 3     # FP32 near and far field
 EXPORTED = (
     "fmm3d_lfmm3dparttarg", "fmm3d_lfmm3dpartself", "fmm3d_l3dpartdirect",
     "fmm3d_lfmm3dparttarg_ex", "fmm3d_lfmm3dparttarg_dev", "fmm3d_lfmm3dparttarg_dev_noalloc",
-    "fmm3d_lfmm3dparttarg_hess",
+    "fmm3d_lfmm3dparttarg_hess", "fmm3d_lfmm3dparttarg_qp",
     "fmm3d_last_timings",
     "fmm3d_last_precision", "fmm3d_version", "fmm3d_d3tstrcr",
     "fmm3d_hfmm3dparttarg", "fmm3d_hfmm3dpartself", "fmm3d_h3dpartdirect",
@@ -72,6 +72,9 @@ def load():
         lib.fmm3d_lfmm3dparttarg_hess.argtypes = [i, i, vp, i, vp, i, vp, vp, i, vp, i, vp, i, vp, i, vp, i, vp,
                                                   i, vp, i, vp]
         lib.fmm3d_lfmm3dparttarg_hess.restype = i
+        lib.fmm3d_lfmm3dparttarg_qp.argtypes = [i, i, vp, i, vp, i, vp, vp, i, vp, vp, i, vp, i, vp, i, vp, i, vp,
+                                                i, vp]
+        lib.fmm3d_lfmm3dparttarg_qp.restype = i
         lib.fmm3d_lfmm3dpartself.argtypes = [i, i, vp, i, vp, i, vp, vp, i, vp, i, vp]
         lib.fmm3d_lfmm3dpartself.restype = i
         lib.fmm3d_l3dpartdirect.argtypes = [i, vp, i, vp, i, vp, vp, i, vp, i, vp, i, vp, i, vp, i, vp]
@@ -207,6 +210,33 @@ def lfmm3dparttarg_hess(iprec, source, charge=None, dipstr=None, dipvec=None, ta
         int(bool(ifpot)), _ptr(pot), int(bool(iffld)), _ptr(fld), int(hs is not None), _ptr(hs),
         nt, _ptr(trg), int(pt is not None), _ptr(pt), int(ft is not None), _ptr(ft), int(ht is not None), _ptr(ht))
     return ier, pot, fld, hs, pt, ft, ht
+
+
+def lfmm3dparttarg_qp(iprec, source, charge=None, dipstr=None, dipvec=None, quadstr=None, quadvec=None,
+                      target=None, ifpot=1, iffld=1, ifpottarg=1, iffldtarg=1):
+    """Laplace FMM with quadrupole sources as well (FP64).  quadstr (n,)
+    complex and quadvec (n, 6) real, ordered (xx, yy, zz, xy, xz, yz), the
+    source of the reference's lpotfld3dhess_qp (src/laprouts3d.f:8691).
+    Returns (ier, pot, fld, pottarg, fldtarg); outputs whose flag is 0 are None."""
+    lib = load()
+    src, q, ds, dv, trg = _prep(source, charge, dipstr, dipvec, target)
+    ns = src.shape[0]
+    nt = 0 if trg is None else trg.shape[0]
+    qs = _in(quadstr, np.complex128)
+    qv = _in(quadvec, np.float64)
+    if (qs is None) != (qv is None):
+        raise ValueError("quadstr and quadvec go together")
+    if qs is not None and (qs.shape != (ns,) or qv.shape != (ns, 6)):
+        raise ValueError("quadstr (n,) and quadvec (n,6)")
+    pot = np.zeros(ns, np.complex128) if ifpot else None
+    fld = np.zeros((ns, 3), np.complex128) if iffld else None
+    pt = np.zeros(nt, np.complex128) if (ifpottarg and nt) else None
+    ft = np.zeros((nt, 3), np.complex128) if (iffldtarg and nt) else None
+    ier = lib.fmm3d_lfmm3dparttarg_qp(
+        int(iprec), ns, _ptr(src), int(q is not None), _ptr(q), int(ds is not None), _ptr(ds), _ptr(dv),
+        int(qs is not None), _ptr(qs), _ptr(qv), int(bool(ifpot)), _ptr(pot), int(bool(iffld)), _ptr(fld),
+        nt, _ptr(trg), int(pt is not None), _ptr(pt), int(ft is not None), _ptr(ft))
+    return ier, pot, fld, pt, ft
 
 
 def _tri_prep(triaflat, charge, dipstr, target, source):

@@ -98,13 +98,15 @@ int fmm_host(const double* zk, int prec_mode, int iprec, int ns, const double* s
              int ifpot, double* pot, int iffld, double* fld, int nt, const double* target,
              int ifpottarg, double* pottarg, int iffldtarg, double* fldtarg, int ifhess = 0,
              double* hess = nullptr, int ifhesstarg = 0, double* hesstarg = nullptr,
-             const double* tria = nullptr, bool tri_mode = false) {
+             const double* tria = nullptr, bool tri_mode = false, int ifquad = 0,
+             const double* quadstr = nullptr, const double* quadvec = nullptr) {
   if (int e = check_args(zk, prec_mode, iprec, ns, nt)) return e;
   if (tri_mode && !tria) return 128;
   if (int e = check_ptrs(ns, source, ifcharge, charge, ifdipole, dipstr, dipvec, ifpot, pot, iffld, fld, nt,
                          target, ifpottarg, pottarg, iffldtarg, fldtarg))
     return e;
   if ((ifhess == 1 && !hess) || (nt > 0 && ifhesstarg == 1 && !hesstarg)) return 128;
+  if (ifquad == 1 && (!quadstr || !quadvec)) return 128;
   Context& ctx = default_context();
   std::lock_guard<std::mutex> lock(ctx.mu);
   return guarded([&] {
@@ -122,6 +124,11 @@ int fmm_host(const double* zk, int prec_mode, int iprec, int ns, const double* s
     c.dv_dev = c.ifdipole ? upload(ctx, "in_dv", dipvec, 3 * (size_t)ns) : nullptr;
     c.trg_dev = nt > 0 ? upload(ctx, "in_trg", target, 3 * (size_t)nt) : nullptr;
     if (tri_mode) c.tri_dev = upload(ctx, "in_tri", tria, 9 * (size_t)ns);
+    c.ifquad = ifquad == 1;
+    if (c.ifquad) {
+      c.qs_dev = upload(ctx, "in_qs", quadstr, 2 * (size_t)ns);
+      c.qv_dev = upload(ctx, "in_qv", quadvec, 6 * (size_t)ns);
+    }
     if (c.ifpot) c.pot_dev = (double*)ctx.arena.get("out_pot", 2 * sizeof(double) * ns);
     if (c.iffld) c.fld_dev = (double*)ctx.arena.get("out_fld", 6 * sizeof(double) * ns);
     if (c.ifpottarg) c.pottarg_dev = (double*)ctx.arena.get("out_pt", 2 * sizeof(double) * nt);
@@ -166,6 +173,16 @@ int fmm3d_lfmm3dparttarg_hess(int iprec, int nsource, const double* source, int 
   return fmm_host(nullptr, 0, iprec, nsource, source, ifcharge, charge, ifdipole, dipstr, dipvec, ifpot, pot,
                   iffld, fld, ntarget, target, ifpottarg, pottarg, iffldtarg, fldtarg, ifhess, hess, ifhesstarg,
                   hesstarg);
+}
+
+int fmm3d_lfmm3dparttarg_qp(int iprec, int nsource, const double* source, int ifcharge,
+                            const double* charge, int ifdipole, const double* dipstr,
+                            const double* dipvec, int ifquad, const double* quadstr, const double* quadvec,
+                            int ifpot, double* pot, int iffld, double* fld, int ntarget, const double* target,
+                            int ifpottarg, double* pottarg, int iffldtarg, double* fldtarg) {
+  return fmm_host(nullptr, 0, iprec, nsource, source, ifcharge, charge, ifdipole, dipstr, dipvec, ifpot, pot,
+                  iffld, fld, ntarget, target, ifpottarg, pottarg, iffldtarg, fldtarg, 0, nullptr, 0, nullptr,
+                  nullptr, false, ifquad, quadstr, quadvec);
 }
 
 int fmm3d_lfmm3dparttarg_ex(int prec_mode, int iprec, int nsource, const double* source,

@@ -43,6 +43,10 @@ struct LapCall {
   int tri_lapc = 1, tri_lapd = 1, tri_nqtri = 0;
   // triangle FMMs: leave out the near field (the far field alone)
   bool far_only = false;
+  // Laplace, FP64 only: quadrupole sources, quadstr (complex) and quadvec
+  // (xx, yy, zz, xy, xz, yz) per source, in the caller's order
+  int ifquad = 0;
+  const double *qs_dev = nullptr, *qv_dev = nullptr;
 };
 
 // The precision mode a call runs in: the requested FMM3D_PREC_* mode while

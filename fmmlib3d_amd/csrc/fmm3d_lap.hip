@@ -22,6 +22,7 @@
 #include "lap_kernels.hip"
 #include "m2l_mfma.hpp"
 #include "hess_kernels.hpp"
+#include "qp_kernels.hpp"
 #include "tria_kernels.hpp"
 #include "tria_rules.hpp"
 #include "octree.hpp"
@@ -229,6 +230,9 @@ int lap_solve(Context& ctx, const LapCall& c) {
       return 128;
     nbox = tri_nbox(c.iprec, c.ns, c.nt);
   }
+  const bool qp = c.ifquad != 0;
+  if (qp && (tri || c.ifhess || c.ifhesstarg || !(std::is_same<T, double>::value && std::is_same<TN, double>::value)))
+    return 128;
   const int ns = c.ns, nt = c.nt;
   const Flags f{c.ifcharge ? 1 : 0, c.ifdipole ? 1 : 0, c.ifpot ? 1 : 0, c.iffld ? 1 : 0,
                 nt > 0 && c.ifpottarg ? 1 : 0, nt > 0 && c.iffldtarg ? 1 : 0};
@@ -357,6 +361,11 @@ int lap_solve(Context& ctx, const LapCall& c) {
                      c.q_dev, c.ds_dev, c.dv_dev, tri ? 0 : f.ifcharge, tri ? 0 : f.ifdipole, spos, str64, str32);
   if (nt > 0)
     hipLaunchKernelGGL(k_gather_targets, dim3(blocks(nt, 256)), dim3(256), 0, s, nt, tperm, c.trg_dev, tpos);
+  double* qstr = nullptr;
+  if (qp) {
+    qstr = (double*)ctx.arena.get("qp_str", sizeof(double) * 8 * (size_t)ns);
+    hipLaunchKernelGGL(k_gather_qp, dim3(blocks(ns, 256)), dim3(256), 0, s, ns, sperm, c.qs_dev, c.qv_dev, qstr);
+  }
   auto zero = [&](auto* a, size_t n) {
     using E = std::remove_pointer_t<decltype(a)>;
     hipLaunchKernelGGL((k_fill_zero<E>), dim3(std::min<size_t>(blocks(n, 256), 4096)), dim3(256), 0, s, n, a);
@@ -434,6 +443,26 @@ int lap_solve(Context& ctx, const LapCall& c) {
     }
   };
   form(std::false_type{}, P.n_p2m, mp);
+  // quadrupoles: their multipoles (degrees >= 2) and, below, their list-4
+  // locals, added to those of the charges and dipoles (FP64)
+  const double* qfac = qp ? hess_factors(ctx, p) : nullptr;
+  auto qp_form = [&](auto irreg_tag, int nbox_form, T* dst) {
+    constexpr bool IR = decltype(irreg_tag)::value;
+    if (!qp || nbox_form == 0 || (!IR && p < 2)) return;
+    const int pt = IR ? p + 2 : p - 2;
+    // tiles within 64 KB (two workgroups per CU) where one source fits, else
+    // within the device's LDS
+    const long per = (long)(qp_form_lds_doubles(p, pt, 1) - 2 * (size_t)K);
+    const long lim = (long)(max_lds_bytes(ctx) / sizeof(double)) - 2L * K;
+    const long small = std::min<long>(65536 / sizeof(double), lim + 2L * K) - 2L * K;
+    const long avail = small >= per ? small : lim;
+    const int ts = (int)std::min<long>(32, avail / per);
+    if (ts < 1) throw AllocError("quadrupole P2M/P2L tile does not fit LDS", 16);
+    const size_t lds = sizeof(double) * qp_form_lds_doubles(p, pt, ts);
+    hipLaunchKernelGGL((k_qp_form<IR>), dim3(nbox_form), dim3(256), lds, s, P, qfac, spos, qstr, ts,
+                       reinterpret_cast<double*>(dst));
+  };
+  qp_form(std::false_type{}, P.n_p2m, mp);
   mark(2);
   // ---- STEP 3: M2M, bottom-up
   const size_t lds_ps = sizeof(T) * 4 * K;
@@ -480,6 +509,19 @@ int lap_solve(Context& ctx, const LapCall& c) {
     launch_p2p<TN>(P, dIB + O.p2p1, O.n_p2p1, 1, spos, strn, str64, tpos, f.ifcharge, f.ifdipole,
                    f.iffldtarg != 0, pot_tn, fld_tn, s2);
   }
+  if (qp) {
+    // after the charges' near field on the same stream: both add into the sums
+    auto qp_near = [&](const int* items, int n, int kind, double* po, double* fo) {
+      if (n == 0) return;
+      DevPlan Q = P;
+      Q.p2p_items = items;
+      Q.n_p2p = n;
+      hipLaunchKernelGGL(k_p2p_qp, dim3(item_groups(n)), dim3(kP2PBlock * kItemWaves), 0, s2, Q, spos, qstr, tpos,
+                         kind, po, fo);
+    };
+    qp_near(dIB + O.p2p0, O.n_p2p0, 0, pot_sn, f.iffld ? fld_sn : nullptr);
+    qp_near(dIB + O.p2p1, O.n_p2p1, 1, pot_tn, f.iffldtarg ? fld_tn : nullptr);
+  }
   if (timed) tn.mark(1);
   FMM_HIP(hipEventRecord(ctx.aux_ev[1], s2));  // near field done
   mark(4);
@@ -492,6 +534,7 @@ int lap_solve(Context& ctx, const LapCall& c) {
   devplan_far(P, O, upload_ints("plan_ic", FC.I, s));
   auto t_lists3 = std::chrono::steady_clock::now();
   form(std::true_type{}, P.n_p2l, lo);
+  qp_form(std::true_type{}, P.n_p2l, lo);
   mark(5);
   // ---- STEP 4: M2L
   if (P.n_m2l) {
@@ -586,7 +629,7 @@ int lap_solve(Context& ctx, const LapCall& c) {
   // ---- STEP 6: M2P (list 3) from packed multipoles; list 3 of each box is
   // cut into slices of about kSlice packed entries, one wave per (64 points,
   // slice), and the slices' sums are added in order (k_slice_reduce)
-  if (P.n_m2p && (f.ifcharge || f.ifdipole)) {
+  if (P.n_m2p && (f.ifcharge || f.ifdipole || qp)) {
     const int grad = (f.iffld || f.iffldtarg) ? 1 : 0;
     const bool want_src = f.ifpot || f.iffld, want_trg = f.ifpottarg || f.iffldtarg;
     const int pd = grad ? p + 1 : p;

@@ -82,6 +82,22 @@ int fmm3d_lfmm3dparttarg_hess(int iprec, int nsource, const double *source,
                               int ifpottarg, double *pottarg, int iffldtarg, double *fldtarg,
                               int ifhesstarg, double *hesstarg);
 
+/* fmm3d_lfmm3dparttarg with quadrupole sources as well: when ifquad is 1,
+ * quadstr(nsource) complex*16 and quadvec(6,nsource) real*8, ordered
+ * (xx, yy, zz, xy, xz, yz), add at each point quadstr * sum_k quadvec_k
+ * V_k(x - y), V_ab = d_a d_b (1/r) (mixed terms weighed once), and its
+ * -gradient to fld.  FP64 throughout.  Extension: the reference's particle
+ * FMM takes charges and dipoles only; the source and its expansions are
+ * those of its library routines lpotfld3dhess_qp (src/laprouts3d.f:8691),
+ * l3dformmp_qp_trunc (:7914) and l3dformta_qp_trunc (:8261). */
+int fmm3d_lfmm3dparttarg_qp(int iprec, int nsource, const double *source,
+                            int ifcharge, const double *charge,
+                            int ifdipole, const double *dipstr, const double *dipvec,
+                            int ifquad, const double *quadstr, const double *quadvec,
+                            int ifpot, double *pot, int iffld, double *fld,
+                            int ntarget, const double *target,
+                            int ifpottarg, double *pottarg, int iffldtarg, double *fldtarg);
+
 /* Replaces lfmm3dtriatarg (src/lfmm3dtria.f:121-141): the triangle FMM.
  * Piecewise-constant single layer (charge) and dipole layer (dipstr) on
  * nsource flat triangles, Green's function 1/r without 1/(4 pi),

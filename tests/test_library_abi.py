@@ -65,6 +65,16 @@ def test_null_arrays_rejected_before_device_use():
                                         0, None, 0, None, 0, None, None) == 128
     assert lib.fmm3d_hfmm3dparttarg(1, p(zk), 4, p(src), 0, None, 1, p(q), None, 1, p(out), 0, None,
                                     0, None, 0, None, 0, None) == 128
+    # quadrupoles: quadstr or quadvec NULL with ifquad = 1, and the iprec check
+    qv = np.zeros((4, 6))
+    assert lib.fmm3d_lfmm3dparttarg_qp(1, 4, p(src), 0, None, 0, None, None, 1, None, p(qv), 1, p(out), 0, None,
+                                       0, None, 0, None, 0, None) == 128
+    assert lib.fmm3d_lfmm3dparttarg_qp(1, 4, p(src), 0, None, 0, None, None, 1, p(q), None, 1, p(out), 0, None,
+                                       0, None, 0, None, 0, None) == 128
+    assert lib.fmm3d_lfmm3dparttarg_qp(7, 4, p(src), 0, None, 0, None, None, 1, p(q), p(qv), 1, p(out), 0, None,
+                                       0, None, 0, None, 0, None) == 32
+    with pytest.raises(ValueError):
+        F.lfmm3dparttarg_qp(1, src, quadstr=q)
     # the direct sum has no error channel: it leaves the outputs untouched
     out[:] = 7
     lib.fmm3d_l3dpartdirect(4, p(src), 1, None, 0, None, None, 1, p(out), 0, None, 0, None, 0, None, 0, None)
