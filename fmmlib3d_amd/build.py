@@ -23,7 +23,11 @@ HIPCC = os.path.join(ROCM, "bin", "hipcc")
 CXX = os.path.join(ROCM, "llvm", "bin", "clang++")
 FC = os.path.join(ROCM, "bin", "amdflang")
 
-HIP_SOURCES = ["runtime.hip", "tree_gpu.hip", "fmm3d_lap.hip", "fmm3d_helm.hip", "fmm3d_api.hip", "server.hip"]
+HIP_SOURCES = ["runtime.hip", "tree_gpu.hip", "fmm3d_lap.hip", "fmm3d_helm.hip", "fmm3d_api.hip", "server.hip",
+               "htria_near.hip"]
+# per-source flags: the Helmholtz triangle near-field kernels without machine
+# LICM (hoisted math-library constants otherwise cost them half their waves)
+HIP_EXTRA = {"htria_near.hip": ["-mllvm", "-disable-machine-licm"]}
 CPP_SOURCES = ["octree.cpp", "tables.cpp", "helm_tables.cpp", "tria_rules.cpp"]
 
 # Packed FP32 instructions are disabled for all device code.
@@ -50,7 +54,7 @@ def build_hip(verbose=False):
         objs.append(o)
     for s in HIP_SOURCES:
         o = os.path.join(OBJDIR, s + ".o")
-        _run([HIPCC] + HIP_FLAGS + ["-I", CSRC, "-c", os.path.join(CSRC, s), "-o", o])
+        _run([HIPCC] + HIP_FLAGS + HIP_EXTRA.get(s, []) + ["-I", CSRC, "-c", os.path.join(CSRC, s), "-o", o])
         objs.append(o)
     tmp = LIB + ".tmp"
     _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fopenmp", "-o", tmp] + objs)

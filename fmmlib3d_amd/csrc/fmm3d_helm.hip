@@ -393,17 +393,7 @@ int helm_solve(Context& ctx, const LapCall& c) {
     FMM_HIP(hipMemcpyAsync(dr, hr, sizeof(double) * sym->size(), hipMemcpyHostToDevice, s2));
     const HtriParams hp{zk.real(), zk.imag(), std::abs(zk), c.tri_lapc, c.tri_lapd, (int)sym->size() / 3, dr};
     auto tri_near = [&](const int* items, int n, int kind, double* po, double* fo) {
-      if (n == 0) return;
-      DevPlan Q = P;
-      Q.p2p_items = items;
-      Q.n_p2p = n;
-      dim3 g(item_groups(n)), bl(kP2PBlock * kItemWaves);
-      if (f.ifcharge && f.ifdipole)
-        hipLaunchKernelGGL((k_htri_p2p<true, true>), g, bl, 0, s2, Q, tgeo, tstr, spos, tpos, kind, hp, po, fo);
-      else if (f.ifcharge)
-        hipLaunchKernelGGL((k_htri_p2p<true, false>), g, bl, 0, s2, Q, tgeo, tstr, spos, tpos, kind, hp, po, fo);
-      else
-        hipLaunchKernelGGL((k_htri_p2p<false, true>), g, bl, 0, s2, Q, tgeo, tstr, spos, tpos, kind, hp, po, fo);
+      launch_htri_p2p(P, items, n, f.ifcharge, f.ifdipole, tgeo, tstr, spos, tpos, kind, hp, po, fo, s2);
     };
     tri_near(dIB + O.p2p0, O.n_p2p0, 0, pot_sn, f.iffld ? fld_sn : nullptr);
     tri_near(dIB + O.p2p1, O.n_p2p1, 1, pot_tn, f.iffldtarg ? fld_tn : nullptr);
@@ -679,15 +669,8 @@ int htri_direct(Context& ctx, const LapCall& c) {
                          2 * (size_t)npts, pot);
       hipLaunchKernelGGL((k_fill_zero<double>), dim3(blocks(6 * (size_t)npts, 256)), dim3(256), 0, s,
                          6 * (size_t)npts, fld);
-    } else if (ifc && ifd) {
-      hipLaunchKernelGGL((k_htri_direct<true, true>), dim3(blocks(npts, 256)), dim3(256), 0, s, n, geo, tstr, npts,
-                         pts, centroids, hp, pot, fld);
-    } else if (ifc) {
-      hipLaunchKernelGGL((k_htri_direct<true, false>), dim3(blocks(npts, 256)), dim3(256), 0, s, n, geo, tstr, npts,
-                         pts, centroids, hp, pot, fld);
     } else {
-      hipLaunchKernelGGL((k_htri_direct<false, true>), dim3(blocks(npts, 256)), dim3(256), 0, s, n, geo, tstr, npts,
-                         pts, centroids, hp, pot, fld);
+      launch_htri_direct(n, geo, tstr, npts, pts, centroids, hp, ifc, ifd, pot, fld, s);
     }
     if (wpot) FMM_HIP(hipMemcpyAsync(po, pot, sizeof(double) * 2 * npts, hipMemcpyDeviceToDevice, s));
     if (wfld) FMM_HIP(hipMemcpyAsync(fo, fld, sizeof(double) * 6 * npts, hipMemcpyDeviceToDevice, s));

@@ -194,7 +194,7 @@ __device__ __forceinline__ void htri_pair(const double* __restrict__ g, const do
 
 // Near field of the Helmholtz triangle FMM over the P2P work items: the
 // triangles of the item's leaf (its own one by the centroid formulas) and of
-// its list 1, one evaluation point per lane, triangles wave-uniform.
+// its list 1, lanes split over points and triangles as tri_lanes.
 template <bool CHG, bool DIP>
 __global__ __launch_bounds__(256) void k_htri_p2p(DevPlan P, const double* __restrict__ geo,
                                                   const double* __restrict__ tstr, const double* __restrict__ cpos,
@@ -206,24 +206,21 @@ __global__ __launch_bounds__(256) void k_htri_p2p(DevPlan P, const double* __res
   if (b < 0) return;  // padding item
   const int n0 = kind ? P.itrg[b] : P.isrc[b];
   const int nn = kind ? P.ntrg[b] : P.nsrc[b];
-  const int i = start + (int)(threadIdx.x & 63);
-  const bool active = i < nn;
-  const int me = n0 + (active ? i : 0);
+  const TriLanes L = tri_lanes(min(64, nn - start));
+  const int i = start + L.pt;
+  const bool active = i < nn && L.g == 0;
+  const int me = n0 + (i < nn ? i : 0);
   const double* mp = (kind ? tpos : cpos) + 4 * (size_t)me;
   const double x = mp[0], y = mp[1], z = mp[2];
+  const int self = kind ? -1 : me;
   double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-  {
-    const int s0 = P.isrc[b], s1 = s0 + P.nsrc[b];
-    const int self = kind ? -1 : me;
-    for (int s = s0; s < s1; s++)
+  for (int jj = P.l1_off[b] - 1; jj < P.l1_off[b + 1]; jj++) {
+    const int j = jj < P.l1_off[b] ? b : P.l1_idx[jj];
+    const int s0 = P.isrc[j], s1 = s0 + P.nsrc[j];
+    for (int s = s0 + L.g; s < s1; s += L.G)
       htri_pair<CHG, DIP>(geo + kTriGeo * (size_t)s, tstr + 4 * (size_t)s, x, y, z, s == self, hp, acc);
   }
-  for (int jj = P.l1_off[b]; jj < P.l1_off[b + 1]; jj++) {
-    const int j = P.l1_idx[jj];
-    const int s0 = P.isrc[j], s1 = s0 + P.nsrc[j];
-    for (int s = s0; s < s1; s++)
-      htri_pair<CHG, DIP>(geo + kTriGeo * (size_t)s, tstr + 4 * (size_t)s, x, y, z, false, hp, acc);
-  }
+  tri_group_sum(L, acc);
   if (!active) return;
   pot[2 * (size_t)me] += acc[0];
   pot[2 * (size_t)me + 1] += acc[1];
@@ -253,5 +250,13 @@ __global__ __launch_bounds__(256) void k_htri_direct(int n, const double* __rest
   pot[2 * (size_t)i + 1] = acc[1];
   for (int k = 0; k < 6; k++) fld[6 * (size_t)i + k] = acc[2 + k];
 }
+
+// Launchers (htria_near.hip): k_htri_p2p over n P2P work items of P (no
+// launch when n is 0 or neither density is on), and k_htri_direct.
+void launch_htri_p2p(const DevPlan& P, const int* items, int n, int ifcharge, int ifdipole, const double* geo,
+                     const double* tstr, const double* cpos, const double* tpos, int kind, const HtriParams& hp,
+                     double* pot, double* fld, hipStream_t s);
+void launch_htri_direct(int n, const double* geo, const double* tstr, int npts, const double* pts, int centroids,
+                        const HtriParams& hp, int ifcharge, int ifdipole, double* pot, double* fld, hipStream_t s);
 
 }  // namespace fmm3d

@@ -203,9 +203,31 @@ __device__ __forceinline__ void tri_pair(const double* __restrict__ g, const dou
   }
 }
 
+// Lanes of a near-field work item of n <= 64 points split into 64 / npad
+// groups of npad lanes (npad the next power of two >= n): lane `pt` of each
+// group evaluates point pt, group g takes sources g, g + G, ... of every box.
+// The triangle FMMs' leaves hold a few points (10 at iprec 2), which one
+// point per lane would leave most lanes of the wave idle.  The groups' sums
+// are added across lanes (tri_group_sum) before the point's lane writes.
+struct TriLanes {
+  int npad, G, g, pt;
+};
+__device__ __forceinline__ TriLanes tri_lanes(int n) {
+  int npad = 1;
+  while (npad < n) npad <<= 1;
+  const int lane = (int)(threadIdx.x & 63);
+  return {npad, 64 / npad, lane / npad, lane & (npad - 1)};
+}
+// every lane of the wave takes part (no lane may have left before)
+__device__ __forceinline__ void tri_group_sum(const TriLanes& L, double (&acc)[8]) {
+  for (int off = L.npad; off < 64; off <<= 1)
+#pragma unroll
+    for (int k = 0; k < 8; k++) acc[k] += __shfl_xor(acc[k], off);
+}
+
 // Near field of the triangle FMM over the P2P work items: the triangles of
 // the item's leaf (its own one by the centroid formulas) and of its list 1,
-// one evaluation point per lane, triangles wave-uniform.
+// lanes split over points and triangles as tri_lanes.
 template <bool CHG, bool DIP>
 __global__ __launch_bounds__(256) void k_tri_p2p(DevPlan P, const double* __restrict__ geo,
                                                  const double* __restrict__ tstr, const double* __restrict__ cpos,
@@ -217,24 +239,21 @@ __global__ __launch_bounds__(256) void k_tri_p2p(DevPlan P, const double* __rest
   if (b < 0) return;  // padding item
   const int n0 = kind ? P.itrg[b] : P.isrc[b];
   const int nn = kind ? P.ntrg[b] : P.nsrc[b];
-  const int i = start + (int)(threadIdx.x & 63);
-  const bool active = i < nn;
-  const int me = n0 + (active ? i : 0);
+  const TriLanes L = tri_lanes(min(64, nn - start));
+  const int i = start + L.pt;
+  const bool active = i < nn && L.g == 0;
+  const int me = n0 + (i < nn ? i : 0);
   const double* mp = (kind ? tpos : cpos) + 4 * (size_t)me;
   const double x = mp[0], y = mp[1], z = mp[2];
+  const int self = kind ? -1 : me;
   double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-  {
-    const int s0 = P.isrc[b], s1 = s0 + P.nsrc[b];
-    const int self = kind ? -1 : me;
-    for (int s = s0; s < s1; s++)
+  for (int jj = P.l1_off[b] - 1; jj < P.l1_off[b + 1]; jj++) {
+    const int j = jj < P.l1_off[b] ? b : P.l1_idx[jj];
+    const int s0 = P.isrc[j], s1 = s0 + P.nsrc[j];
+    for (int s = s0 + L.g; s < s1; s += L.G)
       tri_pair<CHG, DIP>(geo + kTriGeo * (size_t)s, tstr + 4 * (size_t)s, x, y, z, s == self, acc);
   }
-  for (int jj = P.l1_off[b]; jj < P.l1_off[b + 1]; jj++) {
-    const int j = P.l1_idx[jj];
-    const int s0 = P.isrc[j], s1 = s0 + P.nsrc[j];
-    for (int s = s0; s < s1; s++)
-      tri_pair<CHG, DIP>(geo + kTriGeo * (size_t)s, tstr + 4 * (size_t)s, x, y, z, false, acc);
-  }
+  tri_group_sum(L, acc);
   if (!active) return;
   pot[2 * (size_t)me] += acc[0];
   pot[2 * (size_t)me + 1] += acc[1];
