@@ -45,7 +45,7 @@ def test_gpu_qp_one_leaf_equals_direct(rng, ifc, ifd):
 
 
 CASES = [(0, "cube", 0), (1, "cube", 0), (2, "cube", 1), (2, "clusters", 0), (2, "plummer", 1), (3, "clusters", 1),
-         (3, "sphere", 0), (1, "clusters/cube", 0), (3, "clusters/cube", 0)]
+         (3, "sphere", 0), (1, "clusters/cube", 0), (3, "clusters/cube", 0), (-2, "clusters", 1), (-1, "plummer", 0)]
 
 
 @pytest.mark.parametrize("iprec,dist,ifc", CASES)
@@ -54,7 +54,8 @@ def test_gpu_qp_vs_direct(rng, iprec, dist, ifc):
     quadrupoles' P2L and the list-3 evaluation of their multipoles); the
     cube exercises P2M, M2L and the locals; clustered sources with targets
     spread over the cube ("clusters/cube") leave most targets with far-field
-    values alone.  Bar: 10 times the iprec tolerance (the expansion orders
+    values alone; at iprec -2 the expansions have order 2, the lowest whose
+    multipoles carry a quadrupole.  Bar: 10 times the iprec tolerance (the expansion orders
     are those chosen for charges; a quadrupole's coefficients grow with the
     degree by two more factors of n)."""
     ns, nt = 40000, 6000
