@@ -25,9 +25,12 @@ FC = os.path.join(ROCM, "bin", "amdflang")
 
 HIP_SOURCES = ["runtime.hip", "tree_gpu.hip", "fmm3d_lap.hip", "fmm3d_helm.hip", "fmm3d_api.hip", "server.hip",
                "htria_near.hip"]
-# per-source flags: the Helmholtz triangle near-field kernels without machine
-# LICM (hoisted math-library constants otherwise cost them half their waves)
-HIP_EXTRA = {"htria_near.hip": ["-mllvm", "-disable-machine-licm"]}
+# per-source flags: the Helmholtz kernels without machine LICM.  Hoisted
+# constants of the inlined FP64 math routines otherwise cost the triangle
+# near field half its waves (htria_near.hip) and the Helmholtz M2P 12-17%
+# of its time (fmm3d_helm.hip); the results are the same bit for bit.
+HIP_EXTRA = {"htria_near.hip": ["-mllvm", "-disable-machine-licm"],
+             "fmm3d_helm.hip": ["-mllvm", "-disable-machine-licm"]}
 CPP_SOURCES = ["octree.cpp", "tables.cpp", "helm_tables.cpp", "tria_rules.cpp"]
 
 # Packed FP32 instructions are disabled for all device code.

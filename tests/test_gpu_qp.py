@@ -108,3 +108,15 @@ def test_gpu_qp_pure_trace_and_linearity_at_size():
     assert c[0] == 0 and d[0] == 0
     for x, y, z in zip(b[1:], c[1:], d[1:]):
         assert rel(z, x + y) < 1e-12
+
+
+def test_gpu_qp_sources_only(rng):
+    """No targets (the self call): pot and fld at the sources, targets' outputs None."""
+    ns = 30000
+    src = points(rng, ns, "plummer")
+    qs, qv = quads(rng, ns)
+    ier, pot, fld, pt, ft = F.lfmm3dparttarg_qp(1, src, quadstr=qs, quadvec=qv)
+    assert ier == 0 and pt is None and ft is None
+    i_s = rng.choice(ns, 200, replace=False)
+    ps, fs = direct(src, src[i_s], i_s.astype(np.int32), None, None, None, qs, qv)
+    assert rel(pot[i_s], ps) < IPREC_EPS[1] and rel(fld[i_s], fs) < IPREC_EPS[1]
