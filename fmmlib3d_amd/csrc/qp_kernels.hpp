@@ -34,7 +34,9 @@
 
 namespace fmm3d {
 
-// Sorted quadrupoles: 8 doubles each (quadstr re, im, then quadvec).
+// Sorted quadrupoles: 8 doubles each: quadstr (re, im), Qxx, Qyy, Qzz, and
+// Qxy/2, Qxz/2, Qyz/2 (the off-diagonal entries of the symmetric matrix Q;
+// halving is exact).
 static __global__ void k_gather_qp(int ns, const int* __restrict__ perm, const double* __restrict__ qs,
                                    const double* __restrict__ qv, double* __restrict__ out) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -43,7 +45,8 @@ static __global__ void k_gather_qp(int ns, const int* __restrict__ perm, const d
   double* d = out + 8 * (size_t)i;
   d[0] = qs[2 * (size_t)o];
   d[1] = qs[2 * (size_t)o + 1];
-  for (int k = 0; k < 6; k++) d[2 + k] = qv[6 * (size_t)o + k];
+  for (int k = 0; k < 3; k++) d[2 + k] = qv[6 * (size_t)o + k];
+  for (int k = 3; k < 6; k++) d[2 + k] = 0.5 * qv[6 * (size_t)o + k];
 }
 
 __device__ __forceinline__ double qp_sp(int m) { return m >= 0 ? 1.0 : -1.0; }
@@ -118,7 +121,7 @@ __global__ __launch_bounds__(256) void k_qp_form(DevPlan P, const double* __rest
         pos[3 * s] = (x[0] - cx) * ih;
         pos[3 * s + 1] = (x[1] - cy) * ih;
         pos[3 * s + 2] = (x[2] - cz) * ih;
-        const double xx = q[2], yy = q[3], zz = q[4], xy = q[5], xz = q[6], yz = q[7];
+        const double xx = q[2], yy = q[3], zz = q[4], xy = 2.0 * q[5], xz = 2.0 * q[6], yz = 2.0 * q[7];
         // conj(c_j), j = -2..2
         const double cr[5] = {0.25 * (xx - yy), 0.5 * xz, zz - 0.5 * (xx + yy), 0.5 * xz, 0.25 * (xx - yy)};
         const double ci[5] = {-0.25 * xy, -0.5 * yz, 0.0, 0.5 * yz, 0.25 * xy};
@@ -198,16 +201,18 @@ __global__ __launch_bounds__(256) void k_p2p_qp(DevPlan P, const double* __restr
     const double r2 = fma(dx, dx, fma(dy, dy, dz * dz));
     const double ir = rsqrt_t<double>(r2), ir2 = ir * ir;
     const double d3 = ir * ir2, d5 = d3 * ir2, d7 = d5 * ir2;
-    const double xx = q[2], yy = q[3], zz = q[4], xy = q[5], xz = q[6], yz = q[7];
+    const double xx = q[2], yy = q[3], zz = q[4], hxy = q[5], hxz = q[6], hyz = q[7];
     // Q d, d.Q.d and the trace of Q
-    const double qdx = xx * dx + 0.5 * (xy * dy + xz * dz);
-    const double qdy = yy * dy + 0.5 * (xy * dx + yz * dz);
-    const double qdz = zz * dz + 0.5 * (xz * dx + yz * dy);
-    const double dqd = dx * qdx + dy * qdy + dz * qdz, tr = xx + yy + zz;
+    const double qdx = fma(xx, dx, fma(hxy, dy, hxz * dz));
+    const double qdy = fma(yy, dy, fma(hxy, dx, hyz * dz));
+    const double qdz = fma(zz, dz, fma(hxz, dx, hyz * dy));
+    const double dqd = fma(dx, qdx, fma(dy, qdy, dz * qdz)), tr = xx + yy + zz;
     // pot = 3 d.Q.d / r^5 - tr / r^3; grad = 6 Q d / r^5 - 15 d.Q.d d / r^7 + 3 tr d / r^5
-    const double v = 3.0 * dqd * d5 - tr * d3;
-    const double ga = 15.0 * dqd * d7 - 3.0 * tr * d5;
-    const double gx = 6.0 * qdx * d5 - ga * dx, gy = 6.0 * qdy * d5 - ga * dy, gz = 6.0 * qdz * d5 - ga * dz;
+    const double a5 = 3.0 * d5;
+    const double v = fma(dqd, a5, -tr * d3);
+    const double ga = fma(5.0 * dqd, d7, -tr * d5) * 3.0;
+    const double b5 = 2.0 * a5;
+    const double gx = fma(b5, qdx, -ga * dx), gy = fma(b5, qdy, -ga * dy), gz = fma(b5, qdz, -ga * dz);
     pr = fma(q[0], v, pr);
     pi = fma(q[1], v, pi);
     fr[0] = fma(-q[0], gx, fr[0]); fi[0] = fma(-q[1], gx, fi[0]);
@@ -222,6 +227,7 @@ __global__ __launch_bounds__(256) void k_p2p_qp(DevPlan P, const double* __restr
   for (int jj = P.l1_off[b]; jj < P.l1_off[b + 1]; jj++) {
     const int j = P.l1_idx[jj];
     const int s0 = P.isrc[j], s1 = s0 + P.nsrc[j];
+#pragma unroll 2
     for (int s = s0; s < s1; s++) add(s);
   }
   if (!active) return;
