@@ -242,9 +242,16 @@ def lfmm3dparttarg_qp(iprec, source, charge=None, dipstr=None, dipvec=None, quad
 def _tri_prep(triaflat, charge, dipstr, target, source):
     tri = np.ascontiguousarray(triaflat, np.float64).reshape(-1, 3, 3)   # (n, vertex, xyz)
     n = tri.shape[0]
-    cen = tri.mean(axis=1) if source is None else np.ascontiguousarray(source, np.float64).reshape(n, 3)
-    nrm = np.cross(tri[:, 1] - tri[:, 0], tri[:, 2] - tri[:, 0])
-    nrm = np.ascontiguousarray(nrm / np.linalg.norm(nrm, axis=1)[:, None])
+    v0, v1, v2 = tri[:, 0], tri[:, 1], tri[:, 2]
+    # centroids ((v0 + v1) + v2) / 3 and unit normals (v1 - v0) x (v2 - v0), by columns
+    # (numpy's mean over the vertex axis and cross take 10x longer on these shapes)
+    cen = (v0 + v1 + v2) / 3.0 if source is None else np.ascontiguousarray(source, np.float64).reshape(n, 3)
+    a, b = v1 - v0, v2 - v0
+    nrm = np.empty((n, 3))
+    nrm[:, 0] = a[:, 1] * b[:, 2] - a[:, 2] * b[:, 1]
+    nrm[:, 1] = a[:, 2] * b[:, 0] - a[:, 0] * b[:, 2]
+    nrm[:, 2] = a[:, 0] * b[:, 1] - a[:, 1] * b[:, 0]
+    nrm /= np.sqrt(nrm[:, 0] ** 2 + nrm[:, 1] ** 2 + nrm[:, 2] ** 2)[:, None]
     q = None if charge is None else np.ascontiguousarray(charge, np.complex128).reshape(n)
     ds = None if dipstr is None else np.ascontiguousarray(dipstr, np.complex128).reshape(n)
     trg = None if target is None else np.ascontiguousarray(target, np.float64).reshape(-1, 3)

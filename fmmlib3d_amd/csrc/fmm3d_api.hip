@@ -121,7 +121,8 @@ int fmm_host(const double* zk, int prec_mode, int iprec, int ns, const double* s
     c.src_dev = upload(ctx, "in_src", source, 3 * (size_t)ns);
     c.q_dev = c.ifcharge ? upload(ctx, "in_q", charge, 2 * (size_t)ns) : nullptr;
     c.ds_dev = c.ifdipole ? upload(ctx, "in_ds", dipstr, 2 * (size_t)ns) : nullptr;
-    c.dv_dev = c.ifdipole ? upload(ctx, "in_dv", dipvec, 3 * (size_t)ns) : nullptr;
+    // (triangles: the dipole layer is along each triangle's normal, dipvec is not read)
+    c.dv_dev = c.ifdipole && !tri_mode ? upload(ctx, "in_dv", dipvec, 3 * (size_t)ns) : nullptr;
     c.trg_dev = nt > 0 ? upload(ctx, "in_trg", target, 3 * (size_t)nt) : nullptr;
     if (tri_mode) c.tri_dev = upload(ctx, "in_tri", tria, 9 * (size_t)ns);
     c.ifquad = ifquad == 1;
@@ -406,7 +407,7 @@ void direct_host(const double* zk, int nsource, const double* source, int ifchar
     c.src_dev = upload(ctx, "in_src", source, 3 * (size_t)nsource);
     c.q_dev = c.ifcharge ? upload(ctx, "in_q", charge, 2 * (size_t)nsource) : nullptr;
     c.ds_dev = c.ifdipole ? upload(ctx, "in_ds", dipstr, 2 * (size_t)nsource) : nullptr;
-    c.dv_dev = c.ifdipole ? upload(ctx, "in_dv", dipvec, 3 * (size_t)nsource) : nullptr;
+    c.dv_dev = c.ifdipole && !tri_mode ? upload(ctx, "in_dv", dipvec, 3 * (size_t)nsource) : nullptr;
     c.trg_dev = ntarget > 0 ? upload(ctx, "in_trg", target, 3 * (size_t)ntarget) : nullptr;
     if (c.ifpot) c.pot_dev = (double*)ctx.arena.get("out_pot", 2 * sizeof(double) * std::max(nsource, 1));
     if (c.iffld) c.fld_dev = (double*)ctx.arena.get("out_fld", 6 * sizeof(double) * std::max(nsource, 1));
