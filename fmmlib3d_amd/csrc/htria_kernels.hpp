@@ -194,9 +194,11 @@ __device__ __forceinline__ void htri_pair(const double* __restrict__ g, const do
 
 // Near field of the Helmholtz triangle FMM over the P2P work items: the
 // triangles of the item's leaf (its own one by the centroid formulas) and of
-// its list 1, lanes split over points and triangles as tri_lanes.
+// its list 1, lanes split over points and triangles as tri_lanes.  Three waves
+// per SIMD (168 VGPRs, a few values spilled to scratch) beat two without
+// spills: the pair's FP64 sqrt / division / sincos chains are latency bound.
 template <bool CHG, bool DIP>
-__global__ __launch_bounds__(256) void k_htri_p2p(DevPlan P, const double* __restrict__ geo,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k_htri_p2p(DevPlan P, const double* __restrict__ geo,
                                                   const double* __restrict__ tstr, const double* __restrict__ cpos,
                                                   const double* __restrict__ tpos, int kind, HtriParams hp,
                                                   double* __restrict__ pot, double* __restrict__ fld) {
